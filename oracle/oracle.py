@@ -1,0 +1,174 @@
+"""ctypes wrapper for the CPU restatement (oracle/liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, always as
+the checker.  The product path (qlens_amd) never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+SIE, NFW, SHEAR = 1, 2, 3
+
+
+class OrcLens(C.Structure):
+    _fields_ = [("type", C.c_int), ("p", C.c_double * 6)]
+
+
+class OrcSersic(C.Structure):
+    _fields_ = [("p", C.c_double * 7)]
+
+
+class OrcProblem(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int),
+        ("xvals", C.POINTER(C.c_double)), ("yvals", C.POINTER(C.c_double)),
+        ("data", C.POINTER(C.c_double)), ("covinv", C.POINTER(C.c_double)),
+        ("mask", C.POINTER(C.c_ubyte)),
+        ("nsplit", C.c_int),
+        ("src_xmin", C.c_double), ("src_xmax", C.c_double),
+        ("src_ymin", C.c_double), ("src_ymax", C.c_double),
+        ("src_nx", C.c_int), ("src_ny", C.c_int),
+        ("pnx", C.c_int), ("pny", C.c_int), ("psf", C.POINTER(C.c_double)),
+        ("regparam", C.c_double),
+        ("nlens", C.c_int), ("lenses", C.POINTER(OrcLens)),
+        ("integral_tol", C.c_double),
+    ]
+
+
+class OrcResult(C.Structure):
+    _fields_ = [
+        ("minus2lnE", C.c_double), ("chisq0", C.c_double), ("logdetF", C.c_double),
+        ("logdetR", C.c_double), ("Es", C.c_double), ("regterms", C.c_double),
+        ("n_img", C.c_int), ("n_src", C.c_int), ("status", C.c_int),
+        ("n_integrand_evals", C.c_longlong),
+    ]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (make -f oracle/Makefile)")
+        _LIB = C.CDLL(path)
+        _LIB.orc_invert.restype = C.c_int
+        _LIB.orc_mock_image.restype = C.c_int
+        _LIB.orc_gaussian_psf.restype = C.c_int
+    return _LIB
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def lens_array(lenses):
+    """lenses: list of (type, params) -> ctypes array"""
+    arr = (OrcLens * max(1, len(lenses)))()
+    for k, (t, p) in enumerate(lenses):
+        arr[k].type = t
+        for i, v in enumerate(p):
+            arr[k].p[i] = float(v)
+    return arr
+
+
+def deflection(lenses, xy, tol=1e-3):
+    xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros_like(xy)
+    nev = C.c_longlong(0)
+    la = lens_array(lenses)
+    lib().orc_deflection(la, C.c_int(len(lenses)), C.c_double(tol), C.c_int(len(xy)), _dp(xy), _dp(out), C.byref(nev))
+    return out, nev.value
+
+
+def nfw_def_hess(lens_params, xy, tol=1e-6):
+    """lens-frame (x, y) -> (ax, ay, psi_xx, psi_xy, psi_yy) for NFW params {ks, rs, q, ...}"""
+    xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros((len(xy), 5))
+    la = lens_array([(NFW, lens_params)])
+    lib().orc_nfw_def_hess(la, C.c_double(tol), C.c_int(len(xy)), _dp(xy), _dp(out))
+    return out
+
+
+def gaussian_psf(sigx, sigy, threshold, xstep, ystep):
+    nx, ny = C.c_int(0), C.c_int(0)
+    lib().orc_gaussian_psf(C.c_double(sigx), C.c_double(sigy), C.c_double(threshold), C.c_double(xstep),
+                           C.c_double(ystep), C.byref(nx), C.byref(ny), None)
+    psf = np.zeros((nx.value, ny.value))
+    lib().orc_gaussian_psf(C.c_double(sigx), C.c_double(sigy), C.c_double(threshold), C.c_double(xstep),
+                           C.c_double(ystep), C.byref(nx), C.byref(ny), _dp(psf))
+    return psf
+
+
+def normal_deviates(seed, n):
+    out = np.zeros(n)
+    lib().orc_normal_deviates(C.c_ulonglong(seed), C.c_int(n), _dp(out))
+    return out
+
+
+def mock_image(case):
+    """Noisy mock image (x-major [nx, ny]) for a workload dict (see qlens_amd.workloads)."""
+    xv, yv = case["xvals"], case["yvals"]
+    nx, ny = len(xv) - 1, len(yv) - 1
+    img = np.zeros((nx, ny))
+    la = lens_array(case["lenses"])
+    srcs = (OrcSersic * len(case["sources"]))()
+    for k, p in enumerate(case["sources"]):
+        for i, v in enumerate(p):
+            srcs[k].p[i] = float(v)
+    psf = np.ascontiguousarray(case["psf"], dtype=np.float64)
+    lib().orc_mock_image(C.c_int(nx), C.c_int(ny), _dp(xv), _dp(yv), C.c_int(case["nsplit"]), la,
+                         C.c_int(len(case["lenses"])), C.c_double(case["integral_tol"]), srcs,
+                         C.c_int(len(case["sources"])), C.c_int(psf.shape[0]), C.c_int(psf.shape[1]),
+                         _dp(psf), C.c_double(case["sigma"]), C.c_ulonglong(case["seed"]), _dp(img))
+    return img
+
+
+def invert(case, data, lenses=None, want_arrays=True):
+    """Run the restated inversion; returns (result dict, arrays dict)."""
+    xv = np.ascontiguousarray(case["xvals"], dtype=np.float64)
+    yv = np.ascontiguousarray(case["yvals"], dtype=np.float64)
+    nx, ny = len(xv) - 1, len(yv) - 1
+    data = np.ascontiguousarray(data, dtype=np.float64)
+    covinv = np.full((nx, ny), 1.0 / (case["sigma"] * case["sigma"]))
+    if "covinv" in case:
+        covinv = np.ascontiguousarray(case["covinv"], dtype=np.float64)
+    psf = np.ascontiguousarray(case["psf"], dtype=np.float64)
+    lenses = case["lenses"] if lenses is None else lenses
+    la = lens_array(lenses)
+    P = OrcProblem()
+    P.nx, P.ny = nx, ny
+    P.xvals, P.yvals = _dp(xv), _dp(yv)
+    P.data, P.covinv = _dp(data), _dp(covinv)
+    mask = case.get("mask")
+    if mask is not None:
+        mask = np.ascontiguousarray(mask, dtype=np.uint8)
+        P.mask = mask.ctypes.data_as(C.POINTER(C.c_ubyte))
+    P.nsplit = case["nsplit"]
+    P.src_xmin, P.src_xmax, P.src_ymin, P.src_ymax = case["srcgrid"]
+    P.src_nx, P.src_ny = case["src_n"]
+    P.pnx, P.pny = psf.shape
+    P.psf = _dp(psf)
+    P.regparam = case["regparam"]
+    P.nlens = len(lenses)
+    P.lenses = la
+    P.integral_tol = case["integral_tol"]
+    R = OrcResult()
+    snx, sny = case["src_n"]
+    ns = case["nsplit"]
+    arrays = {}
+    if want_arrays:
+        arrays = dict(
+            rays=np.zeros((nx, ny, ns * ns, 2)), src=np.zeros((snx, sny)),
+            active=np.zeros((snx, sny), dtype=np.int32), model=np.zeros((nx, ny)),
+            mapped=np.zeros((nx, ny), dtype=np.uint8))
+        lib().orc_invert(C.byref(P), C.byref(R), _dp(arrays["rays"]), _dp(arrays["src"]),
+                         arrays["active"].ctypes.data_as(C.POINTER(C.c_int)), _dp(arrays["model"]),
+                         arrays["mapped"].ctypes.data_as(C.POINTER(C.c_ubyte)))
+    else:
+        lib().orc_invert(C.byref(P), C.byref(R), None, None, None, None, None)
+    res = {f: getattr(R, f) for f, _ in OrcResult._fields_}
+    return res, arrays

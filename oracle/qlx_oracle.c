@@ -1,0 +1,894 @@
+/* CPU restatement of QLens's pixellated-source log-evidence -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Loaded only by tests/, by the smoke test and by the cpu_baseline leg of bench.py.  It is
+ * the checker for the HIP library, never part of the product.  Each function cites the QLens
+ * reference file:line whose arithmetic it restates; the operation order follows the
+ * reference so that results agree to rounding (build with -ffp-contract=off).
+ *
+ * Written from the reference's behaviour, in C99 with OpenMP; no reference source is copied.
+ */
+#include "qlx_oracle.h"
+#include "../qlens_amd/csrc/patterson_table.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define DEG2RAD 0.017453292519943295 /* mathexpr.h:27 */
+#define QLX_HALFPI_REF 1.57079632679 /* mathexpr.h:10: the reference's truncated pi/2 */
+
+/* ------------------------------------------------------------------------------------------
+ * Lens models
+ * ------------------------------------------------------------------------------------------ */
+
+typedef struct {
+    int type;
+    double xc, yc;
+    double costh, sinth; /* after the orient-north swap, profile.cpp:1625-1637 */
+    /* SIE */
+    double q, qsq, bprime, sprime, ssq;
+    /* NFW */
+    double ks, rs, eps, fsqinv, sqrt1meps;
+    /* shear */
+    double shear1, shear2;
+} lprep;
+
+static void prep_angle(lprep *L, double theta_deg)
+{
+    /* LensProfile::set_angle with orient_major_axis_north (profile.cpp:1625-1637) */
+    double th = DEG2RAD * theta_deg;
+    double c = cos(th), s = sin(th);
+    L->sinth = c;
+    L->costh = -s;
+}
+
+static void prep_lens(const orc_lens *in, lprep *L)
+{
+    memset(L, 0, sizeof(*L));
+    L->type = in->type;
+    if (in->type == ORC_SIE) {
+        /* SPLE_Lens::update_meta_parameters (models.cpp:101-110), emode 1 (profile.cpp:1690-1693) */
+        double b = in->p[0], q = in->p[1], s = in->p[5];
+        double fmaj = 1.0 / sqrt(q);
+        L->q = q;
+        L->bprime = b * fmaj;
+        L->sprime = s * fmaj;
+        L->qsq = q * q;
+        L->ssq = L->sprime * L->sprime;
+        prep_angle(L, in->p[2]);
+        L->xc = in->p[3];
+        L->yc = in->p[4];
+    } else if (in->type == ORC_NFW) {
+        double q = in->p[2];
+        L->ks = in->p[0];
+        L->rs = in->p[1];
+        L->q = q;
+        L->eps = 1 - q * q;      /* LensIntegral::initialize, profile.h:446 */
+        L->fsqinv = q;           /* emode 1, profile.h:448 */
+        L->sqrt1meps = sqrt(1 - L->eps);
+        prep_angle(L, in->p[3]);
+        L->xc = in->p[4];
+        L->yc = in->p[5];
+    } else if (in->type == ORC_SHEAR) {
+        /* Shear::update_meta_parameters (models.cpp:2212-2224), orient-north theta_eff */
+        double gamma = in->p[0];
+        double th = DEG2RAD * in->p[1];
+        double theta_eff = th + QLX_HALFPI_REF;
+        L->shear1 = gamma * cos(2 * theta_eff);
+        L->shear2 = gamma * sin(2 * theta_eff);
+        L->xc = in->p[2];
+        L->yc = in->p[3];
+    }
+}
+
+/* NFW::kappa_rsq and lens_function_xsq (models.cpp:978-999) */
+static double nfw_kappa_rsq(const lprep *L, double rsq)
+{
+    double xsq = rsq / (L->rs * L->rs);
+    if (xsq < 1e-6) return -L->ks * (2 + log(xsq / 4));
+    else if (fabs(xsq - 1) < 1e-5) return 2 * L->ks * (0.3333333333333333 - (xsq - 1) / 5.0);
+    double F = (xsq > 1.0) ? (atan(sqrt(xsq - 1)) / sqrt(xsq - 1))
+                           : (xsq < 1.0) ? (atanh(sqrt(1 - xsq)) / sqrt(1 - xsq)) : 1.0;
+    return 2 * L->ks * (1 - F) / (xsq - 1);
+}
+
+/* NFW::kappa_rsq_deriv (models.cpp:986-994) */
+static double nfw_kappa_rsq_deriv(const lprep *L, double rsq)
+{
+    double xsq = rsq / (L->rs * L->rs);
+    if (xsq < 1e-6) return -L->ks / rsq;
+    else if (fabs(xsq - 1.0) < 1e-5) return 2 * L->ks / (L->rs * L->rs) * (-0.2 + 2 * (xsq - 1) / 7.0);
+    double F = (xsq > 1.0) ? (atan(sqrt(xsq - 1)) / sqrt(xsq - 1))
+                           : (xsq < 1.0) ? (atanh(sqrt(1 - xsq)) / sqrt(1 - xsq)) : 1.0;
+    return -(L->ks / rsq) * ((xsq * (2.0 - 3 * F) + 1) / ((xsq - 1) * (xsq - 1)));
+}
+
+/* NFW::kapavg_spherical_rsq (models.cpp:1001-1010) -- q = 1 deflection */
+static double nfw_kapavg_rsq(const lprep *L, double rsq)
+{
+    double xsq = rsq / (L->rs * L->rs);
+    if (xsq > 1e-5) {
+        double F = (xsq > 1.0) ? (atan(sqrt(xsq - 1)) / sqrt(xsq - 1))
+                               : (xsq < 1.0) ? (atanh(sqrt(1 - xsq)) / sqrt(1 - xsq)) : 1.0;
+        return 2 * L->ks * (2 * F + log(xsq / 4)) / xsq;
+    }
+    return -L->ks * (1 + log(xsq / 4));
+}
+
+typedef struct {
+    const lprep *L;
+    double xsq, ysq;
+    double nph; /* n + 1/2 */
+    int kind;   /* 0: J integrand, 1: K integrand */
+} integrand_ctx;
+
+/* LensIntegral::j_integrand_prime / k_integrand_prime (profile.cpp:2778-2790) */
+static double integrand(const integrand_ctx *c, double w)
+{
+    const lprep *L = c->L;
+    double u = w * w;
+    double qfac = 1 - L->eps * u;
+    if (c->kind == 0)
+        return (2 * w * nfw_kappa_rsq(L, u * (c->xsq + c->ysq / qfac) * L->fsqinv) / pow(qfac, c->nph));
+    return L->fsqinv * (2 * w * u * nfw_kappa_rsq_deriv(L, u * (c->xsq + c->ysq / qfac) * L->fsqinv) / pow(qfac, c->nph));
+}
+
+/* LensIntegral::PattersonIntegrate on [0,1] (profile.cpp:3830-3881): nested levels reuse
+ * the previous level's function values; stop at level > 1 when |I_k - I_{k-1}| < tol |I_k|;
+ * after level 8 the 511-point value is returned unconverged (numberOfPoints = 60 < 511). */
+static double patterson01(const integrand_ctx *c, double tol, long long *nev)
+{
+    double f[QLX_PAT_N];
+    double result = 0, result_old;
+    const double absum = 0.5, abdif = 0.5;
+    int level = 0;
+    do {
+        const double *wt = QLX_PAT_W + QLX_PAT_WOFF[level];
+        int order = (1 << (level + 1)) - 1;
+        int istep = (QLX_PAT_N + 1) / (order + 1);
+        int istart = istep - 1;
+        int i, j;
+        istep *= 2;
+        result_old = result;
+        result = 0;
+        for (j = 0, i = istart; j < order; j += 2, i += istep) {
+            f[i] = integrand(c, absum + abdif * QLX_PAT_X[i]);
+            result += wt[j] * f[i];
+        }
+        if (nev) *nev += (order + 1) / 2;
+        for (j = 1, i = istep - 1; j < order; j += 2, i += istep) result += wt[j] * f[i];
+        if ((level > 1) && (fabs(result - result_old) < tol * fabs(result))) break;
+    } while (++level < 9);
+    return abdif * result;
+}
+
+static double j_integral(const lprep *L, double x, double y, int n, double tol, long long *nev)
+{
+    integrand_ctx c = {L, x * x, y * y, n + 0.5, 0};
+    return L->sqrt1meps * patterson01(&c, tol, nev); /* profile.cpp:2703-2733 */
+}
+
+static double k_integral(const lprep *L, double x, double y, int n, double tol, long long *nev)
+{
+    integrand_ctx c = {L, x * x, y * y, n + 0.5, 1};
+    return L->sqrt1meps * patterson01(&c, tol, nev); /* profile.cpp:2735-2764 */
+}
+
+/* One lens's deflection at (x,y): LensProfile::deflection (profile.cpp:1782-1805) with the
+ * model-specific formula; Shear::deflection overrides the frame handling (models.cpp:2266-2272). */
+static void lens_deflection(const lprep *L, double x, double y, double tol, double *d0, double *d1, long long *nev)
+{
+    if (L->type == ORC_SHEAR) {
+        x -= L->xc;
+        y -= L->yc;
+        *d0 = x * L->shear1 + y * L->shear2;
+        *d1 = -y * L->shear1 + x * L->shear2;
+        return;
+    }
+    x -= L->xc;
+    y -= L->yc;
+    int rot = (L->sinth != 0);
+    if (rot) { /* LensProfile::rotate, profile.cpp:1743-1749 */
+        double xp = x * L->costh + y * L->sinth;
+        y = -x * L->sinth + y * L->costh;
+        x = xp;
+    }
+    double a0, a1;
+    if (L->type == ORC_SIE) {
+        if (L->q != 1.0) {
+            /* SPLE_Lens::deflection_elliptical_iso (models.cpp:208-216) */
+            double psi = sqrt(L->qsq * (L->ssq + x * x) + y * y);
+            double u = sqrt(1 - L->qsq);
+            a0 = (L->bprime * L->q / u) * atan(u * x / (psi + L->sprime));
+            a1 = (L->bprime * L->q / u) * atanh(u * y / (psi + L->qsq * L->sprime));
+        } else {
+            /* spherical default with kapavg_spherical_rsq_iso (profile.cpp:2281-2290, models.cpp:174-177);
+             * for q = 1, f_major_axis = 1 so b' = b, s' = s */
+            double rsq = x * x + y * y;
+            double kapavg = L->bprime * (sqrt(L->sprime * L->sprime + rsq) - L->sprime) / rsq;
+            a0 = kapavg * x;
+            a1 = kapavg * y;
+        }
+    } else { /* NFW */
+        if (L->q != 1.0) {
+            /* LensProfile::deflection_numerical (profile.cpp:2359-2376) */
+            if ((fabs(x) < 1e-14) && (fabs(y) < 1e-14)) {
+                a0 = 0;
+                a1 = 0;
+            } else {
+                a0 = x * j_integral(L, x, y, 0, tol, nev);
+                a1 = y * j_integral(L, x, y, 1, tol, nev);
+            }
+        } else {
+            double kapavg = nfw_kapavg_rsq(L, x * x + y * y);
+            a0 = kapavg * x;
+            a1 = kapavg * y;
+        }
+    }
+    if (rot) { /* lensvector::rotate_back, lensvec.h:100-105 */
+        double xp = a0 * L->costh - a1 * L->sinth;
+        a1 = a0 * L->sinth + a1 * L->costh;
+        a0 = xp;
+    }
+    *d0 = a0;
+    *d1 = a1;
+}
+
+/* QLens::deflection for one lens plane with zfactor 1 (lenscalc.cpp:81-113):
+ * def = 0; def += alpha_l over lenses in order; def *= 1. */
+static void total_deflection(const lprep *Ls, int nlens, double x, double y, double tol, double *d0, double *d1, long long *nev)
+{
+    double s0 = 0, s1 = 0;
+    for (int l = 0; l < nlens; l++) {
+        double a0, a1;
+        lens_deflection(&Ls[l], x, y, tol, &a0, &a1, nev);
+        s0 += a0;
+        s1 += a1;
+    }
+    s0 *= 1.0;
+    s1 *= 1.0;
+    *d0 = 0 + s0;
+    *d1 = 0 + s1;
+}
+
+void orc_deflection(const orc_lens *lenses, int nlens, double tol, int n, const double *xy, double *def, long long *n_evals)
+{
+    lprep *Ls = (lprep *)malloc(sizeof(lprep) * (nlens > 0 ? nlens : 1));
+    for (int l = 0; l < nlens; l++) prep_lens(&lenses[l], &Ls[l]);
+    long long tot = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : tot)
+    for (int p = 0; p < n; p++) {
+        long long nev = 0;
+        total_deflection(Ls, nlens, xy[2 * p], xy[2 * p + 1], tol, &def[2 * p], &def[2 * p + 1], &nev);
+        tot += nev;
+    }
+    if (n_evals) *n_evals = tot;
+    free(Ls);
+}
+
+void orc_nfw_def_hess(const orc_lens *lens, double tol, int n, const double *xy, double *out5)
+{
+    /* LensProfile::deflection_and_hessian_numerical (profile.cpp:2532-2560), in the lens frame
+     * (the caller passes lens-frame coordinates; no centre shift or rotation here) */
+    lprep L;
+    prep_lens(lens, &L);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int p = 0; p < n; p++) {
+        double x = xy[2 * p], y = xy[2 * p + 1];
+        double j0 = j_integral(&L, x, y, 0, tol, NULL);
+        double j1 = j_integral(&L, x, y, 1, tol, NULL);
+        out5[5 * p + 0] = x * j0;
+        out5[5 * p + 1] = y * j1;
+        out5[5 * p + 2] = 2 * x * x * k_integral(&L, x, y, 0, tol, NULL) + j0;
+        out5[5 * p + 4] = 2 * y * y * k_integral(&L, x, y, 2, tol, NULL) + j1;
+        out5[5 * p + 3] = 2 * x * y * k_integral(&L, x, y, 1, tol, NULL);
+    }
+}
+
+/* ------------------------------------------------------------------------------------------
+ * PSF
+ * ------------------------------------------------------------------------------------------ */
+
+int orc_gaussian_psf(double sigx, double sigy, double threshold, double xstep, double ystep, int *pnx, int *pny, double *psf)
+{
+    /* QLens::generate_PSF_matrix without supersampling (pixelgrid.cpp:14601-14668) */
+    if (threshold == 0 || sigx == 0 || sigy == 0) return -1;
+    double sigma_fraction = sqrt(-2 * log(threshold));
+    double nx_half_dec = sigma_fraction * sigx / xstep;
+    double ny_half_dec = sigma_fraction * sigy / ystep;
+    int nx_half = (int)nx_half_dec, ny_half = (int)ny_half_dec;
+    if ((nx_half_dec - nx_half) > 0.5) nx_half++;
+    if ((ny_half_dec - ny_half) > 0.5) ny_half++;
+    int nx = 2 * nx_half + 1, ny = 2 * ny_half + 1;
+    *pnx = nx;
+    *pny = ny;
+    if (!psf) return 0;
+    double xstep_sup = xstep / 1, ystep_sup = ystep / 1;
+    double xmax_sup = ((2 * nx_half + 1) * xstep - xstep_sup) / 2;
+    double ymax_sup = ((2 * ny_half + 1) * ystep - ystep_sup) / 2;
+    double norm = 0, x, y;
+    int i, j;
+    for (i = 0, x = -xmax_sup; i < nx; i++, x += xstep_sup) {
+        for (j = 0, y = -ymax_sup; j < ny; j++, y += ystep_sup) {
+            double a = x / sigx, b = y / sigy;
+            psf[i * ny + j] = exp(-0.5 * (a * a + b * b));
+            norm += psf[i * ny + j];
+        }
+    }
+    for (i = 0; i < nx * ny; i++) psf[i] /= norm;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Packed Cholesky (pixelgrid.cpp:17071-17105, 17201-17210, 17230-17246, 17300-17321)
+ * ------------------------------------------------------------------------------------------ */
+
+static void repack_lower(double *pk, int n)
+{
+    /* upper packed row-major -> lower packed row-major (QLens::repack_matrix_lower) */
+    double *tmp = (double *)malloc(sizeof(double) * (size_t)n * (n + 1) / 2);
+    size_t k = 0;
+    for (int j = 0; j < n; j++)
+        for (int i = j; i < n; i++) tmp[(size_t)i * (i + 1) / 2 + j] = pk[k++];
+    memcpy(pk, tmp, sizeof(double) * (size_t)n * (n + 1) / 2);
+    free(tmp);
+}
+
+static int cholesky_packed(double *a, int n)
+{
+    size_t *indx = (size_t *)malloc(sizeof(size_t) * n);
+    indx[0] = 0;
+    for (int j = 1; j < n; j++) indx[j] = indx[j - 1] + j;
+    int status = 1;
+    a[0] = sqrt(a[0]);
+    for (int j = 1; j < n; j++) a[indx[j]] /= a[0];
+    for (int i = 1; i < n; i++) {
+#pragma omp parallel for schedule(static)
+        for (int j = i; j < n; j++) {
+            const double *a1 = a + indx[i];
+            const double *a2 = a + indx[j];
+            double *a3 = a + indx[j] + i;
+            for (int k = 0; k < i; k++) *a3 -= a1[k] * a2[k];
+        }
+        double *d = a + indx[i] + i;
+        if (*d < 0) status = 0;
+        *d = sqrt(fabs(*d));
+        for (int j = i + 1; j < n; j++) a[indx[j] + i] /= *d;
+    }
+    free(indx);
+    return status;
+}
+
+static double cholesky_logdet_packed(const double *a, int n)
+{
+    double logdet = 0;
+    size_t idx = 0;
+    for (int i = 0; i < n; i++) {
+        logdet += log(fabs(a[idx]));
+        idx += i + 2;
+    }
+    return logdet * 2;
+}
+
+static void cholesky_solve_packed(const double *a, const double *b, double *x, int n)
+{
+    size_t *indx = (size_t *)malloc(sizeof(size_t) * n);
+    indx[0] = 0;
+    for (int i = 0; i < n; i++) {
+        if (i > 0) indx[i] = indx[i - 1] + i;
+        double sum = b[i];
+        for (int k = i - 1; k >= 0; k--) sum -= a[indx[i] + k] * x[k];
+        x[i] = sum / a[indx[i] + i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double sum = x[i];
+        for (int k = i + 1; k < n; k++) sum -= a[indx[k] + i] * x[k];
+        x[i] = sum / a[indx[i] + i];
+    }
+    free(indx);
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Inversion
+ * ------------------------------------------------------------------------------------------ */
+
+int orc_invert(const orc_problem *P, orc_result *res, double *rays_out, double *src_out, int *active_out, double *model_out, unsigned char *mapped_out)
+{
+    memset(res, 0, sizeof(*res));
+    const int nx = P->nx, ny = P->ny, ns = P->nsplit, nsub = ns * ns;
+    const int snx = P->src_nx, sny = P->src_ny;
+    if (snx < 2 || sny < 2) { /* lens.cpp:12453-12458 sentinel */
+        res->minus2lnE = 2e30;
+        res->status = 1;
+        return 1;
+    }
+    lprep *Ls = (lprep *)malloc(sizeof(lprep) * (P->nlens > 0 ? P->nlens : 1));
+    for (int l = 0; l < P->nlens; l++) prep_lens(&P->lenses[l], &Ls[l]);
+
+    /* Image grid geometry, ImagePixelGrid ctor + set_nsplits (pixelgrid.cpp:9643-9677, 10898-10938) */
+    const size_t npix = (size_t)nx * ny;
+    double *beta = (double *)malloc(sizeof(double) * npix * nsub * 2);
+    long long nev_tot = 0;
+#pragma omp parallel for schedule(dynamic) reduction(+ : nev_tot)
+    for (int i = 0; i < nx; i++) {
+        for (int j = 0; j < ny; j++) {
+            int k = 0;
+            for (int ii = 0; ii < ns; ii++) {
+                for (int jj = 0; jj < ns; jj++) {
+                    double u0 = ((double)(1 + 2 * ii)) / (2 * ns);
+                    double w0 = ((double)(1 + 2 * jj)) / (2 * ns);
+                    double sx = (1 - u0) * P->xvals[i] + u0 * P->xvals[i + 1];
+                    double sy = (1 - w0) * P->yvals[j] + w0 * P->yvals[j + 1];
+                    double a0, a1;
+                    long long nev = 0;
+                    /* QLens::find_sourcept (lenscalc.cpp:325-330): beta = theta - alpha */
+                    total_deflection(Ls, P->nlens, sx, sy, P->integral_tol, &a0, &a1, &nev);
+                    nev_tot += nev;
+                    size_t o = (((size_t)i * ny + j) * nsub + k) * 2;
+                    beta[o] = sx - a0;
+                    beta[o + 1] = sy - a1;
+                    k++;
+                }
+            }
+        }
+    }
+    res->n_integrand_evals = nev_tot;
+    if (rays_out) memcpy(rays_out, beta, sizeof(double) * npix * nsub * 2);
+
+    /* Source grid: accumulated corners and 4-corner-mean centres
+     * (SourcePixelGrid::create_pixel_grid pixelgrid.cpp:441-455, SourcePixel ctor 761-783) */
+    double *sxv = (double *)malloc(sizeof(double) * (snx + 1));
+    double *syv = (double *)malloc(sizeof(double) * (sny + 1));
+    {
+        double xstep = (P->src_xmax - P->src_xmin) / snx, ystep = (P->src_ymax - P->src_ymin) / sny;
+        double x = P->src_xmin, y = P->src_ymin;
+        for (int i = 0; i <= snx; i++, x += xstep) sxv[i] = x;
+        for (int j = 0; j <= sny; j++, y += ystep) syv[j] = y;
+    }
+    double *cx = (double *)malloc(sizeof(double) * snx);
+    double *cy = (double *)malloc(sizeof(double) * sny);
+    for (int i = 0; i < snx; i++) cx[i] = (sxv[i] + sxv[i] + sxv[i + 1] + sxv[i + 1]) / 4.0;
+    for (int j = 0; j < sny; j++) cy[j] = (syv[j] + syv[j + 1] + syv[j] + syv[j + 1]) / 4.0;
+
+    /* Mapping: containing cell (half-open) plus the nearer x- and y-neighbour
+     * (assign_source_mapping_flags_interpolate, pixelgrid.cpp:2291-2432).  The reference's
+     * quadrant bisection over these same corner values returns exactly the unique cell with
+     * x_i <= bx < x_{i+1}, y_j <= by < y_{j+1}; a binary search over the monotone corners
+     * gives the same cell. */
+    int *map3 = (int *)malloc(sizeof(int) * npix * nsub * 3); /* cell ids i*sny+j or -1 */
+    unsigned char *pmapped = (unsigned char *)calloc(npix, 1);
+    unsigned char *cmapped = (unsigned char *)calloc((size_t)snx * sny, 1);
+#pragma omp parallel for schedule(static)
+    for (long long p = 0; p < (long long)npix; p++) {
+        int in_mask = P->mask ? P->mask[p] : 1;
+        int any = 0;
+        for (int k = 0; k < nsub; k++) {
+            int *m = map3 + ((size_t)p * nsub + k) * 3;
+            m[0] = m[1] = m[2] = -1;
+            if (!in_mask) continue;
+            double bx = beta[((size_t)p * nsub + k) * 2], by = beta[((size_t)p * nsub + k) * 2 + 1];
+            if (!(bx >= sxv[0] && bx < sxv[snx] && by >= syv[0] && by < syv[sny])) continue;
+            int lo = 0, hi = snx; /* largest i with sxv[i] <= bx */
+            while (hi - lo > 1) {
+                int mid = (lo + hi) / 2;
+                if (sxv[mid] <= bx) lo = mid; else hi = mid;
+            }
+            int ci = lo;
+            lo = 0;
+            hi = sny;
+            while (hi - lo > 1) {
+                int mid = (lo + hi) / 2;
+                if (syv[mid] <= by) lo = mid; else hi = mid;
+            }
+            int cj = lo;
+            int xi = (((bx > cx[ci]) && (ci < snx - 1)) || (ci == 0)) ? ci + 1 : ci - 1;
+            int yj = (((by > cy[cj]) && (cj < sny - 1)) || (cj == 0)) ? cj + 1 : cj - 1;
+            m[0] = ci * sny + cj;
+            m[1] = xi * sny + cj;
+            m[2] = ci * sny + yj;
+            cmapped[m[0]] = 1;
+            cmapped[m[1]] = 1;
+            cmapped[m[2]] = 1;
+            any = 1;
+        }
+        pmapped[p] = (unsigned char)any;
+    }
+
+    /* Active source pixels, j-major (SourcePixel::assign_active_indices, pixelgrid.cpp:3586-3626):
+     * mapped cells only (unmapped cells are never flagged maps_to_image_window on this path). */
+    int *sidx = (int *)malloc(sizeof(int) * snx * sny);
+    int nsrc = 0;
+    for (int j = 0; j < sny; j++)
+        for (int i = 0; i < snx; i++) sidx[i * sny + j] = cmapped[i * sny + j] ? nsrc++ : -1;
+    /* Active image pixels, j-outer i-inner (assign_pixel_mappings, pixelgrid.cpp:12879-12914) */
+    int *pidx = (int *)malloc(sizeof(int) * npix);
+    int nimg = 0;
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            size_t p = (size_t)i * ny + j;
+            int in_mask = P->mask ? P->mask[p] : 1;
+            pidx[p] = (in_mask && pmapped[p]) ? nimg++ : -1;
+        }
+    int *act_i = (int *)malloc(sizeof(int) * (nimg > 0 ? nimg : 1));
+    int *act_j = (int *)malloc(sizeof(int) * (nimg > 0 ? nimg : 1));
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            int r = pidx[(size_t)i * ny + j];
+            if (r >= 0) { act_i[r] = i; act_j[r] = j; }
+        }
+    res->n_img = nimg;
+    res->n_src = nsrc;
+    if (mapped_out) memcpy(mapped_out, pmapped, npix);
+    if (active_out) memcpy(active_out, sidx, sizeof(int) * snx * sny);
+    if (nsrc == 0) { /* pixelgrid.cpp:12807 -> lens.cpp:12528-12530 sentinel */
+        res->minus2lnE = 2e30;
+        res->status = 2;
+        free(Ls); free(beta); free(sxv); free(syv); free(cx); free(cy); free(map3); free(pmapped); free(cmapped);
+        free(sidx); free(pidx); free(act_i); free(act_j);
+        return 2;
+    }
+
+    /* Dense L with duplicates summed in CSR order (assign_Lmatrix 13196-13321 with the
+     * barycentric weights of calculate_Lmatrix_interpolate 2501-2527; convert_Lmatrix_to_dense
+     * 13717-13747) */
+    const int N = nsrc;
+    double *Ld = (double *)calloc((size_t)nimg * N, sizeof(double));
+    const double weight = 1.0 / nsub;
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < nimg; r++) {
+        size_t p = (size_t)act_i[r] * ny + act_j[r];
+        double *row = Ld + (size_t)r * N;
+        for (int k = 0; k < nsub; k++) {
+            const int *m = map3 + (p * nsub + k) * 3;
+            if (m[0] < 0) continue;
+            double X = beta[(p * nsub + k) * 2], Y = beta[(p * nsub + k) * 2 + 1];
+            double p0x = cx[m[0] / sny], p0y = cy[m[0] % sny];
+            double p1x = cx[m[1] / sny], p1y = cy[m[1] % sny];
+            double p2x = cx[m[2] / sny], p2y = cy[m[2] % sny];
+            double d = (p0x - p1x) * (p1y - p2y) - (p1x - p2x) * (p0y - p1y);
+            double w0 = weight * (X * (p1y - p2y) + Y * (p2x - p1x) + p1x * p2y - p1y * p2x) / d;
+            double w1 = weight * (X * (p2y - p0y) + Y * (p0x - p2x) + p0y * p2x - p0x * p2y) / d;
+            double w2 = weight * (X * (p0y - p1y) + Y * (p1x - p0x) + p0x * p1y - p0y * p1x) / d;
+            row[sidx[m[0]]] += w0;
+            row[sidx[m[1]]] += w1;
+            row[sidx[m[2]]] += w2;
+        }
+    }
+
+    /* PSF convolution of the dense L (PSF_convolution_Lmatrix_dense direct branch,
+     * pixelgrid.cpp:14092-14176): taps outside the image or on inactive/unmasked pixels drop. */
+    double *Lb;
+    if (P->pnx > 0 && P->pny > 0) {
+        Lb = (double *)calloc((size_t)nimg * N, sizeof(double));
+        const int nxh = P->pnx / 2, nyh = P->pny / 2;
+#pragma omp parallel for schedule(static)
+        for (int r = 0; r < nimg; r++) {
+            int k = act_i[r], l = act_j[r];
+            double *out = Lb + (size_t)r * N;
+            for (int pk = 0; pk < P->pnx; pk++) {
+                int ii = k + nxh - pk;
+                if (ii < 0 || ii >= nx) continue;
+                for (int pl = 0; pl < P->pny; pl++) {
+                    int jj = l + nyh - pl;
+                    if (jj < 0 || jj >= ny) continue;
+                    int r2 = pidx[(size_t)ii * ny + jj];
+                    if (r2 < 0) continue; /* maps_to_source_pixel && in mask */
+                    double pv = P->psf[pk * P->pny + pl];
+                    const double *in = Ld + (size_t)r2 * N;
+                    for (int c = 0; c < N; c++) out[c] += pv * in[c];
+                }
+            }
+        }
+        free(Ld);
+    } else {
+        Lb = Ld;
+    }
+
+    /* D and packed F (create_lensing_matrices_from_Lmatrix_dense, pixelgrid.cpp:15590-15711,
+     * noise-map branch: F_ij = sum_l Lt_i[l]*Lt_j[l]*c_l) */
+    double *dvec = (double *)malloc(sizeof(double) * nimg);
+    double *cvec = (double *)malloc(sizeof(double) * nimg);
+    for (int r = 0; r < nimg; r++) {
+        size_t p = (size_t)act_i[r] * ny + act_j[r];
+        dvec[r] = P->data[p];  /* fill_surface_brightness_vector, pixelgrid.cpp:11428-11440 */
+        cvec[r] = P->covinv[p];
+    }
+    double *Lt = (double *)malloc(sizeof(double) * (size_t)N * nimg);
+    double *D = (double *)calloc(N, sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; i++) {
+        double acc = 0;
+        for (int j = 0; j < nimg; j++) {
+            double sb_adj = dvec[j] - 0.0;
+            acc += Lb[(size_t)j * N + i] * sb_adj * cvec[j];
+            Lt[(size_t)i * nimg + j] = Lb[(size_t)j * N + i];
+        }
+        D[i] = acc;
+    }
+    const size_t ntot = (size_t)N * (N + 1) / 2;
+    double *F = (double *)malloc(sizeof(double) * ntot);
+    size_t *rowoff = (size_t *)malloc(sizeof(size_t) * N);
+    {
+        size_t o = 0;
+        for (int i = 0; i < N; i++) { rowoff[i] = o; o += N - i; }
+    }
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int i = 0; i < N; i++) {
+        const double *l1 = Lt + (size_t)i * nimg;
+        for (int j = i; j < N; j++) {
+            const double *l2 = Lt + (size_t)j * nimg;
+            double acc = 0;
+            for (int l = 0; l < nimg; l++) acc += l1[l] * l2[l] * cvec[l];
+            F[rowoff[i] + (j - i)] = acc;
+        }
+    }
+    free(Lt);
+
+    /* Curvature R = H0^T H0 + H1^T H1 on the active set (generate_hmatrices 2968-3071 and
+     * generate_Rmatrix_from_hmatrices 3073-3240).  Row of H_l for active cell a: -2 at a and +1
+     * at each ACTIVE +/- neighbour along l; entries are small integers, so R is exact. */
+    double *Rdiag = (double *)calloc(N, sizeof(double));
+    /* sparse upper off-diagonals: at most 8 per row */
+    int *Rcol = (int *)malloc(sizeof(int) * N * 12);
+    double *Rval = (double *)malloc(sizeof(double) * N * 12);
+    int *Rn = (int *)calloc(N, sizeof(int));
+    /* all H0 rows first, then all H1 rows: the reference's per-row insertion order */
+    for (int l = 0; l < 2; l++)
+        for (int j = 0; j < sny; j++)
+            for (int i = 0; i < snx; i++) {
+                int a = sidx[i * sny + j];
+                if (a < 0) continue;
+                int ids[3];
+                double vals[3];
+                int m = 0;
+                ids[m] = a; vals[m++] = -2;
+                int nb[2];
+                if (l == 0) { nb[0] = (i < snx - 1) ? sidx[(i + 1) * sny + j] : -1; nb[1] = (i > 0) ? sidx[(i - 1) * sny + j] : -1; }
+                else { nb[0] = (j < sny - 1) ? sidx[i * sny + j + 1] : -1; nb[1] = (j > 0) ? sidx[i * sny + j - 1] : -1; }
+                for (int t = 0; t < 2; t++)
+                    if (nb[t] >= 0) { ids[m] = nb[t]; vals[m++] = 1; }
+                for (int x = 0; x < m; x++)
+                    for (int y = x; y < m; y++) {
+                        int s1 = ids[x], s2 = ids[y];
+                        double e = vals[x] * vals[y];
+                        if (s1 > s2) { int t = s1; s1 = s2; s2 = t; }
+                        if (s1 == s2) { Rdiag[s1] += e; continue; }
+                        int q;
+                        for (q = 0; q < Rn[s1]; q++)
+                            if (Rcol[s1 * 12 + q] == s2) break;
+                        if (q == Rn[s1]) { Rcol[s1 * 12 + q] = s2; Rval[s1 * 12 + q] = e; Rn[s1]++; }
+                        else Rval[s1 * 12 + q] += e;
+                    }
+            }
+    /* ln det R via dense packed Cholesky (Rmatrix_determinant_dense, pixelgrid.cpp:18176-18207) */
+    double *Rp = (double *)calloc(ntot, sizeof(double));
+    for (int i = 0; i < N; i++) {
+        Rp[rowoff[i]] = Rdiag[i];
+        for (int q = 0; q < Rn[i]; q++) Rp[rowoff[i] + Rcol[i * 12 + q] - i] = Rval[i * 12 + q];
+    }
+    repack_lower(Rp, N);
+    int okR = cholesky_packed(Rp, N);
+    double logdetR = cholesky_logdet_packed(Rp, N);
+    free(Rp);
+
+    /* F += lambda R (add_regularization_term_to_dense_Fmatrix sparse branch, 15808-15818) */
+    const double lam = P->regparam;
+    for (int i = 0; i < N; i++) {
+        F[rowoff[i]] += lam * Rdiag[i];
+        for (int q = 0; q < Rn[i]; q++) F[rowoff[i] + Rcol[i * 12 + q] - i] += lam * Rval[i * 12 + q];
+    }
+
+    /* Cholesky solve + ln det F (invert_lens_mapping_dense non-MKL branch, 17375-17387) */
+    repack_lower(F, N);
+    int okF = cholesky_packed(F, N);
+    double *s = (double *)malloc(sizeof(double) * N);
+    cholesky_solve_packed(F, D, s, N);
+    double logdetF = cholesky_logdet_packed(F, N);
+
+    /* model image = Lb s (calculate_image_pixel_surface_brightness_dense, 18268-18307) */
+    double *model = (double *)calloc(npix, sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < nimg; r++) {
+        double acc = 0;
+        const double *row = Lb + (size_t)r * N;
+        for (int c = 0; c < N; c++) acc += row[c] * s[c];
+        model[(size_t)act_i[r] * ny + act_j[r]] = acc;
+    }
+
+    /* regularization prior term, sparse branch (calculate_regularization_prior_term 15852-15860) */
+    double Es = 0;
+    for (int i = 0; i < N; i++) {
+        Es += Rdiag[i] * (s[i] * s[i]);
+        for (int q = 0; q < Rn[i]; q++) Es += 2 * s[i] * Rval[i * 12 + q] * s[Rcol[i * 12 + q]];
+    }
+    double regterms = 0;
+    double loglike = 0;
+    if (lam != 0) {
+        regterms = lam * Es - N * log(lam) - logdetR;
+        loglike += regterms;
+    }
+    loglike += logdetF;
+
+    /* chi-square over all mask pixels, i-outer j-inner (lens.cpp:12985-13015) */
+    double chisq = 0;
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++) {
+            size_t p = (size_t)i * ny + j;
+            int in_mask = P->mask ? P->mask[p] : 1;
+            if (!in_mask) continue;
+            double ci = P->covinv[p];
+            if (pmapped[p]) {
+                double r = model[p] - P->data[p];
+                chisq += (r * r) * ci;
+            } else {
+                chisq += (P->data[p] * P->data[p]) * ci;
+            }
+        }
+    loglike += chisq;
+
+    res->minus2lnE = loglike;
+    res->chisq0 = chisq;
+    res->logdetF = logdetF;
+    res->logdetR = logdetR;
+    res->Es = Es;
+    res->regterms = regterms;
+    res->status = (okF && okR) ? 0 : 3;
+
+    if (src_out) {
+        for (int c = 0; c < snx * sny; c++) src_out[c] = (sidx[c] >= 0) ? s[sidx[c]] : 0.0;
+    }
+    if (model_out) memcpy(model_out, model, sizeof(double) * npix);
+
+    free(model); free(s); free(F); free(rowoff); free(Rdiag); free(Rcol); free(Rval); free(Rn);
+    free(D); free(dvec); free(cvec); free(Lb);
+    free(Ls); free(beta); free(sxv); free(syv); free(cx); free(cy); free(map3); free(pmapped); free(cmapped);
+    free(sidx); free(pidx); free(act_i); free(act_j);
+    return res->status;
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Mock data
+ * ------------------------------------------------------------------------------------------ */
+
+typedef struct {
+    uint64_t u, v, w;
+    int first;
+    double gset;
+} nr3;
+
+static uint64_t nr3_int64(nr3 *r)
+{
+    /* Numerical Recipes 3rd ed. "Ran" (Press et al. 2007, ch. 7.1), as used by rand.h:48-54 */
+    r->u = r->u * 2862933555777941757ULL + 7046029254386353087ULL;
+    r->v ^= r->v >> 17;
+    r->v ^= r->v << 31;
+    r->v ^= r->v >> 8;
+    r->w = 4294957665U * (r->w & 0xffffffff) + (r->w >> 32);
+    uint64_t x = r->u ^ (r->u << 21);
+    x ^= x >> 35;
+    x ^= x << 4;
+    return (x + r->v) ^ r->w;
+}
+
+static void nr3_seed(nr3 *r, uint64_t j)
+{
+    r->v = 4101842887655102017ULL;
+    r->w = 1;
+    r->u = j ^ r->v;
+    nr3_int64(r);
+    r->v = r->u;
+    nr3_int64(r);
+    nr3_int64(r);
+    r->w = r->v;
+    nr3_int64(r);
+    nr3_int64(r);
+    r->first = 1;
+}
+
+static double nr3_normal(nr3 *r)
+{
+    /* polar Box-Muller (Marsaglia), returning the pair's second member first (rand.h:58-76) */
+    if (r->first) {
+        double v1, v2, rsq, fac;
+        do {
+            v1 = 2.0 * (5.42101086242752217E-20 * nr3_int64(r)) - 1.0;
+            v2 = 2.0 * (5.42101086242752217E-20 * nr3_int64(r)) - 1.0;
+            rsq = v1 * v1 + v2 * v2;
+        } while ((rsq >= 1.0) || (rsq == 0.0));
+        fac = sqrt(-2.0 * log(rsq) / rsq);
+        r->gset = v1 * fac;
+        r->first = 0;
+        return v2 * fac;
+    }
+    r->first = 1;
+    return r->gset;
+}
+
+void orc_normal_deviates(unsigned long long seed, int n, double *out)
+{
+    nr3 r;
+    nr3_seed(&r, seed);
+    for (int i = 0; i < n; i++) out[i] = nr3_normal(&r);
+}
+
+static double sersic_sb(const orc_sersic *S, double x, double y)
+{
+    /* SB_Profile::surface_brightness (sbprofile.cpp:1346-1366) with source orientation from +x
+     * (sbprofile.cpp:12) and emode 1; Sersic::sb_rsq with Ciotti-Bertin b_n (2472-2516) */
+    double s0 = S->p[0], reff = S->p[1], n = S->p[2], q = S->p[3];
+    double th = DEG2RAD * S->p[4];
+    double c = cos(th), s = sin(th);
+    x -= S->p[5];
+    y -= S->p[6];
+    if (th != 0) {
+        double xp = x * c + y * s;
+        y = -x * s + y * c;
+        x = xp;
+    }
+    double rsq_ell = x * x + y * y / (q * q);
+    rsq_ell *= q;
+    double b = 2 * n - 0.33333333333333 + 4.0 / (405 * n) + 46.0 / (25515 * n * n) + 131.0 / (1148175 * n * n * n);
+    return s0 * exp(-b * pow(rsq_ell / (reff * reff), 0.5 / n));
+}
+
+int orc_mock_image(int nx, int ny, const double *xvals, const double *yvals, int nsplit, const orc_lens *lenses, int nlens, double tol,
+                   const orc_sersic *srcs, int nsrc, int pnx, int pny, const double *psf, double sigma, unsigned long long seed, double *img)
+{
+    lprep *Ls = (lprep *)malloc(sizeof(lprep) * (nlens > 0 ? nlens : 1));
+    for (int l = 0; l < nlens; l++) prep_lens(&lenses[l], &Ls[l]);
+    const int nsub = nsplit * nsplit;
+    double *sb = (double *)malloc(sizeof(double) * (size_t)nx * ny);
+    /* ImagePixelGrid::find_surface_brightness split branch (pixelgrid.cpp:11835-11896) */
+#pragma omp parallel for schedule(dynamic)
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++) {
+            double sbtot = 0;
+            for (int ii = 0; ii < nsplit; ii++)
+                for (int jj = 0; jj < nsplit; jj++) {
+                    double u0 = ((double)(1 + 2 * ii)) / (2 * nsplit);
+                    double w0 = ((double)(1 + 2 * jj)) / (2 * nsplit);
+                    double sx = (1 - u0) * xvals[i] + u0 * xvals[i + 1];
+                    double sy = (1 - w0) * yvals[j] + w0 * yvals[j + 1];
+                    double a0, a1;
+                    total_deflection(Ls, nlens, sx, sy, tol, &a0, &a1, NULL);
+                    double bx = sx - a0, by = sy - a1;
+                    for (int k = 0; k < nsrc; k++) {
+                        double v = sersic_sb(&srcs[k], bx, by);
+                        if (v != 0) sbtot += v;
+                    }
+                }
+            sb[(size_t)i * ny + j] = 0 + sbtot / nsub;
+        }
+    /* PSF_convolution_pixel_vector direct branch over all pixels (pixelgrid.cpp:14452-14490) */
+    if (pnx > 0 && pny > 0) {
+        const int nxh = pnx / 2, nyh = pny / 2;
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < nx; k++)
+            for (int l = 0; l < ny; l++) {
+                double acc = 0;
+                for (int pk = 0; pk < pnx; pk++) {
+                    int ii = k + nxh - pk;
+                    if (ii < 0 || ii >= nx) continue;
+                    for (int pl = 0; pl < pny; pl++) {
+                        int jj = l + nyh - pl;
+                        if (jj < 0 || jj >= ny) continue;
+                        acc += psf[pk * pny + pl] * sb[(size_t)ii * ny + jj];
+                    }
+                }
+                img[(size_t)k * ny + l] = acc;
+            }
+    } else {
+        memcpy(img, sb, sizeof(double) * (size_t)nx * ny);
+    }
+    /* ImagePixelGrid::add_pixel_noise (pixelgrid.cpp:10987-10996) */
+    if (sigma > 0) {
+        nr3 r;
+        nr3_seed(&r, seed);
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) img[(size_t)i * ny + j] += sigma * nr3_normal(&r);
+    }
+    free(sb);
+    free(Ls);
+    return 0;
+}

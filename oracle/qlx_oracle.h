@@ -1,0 +1,91 @@
+/* CPU restatement of QLens's pixellated-source log-evidence -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This is the checker the HIP library is compared against.  Only tests/, the smoke test in
+ * __graft_entry__.py and the cpu_baseline leg of bench.py may load it; the product path
+ * (qlens_amd, libqlx.so) never does.  Every function cites the reference file:line it follows
+ * and keeps the reference's arithmetic order (it is compiled with -ffp-contract=off).
+ */
+#ifndef QLX_ORACLE_H
+#define QLX_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_SIE = 1, ORC_NFW = 2, ORC_SHEAR = 3 };
+
+/* One lens in QLens conventions (emode 1, lens angle measured from +y).
+ *   SIE   p = {b, q, theta_deg, xc, yc, s}        SPLE_Lens with alpha = 1 (models.cpp:101-230)
+ *   NFW   p = {ks, rs, q, theta_deg, xc, yc}      NFW, pmode 0 (models.cpp:953-1010)
+ *   SHEAR p = {gamma, theta_deg, xc, yc}          Shear (models.cpp:2140-2272)            */
+typedef struct {
+    int type;
+    double p[6];
+} orc_lens;
+
+typedef struct {
+    /* image grid: corners xvals[nx+1], yvals[ny+1]; per-pixel arrays x-major ([i*ny+j]) */
+    int nx, ny;
+    const double *xvals, *yvals;
+    const double *data;          /* observed surface brightness */
+    const double *covinv;        /* 1/sigma^2 per pixel (the reference's covinv_map) */
+    const unsigned char *mask;   /* NULL = all pixels in mask */
+    int nsplit;                  /* subpixel split per axis (default_imgpixel_nsplit) */
+    /* Cartesian source grid */
+    double src_xmin, src_xmax, src_ymin, src_ymax;
+    int src_nx, src_ny;
+    /* PSF matrix psf[i*pny+j] (the reference's psf_matrix[i][j]); pnx=pny=0 -> no PSF */
+    int pnx, pny;
+    const double *psf;
+    double regparam;             /* lambda (curvature regularization) */
+    /* lens model */
+    int nlens;
+    const orc_lens *lenses;
+    double integral_tol;         /* Gauss-Patterson relative tolerance */
+} orc_problem;
+
+typedef struct {
+    double minus2lnE, chisq0, logdetF, logdetR, Es, regterms;
+    int n_img, n_src, status;
+    long long n_integrand_evals;
+} orc_result;
+
+/* deflection alpha (sum over lenses) at n points xy[2n]; levels (optional, 2 per point for
+ * the J0/J1 Patterson levels of the FIRST numerical lens) and per-call integrand count */
+void orc_deflection(const orc_lens *lenses, int nlens, double tol, int n, const double *xy,
+                    double *def, long long *n_evals);
+
+/* NFW-only Hessian (psi_xx, psi_xy, psi_yy) via the K integrals, profile.cpp:2532-2623 */
+void orc_nfw_def_hess(const orc_lens *lens, double tol, int n, const double *xy, double *out5);
+
+/* Gaussian PSF exactly as QLens::generate_PSF_matrix (pixelgrid.cpp:14601-14668).
+ * Returns the matrix size through pnx/pny (call with psf=NULL first to size). */
+int orc_gaussian_psf(double sigx, double sigy, double threshold, double xstep, double ystep,
+                     int *pnx, int *pny, double *psf);
+
+/* Full log-evidence evaluation (lens.cpp:12293-13195, Cartesian/DENSE/curvature/interpolate).
+ * Optional outputs (may be NULL): rays[nx*ny*nsplit^2*2] (source-plane subpixel points),
+ * src[src_nx*src_ny] (amplitudes, 0 for inactive), active[src_nx*src_ny] (active index or -1),
+ * model[nx*ny] (model image, 0 off the active set), mapped[nx*ny]. */
+int orc_invert(const orc_problem *prob, orc_result *res, double *rays, double *src, int *active,
+               double *model, unsigned char *mapped);
+
+/* Elliptical Sersic source (sbprofile.cpp:1346-1366, 2450-2516) with the reference's
+ * source orientation convention (major axis from +x). params = {s0, Reff, n, q, theta_deg, xc, yc} */
+typedef struct { double p[7]; } orc_sersic;
+
+/* Mock image (lens.cpp:11882-12066 path): ray-traced subpixel Sersic SB averaged per pixel,
+ * PSF convolution, then Gaussian noise sigma from the NR3 generator seeded with seed
+ * (rand.h:19-76; noise added j-outer, i-inner, pixelgrid.cpp:10987-10996). */
+int orc_mock_image(int nx, int ny, const double *xvals, const double *yvals, int nsplit,
+                   const orc_lens *lenses, int nlens, double tol, const orc_sersic *srcs, int nsrc,
+                   int pnx, int pny, const double *psf, double sigma, unsigned long long seed,
+                   double *img_out);
+
+/* NR3 generator + polar Box-Muller (rand.h:19-76): first n deviates for a seed. */
+void orc_normal_deviates(unsigned long long seed, int n, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

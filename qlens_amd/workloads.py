@@ -1,0 +1,114 @@
+"""Concrete, seeded workloads for the pixellated-source evidence (shared by tests and bench).
+
+Each builder returns a dict describing one problem in QLens conventions:
+  xvals, yvals   image pixel corners (nx+1, ny+1); x_i = xmin + i*((xmax-xmin)/nx), the same
+                 multiply-based corners the reference's no-data image grid uses
+                 (pixelgrid.cpp:9582-9595)
+  nsplit         subpixel split per axis (QLens default_imgpixel_nsplit, lens.cpp:497)
+  lenses         [(type, params)]: SIE (b, q, theta, xc, yc, s), NFW (ks, rs, q, theta, xc, yc),
+                 SHEAR (gamma, theta, xc, yc); lens angles from +y (profile.cpp:20)
+  sources        Sersic (s0, Reff, n, q, theta, xc, yc) used only to simulate the data
+  psf            PSF matrix psf[i][j] (QLens psf_matrix orientation)
+  sigma, seed    pixel noise and the NR3 generator seed (rand.h:19-76)
+  srcgrid, src_n Cartesian source grid bounds and cell counts (auto_srcgrid / auto_src_npixels off)
+  regparam       curvature regularization lambda
+  integral_tol   Gauss-Patterson relative tolerance (integral_tolerance)
+Sizes and parameter values follow the five workloads of the benchmark description:
+  C1 SIE+shear 100^2 image / 30^2 source, Gaussian PSF, FP64
+  C2 NFW(quadrature)+SIE+shear 200^2 / 60^2, Moffat 21x21 PSF
+  C3 surrogate deflection evaluation on a 400^2 x 4x4 image
+  C4 256 proposals around C1's truth on a 200^2 image with uneven source grids
+  C5 400^2 image / 100^2 source, mixed precision
+"""
+import numpy as np
+
+SIE, NFW, SHEAR = 1, 2, 3
+FWHM_TO_SIGMA = 1.0 / 2.35482
+
+
+def corners(xmin, xmax, n):
+    step = (xmax - xmin) / n
+    return np.array([xmin + i * step for i in range(n + 1)], dtype=np.float64)
+
+
+def gaussian_psf(sigma, threshold, xstep, ystep):
+    """QLens::generate_PSF_matrix without supersampling (pixelgrid.cpp:14601-14668), restated
+    in numpy for workload setup: point-sampled Gaussian truncated at psf_threshold."""
+    frac = np.sqrt(-2 * np.log(threshold))
+    nxd, nyd = frac * sigma / xstep, frac * sigma / ystep
+    nxh, nyh = int(nxd), int(nyd)
+    if nxd - nxh > 0.5:
+        nxh += 1
+    if nyd - nyh > 0.5:
+        nyh += 1
+    nx, ny = 2 * nxh + 1, 2 * nyh + 1
+    xmax = ((2 * nxh + 1) * xstep - xstep) / 2
+    ymax = ((2 * nyh + 1) * ystep - ystep) / 2
+    psf = np.zeros((nx, ny))
+    x = -xmax
+    for i in range(nx):
+        y = -ymax
+        for j in range(ny):
+            psf[i, j] = np.exp(-0.5 * ((x / sigma) ** 2 + (y / sigma) ** 2))
+            y += ystep
+        x += xstep
+    return psf / psf.sum()
+
+
+def moffat_psf(beta, fwhm, n, step):
+    """Point-sampled circular Moffat (1 + r^2/alpha^2)^-beta on an n x n pixel grid, normalised."""
+    alpha = fwhm / (2.0 * np.sqrt(2.0 ** (1.0 / beta) - 1.0))
+    h = n // 2
+    x = (np.arange(n) - h) * step
+    r2 = x[:, None] ** 2 + x[None, :] ** 2
+    psf = (1.0 + r2 / alpha ** 2) ** (-beta)
+    return psf / psf.sum()
+
+
+def c1():
+    xv = corners(-2.5, 2.5, 100)
+    return dict(
+        name="C1", xvals=xv, yvals=xv.copy(), nsplit=2,
+        lenses=[(SIE, (1.2, 0.8, 30.0, 0.0, 0.0, 0.0)), (SHEAR, (0.05, 20.0, 0.0, 0.0))],
+        sources=[(1.0, 0.2, 1.5, 0.7, 0.0, 0.05, -0.03)],
+        psf=gaussian_psf(0.1 * FWHM_TO_SIGMA, 0.02, 0.05, 0.05),
+        sigma=0.01, seed=10,
+        srcgrid=(-0.6, 0.6, -0.6, 0.6), src_n=(30, 30),
+        regparam=1e3, integral_tol=1e-6,
+    )
+
+
+def c2_sources(seed=20251):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(3):
+        r = 0.3 * np.sqrt(rng.uniform())
+        ph = rng.uniform(0, 2 * np.pi)
+        out.append((float(rng.uniform(0.5, 1.0)), float(rng.uniform(0.08, 0.2)), float(rng.uniform(0.8, 3.0)),
+                    float(rng.uniform(0.6, 0.95)), float(rng.uniform(0, 180)),
+                    float(r * np.cos(ph)), float(r * np.sin(ph))))
+    return out
+
+
+def c2(n_img=200, pix=0.04, n_src=60, src_half=1.0, psf_n=21):
+    half = 0.5 * n_img * pix
+    xv = corners(-half, half, n_img)
+    return dict(
+        name="C2", xvals=xv, yvals=xv.copy(), nsplit=2,
+        lenses=[(NFW, (0.2, 5.0, 0.75, -15.0, 0.0, 0.0)),
+                (SIE, (1.0, 0.85, 0.0, 0.0, 0.0, 0.0)),
+                (SHEAR, (0.03, 0.0, 0.0, 0.0))],
+        sources=c2_sources(),
+        psf=moffat_psf(3.0, 0.12, psf_n, pix),
+        sigma=0.01, seed=10,
+        srcgrid=(-src_half, src_half, -src_half, src_half), src_n=(n_src, n_src),
+        regparam=1e3, integral_tol=1e-6,
+    )
+
+
+def c2_small():
+    """C2's physics at reduced size (oracle- and reference-friendly)."""
+    return c2(n_img=60, pix=0.1333333333333333, n_src=24, src_half=1.0, psf_n=7)
+
+
+WORKLOADS = {"C1": c1, "C2": c2, "C2s": c2_small}
