@@ -1,0 +1,187 @@
+/* qlx -- MI355X-native pixellated-source log-evidence (drop-in for QLens's inversion block).
+ *
+ * C ABI: plain pointers and sizes, no C++ or HIP types.  One context owns one HIP stream and
+ * all device buffers; a context is not thread-safe, many contexts may coexist.  Every entry
+ * point returns a qlx_status (0 = ok, negative = error) and never exits the process (the
+ * reference's die() -> exit(1), errors.cpp:25-54, becomes a status code here).
+ *
+ * The path replaced is the Cartesian / DENSE / curvature / 3-point-interpolate block of
+ * QLens::invert_image_surface_brightness_map (lens.cpp:12364-12615) plus its chi-square loop
+ * (lens.cpp:12950-13015); a QLens-side adapter calls qlx_eval() in its place.
+ * Conventions are QLens's: lens emode 1, lens position angle measured from +y
+ * (profile.cpp:20, 1625-1637), shear angle theta_eff = theta + 90 deg (models.cpp:2212-2224),
+ * images stored x-major (value of pixel (i,j) at [i*ny + j]) like ImagePixelData's [i][j].
+ */
+#ifndef QLX_H
+#define QLX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    QLX_OK = 0,
+    QLX_ERR_ARG = -1,      /* invalid argument (null pointer, bad size) */
+    QLX_ERR_STATE = -2,    /* call out of order (e.g. eval before set_image) */
+    QLX_ERR_HIP = -3,      /* HIP runtime error or no device */
+    QLX_ERR_NOMEM = -4,    /* device allocation failed / admission refused */
+    QLX_ERR_NOT_PD = -5,   /* F or R not positive definite (reference: warn + die, pixelgrid.cpp:17095-17099, 17384) */
+    QLX_ERR_NOSOURCE = -6, /* no source pixel mapped (reference sentinel 2e30, pixelgrid.cpp:12807) */
+    QLX_ERR_RANGE = -7     /* size above a compiled-in maximum */
+} qlx_status;
+
+/* lens types */
+enum { QLX_LENS_SIE = 1, QLX_LENS_NFW = 2, QLX_LENS_SHEAR = 3 };
+/* how a numerical profile's deflection is obtained */
+enum { QLX_DEF_QUADRATURE = 0, QLX_DEF_SURROGATE = 1 };
+/* precision modes */
+enum {
+    QLX_PREC_FP64 = 0,  /* parity mode: FP64 everywhere (MFMA f64 for F and Cholesky) */
+    QLX_PREC_MIXED = 1  /* BF16-in/FP32-acc F, FP32 Cholesky + FP64 refinement, FP64 log-dets */
+};
+/* ray-shooting arithmetic */
+enum { QLX_RAY_FP64 = 0, QLX_RAY_FP32 = 1 };
+
+#define QLX_MAX_LENSES 8
+
+/* One lens in QLens parameters (replaces LensProfile + its subclass, profile.h / models.cpp):
+ *   SIE   p = {b, q, theta_deg, xc, yc, s}     SPLE_Lens with alpha = 1 (models.cpp:101-230)
+ *   NFW   p = {ks, rs, q, theta_deg, xc, yc}   NFW pmode 0 (models.cpp:953-1010)
+ *   SHEAR p = {gamma, theta_deg, xc, yc}       Shear (models.cpp:2140-2272)
+ * def_mode applies to NFW only: Gauss-Patterson quadrature (profile.cpp:2359-2416) or the
+ * trained surrogate (falls back to quadrature outside its training box). */
+typedef struct {
+    int type;
+    int def_mode;
+    double p[6];
+} qlx_lens;
+
+/* A full lens model: the reference's lens_list (qlens.h), one lens plane, zfactor 1. */
+typedef struct {
+    int nlens;
+    qlx_lens lens[QLX_MAX_LENSES];
+} qlx_lens_model;
+
+typedef struct {
+    int device;             /* HIP device ordinal */
+    size_t hbm_budget;      /* bytes this context may use for batch admission (0 = 85% of free) */
+} qlx_opts;
+
+/* Inversion settings (the QLens fields the block reads, qlens.h:750-893):
+ * nsplit = default_imgpixel_nsplit; source grid = srcgrid_x/ymin/max and srcgrid_npixels_x/y
+ * (auto_srcgrid and auto_src_npixels off); regparam = the pixsrc regparam; integral_tol =
+ * integral_tolerance; curvature regularization, 3-point interpolation, dense inversion. */
+typedef struct {
+    int nsplit;
+    double src_xmin, src_xmax, src_ymin, src_ymax;
+    int src_nx, src_ny;
+    double regparam;
+    double integral_tol;
+    int precision;          /* QLX_PREC_* */
+    int ray_precision;      /* QLX_RAY_* */
+    int refine_iters;       /* MIXED: max FP64 iterative-refinement steps (<= 0: default 8) */
+    double refine_tol;      /* MIXED: relative residual target (<= 0: default 1e-10) */
+} qlx_settings;
+
+/* Per-evaluation result (the scalars QLens fills, qlens.h:750-761, 891). */
+typedef struct {
+    double minus2lnE;   /* -2 ln E without noise constants (lens.cpp:13033-13043 off), or a
+                           sentinel: 2e30 (no/too-small source grid, lens.cpp:12453-12458) */
+    double chisq0;      /* chi-square of the best-fit model over mask pixels */
+    double logdetF, logdetR, Es; /* ln det F, ln det R, s^T R s */
+    double refine_resid;/* MIXED: final relative residual of the refined solve (FP64: 0) */
+    int n_img_active;   /* image_npixels */
+    int n_src_active;   /* source_npixels */
+    int status;         /* qlx_status of this item */
+    int refine_steps;
+} qlx_result;
+
+/* ---- context ---------------------------------------------------------------------------- */
+int qlx_ctx_create(const qlx_opts *opts, void **ctx);
+int qlx_ctx_destroy(void *ctx);
+const char *qlx_last_error(void *ctx);
+
+/* ---- inputs (copied at call time) ----------------------------------------------------------
+ * qlx_set_image replaces ImagePixelData (pixelgrid.h:513-641, loaded by load_data
+ * pixelgrid.cpp:5797-5904): corners xvals[nx+1], yvals[ny+1]; data, covinv (1/sigma^2, the
+ * noise map's covinv_map) and mask (1 = in mask; NULL = all) are x-major [nx*ny]. */
+int qlx_set_image(void *ctx, int nx, int ny, const double *xvals, const double *yvals,
+                  const double *data, const double *covinv, const unsigned char *mask);
+/* PSF matrix psf[i*pny + j] = QLens psf_matrix[i][j] (use_input_psf_matrix, qlens.h:877-880). */
+int qlx_set_psf(void *ctx, int pnx, int pny, const double *psf);
+/* Gaussian PSF built as QLens::generate_PSF_matrix (pixelgrid.cpp:14601-14668). */
+int qlx_set_psf_gaussian(void *ctx, double sigma_x, double sigma_y, double threshold);
+int qlx_set_settings(void *ctx, const qlx_settings *s);
+
+/* ---- evaluation ----------------------------------------------------------------------------
+ * Replaces one call of QLens::invert_image_surface_brightness_map (lens.cpp:12293) per model:
+ * nbatch independent lens models (e.g. sampler proposals) are evaluated on the same image and
+ * settings.  Synchronous.  The per-item status is in out[b].status; the return value is the
+ * first non-ok item status or an API error. */
+int qlx_eval(void *ctx, int nbatch, const qlx_lens_model *models, qlx_result *out);
+
+/* Batched evaluation with per-item source grids (uneven source sizes in one submission).
+ * src_n[2*b], src_n[2*b+1] = source cells of item b (bounds from the settings). */
+int qlx_eval_batch_grids(void *ctx, int nbatch, const qlx_lens_model *models, const int *src_n,
+                         qlx_result *out);
+
+/* Read back item `item` of the last qlx_eval: source amplitudes per cell (x-major
+ * [src_nx*src_ny], 0 for inactive cells; update_source_amplitudes pixelgrid.cpp:17850-17881)
+ * and the model image (x-major [nx*ny]; store_image_pixel_surface_brightness 18483-18498).
+ * Either pointer may be NULL.  Only the last item of the last call is kept. */
+int qlx_get_solution(void *ctx, double *src_cells, double *model_image);
+
+/* ---- ray shooting and data (building blocks, host buffers) --------------------------------
+ * beta = theta - alpha at n points (QLens::find_sourcept, lenscalc.cpp:318-330). */
+int qlx_raytrace(void *ctx, const qlx_lens_model *model, int n, const double *xy, double *beta);
+/* Lens-frame NFW deflection + Hessian (alpha_x, alpha_y, psi_xx, psi_xy, psi_yy) by quadrature
+ * (LensProfile::deflection_and_hessian_numerical, profile.cpp:2532-2560): surrogate targets. */
+int qlx_nfw_def_hess(void *ctx, const double *nfw_params, double tol, int n, const double *xy,
+                     double *out5);
+/* Mock image: ray-traced Sersic sources averaged over nsplit^2 subpixels, PSF-convolved, plus
+ * Gaussian noise from the NR3 generator with `seed` (lens.cpp:11882-12066 path;
+ * add_pixel_noise pixelgrid.cpp:10987-10996).  sersic[7*k] = {s0, Reff, n, q, theta, xc, yc}. */
+int qlx_mock_image(void *ctx, const qlx_lens_model *model, int nsrc, const double *sersic,
+                   double sigma, unsigned long long seed, double *img_out);
+
+/* ---- learned deflection surrogate (replaces the tab/qtab option, models.cpp:4054-5148) ----
+ * MLP 4 -> 128 x4 (SiLU) -> 5 over (log r, phi, q, log rs) in the NFW lens frame, predicting
+ * (alpha_x, alpha_y, psi_xx, psi_xy, psi_yy) / ks.  BF16 weights, FP32 accumulation. */
+typedef struct {
+    int n_samples;       /* training samples generated by the quadrature kernel */
+    int steps;           /* optimizer steps */
+    int batch;           /* minibatch size */
+    double lr;           /* Adam learning rate */
+    double r_min, r_max, q_min, q_max, rs_min, rs_max; /* training box */
+    unsigned long long seed;
+    double tol;          /* quadrature tolerance for the targets */
+} qlx_surrogate_cfg;
+int qlx_surrogate_train(void *ctx, const qlx_surrogate_cfg *cfg, double *final_loss);
+int qlx_surrogate_save(void *ctx, const char *path);
+int qlx_surrogate_load(void *ctx, const char *path);
+/* Forward pass on n lens-frame inputs x[4n] -> y[5n] (FP32 outputs). */
+int qlx_surrogate_eval(void *ctx, int n, const float *x, float *y);
+/* One forward+backward on a batch; returns loss and writes the FP32 gradient of every
+ * parameter (flattened, see qlx_surrogate_param_count) -- the gradient check entry point. */
+int qlx_surrogate_param_count(void *ctx);
+int qlx_surrogate_get_params(void *ctx, float *params);
+int qlx_surrogate_set_params(void *ctx, const float *params);
+int qlx_surrogate_grad(void *ctx, int n, const float *x, const float *y, float *loss, float *grad);
+
+/* ---- timing support for benchmarks ---------------------------------------------------------
+ * Average device time (ms) of the named kernel over the last qlx_eval, from HIP events on the
+ * context's stream; names: "syrk", "lbuild", "raytrace", "cholesky", "total". */
+int qlx_kernel_time(void *ctx, const char *name, double *ms, int *launches);
+
+/* Diagnostic (no reference counterpart): one v_mfma_f64_16x16x4f64 on fixed operands
+ * A[i][k] = i + 1 + 0.01k, B[k][j] = 1 + k + 0.1j; out[lane*4 + r] = result register r of lane.
+ * Lets a test confirm the accumulator layout the FP64 tile kernels assume
+ * (register r of lane l = D[l/16 + 4r][l%16] on gfx950). */
+int qlx_selftest_mfma(void *ctx, double *out256);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

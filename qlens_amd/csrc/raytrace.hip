@@ -1,0 +1,450 @@
+// Ray shooting and source-plane mapping kernels for gfx950.
+//
+// One thread per (supersampled) image ray: deflection through the lens model (SIE and shear in
+// closed form, NFW by nested Gauss-Patterson quadrature), beta = theta - alpha, then the
+// 3-point interpolation mapping onto the Cartesian source grid.  Compiled with
+// -ffp-contract=off so the FP64 path keeps the reference's operation order.
+//
+// Reference behaviour followed:
+//   subpixel centres        ImagePixelGrid::set_nsplits            pixelgrid.cpp:10898-10938
+//   lens equation / sum     QLens::find_sourcept, deflection       lenscalc.cpp:81-113, 318-330
+//   frame shift + rotation  LensProfile::deflection, rotate        profile.cpp:1743-1805
+//   SIE                     SPLE_Lens::deflection_elliptical_iso   models.cpp:208-216
+//   shear                   Shear::deflection                      models.cpp:2266-2272
+//   NFW                     deflection_numerical + J integrals     profile.cpp:2359-2376, 2703-2783
+//   kappa                   NFW::kappa_rsq                         models.cpp:978-999
+//   quadrature              LensIntegral::PattersonIntegrate       profile.cpp:3830-3881
+//   mapping                 assign_source_mapping_flags_interpolate pixelgrid.cpp:2291-2432
+//   L weights               calculate_Lmatrix_interpolate          pixelgrid.cpp:2501-2527
+#include "qlx_internal.h"
+#include "patterson_table.h"
+
+namespace qlx {
+
+// Patterson nodes and a dense per-level weight table WL[level][node] (0 when the node is not
+// part of that level's rule).  Instead of keeping all 511 integrand values per lane (the
+// reference's pat_funcs array), each new value is added at once into the running sums of every
+// level that contains it, so a lane holds 9 partial sums per integral.
+__constant__ double c_pat_x[QLX_PAT_N];
+__constant__ double c_pat_wl[QLX_PAT_NLEV][QLX_PAT_N];
+__constant__ float c_pat_x32[QLX_PAT_N];
+__constant__ float c_pat_wl32[QLX_PAT_NLEV][QLX_PAT_N];
+
+static bool g_pat_uploaded[64] = {false};
+
+void upload_patterson(int device)
+{
+    if (device >= 0 && device < 64 && g_pat_uploaded[device]) return;
+    static double wl[QLX_PAT_NLEV][QLX_PAT_N];
+    static float wl32[QLX_PAT_NLEV][QLX_PAT_N];
+    static float x32[QLX_PAT_N];
+    for (int L = 0; L < QLX_PAT_NLEV; L++) {
+        for (int i = 0; i < QLX_PAT_N; i++) wl[L][i] = 0.0;
+        int order = (1 << (L + 1)) - 1;
+        int step = (QLX_PAT_N + 1) / (order + 1);
+        for (int k = 0; k < order; k++) wl[L][(k + 1) * step - 1] = QLX_PAT_W[QLX_PAT_WOFF[L] + k];
+        for (int i = 0; i < QLX_PAT_N; i++) wl32[L][i] = (float)wl[L][i];
+    }
+    for (int i = 0; i < QLX_PAT_N; i++) x32[i] = (float)QLX_PAT_X[i];
+    hipMemcpyToSymbol(HIP_SYMBOL(c_pat_x), QLX_PAT_X, sizeof(QLX_PAT_X));
+    hipMemcpyToSymbol(HIP_SYMBOL(c_pat_wl), wl, sizeof(wl));
+    hipMemcpyToSymbol(HIP_SYMBOL(c_pat_x32), x32, sizeof(x32));
+    hipMemcpyToSymbol(HIP_SYMBOL(c_pat_wl32), wl32, sizeof(wl32));
+    if (device >= 0 && device < 64) g_pat_uploaded[device] = true;
+}
+
+template <typename T> struct PatTab;
+template <> struct PatTab<double> {
+    __device__ static double x(int i) { return c_pat_x[i]; }
+    __device__ static double w(int L, int i) { return c_pat_wl[L][i]; }
+};
+template <> struct PatTab<float> {
+    __device__ static float x(int i) { return c_pat_x32[i]; }
+    __device__ static float w(int L, int i) { return c_pat_wl32[L][i]; }
+};
+
+template <typename T>
+__device__ __forceinline__ T nfw_kappa_rsq(const LensPrep& L, T rsq)
+{
+    const T ks = (T)L.ks;
+    T xsq = rsq / (T)L.rs2;
+    if (xsq < (T)1e-6) return -ks * ((T)2 + log(xsq / (T)4));
+    if (fabs(xsq - (T)1) < (T)1e-5) return (T)2 * ks * ((T)0.3333333333333333 - (xsq - (T)1) / (T)5.0);
+    T F;
+    if (xsq > (T)1.0) {
+        T s = sqrt(xsq - (T)1);
+        F = atan(s) / s;
+    } else {
+        T s = sqrt((T)1 - xsq);
+        F = atanh(s) / s;
+    }
+    return (T)2 * ks * ((T)1 - F) / (xsq - (T)1);
+}
+
+template <typename T>
+__device__ __forceinline__ T nfw_kappa_rsq_deriv(const LensPrep& L, T rsq)
+{
+    const T ks = (T)L.ks;
+    T xsq = rsq / (T)L.rs2;
+    if (xsq < (T)1e-6) return -ks / rsq;
+    if (fabs(xsq - (T)1.0) < (T)1e-5) return (T)2 * ks / (T)L.rs2 * ((T)-0.2 + (T)2 * (xsq - (T)1) / (T)7.0);
+    T F;
+    if (xsq > (T)1.0) {
+        T s = sqrt(xsq - (T)1);
+        F = atan(s) / s;
+    } else {
+        T s = sqrt((T)1 - xsq);
+        F = atanh(s) / s;
+    }
+    return -(ks / rsq) * ((xsq * ((T)2.0 - (T)3 * F) + (T)1) / ((xsq - (T)1) * (xsq - (T)1)));
+}
+
+// J0 and J1 of one ray together: both integrands share kappa at every node; each integral keeps
+// its own stop level (profile.cpp:3858: level > 1 and |I_k - I_{k-1}| < tol |I_k|), and the
+// 511-point value is returned when a level never converges.  Returns the number of integrand
+// evaluations this lane performed (2 per node, like two separate reference integrals would when
+// their levels agree).
+template <typename T>
+__device__ __forceinline__ int nfw_j01(const LensPrep& L, T x, T y, T tol, T& j0, T& j1)
+{
+    const T xsq = x * x, ysq = y * y;
+    const T eps = (T)L.eps, fsqinv = (T)L.fsqinv;
+    T S0[QLX_PAT_NLEV], S1[QLX_PAT_NLEV];
+#pragma unroll
+    for (int k = 0; k < QLX_PAT_NLEV; k++) { S0[k] = 0; S1[k] = 0; }
+    bool d0 = false, d1 = false;
+    T r0 = 0, r1 = 0;
+    int nev = 0;
+#pragma unroll
+    for (int m = 0; m < QLX_PAT_NLEV; m++) {
+        const int order = (1 << (m + 1)) - 1;
+        const int step = (QLX_PAT_N + 1) / (order + 1);
+        for (int i = step - 1; i < QLX_PAT_N; i += 2 * step) {
+            T w = (T)0.5 + (T)0.5 * PatTab<T>::x(i);
+            T u = w * w;
+            T qfac = (T)1 - eps * u;
+            T kap = nfw_kappa_rsq<T>(L, u * (xsq + ysq / qfac) * fsqinv);
+            T t = (T)2 * w * kap;
+            T f0 = t / sqrt(qfac);          // pow(qfac, 0.5): both correctly rounded
+            T f1 = t / pow(qfac, (T)1.5);
+#pragma unroll
+            for (int k = m; k < QLX_PAT_NLEV; k++) {
+                T wk = PatTab<T>::w(k, i);
+                S0[k] += wk * f0;
+                S1[k] += wk * f1;
+            }
+            nev += 2;
+        }
+        if (m > 1) {
+            if (!d0 && fabs(S0[m] - S0[m - 1]) < tol * fabs(S0[m])) { d0 = true; r0 = S0[m]; }
+            if (!d1 && fabs(S1[m] - S1[m - 1]) < tol * fabs(S1[m])) { d1 = true; r1 = S1[m]; }
+            if (d0 && d1) break;
+        }
+    }
+    if (!d0) r0 = S0[QLX_PAT_NLEV - 1];
+    if (!d1) r1 = S1[QLX_PAT_NLEV - 1];
+    j0 = (T)L.sqrt1meps * ((T)0.5 * r0);
+    j1 = (T)L.sqrt1meps * ((T)0.5 * r1);
+    return nev;
+}
+
+template <typename T>
+__device__ __forceinline__ T patterson_k(const LensPrep& L, T x, T y, T tol, T nph)
+{
+    // single K integral (profile.cpp:2735-2764, 2785-2790) for the surrogate's Hessian targets
+    const T xsq = x * x, ysq = y * y;
+    const T eps = (T)L.eps, fsqinv = (T)L.fsqinv;
+    T S[QLX_PAT_NLEV];
+#pragma unroll
+    for (int k = 0; k < QLX_PAT_NLEV; k++) S[k] = 0;
+    T r = 0;
+    bool done = false;
+#pragma unroll
+    for (int m = 0; m < QLX_PAT_NLEV; m++) {
+        const int order = (1 << (m + 1)) - 1;
+        const int step = (QLX_PAT_N + 1) / (order + 1);
+        for (int i = step - 1; i < QLX_PAT_N; i += 2 * step) {
+            T w = (T)0.5 + (T)0.5 * PatTab<T>::x(i);
+            T u = w * w;
+            T qfac = (T)1 - eps * u;
+            T f = fsqinv * ((T)2 * w * u * nfw_kappa_rsq_deriv<T>(L, u * (xsq + ysq / qfac) * fsqinv) / pow(qfac, nph));
+#pragma unroll
+            for (int k = m; k < QLX_PAT_NLEV; k++) S[k] += PatTab<T>::w(k, i) * f;
+        }
+        if (m > 1 && fabs(S[m] - S[m - 1]) < tol * fabs(S[m])) { done = true; r = S[m]; break; }
+    }
+    if (!done) r = S[QLX_PAT_NLEV - 1];
+    return (T)L.sqrt1meps * ((T)0.5 * r);
+}
+
+// Deflection of one lens at (x, y) (profile.cpp:1782-1805; shear models.cpp:2266-2272).
+template <typename T>
+__device__ __forceinline__ void lens_deflection(const LensPrep& L, T x, T y, T tol, T& d0, T& d1, int& nev)
+{
+    x -= (T)L.xc;
+    y -= (T)L.yc;
+    if (L.type == QLX_LENS_SHEAR) {
+        d0 = x * (T)L.shear1 + y * (T)L.shear2;
+        d1 = -y * (T)L.shear1 + x * (T)L.shear2;
+        return;
+    }
+    if (L.rotate) {
+        T xp = x * (T)L.costh + y * (T)L.sinth;
+        y = -x * (T)L.sinth + y * (T)L.costh;
+        x = xp;
+    }
+    T a0, a1;
+    if (L.type == QLX_LENS_SIE) {
+        if (L.q != 1.0) {
+            // u = sqrt(1-q^2) and bprime*q/u are per-lens constants computed on the host
+            T psi = sqrt((T)L.qsq * ((T)L.ssq + x * x) + y * y);
+            T u = (T)L.sie_u;
+            T fac = (T)L.sie_fac;
+            a0 = fac * atan(u * x / (psi + (T)L.sprime));
+            a1 = fac * atanh(u * y / (psi + (T)L.qsq * (T)L.sprime));
+        } else {
+            T rsq = x * x + y * y;
+            T kapavg = (T)L.bprime * (sqrt((T)L.sprime * (T)L.sprime + rsq) - (T)L.sprime) / rsq;
+            a0 = kapavg * x;
+            a1 = kapavg * y;
+        }
+    } else {  // NFW
+        if (L.q != 1.0) {
+            if ((fabs(x) < (T)1e-14) && (fabs(y) < (T)1e-14)) {
+                a0 = 0;
+                a1 = 0;
+            } else {
+                T j0, j1;
+                nev += nfw_j01<T>(L, x, y, tol, j0, j1);
+                a0 = x * j0;
+                a1 = y * j1;
+            }
+        } else {
+            T rsq = x * x + y * y;
+            T xsq = rsq / (T)L.rs2;
+            T kapavg;
+            if (xsq > (T)1e-5) {
+                T F;
+                if (xsq > (T)1.0) { T s = sqrt(xsq - (T)1); F = atan(s) / s; }
+                else if (xsq < (T)1.0) { T s = sqrt((T)1 - xsq); F = atanh(s) / s; }
+                else F = (T)1.0;
+                kapavg = (T)2 * (T)L.ks * ((T)2 * F + log(xsq / (T)4)) / xsq;
+            } else {
+                kapavg = -(T)L.ks * ((T)1 + log(xsq / (T)4));
+            }
+            a0 = kapavg * x;
+            a1 = kapavg * y;
+        }
+    }
+    if (L.rotate) {
+        T xp = a0 * (T)L.costh - a1 * (T)L.sinth;
+        a1 = a0 * (T)L.sinth + a1 * (T)L.costh;
+        a0 = xp;
+    }
+    d0 = a0;
+    d1 = a1;
+}
+
+template <typename T>
+__device__ __forceinline__ void total_deflection(const ModelPrep& m, T x, T y, T tol, T& d0, T& d1, int& nev)
+{
+    T s0 = 0, s1 = 0;
+    for (int l = 0; l < m.nlens; l++) {
+        T a0, a1;
+        lens_deflection<T>(m.lens[l], x, y, tol, a0, a1, nev);
+        s0 += a0;
+        s1 += a1;
+    }
+    d0 = s0;
+    d1 = s1;
+}
+
+__device__ __forceinline__ int upper_index(const double* v, int n, double x)
+{
+    // largest i in [0, n) with v[i] <= x, for v monotone increasing and v[0] <= x < v[n]
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (v[mid] <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_raytrace_map(GridDev g, ModelPrep m, T tol, ItemBufs b)
+{
+    const int nsub = g.nsub;
+    const long long total = (long long)g.nx * g.ny * nsub;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int k = (int)(t % nsub);
+    const long long p = t / nsub;  // pixel id i*ny + j
+    const int i = (int)(p / g.ny), j = (int)(p % g.ny);
+    int* mc = b.map_cell + t * 3;
+    double* mw = b.map_w + t * 3;
+    if (g.mask && !g.mask[p]) {
+        mc[0] = mc[1] = mc[2] = -1;
+        b.beta[t * 2] = 0;
+        b.beta[t * 2 + 1] = 0;
+        return;
+    }
+    const int ii = k / g.nsplit, jj = k % g.nsplit;
+    const double u0 = ((double)(1 + 2 * ii)) / (2 * g.nsplit);
+    const double w0 = ((double)(1 + 2 * jj)) / (2 * g.nsplit);
+    const double sx = (1 - u0) * g.xvals[i] + u0 * g.xvals[i + 1];
+    const double sy = (1 - w0) * g.yvals[j] + w0 * g.yvals[j + 1];
+    T a0, a1;
+    int nev = 0;
+    total_deflection<T>(m, (T)sx, (T)sy, tol, a0, a1, nev);
+    const double bx = (double)((T)sx - a0), by = (double)((T)sy - a1);
+    b.beta[t * 2] = bx;
+    b.beta[t * 2 + 1] = by;
+    if (nev) atomicAdd((unsigned long long*)b.nevals, (unsigned long long)nev);
+
+    const int snx = g.snx, sny = g.sny;
+    if (!(bx >= g.sxv[0] && bx < g.sxv[snx] && by >= g.syv[0] && by < g.syv[sny])) {
+        mc[0] = mc[1] = mc[2] = -1;
+        return;
+    }
+    const int ci = upper_index(g.sxv, snx, bx);
+    const int cj = upper_index(g.syv, sny, by);
+    const int xi = (((bx > g.scx[ci]) && (ci < snx - 1)) || (ci == 0)) ? ci + 1 : ci - 1;
+    const int yj = (((by > g.scy[cj]) && (cj < sny - 1)) || (cj == 0)) ? cj + 1 : cj - 1;
+    const int c0 = ci * sny + cj, c1 = xi * sny + cj, c2 = ci * sny + yj;
+    mc[0] = c0;
+    mc[1] = c1;
+    mc[2] = c2;
+    const double weight = 1.0 / nsub;
+    const double p0x = g.scx[ci], p0y = g.scy[cj];
+    const double p1x = g.scx[xi], p1y = g.scy[cj];
+    const double p2x = g.scx[ci], p2y = g.scy[yj];
+    const double d = (p0x - p1x) * (p1y - p2y) - (p1x - p2x) * (p0y - p1y);
+    mw[0] = weight * (bx * (p1y - p2y) + by * (p2x - p1x) + p1x * p2y - p1y * p2x) / d;
+    mw[1] = weight * (bx * (p2y - p0y) + by * (p0x - p2x) + p0y * p2x - p0x * p2y) / d;
+    mw[2] = weight * (bx * (p0y - p1y) + by * (p1x - p0x) + p0x * p1y - p0y * p1x) / d;
+    b.cmapped[c0] = 1;
+    b.cmapped[c1] = 1;
+    b.cmapped[c2] = 1;
+    b.pmapped[p] = 1;
+}
+
+void launch_raytrace_map(hipStream_t stream, const GridDev& g, const ModelPrep& m, double tol, int ray_fp32,
+                         const ItemBufs& b, const void* /*surrogate*/)
+{
+    long long total = (long long)g.nx * g.ny * g.nsub;
+    int blocks = (int)((total + 255) / 256);
+    if (ray_fp32)
+        hipLaunchKernelGGL(k_raytrace_map<float>, dim3(blocks), dim3(256), 0, stream, g, m, (float)tol, b);
+    else
+        hipLaunchKernelGGL(k_raytrace_map<double>, dim3(blocks), dim3(256), 0, stream, g, m, tol, b);
+}
+
+__global__ void __launch_bounds__(256) k_raytrace_points(ModelPrep m, double tol, int n, const double* xy, double* beta)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double x = xy[2 * t], y = xy[2 * t + 1];
+    double a0, a1;
+    int nev = 0;
+    total_deflection<double>(m, x, y, tol, a0, a1, nev);
+    beta[2 * t] = x - a0;
+    beta[2 * t + 1] = y - a1;
+}
+
+void launch_raytrace_points(hipStream_t stream, const ModelPrep& m, double tol, int n, const double* xy, double* beta)
+{
+    hipLaunchKernelGGL(k_raytrace_points, dim3((n + 255) / 256), dim3(256), 0, stream, m, tol, n, xy, beta);
+}
+
+// Lens-frame NFW deflection and Hessian (profile.cpp:2532-2560): (ax, ay, psi_xx, psi_xy, psi_yy)
+__global__ void __launch_bounds__(256) k_nfw_def_hess(LensPrep L, double tol, int n, const double* xy, double* out5)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double x = xy[2 * t], y = xy[2 * t + 1];
+    double j0, j1;
+    nfw_j01<double>(L, x, y, tol, j0, j1);
+    double k0 = patterson_k<double>(L, x, y, tol, 0.5);
+    double k1 = patterson_k<double>(L, x, y, tol, 1.5);
+    double k2 = patterson_k<double>(L, x, y, tol, 2.5);
+    out5[5 * t + 0] = x * j0;
+    out5[5 * t + 1] = y * j1;
+    out5[5 * t + 2] = 2 * x * x * k0 + j0;
+    out5[5 * t + 3] = 2 * x * y * k1;
+    out5[5 * t + 4] = 2 * y * y * k2 + j1;
+}
+
+void launch_nfw_def_hess(hipStream_t stream, const LensPrep& L, double tol, int n, const double* xy, double* out5)
+{
+    hipLaunchKernelGGL(k_nfw_def_hess, dim3((n + 255) / 256), dim3(256), 0, stream, L, tol, n, xy, out5);
+}
+
+// ---- mock surface brightness (find_surface_brightness split branch, pixelgrid.cpp:11835-11896;
+// SB_Profile::surface_brightness + Sersic::sb_rsq, sbprofile.cpp:1346-1366, 2472-2516) ----
+__global__ void __launch_bounds__(256) k_mock_sb(GridDev g, ModelPrep m, double tol, int nsrc, const double* sersic, double* sb)
+{
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long long)g.nx * g.ny) return;
+    const int i = (int)(p / g.ny), j = (int)(p % g.ny);
+    double sbtot = 0;
+    for (int ii = 0; ii < g.nsplit; ii++)
+        for (int jj = 0; jj < g.nsplit; jj++) {
+            const double u0 = ((double)(1 + 2 * ii)) / (2 * g.nsplit);
+            const double w0 = ((double)(1 + 2 * jj)) / (2 * g.nsplit);
+            const double sx = (1 - u0) * g.xvals[i] + u0 * g.xvals[i + 1];
+            const double sy = (1 - w0) * g.yvals[j] + w0 * g.yvals[j + 1];
+            double a0, a1;
+            int nev = 0;
+            total_deflection<double>(m, sx, sy, tol, a0, a1, nev);
+            double bx = sx - a0, by = sy - a1;
+            for (int k = 0; k < nsrc; k++) {
+                const double* S = sersic + 11 * k;  // s0, Reff, n, q, theta, xc, yc, cos, sin, b, rot
+                double x = bx - S[5], y = by - S[6];
+                if (S[10] != 0) {
+                    double xp = x * S[7] + y * S[8];
+                    y = -x * S[8] + y * S[7];
+                    x = xp;
+                }
+                double q = S[3];
+                double rsq_ell = x * x + y * y / (q * q);
+                rsq_ell *= q;
+                double v = S[0] * exp(-S[9] * pow(rsq_ell / (S[1] * S[1]), 0.5 / S[2]));
+                if (v != 0) sbtot += v;
+            }
+        }
+    sb[p] = 0 + sbtot / (g.nsplit * g.nsplit);
+}
+
+void launch_mock_sb(hipStream_t stream, const GridDev& g, const ModelPrep& m, double tol, int nsrc, const double* sersic, double* sb)
+{
+    long long n = (long long)g.nx * g.ny;
+    hipLaunchKernelGGL(k_mock_sb, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, g, m, tol, nsrc, sersic, sb);
+}
+
+// PSF_convolution_pixel_vector direct branch over every pixel (pixelgrid.cpp:14452-14490)
+__global__ void __launch_bounds__(256) k_conv_image(int nx, int ny, const double* psf, int pnx, int pny, const double* in, double* out)
+{
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long long)nx * ny) return;
+    const int k = (int)(p / ny), l = (int)(p % ny);
+    const int nxh = pnx / 2, nyh = pny / 2;
+    double acc = 0;
+    for (int pk = 0; pk < pnx; pk++) {
+        int ii = k + nxh - pk;
+        if (ii < 0 || ii >= nx) continue;
+        for (int pl = 0; pl < pny; pl++) {
+            int jj = l + nyh - pl;
+            if (jj < 0 || jj >= ny) continue;
+            acc += psf[pk * pny + pl] * in[(size_t)ii * ny + jj];
+        }
+    }
+    out[p] = acc;
+}
+
+void launch_conv_image(hipStream_t stream, int nx, int ny, const double* psf, int pnx, int pny, const double* in, double* out)
+{
+    long long n = (long long)nx * ny;
+    hipLaunchKernelGGL(k_conv_image, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, nx, ny, psf, pnx, pny, in, out);
+}
+
+}  // namespace qlx

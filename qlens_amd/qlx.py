@@ -1,0 +1,232 @@
+"""Python host mirror of QLens's pixel-inversion interface over the qlx C ABI (libqlx.so).
+
+`PixelInversion` plays the role of the QLens object for this path: image data, PSF and
+settings are loaded once, then `invert_image_surface_brightness_map(models)` returns -2 ln E for
+each lens model, as QLens::invert_image_surface_brightness_map does (lens.cpp:12293-13195).
+All numerical work runs in the HIP library; there is no CPU fallback: if libqlx.so is missing or
+no GPU is visible, construction raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqlx.so")
+
+QLX_LENS_SIE, QLX_LENS_NFW, QLX_LENS_SHEAR = 1, 2, 3
+QLX_DEF_QUADRATURE, QLX_DEF_SURROGATE = 0, 1
+QLX_PREC_FP64, QLX_PREC_MIXED = 0, 1
+QLX_RAY_FP64, QLX_RAY_FP32 = 0, 1
+QLX_MAX_LENSES = 8
+
+STATUS = {0: "ok", -1: "bad argument", -2: "bad state", -3: "HIP error", -4: "out of memory",
+          -5: "not positive definite", -6: "no source pixels", -7: "out of range"}
+
+
+class QlxLens(C.Structure):
+    _fields_ = [("type", C.c_int), ("def_mode", C.c_int), ("p", C.c_double * 6)]
+
+
+class QlxLensModel(C.Structure):
+    _fields_ = [("nlens", C.c_int), ("lens", QlxLens * QLX_MAX_LENSES)]
+
+
+class QlxOpts(C.Structure):
+    _fields_ = [("device", C.c_int), ("hbm_budget", C.c_size_t)]
+
+
+class QlxSettings(C.Structure):
+    _fields_ = [("nsplit", C.c_int), ("src_xmin", C.c_double), ("src_xmax", C.c_double),
+                ("src_ymin", C.c_double), ("src_ymax", C.c_double), ("src_nx", C.c_int), ("src_ny", C.c_int),
+                ("regparam", C.c_double), ("integral_tol", C.c_double), ("precision", C.c_int),
+                ("ray_precision", C.c_int), ("refine_iters", C.c_int), ("refine_tol", C.c_double)]
+
+
+class QlxResult(C.Structure):
+    _fields_ = [("minus2lnE", C.c_double), ("chisq0", C.c_double), ("logdetF", C.c_double),
+                ("logdetR", C.c_double), ("Es", C.c_double), ("refine_resid", C.c_double),
+                ("n_img_active", C.c_int), ("n_src_active", C.c_int), ("status", C.c_int),
+                ("refine_steps", C.c_int)]
+
+
+class QlxSurrogateCfg(C.Structure):
+    _fields_ = [("n_samples", C.c_int), ("steps", C.c_int), ("batch", C.c_int), ("lr", C.c_double),
+                ("r_min", C.c_double), ("r_max", C.c_double), ("q_min", C.c_double), ("q_max", C.c_double),
+                ("rs_min", C.c_double), ("rs_max", C.c_double), ("seed", C.c_ulonglong), ("tol", C.c_double)]
+
+
+_LIB = None
+
+
+def load_library(path=None):
+    """Load libqlx.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError("libqlx.so not found at %s; build it with `python -m qlens_amd.build`" % p)
+    lib = C.CDLL(p)
+    lib.qlx_last_error.restype = C.c_char_p
+    lib.qlx_last_error.argtypes = [C.c_void_p]
+    for name in ["qlx_ctx_create", "qlx_ctx_destroy", "qlx_set_image", "qlx_set_psf", "qlx_set_psf_gaussian",
+                 "qlx_set_settings", "qlx_eval", "qlx_eval_batch_grids", "qlx_get_solution", "qlx_raytrace",
+                 "qlx_nfw_def_hess", "qlx_mock_image", "qlx_kernel_time", "qlx_selftest_mfma",
+                 "qlx_surrogate_train", "qlx_surrogate_eval", "qlx_surrogate_grad", "qlx_surrogate_param_count",
+                 "qlx_surrogate_get_params", "qlx_surrogate_set_params", "qlx_surrogate_save", "qlx_surrogate_load"]:
+        getattr(lib, name).restype = C.c_int
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_model(lenses, def_mode=QLX_DEF_QUADRATURE):
+    """lenses: [(type, params)] in QLens conventions -> QlxLensModel"""
+    m = QlxLensModel()
+    m.nlens = len(lenses)
+    for k, (t, p) in enumerate(lenses):
+        m.lens[k].type = t
+        m.lens[k].def_mode = def_mode if t == QLX_LENS_NFW else QLX_DEF_QUADRATURE
+        for i, v in enumerate(p):
+            m.lens[k].p[i] = float(v)
+    return m
+
+
+class QlxError(RuntimeError):
+    pass
+
+
+class PixelInversion:
+    """One qlx context (one device stream) holding the image, PSF and inversion settings."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        self.ctx = C.c_void_p()
+        opts = QlxOpts(device, 0)
+        rc = self.lib.qlx_ctx_create(C.byref(opts), C.byref(self.ctx))
+        if rc != 0:
+            raise QlxError("qlx_ctx_create failed (%s): %s" % (STATUS.get(rc, rc), self.lib.qlx_last_error(None).decode()))
+        self.nx = self.ny = 0
+        self.settings = None
+
+    def close(self):
+        if self.ctx:
+            self.lib.qlx_ctx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise QlxError("%s failed (%s): %s" % (what, STATUS.get(rc, rc), self.lib.qlx_last_error(self.ctx).decode()))
+
+    # -- inputs ------------------------------------------------------------------------------
+    def load_image(self, xvals, yvals, data, sigma=None, covinv=None, mask=None):
+        """ImagePixelData equivalent: corners + x-major data; noise as sigma or a covinv map."""
+        xv = np.ascontiguousarray(xvals, dtype=np.float64)
+        yv = np.ascontiguousarray(yvals, dtype=np.float64)
+        nx, ny = len(xv) - 1, len(yv) - 1
+        d = np.ascontiguousarray(data, dtype=np.float64).reshape(nx, ny)
+        if covinv is None:
+            covinv = np.full((nx, ny), 1.0 / (sigma * sigma))
+        ci = np.ascontiguousarray(covinv, dtype=np.float64).reshape(nx, ny)
+        mk = None
+        if mask is not None:
+            mk = np.ascontiguousarray(mask, dtype=np.uint8).reshape(nx, ny)
+        self._check(self.lib.qlx_set_image(self.ctx, nx, ny, _dp(xv), _dp(yv), _dp(d), _dp(ci),
+                                           mk.ctypes.data_as(C.POINTER(C.c_ubyte)) if mk is not None else None),
+                    "qlx_set_image")
+        self.nx, self.ny = nx, ny
+        self._keep = (xv, yv, d, ci, mk)
+
+    def set_psf(self, psf):
+        psf = np.ascontiguousarray(psf, dtype=np.float64)
+        self._check(self.lib.qlx_set_psf(self.ctx, psf.shape[0], psf.shape[1], _dp(psf)), "qlx_set_psf")
+
+    def set_psf_gaussian(self, sigma_x, sigma_y, threshold):
+        self._check(self.lib.qlx_set_psf_gaussian(self.ctx, C.c_double(sigma_x), C.c_double(sigma_y),
+                                                  C.c_double(threshold)), "qlx_set_psf_gaussian")
+
+    def set_settings(self, nsplit, srcgrid, src_n, regparam, integral_tol=1e-3, precision=QLX_PREC_FP64,
+                     ray_precision=QLX_RAY_FP64, refine_iters=0, refine_tol=0.0):
+        s = QlxSettings(nsplit, srcgrid[0], srcgrid[1], srcgrid[2], srcgrid[3], src_n[0], src_n[1], regparam,
+                        integral_tol, precision, ray_precision, refine_iters, refine_tol)
+        self._check(self.lib.qlx_set_settings(self.ctx, C.byref(s)), "qlx_set_settings")
+        self.settings = dict(nsplit=nsplit, srcgrid=srcgrid, src_n=src_n, regparam=regparam,
+                             integral_tol=integral_tol, precision=precision)
+
+    def load_workload(self, case, data, precision=QLX_PREC_FP64, ray_precision=QLX_RAY_FP64):
+        self.load_image(case["xvals"], case["yvals"], data, sigma=case["sigma"], mask=case.get("mask"))
+        self.set_psf(case["psf"])
+        self.set_settings(case["nsplit"], case["srcgrid"], case["src_n"], case["regparam"], case["integral_tol"],
+                          precision=precision, ray_precision=ray_precision)
+
+    # -- evaluation --------------------------------------------------------------------------
+    def invert_image_surface_brightness_map(self, lens_models, def_mode=QLX_DEF_QUADRATURE, src_n=None):
+        """Evaluate -2 ln E for one lens model ([(type, params)]) or a list of them.
+        Returns a list of result dicts (one per model)."""
+        single = len(lens_models) > 0 and isinstance(lens_models[0], tuple) and isinstance(lens_models[0][0], int)
+        models = [lens_models] if single else list(lens_models)
+        arr = (QlxLensModel * max(1, len(models)))()
+        for k, lm in enumerate(models):
+            arr[k] = make_model(lm, def_mode)
+        res = (QlxResult * max(1, len(models)))()
+        if src_n is None:
+            rc = self.lib.qlx_eval(self.ctx, len(models), arr, res)
+        else:
+            sn = np.ascontiguousarray(np.asarray(src_n, dtype=np.int32).reshape(-1, 2))
+            rc = self.lib.qlx_eval_batch_grids(self.ctx, len(models), arr, sn.ctypes.data_as(C.POINTER(C.c_int)), res)
+        if rc < 0 and rc not in (-5, -6):
+            self._check(rc, "qlx_eval")
+        out = [{f: getattr(res[k], f) for f, _ in QlxResult._fields_} for k in range(len(models))]
+        return out
+
+    def get_solution(self):
+        nx, ny = self.settings["src_n"]
+        src = np.zeros((nx, ny))
+        model = np.zeros((self.nx, self.ny))
+        self._check(self.lib.qlx_get_solution(self.ctx, _dp(src), _dp(model)), "qlx_get_solution")
+        return src, model
+
+    def raytrace(self, lenses, xy):
+        xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+        out = np.zeros_like(xy)
+        m = make_model(lenses)
+        self._check(self.lib.qlx_raytrace(self.ctx, C.byref(m), len(xy), _dp(xy), _dp(out)), "qlx_raytrace")
+        return out
+
+    def nfw_def_hess(self, nfw_params, xy, tol=1e-6):
+        xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+        out = np.zeros((len(xy), 5))
+        p = np.ascontiguousarray(nfw_params, dtype=np.float64)
+        self._check(self.lib.qlx_nfw_def_hess(self.ctx, _dp(p), C.c_double(tol), len(xy), _dp(xy), _dp(out)),
+                    "qlx_nfw_def_hess")
+        return out
+
+    def mock_image(self, lenses, sersics, sigma, seed):
+        out = np.zeros((self.nx, self.ny))
+        s = np.ascontiguousarray(np.asarray(sersics, dtype=np.float64).reshape(-1, 7))
+        m = make_model(lenses)
+        self._check(self.lib.qlx_mock_image(self.ctx, C.byref(m), len(s), _dp(s), C.c_double(sigma),
+                                            C.c_ulonglong(seed), _dp(out)), "qlx_mock_image")
+        return out
+
+    def kernel_time(self, name):
+        ms = C.c_double(0)
+        n = C.c_int(0)
+        rc = self.lib.qlx_kernel_time(self.ctx, name.encode(), C.byref(ms), C.byref(n))
+        return (ms.value, n.value) if rc == 0 else (None, 0)
+
+    def selftest_mfma(self):
+        out = np.zeros(256)
+        self._check(self.lib.qlx_selftest_mfma(self.ctx, _dp(out)), "qlx_selftest_mfma")
+        return out

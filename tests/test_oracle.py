@@ -1,0 +1,161 @@
+"""CPU tests pinning the oracle (the checker) to the compiled reference's golden vectors and to
+mathematics.  No GPU needed."""
+import hashlib
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from qlens_amd import workloads as W
+
+REPO = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+
+
+def load(golden_dir, name):
+    return np.load(os.path.join(golden_dir, name))
+
+
+@pytest.mark.parametrize("name,builder", [("C1", W.c1), ("C2s", W.c2_small)])
+def test_oracle_inversion_matches_reference(golden_dir, name, builder):
+    g = load(golden_dir, "inversion_%s.npz" % name)
+    case = builder()
+    res, arr = O.invert(case, g["data"])
+    # same operation order as the reference: equal to the last bit
+    assert res["minus2lnE"] == float(g["minus2lnE"])
+    assert res["chisq0"] == float(g["chisq0"])
+    assert res["logdetF"] == float(g["logdetF"])
+    assert res["logdetR"] == float(g["logdetR"])
+    assert res["n_img"] == int(g["n_img"]) and res["n_src"] == int(g["n_src"])
+    assert np.array_equal(arr["mapped"], g["mapped"])
+    assert np.array_equal(arr["active"], g["active"])
+    assert np.array_equal(arr["src"], g["src"])
+    assert np.array_equal(arr["model"], g["model"])
+    nx, ny = arr["mapped"].shape
+    rays = arr["rays"].reshape(nx * ny, -1, 2)[g["ray_pixels"]]
+    assert np.array_equal(rays, g["rays"])
+
+
+def test_oracle_mock_reproduces_fixture_data(golden_dir):
+    g = load(golden_dir, "inversion_C1.npz")
+    assert np.array_equal(O.mock_image(W.c1()), g["data"])
+
+
+def test_oracle_deflection_matches_reference(golden_dir):
+    g = load(golden_dir, "deflection.npz")
+    with open(os.path.join(golden_dir, "deflection_lenses.json")) as f:
+        sets = json.load(f)
+    for name, lenses in sets.items():
+        lenses = [(int(t), tuple(p)) for t, p in lenses]
+        d, _ = O.deflection(lenses, g["points"], tol=1e-6)
+        # the SIE formula is 0/0 at its exact centre in the reference too (NaN == NaN here)
+        assert np.array_equal(d, g[name + "_total"], equal_nan=True), name
+
+
+def _header_table():
+    h = open(os.path.join(REPO, "qlens_amd", "csrc", "patterson_table.h")).read()
+    X = np.array([float.fromhex(v) for v in re.search(r"QLX_PAT_X\[511\] = \{(.*?)\};", h, re.S).group(1).split(",") if v.strip()])
+    Wt = np.array([float.fromhex(v) for v in re.search(r"QLX_PAT_W\[\d+\] = \{(.*?)\};", h, re.S).group(1).split(",") if v.strip()])
+    return X, Wt
+
+
+def test_patterson_table_equals_reference_digest(golden_dir):
+    X, Wt = _header_table()
+    dig = dict(line.split() for line in open(os.path.join(golden_dir, "patterson_digest.txt")))
+    assert hashlib.sha256(X.tobytes()).hexdigest() == dig["points_sha256"]
+    assert hashlib.sha256(Wt.tobytes()).hexdigest() == dig["weights_sha256"]
+
+
+def test_patterson_levels_integrate_polynomials_exactly():
+    # level L >= 1 (n = 2^(L+1)-1 points) integrates polynomials exactly up to degree 3*2^L - 1
+    # (3-point Gauss: 5, 7 points: 11, 15 points: 23, ...; Patterson 1968); level 0 (midpoint): 1.
+    # Check every monomial up to min(degree, 60) on [-1, 1], and that degree+1 (even) is NOT exact.
+    X, Wt = _header_table()
+    off = 0
+    for L in range(9):
+        n = 2 ** (L + 1) - 1
+        step = 512 // (n + 1)
+        nodes = X[[(k + 1) * step - 1 for k in range(n)]]
+        w = Wt[off:off + n]
+        off += n
+        deg = 1 if L == 0 else 3 * 2 ** L - 1
+        for p in range(0, min(deg, 60) + 1):
+            exact = 0.0 if p % 2 else 2.0 / (p + 1)
+            assert abs(np.dot(w, nodes ** p) - exact) < 1e-13, (L, p)
+        if deg + 1 <= 24:
+            assert abs(np.dot(w, nodes ** (deg + 1)) - 2.0 / (deg + 2)) > 1e-10, (L, "too exact")
+
+
+def test_nfw_quadrature_q_to_one_matches_closed_form():
+    # q -> 1: alpha = kappa_avg(r) * x with the closed-form spherical kappa_avg (models.cpp:1001-1010)
+    pts = np.array([[0.3, 0.4], [1.5, -2.0], [4.0, 1.0], [7.0, 3.0]])
+    ks, rs = 0.2, 5.0
+    d_sph, _ = O.deflection([(W.NFW, (ks, rs, 1.0, 0.0, 0.0, 0.0))], pts)
+    d_q, _ = O.deflection([(W.NFW, (ks, rs, 1.0 - 1e-9, 0.0, 0.0, 0.0))], pts, tol=1e-10)
+    r2 = (pts ** 2).sum(1)
+    x2 = r2 / rs ** 2
+    F = np.where(x2 > 1, np.arctan(np.sqrt(np.abs(x2 - 1))) / np.sqrt(np.abs(x2 - 1)),
+                 np.arctanh(np.sqrt(np.abs(1 - x2))) / np.sqrt(np.abs(1 - x2)))
+    kap = 2 * ks * (2 * F + np.log(x2 / 4)) / x2
+    assert np.allclose(d_sph, kap[:, None] * pts, rtol=1e-13)
+    assert np.allclose(d_q, d_sph, rtol=1e-7)
+
+
+def test_nfw_laplacian_equals_two_kappa():
+    # psi_xx + psi_yy = 2 kappa (lens frame), from the K and J integrals
+    lens = (0.2, 5.0, 0.7, 0.0, 0.0, 0.0)
+    pts = np.array([[0.5, 0.2], [2.0, -1.0], [6.0, 3.0], [0.05, 0.01]])
+    out = O.nfw_def_hess(lens, pts, tol=1e-10)
+    q = 0.7
+    xisq = (pts[:, 0] ** 2 + pts[:, 1] ** 2 / q ** 2) * q   # emode 1 elliptical radius^2
+    x2 = xisq / 25.0
+    F = np.where(x2 > 1, np.arctan(np.sqrt(np.abs(x2 - 1))) / np.sqrt(np.abs(x2 - 1)),
+                 np.arctanh(np.sqrt(np.abs(1 - x2))) / np.sqrt(np.abs(1 - x2)))
+    kappa = 2 * 0.2 * (1 - F) / (x2 - 1)
+    assert np.allclose(out[:, 2] + out[:, 4], 2 * kappa, rtol=1e-7)
+
+
+def test_sie_deflection_is_potential_gradient_consistent():
+    # curl-free: d(alpha_x)/dy == d(alpha_y)/dx (finite differences), and the known
+    # Kormann-Keeton closed form with s = 0 at q = 0.8
+    lens = [(W.SIE, (1.2, 0.8, 0.0, 0.0, 0.0, 0.0))]
+    h = 1e-5
+    for x, y in [(0.7, 0.3), (-1.1, 0.9), (0.2, -1.5)]:
+        pts = np.array([[x, y + h], [x, y - h], [x + h, y], [x - h, y]])
+        d, _ = O.deflection(lens, pts)
+        dax_dy = (d[0, 0] - d[1, 0]) / (2 * h)
+        day_dx = (d[2, 1] - d[3, 1]) / (2 * h)
+        assert abs(dax_dy - day_dx) < 1e-7
+
+
+def test_normal_deviates_are_standard_normal():
+    z = O.normal_deviates(10, 200000)
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
+    # pairs come out second-member-first; both seeds deterministic
+    assert np.array_equal(O.normal_deviates(10, 8), O.normal_deviates(10, 8))
+
+
+def test_gaussian_psf_generation_matches_reference_rule():
+    # threshold rounding: sigma_fraction*sigma/step = 2.3756 -> half-width 2 -> 5x5 (pixelgrid.cpp:14616-14621)
+    p = O.gaussian_psf(0.1 / 2.35482, 0.1 / 2.35482, 0.02, 0.05, 0.05)
+    assert p.shape == (5, 5)
+    assert abs(p.sum() - 1) < 1e-15
+    assert np.array_equal(p, W.gaussian_psf(0.1 / 2.35482, 0.02, 0.05, 0.05))
+    # default threshold 0.01 gives 7x7 for the same sigma
+    assert O.gaussian_psf(0.1 / 2.35482, 0.1 / 2.35482, 0.01, 0.05, 0.05).shape == (7, 7)
+
+
+def test_sentinel_for_too_small_source_grid():
+    case = W.c1()
+    case["src_n"] = (1, 30)
+    res, _ = O.invert(case, np.zeros((100, 100)), want_arrays=False)
+    assert res["minus2lnE"] == 2e30
+
+
+def test_zero_source_pixels_sentinel():
+    case = W.c1()
+    case["srcgrid"] = (50.0, 51.0, 50.0, 51.0)  # far from every ray
+    res, _ = O.invert(case, np.zeros((100, 100)), want_arrays=False)
+    assert res["minus2lnE"] == 2e30 and res["n_src"] == 0
