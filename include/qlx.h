@@ -175,6 +175,20 @@ int qlx_surrogate_grad(void *ctx, int n, const float *x, const float *y, float *
  * context's stream; names: "syrk", "lbuild", "raytrace", "cholesky", "total". */
 int qlx_kernel_time(void *ctx, const char *name, double *ms, int *launches);
 
+/* Work counters of the last evaluated item (no reference counterpart; benchmark accounting).
+ *   syrk_dense_flops   2*64^3 per (lower tile, 64-row chunk) pair of the dense SYRK
+ *   syrk_exec_flops    the same over the pairs actually executed (empty tiles skipped)
+ *   syrk_alg_flops     N_img*N_src*(N_src+1) + 2*N_img*N_src (the F and D arithmetic)
+ *   lb_bytes           bytes of the blurred lensing matrix written (rows x padded cols x elem)
+ *   integrand_evals    NFW integrand evaluations of the ray shooting
+ *   chol_flops         2 * N_src^3 / 3 (two factorizations: F and R) */
+typedef struct {
+    double syrk_dense_flops, syrk_exec_flops, syrk_alg_flops;
+    double lb_bytes, integrand_evals, chol_flops;
+    int n_img, n_src, ld_src, rows_pad;
+} qlx_stats;
+int qlx_get_stats(void *ctx, qlx_stats *out);
+
 /* Diagnostic (no reference counterpart): one v_mfma_f64_16x16x4f64 on fixed operands
  * A[i][k] = i + 1 + 0.01k, B[k][j] = 1 + k + 0.1j; out[lane*4 + r] = result register r of lane.
  * Lets a test confirm the accumulator layout the FP64 tile kernels assume

@@ -2,8 +2,8 @@
 
 Writes the workload's image data as QLens text files (root.dat x-major, root.x, root.y, all
 with 17 significant digits so every double round-trips), a QLens command script, and runs the
-harness.  Used by tests/golden/make_golden.py to produce the committed fixtures; never used by
-the product.
+harness.  Used by tests/golden/make_golden.py (fixtures) and by bench.py's cpu_baseline leg;
+never by the product.
 """
 import os
 import subprocess
@@ -11,7 +11,7 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-REPO = os.path.abspath(os.path.join(HERE, "..", ".."))
+REPO = os.path.abspath(os.path.join(HERE, ".."))
 HARNESS = os.path.join(REPO, "oracle", "_ref", "ref_harness")
 
 SIE, NFW, SHEAR = 1, 2, 3

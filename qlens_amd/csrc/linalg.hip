@@ -41,6 +41,17 @@ static inline unsigned grid_for(size_t n, unsigned bs = 256, unsigned cap = 6553
     if (g > cap) g = cap;
     return (unsigned)g;
 }
+// gfx950 has 160 KB of LDS per CU; allow kernels with dynamic LDS to use it (set once per kernel).
+template <typename K>
+static void allow_big_lds(K kernel)
+{
+    static bool done = false;
+    if (!done) {
+        hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 48 * 1024);
+        done = true;
+    }
+}
+
 void launch_fill_i32(hipStream_t s, int* p, int v, size_t n)
 {
     if (n) hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n)), dim3(256), 0, s, p, v, n);
@@ -211,6 +222,7 @@ void launch_build_lb(hipStream_t s, const GridDev& g, const ItemBufs& b, int n_i
 {
     if (n_img <= 0) return;
     size_t lds = (size_t)ldL * sizeof(double);
+    allow_big_lds(k_build_lb);
     hipLaunchKernelGGL(k_build_lb, dim3(n_img), dim3(64), lds, s, g, b, n_img, n_src, psf, pnx, pny, data, Lb, ldL, nzmask, ncbw);
 }
 
@@ -401,9 +413,11 @@ __global__ void __launch_bounds__(256) k_curvature(GridDev g, ItemBufs b, double
     }
 }
 
-// F_lower += lambda*R; D <- row n of F (the data column), then rows/cols >= n become identity
-// padding in both F and R so the padded factorization is well defined with log det unchanged.
-__global__ void __launch_bounds__(256) k_prep_fr(double* F, double* R, int ld, int n, double lambda, double* D)
+// F_lower += lambda*R for the n x n source block.  Row n of F keeps the data row (D^T and d^T C d
+// from the SYRK) so the Cholesky of the augmented matrix [[F, D], [D^T, d^T C d]] delivers
+// y = L^-1 D as its last row; rows/cols > n (and >= n in R) become identity padding, which leaves
+// the log-determinants of the leading n x n blocks unchanged.
+__global__ void __launch_bounds__(256) k_prep_fr(double* F, double* R, int ld, int n, double lambda)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long total = (long long)ld * ld;
@@ -413,9 +427,8 @@ __global__ void __launch_bounds__(256) k_prep_fr(double* F, double* R, int ld, i
         if (r < n) {
             F[t] += lambda * R[t];
         } else {
-            if (r == n && c < n) D[c] = F[t];
             double v = (r == c) ? 1.0 : 0.0;
-            F[t] = v;
+            if (r > n) F[t] = v;
             R[t] = v;
         }
     }
@@ -432,68 +445,17 @@ void launch_add_curvature(hipStream_t s, const GridDev& g, const ItemBufs& b, in
     (void)ldF;
 }
 
-void launch_prep_fr(hipStream_t s, double* F, double* R, int ld, int n, double lambda, double* D)
+void launch_prep_fr(hipStream_t s, double* F, double* R, int ld, int n, double lambda)
 {
     size_t total = (size_t)ld * ld;
-    hipLaunchKernelGGL(k_prep_fr, dim3(grid_for(total, 256, 8192)), dim3(256), 0, s, F, R, ld, n, lambda, D);
+    hipLaunchKernelGGL(k_prep_fr, dim3(grid_for(total, 256, 8192)), dim3(256), 0, s, F, R, ld, n, lambda);
 }
 
 // ------------------------------------------------------------------------------------------------
 // Blocked Cholesky (lower, row-major), 64x64 blocks: diagonal factor + inverse, panel solve by
 // multiplication with the inverse (MFMA), trailing update (MFMA).
 // ------------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_chol_diag(double* A, int ld, int kb, double* diag, int* status, double* Winv)
-{
-    __shared__ double L[TS][TS + 1];
-    __shared__ double X[TS][TS + 1];
-    const int tid = threadIdx.x;
-    const long long base = (long long)kb * TS;
-    for (int e = tid; e < TS * TS; e += 256) {
-        int r = e >> 6, c = e & 63;
-        L[r][c] = A[(base + r) * ld + base + c];
-    }
-    __syncthreads();
-    for (int c = 0; c < TS; c++) {
-        if (tid == 0) {
-            double d = L[c][c];
-            if (d < 0) atomicOr(status, 1);
-            d = sqrt(fabs(d));
-            L[c][c] = d;
-            diag[base + c] = d;
-        }
-        __syncthreads();
-        if (tid > c && tid < TS) L[tid][c] /= L[c][c];
-        __syncthreads();
-        for (int e = tid; e < TS * TS; e += 256) {
-            int r = e >> 6, q = e & 63;
-            if (q > c && r >= q) L[r][q] -= L[r][c] * L[q][c];
-        }
-        __syncthreads();
-    }
-    // write back L (lower) with zeros above the diagonal
-    for (int e = tid; e < TS * TS; e += 256) {
-        int r = e >> 6, c = e & 63;
-        A[(base + r) * ld + base + c] = (c <= r) ? L[r][c] : 0.0;
-    }
-    // inverse of the lower-triangular block, column by column
-    if (tid < TS) {
-        const int c = tid;
-        for (int r = 0; r < TS; r++) {
-            if (r < c) { X[r][c] = 0.0; continue; }
-            double s = (r == c) ? 1.0 : 0.0;
-            for (int q = c; q < r; q++) s -= L[r][q] * X[q][c];
-            X[r][c] = s / L[r][r];
-        }
-    }
-    __syncthreads();
-    double* W = Winv + (size_t)kb * TS * TS;
-    for (int e = tid; e < TS * TS; e += 256) {
-        int r = e >> 6, c = e & 63;
-        W[r * TS + c] = X[r][c];
-    }
-}
-
-// A[ib][kb] <- A[ib][kb] * Winv^T for ib > kb (one workgroup per row block)
+// A[ib][kb] <- A[ib][kb] * Winv^T for ib > kb (one workgroup per row block), Winv = L[kb][kb]^-1
 __global__ void __launch_bounds__(256) k_chol_panel(double* A, int ld, int kb, const double* Winv)
 {
     __shared__ double As[TS][LDSP];
@@ -537,13 +499,14 @@ __global__ void __launch_bounds__(256) k_chol_update(double* A, int ld, int kb)
     tile_store(A + (long long)I * TS * ld + (long long)J * TS, ld, acc, 1);
 }
 
+void launch_chol_diag(hipStream_t s, double* A, int ld, int kb, double* diag, int* status, double* Winv, int nflag);
+
 int cholesky_f64(hipStream_t s, double* A, int ld, int n, double* diag, int* status, double* work)
 {
     const int nb = ld / TS;
-    (void)n;
     for (int kb = 0; kb < nb; kb++) {
-        hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, s, A, ld, kb, diag, status, work);
-        int rest = nb - kb - 1;
+        const int rest = nb - kb - 1;
+        launch_chol_diag(s, A, ld, kb, diag, status, work, n);
         if (rest > 0) {
             hipLaunchKernelGGL(k_chol_panel, dim3(rest), dim3(256), 0, s, A, ld, kb, work);
             hipLaunchKernelGGL(k_chol_update, dim3(rest * (rest + 1) / 2), dim3(256), 0, s, A, ld, kb);
@@ -553,53 +516,85 @@ int cholesky_f64(hipStream_t s, double* A, int ld, int n, double* diag, int* sta
 }
 
 // Forward + backward substitution with the stored block inverses (single workgroup).
-__global__ void __launch_bounds__(1024) k_trisolve_f64(const double* L, int ld, int nb, const double* Winv, const double* bvec,
-                                                       double* x)
+// Forward: rhs_k = b_k - L[k, <k] y_<k (16 threads per row read consecutive columns), then
+// y_k = Winv_k rhs_k.  Backward: rhs_k = y_k - L[>k, k]^T x_>k (threads over columns), then
+// x_k = Winv_k^T rhs_k.  Each block inverse is staged through LDS.
+template <typename T>
+__global__ void __launch_bounds__(1024) k_trisolve(const T* L, int ld, int nb, const T* Winv, const T* bvec, const T* yrow, int n,
+                                                   T* x)
 {
-    extern __shared__ double sx[];  // ld doubles
-    __shared__ double part[16][TS + 1];
-    __shared__ double rhs[TS];
+    extern __shared__ unsigned char smem_raw[];
+    T* sx = (T*)smem_raw;  // ld entries
+    __shared__ T part[16][TS + 1];
+    __shared__ T rhs[TS];
+    __shared__ T Wl[TS][TS + 1];
     const int tid = threadIdx.x;
-    const int r = tid & 63, t16 = tid >> 6;
-    // forward: y = L^-1 b
-    for (int kb = 0; kb < nb; kb++) {
-        const int row = kb * TS + r;
-        double acc = 0.0;
-        for (int c = t16; c < kb * TS; c += 16) acc += L[(long long)row * ld + c] * sx[c];
-        part[t16][r] = acc;
+    if (yrow) {
+        // y = L^-1 D was produced by the augmented factorization (row n of L)
+        for (int c = tid; c < nb * TS; c += 1024) sx[c] = (c < n) ? yrow[c] : (T)0;
         __syncthreads();
-        if (tid < TS) {
-            double s = bvec[row];
-            for (int q = 0; q < 16; q++) s -= part[q][r];
-            rhs[r] = s;
+    } else {
+        for (int kb = 0; kb < nb; kb++) {
+            {
+                const int r = tid >> 4, t16 = tid & 15;
+                const int row = kb * TS + r;
+                const T* Lr = L + (long long)row * ld;
+                T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+                int c = t16;
+                for (; c + 48 < kb * TS; c += 64) {
+                    a0 = fma(Lr[c], sx[c], a0);
+                    a1 = fma(Lr[c + 16], sx[c + 16], a1);
+                    a2 = fma(Lr[c + 32], sx[c + 32], a2);
+                    a3 = fma(Lr[c + 48], sx[c + 48], a3);
+                }
+                for (; c < kb * TS; c += 16) a0 = fma(Lr[c], sx[c], a0);
+                part[t16][r] = (a0 + a1) + (a2 + a3);
+            }
+            const T* W = Winv + (size_t)kb * TS * TS;
+            for (int e = tid; e < TS * TS; e += 1024) Wl[e >> 6][e & 63] = W[e];
+            __syncthreads();
+            if (tid < TS) {
+                T s = bvec[kb * TS + tid];
+                for (int q = 0; q < 16; q++) s -= part[q][tid];
+                rhs[tid] = s;
+            }
+            __syncthreads();
+            if (tid < TS) {
+                T y = 0;
+                for (int q = 0; q <= tid; q++) y = fma(Wl[tid][q], rhs[q], y);
+                sx[kb * TS + tid] = y;
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        if (tid < TS) {
-            const double* W = Winv + (size_t)kb * TS * TS;
-            double y = 0.0;
-            for (int q = 0; q <= r; q++) y += W[r * TS + q] * rhs[q];
-            sx[row] = y;
-        }
-        __syncthreads();
     }
-    // backward: x = L^-T y
     for (int kb = nb - 1; kb >= 0; kb--) {
-        const int col = kb * TS + r;
-        double acc = 0.0;
-        for (int rr = (kb + 1) * TS + t16; rr < nb * TS; rr += 16) acc += L[(long long)rr * ld + col] * sx[rr];
-        part[t16][r] = acc;
+        {
+            const int r = tid & 63, t16 = tid >> 6;
+            const int col = kb * TS + r;
+            T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+            int rr = (kb + 1) * TS + t16;
+            for (; rr + 48 < nb * TS; rr += 64) {
+                a0 = fma(L[(long long)rr * ld + col], sx[rr], a0);
+                a1 = fma(L[(long long)(rr + 16) * ld + col], sx[rr + 16], a1);
+                a2 = fma(L[(long long)(rr + 32) * ld + col], sx[rr + 32], a2);
+                a3 = fma(L[(long long)(rr + 48) * ld + col], sx[rr + 48], a3);
+            }
+            for (; rr < nb * TS; rr += 16) a0 = fma(L[(long long)rr * ld + col], sx[rr], a0);
+            part[t16][r] = (a0 + a1) + (a2 + a3);
+        }
+        const T* W = Winv + (size_t)kb * TS * TS;
+        for (int e = tid; e < TS * TS; e += 1024) Wl[e >> 6][e & 63] = W[e];
         __syncthreads();
         if (tid < TS) {
-            double s = sx[col];
-            for (int q = 0; q < 16; q++) s -= part[q][r];
-            rhs[r] = s;
+            T s = sx[kb * TS + tid];
+            for (int q = 0; q < 16; q++) s -= part[q][tid];
+            rhs[tid] = s;
         }
         __syncthreads();
         if (tid < TS) {
-            const double* W = Winv + (size_t)kb * TS * TS;
-            double v = 0.0;
-            for (int q = r; q < TS; q++) v += W[q * TS + r] * rhs[q];
-            sx[col] = v;
+            T v = 0;
+            for (int q = tid; q < TS; q++) v = fma(Wl[q][tid], rhs[q], v);
+            sx[kb * TS + tid] = v;
         }
         __syncthreads();
     }
@@ -617,11 +612,23 @@ void launch_trisolve_f64(hipStream_t s, const double* L, int ld, int n, const do
     (void)x;
 }
 
-void launch_trisolve_f64_w(hipStream_t s, const double* L, int ld, const double* Winv, const double* b, double* x)
+// x = F^-1 b with F = L L^T; with yrow != nullptr the forward half is skipped and y = yrow[0..n)
+void launch_trisolve_f64_w(hipStream_t s, const double* L, int ld, const double* Winv, const double* b, const double* yrow, int n,
+                           double* x)
 {
     const int nb = ld / TS;
     size_t lds = (size_t)ld * sizeof(double);
-    hipLaunchKernelGGL(k_trisolve_f64, dim3(1), dim3(1024), lds, s, L, ld, nb, Winv, b, x);
+    allow_big_lds(k_trisolve<double>);
+    hipLaunchKernelGGL(k_trisolve<double>, dim3(1), dim3(1024), lds, s, L, ld, nb, Winv, b, yrow, n, x);
+}
+
+void launch_trisolve_f32_w(hipStream_t s, const float* L, int ld, const float* Winv, const float* b, const float* yrow, int n,
+                           float* x)
+{
+    const int nb = ld / TS;
+    size_t lds = (size_t)ld * sizeof(float);
+    allow_big_lds(k_trisolve<float>);
+    hipLaunchKernelGGL(k_trisolve<float>, dim3(1), dim3(1024), lds, s, L, ld, nb, Winv, b, yrow, n, x);
 }
 
 // 2 * sum log(diag) in a fixed order (Cholesky_logdet_lower_packed, pixelgrid.cpp:17201-17210)

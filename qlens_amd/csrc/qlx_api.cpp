@@ -16,8 +16,9 @@
 
 namespace qlx {
 void upload_patterson(int device);
-void launch_prep_fr(hipStream_t s, double* F, double* R, int ld, int n, double lambda, double* D);
-void launch_trisolve_f64_w(hipStream_t s, const double* L, int ld, const double* Winv, const double* b, double* x);
+void launch_prep_fr(hipStream_t s, double* F, double* R, int ld, int n, double lambda);
+void launch_trisolve_f64_w(hipStream_t s, const double* L, int ld, const double* Winv, const double* b, const double* yrow, int n,
+                           double* x);
 void launch_selftest_mfma(hipStream_t s, double* out);
 }  // namespace qlx
 
@@ -63,6 +64,8 @@ struct Timer {
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;        // runs the R factorization beside F's
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::string err;
     // image
     int nx = 0, ny = 0;
@@ -82,13 +85,15 @@ struct Ctx {
     DevBuf sxv, syv, scx, scy;
     // per-item work
     DevBuf beta, map_cell, map_w, pmapped, cmapped, sidx, pidx, act_pix, counts, ucol, uval, ucnt, nevals;
+    DevBuf key, bucket_cnt, bucket_off, bucket_fill, perm;
     // dense
-    DevBuf Lb, cvec, F, R, D, s, diagF, diagR, work, nz, model, out, logdets, status;
+    DevBuf Lb, cvec, F, R, D, s, diagF, diagR, work, work2, nz, model, out, logdets, status;
     // last solution
     std::vector<double> last_src, last_model;
     int last_snx = 0, last_sny = 0;
     // timers
     std::map<std::string, Timer> timers;
+    qlx_stats stats{};
 };
 
 thread_local std::string g_err;
@@ -268,7 +273,9 @@ int ensure_item_bufs(Ctx* c, int snx, int sny)
               c->pmapped.ensure(npix * 4) && c->cmapped.ensure(ncell * 4) && c->sidx.ensure(ncell * 4) && c->pidx.ensure(npix * 4) &&
               c->act_pix.ensure(npix * 4) && c->counts.ensure(16 * 4) && c->ucol.ensure(npix * U * 4) && c->uval.ensure(npix * U * 8) &&
               c->ucnt.ensure(npix * 4) && c->nevals.ensure(8) && c->out.ensure(16 * 8) && c->logdets.ensure(4 * 8) &&
-              c->status.ensure(4 * 4) && c->model.ensure((npix * 2 + 64) * 8);
+              c->status.ensure(4 * 4) && c->model.ensure((npix * 2 + 64) * 8) && c->key.ensure(npix * 4) &&
+              c->bucket_cnt.ensure((ncell + 2) * 4) && c->bucket_off.ensure((ncell + 2) * 4) &&
+              c->bucket_fill.ensure((ncell + 2) * 4) && c->perm.ensure(npix * 4);
     if (!ok) return fail(c, QLX_ERR_NOMEM, "work buffer allocation failed");
     return QLX_OK;
 }
@@ -289,6 +296,11 @@ ItemBufs item_bufs(Ctx* c)
     b.uval = c->uval.as<double>();
     b.ucnt = c->ucnt.as<int>();
     b.nevals = (long long*)c->nevals.p;
+    b.key = c->key.as<int>();
+    b.bucket_cnt = c->bucket_cnt.as<int>();
+    b.bucket_off = c->bucket_off.as<int>();
+    b.bucket_fill = c->bucket_fill.as<int>();
+    b.perm = c->perm.as<int>();
     return b;
 }
 
@@ -335,31 +347,38 @@ int eval_one_fp64(Ctx* c, const qlx_lens_model& model, int snx, int sny, qlx_res
     const int ncb = ldN / 64, ncbw = (ncb + 31) / 32, nkt = nrows / 64;
     if (!c->Lb.ensure((size_t)nrows * ldN * 8) || !c->cvec.ensure((size_t)nrows * 8) || !c->F.ensure((size_t)ldN * ldN * 8) ||
         !c->R.ensure((size_t)ldN * ldN * 8) || !c->D.ensure((size_t)ldN * 8) || !c->s.ensure((size_t)ldN * 8) ||
-        !c->diagF.ensure((size_t)ldN * 8) || !c->diagR.ensure((size_t)ldN * 8) || !c->work.ensure((size_t)ldN * 64 * 8) ||
+        !c->diagF.ensure((size_t)ldN * 8) || !c->diagR.ensure((size_t)ldN * 8) || !c->work.ensure((size_t)ldN * 64 * 8) || !c->work2.ensure((size_t)ldN * 64 * 8) ||
         !c->nz.ensure((size_t)nkt * ncbw * 4))
         return fail(c, QLX_ERR_NOMEM, "dense buffer allocation failed (Lb/F/R)");
     double* Lb = c->Lb.as<double>();
-    launch_fill_f64(st, Lb + (size_t)n_img * ldN, 0.0, (size_t)(nrows - n_img) * ldN);
     launch_fill_i32(st, c->nz.as<int>(), 0, (size_t)nkt * ncbw);
     timer_begin(c, "lbuild");
-    launch_build_lb(st, g, b, n_img, n_src, c->psf.as<double>(), c->pnx, c->pny, c->data.as<double>(), Lb, ldN,
-                    c->nz.as<unsigned int>(), ncbw);
+    launch_sort_rows(st, g, b, n_img, n_src);
+    launch_row_support(st, g, b, n_img, n_src, c->pnx, c->pny, c->nz.as<unsigned int>(), ncbw);
+    launch_build_lb_blocks_f64(st, g, b, n_img, nrows, n_src, c->psf.as<double>(), c->pnx, c->pny, c->data.as<double>(), Lb, ncb,
+                               c->nz.as<unsigned int>(), ncbw);
     timer_end(c, "lbuild");
-    launch_gather_cvec(st, b, c->covinv.as<double>(), nullptr, n_img, c->cvec.as<double>(), nullptr);
+    launch_gather_cvec_sorted(st, b, c->covinv.as<double>(), n_img, nrows, c->cvec.as<double>());
     timer_begin(c, "syrk");
-    launch_syrk_f64(st, Lb, ldN, c->cvec.as<double>(), nrows, ldN, c->F.as<double>(), ldN, c->nz.as<unsigned int>(), ncbw);
+    launch_syrk_blocks_f64(st, Lb, ncb, nkt, c->cvec.as<double>(), c->F.as<double>(), ldN, c->nz.as<unsigned int>(), ncbw);
     timer_end(c, "syrk");
     launch_fill_f64(st, c->R.as<double>(), 0.0, (size_t)ldN * ldN);
     launch_add_curvature(st, g, b, n_src, c->set.regparam, c->F.as<double>(), ldN, c->R.as<double>(), ldN);
-    launch_prep_fr(st, c->F.as<double>(), c->R.as<double>(), ldN, n_src, c->set.regparam, c->D.as<double>());
+    launch_prep_fr(st, c->F.as<double>(), c->R.as<double>(), ldN, n_src, c->set.regparam);
+    // R (ln det R only) and F (solve + ln det F) factor concurrently on two streams
     timer_begin(c, "cholesky");
-    cholesky_f64(st, c->R.as<double>(), ldN, n_src, c->diagR.as<double>(), c->status.as<int>() + 1, c->work.as<double>());
-    launch_logdet(st, c->diagR.as<double>(), n_src, c->logdets.as<double>() + 1);
+    HIPCHK(c, hipEventRecord(c->ev_fork, st));
+    HIPCHK(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    cholesky_f64(c->stream2, c->R.as<double>(), ldN, n_src, c->diagR.as<double>(), c->status.as<int>() + 1, c->work2.as<double>());
+    launch_logdet(c->stream2, c->diagR.as<double>(), n_src, c->logdets.as<double>() + 1);
+    HIPCHK(c, hipEventRecord(c->ev_join, c->stream2));
     cholesky_f64(st, c->F.as<double>(), ldN, n_src, c->diagF.as<double>(), c->status.as<int>(), c->work.as<double>());
     launch_logdet(st, c->diagF.as<double>(), n_src, c->logdets.as<double>());
+    HIPCHK(c, hipStreamWaitEvent(st, c->ev_join, 0));
     timer_end(c, "cholesky");
-    launch_fill_f64(st, c->D.as<double>() + n_src, 0.0, (size_t)(ldN - n_src));
-    launch_trisolve_f64_w(st, c->F.as<double>(), ldN, c->work.as<double>(), c->D.as<double>(), c->s.as<double>());
+    // backward solve only: y = L^-1 D is row n_src of the factored augmented matrix
+    launch_trisolve_f64_w(st, c->F.as<double>(), ldN, c->work.as<double>(), nullptr, c->F.as<double>() + (size_t)n_src * ldN, n_src,
+                          c->s.as<double>());
     launch_evidence(st, g, b, n_img, n_src, c->s.as<double>(), c->psf.as<double>(), c->pnx, c->pny, c->data.as<double>(),
                     c->covinv.as<double>(), c->set.regparam, c->logdets.as<double>(), c->model.as<double>(), c->out.as<double>());
     timer_end(c, "total");
@@ -377,6 +396,32 @@ int eval_one_fp64(Ctx* c, const qlx_lens_model& model, int snx, int sny, qlx_res
     res->logdetR = out[3];
     res->Es = out[4];
     res->status = (status[0] || status[1]) ? QLX_ERR_NOT_PD : QLX_OK;
+
+    // work accounting: executed vs dense SYRK tile-chunks from the occupancy bitmap
+    {
+        std::vector<unsigned int> nzh((size_t)nkt * ncbw);
+        HIPCHK(c, hipMemcpy(nzh.data(), c->nz.p, nzh.size() * 4, hipMemcpyDeviceToHost));
+        double exec_pairs = 0;
+        for (int kt = 0; kt < nkt; kt++) {
+            long long bits = 0;
+            for (int w = 0; w < ncbw; w++) bits += __builtin_popcount(nzh[(size_t)kt * ncbw + w]);
+            exec_pairs += (double)bits * (bits + 1) / 2;
+        }
+        const double tile = 2.0 * 64 * 64 * 64;
+        qlx_stats& S = c->stats;
+        S.syrk_exec_flops = exec_pairs * tile;
+        S.syrk_dense_flops = (double)ncb * (ncb + 1) / 2 * nkt * tile;
+        S.syrk_alg_flops = (double)n_img * n_src * (n_src + 1) + 2.0 * n_img * n_src;
+        double nblk = 0;
+        for (size_t w = 0; w < nzh.size(); w++) nblk += __builtin_popcount(nzh[w]);
+        S.lb_bytes = nblk * 64 * 64 * 8;
+        S.integrand_evals = (double)nev;
+        S.chol_flops = 2.0 * (double)n_src * n_src * n_src / 3.0;
+        S.n_img = n_img;
+        S.n_src = n_src;
+        S.ld_src = ldN;
+        S.rows_pad = nrows;
+    }
 
     // keep the solution of the last item for qlx_get_solution
     std::vector<double> s(n_src);
@@ -410,7 +455,10 @@ int qlx_ctx_create(const qlx_opts* opts, void** out)
         delete c;
         return fail(nullptr, QLX_ERR_ARG, "invalid device ordinal");
     }
-    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
         delete c;
         return fail(nullptr, QLX_ERR_HIP, "stream creation failed");
     }
@@ -429,6 +477,10 @@ int qlx_ctx_destroy(void* ctx)
         if (kv.second.a) hipEventDestroy(kv.second.a);
         if (kv.second.b) hipEventDestroy(kv.second.b);
     }
+    hipStreamSynchronize(c->stream2);
+    hipEventDestroy(c->ev_fork);
+    hipEventDestroy(c->ev_join);
+    hipStreamDestroy(c->stream2);
     hipStreamDestroy(c->stream);
     delete c;
     return QLX_OK;
@@ -755,6 +807,14 @@ int qlx_kernel_time(void* ctx, const char* name, double* ms, int* launches)
     }
     *ms = it->second.total_ms / it->second.launches;
     if (launches) *launches = it->second.launches;
+    return QLX_OK;
+}
+
+int qlx_get_stats(void* ctx, qlx_stats* out)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !out) return QLX_ERR_ARG;
+    *out = c->stats;
     return QLX_OK;
 }
 

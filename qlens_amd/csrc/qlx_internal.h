@@ -58,6 +58,11 @@ struct ItemBufs {
     double* uval;        // n_img*12
     int* ucnt;           // n_img
     long long* nevals;   // integrand evaluation counter
+    int* key;            // n_img: source cell (active index) of each active pixel's first ray
+    int* bucket_cnt;     // n_src+1
+    int* bucket_off;     // n_src+1
+    int* bucket_fill;    // n_src+1
+    int* perm;           // n_img: sorted row -> active row
 };
 
 // ---- kernel launchers (all asynchronous on `stream`) ----
@@ -115,6 +120,19 @@ void launch_raytrace_points(hipStream_t stream, const ModelPrep& m, double tol, 
                             double* beta);
 void launch_nfw_def_hess(hipStream_t stream, const LensPrep& L, double tol, int n, const double* xy,
                          double* out5);
+
+// block-sparse blurred L and SYRK (blocksparse.hip)
+void launch_sort_rows(hipStream_t s, const GridDev& g, const ItemBufs& b, int n_img, int n_src);
+void launch_row_support(hipStream_t s, const GridDev& g, const ItemBufs& b, int n_img, int n_src, int pnx, int pny, unsigned* nzmask,
+                        int ncbw);
+void launch_build_lb_blocks_f64(hipStream_t s, const GridDev& g, const ItemBufs& b, int n_img, int nrows, int n_src, const double* psf,
+                                int pnx, int pny, const double* data, double* pool, int ncb, const unsigned* nzmask, int ncbw);
+void launch_build_lb_blocks_bf16(hipStream_t s, const GridDev& g, const ItemBufs& b, int n_img, int nrows, int n_src, const double* psf,
+                                 int pnx, int pny, const double* data, const double* covinv, uint16_t* pool, int ncb,
+                                 const unsigned* nzmask, int ncbw);
+void launch_gather_cvec_sorted(hipStream_t s, const ItemBufs& b, const double* covinv, int n_img, int nrows, double* cvec);
+void launch_syrk_blocks_f64(hipStream_t s, const double* pool, int ncb, int nkt, const double* cvec, double* F, int ldF,
+                            const unsigned* nz, int ncbw);
 
 // surrogate (MLP) kernels
 struct MlpDev;

@@ -230,3 +230,17 @@ class PixelInversion:
         out = np.zeros(256)
         self._check(self.lib.qlx_selftest_mfma(self.ctx, _dp(out)), "qlx_selftest_mfma")
         return out
+
+
+class QlxStats(C.Structure):
+    _fields_ = [("syrk_dense_flops", C.c_double), ("syrk_exec_flops", C.c_double), ("syrk_alg_flops", C.c_double),
+                ("lb_bytes", C.c_double), ("integrand_evals", C.c_double), ("chol_flops", C.c_double),
+                ("n_img", C.c_int), ("n_src", C.c_int), ("ld_src", C.c_int), ("rows_pad", C.c_int)]
+
+
+def get_stats(inv):
+    """Work counters of the last evaluated item (qlx_get_stats)."""
+    s = QlxStats()
+    inv.lib.qlx_get_stats.restype = C.c_int
+    inv._check(inv.lib.qlx_get_stats(inv.ctx, C.byref(s)), "qlx_get_stats")
+    return {f: getattr(s, f) for f, _ in QlxStats._fields_}

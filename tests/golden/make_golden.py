@@ -19,8 +19,8 @@ sys.path.insert(0, REPO)
 sys.path.insert(0, HERE)
 
 from oracle import oracle as O  # noqa: E402
+from oracle import refrun  # noqa: E402
 from qlens_amd import workloads as W  # noqa: E402
-import refrun  # noqa: E402
 
 WORK = os.path.join(REPO, "_work", "golden_runs")
 
