@@ -81,7 +81,7 @@ typedef struct {
     double integral_tol;
     int precision;          /* QLX_PREC_* */
     int ray_precision;      /* QLX_RAY_* */
-    int refine_iters;       /* MIXED: max FP64 iterative-refinement steps (<= 0: default 8) */
+    int refine_iters;       /* MIXED: max FP64 iterative-refinement steps (<= 0: default 6) */
     double refine_tol;      /* MIXED: relative residual target (<= 0: default 1e-10) */
 } qlx_settings;
 
@@ -171,29 +171,38 @@ int qlx_surrogate_set_params(void *ctx, const float *params);
 int qlx_surrogate_grad(void *ctx, int n, const float *x, const float *y, float *loss, float *grad);
 
 /* ---- timing support for benchmarks ---------------------------------------------------------
- * Average device time (ms) of the named kernel over the last qlx_eval, from HIP events on the
- * context's stream; names: "syrk", "lbuild", "raytrace", "cholesky", "total". */
+ * Average device time (ms) per launch of the named stage over the last qlx_eval, from HIP events
+ * on the context's stream; one launch covers one wave (every item of the wave); names:
+ * "raytrace", "lbuild", "syrk", "cholesky", "solve", "total". */
 int qlx_kernel_time(void *ctx, const char *name, double *ms, int *launches);
 
-/* Work counters of the last evaluated item (no reference counterpart; benchmark accounting).
- *   syrk_dense_flops   2*64^3 per (lower tile, 64-row chunk) pair of the dense SYRK
- *   syrk_exec_flops    the same over the pairs actually executed (empty tiles skipped)
+/* Work counters of the last wave of the last qlx_eval (no reference counterpart; benchmark
+ * accounting).  Sums over the wave's items unless noted:
+ *   syrk_dense_flops   2*64^3 per (lower tile, 64-row chunk) pair of a dense SYRK
+ *   syrk_exec_flops    MFMA flops executed: the pairs whose chunk holds both blocks (x3 in
+ *                      MIXED mode: hi*hi + hi*lo + lo*hi products)
  *   syrk_alg_flops     N_img*N_src*(N_src+1) + 2*N_img*N_src (the F and D arithmetic)
- *   lb_bytes           bytes of the blurred lensing matrix written (rows x padded cols x elem)
+ *   lb_bytes           bytes of the blurred lensing matrix blocks written
  *   integrand_evals    NFW integrand evaluations of the ray shooting
- *   chol_flops         2 * N_src^3 / 3 (two factorizations: F and R) */
+ *   chol_flops         N_src^3 / 3 (F's factorization; R's banded one is not counted)
+ *   n_items            items of the wave
+ *   n_img, n_src, ld_src, rows_pad: sizes of the wave's last item */
 typedef struct {
     double syrk_dense_flops, syrk_exec_flops, syrk_alg_flops;
     double lb_bytes, integrand_evals, chol_flops;
+    int n_items;
     int n_img, n_src, ld_src, rows_pad;
 } qlx_stats;
 int qlx_get_stats(void *ctx, qlx_stats *out);
 
-/* Diagnostic (no reference counterpart): one v_mfma_f64_16x16x4f64 on fixed operands
- * A[i][k] = i + 1 + 0.01k, B[k][j] = 1 + k + 0.1j; out[lane*4 + r] = result register r of lane.
- * Lets a test confirm the accumulator layout the FP64 tile kernels assume
- * (register r of lane l = D[l/16 + 4r][l%16] on gfx950). */
-int qlx_selftest_mfma(void *ctx, double *out256);
+/* Diagnostic (no reference counterpart): one MFMA on fixed operands, out[lane*4 + r] = result
+ * register r of lane.  kind 0: v_mfma_f64_16x16x4f64 with A[i][k] = i + 1 + 0.01k,
+ * B[k][j] = 1 + k + 0.1j (register r of lane l = D[l/16 + 4r][l%16] on gfx950); kind 1: the same
+ * operands on v_mfma_f32_16x16x4f32 (register r of lane l = D[4(l/16) + r][l%16]); kind 2:
+ * v_mfma_f32_16x16x32_bf16 with A[i][k] = (i+1)(k%4+1)/4, B[k][j] = (j+1)((k/4)%8+1)/8 where lane
+ * l holds row/column l%16 and k = 8(l/16) .. 8(l/16)+7 (output layout as kind 1).  Lets a test
+ * confirm the operand and accumulator layouts the tile kernels assume. */
+int qlx_selftest_mfma(void *ctx, int kind, double *out256);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,1490 @@
+// Batched inversion kernels for gfx950: active sets, merged lensing rows, row ordering, block-sparse
+// blurred L, F = Lb^T C Lb (FP64 MFMA, or split-BF16 MFMA with FP32 accumulation), curvature R
+// (banded FP64 factorization for ln det R), blocked dense Cholesky of F (FP64 or FP32), solves,
+// FP64 iterative refinement and the evidence terms.
+//
+// Item b of a batch is blockIdx.y (matrix index for the Cholesky kernels); per-item sizes and pool
+// offsets come from BatchDev::meta.  Layouts: Lb blocks are 64x64 row-major at
+// (chunk*ncb + colblock)*4096 of the item's pool; column n_src carries the data vector so the SYRK
+// also yields D = Lb^T C d (row n_src of F).  F and R are row-major ldN x ldN (lower triangle used),
+// padding beyond the source block is identity.
+//
+// Reference behaviour followed:
+//   active sets              assign_active_indices / assign_pixel_mappings  pixelgrid.cpp:3586-3626, 12879-12914
+//   dense L (merged in CSR)  assign_Lmatrix + convert_Lmatrix_to_dense      pixelgrid.cpp:13196-13321, 13717-13747
+//   PSF on L                 PSF_convolution_Lmatrix_dense (direct)         pixelgrid.cpp:14092-14176
+//   D, F                     create_lensing_matrices_from_Lmatrix_dense     pixelgrid.cpp:15590-15711
+//   curvature R              generate_hmatrices / Rmatrix_from_hmatrices    pixelgrid.cpp:2968-3240
+//   F += lambda R            add_regularization_term_to_dense_Fmatrix      pixelgrid.cpp:15808-15818
+//   ln det R                 Rmatrix_determinant_dense                      pixelgrid.cpp:18176-18191
+//   Cholesky, solve, logdet  Cholesky_dcmp/solve/logdet_lower_packed        pixelgrid.cpp:17071-17246
+//   model, chi^2, Es         lens.cpp:12950-13015, pixelgrid.cpp:15821-15863, 18268-18307
+#include "qlx_internal.h"
+
+namespace qlx {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+#define TS 64
+#define LDSP 66
+
+// raise the dynamic-LDS limit of a kernel once per process (kernels are told apart by address)
+static void allow_big_lds(const void* kernel)
+{
+    static const void* done[32];
+    static int ndone = 0;
+    for (int i = 0; i < ndone; i++)
+        if (done[i] == kernel) return;
+    hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 16 * 1024);
+    if (ndone < 32) done[ndone++] = kernel;
+}
+
+static inline unsigned grid_for(size_t n, unsigned bs = 256, unsigned cap = 65535u * 4u)
+{
+    size_t g = (n + bs - 1) / bs;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+// ------------------------------------------------------------------------------------------------
+// utilities
+// ------------------------------------------------------------------------------------------------
+__global__ void k_fill_i32(int* p, int v, size_t n)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = v;
+}
+__global__ void k_fill_f64(double* p, double v, size_t n)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = v;
+}
+void launch_fill_i32(hipStream_t s, int* p, int v, size_t n)
+{
+    if (n) hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n)), dim3(256), 0, s, p, v, n);
+}
+void launch_fill_f64(hipStream_t s, double* p, double v, size_t n)
+{
+    if (n) hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(n)), dim3(256), 0, s, p, v, n);
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// 1024-thread exclusive scan of one flag per thread; returns this thread's exclusive index
+__device__ __forceinline__ int block_excl_scan(int flag, int* sh, int& total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int incl = wave_incl_scan(flag);
+    if (lane == 63) sh[w] = incl;
+    __syncthreads();
+    if (w == 0) {
+        int v = (lane < 16) ? sh[lane] : 0;
+        int s = wave_incl_scan(v);
+        if (lane < 16) sh[16 + lane] = s - v;
+        if (lane == 15) sh[32] = s;
+    }
+    __syncthreads();
+    int excl = sh[16 + w] + incl - flag;
+    total = sh[32];
+    __syncthreads();
+    return excl;
+}
+
+// ------------------------------------------------------------------------------------------------
+// active sets: source cells in j-major order, image pixels in j-outer / i-inner order
+// ------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_compact_b(BatchDev bd)
+{
+    __shared__ int sh[40];
+    const int item = blockIdx.y;
+    const int snx = bd.src_n[2 * item], sny = bd.src_n[2 * item + 1];
+    const int* cm = bd.cmapped + item * bd.s_cell;
+    int* sidx = bd.sidx + item * bd.s_cell;
+    const int* pm = bd.pmapped + item * bd.s_pix;
+    int* pidx = bd.pidx + item * bd.s_pix;
+    int* act = bd.act_pix + item * bd.s_pix;
+    int base = 0;
+    const int ncell = snx * sny;
+    for (int t0 = 0; t0 < ncell; t0 += 1024) {
+        int t = t0 + threadIdx.x, c = -1, flag = 0;
+        if (t < ncell) {
+            c = (t % snx) * sny + t / snx;
+            flag = cm[c] ? 1 : 0;
+        }
+        int tot, ex = block_excl_scan(flag, sh, tot);
+        if (c >= 0) sidx[c] = flag ? base + ex : -1;
+        base += tot;
+    }
+    if (threadIdx.x == 0) bd.counts[4 * item] = base;
+    base = 0;
+    const int npix = bd.nx * bd.ny;
+    for (int t0 = 0; t0 < npix; t0 += 1024) {
+        int t = t0 + threadIdx.x, p = -1, flag = 0;
+        if (t < npix) {
+            p = (t % bd.nx) * bd.ny + t / bd.nx;
+            flag = ((!bd.mask || bd.mask[p]) && pm[p]) ? 1 : 0;
+        }
+        int tot, ex = block_excl_scan(flag, sh, tot);
+        if (p >= 0) {
+            pidx[p] = flag ? base + ex : -1;
+            if (flag) act[base + ex] = p;
+        }
+        base += tot;
+    }
+    if (threadIdx.x == 0) bd.counts[4 * item + 1] = base;
+}
+
+void launch_compact_b(hipStream_t s, const BatchDev& bd)
+{
+    hipLaunchKernelGGL(k_compact_b, dim3(1, bd.B), dim3(1024), 0, s, bd);
+}
+
+// merged row of L per active pixel: CSR order, duplicates summed from 0 in that order
+__global__ void __launch_bounds__(256) k_merge_lrows_b(BatchDev bd)
+{
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!M.valid || r >= M.n_img) return;
+    const int nsub = bd.nsub, U = 3 * nsub;
+    const long long p = bd.act_pix[item * bd.s_pix + r];
+    const int* mcb = bd.map_cell + (size_t)item * bd.s_pix * nsub * 3;
+    const double* mwb = bd.map_w + (size_t)item * bd.s_pix * nsub * 3;
+    const int* sidx = bd.sidx + item * bd.s_cell;
+    int* uc = bd.ucol + ((size_t)item * bd.s_pix + r) * U;
+    double* uv = bd.uval + ((size_t)item * bd.s_pix + r) * U;
+    int cnt = 0;
+    for (int k = 0; k < nsub; k++) {
+        const int* mc = mcb + (p * nsub + k) * 3;
+        const double* mw = mwb + (p * nsub + k) * 3;
+        if (mc[0] < 0) continue;
+        for (int e = 0; e < 3; e++) {
+            int col = sidx[mc[e]];
+            int q = 0;
+            while (q < cnt && uc[q] != col) q++;
+            if (q == cnt) {
+                uc[cnt] = col;
+                uv[cnt] = 0.0 + mw[e];
+                cnt++;
+            } else {
+                uv[q] += mw[e];
+            }
+        }
+    }
+    bd.ucnt[item * bd.s_pix + r] = cnt;
+}
+
+void launch_merge_lrows_b(hipStream_t s, const BatchDev& bd, int max_img)
+{
+    if (max_img > 0) hipLaunchKernelGGL(k_merge_lrows_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, s, bd);
+}
+
+// ------------------------------------------------------------------------------------------------
+// row ordering by the source cell of each pixel's first ray (stable: ties keep pixel order)
+// ------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_row_keys_b(BatchDev bd)
+{
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!M.valid || r >= M.n_img) return;
+    const int nsub = bd.nsub;
+    const long long p = bd.act_pix[item * bd.s_pix + r];
+    const int* mcb = bd.map_cell + (size_t)item * bd.s_pix * nsub * 3;
+    const int* sidx = bd.sidx + item * bd.s_cell;
+    int key = 0;
+    for (int k = 0; k < nsub; k++) {
+        int c0 = mcb[(p * nsub + k) * 3];
+        if (c0 >= 0) {
+            key = sidx[c0];
+            break;
+        }
+    }
+    bd.key[item * bd.s_pix + r] = key;
+    atomicAdd(&bd.bcnt[item * (bd.s_cell + 2) + key], 1);
+}
+
+__global__ void __launch_bounds__(1024) k_bucket_scan_b(BatchDev bd)
+{
+    __shared__ int sh[40];
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    if (!M.valid) return;
+    const int n = M.n_src;
+    const int* cnt = bd.bcnt + item * (bd.s_cell + 2);
+    int* off = bd.boff + item * (bd.s_cell + 2);
+    int* fill = bd.bfill + item * (bd.s_cell + 2);
+    int base = 0;
+    for (int t0 = 0; t0 < n; t0 += 1024) {
+        int t = t0 + threadIdx.x;
+        int v = (t < n) ? cnt[t] : 0;
+        // exclusive scan of counts (values may exceed 1): wave scan + wave offsets
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        int incl = wave_incl_scan(v);
+        if (lane == 63) sh[w] = incl;
+        __syncthreads();
+        if (w == 0) {
+            int x = (lane < 16) ? sh[lane] : 0;
+            int sx = wave_incl_scan(x);
+            if (lane < 16) sh[16 + lane] = sx - x;
+            if (lane == 15) sh[32] = sx;
+        }
+        __syncthreads();
+        if (t < n) {
+            off[t] = base + sh[16 + w] + incl - v;
+            fill[t] = 0;
+        }
+        int tot = sh[32];
+        __syncthreads();
+        base += tot;
+    }
+    if (threadIdx.x == 0) off[n] = base;
+}
+
+__global__ void __launch_bounds__(256) k_row_scatter_b(BatchDev bd)
+{
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!M.valid || r >= M.n_img) return;
+    const int key = bd.key[item * bd.s_pix + r];
+    const int pos = bd.boff[item * (bd.s_cell + 2) + key] + atomicAdd(&bd.bfill[item * (bd.s_cell + 2) + key], 1);
+    bd.perm[item * bd.s_pix + pos] = r;
+}
+
+// ascending original order inside each bucket: each bucket's slice is ranked by a workgroup
+__global__ void __launch_bounds__(256) k_bucket_rank_b(BatchDev bd)
+{
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    if (!M.valid) return;
+    const int* off = bd.boff + item * (bd.s_cell + 2);
+    int* perm = bd.perm + item * bd.s_pix;
+    int* key = bd.key + item * bd.s_pix;  // reused as scratch for the ranked slice
+    for (int k = blockIdx.x; k < M.n_src; k += gridDim.x) {
+        const int lo = off[k], hi = off[k + 1], n = hi - lo;
+        if (n <= 1) continue;
+        // rank = number of smaller row ids in the slice (ids are distinct); write to key[lo+rank]
+        for (int i = threadIdx.x; i < n; i += 256) {
+            const int v = perm[lo + i];
+            int rank = 0;
+            for (int j = 0; j < n; j++) rank += (perm[lo + j] < v);
+            key[lo + rank] = v;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += 256) perm[lo + i] = key[lo + i];
+        __syncthreads();
+    }
+}
+
+void launch_sort_rows_b(hipStream_t s, const BatchDev& bd, int max_img, int max_src)
+{
+    if (max_img <= 0) return;
+    launch_fill_i32(s, bd.bcnt, 0, (size_t)bd.B * (bd.s_cell + 2));
+    hipLaunchKernelGGL(k_row_keys_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, s, bd);
+    hipLaunchKernelGGL(k_bucket_scan_b, dim3(1, bd.B), dim3(1024), 0, s, bd);
+    hipLaunchKernelGGL(k_row_scatter_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, s, bd);
+    hipLaunchKernelGGL(k_bucket_rank_b, dim3(std::min(max_src, 256), bd.B), dim3(256), 0, s, bd);
+}
+
+// ------------------------------------------------------------------------------------------------
+// chunk support: union of the 64-column blocks touched by the PSF taps of a chunk's 64 rows
+// (one workgroup per chunk), then per-item slot bases of the compact block pool.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void mask_set(unsigned (&m)[QLX_MAX_CBW], int cb)
+{
+#pragma unroll
+    for (int w = 0; w < QLX_MAX_CBW; w++)
+        if (w == (cb >> 5)) m[w] |= 1u << (cb & 31);
+}
+
+__global__ void __launch_bounds__(256) k_row_support_b(BatchDev bd)
+{
+    __shared__ unsigned msk[QLX_MAX_CBW];
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    const int kt = blockIdx.x;
+    if (!M.valid || kt >= M.nkt) return;
+    const int tid = threadIdx.x;
+    if (tid < QLX_MAX_CBW) msk[tid] = 0;
+    __syncthreads();
+    unsigned m[QLX_MAX_CBW];
+#pragma unroll
+    for (int w = 0; w < QLX_MAX_CBW; w++) m[w] = 0;
+    if (tid == 0) mask_set(m, M.n_src >> 6);  // the data column
+    const int ntap = bd.pnx * bd.pny;
+    const int nxh = bd.pnx / 2, nyh = bd.pny / 2;
+    const int U = 3 * bd.nsub;
+    const int* pidx = bd.pidx + item * bd.s_pix;
+    const int* ucnt = bd.ucnt + item * bd.s_pix;
+    const int* ucol = bd.ucol + (size_t)item * bd.s_pix * U;
+    const int* perm = bd.perm + item * bd.s_pix;
+    const int* act = bd.act_pix + item * bd.s_pix;
+    const int rows = min(TS, M.n_img - kt * TS);
+    for (int q = tid; q < rows * ntap; q += 256) {
+        const int rr = q / ntap, t = q - rr * ntap;
+        const int p = act[perm[kt * TS + rr]];
+        const int k = p / bd.ny, l = p % bd.ny;
+        const int pk = t / bd.pny, pl = t - pk * bd.pny;
+        const int ii = k + nxh - pk, jj = l + nyh - pl;
+        if (ii < 0 || ii >= bd.nx || jj < 0 || jj >= bd.ny) continue;
+        const int r2 = pidx[ii * bd.ny + jj];
+        if (r2 < 0) continue;
+        const int cnt = ucnt[r2];
+        for (int e = 0; e < cnt; e++) {
+            const int cb = ucol[(size_t)r2 * U + e] >> 6;
+            const unsigned bit = 1u << (cb & 31);
+#pragma unroll
+            for (int w = 0; w < QLX_MAX_CBW; w++) m[w] |= (w == (cb >> 5)) ? bit : 0u;
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < QLX_MAX_CBW; w++)
+        if (m[w]) atomicOr(&msk[w], m[w]);
+    __syncthreads();
+    if (tid < M.ncbw) bd.nz[M.nz_off + (size_t)kt * M.ncbw + tid] = msk[tid];
+}
+
+__global__ void __launch_bounds__(1024) k_block_scan_b(BatchDev bd)
+{
+    __shared__ int sh[40];
+    __shared__ int smax[1024];
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    if (!M.valid) return;
+    const unsigned* nz = bd.nz + M.nz_off;
+    int* bb = bd.bbase + M.bb_off;
+    int base = 0, best = 0;
+    for (int t0 = 0; t0 < M.nkt; t0 += 1024) {
+        const int t = t0 + threadIdx.x;
+        int v = 0;
+        if (t < M.nkt)
+            for (int w = 0; w < M.ncbw; w++) v += __popc(nz[(size_t)t * M.ncbw + w]);
+        best = max(best, v);
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        int incl = wave_incl_scan(v);
+        if (lane == 63) sh[wv] = incl;
+        __syncthreads();
+        if (wv == 0) {
+            int x = (lane < 16) ? sh[lane] : 0;
+            int sx = wave_incl_scan(x);
+            if (lane < 16) sh[16 + lane] = sx - x;
+            if (lane == 15) sh[32] = sx;
+        }
+        __syncthreads();
+        if (t < M.nkt) bb[t] = base + sh[16 + wv] + incl - v;
+        const int tot = sh[32];
+        __syncthreads();
+        base += tot;
+    }
+    smax[threadIdx.x] = best;
+    __syncthreads();
+    for (int st = 512; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        bb[M.nkt] = base;
+        bd.counts[4 * item + 2] = base;
+        bd.counts[4 * item + 3] = smax[0];
+    }
+}
+
+void launch_row_support_b(hipStream_t s, const BatchDev& bd, int max_nkt)
+{
+    if (max_nkt <= 0) return;
+    hipLaunchKernelGGL(k_row_support_b, dim3(max_nkt, bd.B), dim3(256), 0, s, bd);
+    hipLaunchKernelGGL(k_block_scan_b, dim3(1, bd.B), dim3(1024), 0, s, bd);
+}
+
+// rank of block cb among the set bits of a chunk's support mask
+__device__ __forceinline__ int mask_rank(const unsigned* m, int cb)
+{
+    int s = 0;
+    for (int w = 0; w < (cb >> 5); w++) s += __popc(m[w]);
+    return s + __popc(m[cb >> 5] & ((1u << (cb & 31)) - 1u));
+}
+__device__ __forceinline__ bool mask_has(const unsigned* m, int cb) { return (m[cb >> 5] >> (cb & 31)) & 1u; }
+
+// ------------------------------------------------------------------------------------------------
+// blurred rows into the chunk's support blocks (one wave per row; the row's support columns are
+// accumulated in LDS in CSR-tap order, then each support block row is stored as 64 contiguous
+// values).  FP64: pool of doubles.  Mixed: rows scaled by sqrt(c_r) and split x = hi + lo into
+// two BF16 pools (hi = bf16(x), lo = bf16(x - hi)); the lo pool follows the hi pool.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)
+{
+    unsigned u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(((unsigned)h) << 16); }
+
+__global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int bf16, int nsup_cap)
+{
+    extern __shared__ double lds_rows[];
+    __shared__ short loc[QLX_MAX_LD / TS];
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    const int nw = blockDim.x >> 6;
+    const int r0 = blockIdx.x * nw;
+    if (!M.valid || r0 >= M.nrows) return;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int rs = r0 + w;
+    const int kt = r0 / TS, rr = rs % TS;
+    const unsigned* m = bd.nz + M.nz_off + (size_t)kt * M.ncbw;
+    for (int cb = tid; cb < M.ncb; cb += blockDim.x) loc[cb] = mask_has(m, cb) ? (short)mask_rank(m, cb) : (short)-1;
+    const int sbase = bd.bbase[M.bb_off + kt];
+    const int nsup = bd.bbase[M.bb_off + kt + 1] - sbase;
+    double* row = lds_rows + (size_t)w * nsup_cap * TS;
+    for (int c = lane; c < nsup * TS; c += 64) row[c] = 0.0;
+    __syncthreads();
+    const int U = 3 * bd.nsub;
+    if (rs < M.n_img) {
+        const int p = bd.act_pix[item * bd.s_pix + bd.perm[item * bd.s_pix + rs]];
+        const int k = p / bd.ny, l = p % bd.ny;
+        const int nxh = bd.pnx / 2, nyh = bd.pny / 2;
+        const int* pidx = bd.pidx + item * bd.s_pix;
+        const int* ucnt = bd.ucnt + item * bd.s_pix;
+        const int* ucol = bd.ucol + (size_t)item * bd.s_pix * U;
+        const double* uval = bd.uval + (size_t)item * bd.s_pix * U;
+        for (int pk = 0; pk < bd.pnx; pk++) {
+            const int ii = k + nxh - pk;
+            if (ii < 0 || ii >= bd.nx) continue;
+            for (int pl = 0; pl < bd.pny; pl++) {
+                const int jj = l + nyh - pl;
+                if (jj < 0 || jj >= bd.ny) continue;
+                const int r2 = pidx[ii * bd.ny + jj];
+                if (r2 < 0) continue;
+                const double pv = bd.psf[pk * bd.pny + pl];
+                if (lane < ucnt[r2]) {
+                    const int c = ucol[(size_t)r2 * U + lane];
+                    double* dst = row + loc[c >> 6] * TS + (c & 63);
+                    *dst += pv * uval[(size_t)r2 * U + lane];
+                }
+                __builtin_amdgcn_wave_barrier();  // taps are applied in order across lanes of the wave
+            }
+        }
+        if (lane == 0) row[loc[M.n_src >> 6] * TS + (M.n_src & 63)] = bd.data[p];
+        __builtin_amdgcn_wave_barrier();
+        if (bf16) {
+            const double sc = sqrt(bd.covinv[p]);
+            for (int c = lane; c < nsup * TS; c += 64) row[c] *= sc;
+        }
+    }
+    __syncthreads();
+    if (!bf16) {
+        double* pool = (double*)bd.lb + M.lb_off;
+        for (int q = 0; q < nsup; q++) pool[(size_t)(sbase + q) * TS * TS + (size_t)rr * TS + lane] = row[q * TS + lane];
+    } else {
+        uint16_t* hi = (uint16_t*)bd.lb + M.lb_off;
+        uint16_t* lo = hi + (size_t)M.nblk * TS * TS;
+        for (int q = 0; q < nsup; q++) {
+            const float x = (float)row[q * TS + lane];
+            const uint16_t h = f32_to_bf16_rne(x);
+            const uint16_t g = f32_to_bf16_rne(x - bf16_to_f32(h));
+            const size_t o = (size_t)(sbase + q) * TS * TS + (size_t)rr * TS + lane;
+            hi[o] = h;
+            lo[o] = g;
+        }
+    }
+}
+
+void launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_nsup, int bf16)
+{
+    if (max_rows <= 0 || max_nsup <= 0) return;
+    const size_t per_wave = (size_t)max_nsup * TS * sizeof(double);
+    int nw = 4;
+    while (nw > 1 && per_wave * nw > 144 * 1024) nw >>= 1;
+    allow_big_lds((const void*)k_build_lb_b);
+    hipLaunchKernelGGL(k_build_lb_b, dim3((max_rows + nw - 1) / nw, bd.B), dim3(64 * nw), per_wave * nw, s, bd, bf16, max_nsup);
+}
+
+__global__ void k_gather_cvec_b(BatchDev bd)
+{
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!M.valid || r >= M.nrows) return;
+    bd.cvec[M.cvec_off + r] =
+        (r < M.n_img) ? bd.covinv[bd.act_pix[item * bd.s_pix + bd.perm[item * bd.s_pix + r]]] : 0.0;
+}
+
+void launch_gather_cvec_b(hipStream_t s, const BatchDev& bd, int max_rows)
+{
+    if (max_rows > 0) hipLaunchKernelGGL(k_gather_cvec_b, dim3((max_rows + 255) / 256, bd.B), dim3(256), 0, s, bd);
+}
+
+__device__ __forceinline__ void tri_decode(int t, int& I, int& J)
+{
+    int i = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((i + 1) * (i + 2) / 2 <= t) i++;
+    while (i * (i + 1) / 2 > t) i--;
+    I = i;
+    J = t - i * (i + 1) / 2;
+}
+
+// ------------------------------------------------------------------------------------------------
+// F = Lb^T C Lb, FP64 MFMA (v_mfma_f64_16x16x4f64; accumulator register v of lane l holds
+// D[l/16 + 4v][l%16] on gfx950).  One workgroup per lower tile; only chunks holding both blocks.
+// ------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_syrk_f64_b(BatchDev bd)
+{
+    __shared__ double As[TS][LDSP];
+    __shared__ double Bs[TS][LDSP];
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    if (!M.valid) return;
+    const int ntiles = M.ncb * (M.ncb + 1) / 2;
+    if ((int)blockIdx.x >= ntiles) return;
+    int I, J;
+    tri_decode(blockIdx.x, I, J);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    const double* pool = (const double*)bd.lb + M.lb_off;
+    const double* cvec = bd.cvec + M.cvec_off;
+    const unsigned* nz = bd.nz + M.nz_off;
+    const int* bb = bd.bbase + M.bb_off;
+    double4_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) acc[a][c] = (double4_t){0, 0, 0, 0};
+    for (int kt = 0; kt < M.nkt; kt++) {
+        const unsigned* m = nz + (size_t)kt * M.ncbw;
+        if (!(mask_has(m, I) && mask_has(m, J))) continue;
+        const double* Ab = pool + (size_t)(bb[kt] + mask_rank(m, I)) * TS * TS;
+        const double* Bb = pool + (size_t)(bb[kt] + mask_rank(m, J)) * TS * TS;
+        const double* cv = cvec + (size_t)kt * TS;
+        __syncthreads();
+#pragma unroll 4
+        for (int e = tid; e < TS * TS; e += 256) {
+            const int kk = e >> 6, ii = e & 63;
+            As[kk][ii] = Ab[e];
+            Bs[kk][ii] = Bb[e] * cv[kk];
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < TS; kk += 4) {
+            const int kq = kk + (lane >> 4);
+            double a0 = As[kq][wr * 32 + (lane & 15)];
+            double a1 = As[kq][wr * 32 + 16 + (lane & 15)];
+            double b0 = Bs[kq][wc * 32 + (lane & 15)];
+            double b1 = Bs[kq][wc * 32 + 16 + (lane & 15)];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+    }
+    double* C = bd.F + M.f_off + (size_t)I * TS * M.ldN + (size_t)J * TS;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int rr = wr * 32 + a * 16 + (lane >> 4) + 4 * v;
+                const int cc = wc * 32 + c * 16 + (lane & 15);
+                C[(size_t)rr * M.ldN + cc] = acc[a][c][v];
+            }
+}
+
+void launch_syrk_f64_b(hipStream_t s, const BatchDev& bd, int max_ncb)
+{
+    const int nt = max_ncb * (max_ncb + 1) / 2;
+    hipLaunchKernelGGL(k_syrk_f64_b, dim3(nt, bd.B), dim3(256), 0, s, bd);
+}
+
+// ------------------------------------------------------------------------------------------------
+// F32 = Lb^T Lb with Lb = hi + lo (BF16 pools of sqrt(c)-scaled rows), on
+// v_mfma_f32_16x16x32_bf16: hi*hi + hi*lo + lo*hi accumulated in FP32.  A operand lane l holds
+// A[i = l%16][k = 8*(l/16) .. +8]; B the same with j; accumulator register v of lane l holds
+// D[4*(l/16) + v][l%16].  Each wave computes a 32x32 quadrant of the 64x64 tile.
+// ------------------------------------------------------------------------------------------------
+#define BKP (TS + 8)
+__global__ void __launch_bounds__(256) k_syrk_bf16_b(BatchDev bd)
+{
+    __shared__ uint16_t Ah[TS][BKP], Al[TS][BKP], Bh[TS][BKP], Bl[TS][BKP];  // [i][k]
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    if (!M.valid) return;
+    const int ntiles = M.ncb * (M.ncb + 1) / 2;
+    if ((int)blockIdx.x >= ntiles) return;
+    int I, J;
+    tri_decode(blockIdx.x, I, J);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    const uint16_t* hi = (const uint16_t*)bd.lb + M.lb_off;
+    const uint16_t* lo = hi + (size_t)M.nblk * TS * TS;
+    const unsigned* nz = bd.nz + M.nz_off;
+    const int* bb = bd.bbase + M.bb_off;
+    float4_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) acc[a][c] = (float4_t){0, 0, 0, 0};
+    for (int kt = 0; kt < M.nkt; kt++) {
+        const unsigned* m = nz + (size_t)kt * M.ncbw;
+        if (!(mask_has(m, I) && mask_has(m, J))) continue;
+        const size_t oA = (size_t)(bb[kt] + mask_rank(m, I)) * TS * TS;
+        const size_t oB = (size_t)(bb[kt] + mask_rank(m, J)) * TS * TS;
+        __syncthreads();
+        // blocks are [k][i] in memory; stored transposed [i][k] so each lane's 8 k-values are contiguous
+        for (int e = tid; e < TS * TS; e += 256) {
+            const int kk = e >> 6, ii = e & 63;
+            Ah[ii][kk] = hi[oA + e];
+            Al[ii][kk] = lo[oA + e];
+            Bh[ii][kk] = hi[oB + e];
+            Bl[ii][kk] = lo[oB + e];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k0 = 0; k0 < TS; k0 += 32) {
+            const int kq = k0 + 8 * (lane >> 4);
+            bf16x8_t ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const int ia = wr * 32 + t * 16 + (lane & 15);
+                const int jb = wc * 32 + t * 16 + (lane & 15);
+                ah[t] = *(const bf16x8_t*)&Ah[ia][kq];
+                al[t] = *(const bf16x8_t*)&Al[ia][kq];
+                bh[t] = *(const bf16x8_t*)&Bh[jb][kq];
+                bl[t] = *(const bf16x8_t*)&Bl[jb][kq];
+            }
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[c], acc[a][c], 0, 0, 0);
+                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[c], acc[a][c], 0, 0, 0);
+                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[c], acc[a][c], 0, 0, 0);
+                }
+        }
+    }
+    float* C = bd.F32 + M.f_off + (size_t)I * TS * M.ldN + (size_t)J * TS;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int rr = wr * 32 + a * 16 + 4 * (lane >> 4) + v;
+                const int cc = wc * 32 + c * 16 + (lane & 15);
+                C[(size_t)rr * M.ldN + cc] = acc[a][c][v];
+            }
+}
+
+void launch_syrk_bf16_b(hipStream_t s, const BatchDev& bd, int max_ncb)
+{
+    const int nt = max_ncb * (max_ncb + 1) / 2;
+    hipLaunchKernelGGL(k_syrk_bf16_b, dim3(nt, bd.B), dim3(256), 0, s, bd);
+}
+
+// ------------------------------------------------------------------------------------------------
+// curvature R (exact small integers) in banded tiles.  In the active (j-major) order an R entry
+// couples cells at most two source rows apart, |a - b| <= 2*snx, so the lower band in 64-blocks is
+// bwb = (2*snx)/64 + 1 and the Cholesky factor stays inside it.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ double* rtile(const BatchDev& bd, const ItemMeta& M, int r, int c)
+{
+    const int I = r >> 6, J = c >> 6;
+    return bd.R + M.r_off + ((size_t)I * (M.bwb + 1) + (I - J)) * TS * TS + (r & 63) * TS + (c & 63);
+}
+
+__global__ void __launch_bounds__(256) k_zero_r_b(BatchDev bd)
+{
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    if (!M.valid) return;
+    double* R = bd.R + M.r_off;
+    const size_t n = (size_t)(M.ldN / TS) * (M.bwb + 1) * TS * TS;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) R[t] = 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_curvature_b(BatchDev bd)
+{
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!M.valid) return;
+    if (c >= M.snx * M.sny) {
+        // identity on the padding rows (the data row of F's augmented matrix has no R entries)
+        const int r = c - M.snx * M.sny + M.n_src;
+        if (r < M.ldN && c - M.snx * M.sny < M.ldN - M.n_src) *rtile(bd, M, r, r) = 1.0;
+        return;
+    }
+    const int* sidx = bd.sidx + item * bd.s_cell;
+    const int a = sidx[c];
+    if (a < 0) return;
+    const int snx = M.snx, sny = M.sny;
+    const int i = c / sny, j = c % sny;
+    for (int l = 0; l < 2; l++) {
+        int ids[3];
+        double h[3];
+        int m = 0;
+        ids[m] = a;
+        h[m++] = -2.0;
+        int nb0, nb1;
+        if (l == 0) {
+            nb0 = (i < snx - 1) ? sidx[(i + 1) * sny + j] : -1;
+            nb1 = (i > 0) ? sidx[(i - 1) * sny + j] : -1;
+        } else {
+            nb0 = (j < sny - 1) ? sidx[i * sny + j + 1] : -1;
+            nb1 = (j > 0) ? sidx[i * sny + j - 1] : -1;
+        }
+        if (nb0 >= 0) { ids[m] = nb0; h[m++] = 1.0; }
+        if (nb1 >= 0) { ids[m] = nb1; h[m++] = 1.0; }
+        for (int x = 0; x < m; x++)
+            for (int y = x; y < m; y++) {
+                const int r0 = ids[x] > ids[y] ? ids[x] : ids[y];
+                const int c0 = ids[x] > ids[y] ? ids[y] : ids[x];
+                atomicAdd(rtile(bd, M, r0, c0), h[x] * h[y]);
+            }
+    }
+}
+
+void launch_curvature_b(hipStream_t s, const BatchDev& bd, int max_cells, int max_rtiles)
+{
+    hipLaunchKernelGGL(k_zero_r_b, dim3(grid_for((size_t)max_rtiles * TS * TS, 256, 2048), bd.B), dim3(256), 0, s, bd);
+    const int n = max_cells + QLX_MAX_LD;
+    hipLaunchKernelGGL(k_curvature_b, dim3((n + 255) / 256, bd.B), dim3(256), 0, s, bd);
+}
+
+// F_lower += lambda R over the n x n source block; row n (the data row D^T | d^T C d of the
+// augmented factorization) is kept, rows > n are identity.
+// Mixed (FP32 F): every regularized lower entry is also copied to the transposed upper position
+// (the residual of the iterative refinement reads F there after the lower triangle has been
+// factored), the diagonal goes to the Fdiag slot in FP64, and the data-row pivot is set to a
+// large value: the refined solve takes D from the upper copy and y from row n, never the pivot.
+template <typename T>
+__global__ void __launch_bounds__(256) k_prep_f_b(BatchDev bd)
+{
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    if (!M.valid) return;
+    const int ld = M.ldN, n = M.n_src;
+    const bool mixed = sizeof(T) == 4;
+    T* F = mixed ? (T*)(bd.F32 + M.f_off) : (T*)(bd.F + M.f_off);
+    double* fdiag = bd.vec + M.vec_off + 3 * (size_t)ld;
+    const double lam = bd.lambda;
+    const size_t total = (size_t)ld * ld;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(t / ld), c = (int)(t % ld);
+        if (c > r) continue;
+        T v;
+        if (r < n) {
+            const double rv = ((r >> 6) - (c >> 6) <= M.bwb) ? *rtile(bd, M, r, c) : 0.0;
+            v = (T)((double)F[t] + lam * rv);
+        } else if (r == n) {
+            v = (c == n && mixed) ? (T)1e30 : F[t];
+        } else {
+            v = (r == c) ? (T)1 : (T)0;
+        }
+        F[t] = v;
+        if (mixed) {
+            if (c < r && r <= n) F[(size_t)c * ld + r] = v;
+            if (c == r && r < n) fdiag[r] = (double)v;
+        }
+    }
+}
+
+void launch_prep_fr_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32)
+{
+    const unsigned g = grid_for((size_t)max_ld * max_ld, 256, 4096);
+    if (f32) hipLaunchKernelGGL(k_prep_f_b<float>, dim3(g, bd.B), dim3(256), 0, s, bd);
+    else hipLaunchKernelGGL(k_prep_f_b<double>, dim3(g, bd.B), dim3(256), 0, s, bd);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Batched blocked Cholesky (lower, 64x64 tiles) of many matrices, FP64 or FP32 per matrix, dense
+// or banded per matrix: per block step a diagonal kernel (factor + block inverse), a panel kernel
+// (A_ib,kb <- A_ib,kb W^T on MFMA) and a trailing-update kernel (A_IJ -= P_I P_J^T on MFMA).
+// Matrices with fewer blocks skip the step; the upper triangle is never written.
+// Restates Cholesky_dcmp_packed (pixelgrid.cpp:17071-17105) blockwise: a negative pivot of a row
+// < n sets *status and the factorization continues with sqrt(|a|) as the reference does.
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T* tile_ptr(const MatDesc& D, int I, int J, int& ld)
+{
+    if (D.bwb > 0) {
+        ld = TS;
+        return (T*)D.A + ((size_t)I * (D.bwb + 1) + (I - J)) * TS * TS;
+    }
+    ld = D.ld;
+    return (T*)D.A + (size_t)I * TS * D.ld + (size_t)J * TS;
+}
+
+__device__ __forceinline__ int band_last(const MatDesc& D, int kb)
+{
+    return D.bwb > 0 ? min(D.nb - 1, kb + D.bwb) : D.nb - 1;
+}
+
+#define CHOL_DIAG_LDS (2 * TS + 2 * TS * (TS + 1) + 4 * TS)
+template <typename T>
+__device__ void chol_diag_body(const MatDesc& D, int kb, double* raw, int* bad)
+{
+    T* colbuf = (T*)raw;                          // [2][64]
+    T(*Ls)[TS + 1] = (T(*)[TS + 1])(colbuf + 2 * TS);
+    T(*Xs)[TS + 1] = (T(*)[TS + 1])(colbuf + 2 * TS + TS * (TS + 1));
+    T(*part)[TS] = (T(*)[TS])(colbuf + 2 * TS + 2 * TS * (TS + 1));
+    const int tid = threadIdx.x;
+    const int r = tid >> 2, g = tid & 3;
+    const long long base = (long long)kb * TS;
+    int ld;
+    T* tile = tile_ptr<T>(D, kb, kb, ld);
+    T a[16];
+    const T* arow = tile + (size_t)r * ld + 16 * g;
+#pragma unroll
+    for (int j = 0; j < 16; j++) a[j] = (16 * g + j <= r) ? arow[j] : (T)0;
+    if (tid == 0) *bad = 0;
+    for (int c = 0; c < TS; c++) {
+        T* col = colbuf + (c & 1) * TS;  // step c+1 writes the other buffer while step c may still be read
+        const int gc = c >> 4, jc = c & 15;
+        T v = 0;
+#pragma unroll
+        for (int j = 0; j < 16; j++) v = (j == jc) ? a[j] : v;
+        if (r == c && g == gc) {
+            if (v < 0 && base + c < D.n) *bad = 1;
+            col[c] = sqrt(fabs(v));
+        }
+        __syncthreads();
+        const T d = col[c];
+        const T lrc = (r > c) ? v / d : d;
+        if (g == gc && r >= c) {
+#pragma unroll
+            for (int j = 0; j < 16; j++) a[j] = (j == jc) ? lrc : a[j];
+        }
+        if (g == gc && r > c) col[r] = lrc;
+        __syncthreads();
+        if (r > c) {
+            const T l = col[r];
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const int q = 16 * g + j;
+                if (q > c && q <= r) a[j] = fma(-l, col[q], a[j]);
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0 && *bad) atomicOr(D.status, 1);
+    T* drow = tile + (size_t)r * ld + 16 * g;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int q = 16 * g + j;
+        const T v = (q <= r) ? a[j] : (T)0;
+        if (q <= r) drow[j] = v;
+        Ls[r][q] = v;
+    }
+    if (g == (r >> 4)) ((T*)D.diag)[base + r] = a[r & 15];
+    __syncthreads();
+    // X = L^-1 by rows: X[i][c] = (delta_ic - sum_{q<i} L[i][q] X[q][c]) / L[i][i]; thread (c, g)
+    // sums q = g, g+4, ... and the four partial sums are combined in LDS.
+    const int cidx = tid >> 2;
+    for (int i = 0; i < TS; i++) {
+        T acc = 0;
+        for (int q = g; q < i; q += 4) acc = fma(Ls[i][q], Xs[q][cidx], acc);
+        part[g][cidx] = acc;
+        __syncthreads();
+        if (g == 0) {
+            T s = (i == cidx ? (T)1 : (T)0) - (part[0][cidx] + part[1][cidx] + part[2][cidx] + part[3][cidx]);
+            Xs[i][cidx] = (cidx <= i) ? s / Ls[i][i] : (T)0;
+        }
+        __syncthreads();
+    }
+    T* W = (T*)D.W + (size_t)kb * TS * TS;
+    for (int e = tid; e < TS * TS; e += 256) W[e] = Xs[e >> 6][e & 63];
+}
+
+__global__ void __launch_bounds__(256) k_chol_diag_m(const MatDesc* mats, int kb)
+{
+    __shared__ double raw[CHOL_DIAG_LDS];
+    __shared__ int bad;
+    const MatDesc D = mats[blockIdx.y];
+    if (kb >= D.nb) return;
+    if (D.f32) chol_diag_body<float>(D, kb, raw, &bad);
+    else chol_diag_body<double>(D, kb, raw, &bad);
+}
+
+// 64x64x64 MFMA product C = sum_q A(i, q) B(j, q) (both operands "row holds the k-vector")
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+    typedef double4_t acc_t;
+    __device__ static acc_t op(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <>
+struct Mfma16<float> {
+    typedef float4_t acc_t;
+    __device__ static acc_t op(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+#define MM_LDS (2 * TS * LDSP)
+template <typename T>
+__device__ __forceinline__ void mm64(const T* Ap, int sa, const T* Bp, int sb, double* raw, typename Mfma16<T>::acc_t acc[2][2])
+{
+    T(*As)[LDSP] = (T(*)[LDSP])raw;
+    T(*Bs)[LDSP] = (T(*)[LDSP])((T*)raw + TS * LDSP);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    for (int e = tid; e < TS * TS; e += 256) {
+        const int ii = e >> 6, kk = e & 63;
+        As[kk][ii] = Ap[(size_t)ii * sa + kk];
+        Bs[kk][ii] = Bp[(size_t)ii * sb + kk];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) acc[a][c] = (typename Mfma16<T>::acc_t){0, 0, 0, 0};
+#pragma unroll 4
+    for (int kk = 0; kk < TS; kk += 4) {
+        const int kq = kk + (lane >> 4);
+        T a0 = As[kq][wr * 32 + (lane & 15)];
+        T a1 = As[kq][wr * 32 + 16 + (lane & 15)];
+        T b0 = Bs[kq][wc * 32 + (lane & 15)];
+        T b1 = Bs[kq][wc * 32 + 16 + (lane & 15)];
+        acc[0][0] = Mfma16<T>::op(a0, b0, acc[0][0]);
+        acc[0][1] = Mfma16<T>::op(a0, b1, acc[0][1]);
+        acc[1][0] = Mfma16<T>::op(a1, b0, acc[1][0]);
+        acc[1][1] = Mfma16<T>::op(a1, b1, acc[1][1]);
+    }
+}
+
+// mode 0: store, 1: subtract, 2: subtract on the lower triangle only (diagonal tile)
+template <typename T>
+__device__ __forceinline__ void store64(T* C, int ldc, const typename Mfma16<T>::acc_t acc[2][2], int mode)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = w >> 1, wc = w & 1;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int rr = wr * 32 + a * 16 + Mfma16<T>::row(lane, v);
+                const int cc = wc * 32 + c * 16 + (lane & 15);
+                T* d = C + (size_t)rr * ldc + cc;
+                if (mode == 0) *d = acc[a][c][v];
+                else if (mode == 1 || cc <= rr) *d -= acc[a][c][v];
+            }
+}
+
+template <typename T>
+__device__ void chol_panel_body(const MatDesc& D, int kb, int ib, double* raw)
+{
+    int ld, ldw = TS;
+    T* blk = tile_ptr<T>(D, ib, kb, ld);
+    const T* W = (const T*)D.W + (size_t)kb * TS * TS;
+    typename Mfma16<T>::acc_t acc[2][2];
+    mm64<T>(blk, ld, W, ldw, raw, acc);  // C(i,j) = sum_q A_ib[i][q] W[j][q]
+    __syncthreads();
+    store64<T>(blk, ld, acc, 0);
+}
+
+__global__ void __launch_bounds__(256) k_chol_panel_m(const MatDesc* mats, int kb)
+{
+    __shared__ double raw[MM_LDS];
+    const MatDesc D = mats[blockIdx.y];
+    if (kb >= D.nb) return;
+    const int ib = kb + 1 + blockIdx.x;
+    if (ib > band_last(D, kb)) return;
+    if (D.f32) chol_panel_body<float>(D, kb, ib, raw);
+    else chol_panel_body<double>(D, kb, ib, raw);
+}
+
+template <typename T>
+__device__ void chol_update_body(const MatDesc& D, int kb, int I, int J, double* raw)
+{
+    int lda, ldb, ldc;
+    const T* Pa = tile_ptr<T>(D, I, kb, lda);
+    const T* Pb = tile_ptr<T>(D, J, kb, ldb);
+    T* C = tile_ptr<T>(D, I, J, ldc);
+    typename Mfma16<T>::acc_t acc[2][2];
+    mm64<T>(Pa, lda, Pb, ldb, raw, acc);
+    store64<T>(C, ldc, acc, I == J ? 2 : 1);
+}
+
+__global__ void __launch_bounds__(256) k_chol_update_m(const MatDesc* mats, int kb)
+{
+    __shared__ double raw[MM_LDS];
+    const MatDesc D = mats[blockIdx.y];
+    if (kb >= D.nb) return;
+    const int rest = band_last(D, kb) - kb;
+    if ((int)blockIdx.x >= rest * (rest + 1) / 2) return;
+    int I, J;
+    tri_decode(blockIdx.x, I, J);
+    I += kb + 1;
+    J += kb + 1;
+    if (D.f32) chol_update_body<float>(D, kb, I, J, raw);
+    else chol_update_body<double>(D, kb, I, J, raw);
+}
+
+void cholesky_batched(hipStream_t s, const MatDesc* d_mats, int nmat, int max_nb)
+{
+    for (int kb = 0; kb < max_nb; kb++) {
+        const int rest = max_nb - kb - 1;
+        hipLaunchKernelGGL(k_chol_diag_m, dim3(1, nmat), dim3(256), 0, s, d_mats, kb);
+        if (rest > 0) {
+            hipLaunchKernelGGL(k_chol_panel_m, dim3(rest, nmat), dim3(256), 0, s, d_mats, kb);
+            hipLaunchKernelGGL(k_chol_update_m, dim3(rest * (rest + 1) / 2, nmat), dim3(256), 0, s, d_mats, kb);
+        }
+    }
+}
+
+// 2 sum log(diag) in a fixed order (Cholesky_logdet_lower_packed, pixelgrid.cpp:17201-17210)
+template <typename T>
+__device__ double block_sum_log(const T* diag, int n, double* sh)
+{
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += log(fabs((double)diag[i]));
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    double r = 2.0 * sh[0];
+    __syncthreads();
+    return r;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_logdets_b(BatchDev bd)
+{
+    __shared__ double sh[256];
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    if (!M.valid) return;
+    const double* v = bd.vec + M.vec_off;
+    const T* diagF = (const T*)(v + M.ldN);
+    const double* diagR = v + 2 * (size_t)M.ldN;
+    const double lf = block_sum_log<T>(diagF, M.n_src, sh);
+    const double lr = block_sum_log<double>(diagR, M.n_src, sh);
+    if (threadIdx.x == 0) {
+        bd.out[QLX_OUT_STRIDE * item + 2] = lf;
+        bd.out[QLX_OUT_STRIDE * item + 3] = lr;
+    }
+}
+
+void launch_logdets_b(hipStream_t s, const BatchDev& bd, int f32)
+{
+    if (f32) hipLaunchKernelGGL(k_logdets_b<float>, dim3(1, bd.B), dim3(256), 0, s, bd);
+    else hipLaunchKernelGGL(k_logdets_b<double>, dim3(1, bd.B), dim3(256), 0, s, bd);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Triangular solves with the stored block inverses (one workgroup per item).
+// y = L^-1 D is row n of the augmented factor, so the first solve is backward only: x = L^-T y.
+// ------------------------------------------------------------------------------------------------
+template <typename T>
+__device__ void tri_fwd(const T* L, int ld, int nb, const T* W, T* sx, T (*part)[TS + 1], T* rhs, T (*Wl)[TS + 1])
+{
+    const int tid = threadIdx.x;
+    for (int kb = 0; kb < nb; kb++) {
+        {
+            const int r = tid >> 4, t16 = tid & 15;
+            const T* Lr = L + (long long)(kb * TS + r) * ld;
+            T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+            int c = t16;
+            for (; c + 48 < kb * TS; c += 64) {
+                a0 = fma(Lr[c], sx[c], a0);
+                a1 = fma(Lr[c + 16], sx[c + 16], a1);
+                a2 = fma(Lr[c + 32], sx[c + 32], a2);
+                a3 = fma(Lr[c + 48], sx[c + 48], a3);
+            }
+            for (; c < kb * TS; c += 16) a0 = fma(Lr[c], sx[c], a0);
+            part[t16][r] = (a0 + a1) + (a2 + a3);
+        }
+        const T* Wk = W + (size_t)kb * TS * TS;
+        for (int e = tid; e < TS * TS; e += 1024) Wl[e >> 6][e & 63] = Wk[e];
+        __syncthreads();
+        if (tid < TS) {
+            T s = sx[kb * TS + tid];
+            for (int q = 0; q < 16; q++) s -= part[q][tid];
+            rhs[tid] = s;
+        }
+        __syncthreads();
+        if (tid < TS) {
+            T y = 0;
+            for (int q = 0; q <= tid; q++) y = fma(Wl[tid][q], rhs[q], y);
+            sx[kb * TS + tid] = y;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__device__ void tri_bwd(const T* L, int ld, int nb, const T* W, T* sx, T (*part)[TS + 1], T* rhs, T (*Wl)[TS + 1])
+{
+    const int tid = threadIdx.x;
+    for (int kb = nb - 1; kb >= 0; kb--) {
+        {
+            const int r = tid & 63, t16 = tid >> 6;
+            const int col = kb * TS + r;
+            T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+            int rr = (kb + 1) * TS + t16;
+            for (; rr + 48 < nb * TS; rr += 64) {
+                a0 = fma(L[(long long)rr * ld + col], sx[rr], a0);
+                a1 = fma(L[(long long)(rr + 16) * ld + col], sx[rr + 16], a1);
+                a2 = fma(L[(long long)(rr + 32) * ld + col], sx[rr + 32], a2);
+                a3 = fma(L[(long long)(rr + 48) * ld + col], sx[rr + 48], a3);
+            }
+            for (; rr < nb * TS; rr += 16) a0 = fma(L[(long long)rr * ld + col], sx[rr], a0);
+            part[t16][r] = (a0 + a1) + (a2 + a3);
+        }
+        const T* Wk = W + (size_t)kb * TS * TS;
+        for (int e = tid; e < TS * TS; e += 1024) Wl[e >> 6][e & 63] = Wk[e];
+        __syncthreads();
+        if (tid < TS) {
+            T s = sx[kb * TS + tid];
+            for (int q = 0; q < 16; q++) s -= part[q][tid];
+            rhs[tid] = s;
+        }
+        __syncthreads();
+        if (tid < TS) {
+            T v = 0;
+            for (int q = tid; q < TS; q++) v = fma(Wl[q][tid], rhs[q], v);
+            sx[kb * TS + tid] = v;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_trisolve_b(BatchDev bd)
+{
+    extern __shared__ double sx_raw[];
+    __shared__ T part[16][TS + 1];
+    __shared__ T rhs[TS];
+    __shared__ T Wl[TS][TS + 1];
+    T* sx = (T*)sx_raw;
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    if (!M.valid) return;
+    const int ld = M.ldN, nb = ld / TS, n = M.n_src;
+    const T* L = (sizeof(T) == 8) ? (const T*)(bd.F + M.f_off) : (const T*)(bd.F32 + M.f_off);
+    const T* W = (const T*)(bd.winv + M.w_off);
+    for (int c = threadIdx.x; c < ld; c += 1024) sx[c] = (c < n) ? L[(size_t)n * ld + c] : (T)0;
+    __syncthreads();
+    tri_bwd<T>(L, ld, nb, W, sx, part, rhs, Wl);
+    double* s = bd.vec + M.vec_off;
+    for (int c = threadIdx.x; c < ld; c += 1024) s[c] = (c < n) ? (double)sx[c] : 0.0;
+}
+
+void launch_trisolve_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32)
+{
+    if (f32) {
+        allow_big_lds((const void*)k_trisolve_b<float>);
+        hipLaunchKernelGGL(k_trisolve_b<float>, dim3(1, bd.B), dim3(1024), (size_t)max_ld * 4, s, bd);
+    } else {
+        allow_big_lds((const void*)k_trisolve_b<double>);
+        hipLaunchKernelGGL(k_trisolve_b<double>, dim3(1, bd.B), dim3(1024), (size_t)max_ld * 8, s, bd);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// FP64 iterative refinement of the mixed-precision solve (classical mixed-precision refinement,
+// Wilkinson / Moler): r = D - F s in FP64 arithmetic on the stored FP32 F (upper copy + Fdiag),
+// dx = F^-1 r with the FP32 factor, s += dx.  The residual kernel writes per-row-block partial
+// sums of r^2 and D^2; the refine kernel stops an item once ||r|| <= tol ||D||.
+// status[4*item+2] counts steps, status[4*item+3] marks a converged item.
+// ------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_resid_b(BatchDev bd)
+{
+    __shared__ double colacc[4][TS];
+    __shared__ double rowacc[TS];
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    if (!M.valid || bd.status[4 * item + 3]) return;
+    const int ld = M.ldN, n = M.n_src, nb = ld / TS;
+    const int I = blockIdx.x;
+    if (I >= nb || I * TS >= n) return;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const float* U = bd.F32 + M.f_off;
+    double* v = bd.vec + M.vec_off;
+    const double* s = v;
+    const double* fdiag = v + 3 * (size_t)ld;
+    double* r = v + 4 * (size_t)ld;
+    double* part = v + 5 * (size_t)ld;
+    const int i = I * TS + lane;
+    // column strip: sum_{c < i} F(i,c) s_c = sum_c U[c][i] s_c, c < min(i, n)
+    double acc = 0.0;
+    const int cend = min(I * TS + TS, n);
+    for (int c = w; c < cend; c += 4)
+        if (c < i && i < n) acc += (double)U[(size_t)c * ld + i] * s[c];
+    colacc[w][lane] = acc;
+    // row strip: sum_{c > i} F(i,c) s_c = sum_c U[i][c] s_c, one wave per row
+    for (int q = w; q < TS; q += 4) {
+        const int ir = I * TS + q;
+        double a = 0.0;
+        if (ir < n)
+            for (int c = ir + 1 + lane; c < n; c += 64) a += (double)U[(size_t)ir * ld + c] * s[c];
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+        if (lane == 0) rowacc[q] = a;
+    }
+    __syncthreads();
+    if (tid < TS) {
+        double rr = 0.0, dd = 0.0;
+        if (i < n) {
+            const double D = (double)U[(size_t)i * ld + n];
+            const double Fs = fdiag[i] * s[i] + ((colacc[0][tid] + colacc[1][tid]) + (colacc[2][tid] + colacc[3][tid])) + rowacc[tid];
+            rr = D - Fs;
+            r[i] = rr;
+            rr = rr * rr;
+            dd = D * D;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            rr += __shfl_xor(rr, off, 64);
+            dd += __shfl_xor(dd, off, 64);
+        }
+        if (tid == 0) {
+            part[I] = rr;
+            part[nb + I] = dd;
+        }
+    }
+}
+
+// one refinement step per item: check convergence from the residual partials, else correct s
+__global__ void __launch_bounds__(1024) k_refine_b(BatchDev bd, int final_check, double tol)
+{
+    extern __shared__ double sx_raw[];
+    __shared__ float part[16][TS + 1];
+    __shared__ float rhs[TS];
+    __shared__ float Wl[TS][TS + 1];
+    __shared__ double red[2][64];
+    __shared__ int done;
+    float* sx = (float*)sx_raw;
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    if (!M.valid || bd.status[4 * item + 3]) return;
+    const int ld = M.ldN, nb = ld / TS, n = M.n_src;
+    const int tid = threadIdx.x;
+    double* v = bd.vec + M.vec_off;
+    double* s = v;
+    const double* r = v + 4 * (size_t)ld;
+    const double* pr = v + 5 * (size_t)ld;
+    const int nbr = (n + TS - 1) / TS;
+    if (tid < 64) {
+        double a = 0.0, b = 0.0;
+        for (int k = tid; k < nbr; k += 64) {
+            a += pr[k];
+            b += pr[nb + k];
+        }
+        red[0][tid] = a;
+        red[1][tid] = b;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double a = 0.0, b = 0.0;
+        for (int k = 0; k < 64; k++) {
+            a += red[0][k];
+            b += red[1][k];
+        }
+        const double rel = (b > 0) ? sqrt(a / b) : 0.0;
+        bd.out[QLX_OUT_STRIDE * item + 6] = rel;
+        done = (rel <= tol || final_check) ? 1 : 0;
+        if (done) bd.status[4 * item + 3] = 1;
+        else bd.status[4 * item + 2] += 1;
+    }
+    __syncthreads();
+    if (done) return;
+    const float* L = bd.F32 + M.f_off;
+    const float* W = (const float*)(bd.winv + M.w_off);
+    for (int c = tid; c < ld; c += 1024) sx[c] = (c < n) ? (float)r[c] : 0.0f;
+    __syncthreads();
+    tri_fwd<float>(L, ld, nb, W, sx, part, rhs, Wl);
+    tri_bwd<float>(L, ld, nb, W, sx, part, rhs, Wl);
+    for (int c = tid; c < n; c += 1024) s[c] += (double)sx[c];
+}
+
+void launch_refine_b(hipStream_t s, const BatchDev& bd, int max_ld, int iters, double tol)
+{
+    const int nb = max_ld / TS;
+    allow_big_lds((const void*)k_refine_b);
+    for (int it = 0; it <= iters; it++) {
+        hipLaunchKernelGGL(k_resid_b, dim3(nb, bd.B), dim3(256), 0, s, bd);
+        hipLaunchKernelGGL(k_refine_b, dim3(1, bd.B), dim3(1024), (size_t)max_ld * 4, s, bd, it == iters ? 1 : 0, tol);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// evidence: v = L s per active pixel, model = PSF (x) v over active taps, chi^2 over the mask,
+// Es = sum_l ||H_l s||^2 (= s^T R s), -2 ln E = ((0 + regterms) + ln det F) + chi^2
+// ------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_Ls_b(BatchDev bd)
+{
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!M.valid || r >= M.n_img) return;
+    const int U = 3 * bd.nsub;
+    const int* ucol = bd.ucol + ((size_t)item * bd.s_pix + r) * U;
+    const double* uval = bd.uval + ((size_t)item * bd.s_pix + r) * U;
+    const double* s = bd.vec + M.vec_off;
+    double acc = 0.0;
+    for (int e = 0; e < bd.ucnt[item * bd.s_pix + r]; e++) acc += uval[e] * s[ucol[e]];
+    bd.model[(size_t)item * 2 * bd.s_pix + bd.s_pix + r] = acc;
+}
+
+__global__ void __launch_bounds__(256) k_model_b(BatchDev bd)
+{
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!M.valid || r >= M.n_img) return;
+    const int p = bd.act_pix[item * bd.s_pix + r];
+    const int k = p / bd.ny, l = p % bd.ny;
+    const int nxh = bd.pnx / 2, nyh = bd.pny / 2;
+    const int* pidx = bd.pidx + item * bd.s_pix;
+    double* model = bd.model + (size_t)item * 2 * bd.s_pix;
+    const double* v = model + bd.s_pix;
+    double acc = 0.0;
+    for (int pk = 0; pk < bd.pnx; pk++) {
+        const int ii = k + nxh - pk;
+        if (ii < 0 || ii >= bd.nx) continue;
+        for (int pl = 0; pl < bd.pny; pl++) {
+            const int jj = l + nyh - pl;
+            if (jj < 0 || jj >= bd.ny) continue;
+            const int r2 = pidx[ii * bd.ny + jj];
+            if (r2 < 0) continue;
+            acc += bd.psf[pk * bd.pny + pl] * v[r2];
+        }
+    }
+    model[p] = acc;
+}
+
+__global__ void __launch_bounds__(1024) k_evidence_b(BatchDev bd)
+{
+    __shared__ double sh[1024];
+    __shared__ double sh2[1024];
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    if (!M.valid) return;
+    const int npix = bd.nx * bd.ny;
+    const int* pm = bd.pmapped + item * bd.s_pix;
+    const double* model = bd.model + (size_t)item * 2 * bd.s_pix;
+    const int* sidx = bd.sidx + item * bd.s_cell;
+    const double* s = bd.vec + M.vec_off;
+    double chi = 0.0;
+    for (int p = threadIdx.x; p < npix; p += 1024) {
+        if (bd.mask && !bd.mask[p]) continue;
+        const double d = bd.data[p];
+        if (pm[p]) {
+            const double r = model[p] - d;
+            chi += (r * r) * bd.covinv[p];
+        } else {
+            chi += (d * d) * bd.covinv[p];
+        }
+    }
+    double es = 0.0;
+    const int snx = M.snx, sny = M.sny;
+    for (int c = threadIdx.x; c < snx * sny; c += 1024) {
+        const int a = sidx[c];
+        if (a < 0) continue;
+        const int i = c / sny, j = c % sny;
+        const double sa = s[a];
+        int n0 = (i < snx - 1) ? sidx[(i + 1) * sny + j] : -1;
+        int n1 = (i > 0) ? sidx[(i - 1) * sny + j] : -1;
+        double h = -2.0 * sa + (n0 >= 0 ? s[n0] : 0.0) + (n1 >= 0 ? s[n1] : 0.0);
+        es += h * h;
+        n0 = (j < sny - 1) ? sidx[i * sny + j + 1] : -1;
+        n1 = (j > 0) ? sidx[i * sny + j - 1] : -1;
+        h = -2.0 * sa + (n0 >= 0 ? s[n0] : 0.0) + (n1 >= 0 ? s[n1] : 0.0);
+        es += h * h;
+    }
+    sh[threadIdx.x] = chi;
+    sh2[threadIdx.x] = es;
+    __syncthreads();
+    for (int st = 512; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            sh[threadIdx.x] += sh[threadIdx.x + st];
+            sh2[threadIdx.x] += sh2[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double* o = bd.out + QLX_OUT_STRIDE * item;
+        const double chisq = sh[0], Es = sh2[0];
+        const double logdetF = o[2], logdetR = o[3];
+        double loglike = 0.0, regterms = 0.0;
+        if (bd.lambda != 0) {
+            regterms = bd.lambda * Es - M.n_src * log(bd.lambda) - logdetR;
+            loglike += regterms;
+        }
+        loglike += logdetF;
+        loglike += chisq;
+        o[0] = loglike;
+        o[1] = chisq;
+        o[4] = Es;
+        o[5] = regterms;
+        o[7] = (double)bd.status[4 * item + 2];
+    }
+}
+
+void launch_evidence_b(hipStream_t st, const BatchDev& bd, int max_img)
+{
+    launch_fill_f64(st, bd.model, 0.0, (size_t)bd.B * 2 * bd.s_pix);
+    if (max_img > 0) {
+        hipLaunchKernelGGL(k_Ls_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, st, bd);
+        hipLaunchKernelGGL(k_model_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, st, bd);
+    }
+    hipLaunchKernelGGL(k_evidence_b, dim3(1, bd.B), dim3(1024), 0, st, bd);
+}
+
+// ------------------------------------------------------------------------------------------------
+// MFMA layout self-tests (fixed operands, results per lane/register)
+// ------------------------------------------------------------------------------------------------
+__global__ void k_selftest_mfma(double* out)
+{
+    const int lane = threadIdx.x;
+    // f64 16x16x4: A[i][k] = i + 1 + 0.01 k, B[k][j] = 1 + k + 0.1 j
+    {
+        const int i = lane & 15, k = lane >> 4;
+        double a = i + 1 + 0.01 * k, b = 1 + k + 0.1 * (lane & 15);
+        double4_t acc = {0, 0, 0, 0};
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        for (int v = 0; v < 4; v++) out[lane * 4 + v] = acc[v];
+    }
+    // f32 16x16x4: same operands in float
+    {
+        const int i = lane & 15, k = lane >> 4;
+        float a = i + 1 + 0.01f * k, b = 1 + k + 0.1f * (lane & 15);
+        float4_t acc = {0, 0, 0, 0};
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+        for (int v = 0; v < 4; v++) out[256 + lane * 4 + v] = acc[v];
+    }
+    // bf16 16x16x32: lane holds k = 8*(lane/16) .. +8 of row/col lane%16;
+    // A[i][k] = (i+1) * (k%4+1) / 4, B[k][j] = (j+1) * ((k/4)%8 + 1) / 8
+    {
+        bf16x8_t a, b;
+        for (int t = 0; t < 8; t++) {
+            const int k = 8 * (lane >> 4) + t;
+            a[t] = (__bf16)((float)((lane & 15) + 1) * (float)(k % 4 + 1) * 0.25f);
+            b[t] = (__bf16)((float)((lane & 15) + 1) * (float)((k / 4) % 8 + 1) * 0.125f);
+        }
+        float4_t acc = {0, 0, 0, 0};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+        for (int v = 0; v < 4; v++) out[512 + lane * 4 + v] = acc[v];
+    }
+}
+
+void launch_selftest_mfma(hipStream_t s, double* out)
+{
+    hipLaunchKernelGGL(k_selftest_mfma, dim3(1), dim3(64), 0, s, out);
+}
+
+}  // namespace qlx

@@ -1,6 +1,11 @@
-// Host side of the qlx C ABI: context, device buffers, and the per-evaluation pipeline that
+// Host side of the qlx C ABI: context, device buffers, and the batched evaluation pipeline that
 // replaces the Cartesian/DENSE/curvature block of QLens::invert_image_surface_brightness_map
 // (lens.cpp:12364-12615) and its chi-square loop (lens.cpp:12950-13015).
+//
+// A call evaluates its items in waves.  Per wave: ray shooting + active sets for every item,
+// one small device->host copy of the active counts (sizes of each item's dense problem), the
+// lensing rows and block-support masks, a second copy of the support-block counts (sizes of the
+// compact Lb pools), then the dense stages for all items in shared launches.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -13,14 +18,6 @@
 #include <vector>
 
 #include "qlx_internal.h"
-
-namespace qlx {
-void upload_patterson(int device);
-void launch_prep_fr(hipStream_t s, double* F, double* R, int ld, int n, double lambda);
-void launch_trisolve_f64_w(hipStream_t s, const double* L, int ld, const double* Winv, const double* b, const double* yrow, int n,
-                           double* x);
-void launch_selftest_mfma(hipStream_t s, double* out);
-}  // namespace qlx
 
 using namespace qlx;
 
@@ -44,6 +41,7 @@ struct DevBuf {
         if (n <= bytes && p) return true;
         release();
         if (n == 0) n = 8;
+        n = (n + 255) & ~(size_t)255;
         if (hipMalloc(&p, n) != hipSuccess) {
             p = nullptr;
             return false;
@@ -64,9 +62,8 @@ struct Timer {
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipStream_t stream2 = nullptr;        // runs the R factorization beside F's
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::string err;
+    size_t hbm_budget = 0;
     // image
     int nx = 0, ny = 0;
     bool image_set = false;
@@ -82,15 +79,15 @@ struct Ctx {
     // settings
     qlx_settings set{};
     bool settings_set = false;
-    DevBuf sxv, syv, scx, scy;
-    // per-item work
+    // per-item inputs
+    DevBuf d_models, d_sgrid, d_srcn, d_meta, d_mats;
+    // per-item work (stride s_pix / s_cell)
     DevBuf beta, map_cell, map_w, pmapped, cmapped, sidx, pidx, act_pix, counts, ucol, uval, ucnt, nevals;
-    DevBuf key, bucket_cnt, bucket_off, bucket_fill, perm;
-    // dense
-    DevBuf Lb, cvec, F, R, D, s, diagF, diagR, work, work2, nz, model, out, logdets, status;
+    DevBuf key, bcnt, boff, bfill, perm;
+    // pools
+    DevBuf lb, F, R, cvec, vec, nz, bbase, winv, model, out, status;
     // last solution
     std::vector<double> last_src, last_model;
-    int last_snx = 0, last_sny = 0;
     // timers
     std::map<std::string, Timer> timers;
     qlx_stats stats{};
@@ -156,6 +153,7 @@ void prep_model(const qlx_lens_model& in, ModelPrep& out)
 {
     memset(&out, 0, sizeof(out));
     out.nlens = in.nlens;
+    out.sur_lens = -1;
     for (int l = 0; l < in.nlens; l++) {
         const qlx_lens& s = in.lens[l];
         LensPrep& L = out.lens[l];
@@ -218,225 +216,431 @@ int check_model(Ctx* c, const qlx_lens_model* m)
         if (t == QLX_LENS_SIE && !(m->lens[l].p[1] > 0 && m->lens[l].p[1] <= 1)) return fail(c, QLX_ERR_ARG, "SIE q must be in (0,1]");
         if (t == QLX_LENS_NFW && !(m->lens[l].p[2] > 0 && m->lens[l].p[2] <= 1 && m->lens[l].p[1] > 0))
             return fail(c, QLX_ERR_ARG, "NFW needs rs > 0 and q in (0,1]");
+        if (m->lens[l].def_mode != QLX_DEF_QUADRATURE && m->lens[l].def_mode != QLX_DEF_SURROGATE)
+            return fail(c, QLX_ERR_ARG, "unknown deflection mode");
+        if (t == QLX_LENS_NFW && m->lens[l].def_mode == QLX_DEF_SURROGATE)
+            return fail(c, QLX_ERR_STATE, "surrogate deflections are not available in this build");
     }
     return QLX_OK;
 }
 
-GridDev make_grid(Ctx* c, int snx, int sny)
-{
-    GridDev g;
-    g.nx = c->nx;
-    g.ny = c->ny;
-    g.nsplit = c->set.nsplit;
-    g.nsub = c->set.nsplit * c->set.nsplit;
-    g.xvals = c->xvals.as<double>();
-    g.yvals = c->yvals.as<double>();
-    g.mask = c->has_mask ? c->mask.as<unsigned char>() : nullptr;
-    g.snx = snx;
-    g.sny = sny;
-    g.sxv = c->sxv.as<double>();
-    g.syv = c->syv.as<double>();
-    g.scx = c->scx.as<double>();
-    g.scy = c->scy.as<double>();
-    return g;
-}
-
 // Source grid corners accumulated x += xstep and 4-corner-mean centres
 // (SourcePixelGrid::create_pixel_grid pixelgrid.cpp:441-455, SourcePixel ctor 761-783).
-int upload_source_grid(Ctx* c, int snx, int sny)
+void fill_source_grid(const qlx_settings& s, int snx, int sny, double* g)
 {
-    const qlx_settings& s = c->set;
-    std::vector<double> sx(snx + 1), sy(sny + 1), cx(snx), cy(sny);
-    double xstep = (s.src_xmax - s.src_xmin) / snx, ystep = (s.src_ymax - s.src_ymin) / sny;
+    double* sx = g;
+    double* sy = g + QLX_SG_SYV;
+    double* cx = g + QLX_SG_SCX;
+    double* cy = g + QLX_SG_SCY;
+    const double xstep = (s.src_xmax - s.src_xmin) / snx, ystep = (s.src_ymax - s.src_ymin) / sny;
     double x = s.src_xmin, y = s.src_ymin;
     for (int i = 0; i <= snx; i++, x += xstep) sx[i] = x;
     for (int j = 0; j <= sny; j++, y += ystep) sy[j] = y;
     for (int i = 0; i < snx; i++) cx[i] = (sx[i] + sx[i] + sx[i + 1] + sx[i + 1]) / 4.0;
     for (int j = 0; j < sny; j++) cy[j] = (sy[j] + sy[j + 1] + sy[j] + sy[j + 1]) / 4.0;
-    if (!c->sxv.ensure(sizeof(double) * (snx + 1)) || !c->syv.ensure(sizeof(double) * (sny + 1)) ||
-        !c->scx.ensure(sizeof(double) * snx) || !c->scy.ensure(sizeof(double) * sny))
-        return fail(c, QLX_ERR_NOMEM, "source grid allocation failed");
-    HIPCHK(c, hipMemcpyAsync(c->sxv.p, sx.data(), sizeof(double) * (snx + 1), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->syv.p, sy.data(), sizeof(double) * (sny + 1), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->scx.p, cx.data(), sizeof(double) * snx, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->scy.p, cy.data(), sizeof(double) * sny, hipMemcpyHostToDevice, c->stream));
-    return QLX_OK;
 }
 
-int ensure_item_bufs(Ctx* c, int snx, int sny)
+// bytes of the per-item ray/row work arrays
+size_t stage1_bytes(size_t s_pix, size_t s_cell, size_t nsub)
 {
-    size_t npix = (size_t)c->nx * c->ny;
-    size_t nsub = (size_t)c->set.nsplit * c->set.nsplit;
-    size_t ncell = (size_t)snx * sny;
-    size_t U = 3 * nsub;
-    bool ok = c->beta.ensure(npix * nsub * 2 * 8) && c->map_cell.ensure(npix * nsub * 3 * 4) && c->map_w.ensure(npix * nsub * 3 * 8) &&
-              c->pmapped.ensure(npix * 4) && c->cmapped.ensure(ncell * 4) && c->sidx.ensure(ncell * 4) && c->pidx.ensure(npix * 4) &&
-              c->act_pix.ensure(npix * 4) && c->counts.ensure(16 * 4) && c->ucol.ensure(npix * U * 4) && c->uval.ensure(npix * U * 8) &&
-              c->ucnt.ensure(npix * 4) && c->nevals.ensure(8) && c->out.ensure(16 * 8) && c->logdets.ensure(4 * 8) &&
-              c->status.ensure(4 * 4) && c->model.ensure((npix * 2 + 64) * 8) && c->key.ensure(npix * 4) &&
-              c->bucket_cnt.ensure((ncell + 2) * 4) && c->bucket_off.ensure((ncell + 2) * 4) &&
-              c->bucket_fill.ensure((ncell + 2) * 4) && c->perm.ensure(npix * 4);
-    if (!ok) return fail(c, QLX_ERR_NOMEM, "work buffer allocation failed");
-    return QLX_OK;
+    const size_t U = 3 * nsub;
+    return s_pix * nsub * (16 + 12 + 24) + s_pix * U * 12 + s_pix * 4 * 7 + s_cell * 4 * 5 + 64 + s_pix * 16;
 }
 
-ItemBufs item_bufs(Ctx* c)
+// upper bound of the dense bytes of one item with at most ncell active source cells
+size_t dense_bytes_bound(size_t ncell, size_t snx, bool mixed)
 {
-    ItemBufs b;
-    b.beta = c->beta.as<double>();
-    b.map_cell = c->map_cell.as<int>();
-    b.map_w = c->map_w.as<double>();
-    b.pmapped = c->pmapped.as<int>();
-    b.cmapped = c->cmapped.as<int>();
-    b.sidx = c->sidx.as<int>();
-    b.pidx = c->pidx.as<int>();
-    b.act_pix = c->act_pix.as<int>();
-    b.counts = c->counts.as<int>();
-    b.ucol = c->ucol.as<int>();
-    b.uval = c->uval.as<double>();
-    b.ucnt = c->ucnt.as<int>();
-    b.nevals = (long long*)c->nevals.p;
-    b.key = c->key.as<int>();
-    b.bucket_cnt = c->bucket_cnt.as<int>();
-    b.bucket_off = c->bucket_off.as<int>();
-    b.bucket_fill = c->bucket_fill.as<int>();
-    b.perm = c->perm.as<int>();
-    return b;
+    const size_t ldN = round_up(ncell + 1, 64), nb = ldN / 64;
+    const size_t bwb = (2 * snx) / 64 + 1;
+    const size_t f = ldN * ldN * (mixed ? 4 : 8);
+    const size_t r = nb * (bwb + 1) * 4096 * 8;
+    return f + r + ldN * 8 * (QLX_VEC_SLOTS + 128);
 }
 
-// One evaluation in FP64 (parity) mode.
-int eval_one_fp64(Ctx* c, const qlx_lens_model& model, int snx, int sny, qlx_result* res)
+struct WaveItem {
+    int idx;       // index in the caller's arrays
+    int snx, sny;
+};
+
+// One wave of items through the whole pipeline.  Results go to out[item.idx].
+int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& items, qlx_result* out, bool keep_last)
 {
-    memset(res, 0, sizeof(*res));
-    int rc = upload_source_grid(c, snx, sny);
-    if (rc) return rc;
-    rc = ensure_item_bufs(c, snx, sny);
-    if (rc) return rc;
-    ModelPrep mp;
-    prep_model(model, mp);
-    GridDev g = make_grid(c, snx, sny);
-    ItemBufs b = item_bufs(c);
+    const int B = (int)items.size();
+    if (B == 0) return QLX_OK;
+    const qlx_settings& S = c->set;
+    const bool mixed = S.precision == QLX_PREC_MIXED;
+    const int nsub = S.nsplit * S.nsplit;
+    const size_t s_pix = (size_t)c->nx * c->ny;
+    size_t s_cell = 4;
+    for (auto& it : items) s_cell = std::max(s_cell, (size_t)it.snx * it.sny);
+    const size_t U = 3 * (size_t)nsub;
     hipStream_t st = c->stream;
-    const size_t npix = (size_t)c->nx * c->ny;
-    launch_fill_i32(st, b.pmapped, 0, npix);
-    launch_fill_i32(st, b.cmapped, 0, (size_t)snx * sny);
-    launch_fill_i32(st, c->status.as<int>(), 0, 4);
-    launch_fill_i32(st, (int*)c->nevals.p, 0, 2);
+
+    // ---- inputs ----
+    std::vector<ModelPrep> mp(B);
+    std::vector<double> sg((size_t)B * QLX_SG_STRIDE, 0.0);
+    std::vector<int> srcn(2 * B);
+    for (int b = 0; b < B; b++) {
+        prep_model(models[items[b].idx], mp[b]);
+        fill_source_grid(S, items[b].snx, items[b].sny, sg.data() + (size_t)b * QLX_SG_STRIDE);
+        srcn[2 * b] = items[b].snx;
+        srcn[2 * b + 1] = items[b].sny;
+    }
+    bool ok = c->d_models.ensure(sizeof(ModelPrep) * B) && c->d_sgrid.ensure(sg.size() * 8) && c->d_srcn.ensure(srcn.size() * 4) &&
+              c->d_meta.ensure(sizeof(ItemMeta) * B) && c->d_mats.ensure(sizeof(MatDesc) * 2 * B) &&
+              c->beta.ensure(B * s_pix * nsub * 16) && c->map_cell.ensure(B * s_pix * nsub * 12) &&
+              c->map_w.ensure(B * s_pix * nsub * 24) && c->pmapped.ensure(B * s_pix * 4) && c->cmapped.ensure(B * s_cell * 4) &&
+              c->sidx.ensure(B * s_cell * 4) && c->pidx.ensure(B * s_pix * 4) && c->act_pix.ensure(B * s_pix * 4) &&
+              c->counts.ensure((size_t)B * 16) && c->ucol.ensure(B * s_pix * U * 4) && c->uval.ensure(B * s_pix * U * 8) &&
+              c->ucnt.ensure(B * s_pix * 4) && c->nevals.ensure((size_t)B * 8) && c->key.ensure(B * s_pix * 4) &&
+              c->bcnt.ensure(B * (s_cell + 2) * 4) && c->boff.ensure(B * (s_cell + 2) * 4) && c->bfill.ensure(B * (s_cell + 2) * 4) &&
+              c->perm.ensure(B * s_pix * 4) && c->model.ensure(B * s_pix * 16) && c->out.ensure((size_t)B * QLX_OUT_STRIDE * 8) &&
+              c->status.ensure((size_t)B * 16);
+    if (!ok) return fail(c, QLX_ERR_NOMEM, "work buffer allocation failed");
+    HIPCHK(c, hipMemcpyAsync(c->d_models.p, mp.data(), sizeof(ModelPrep) * B, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(c->d_sgrid.p, sg.data(), sg.size() * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(c->d_srcn.p, srcn.data(), srcn.size() * 4, hipMemcpyHostToDevice, st));
+
+    BatchDev bd{};
+    bd.B = B;
+    bd.nx = c->nx;
+    bd.ny = c->ny;
+    bd.nsplit = S.nsplit;
+    bd.nsub = nsub;
+    bd.xvals = c->xvals.as<double>();
+    bd.yvals = c->yvals.as<double>();
+    bd.mask = c->has_mask ? c->mask.as<unsigned char>() : nullptr;
+    bd.data = c->data.as<double>();
+    bd.covinv = c->covinv.as<double>();
+    bd.psf = c->psf.as<double>();
+    bd.pnx = c->pnx;
+    bd.pny = c->pny;
+    bd.tol = S.integral_tol;
+    bd.lambda = S.regparam;
+    bd.models = c->d_models.as<ModelPrep>();
+    bd.sgrid = c->d_sgrid.as<double>();
+    bd.src_n = c->d_srcn.as<int>();
+    bd.sur_alpha = nullptr;
+    bd.beta = c->beta.as<double>();
+    bd.map_cell = c->map_cell.as<int>();
+    bd.map_w = c->map_w.as<double>();
+    bd.pmapped = c->pmapped.as<int>();
+    bd.cmapped = c->cmapped.as<int>();
+    bd.sidx = c->sidx.as<int>();
+    bd.pidx = c->pidx.as<int>();
+    bd.act_pix = c->act_pix.as<int>();
+    bd.counts = c->counts.as<int>();
+    bd.ucol = c->ucol.as<int>();
+    bd.uval = c->uval.as<double>();
+    bd.ucnt = c->ucnt.as<int>();
+    bd.key = c->key.as<int>();
+    bd.bcnt = c->bcnt.as<int>();
+    bd.boff = c->boff.as<int>();
+    bd.bfill = c->bfill.as<int>();
+    bd.perm = c->perm.as<int>();
+    bd.nevals = c->nevals.as<unsigned long long>();
+    bd.s_pix = (long long)s_pix;
+    bd.s_cell = (long long)s_cell;
+    bd.meta = c->d_meta.as<ItemMeta>();
+    bd.model = c->model.as<double>();
+    bd.out = c->out.as<double>();
+    bd.status = c->status.as<int>();
+
+    launch_fill_i32(st, bd.pmapped, 0, B * s_pix);
+    launch_fill_i32(st, bd.cmapped, 0, B * s_cell);
+    launch_fill_i32(st, bd.status, 0, (size_t)B * 4);
+    launch_fill_i32(st, (int*)bd.nevals, 0, (size_t)B * 2);
+    launch_fill_f64(st, bd.out, 0.0, (size_t)B * QLX_OUT_STRIDE);
 
     timer_begin(c, "total");
     timer_begin(c, "raytrace");
-    launch_raytrace_map(st, g, mp, c->set.integral_tol, c->set.ray_precision == QLX_RAY_FP32, b, nullptr);
+    launch_raytrace_map_b(st, bd, S.ray_precision == QLX_RAY_FP32);
     timer_end(c, "raytrace");
-    launch_compact(st, g, b);
-    int counts[2];
-    HIPCHK(c, hipMemcpyAsync(counts, b.counts, 8, hipMemcpyDeviceToHost, st));
+    launch_compact_b(st, bd);
+    std::vector<int> counts((size_t)B * 4);
+    HIPCHK(c, hipMemcpyAsync(counts.data(), bd.counts, (size_t)B * 16, hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipStreamSynchronize(st));
-    const int n_src = counts[0], n_img = counts[1];
-    res->n_src_active = n_src;
-    res->n_img_active = n_img;
-    if (n_src == 0) {
+
+    // ---- per-item dense sizes ----
+    std::vector<ItemMeta> meta(B);
+    size_t f_tot = 0, r_tot = 0, cv_tot = 0, vec_tot = 0, nz_tot = 0, bb_tot = 0, w_tot = 0;
+    int max_img = 0, max_src = 0, max_rows = 0, max_nkt = 0, max_ncb = 0, max_ld = 0, max_cells = 0, max_rtiles = 0;
+    for (int b = 0; b < B; b++) {
+        ItemMeta& M = meta[b];
+        memset(&M, 0, sizeof(M));
+        const int n_src = counts[4 * b], n_img = counts[4 * b + 1];
+        qlx_result* r = &out[items[b].idx];
+        memset(r, 0, sizeof(*r));
+        r->n_src_active = n_src;
+        r->n_img_active = n_img;
+        M.n_src = n_src;
+        M.n_img = n_img;
+        M.snx = items[b].snx;
+        M.sny = items[b].sny;
+        if (n_src == 0) {  // pixelgrid.cpp:12807 -> lens.cpp:12528-12530
+            r->minus2lnE = 2e30;
+            r->status = QLX_ERR_NOSOURCE;
+            continue;
+        }
+        if (n_src + 1 > QLX_MAX_LD) {
+            r->status = QLX_ERR_RANGE;
+            continue;
+        }
+        M.valid = 1;
+        M.ldN = (int)round_up((size_t)n_src + 1, 64);
+        M.nrows = (int)std::max<size_t>(64, round_up((size_t)n_img, 64));
+        M.ncb = M.ldN / 64;
+        M.ncbw = (M.ncb + 31) / 32;
+        M.nkt = M.nrows / 64;
+        M.bwb = (2 * M.snx) / 64 + 1;
+        const size_t nb = M.ldN / 64;
+        M.f_off = (long long)f_tot;
+        f_tot += (size_t)M.ldN * M.ldN;
+        M.r_off = (long long)r_tot;
+        r_tot += nb * (M.bwb + 1) * 4096;
+        M.cvec_off = (long long)cv_tot;
+        cv_tot += M.nrows;
+        M.vec_off = (long long)vec_tot;
+        vec_tot += (size_t)QLX_VEC_SLOTS * M.ldN;
+        M.nz_off = (long long)nz_tot;
+        nz_tot += (size_t)M.nkt * M.ncbw;
+        M.bb_off = (long long)bb_tot;
+        bb_tot += M.nkt + 1;
+        M.w_off = (long long)w_tot;
+        w_tot += (size_t)2 * M.ldN * 64;
+        max_img = std::max(max_img, n_img);
+        max_src = std::max(max_src, n_src);
+        max_rows = std::max(max_rows, M.nrows);
+        max_nkt = std::max(max_nkt, M.nkt);
+        max_ncb = std::max(max_ncb, M.ncb);
+        max_ld = std::max(max_ld, M.ldN);
+        max_cells = std::max(max_cells, M.snx * M.sny);
+        max_rtiles = std::max(max_rtiles, (int)(nb * (M.bwb + 1)));
+    }
+    if (max_src == 0) {
         timer_end(c, "total");
-        res->minus2lnE = 2e30;  // pixelgrid.cpp:12807 -> lens.cpp:12528-12530
-        res->status = QLX_ERR_NOSOURCE;
+        HIPCHK(c, hipStreamSynchronize(st));
+        timers_collect(c);
         return QLX_OK;
     }
-    launch_merge_lrows(st, g, b, n_img);
+    ok = c->R.ensure(r_tot * 8) && c->cvec.ensure(cv_tot * 8) && c->vec.ensure(vec_tot * 8) && c->nz.ensure(nz_tot * 4) &&
+         c->bbase.ensure(bb_tot * 4) && c->winv.ensure(w_tot * 8) && c->F.ensure(f_tot * (mixed ? 4 : 8));
+    if (!ok) return fail(c, QLX_ERR_NOMEM, "dense buffer allocation failed (F/R)");
+    bd.F = mixed ? nullptr : c->F.as<double>();
+    bd.F32 = mixed ? c->F.as<float>() : nullptr;
+    bd.R = c->R.as<double>();
+    bd.cvec = c->cvec.as<double>();
+    bd.vec = c->vec.as<double>();
+    bd.nz = c->nz.as<unsigned>();
+    bd.bbase = c->bbase.as<int>();
+    bd.winv = c->winv.as<double>();
+    HIPCHK(c, hipMemcpyAsync(c->d_meta.p, meta.data(), sizeof(ItemMeta) * B, hipMemcpyHostToDevice, st));
 
-    const int ldN = (int)round_up((size_t)n_src + 1, 64);
-    const int nrows = (int)std::max<size_t>(64, round_up((size_t)n_img, 64));
-    const int ncb = ldN / 64, ncbw = (ncb + 31) / 32, nkt = nrows / 64;
-    if (!c->Lb.ensure((size_t)nrows * ldN * 8) || !c->cvec.ensure((size_t)nrows * 8) || !c->F.ensure((size_t)ldN * ldN * 8) ||
-        !c->R.ensure((size_t)ldN * ldN * 8) || !c->D.ensure((size_t)ldN * 8) || !c->s.ensure((size_t)ldN * 8) ||
-        !c->diagF.ensure((size_t)ldN * 8) || !c->diagR.ensure((size_t)ldN * 8) || !c->work.ensure((size_t)ldN * 64 * 8) || !c->work2.ensure((size_t)ldN * 64 * 8) ||
-        !c->nz.ensure((size_t)nkt * ncbw * 4))
-        return fail(c, QLX_ERR_NOMEM, "dense buffer allocation failed (Lb/F/R)");
-    double* Lb = c->Lb.as<double>();
-    launch_fill_i32(st, c->nz.as<int>(), 0, (size_t)nkt * ncbw);
     timer_begin(c, "lbuild");
-    launch_sort_rows(st, g, b, n_img, n_src);
-    launch_row_support(st, g, b, n_img, n_src, c->pnx, c->pny, c->nz.as<unsigned int>(), ncbw);
-    launch_build_lb_blocks_f64(st, g, b, n_img, nrows, n_src, c->psf.as<double>(), c->pnx, c->pny, c->data.as<double>(), Lb, ncb,
-                               c->nz.as<unsigned int>(), ncbw);
+    launch_merge_lrows_b(st, bd, max_img);
+    launch_sort_rows_b(st, bd, max_img, max_src);
+    launch_row_support_b(st, bd, max_nkt);
+    HIPCHK(c, hipMemcpyAsync(counts.data(), bd.counts, (size_t)B * 16, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    size_t lb_tot = 0;
+    int max_nsup = 0;
+    for (int b = 0; b < B; b++) {
+        ItemMeta& M = meta[b];
+        if (!M.valid) continue;
+        M.nblk = counts[4 * b + 2];
+        M.nsup_max = counts[4 * b + 3];
+        M.lb_off = (long long)lb_tot;
+        lb_tot += (size_t)M.nblk * 4096 * (mixed ? 2 : 1);
+        max_nsup = std::max(max_nsup, M.nsup_max);
+    }
+    if (!c->lb.ensure(lb_tot * (mixed ? 2 : 8))) return fail(c, QLX_ERR_NOMEM, "Lb pool allocation failed");
+    bd.lb = c->lb.p;
+    HIPCHK(c, hipMemcpyAsync(c->d_meta.p, meta.data(), sizeof(ItemMeta) * B, hipMemcpyHostToDevice, st));
+    launch_build_lb_b(st, bd, max_rows, max_nsup, mixed ? 1 : 0);
     timer_end(c, "lbuild");
-    launch_gather_cvec_sorted(st, b, c->covinv.as<double>(), n_img, nrows, c->cvec.as<double>());
+    if (!mixed) launch_gather_cvec_b(st, bd, max_rows);
     timer_begin(c, "syrk");
-    launch_syrk_blocks_f64(st, Lb, ncb, nkt, c->cvec.as<double>(), c->F.as<double>(), ldN, c->nz.as<unsigned int>(), ncbw);
+    if (mixed) launch_syrk_bf16_b(st, bd, max_ncb);
+    else launch_syrk_f64_b(st, bd, max_ncb);
     timer_end(c, "syrk");
-    launch_fill_f64(st, c->R.as<double>(), 0.0, (size_t)ldN * ldN);
-    launch_add_curvature(st, g, b, n_src, c->set.regparam, c->F.as<double>(), ldN, c->R.as<double>(), ldN);
-    launch_prep_fr(st, c->F.as<double>(), c->R.as<double>(), ldN, n_src, c->set.regparam);
-    // R (ln det R only) and F (solve + ln det F) factor concurrently on two streams
+    launch_curvature_b(st, bd, max_cells, max_rtiles);
+    launch_prep_fr_b(st, bd, max_ld, mixed ? 1 : 0);
+
+    // ---- factorizations: F (dense, FP64 or FP32) and R (banded FP64) of every item together ----
+    std::vector<MatDesc> mats;
+    mats.reserve(2 * B);
+    for (int b = 0; b < B; b++) {
+        const ItemMeta& M = meta[b];
+        if (!M.valid) continue;
+        double* v = bd.vec + M.vec_off;
+        MatDesc f{};
+        f.A = mixed ? (void*)(bd.F32 + M.f_off) : (void*)(bd.F + M.f_off);
+        f.ld = M.ldN;
+        f.n = M.n_src;
+        f.nb = M.ldN / 64;
+        f.f32 = mixed ? 1 : 0;
+        f.bwb = 0;
+        f.diag = (void*)(v + M.ldN);
+        f.status = bd.status + 4 * b;
+        f.W = (void*)(bd.winv + M.w_off);
+        mats.push_back(f);
+        MatDesc r{};
+        r.A = (void*)(bd.R + M.r_off);
+        r.ld = 64;
+        r.n = M.n_src;
+        r.nb = M.ldN / 64;
+        r.f32 = 0;
+        r.bwb = M.bwb;
+        r.diag = (void*)(v + 2 * (size_t)M.ldN);
+        r.status = bd.status + 4 * b + 1;
+        r.W = (void*)(bd.winv + M.w_off + (size_t)M.ldN * 64);
+        mats.push_back(r);
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_mats.p, mats.data(), sizeof(MatDesc) * mats.size(), hipMemcpyHostToDevice, st));
     timer_begin(c, "cholesky");
-    HIPCHK(c, hipEventRecord(c->ev_fork, st));
-    HIPCHK(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
-    cholesky_f64(c->stream2, c->R.as<double>(), ldN, n_src, c->diagR.as<double>(), c->status.as<int>() + 1, c->work2.as<double>());
-    launch_logdet(c->stream2, c->diagR.as<double>(), n_src, c->logdets.as<double>() + 1);
-    HIPCHK(c, hipEventRecord(c->ev_join, c->stream2));
-    cholesky_f64(st, c->F.as<double>(), ldN, n_src, c->diagF.as<double>(), c->status.as<int>(), c->work.as<double>());
-    launch_logdet(st, c->diagF.as<double>(), n_src, c->logdets.as<double>());
-    HIPCHK(c, hipStreamWaitEvent(st, c->ev_join, 0));
+    cholesky_batched(st, c->d_mats.as<MatDesc>(), (int)mats.size(), max_ld / 64);
     timer_end(c, "cholesky");
-    // backward solve only: y = L^-1 D is row n_src of the factored augmented matrix
-    launch_trisolve_f64_w(st, c->F.as<double>(), ldN, c->work.as<double>(), nullptr, c->F.as<double>() + (size_t)n_src * ldN, n_src,
-                          c->s.as<double>());
-    launch_evidence(st, g, b, n_img, n_src, c->s.as<double>(), c->psf.as<double>(), c->pnx, c->pny, c->data.as<double>(),
-                    c->covinv.as<double>(), c->set.regparam, c->logdets.as<double>(), c->model.as<double>(), c->out.as<double>());
+    launch_logdets_b(st, bd, mixed ? 1 : 0);
+    timer_begin(c, "solve");
+    launch_trisolve_b(st, bd, max_ld, mixed ? 1 : 0);
+    if (mixed) {
+        const int iters = S.refine_iters > 0 ? S.refine_iters : 6;
+        const double rtol = S.refine_tol > 0 ? S.refine_tol : 1e-10;
+        launch_refine_b(st, bd, max_ld, iters, rtol);
+    }
+    timer_end(c, "solve");
+    launch_evidence_b(st, bd, max_img);
     timer_end(c, "total");
-    double out[6];
-    int status[2];
-    long long nev = 0;
-    HIPCHK(c, hipMemcpyAsync(out, c->out.p, sizeof(out), hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipMemcpyAsync(status, c->status.p, sizeof(status), hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipMemcpyAsync(&nev, c->nevals.p, 8, hipMemcpyDeviceToHost, st));
+
+    std::vector<double> o((size_t)B * QLX_OUT_STRIDE);
+    std::vector<int> stat((size_t)B * 4);
+    std::vector<unsigned long long> nev(B);
+    HIPCHK(c, hipMemcpyAsync(o.data(), bd.out, o.size() * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(stat.data(), bd.status, stat.size() * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(nev.data(), bd.nevals, (size_t)B * 8, hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipStreamSynchronize(st));
     timers_collect(c);
-    res->minus2lnE = out[0];
-    res->chisq0 = out[1];
-    res->logdetF = out[2];
-    res->logdetR = out[3];
-    res->Es = out[4];
-    res->status = (status[0] || status[1]) ? QLX_ERR_NOT_PD : QLX_OK;
-
-    // work accounting: executed vs dense SYRK tile-chunks from the occupancy bitmap
-    {
-        std::vector<unsigned int> nzh((size_t)nkt * ncbw);
-        HIPCHK(c, hipMemcpy(nzh.data(), c->nz.p, nzh.size() * 4, hipMemcpyDeviceToHost));
-        double exec_pairs = 0;
-        for (int kt = 0; kt < nkt; kt++) {
-            long long bits = 0;
-            for (int w = 0; w < ncbw; w++) bits += __builtin_popcount(nzh[(size_t)kt * ncbw + w]);
-            exec_pairs += (double)bits * (bits + 1) / 2;
-        }
-        const double tile = 2.0 * 64 * 64 * 64;
-        qlx_stats& S = c->stats;
-        S.syrk_exec_flops = exec_pairs * tile;
-        S.syrk_dense_flops = (double)ncb * (ncb + 1) / 2 * nkt * tile;
-        S.syrk_alg_flops = (double)n_img * n_src * (n_src + 1) + 2.0 * n_img * n_src;
-        double nblk = 0;
-        for (size_t w = 0; w < nzh.size(); w++) nblk += __builtin_popcount(nzh[w]);
-        S.lb_bytes = nblk * 64 * 64 * 8;
-        S.integrand_evals = (double)nev;
-        S.chol_flops = 2.0 * (double)n_src * n_src * n_src / 3.0;
-        S.n_img = n_img;
-        S.n_src = n_src;
-        S.ld_src = ldN;
-        S.rows_pad = nrows;
+    for (int b = 0; b < B; b++) {
+        if (!meta[b].valid) continue;
+        qlx_result* r = &out[items[b].idx];
+        const double* ob = o.data() + (size_t)b * QLX_OUT_STRIDE;
+        r->minus2lnE = ob[0];
+        r->chisq0 = ob[1];
+        r->logdetF = ob[2];
+        r->logdetR = ob[3];
+        r->Es = ob[4];
+        r->refine_resid = ob[6];
+        r->refine_steps = (int)ob[7];
+        r->status = (stat[4 * b] || stat[4 * b + 1]) ? QLX_ERR_NOT_PD : QLX_OK;
     }
 
-    // keep the solution of the last item for qlx_get_solution
-    std::vector<double> s(n_src);
-    std::vector<int> sidx((size_t)snx * sny);
-    c->last_model.resize(npix);
-    HIPCHK(c, hipMemcpyAsync(s.data(), c->s.p, sizeof(double) * n_src, hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipMemcpyAsync(sidx.data(), b.sidx, sizeof(int) * sidx.size(), hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipMemcpyAsync(c->last_model.data(), c->model.p, sizeof(double) * npix, hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipStreamSynchronize(st));
-    c->last_src.assign((size_t)snx * sny, 0.0);
-    for (size_t k = 0; k < sidx.size(); k++)
-        if (sidx[k] >= 0) c->last_src[k] = s[sidx[k]];
-    c->last_snx = snx;
-    c->last_sny = sny;
+    // ---- work accounting over the wave (the launches timed above) ----
+    {
+        std::vector<unsigned> nzh(nz_tot);
+        HIPCHK(c, hipMemcpy(nzh.data(), bd.nz, nz_tot * 4, hipMemcpyDeviceToHost));
+        qlx_stats& T = c->stats;
+        memset(&T, 0, sizeof(T));
+        const double tile = 2.0 * 64 * 64 * 64;
+        for (int b = 0; b < B; b++) {
+            const ItemMeta& M = meta[b];
+            if (!M.valid) continue;
+            double exec_pairs = 0, nblk = 0;
+            for (int kt = 0; kt < M.nkt; kt++) {
+                double bits = 0;
+                for (int w = 0; w < M.ncbw; w++) bits += __builtin_popcount(nzh[M.nz_off + (size_t)kt * M.ncbw + w]);
+                exec_pairs += bits * (bits + 1) / 2;
+                nblk += bits;
+            }
+            T.syrk_exec_flops += exec_pairs * tile * (mixed ? 3 : 1);
+            T.syrk_dense_flops += (double)M.ncb * (M.ncb + 1) / 2 * M.nkt * tile;
+            T.syrk_alg_flops += (double)M.n_img * M.n_src * (M.n_src + 1) + 2.0 * M.n_img * M.n_src;
+            T.lb_bytes += nblk * 64 * 64 * (mixed ? 4 : 8);
+            T.integrand_evals += (double)nev[b];
+            T.chol_flops += (double)M.n_src * M.n_src * M.n_src / 3.0;
+            T.n_items++;
+            T.n_img = M.n_img;
+            T.n_src = M.n_src;
+            T.ld_src = M.ldN;
+            T.rows_pad = M.nrows;
+        }
+    }
+
+    // ---- keep the last valid item's solution for qlx_get_solution ----
+    if (keep_last) {
+        int lb = -1;
+        for (int b = B - 1; b >= 0; b--)
+            if (meta[b].valid) {
+                lb = b;
+                break;
+            }
+        if (lb >= 0) {
+            const ItemMeta& M = meta[lb];
+            std::vector<double> s(M.n_src);
+            std::vector<int> sidx((size_t)M.snx * M.sny);
+            c->last_model.resize(s_pix);
+            HIPCHK(c, hipMemcpyAsync(s.data(), bd.vec + M.vec_off, sizeof(double) * M.n_src, hipMemcpyDeviceToHost, st));
+            HIPCHK(c, hipMemcpyAsync(sidx.data(), bd.sidx + (size_t)lb * s_cell, sizeof(int) * sidx.size(), hipMemcpyDeviceToHost, st));
+            HIPCHK(c, hipMemcpyAsync(c->last_model.data(), bd.model + (size_t)lb * 2 * s_pix, sizeof(double) * s_pix,
+                                     hipMemcpyDeviceToHost, st));
+            HIPCHK(c, hipStreamSynchronize(st));
+            c->last_src.assign(sidx.size(), 0.0);
+            for (size_t k = 0; k < sidx.size(); k++)
+                if (sidx[k] >= 0) c->last_src[k] = s[sidx[k]];
+        }
+    }
     return QLX_OK;
+}
+
+// Evaluate nbatch items (per-item source grids) in waves sized to the HBM budget.
+int eval_items(Ctx* c, int nbatch, const qlx_lens_model* models, const int* src_n, qlx_result* out)
+{
+    for (auto& kv : c->timers) {
+        kv.second.total_ms = 0;
+        kv.second.launches = 0;
+    }
+    const bool mixed = c->set.precision == QLX_PREC_MIXED;
+    const int nsub = c->set.nsplit * c->set.nsplit;
+    const size_t s_pix = (size_t)c->nx * c->ny;
+    std::vector<WaveItem> todo;
+    int first_bad = QLX_OK;
+    for (int b = 0; b < nbatch; b++) {
+        int rc = check_model(c, &models[b]);
+        if (rc) return rc;
+        const int snx = src_n[2 * b], sny = src_n[2 * b + 1];
+        if (snx > QLX_MAX_SRC_AXIS || sny > QLX_MAX_SRC_AXIS) return fail(c, QLX_ERR_RANGE, "source grid larger than 512 per axis");
+        if (snx < 2 || sny < 2) {  // lens.cpp:12453-12458
+            memset(&out[b], 0, sizeof(out[b]));
+            out[b].minus2lnE = 2e30;
+            out[b].status = QLX_ERR_NOSOURCE;
+            continue;
+        }
+        todo.push_back({b, snx, sny});
+    }
+    // admission: items per wave from the budget (ray/row arrays + a dense bound + an Lb estimate)
+    size_t max_cell = 4, max_snx = 2;
+    for (auto& it : todo) {
+        max_cell = std::max(max_cell, (size_t)it.snx * it.sny);
+        max_snx = std::max(max_snx, (size_t)it.snx);
+    }
+    const size_t ncell_bound = std::min<size_t>(max_cell, QLX_MAX_LD - 1);
+    const size_t ldN = round_up(ncell_bound + 1, 64);
+    const size_t lb_est = round_up(std::max<size_t>(s_pix, 64), 64) * std::min<size_t>(ldN, 32 * 64) * (mixed ? 4 : 8);
+    const size_t per_item = stage1_bytes(s_pix, max_cell, nsub) + dense_bytes_bound(ncell_bound, max_snx, mixed) + lb_est;
+    size_t budget = c->hbm_budget;
+    if (budget == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess) budget = (size_t)(0.85 * (double)fr);
+        else budget = (size_t)16 << 30;
+    }
+    int wave = (int)std::max<size_t>(1, std::min<size_t>(budget / std::max<size_t>(per_item, 1), 1024));
+    for (size_t w0 = 0; w0 < todo.size(); w0 += wave) {
+        std::vector<WaveItem> items(todo.begin() + w0, todo.begin() + std::min(todo.size(), w0 + wave));
+        int rc = run_wave(c, models, items, out, w0 + wave >= todo.size());
+        if (rc) return rc;
+    }
+    for (int b = 0; b < nbatch; b++)
+        if (out[b].status != QLX_OK && first_bad == QLX_OK) first_bad = out[b].status;
+    return first_bad;
 }
 
 }  // namespace
@@ -451,14 +655,12 @@ int qlx_ctx_create(const qlx_opts* opts, void** out)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, QLX_ERR_HIP, "no HIP device");
     Ctx* c = new Ctx();
     c->device = opts ? opts->device : 0;
+    c->hbm_budget = opts ? opts->hbm_budget : 0;
     if (c->device < 0 || c->device >= ndev) {
         delete c;
         return fail(nullptr, QLX_ERR_ARG, "invalid device ordinal");
     }
-    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return fail(nullptr, QLX_ERR_HIP, "stream creation failed");
     }
@@ -477,10 +679,6 @@ int qlx_ctx_destroy(void* ctx)
         if (kv.second.a) hipEventDestroy(kv.second.a);
         if (kv.second.b) hipEventDestroy(kv.second.b);
     }
-    hipStreamSynchronize(c->stream2);
-    hipEventDestroy(c->ev_fork);
-    hipEventDestroy(c->ev_join);
-    hipStreamDestroy(c->stream2);
     hipStreamDestroy(c->stream);
     delete c;
     return QLX_OK;
@@ -498,7 +696,7 @@ int qlx_set_image(void* ctx, int nx, int ny, const double* xvals, const double* 
     Ctx* c = (Ctx*)ctx;
     if (!c) return QLX_ERR_ARG;
     if (nx < 1 || ny < 1 || !xvals || !yvals || !data || !covinv) return fail(c, QLX_ERR_ARG, "qlx_set_image: bad arguments");
-    if ((long long)nx * ny > (1LL << 26)) return fail(c, QLX_ERR_RANGE, "image too large");
+    if ((long long)nx * ny > (1LL << 24)) return fail(c, QLX_ERR_RANGE, "image too large");
     hipSetDevice(c->device);
     size_t npix = (size_t)nx * ny;
     c->nx = nx;
@@ -527,6 +725,7 @@ int qlx_set_psf(void* ctx, int pnx, int pny, const double* psf)
     Ctx* c = (Ctx*)ctx;
     if (!c) return QLX_ERR_ARG;
     if (pnx < 1 || pny < 1 || !psf || (pnx % 2) == 0 || (pny % 2) == 0) return fail(c, QLX_ERR_ARG, "PSF must be odd-sized and non-empty");
+    if ((long long)pnx * pny > 4096 * 4096) return fail(c, QLX_ERR_RANGE, "PSF too large");
     hipSetDevice(c->device);
     c->pnx = pnx;
     c->pny = pny;
@@ -553,7 +752,7 @@ int qlx_set_psf_gaussian(void* ctx, double sigx, double sigy, double threshold)
     if ((nxd - nxh) > 0.5) nxh++;
     if ((nyd - nyh) > 0.5) nyh++;
     int nx = 2 * nxh + 1, ny = 2 * nyh + 1;
-    if (nx * ny > 4096 * 4096) return fail(c, QLX_ERR_RANGE, "PSF too large");
+    if ((long long)nx * ny > 4096 * 4096) return fail(c, QLX_ERR_RANGE, "PSF too large");
     double xmax_sup = ((2 * nxh + 1) * xstep - xstep) / 2;
     double ymax_sup = ((2 * nyh + 1) * ystep - ystep) / 2;
     std::vector<double> psf((size_t)nx * ny);
@@ -576,8 +775,9 @@ int qlx_set_settings(void* ctx, const qlx_settings* s)
     if (!s) return fail(c, QLX_ERR_ARG, "null settings");
     if (s->nsplit < 1 || s->nsplit > 4) return fail(c, QLX_ERR_RANGE, "nsplit must be 1..4");
     if (!(s->src_xmax > s->src_xmin) || !(s->src_ymax > s->src_ymin)) return fail(c, QLX_ERR_ARG, "empty source grid bounds");
-    if (s->src_nx > 512 || s->src_ny > 512) return fail(c, QLX_ERR_RANGE, "source grid larger than 512 per axis");
+    if (s->src_nx > QLX_MAX_SRC_AXIS || s->src_ny > QLX_MAX_SRC_AXIS) return fail(c, QLX_ERR_RANGE, "source grid larger than 512 per axis");
     if (s->precision != QLX_PREC_FP64 && s->precision != QLX_PREC_MIXED) return fail(c, QLX_ERR_ARG, "unknown precision mode");
+    if (s->ray_precision != QLX_RAY_FP64 && s->ray_precision != QLX_RAY_FP32) return fail(c, QLX_ERR_ARG, "unknown ray precision");
     if (s->integral_tol <= 0) return fail(c, QLX_ERR_ARG, "integral tolerance must be > 0");
     c->set = *s;
     c->settings_set = true;
@@ -591,26 +791,12 @@ int qlx_eval(void* ctx, int nbatch, const qlx_lens_model* models, qlx_result* ou
     if (nbatch < 0 || (nbatch > 0 && (!models || !out))) return fail(c, QLX_ERR_ARG, "qlx_eval: bad arguments");
     if (!c->image_set || !c->settings_set || !c->psf_set) return fail(c, QLX_ERR_STATE, "image, PSF and settings must be set first");
     hipSetDevice(c->device);
-    for (auto& kv : c->timers) {
-        kv.second.total_ms = 0;
-        kv.second.launches = 0;
-    }
-    int first_bad = QLX_OK;
+    std::vector<int> srcn(2 * (size_t)nbatch);
     for (int b = 0; b < nbatch; b++) {
-        int rc = check_model(c, &models[b]);
-        if (rc) return rc;
-        qlx_result* r = &out[b];
-        if (c->set.src_nx < 2 || c->set.src_ny < 2) {  // lens.cpp:12453-12458
-            memset(r, 0, sizeof(*r));
-            r->minus2lnE = 2e30;
-            r->status = QLX_ERR_NOSOURCE;
-        } else {
-            rc = eval_one_fp64(c, models[b], c->set.src_nx, c->set.src_ny, r);
-            if (rc) return rc;
-        }
-        if (r->status != QLX_OK && first_bad == QLX_OK) first_bad = r->status;
+        srcn[2 * b] = c->set.src_nx;
+        srcn[2 * b + 1] = c->set.src_ny;
     }
-    return first_bad;
+    return eval_items(c, nbatch, models, srcn.data(), out);
 }
 
 int qlx_eval_batch_grids(void* ctx, int nbatch, const qlx_lens_model* models, const int* src_n, qlx_result* out)
@@ -620,24 +806,7 @@ int qlx_eval_batch_grids(void* ctx, int nbatch, const qlx_lens_model* models, co
     if (nbatch < 0 || (nbatch > 0 && (!models || !out || !src_n))) return fail(c, QLX_ERR_ARG, "bad arguments");
     if (!c->image_set || !c->settings_set || !c->psf_set) return fail(c, QLX_ERR_STATE, "image, PSF and settings must be set first");
     hipSetDevice(c->device);
-    int first_bad = QLX_OK;
-    for (int b = 0; b < nbatch; b++) {
-        int rc = check_model(c, &models[b]);
-        if (rc) return rc;
-        int snx = src_n[2 * b], sny = src_n[2 * b + 1];
-        if (snx > 512 || sny > 512) return fail(c, QLX_ERR_RANGE, "source grid larger than 512 per axis");
-        qlx_result* r = &out[b];
-        if (snx < 2 || sny < 2) {
-            memset(r, 0, sizeof(*r));
-            r->minus2lnE = 2e30;
-            r->status = QLX_ERR_NOSOURCE;
-        } else {
-            rc = eval_one_fp64(c, models[b], snx, sny, r);
-            if (rc) return rc;
-        }
-        if (r->status != QLX_OK && first_bad == QLX_OK) first_bad = r->status;
-    }
-    return first_bad;
+    return eval_items(c, nbatch, models, src_n, out);
 }
 
 int qlx_get_solution(void* ctx, double* src_cells, double* model_image)
@@ -772,18 +941,12 @@ int qlx_mock_image(void* ctx, const qlx_lens_model* model, int nsrc, const doubl
     }
     ModelPrep mp;
     prep_model(*model, mp);
-    qlx_settings saved = c->set;
-    c->set.nsplit = nsplit;
-    GridDev g = make_grid(c, 2, 2);
-    c->set = saved;
-    g.nsplit = nsplit;
-    g.nsub = nsplit * nsplit;
-    g.mask = nullptr;
     size_t npix = (size_t)c->nx * c->ny;
     DevBuf dsp, dsb, dimg;
     if (!dsp.ensure(sp.size() * 8) || !dsb.ensure(npix * 8) || !dimg.ensure(npix * 8)) return fail(c, QLX_ERR_NOMEM, "allocation failed");
     HIPCHK(c, hipMemcpyAsync(dsp.p, sp.data(), sp.size() * 8, hipMemcpyHostToDevice, c->stream));
-    launch_mock_sb(c->stream, g, mp, tol, nsrc, dsp.as<double>(), dsb.as<double>());
+    launch_mock_sb(c->stream, c->nx, c->ny, nsplit, c->xvals.as<double>(), c->yvals.as<double>(), mp, tol, nsrc, dsp.as<double>(),
+                   dsb.as<double>());
     launch_conv_image(c->stream, c->nx, c->ny, c->psf.as<double>(), c->pnx, c->pny, dsb.as<double>(), dimg.as<double>());
     HIPCHK(c, hipMemcpyAsync(img_out, dimg.p, npix * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -818,15 +981,15 @@ int qlx_get_stats(void* ctx, qlx_stats* out)
     return QLX_OK;
 }
 
-int qlx_selftest_mfma(void* ctx, double* out256)
+int qlx_selftest_mfma(void* ctx, int kind, double* out256)
 {
     Ctx* c = (Ctx*)ctx;
-    if (!c || !out256) return QLX_ERR_ARG;
+    if (!c || !out256 || kind < 0 || kind > 2) return QLX_ERR_ARG;
     hipSetDevice(c->device);
     DevBuf d;
-    if (!d.ensure(256 * 8)) return QLX_ERR_NOMEM;
+    if (!d.ensure(768 * 8)) return QLX_ERR_NOMEM;
     launch_selftest_mfma(c->stream, d.as<double>());
-    HIPCHK(c, hipMemcpyAsync(out256, d.p, 256 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out256, d.as<double>() + 256 * kind, 256 * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return QLX_OK;
 }

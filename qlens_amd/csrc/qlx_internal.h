@@ -1,5 +1,10 @@
 // Internal declarations shared by the qlx host orchestration (qlx_api.cpp) and the HIP kernel
 // translation units.  Not part of the public ABI (include/qlx.h).
+//
+// Every evaluation kernel works on a batch of items (lens-model proposals): the item index is
+// blockIdx.y (blockIdx.y = matrix index for the Cholesky kernels), per-item work arrays sit at
+// fixed strides, and per-item sizes / pool offsets come from a device array of ItemMeta filled by
+// the host once the active sets are known.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -8,6 +13,18 @@
 #include "../../include/qlx.h"
 
 namespace qlx {
+
+#define QLX_MAX_SRC_AXIS 512
+#define QLX_MAX_LD 16384          // padded source dimension limit of the dense F (n_src <= 16383)
+#define QLX_MAX_CBW 8             // 32-bit support-mask words per 64-row chunk (QLX_MAX_LD / 64 / 32)
+// per-item source grid block: sxv[513], syv[513], scx[512], scy[512]
+#define QLX_SG_STRIDE 2052
+#define QLX_SG_SYV 513
+#define QLX_SG_SCX 1026
+#define QLX_SG_SCY 1538
+// per-item output slots
+#define QLX_OUT_STRIDE 8          // -2lnE, chisq, logdetF, logdetR, Es, regterms, refine_resid, refine_steps
+#define QLX_VEC_SLOTS 6           // per item, ldN doubles each: s, diagF, diagR, Fdiag, resid, partial sums
 
 // Per-lens meta-parameters, computed on the host with the host libm exactly as the reference's
 // update_meta_parameters does, so the device evaluates only per-ray arithmetic.
@@ -25,117 +42,117 @@ struct LensPrep {
 
 struct ModelPrep {
     int nlens;
+    int sur_lens;              // index of the lens whose deflection comes from sur_alpha, or -1
     LensPrep lens[QLX_MAX_LENSES];
 };
 
 void prep_model(const qlx_lens_model& in, ModelPrep& out);
 
-// ---- image / source grid geometry (device pointers) ----
-struct GridDev {
+// sizes and pool offsets of one item
+struct ItemMeta {
+    int valid;                       // item takes part in the dense stages
+    int n_src, n_img, ldN, nrows;    // active counts; padded source dim (>= n_src+1); padded rows
+    int ncb, ncbw, nkt;              // column blocks, mask words per chunk, 64-row chunks
+    int snx, sny, bwb, nblk;         // source grid; R band in blocks; Lb support blocks
+    int nsup_max, pad0;              // most support blocks of one chunk
+    long long lb_off;                // Lb pool (elements: double, or 2 x nblk x 4096 bf16)
+    long long f_off;                 // F (elements of F or F32)
+    long long r_off;                 // banded R tiles (doubles)
+    long long cvec_off, vec_off, nz_off, bb_off, w_off;
+};
+
+struct BatchDev {
+    int B;
+    // image (shared by all items)
     int nx, ny, nsplit, nsub;
-    const double* xvals;   // nx+1
-    const double* yvals;   // ny+1
-    const unsigned char* mask;  // nx*ny
-    int snx, sny;
-    const double* sxv;     // snx+1 accumulated source corners
-    const double* syv;     // sny+1
-    const double* scx;     // snx centres
-    const double* scy;     // sny centres
+    const double* xvals;
+    const double* yvals;
+    const unsigned char* mask;  // nullptr = all pixels
+    const double* data;
+    const double* covinv;
+    const double* psf;
+    int pnx, pny;
+    double tol, lambda;
+    // per-item inputs
+    const ModelPrep* models;
+    const double* sgrid;        // QLX_SG_STRIDE doubles per item
+    const int* src_n;           // 2 ints per item (snx, sny)
+    const float* sur_alpha;     // surrogate deflections (2 floats per ray and item), or null
+    // per-item work arrays (strides from s_pix, s_cell)
+    double* beta;               // s_pix*nsub*2
+    int* map_cell;              // s_pix*nsub*3
+    double* map_w;              // s_pix*nsub*3
+    int* pmapped;               // s_pix
+    int* cmapped;               // s_cell
+    int* sidx;                  // s_cell
+    int* pidx;                  // s_pix
+    int* act_pix;               // s_pix
+    int* counts;                // 4 per item: n_src, n_img, support blocks, largest chunk support
+    int* ucol;                  // s_pix*3*nsub
+    double* uval;               // s_pix*3*nsub
+    int* ucnt;                  // s_pix
+    int* key;                   // s_pix
+    int* bcnt;                  // s_cell+2
+    int* boff;                  // s_cell+2
+    int* bfill;                 // s_cell+2
+    int* perm;                  // s_pix
+    unsigned long long* nevals; // 1 per item
+    long long s_pix, s_cell;
+    // pools and outputs
+    const ItemMeta* meta;
+    void* lb;                   // blurred L support blocks (double, or bf16 hi/lo when mixed)
+    double* F;                  // FP64 F (parity mode)
+    float* F32;                 // FP32 F (mixed mode)
+    double* R;                  // banded R tiles
+    double* cvec;
+    double* vec;                // QLX_VEC_SLOTS * ldN doubles per item
+    unsigned* nz;               // nkt * ncbw support words per item
+    int* bbase;                 // nkt + 1 slot bases per item
+    double* winv;               // per item 2 * ldN*64 doubles (F then R block inverses)
+    double* model;              // per item 2*s_pix
+    double* out;                // QLX_OUT_STRIDE per item
+    int* status;                // per item 4
 };
 
-// per-item work buffers (device)
-struct ItemBufs {
-    double* beta;        // nx*ny*nsub*2
-    int* map_cell;       // nx*ny*nsub*3
-    double* map_w;       // nx*ny*nsub*3
-    int* pmapped;        // nx*ny
-    int* cmapped;        // snx*sny
-    int* sidx;           // snx*sny active index or -1
-    int* pidx;           // nx*ny active row or -1
-    int* act_pix;        // active rows -> pixel id
-    int* counts;         // [0] n_src, [1] n_img, [2] status flags
-    int* ucol;           // n_img*12 merged L columns
-    double* uval;        // n_img*12
-    int* ucnt;           // n_img
-    long long* nevals;   // integrand evaluation counter
-    int* key;            // n_img: source cell (active index) of each active pixel's first ray
-    int* bucket_cnt;     // n_src+1
-    int* bucket_off;     // n_src+1
-    int* bucket_fill;    // n_src+1
-    int* perm;           // n_img: sorted row -> active row
+// one matrix of a batched Cholesky (lower, row-major tiles of 64x64)
+//   bwb == 0: dense, tile (I,J) at A + I*64*ld + J*64, row stride ld
+//   bwb  > 0: banded, tile (I,J) (I-J <= bwb) at A + (I*(bwb+1) + I-J)*4096, row stride 64
+struct MatDesc {
+    void* A;
+    int ld, n, nb, f32, bwb, pad;
+    void* diag;                 // nb*64 pivots (element type of A)
+    int* status;                // bit 0 set on a negative pivot of a row < n
+    void* W;                    // nb block inverses (64x64 each, element type of A)
 };
 
-// ---- kernel launchers (all asynchronous on `stream`) ----
-void launch_raytrace_map(hipStream_t stream, const GridDev& g, const ModelPrep& m, double tol, int ray_fp32,
-                         const ItemBufs& b, const void* surrogate);
-void launch_fill_i32(hipStream_t stream, int* p, int v, size_t n);
-void launch_fill_f64(hipStream_t stream, double* p, double v, size_t n);
-void launch_compact(hipStream_t stream, const GridDev& g, const ItemBufs& b);
-void launch_merge_lrows(hipStream_t stream, const GridDev& g, const ItemBufs& b, int n_img);
-void launch_build_lb(hipStream_t stream, const GridDev& g, const ItemBufs& b, int n_img, int n_src,
-                     const double* psf, int pnx, int pny, const double* data, double* Lb, int ldL,
-                     unsigned int* nzmask, int ncb);
-void launch_build_lb_bf16(hipStream_t stream, const GridDev& g, const ItemBufs& b, int n_img, int n_src,
-                          const double* psf, int pnx, int pny, const double* data, const double* covinv,
-                          uint16_t* Lb16, int ldL, unsigned int* nzmask, int ncb, float* dcol);
-void launch_syrk_f64(hipStream_t stream, const double* Lb, int ldL, const double* cvec, int nrows, int ncols,
-                     double* F, int ldF, const unsigned int* nzmask, int ncb);
-void launch_syrk_bf16(hipStream_t stream, const uint16_t* Lb16, int ldL, int nrows, int ncols, float* F32,
-                      int ldF, const unsigned int* nzmask, int ncb);
-void launch_gather_cvec(hipStream_t stream, const ItemBufs& b, const double* covinv, const double* data,
-                        int n_img, double* cvec, double* dvec);
-void launch_add_curvature(hipStream_t stream, const GridDev& g, const ItemBufs& b, int n_src, double lambda,
-                          double* F, int ldF, double* R, int ldR);
-void launch_add_curvature_f32(hipStream_t stream, const GridDev& g, const ItemBufs& b, int n_src, double lambda,
-                              float* F, int ldF);
-// Cholesky (lower, in place, row-major ld) of an n x n matrix padded to a multiple of 64.
-int cholesky_f64(hipStream_t stream, double* A, int ld, int n, double* diag, int* status, double* work);
-int cholesky_f32(hipStream_t stream, float* A, int ld, int n, float* diag, int* status, float* work);
-void launch_trisolve_f64(hipStream_t stream, const double* L, int ld, int n, const double* b, double* x);
-void launch_trisolve_f32(hipStream_t stream, const float* L, int ld, int n, const float* b, float* x);
-void launch_logdet(hipStream_t stream, const double* diag, int n, double* out);
-void launch_logdet_f32diag(hipStream_t stream, const float* diag, int n, double* out);
-// model image, chi-square, regularization terms, evidence
-void launch_evidence(hipStream_t stream, const GridDev& g, const ItemBufs& b, int n_img, int n_src,
-                     const double* s, const double* psf, int pnx, int pny, const double* data,
-                     const double* covinv, double lambda, const double* logdets, double* model, double* out);
-// mixed precision: r = D - (F s) with F = symmetric lower (FP64 F built from FP32 pieces)
-void launch_symv_resid(hipStream_t stream, const double* F, int ldF, int n, const double* s, const double* D,
-                       double* r);
-void launch_symv_resid_f32F(hipStream_t stream, const float* F, int ldF, int n, const double* s,
-                            const double* D, double* r);
-void launch_axpy_f64(hipStream_t stream, int n, const double* dx, double* x);
-void launch_norm2_pair(hipStream_t stream, int n, const double* a, const double* b, double* out2);
-void launch_f64_to_f32(hipStream_t stream, const double* a, float* b, int n);
-void launch_f32_to_f64(hipStream_t stream, const float* a, double* b, int n);
-void launch_extract_col_f32(hipStream_t stream, const float* F, int ldF, int col, int n, double* out);
-void launch_extract_col_f64(hipStream_t stream, const double* F, int ldF, int col, int n, double* out);
+void upload_patterson(int device);
 
-// mock image
-void launch_mock_sb(hipStream_t stream, const GridDev& g, const ModelPrep& m, double tol, int nsrc,
-                    const double* sersic, double* sb);
-void launch_conv_image(hipStream_t stream, int nx, int ny, const double* psf, int pnx, int pny,
-                       const double* in, double* out);
-void launch_raytrace_points(hipStream_t stream, const ModelPrep& m, double tol, int n, const double* xy,
-                            double* beta);
-void launch_nfw_def_hess(hipStream_t stream, const LensPrep& L, double tol, int n, const double* xy,
-                         double* out5);
+// ---- raytrace.hip ----
+void launch_raytrace_map_b(hipStream_t s, const BatchDev& bd, int ray_fp32);
+void launch_raytrace_points(hipStream_t stream, const ModelPrep& m, double tol, int n, const double* xy, double* beta);
+void launch_nfw_def_hess(hipStream_t stream, const LensPrep& L, double tol, int n, const double* xy, double* out5);
+void launch_mock_sb(hipStream_t stream, int nx, int ny, int nsplit, const double* xvals, const double* yvals, const ModelPrep& m,
+                    double tol, int nsrc, const double* sersic, double* sb);
+void launch_conv_image(hipStream_t stream, int nx, int ny, const double* psf, int pnx, int pny, const double* in, double* out);
 
-// block-sparse blurred L and SYRK (blocksparse.hip)
-void launch_sort_rows(hipStream_t s, const GridDev& g, const ItemBufs& b, int n_img, int n_src);
-void launch_row_support(hipStream_t s, const GridDev& g, const ItemBufs& b, int n_img, int n_src, int pnx, int pny, unsigned* nzmask,
-                        int ncbw);
-void launch_build_lb_blocks_f64(hipStream_t s, const GridDev& g, const ItemBufs& b, int n_img, int nrows, int n_src, const double* psf,
-                                int pnx, int pny, const double* data, double* pool, int ncb, const unsigned* nzmask, int ncbw);
-void launch_build_lb_blocks_bf16(hipStream_t s, const GridDev& g, const ItemBufs& b, int n_img, int nrows, int n_src, const double* psf,
-                                 int pnx, int pny, const double* data, const double* covinv, uint16_t* pool, int ncb,
-                                 const unsigned* nzmask, int ncbw);
-void launch_gather_cvec_sorted(hipStream_t s, const ItemBufs& b, const double* covinv, int n_img, int nrows, double* cvec);
-void launch_syrk_blocks_f64(hipStream_t s, const double* pool, int ncb, int nkt, const double* cvec, double* F, int ldF,
-                            const unsigned* nz, int ncbw);
-
-// surrogate (MLP) kernels
-struct MlpDev;
-int surrogate_kernels_forward(hipStream_t stream, const MlpDev* mlp, int n, const float* x, float* y);
+// ---- batch.hip ----
+void launch_fill_i32(hipStream_t s, int* p, int v, size_t n);
+void launch_fill_f64(hipStream_t s, double* p, double v, size_t n);
+void launch_compact_b(hipStream_t s, const BatchDev& bd);
+void launch_merge_lrows_b(hipStream_t s, const BatchDev& bd, int max_img);
+void launch_sort_rows_b(hipStream_t s, const BatchDev& bd, int max_img, int max_src);
+void launch_row_support_b(hipStream_t s, const BatchDev& bd, int max_nkt);
+void launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_nsup, int bf16);
+void launch_gather_cvec_b(hipStream_t s, const BatchDev& bd, int max_rows);
+void launch_syrk_f64_b(hipStream_t s, const BatchDev& bd, int max_ncb);
+void launch_syrk_bf16_b(hipStream_t s, const BatchDev& bd, int max_ncb);
+void launch_curvature_b(hipStream_t s, const BatchDev& bd, int max_cells, int max_rtiles);
+void launch_prep_fr_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32);
+void cholesky_batched(hipStream_t s, const MatDesc* d_mats, int nmat, int max_nb);
+void launch_logdets_b(hipStream_t s, const BatchDev& bd, int f32);
+void launch_trisolve_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32);
+void launch_refine_b(hipStream_t s, const BatchDev& bd, int max_ld, int iters, double tol);
+void launch_evidence_b(hipStream_t s, const BatchDev& bd, int max_img);
+void launch_selftest_mfma(hipStream_t s, double* out);
 
 }  // namespace qlx

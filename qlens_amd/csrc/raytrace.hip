@@ -270,73 +270,93 @@ __device__ __forceinline__ int upper_index(const double* v, int n, double x)
     return lo;
 }
 
+// One thread per (item, ray): item = blockIdx.y.  A lens listed as sur_lens takes its deflection
+// from sur_alpha (image-frame FP32 values written by the surrogate kernel for the same rays).
 template <typename T>
-__global__ void __launch_bounds__(256) k_raytrace_map(GridDev g, ModelPrep m, T tol, ItemBufs b)
+__global__ void __launch_bounds__(256) k_raytrace_map_b(BatchDev bd, T tol)
 {
-    const int nsub = g.nsub;
-    const long long total = (long long)g.nx * g.ny * nsub;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
+    const int item = blockIdx.y;
+    const int nsub = bd.nsub;
+    const long long nray = (long long)bd.nx * bd.ny * nsub;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nray) return;
+    const ModelPrep& m = bd.models[item];
+    const double* sg = bd.sgrid + (size_t)item * QLX_SG_STRIDE;
+    const double* sxv = sg;
+    const double* syv = sg + QLX_SG_SYV;
+    const double* scx = sg + QLX_SG_SCX;
+    const double* scy = sg + QLX_SG_SCY;
+    const int snx = bd.src_n[2 * item], sny = bd.src_n[2 * item + 1];
+    const size_t ro = (size_t)item * bd.s_pix * nsub + t;  // ray slot of this item
     const int k = (int)(t % nsub);
     const long long p = t / nsub;  // pixel id i*ny + j
-    const int i = (int)(p / g.ny), j = (int)(p % g.ny);
-    int* mc = b.map_cell + t * 3;
-    double* mw = b.map_w + t * 3;
-    if (g.mask && !g.mask[p]) {
+    const int i = (int)(p / bd.ny), j = (int)(p % bd.ny);
+    int* mc = bd.map_cell + ro * 3;
+    double* mw = bd.map_w + ro * 3;
+    if (bd.mask && !bd.mask[p]) {
         mc[0] = mc[1] = mc[2] = -1;
-        b.beta[t * 2] = 0;
-        b.beta[t * 2 + 1] = 0;
+        bd.beta[ro * 2] = 0;
+        bd.beta[ro * 2 + 1] = 0;
         return;
     }
-    const int ii = k / g.nsplit, jj = k % g.nsplit;
-    const double u0 = ((double)(1 + 2 * ii)) / (2 * g.nsplit);
-    const double w0 = ((double)(1 + 2 * jj)) / (2 * g.nsplit);
-    const double sx = (1 - u0) * g.xvals[i] + u0 * g.xvals[i + 1];
-    const double sy = (1 - w0) * g.yvals[j] + w0 * g.yvals[j + 1];
-    T a0, a1;
+    const int ii = k / bd.nsplit, jj = k % bd.nsplit;
+    const double u0 = ((double)(1 + 2 * ii)) / (2 * bd.nsplit);
+    const double w0 = ((double)(1 + 2 * jj)) / (2 * bd.nsplit);
+    const double sx = (1 - u0) * bd.xvals[i] + u0 * bd.xvals[i + 1];
+    const double sy = (1 - w0) * bd.yvals[j] + w0 * bd.yvals[j + 1];
+    T a0 = 0, a1 = 0;
     int nev = 0;
-    total_deflection<T>(m, (T)sx, (T)sy, tol, a0, a1, nev);
+    for (int l = 0; l < m.nlens; l++) {
+        T d0, d1;
+        if (l == m.sur_lens && bd.sur_alpha) {
+            d0 = (T)bd.sur_alpha[ro * 2];
+            d1 = (T)bd.sur_alpha[ro * 2 + 1];
+        } else {
+            lens_deflection<T>(m.lens[l], (T)sx, (T)sy, tol, d0, d1, nev);
+        }
+        a0 += d0;
+        a1 += d1;
+    }
     const double bx = (double)((T)sx - a0), by = (double)((T)sy - a1);
-    b.beta[t * 2] = bx;
-    b.beta[t * 2 + 1] = by;
-    if (nev) atomicAdd((unsigned long long*)b.nevals, (unsigned long long)nev);
+    bd.beta[ro * 2] = bx;
+    bd.beta[ro * 2 + 1] = by;
+    if (nev) atomicAdd(bd.nevals + item, (unsigned long long)nev);
 
-    const int snx = g.snx, sny = g.sny;
-    if (!(bx >= g.sxv[0] && bx < g.sxv[snx] && by >= g.syv[0] && by < g.syv[sny])) {
+    if (!(bx >= sxv[0] && bx < sxv[snx] && by >= syv[0] && by < syv[sny])) {
         mc[0] = mc[1] = mc[2] = -1;
         return;
     }
-    const int ci = upper_index(g.sxv, snx, bx);
-    const int cj = upper_index(g.syv, sny, by);
-    const int xi = (((bx > g.scx[ci]) && (ci < snx - 1)) || (ci == 0)) ? ci + 1 : ci - 1;
-    const int yj = (((by > g.scy[cj]) && (cj < sny - 1)) || (cj == 0)) ? cj + 1 : cj - 1;
+    const int ci = upper_index(sxv, snx, bx);
+    const int cj = upper_index(syv, sny, by);
+    const int xi = (((bx > scx[ci]) && (ci < snx - 1)) || (ci == 0)) ? ci + 1 : ci - 1;
+    const int yj = (((by > scy[cj]) && (cj < sny - 1)) || (cj == 0)) ? cj + 1 : cj - 1;
     const int c0 = ci * sny + cj, c1 = xi * sny + cj, c2 = ci * sny + yj;
     mc[0] = c0;
     mc[1] = c1;
     mc[2] = c2;
     const double weight = 1.0 / nsub;
-    const double p0x = g.scx[ci], p0y = g.scy[cj];
-    const double p1x = g.scx[xi], p1y = g.scy[cj];
-    const double p2x = g.scx[ci], p2y = g.scy[yj];
+    const double p0x = scx[ci], p0y = scy[cj];
+    const double p1x = scx[xi], p1y = scy[cj];
+    const double p2x = scx[ci], p2y = scy[yj];
     const double d = (p0x - p1x) * (p1y - p2y) - (p1x - p2x) * (p0y - p1y);
     mw[0] = weight * (bx * (p1y - p2y) + by * (p2x - p1x) + p1x * p2y - p1y * p2x) / d;
     mw[1] = weight * (bx * (p2y - p0y) + by * (p0x - p2x) + p0y * p2x - p0x * p2y) / d;
     mw[2] = weight * (bx * (p0y - p1y) + by * (p1x - p0x) + p0x * p1y - p0y * p1x) / d;
-    b.cmapped[c0] = 1;
-    b.cmapped[c1] = 1;
-    b.cmapped[c2] = 1;
-    b.pmapped[p] = 1;
+    int* cm = bd.cmapped + (size_t)item * bd.s_cell;
+    cm[c0] = 1;
+    cm[c1] = 1;
+    cm[c2] = 1;
+    bd.pmapped[(size_t)item * bd.s_pix + p] = 1;
 }
 
-void launch_raytrace_map(hipStream_t stream, const GridDev& g, const ModelPrep& m, double tol, int ray_fp32,
-                         const ItemBufs& b, const void* /*surrogate*/)
+void launch_raytrace_map_b(hipStream_t stream, const BatchDev& bd, int ray_fp32)
 {
-    long long total = (long long)g.nx * g.ny * g.nsub;
-    int blocks = (int)((total + 255) / 256);
+    const long long total = (long long)bd.nx * bd.ny * bd.nsub;
+    const unsigned blocks = (unsigned)((total + 255) / 256);
     if (ray_fp32)
-        hipLaunchKernelGGL(k_raytrace_map<float>, dim3(blocks), dim3(256), 0, stream, g, m, (float)tol, b);
+        hipLaunchKernelGGL(k_raytrace_map_b<float>, dim3(blocks, bd.B), dim3(256), 0, stream, bd, (float)bd.tol);
     else
-        hipLaunchKernelGGL(k_raytrace_map<double>, dim3(blocks), dim3(256), 0, stream, g, m, tol, b);
+        hipLaunchKernelGGL(k_raytrace_map_b<double>, dim3(blocks, bd.B), dim3(256), 0, stream, bd, bd.tol);
 }
 
 __global__ void __launch_bounds__(256) k_raytrace_points(ModelPrep m, double tol, int n, const double* xy, double* beta)
@@ -381,18 +401,19 @@ void launch_nfw_def_hess(hipStream_t stream, const LensPrep& L, double tol, int 
 
 // ---- mock surface brightness (find_surface_brightness split branch, pixelgrid.cpp:11835-11896;
 // SB_Profile::surface_brightness + Sersic::sb_rsq, sbprofile.cpp:1346-1366, 2472-2516) ----
-__global__ void __launch_bounds__(256) k_mock_sb(GridDev g, ModelPrep m, double tol, int nsrc, const double* sersic, double* sb)
+__global__ void __launch_bounds__(256) k_mock_sb(int nx, int ny, int nsplit, const double* xvals, const double* yvals, ModelPrep m,
+                                                 double tol, int nsrc, const double* sersic, double* sb)
 {
     long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= (long long)g.nx * g.ny) return;
-    const int i = (int)(p / g.ny), j = (int)(p % g.ny);
+    if (p >= (long long)nx * ny) return;
+    const int i = (int)(p / ny), j = (int)(p % ny);
     double sbtot = 0;
-    for (int ii = 0; ii < g.nsplit; ii++)
-        for (int jj = 0; jj < g.nsplit; jj++) {
-            const double u0 = ((double)(1 + 2 * ii)) / (2 * g.nsplit);
-            const double w0 = ((double)(1 + 2 * jj)) / (2 * g.nsplit);
-            const double sx = (1 - u0) * g.xvals[i] + u0 * g.xvals[i + 1];
-            const double sy = (1 - w0) * g.yvals[j] + w0 * g.yvals[j + 1];
+    for (int ii = 0; ii < nsplit; ii++)
+        for (int jj = 0; jj < nsplit; jj++) {
+            const double u0 = ((double)(1 + 2 * ii)) / (2 * nsplit);
+            const double w0 = ((double)(1 + 2 * jj)) / (2 * nsplit);
+            const double sx = (1 - u0) * xvals[i] + u0 * xvals[i + 1];
+            const double sy = (1 - w0) * yvals[j] + w0 * yvals[j + 1];
             double a0, a1;
             int nev = 0;
             total_deflection<double>(m, sx, sy, tol, a0, a1, nev);
@@ -412,13 +433,15 @@ __global__ void __launch_bounds__(256) k_mock_sb(GridDev g, ModelPrep m, double 
                 if (v != 0) sbtot += v;
             }
         }
-    sb[p] = 0 + sbtot / (g.nsplit * g.nsplit);
+    sb[p] = 0 + sbtot / (nsplit * nsplit);
 }
 
-void launch_mock_sb(hipStream_t stream, const GridDev& g, const ModelPrep& m, double tol, int nsrc, const double* sersic, double* sb)
+void launch_mock_sb(hipStream_t stream, int nx, int ny, int nsplit, const double* xvals, const double* yvals, const ModelPrep& m,
+                    double tol, int nsrc, const double* sersic, double* sb)
 {
-    long long n = (long long)g.nx * g.ny;
-    hipLaunchKernelGGL(k_mock_sb, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, g, m, tol, nsrc, sersic, sb);
+    long long n = (long long)nx * ny;
+    hipLaunchKernelGGL(k_mock_sb, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, nx, ny, nsplit, xvals, yvals, m, tol, nsrc,
+                       sersic, sb);
 }
 
 // PSF_convolution_pixel_vector direct branch over every pixel (pixelgrid.cpp:14452-14490)

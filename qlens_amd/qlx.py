@@ -226,20 +226,22 @@ class PixelInversion:
         rc = self.lib.qlx_kernel_time(self.ctx, name.encode(), C.byref(ms), C.byref(n))
         return (ms.value, n.value) if rc == 0 else (None, 0)
 
-    def selftest_mfma(self):
+    def selftest_mfma(self, kind=0):
+        """kind 0: f64 16x16x4, 1: f32 16x16x4, 2: bf16 16x16x32 (see include/qlx.h)."""
         out = np.zeros(256)
-        self._check(self.lib.qlx_selftest_mfma(self.ctx, _dp(out)), "qlx_selftest_mfma")
+        self._check(self.lib.qlx_selftest_mfma(self.ctx, int(kind), _dp(out)), "qlx_selftest_mfma")
         return out
 
 
 class QlxStats(C.Structure):
     _fields_ = [("syrk_dense_flops", C.c_double), ("syrk_exec_flops", C.c_double), ("syrk_alg_flops", C.c_double),
                 ("lb_bytes", C.c_double), ("integrand_evals", C.c_double), ("chol_flops", C.c_double),
-                ("n_img", C.c_int), ("n_src", C.c_int), ("ld_src", C.c_int), ("rows_pad", C.c_int)]
+                ("n_items", C.c_int), ("n_img", C.c_int), ("n_src", C.c_int), ("ld_src", C.c_int),
+                ("rows_pad", C.c_int)]
 
 
 def get_stats(inv):
-    """Work counters of the last evaluated item (qlx_get_stats)."""
+    """Work counters of the last wave of the last evaluation (qlx_get_stats)."""
     s = QlxStats()
     inv.lib.qlx_get_stats.restype = C.c_int
     inv._check(inv.lib.qlx_get_stats(inv.ctx, C.byref(s)), "qlx_get_stats")

@@ -18,16 +18,37 @@ def rel(a, b):
     return abs(a - b) / max(abs(b), 1e-300)
 
 
-def test_mfma_f64_accumulator_layout(qlx_ctx):
-    out = qlx_ctx.selftest_mfma()
-    A = np.array([[i + 1 + 0.01 * k for k in range(4)] for i in range(16)])
-    B = np.array([[1 + k + 0.1 * j for j in range(16)] for k in range(4)])
-    D = A @ B
+def _mfma_got(out, row_of):
     got = np.zeros((16, 16))
     for lane in range(64):
         for v in range(4):
-            got[lane // 16 + 4 * v, lane % 16] = out[lane * 4 + v]
-    assert np.allclose(got, D, rtol=1e-14), "row = lane/16 + 4*reg, col = lane%16 layout does not hold"
+            got[row_of(lane, v), lane % 16] = out[lane * 4 + v]
+    return got
+
+
+def test_mfma_f64_accumulator_layout(qlx_ctx):
+    out = qlx_ctx.selftest_mfma(0)
+    A = np.array([[i + 1 + 0.01 * k for k in range(4)] for i in range(16)])
+    B = np.array([[1 + k + 0.1 * j for j in range(16)] for k in range(4)])
+    got = _mfma_got(out, lambda l, v: l // 16 + 4 * v)
+    assert np.allclose(got, A @ B, rtol=1e-14), "row = lane/16 + 4*reg, col = lane%16 layout does not hold"
+
+
+def test_mfma_f32_accumulator_layout(qlx_ctx):
+    out = qlx_ctx.selftest_mfma(1)
+    A = np.array([[np.float32(i + 1 + np.float32(0.01) * k) for k in range(4)] for i in range(16)], dtype=np.float64)
+    B = np.array([[np.float32(1 + k + np.float32(0.1) * j) for j in range(16)] for k in range(4)], dtype=np.float64)
+    got = _mfma_got(out, lambda l, v: 4 * (l // 16) + v)
+    assert np.allclose(got, A @ B, rtol=1e-6), "row = 4*(lane/16) + reg, col = lane%16 layout does not hold"
+
+
+def test_mfma_bf16_operand_and_accumulator_layout(qlx_ctx):
+    out = qlx_ctx.selftest_mfma(2)
+    A = np.array([[(i + 1) * (k % 4 + 1) / 4 for k in range(32)] for i in range(16)])
+    B = np.array([[(j + 1) * ((k // 4) % 8 + 1) / 8 for j in range(16)] for k in range(32)])
+    got = _mfma_got(out, lambda l, v: 4 * (l // 16) + v)
+    # every operand value is exact in bf16 and every partial sum exact in fp32
+    assert np.array_equal(got, A @ B), "bf16 16x16x32 operand / accumulator layout does not hold"
 
 
 def test_raytrace_matches_oracle(qlx_ctx, golden_dir):
@@ -113,3 +134,70 @@ def test_sentinels(qlx_ctx, golden_dir):
     qlx_ctx.load_workload(case, g["data"])
     r = qlx_ctx.invert_image_surface_brightness_map(case["lenses"], src_n=[(1, 30)])[0]
     assert r["minus2lnE"] == 2e30
+
+
+def _perturbed(case, n, seed):
+    rng = np.random.default_rng(seed)
+    models = []
+    for _ in range(n):
+        m = []
+        for t, p in case["lenses"]:
+            p = list(p)
+            for i in {1: (0, 1), 2: (0, 1, 2), 3: (0,)}[t]:
+                p[i] *= 1 + 0.01 * rng.standard_normal()
+            if t in (1, 2):
+                k = 1 if t == 1 else 2
+                p[k] = min(p[k], 0.999)
+            m.append((t, tuple(p)))
+        models.append(m)
+    return models
+
+
+def test_batch_equals_single_bitwise(qlx_ctx, golden_dir):
+    """Items of one batch share launches but not arithmetic: each equals its own solo evaluation."""
+    g = np.load(os.path.join(golden_dir, "inversion_C2s.npz"))
+    case = W.c2_small()
+    qlx_ctx.load_workload(case, g["data"])
+    models = _perturbed(case, 6, 5)
+    batch = qlx_ctx.invert_image_surface_brightness_map(models)
+    for m, rb in zip(models, batch):
+        rs = qlx_ctx.invert_image_surface_brightness_map(m)[0]
+        assert rb["minus2lnE"] == rs["minus2lnE"] and rb["logdetR"] == rs["logdetR"]
+
+
+def test_uneven_source_grids_in_one_batch(qlx_ctx, golden_dir):
+    """One submission with per-item source grids (and a too-small grid) against the oracle."""
+    g = np.load(os.path.join(golden_dir, "inversion_C1.npz"))
+    case = W.c1()
+    qlx_ctx.load_workload(case, g["data"])
+    sizes = [(20, 20), (1, 7), (30, 24), (12, 40), (24, 24)]
+    res = qlx_ctx.invert_image_surface_brightness_map([case["lenses"]] * len(sizes), src_n=sizes)
+    for (nx, ny), r in zip(sizes, res):
+        if nx < 2 or ny < 2:
+            assert r["minus2lnE"] == 2e30
+            continue
+        c = dict(case)
+        c["src_n"] = (nx, ny)
+        ro, _ = O.invert(c, g["data"], want_arrays=False)
+        assert r["n_src_active"] == ro["n_src"] and r["n_img_active"] == ro["n_img"]
+        assert rel(r["minus2lnE"], ro["minus2lnE"]) < RTOL_EVIDENCE, (nx, ny)
+
+
+# Mixed precision (BF16-split F, FP32 Cholesky, FP64 refinement and log-dets): the accuracy
+# contract is |delta ln E| < 0.1 against the FP64 evidence; the refined solve must converge.
+@pytest.mark.parametrize("name,builder", [("C1", W.c1), ("C2s", W.c2_small)])
+def test_mixed_precision_evidence(qlx_ctx, golden_dir, name, builder):
+    from qlens_amd import qlx
+    g = np.load(os.path.join(golden_dir, "inversion_%s.npz" % name))
+    case = builder()
+    qlx_ctx.load_workload(case, g["data"], precision=qlx.QLX_PREC_MIXED)
+    models = [case["lenses"]] + _perturbed(case, 3, 9)
+    res = qlx_ctx.invert_image_surface_brightness_map(models)
+    qlx_ctx.load_workload(case, g["data"])
+    ref = qlx_ctx.invert_image_surface_brightness_map(models)
+    assert abs(res[0]["minus2lnE"] - float(g["minus2lnE"])) / 2 < 0.1
+    for r, f in zip(res, ref):
+        assert r["status"] == 0 and r["n_src_active"] == f["n_src_active"]
+        assert abs(r["minus2lnE"] - f["minus2lnE"]) / 2 < 0.1, (r["minus2lnE"], f["minus2lnE"])
+        assert r["logdetR"] == f["logdetR"]  # R stays FP64 in both modes
+        assert r["refine_resid"] < 1e-6, r["refine_resid"]
