@@ -416,10 +416,11 @@ __device__ __forceinline__ int mask_rank(const unsigned* m, int cb)
 __device__ __forceinline__ bool mask_has(const unsigned* m, int cb) { return (m[cb >> 5] >> (cb & 31)) & 1u; }
 
 // ------------------------------------------------------------------------------------------------
-// blurred rows into the chunk's support blocks (one wave per row; the row's support columns are
-// accumulated in LDS in CSR-tap order, then each support block row is stored as 64 contiguous
-// values).  FP64: pool of doubles.  Mixed: rows scaled by sqrt(c_r) and split x = hi + lo into
-// two BF16 pools (hi = bf16(x), lo = bf16(x - hi)); the lo pool follows the hi pool.
+// blurred rows into the chunk's support blocks (one wave per row).  The lanes of the wave take the
+// PSF taps (lane t, t+64, ...), and every (tap, merged L entry) product is added into the row's
+// support columns in LDS with ds_add_f64; each support block row is then stored as 64 contiguous
+// values.  FP64: pool of doubles.  Mixed: rows scaled by sqrt(c_r) and split x = hi + lo into two
+// BF16 pools (hi = bf16(x), lo = bf16(x - hi)); the lo pool follows the hi pool.
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)
 {
@@ -449,52 +450,72 @@ __global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int bf16, int n
     for (int c = lane; c < nsup * TS; c += 64) row[c] = 0.0;
     __syncthreads();
     const int U = 3 * bd.nsub;
-    if (rs < M.n_img) {
-        const int p = bd.act_pix[item * bd.s_pix + bd.perm[item * bd.s_pix + rs]];
+    const bool live = rs < M.n_img;
+    const int p = live ? bd.act_pix[item * bd.s_pix + bd.perm[item * bd.s_pix + rs]] : 0;
+    if (live) {
         const int k = p / bd.ny, l = p % bd.ny;
         const int nxh = bd.pnx / 2, nyh = bd.pny / 2;
+        const int ntap = bd.pnx * bd.pny;
         const int* pidx = bd.pidx + item * bd.s_pix;
         const int* ucnt = bd.ucnt + item * bd.s_pix;
         const int* ucol = bd.ucol + (size_t)item * bd.s_pix * U;
         const double* uval = bd.uval + (size_t)item * bd.s_pix * U;
-        for (int pk = 0; pk < bd.pnx; pk++) {
-            const int ii = k + nxh - pk;
-            if (ii < 0 || ii >= bd.nx) continue;
-            for (int pl = 0; pl < bd.pny; pl++) {
-                const int jj = l + nyh - pl;
-                if (jj < 0 || jj >= bd.ny) continue;
-                const int r2 = pidx[ii * bd.ny + jj];
-                if (r2 < 0) continue;
-                const double pv = bd.psf[pk * bd.pny + pl];
-                if (lane < ucnt[r2]) {
-                    const int c = ucol[(size_t)r2 * U + lane];
-                    double* dst = row + loc[c >> 6] * TS + (c & 63);
-                    *dst += pv * uval[(size_t)r2 * U + lane];
-                }
-                __builtin_amdgcn_wave_barrier();  // taps are applied in order across lanes of the wave
+        for (int t = lane; t < ntap; t += 64) {
+            const int pk = t / bd.pny, pl = t - pk * bd.pny;
+            const int ii = k + nxh - pk, jj = l + nyh - pl;
+            if (ii < 0 || ii >= bd.nx || jj < 0 || jj >= bd.ny) continue;
+            const int r2 = pidx[ii * bd.ny + jj];
+            if (r2 < 0) continue;
+            const double pv = bd.psf[t];
+            const int cnt = ucnt[r2];
+            const int* uc = ucol + (size_t)r2 * U;
+            const double* uv = uval + (size_t)r2 * U;
+            for (int e = 0; e < cnt; e++) {
+                const int c = uc[e];
+                atomicAdd(row + loc[c >> 6] * TS + (c & 63), pv * uv[e]);
             }
         }
-        if (lane == 0) row[loc[M.n_src >> 6] * TS + (M.n_src & 63)] = bd.data[p];
-        __builtin_amdgcn_wave_barrier();
-        if (bf16) {
-            const double sc = sqrt(bd.covinv[p]);
-            for (int c = lane; c < nsup * TS; c += 64) row[c] *= sc;
-        }
+    }
+    __syncthreads();
+    if (live && lane == 0) row[loc[M.n_src >> 6] * TS + (M.n_src & 63)] = bd.data[p];
+    __syncthreads();
+    if (live && bf16) {
+        const double sc = sqrt(bd.covinv[p]);
+        for (int c = lane; c < nsup * TS; c += 64) row[c] *= sc;
     }
     __syncthreads();
     if (!bf16) {
         double* pool = (double*)bd.lb + M.lb_off;
         for (int q = 0; q < nsup; q++) pool[(size_t)(sbase + q) * TS * TS + (size_t)rr * TS + lane] = row[q * TS + lane];
     } else {
+        // k4-interleaved BF16 blocks: element (k, i) of a block at ((k >> 2) * 64 + i) * 4 + (k & 3),
+        // so the SYRK stages blocks with plain 16-byte copies and reads 8 consecutive k per lane
         uint16_t* hi = (uint16_t*)bd.lb + M.lb_off;
         uint16_t* lo = hi + (size_t)M.nblk * TS * TS;
-        for (int q = 0; q < nsup; q++) {
-            const float x = (float)row[q * TS + lane];
-            const uint16_t h = f32_to_bf16_rne(x);
-            const uint16_t g = f32_to_bf16_rne(x - bf16_to_f32(h));
-            const size_t o = (size_t)(sbase + q) * TS * TS + (size_t)rr * TS + lane;
-            hi[o] = h;
-            lo[o] = g;
+        if (nw == 4) {
+            const int kg = (r0 % TS) >> 2;
+            for (int idx = tid; idx < nsup * TS; idx += 256) {
+                const int q = idx >> 6, i = idx & 63;
+                uint16_t h[4], g[4];
+#pragma unroll
+                for (int w2 = 0; w2 < 4; w2++) {
+                    const float x = (float)lds_rows[(size_t)w2 * nsup_cap * TS + q * TS + i];
+                    h[w2] = f32_to_bf16_rne(x);
+                    g[w2] = f32_to_bf16_rne(x - bf16_to_f32(h[w2]));
+                }
+                const size_t o = (size_t)(sbase + q) * TS * TS + ((size_t)kg * TS + i) * 4;
+                *(uint2*)&hi[o] = make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
+                *(uint2*)&lo[o] = make_uint2(g[0] | ((unsigned)g[1] << 16), g[2] | ((unsigned)g[3] << 16));
+            }
+        } else {
+            for (int q = 0; q < nsup; q++) {
+                const float x = (float)row[q * TS + lane];
+                const uint16_t h = f32_to_bf16_rne(x);
+                const uint16_t g = f32_to_bf16_rne(x - bf16_to_f32(h));
+                const size_t o = (size_t)(sbase + q) * TS * TS + ((size_t)(rr >> 2) * TS + lane) * 4 + (rr & 3);
+                hi[o] = h;
+                lo[o] = g;
+            }
         }
     }
 }
@@ -606,15 +627,24 @@ void launch_syrk_f64_b(hipStream_t s, const BatchDev& bd, int max_ncb)
 }
 
 // ------------------------------------------------------------------------------------------------
-// F32 = Lb^T Lb with Lb = hi + lo (BF16 pools of sqrt(c)-scaled rows), on
+// F32 = Lb^T Lb with Lb = hi + lo (BF16 pools of sqrt(c)-scaled rows, k4-interleaved blocks), on
 // v_mfma_f32_16x16x32_bf16: hi*hi + hi*lo + lo*hi accumulated in FP32.  A operand lane l holds
-// A[i = l%16][k = 8*(l/16) .. +8]; B the same with j; accumulator register v of lane l holds
-// D[4*(l/16) + v][l%16].  Each wave computes a 32x32 quadrant of the 64x64 tile.
+// A[i = l%16][8 consecutive k]; B the same with j; accumulator register v of lane l holds
+// D[4*(l/16) + v][l%16].  Each wave computes a 32x32 quadrant of the 64x64 tile.  The four blocks
+// of the next chunk are loaded into registers while the current chunk is multiplied from LDS.
 // ------------------------------------------------------------------------------------------------
-#define BKP (TS + 8)
+#define BPITCH 288  // LDS elements per k-group (64 columns x 4 k + 32: the four lane groups of a
+                    // 64-bit operand read fall on alternating bank halves)
+__device__ __forceinline__ bf16x8_t lds_k8(const uint16_t* arr, int g, int i)
+{
+    const uint2 a = *(const uint2*)&arr[g * BPITCH + i * 4];
+    const uint2 b = *(const uint2*)&arr[(g + 1) * BPITCH + i * 4];
+    return __builtin_bit_cast(bf16x8_t, make_uint4(a.x, a.y, b.x, b.y));
+}
+
 __global__ void __launch_bounds__(256) k_syrk_bf16_b(BatchDev bd)
 {
-    __shared__ uint16_t Ah[TS][BKP], Al[TS][BKP], Bh[TS][BKP], Bl[TS][BKP];  // [i][k]
+    __shared__ __attribute__((aligned(16))) uint16_t lds[4][16 * BPITCH];  // A hi, A lo, B hi, B lo
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     if (!M.valid) return;
@@ -628,38 +658,63 @@ __global__ void __launch_bounds__(256) k_syrk_bf16_b(BatchDev bd)
     const uint16_t* lo = hi + (size_t)M.nblk * TS * TS;
     const unsigned* nz = bd.nz + M.nz_off;
     const int* bb = bd.bbase + M.bb_off;
+    const int nkt = M.nkt, ncbw = M.ncbw;
+    auto next_chunk = [&](int kt) {
+        for (kt++; kt < nkt; kt++) {
+            const unsigned* m = nz + (size_t)kt * ncbw;
+            if (mask_has(m, I) && mask_has(m, J)) break;
+        }
+        return kt;
+    };
+    uint4 p0, p1, p2, p3, p4, p5, p6, p7;  // the next chunk's four blocks, two 16-byte pieces each
+#define SYRK_BF16_LOAD(kt_)                                                                 \
+    {                                                                                       \
+        const unsigned* m_ = nz + (size_t)(kt_) * ncbw;                                     \
+        const size_t sI_ = (size_t)(bb[kt_] + mask_rank(m_, I)) * TS * TS;                  \
+        const size_t sJ_ = (size_t)(bb[kt_] + mask_rank(m_, J)) * TS * TS;                  \
+        p0 = ((const uint4*)(hi + sI_))[tid];                                               \
+        p1 = ((const uint4*)(hi + sI_))[tid + 256];                                         \
+        p2 = ((const uint4*)(lo + sI_))[tid];                                               \
+        p3 = ((const uint4*)(lo + sI_))[tid + 256];                                         \
+        p4 = ((const uint4*)(hi + sJ_))[tid];                                               \
+        p5 = ((const uint4*)(hi + sJ_))[tid + 256];                                         \
+        p6 = ((const uint4*)(lo + sJ_))[tid];                                               \
+        p7 = ((const uint4*)(lo + sJ_))[tid + 256];                                         \
+    }
+    const int e0 = tid * 8, e1 = (tid + 256) * 8;
+    const int o0 = (e0 >> 8) * BPITCH + (e0 & 255), o1 = (e1 >> 8) * BPITCH + (e1 & 255);
     float4_t acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int c = 0; c < 2; c++) acc[a][c] = (float4_t){0, 0, 0, 0};
-    for (int kt = 0; kt < M.nkt; kt++) {
-        const unsigned* m = nz + (size_t)kt * M.ncbw;
-        if (!(mask_has(m, I) && mask_has(m, J))) continue;
-        const size_t oA = (size_t)(bb[kt] + mask_rank(m, I)) * TS * TS;
-        const size_t oB = (size_t)(bb[kt] + mask_rank(m, J)) * TS * TS;
+    int kt = next_chunk(-1);
+    if (kt < nkt) SYRK_BF16_LOAD(kt);
+    while (kt < nkt) {
         __syncthreads();
-        // blocks are [k][i] in memory; stored transposed [i][k] so each lane's 8 k-values are contiguous
-        for (int e = tid; e < TS * TS; e += 256) {
-            const int kk = e >> 6, ii = e & 63;
-            Ah[ii][kk] = hi[oA + e];
-            Al[ii][kk] = lo[oA + e];
-            Bh[ii][kk] = hi[oB + e];
-            Bl[ii][kk] = lo[oB + e];
-        }
+        *(uint4*)&lds[0][o0] = p0;
+        *(uint4*)&lds[0][o1] = p1;
+        *(uint4*)&lds[1][o0] = p2;
+        *(uint4*)&lds[1][o1] = p3;
+        *(uint4*)&lds[2][o0] = p4;
+        *(uint4*)&lds[2][o1] = p5;
+        *(uint4*)&lds[3][o0] = p6;
+        *(uint4*)&lds[3][o1] = p7;
         __syncthreads();
+        const int nk = next_chunk(kt);
+        if (nk < nkt) SYRK_BF16_LOAD(nk);
 #pragma unroll
         for (int k0 = 0; k0 < TS; k0 += 32) {
-            const int kq = k0 + 8 * (lane >> 4);
+            const int g = (k0 >> 2) + 2 * (lane >> 4);
             bf16x8_t ah[2], al[2], bh[2], bl[2];
 #pragma unroll
             for (int t = 0; t < 2; t++) {
                 const int ia = wr * 32 + t * 16 + (lane & 15);
                 const int jb = wc * 32 + t * 16 + (lane & 15);
-                ah[t] = *(const bf16x8_t*)&Ah[ia][kq];
-                al[t] = *(const bf16x8_t*)&Al[ia][kq];
-                bh[t] = *(const bf16x8_t*)&Bh[jb][kq];
-                bl[t] = *(const bf16x8_t*)&Bl[jb][kq];
+                ah[t] = lds_k8(lds[0], g, ia);
+                al[t] = lds_k8(lds[1], g, ia);
+                bh[t] = lds_k8(lds[2], g, jb);
+                bl[t] = lds_k8(lds[3], g, jb);
             }
 #pragma unroll
             for (int a = 0; a < 2; a++)
@@ -670,6 +725,7 @@ __global__ void __launch_bounds__(256) k_syrk_bf16_b(BatchDev bd)
                     acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[c], acc[a][c], 0, 0, 0);
                 }
         }
+        kt = nk;
     }
     float* C = bd.F32 + M.f_off + (size_t)I * TS * M.ldN + (size_t)J * TS;
 #pragma unroll
@@ -682,6 +738,7 @@ __global__ void __launch_bounds__(256) k_syrk_bf16_b(BatchDev bd)
                 const int cc = wc * 32 + c * 16 + (lane & 15);
                 C[(size_t)rr * M.ldN + cc] = acc[a][c][v];
             }
+#undef SYRK_BF16_LOAD
 }
 
 void launch_syrk_bf16_b(hipStream_t s, const BatchDev& bd, int max_ncb)
@@ -769,18 +826,23 @@ void launch_curvature_b(hipStream_t s, const BatchDev& bd, int max_cells, int ma
 template <typename T>
 __global__ void __launch_bounds__(256) k_prep_f_b(BatchDev bd)
 {
+    __shared__ T tt[TS][TS + 1];
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     if (!M.valid) return;
-    const int ld = M.ldN, n = M.n_src;
+    const int ld = M.ldN, n = M.n_src, nb = ld / TS;
+    if ((int)blockIdx.x >= nb * (nb + 1) / 2) return;
+    int I, J;
+    tri_decode(blockIdx.x, I, J);
     const bool mixed = sizeof(T) == 4;
     T* F = mixed ? (T*)(bd.F32 + M.f_off) : (T*)(bd.F + M.f_off);
     double* fdiag = bd.vec + M.vec_off + 3 * (size_t)ld;
     const double lam = bd.lambda;
-    const size_t total = (size_t)ld * ld;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
-        const int r = (int)(t / ld), c = (int)(t % ld);
+    for (int e = threadIdx.x; e < TS * TS; e += 256) {
+        const int rr = e >> 6, cc = e & 63;
+        const int r = I * TS + rr, c = J * TS + cc;
         if (c > r) continue;
+        const size_t t = (size_t)r * ld + c;
         T v;
         if (r < n) {
             const double rv = ((r >> 6) - (c >> 6) <= M.bwb) ? *rtile(bd, M, r, c) : 0.0;
@@ -792,15 +854,24 @@ __global__ void __launch_bounds__(256) k_prep_f_b(BatchDev bd)
         }
         F[t] = v;
         if (mixed) {
-            if (c < r && r <= n) F[(size_t)c * ld + r] = v;
+            tt[cc][rr] = v;
             if (c == r && r < n) fdiag[r] = (double)v;
         }
+    }
+    if (!mixed) return;
+    __syncthreads();
+    // transposed copy into the upper triangle, written row by row of the upper tile (J, I)
+    for (int e = threadIdx.x; e < TS * TS; e += 256) {
+        const int rr2 = e >> 6, cc2 = e & 63;
+        const int c = J * TS + rr2, r = I * TS + cc2;
+        if (c < r && r <= n) F[(size_t)c * ld + r] = tt[rr2][cc2];
     }
 }
 
 void launch_prep_fr_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32)
 {
-    const unsigned g = grid_for((size_t)max_ld * max_ld, 256, 4096);
+    const int nb = max_ld / TS;
+    const unsigned g = nb * (nb + 1) / 2;
     if (f32) hipLaunchKernelGGL(k_prep_f_b<float>, dim3(g, bd.B), dim3(256), 0, s, bd);
     else hipLaunchKernelGGL(k_prep_f_b<double>, dim3(g, bd.B), dim3(256), 0, s, bd);
 }
@@ -829,122 +900,107 @@ __device__ __forceinline__ int band_last(const MatDesc& D, int kb)
     return D.bwb > 0 ? min(D.nb - 1, kb + D.bwb) : D.nb - 1;
 }
 
-#define CHOL_DIAG_LDS (2 * TS + 2 * TS * (TS + 1) + 4 * TS)
+// Diagonal 64x64 block, one wave: lane r keeps row r in registers.  Column step c: the pivot
+// and the column entries l_qc go through LDS (broadcast reads); every lane scales its own entry
+// and applies the rank-1 update to all of its entries right of c without branching (entries
+// above the diagonal collect values that are never stored).  Then lane c forms column c of L^-1
+// by forward substitution reading L from LDS.
 template <typename T>
-__device__ void chol_diag_body(const MatDesc& D, int kb, double* raw, int* bad)
+__device__ void chol_diag_wave(const MatDesc& D, int kb, double* raw)
 {
-    T* colbuf = (T*)raw;                          // [2][64]
-    T(*Ls)[TS + 1] = (T(*)[TS + 1])(colbuf + 2 * TS);
-    T(*Xs)[TS + 1] = (T(*)[TS + 1])(colbuf + 2 * TS + TS * (TS + 1));
-    T(*part)[TS] = (T(*)[TS])(colbuf + 2 * TS + 2 * TS * (TS + 1));
-    const int tid = threadIdx.x;
-    const int r = tid >> 2, g = tid & 3;
+    T(*Ls)[TS + 1] = (T(*)[TS + 1])raw;
+    T* col = (T*)raw + TS * (TS + 1);
+    const int r = threadIdx.x;
     const long long base = (long long)kb * TS;
     int ld;
     T* tile = tile_ptr<T>(D, kb, kb, ld);
-    T a[16];
-    const T* arow = tile + (size_t)r * ld + 16 * g;
+    for (int e = r; e < TS * TS; e += 64) Ls[e >> 6][e & 63] = tile[(size_t)(e >> 6) * ld + (e & 63)];
+    __syncthreads();
+    T a[TS];
 #pragma unroll
-    for (int j = 0; j < 16; j++) a[j] = (16 * g + j <= r) ? arow[j] : (T)0;
-    if (tid == 0) *bad = 0;
+    for (int q = 0; q < TS; q++) a[q] = Ls[r][q];
+    bool bad = false;
+#pragma unroll
     for (int c = 0; c < TS; c++) {
-        T* col = colbuf + (c & 1) * TS;  // step c+1 writes the other buffer while step c may still be read
-        const int gc = c >> 4, jc = c & 15;
-        T v = 0;
-#pragma unroll
-        for (int j = 0; j < 16; j++) v = (j == jc) ? a[j] : v;
-        if (r == c && g == gc) {
-            if (v < 0 && base + c < D.n) *bad = 1;
-            col[c] = sqrt(fabs(v));
-        }
+        if (r == c) col[c] = a[c];
         __syncthreads();
-        const T d = col[c];
-        const T lrc = (r > c) ? v / d : d;
-        if (g == gc && r >= c) {
-#pragma unroll
-            for (int j = 0; j < 16; j++) a[j] = (j == jc) ? lrc : a[j];
-        }
-        if (g == gc && r > c) col[r] = lrc;
+        const T piv = col[c];
+        bad |= (piv < 0 && base + c < D.n);
+        const T d = sqrt(fabs(piv));
+        const T l = (r > c) ? a[c] / d : d;
+        a[c] = l;
+        col[r] = l;
         __syncthreads();
-        if (r > c) {
-            const T l = col[r];
 #pragma unroll
-            for (int j = 0; j < 16; j++) {
-                const int q = 16 * g + j;
-                if (q > c && q <= r) a[j] = fma(-l, col[q], a[j]);
-            }
-        }
+        for (int q = c + 1; q < TS; q++) a[q] = fma(-l, col[q], a[q]);
     }
+    if (r == 0 && bad) atomicOr(D.status, 1);
     __syncthreads();
-    if (tid == 0 && *bad) atomicOr(D.status, 1);
-    T* drow = tile + (size_t)r * ld + 16 * g;
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-        const int q = 16 * g + j;
-        const T v = (q <= r) ? a[j] : (T)0;
-        if (q <= r) drow[j] = v;
-        Ls[r][q] = v;
-    }
-    if (g == (r >> 4)) ((T*)D.diag)[base + r] = a[r & 15];
+    for (int q = 0; q < TS; q++) Ls[r][q] = (q <= r) ? a[q] : (T)0;
     __syncthreads();
-    // X = L^-1 by rows: X[i][c] = (delta_ic - sum_{q<i} L[i][q] X[q][c]) / L[i][i]; thread (c, g)
-    // sums q = g, g+4, ... and the four partial sums are combined in LDS.
-    const int cidx = tid >> 2;
+    for (int e = r; e < TS * TS; e += 64)
+        if ((e & 63) <= (e >> 6)) tile[(size_t)(e >> 6) * ld + (e & 63)] = Ls[e >> 6][e & 63];
+    ((T*)D.diag)[base + r] = Ls[r][r];
+    T x[TS];
+#pragma unroll
     for (int i = 0; i < TS; i++) {
-        T acc = 0;
-        for (int q = g; q < i; q += 4) acc = fma(Ls[i][q], Xs[q][cidx], acc);
-        part[g][cidx] = acc;
-        __syncthreads();
-        if (g == 0) {
-            T s = (i == cidx ? (T)1 : (T)0) - (part[0][cidx] + part[1][cidx] + part[2][cidx] + part[3][cidx]);
-            Xs[i][cidx] = (cidx <= i) ? s / Ls[i][i] : (T)0;
-        }
-        __syncthreads();
+        T sacc = (i == r) ? (T)1 : (T)0;
+#pragma unroll
+        for (int q = 0; q < i; q++) sacc = fma(-Ls[i][q], x[q], sacc);
+        x[i] = (r <= i) ? sacc / Ls[i][i] : (T)0;
     }
     T* W = (T*)D.W + (size_t)kb * TS * TS;
-    for (int e = tid; e < TS * TS; e += 256) W[e] = Xs[e >> 6][e & 63];
+#pragma unroll
+    for (int i = 0; i < TS; i++) W[i * TS + r] = x[i];
 }
 
-__global__ void __launch_bounds__(256) k_chol_diag_m(const MatDesc* mats, int kb)
+template <typename T>
+__global__ void __launch_bounds__(64) k_chol_diag_m(const MatDesc* mats, int kb)
 {
-    __shared__ double raw[CHOL_DIAG_LDS];
-    __shared__ int bad;
+    __shared__ double raw[TS * (TS + 1) + TS];
     const MatDesc D = mats[blockIdx.y];
     if (kb >= D.nb) return;
-    if (D.f32) chol_diag_body<float>(D, kb, raw, &bad);
-    else chol_diag_body<double>(D, kb, raw, &bad);
+    chol_diag_wave<T>(D, kb, raw);
 }
 
-// 64x64x64 MFMA product C = sum_q A(i, q) B(j, q) (both operands "row holds the k-vector")
+// 64x64x64 MFMA product acc = sum_q A(i, q) B(j, q) with A and B stored "row holds the k-vector"
+// (the panel blocks as they lie in the factor), staged into LDS unchanged ([i][k], row pitch
+// LDT) and read as MFMA operands A[i = lane%16][k = kk + lane/16].
 template <typename T>
 struct Mfma16;
 template <>
 struct Mfma16<double> {
     typedef double4_t acc_t;
+    typedef double2 vec_t;
     __device__ static acc_t op(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
     __device__ static int row(int lane, int v) { return (lane >> 4) + 4 * v; }
 };
 template <>
 struct Mfma16<float> {
     typedef float4_t acc_t;
+    typedef float4 vec_t;
     __device__ static acc_t op(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
     __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-#define MM_LDS (2 * TS * LDSP)
+#define LDT 68
 template <typename T>
-__device__ __forceinline__ void mm64(const T* Ap, int sa, const T* Bp, int sb, double* raw, typename Mfma16<T>::acc_t acc[2][2])
+__device__ __forceinline__ void stage64(const T* P, int sp, T (*S)[LDT])
 {
-    T(*As)[LDSP] = (T(*)[LDSP])raw;
-    T(*Bs)[LDSP] = (T(*)[LDSP])((T*)raw + TS * LDSP);
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int wr = w >> 1, wc = w & 1;
-    for (int e = tid; e < TS * TS; e += 256) {
-        const int ii = e >> 6, kk = e & 63;
-        As[kk][ii] = Ap[(size_t)ii * sa + kk];
-        Bs[kk][ii] = Bp[(size_t)ii * sb + kk];
+    typedef typename Mfma16<T>::vec_t V;
+    constexpr int VW = sizeof(V) / sizeof(T);
+    for (int e = threadIdx.x * VW; e < TS * TS; e += 256 * VW) {
+        const int i = e >> 6, k = e & 63;
+        *(V*)&S[i][k] = *(const V*)&P[(size_t)i * sp + k];
     }
-    __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ void mma64(T (*As)[LDT], T (*Bs)[LDT], typename Mfma16<T>::acc_t acc[2][2])
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = w >> 1, wc = w & 1;
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -952,10 +1008,10 @@ __device__ __forceinline__ void mm64(const T* Ap, int sa, const T* Bp, int sb, d
 #pragma unroll 4
     for (int kk = 0; kk < TS; kk += 4) {
         const int kq = kk + (lane >> 4);
-        T a0 = As[kq][wr * 32 + (lane & 15)];
-        T a1 = As[kq][wr * 32 + 16 + (lane & 15)];
-        T b0 = Bs[kq][wc * 32 + (lane & 15)];
-        T b1 = Bs[kq][wc * 32 + 16 + (lane & 15)];
+        T a0 = As[wr * 32 + (lane & 15)][kq];
+        T a1 = As[wr * 32 + 16 + (lane & 15)][kq];
+        T b0 = Bs[wc * 32 + (lane & 15)][kq];
+        T b1 = Bs[wc * 32 + 16 + (lane & 15)][kq];
         acc[0][0] = Mfma16<T>::op(a0, b0, acc[0][0]);
         acc[0][1] = Mfma16<T>::op(a0, b1, acc[0][1]);
         acc[1][0] = Mfma16<T>::op(a1, b0, acc[1][0]);
@@ -963,7 +1019,7 @@ __device__ __forceinline__ void mm64(const T* Ap, int sa, const T* Bp, int sb, d
     }
 }
 
-// mode 0: store, 1: subtract, 2: subtract on the lower triangle only (diagonal tile)
+// C (64x64 at row pitch ldc) = acc (mode 0), C -= acc (mode 1), lower triangle only (mode 2)
 template <typename T>
 __device__ __forceinline__ void store64(T* C, int ldc, const typename Mfma16<T>::acc_t acc[2][2], int mode)
 {
@@ -983,44 +1039,40 @@ __device__ __forceinline__ void store64(T* C, int ldc, const typename Mfma16<T>:
             }
 }
 
+#define MM_LDS (2 * TS * LDT)
 template <typename T>
 __device__ void chol_panel_body(const MatDesc& D, int kb, int ib, double* raw)
 {
-    int ld, ldw = TS;
+    T(*As)[LDT] = (T(*)[LDT])raw;
+    T(*Bs)[LDT] = (T(*)[LDT])((T*)raw + TS * LDT);
+    int ld;
     T* blk = tile_ptr<T>(D, ib, kb, ld);
     const T* W = (const T*)D.W + (size_t)kb * TS * TS;
-    typename Mfma16<T>::acc_t acc[2][2];
-    mm64<T>(blk, ld, W, ldw, raw, acc);  // C(i,j) = sum_q A_ib[i][q] W[j][q]
+    stage64<T>(blk, ld, As);
+    stage64<T>(W, TS, Bs);
     __syncthreads();
+    typename Mfma16<T>::acc_t acc[2][2];
+    mma64<T>(As, Bs, acc);  // C(i,j) = sum_q A_ib[i][q] W[j][q]
     store64<T>(blk, ld, acc, 0);
 }
 
+template <typename T>
 __global__ void __launch_bounds__(256) k_chol_panel_m(const MatDesc* mats, int kb)
 {
-    __shared__ double raw[MM_LDS];
+    __shared__ T raw[MM_LDS];
     const MatDesc D = mats[blockIdx.y];
     if (kb >= D.nb) return;
     const int ib = kb + 1 + blockIdx.x;
     if (ib > band_last(D, kb)) return;
-    if (D.f32) chol_panel_body<float>(D, kb, ib, raw);
-    else chol_panel_body<double>(D, kb, ib, raw);
+    chol_panel_body<T>(D, kb, ib, (double*)raw);
 }
 
+// trailing update A_IJ -= P_I P_J^T for one element type (matrices of one launch share it)
 template <typename T>
-__device__ void chol_update_body(const MatDesc& D, int kb, int I, int J, double* raw)
-{
-    int lda, ldb, ldc;
-    const T* Pa = tile_ptr<T>(D, I, kb, lda);
-    const T* Pb = tile_ptr<T>(D, J, kb, ldb);
-    T* C = tile_ptr<T>(D, I, J, ldc);
-    typename Mfma16<T>::acc_t acc[2][2];
-    mm64<T>(Pa, lda, Pb, ldb, raw, acc);
-    store64<T>(C, ldc, acc, I == J ? 2 : 1);
-}
-
 __global__ void __launch_bounds__(256) k_chol_update_m(const MatDesc* mats, int kb)
 {
-    __shared__ double raw[MM_LDS];
+    __shared__ T As[TS][LDT];
+    __shared__ T Bs[TS][LDT];
     const MatDesc D = mats[blockIdx.y];
     if (kb >= D.nb) return;
     const int rest = band_last(D, kb) - kb;
@@ -1029,19 +1081,35 @@ __global__ void __launch_bounds__(256) k_chol_update_m(const MatDesc* mats, int 
     tri_decode(blockIdx.x, I, J);
     I += kb + 1;
     J += kb + 1;
-    if (D.f32) chol_update_body<float>(D, kb, I, J, raw);
-    else chol_update_body<double>(D, kb, I, J, raw);
+    int lda, ldb, ldc;
+    const T* Pa = tile_ptr<T>(D, I, kb, lda);
+    const T* Pb = tile_ptr<T>(D, J, kb, ldb);
+    T* C = tile_ptr<T>(D, I, J, ldc);
+    stage64<T>(Pa, lda, As);
+    stage64<T>(Pb, ldb, Bs);
+    __syncthreads();
+    typename Mfma16<T>::acc_t acc[2][2];
+    mma64<T>(As, Bs, acc);
+    store64<T>(C, ldc, acc, I == J ? 2 : 1);
 }
 
-void cholesky_batched(hipStream_t s, const MatDesc* d_mats, int nmat, int max_nb)
+template <typename T>
+static void chol_step(hipStream_t s, const MatDesc* mats, int nmat, int kb, int rest)
+{
+    hipLaunchKernelGGL(k_chol_diag_m<T>, dim3(1, nmat), dim3(64), 0, s, mats, kb);
+    if (rest > 0) {
+        hipLaunchKernelGGL(k_chol_panel_m<T>, dim3(rest, nmat), dim3(256), 0, s, mats, kb);
+        hipLaunchKernelGGL(k_chol_update_m<T>, dim3(rest * (rest + 1) / 2, nmat), dim3(256), 0, s, mats, kb);
+    }
+}
+
+// d_mats holds n32 FP32 matrices first, then n64 FP64 matrices
+void cholesky_batched(hipStream_t s, const MatDesc* d_mats, int n32, int n64, int max_nb)
 {
     for (int kb = 0; kb < max_nb; kb++) {
         const int rest = max_nb - kb - 1;
-        hipLaunchKernelGGL(k_chol_diag_m, dim3(1, nmat), dim3(256), 0, s, d_mats, kb);
-        if (rest > 0) {
-            hipLaunchKernelGGL(k_chol_panel_m, dim3(rest, nmat), dim3(256), 0, s, d_mats, kb);
-            hipLaunchKernelGGL(k_chol_update_m, dim3(rest * (rest + 1) / 2, nmat), dim3(256), 0, s, d_mats, kb);
-        }
+        if (n32) chol_step<float>(s, d_mats, n32, kb, rest);
+        if (n64) chol_step<double>(s, d_mats + n32, n64, kb, rest);
     }
 }
 

@@ -467,8 +467,8 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     launch_prep_fr_b(st, bd, max_ld, mixed ? 1 : 0);
 
     // ---- factorizations: F (dense, FP64 or FP32) and R (banded FP64) of every item together ----
-    std::vector<MatDesc> mats;
-    mats.reserve(2 * B);
+    // mixed mode: the FP32 F matrices first, then the FP64 R matrices (one update launch per type)
+    std::vector<MatDesc> fm, rm;
     for (int b = 0; b < B; b++) {
         const ItemMeta& M = meta[b];
         if (!M.valid) continue;
@@ -483,7 +483,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         f.diag = (void*)(v + M.ldN);
         f.status = bd.status + 4 * b;
         f.W = (void*)(bd.winv + M.w_off);
-        mats.push_back(f);
+        fm.push_back(f);
         MatDesc r{};
         r.A = (void*)(bd.R + M.r_off);
         r.ld = 64;
@@ -494,11 +494,15 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         r.diag = (void*)(v + 2 * (size_t)M.ldN);
         r.status = bd.status + 4 * b + 1;
         r.W = (void*)(bd.winv + M.w_off + (size_t)M.ldN * 64);
-        mats.push_back(r);
+        rm.push_back(r);
     }
+    std::vector<MatDesc> mats(fm);
+    mats.insert(mats.end(), rm.begin(), rm.end());
+    const int n32 = mixed ? (int)fm.size() : 0;
+    const int n64 = (int)mats.size() - n32;
     HIPCHK(c, hipMemcpyAsync(c->d_mats.p, mats.data(), sizeof(MatDesc) * mats.size(), hipMemcpyHostToDevice, st));
     timer_begin(c, "cholesky");
-    cholesky_batched(st, c->d_mats.as<MatDesc>(), (int)mats.size(), max_ld / 64);
+    cholesky_batched(st, c->d_mats.as<MatDesc>(), n32, n64, max_ld / 64);
     timer_end(c, "cholesky");
     launch_logdets_b(st, bd, mixed ? 1 : 0);
     timer_begin(c, "solve");
