@@ -307,8 +307,9 @@ __device__ __forceinline__ void mask_set(unsigned (&m)[QLX_MAX_CBW], int cb)
         if (w == (cb >> 5)) m[w] |= 1u << (cb & 31);
 }
 
-__global__ void __launch_bounds__(256) k_row_support_b(BatchDev bd)
+__global__ void __launch_bounds__(256) k_row_support_b(BatchDev bd, int use_seen)
 {
+    extern __shared__ unsigned seen[];  // one bit per active image pixel (when use_seen)
     __shared__ unsigned msk[QLX_MAX_CBW];
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
@@ -316,11 +317,13 @@ __global__ void __launch_bounds__(256) k_row_support_b(BatchDev bd)
     if (!M.valid || kt >= M.nkt) return;
     const int tid = threadIdx.x;
     if (tid < QLX_MAX_CBW) msk[tid] = 0;
+    if (use_seen)
+        for (int wd = tid; wd < (M.n_img + 31) / 32; wd += 256) seen[wd] = 0;
     __syncthreads();
     unsigned m[QLX_MAX_CBW];
 #pragma unroll
     for (int w = 0; w < QLX_MAX_CBW; w++) m[w] = 0;
-    if (tid == 0) mask_set(m, M.n_src >> 6);  // the data column
+    if (tid == 0) mask_set(m, M.n_src >> M.cbs);  // the data column
     const int ntap = bd.pnx * bd.pny;
     const int nxh = bd.pnx / 2, nyh = bd.pny / 2;
     const int U = 3 * bd.nsub;
@@ -339,9 +342,13 @@ __global__ void __launch_bounds__(256) k_row_support_b(BatchDev bd)
         if (ii < 0 || ii >= bd.nx || jj < 0 || jj >= bd.ny) continue;
         const int r2 = pidx[ii * bd.ny + jj];
         if (r2 < 0) continue;
+        if (use_seen) {  // each neighbouring pixel's columns are taken once per chunk
+            const unsigned bit = 1u << (r2 & 31);
+            if (atomicOr(&seen[r2 >> 5], bit) & bit) continue;
+        }
         const int cnt = ucnt[r2];
         for (int e = 0; e < cnt; e++) {
-            const int cb = ucol[(size_t)r2 * U + e] >> 6;
+            const int cb = ucol[(size_t)r2 * U + e] >> M.cbs;
             const unsigned bit = 1u << (cb & 31);
 #pragma unroll
             for (int w = 0; w < QLX_MAX_CBW; w++) m[w] |= (w == (cb >> 5)) ? bit : 0u;
@@ -402,7 +409,10 @@ __global__ void __launch_bounds__(1024) k_block_scan_b(BatchDev bd)
 void launch_row_support_b(hipStream_t s, const BatchDev& bd, int max_nkt)
 {
     if (max_nkt <= 0) return;
-    hipLaunchKernelGGL(k_row_support_b, dim3(max_nkt, bd.B), dim3(256), 0, s, bd);
+    const size_t seen_bytes = ((size_t)bd.s_pix + 31) / 32 * 4;
+    const int use_seen = seen_bytes <= 96 * 1024;
+    allow_big_lds((const void*)k_row_support_b);
+    hipLaunchKernelGGL(k_row_support_b, dim3(max_nkt, bd.B), dim3(256), use_seen ? seen_bytes : 0, s, bd, use_seen);
     hipLaunchKernelGGL(k_block_scan_b, dim3(1, bd.B), dim3(1024), 0, s, bd);
 }
 
@@ -430,15 +440,21 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)
 }
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(((unsigned)h) << 16); }
 
-__global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int bf16, int nsup_cap)
+// AccT: LDS accumulation type (double for the FP64 pool; float for the mixed path, whose values
+// are split into two BF16 terms resolving 2^-17 of each entry)
+template <typename AccT>
+__global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int nsup_cap)
 {
-    extern __shared__ double lds_rows[];
+    extern __shared__ double lds_raw[];
+    AccT* lds_rows = (AccT*)lds_raw;
     __shared__ short loc[QLX_MAX_LD / TS];
+    const bool bf16 = sizeof(AccT) == 4;
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     const int nw = blockDim.x >> 6;
     const int r0 = blockIdx.x * nw;
     if (!M.valid || r0 >= M.nrows) return;
+    const int cbs = M.cbs, CB = 1 << cbs;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int rs = r0 + w;
     const int kt = r0 / TS, rr = rs % TS;
@@ -446,8 +462,9 @@ __global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int bf16, int n
     for (int cb = tid; cb < M.ncb; cb += blockDim.x) loc[cb] = mask_has(m, cb) ? (short)mask_rank(m, cb) : (short)-1;
     const int sbase = bd.bbase[M.bb_off + kt];
     const int nsup = bd.bbase[M.bb_off + kt + 1] - sbase;
-    double* row = lds_rows + (size_t)w * nsup_cap * TS;
-    for (int c = lane; c < nsup * TS; c += 64) row[c] = 0.0;
+    const size_t bsz = (size_t)TS * CB;  // elements of one pool block
+    AccT* row = lds_rows + (size_t)w * nsup_cap * CB;
+    for (int c = lane; c < nsup * CB; c += 64) row[c] = (AccT)0;
     __syncthreads();
     const int U = 3 * bd.nsub;
     const bool live = rs < M.n_img;
@@ -460,74 +477,101 @@ __global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int bf16, int n
         const int* ucnt = bd.ucnt + item * bd.s_pix;
         const int* ucol = bd.ucol + (size_t)item * bd.s_pix * U;
         const double* uval = bd.uval + (size_t)item * bd.s_pix * U;
-        for (int t = lane; t < ntap; t += 64) {
-            const int pk = t / bd.pny, pl = t - pk * bd.pny;
-            const int ii = k + nxh - pk, jj = l + nyh - pl;
-            if (ii < 0 || ii >= bd.nx || jj < 0 || jj >= bd.ny) continue;
-            const int r2 = pidx[ii * bd.ny + jj];
-            if (r2 < 0) continue;
-            const double pv = bd.psf[t];
-            const int cnt = ucnt[r2];
-            const int* uc = ucol + (size_t)r2 * U;
-            const double* uv = uval + (size_t)r2 * U;
-            for (int e = 0; e < cnt; e++) {
-                const int c = uc[e];
-                atomicAdd(row + loc[c >> 6] * TS + (c & 63), pv * uv[e]);
+        // lane l takes taps tstride*(l + 64 i) mod ntap (a bijection: tstride is coprime with ntap),
+        // so the lanes of one step sit far apart in the PSF and rarely add into the same column;
+        // eight taps per lane are looked up together to keep their loads in flight at once
+        for (int tb = 0; tb < ntap; tb += 64 * 8) {
+            int r2s[8];
+            double pvs[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int t0 = tb + 64 * u + lane;
+                r2s[u] = -1;
+                pvs[u] = 0.0;
+                if (t0 < ntap) {
+                    const int t = (int)(((long long)t0 * bd.tstride) % ntap);
+                    const int pk = t / bd.pny, pl = t - pk * bd.pny;
+                    const int ii = k + nxh - pk, jj = l + nyh - pl;
+                    if (ii >= 0 && ii < bd.nx && jj >= 0 && jj < bd.ny) {
+                        r2s[u] = pidx[ii * bd.ny + jj];
+                        pvs[u] = bd.psf[t];
+                    }
+                }
+            }
+            int cnts[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) cnts[u] = (r2s[u] >= 0) ? ucnt[r2s[u]] : 0;
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int* uc = ucol + (size_t)max(r2s[u], 0) * U;
+                const double* uv = uval + (size_t)max(r2s[u], 0) * U;
+                for (int e = 0; e < cnts[u]; e++) {
+                    const int c = uc[e];
+                    atomicAdd(row + (loc[c >> cbs] << cbs) + (c & (CB - 1)), (AccT)(pvs[u] * uv[e]));
+                }
             }
         }
     }
     __syncthreads();
-    if (live && lane == 0) row[loc[M.n_src >> 6] * TS + (M.n_src & 63)] = bd.data[p];
+    if (live && lane == 0) row[(loc[M.n_src >> cbs] << cbs) + (M.n_src & (CB - 1))] = (AccT)bd.data[p];
     __syncthreads();
     if (live && bf16) {
-        const double sc = sqrt(bd.covinv[p]);
-        for (int c = lane; c < nsup * TS; c += 64) row[c] *= sc;
+        const AccT sc = (AccT)sqrt(bd.covinv[p]);
+        for (int c = lane; c < nsup * CB; c += 64) row[c] *= sc;
     }
     __syncthreads();
     if (!bf16) {
         double* pool = (double*)bd.lb + M.lb_off;
-        for (int q = 0; q < nsup; q++) pool[(size_t)(sbase + q) * TS * TS + (size_t)rr * TS + lane] = row[q * TS + lane];
+        for (int q = 0; q < nsup; q++)
+            for (int cc = lane; cc < CB; cc += 64) pool[(size_t)(sbase + q) * bsz + (size_t)rr * CB + cc] = (double)row[q * CB + cc];
     } else {
-        // k4-interleaved BF16 blocks: element (k, i) of a block at ((k >> 2) * 64 + i) * 4 + (k & 3),
+        // k4-interleaved BF16 blocks: element (k, i) of a block at ((k >> 2) * CB + i) * 4 + (k & 3),
         // so the SYRK stages blocks with plain 16-byte copies and reads 8 consecutive k per lane
         uint16_t* hi = (uint16_t*)bd.lb + M.lb_off;
-        uint16_t* lo = hi + (size_t)M.nblk * TS * TS;
+        uint16_t* lo = hi + (size_t)M.nblk * bsz;
         if (nw == 4) {
             const int kg = (r0 % TS) >> 2;
-            for (int idx = tid; idx < nsup * TS; idx += 256) {
-                const int q = idx >> 6, i = idx & 63;
+            for (int idx = tid; idx < nsup * CB; idx += 256) {
+                const int q = idx >> cbs, i = idx & (CB - 1);
                 uint16_t h[4], g[4];
 #pragma unroll
                 for (int w2 = 0; w2 < 4; w2++) {
-                    const float x = (float)lds_rows[(size_t)w2 * nsup_cap * TS + q * TS + i];
+                    const float x = (float)lds_rows[(size_t)w2 * nsup_cap * CB + idx];
                     h[w2] = f32_to_bf16_rne(x);
                     g[w2] = f32_to_bf16_rne(x - bf16_to_f32(h[w2]));
                 }
-                const size_t o = (size_t)(sbase + q) * TS * TS + ((size_t)kg * TS + i) * 4;
+                const size_t o = (size_t)(sbase + q) * bsz + ((size_t)kg * CB + i) * 4;
                 *(uint2*)&hi[o] = make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
                 *(uint2*)&lo[o] = make_uint2(g[0] | ((unsigned)g[1] << 16), g[2] | ((unsigned)g[3] << 16));
             }
         } else {
-            for (int q = 0; q < nsup; q++) {
-                const float x = (float)row[q * TS + lane];
-                const uint16_t h = f32_to_bf16_rne(x);
-                const uint16_t g = f32_to_bf16_rne(x - bf16_to_f32(h));
-                const size_t o = (size_t)(sbase + q) * TS * TS + ((size_t)(rr >> 2) * TS + lane) * 4 + (rr & 3);
-                hi[o] = h;
-                lo[o] = g;
-            }
+            for (int q = 0; q < nsup; q++)
+                for (int i = lane; i < CB; i += 64) {
+                    const float x = (float)row[q * CB + i];
+                    const uint16_t h = f32_to_bf16_rne(x);
+                    const uint16_t g = f32_to_bf16_rne(x - bf16_to_f32(h));
+                    const size_t o = (size_t)(sbase + q) * bsz + ((size_t)(rr >> 2) * CB + i) * 4 + (rr & 3);
+                    hi[o] = h;
+                    lo[o] = g;
+                }
         }
     }
 }
 
-void launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_nsup, int bf16)
+void launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_nsup, int bf16, int cbw)
 {
     if (max_rows <= 0 || max_nsup <= 0) return;
-    const size_t per_wave = (size_t)max_nsup * TS * sizeof(double);
+    const size_t per_wave = (size_t)max_nsup * cbw * (bf16 ? sizeof(float) : sizeof(double));
     int nw = 4;
     while (nw > 1 && per_wave * nw > 144 * 1024) nw >>= 1;
-    allow_big_lds((const void*)k_build_lb_b);
-    hipLaunchKernelGGL(k_build_lb_b, dim3((max_rows + nw - 1) / nw, bd.B), dim3(64 * nw), per_wave * nw, s, bd, bf16, max_nsup);
+    const dim3 grid((max_rows + nw - 1) / nw, bd.B);
+    if (bf16) {
+        allow_big_lds((const void*)k_build_lb_b<float>);
+        hipLaunchKernelGGL(k_build_lb_b<float>, grid, dim3(64 * nw), per_wave * nw, s, bd, max_nsup);
+    } else {
+        allow_big_lds((const void*)k_build_lb_b<double>);
+        hipLaunchKernelGGL(k_build_lb_b<double>, grid, dim3(64 * nw), per_wave * nw, s, bd, max_nsup);
+    }
 }
 
 __global__ void k_gather_cvec_b(BatchDev bd)
@@ -627,24 +671,26 @@ void launch_syrk_f64_b(hipStream_t s, const BatchDev& bd, int max_ncb)
 }
 
 // ------------------------------------------------------------------------------------------------
-// F32 = Lb^T Lb with Lb = hi + lo (BF16 pools of sqrt(c)-scaled rows, k4-interleaved blocks), on
-// v_mfma_f32_16x16x32_bf16: hi*hi + hi*lo + lo*hi accumulated in FP32.  A operand lane l holds
-// A[i = l%16][8 consecutive k]; B the same with j; accumulator register v of lane l holds
-// D[4*(l/16) + v][l%16].  Each wave computes a 32x32 quadrant of the 64x64 tile.  The four blocks
-// of the next chunk are loaded into registers while the current chunk is multiplied from LDS.
+// F32 = Lb^T Lb with Lb = hi + lo (BF16 pools of sqrt(c)-scaled rows in k4-interleaved 64 x 128
+// blocks), on v_mfma_f32_16x16x32_bf16: hi*hi + hi*lo + lo*hi accumulated in FP32.  One
+// workgroup per 128x128 lower tile of F; wave (wr, wc) computes the 64x64 quadrant (wr, wc).  A
+// operand lane l holds A[i = l%16][8 consecutive k], B the same with j; accumulator register v of
+// lane l holds D[4*(l/16) + v][l%16].  The next chunk's four blocks are loaded into registers
+// while the current chunk is multiplied from LDS.
 // ------------------------------------------------------------------------------------------------
-#define BPITCH 288  // LDS elements per k-group (64 columns x 4 k + 32: the four lane groups of a
-                    // 64-bit operand read fall on alternating bank halves)
+#define SCB 128  // column-block width of the mixed path
+#define BP2 544  // LDS elements per k-group: 128 columns x 4 k + 32, so the four lane groups of a
+                 // 64-bit operand read fall on alternating bank halves
 __device__ __forceinline__ bf16x8_t lds_k8(const uint16_t* arr, int g, int i)
 {
-    const uint2 a = *(const uint2*)&arr[g * BPITCH + i * 4];
-    const uint2 b = *(const uint2*)&arr[(g + 1) * BPITCH + i * 4];
+    const uint2 a = *(const uint2*)&arr[g * BP2 + i * 4];
+    const uint2 b = *(const uint2*)&arr[(g + 1) * BP2 + i * 4];
     return __builtin_bit_cast(bf16x8_t, make_uint4(a.x, a.y, b.x, b.y));
 }
 
 __global__ void __launch_bounds__(256) k_syrk_bf16_b(BatchDev bd)
 {
-    __shared__ __attribute__((aligned(16))) uint16_t lds[4][16 * BPITCH];  // A hi, A lo, B hi, B lo
+    extern __shared__ __attribute__((aligned(16))) uint16_t lds2[];  // A hi, A lo, B hi, B lo: 16 x BP2 each
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     if (!M.valid) return;
@@ -654,72 +700,76 @@ __global__ void __launch_bounds__(256) k_syrk_bf16_b(BatchDev bd)
     tri_decode(blockIdx.x, I, J);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wr = w >> 1, wc = w & 1;
+    const size_t bsz = (size_t)TS * SCB;
     const uint16_t* hi = (const uint16_t*)bd.lb + M.lb_off;
-    const uint16_t* lo = hi + (size_t)M.nblk * TS * TS;
+    const uint16_t* lo = hi + (size_t)M.nblk * bsz;
     const unsigned* nz = bd.nz + M.nz_off;
     const int* bb = bd.bbase + M.bb_off;
     const int nkt = M.nkt, ncbw = M.ncbw;
-    auto next_chunk = [&](int kt) {
-        for (kt++; kt < nkt; kt++) {
-            const unsigned* m = nz + (size_t)kt * ncbw;
-            if (mask_has(m, I) && mask_has(m, J)) break;
-        }
-        return kt;
-    };
-    uint4 p0, p1, p2, p3, p4, p5, p6, p7;  // the next chunk's four blocks, two 16-byte pieces each
-#define SYRK_BF16_LOAD(kt_)                                                                 \
-    {                                                                                       \
-        const unsigned* m_ = nz + (size_t)(kt_) * ncbw;                                     \
-        const size_t sI_ = (size_t)(bb[kt_] + mask_rank(m_, I)) * TS * TS;                  \
-        const size_t sJ_ = (size_t)(bb[kt_] + mask_rank(m_, J)) * TS * TS;                  \
-        p0 = ((const uint4*)(hi + sI_))[tid];                                               \
-        p1 = ((const uint4*)(hi + sI_))[tid + 256];                                         \
-        p2 = ((const uint4*)(lo + sI_))[tid];                                               \
-        p3 = ((const uint4*)(lo + sI_))[tid + 256];                                         \
-        p4 = ((const uint4*)(hi + sJ_))[tid];                                               \
-        p5 = ((const uint4*)(hi + sJ_))[tid + 256];                                         \
-        p6 = ((const uint4*)(lo + sJ_))[tid];                                               \
-        p7 = ((const uint4*)(lo + sJ_))[tid + 256];                                         \
+    uint16_t* const a_hi = lds2;
+    uint16_t* const a_lo = lds2 + 16 * BP2;
+    uint16_t* const b_hi = lds2 + 32 * BP2;
+    uint16_t* const b_lo = lds2 + 48 * BP2;
+    // thread piece h (of 4) of every block: 16 bytes at element (tid + 256 h) * 8
+    const int e0 = tid * 8;
+    const int o0 = (e0 >> 9) * BP2 + (e0 & 511);                    // pieces h = 0..3 are 2048
+    const int o1 = ((e0 + 2048) >> 9) * BP2 + ((e0 + 2048) & 511);  // elements apart: 4 k-groups
+    const int o2 = ((e0 + 4096) >> 9) * BP2 + ((e0 + 4096) & 511);
+    const int o3 = ((e0 + 6144) >> 9) * BP2 + ((e0 + 6144) & 511);
+    uint4 pa0, pa1, pa2, pa3, pb0, pb1, pb2, pb3, pc0, pc1, pc2, pc3, pd0, pd1, pd2, pd3;
+#define SYRK_LOAD(kk_)                                                                                 \
+    {                                                                                                  \
+        const unsigned* m_ = nz + (size_t)(kk_) * ncbw;                                                \
+        const uint4* ah_ = (const uint4*)(hi + (size_t)(bb[kk_] + mask_rank(m_, I)) * bsz) + tid;      \
+        const uint4* al_ = (const uint4*)(lo + (size_t)(bb[kk_] + mask_rank(m_, I)) * bsz) + tid;      \
+        const uint4* bh_ = (const uint4*)(hi + (size_t)(bb[kk_] + mask_rank(m_, J)) * bsz) + tid;      \
+        const uint4* bl_ = (const uint4*)(lo + (size_t)(bb[kk_] + mask_rank(m_, J)) * bsz) + tid;      \
+        pa0 = ah_[0]; pa1 = ah_[256]; pa2 = ah_[512]; pa3 = ah_[768];                                  \
+        pb0 = al_[0]; pb1 = al_[256]; pb2 = al_[512]; pb3 = al_[768];                                  \
+        pc0 = bh_[0]; pc1 = bh_[256]; pc2 = bh_[512]; pc3 = bh_[768];                                  \
+        pd0 = bl_[0]; pd1 = bl_[256]; pd2 = bl_[512]; pd3 = bl_[768];                                  \
     }
-    const int e0 = tid * 8, e1 = (tid + 256) * 8;
-    const int o0 = (e0 >> 8) * BPITCH + (e0 & 255), o1 = (e1 >> 8) * BPITCH + (e1 & 255);
-    float4_t acc[2][2];
+    float4_t acc[4][4];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int c = 0; c < 2; c++) acc[a][c] = (float4_t){0, 0, 0, 0};
-    int kt = next_chunk(-1);
-    if (kt < nkt) SYRK_BF16_LOAD(kt);
+        for (int c = 0; c < 4; c++) acc[a][c] = (float4_t){0, 0, 0, 0};
+    int kt = 0;
+    for (; kt < nkt; kt++) {
+        const unsigned* m = nz + (size_t)kt * ncbw;
+        if (mask_has(m, I) && mask_has(m, J)) break;
+    }
+    if (kt < nkt) SYRK_LOAD(kt);
     while (kt < nkt) {
         __syncthreads();
-        *(uint4*)&lds[0][o0] = p0;
-        *(uint4*)&lds[0][o1] = p1;
-        *(uint4*)&lds[1][o0] = p2;
-        *(uint4*)&lds[1][o1] = p3;
-        *(uint4*)&lds[2][o0] = p4;
-        *(uint4*)&lds[2][o1] = p5;
-        *(uint4*)&lds[3][o0] = p6;
-        *(uint4*)&lds[3][o1] = p7;
+        *(uint4*)&a_hi[o0] = pa0; *(uint4*)&a_hi[o1] = pa1; *(uint4*)&a_hi[o2] = pa2; *(uint4*)&a_hi[o3] = pa3;
+        *(uint4*)&a_lo[o0] = pb0; *(uint4*)&a_lo[o1] = pb1; *(uint4*)&a_lo[o2] = pb2; *(uint4*)&a_lo[o3] = pb3;
+        *(uint4*)&b_hi[o0] = pc0; *(uint4*)&b_hi[o1] = pc1; *(uint4*)&b_hi[o2] = pc2; *(uint4*)&b_hi[o3] = pc3;
+        *(uint4*)&b_lo[o0] = pd0; *(uint4*)&b_lo[o1] = pd1; *(uint4*)&b_lo[o2] = pd2; *(uint4*)&b_lo[o3] = pd3;
         __syncthreads();
-        const int nk = next_chunk(kt);
-        if (nk < nkt) SYRK_BF16_LOAD(nk);
+        int nk = kt + 1;
+        for (; nk < nkt; nk++) {
+            const unsigned* m = nz + (size_t)nk * ncbw;
+            if (mask_has(m, I) && mask_has(m, J)) break;
+        }
+        if (nk < nkt) SYRK_LOAD(nk);
 #pragma unroll
         for (int k0 = 0; k0 < TS; k0 += 32) {
             const int g = (k0 >> 2) + 2 * (lane >> 4);
-            bf16x8_t ah[2], al[2], bh[2], bl[2];
+            bf16x8_t ah[4], al[4], bh[4], bl[4];
 #pragma unroll
-            for (int t = 0; t < 2; t++) {
-                const int ia = wr * 32 + t * 16 + (lane & 15);
-                const int jb = wc * 32 + t * 16 + (lane & 15);
-                ah[t] = lds_k8(lds[0], g, ia);
-                al[t] = lds_k8(lds[1], g, ia);
-                bh[t] = lds_k8(lds[2], g, jb);
-                bl[t] = lds_k8(lds[3], g, jb);
+            for (int t = 0; t < 4; t++) {
+                const int ia = wr * 64 + t * 16 + (lane & 15);
+                const int jb = wc * 64 + t * 16 + (lane & 15);
+                ah[t] = lds_k8(a_hi, g, ia);
+                al[t] = lds_k8(a_lo, g, ia);
+                bh[t] = lds_k8(b_hi, g, jb);
+                bl[t] = lds_k8(b_lo, g, jb);
             }
 #pragma unroll
-            for (int a = 0; a < 2; a++)
+            for (int a = 0; a < 4; a++)
 #pragma unroll
-                for (int c = 0; c < 2; c++) {
+                for (int c = 0; c < 4; c++) {
                     acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[c], acc[a][c], 0, 0, 0);
                     acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[c], acc[a][c], 0, 0, 0);
                     acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[c], acc[a][c], 0, 0, 0);
@@ -727,24 +777,26 @@ __global__ void __launch_bounds__(256) k_syrk_bf16_b(BatchDev bd)
         }
         kt = nk;
     }
-    float* C = bd.F32 + M.f_off + (size_t)I * TS * M.ldN + (size_t)J * TS;
+#undef SYRK_LOAD
+    float* C = bd.F32 + M.f_off + (size_t)I * SCB * M.ldN + (size_t)J * SCB;
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int c = 0; c < 2; c++)
+        for (int c = 0; c < 4; c++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int rr = wr * 32 + a * 16 + 4 * (lane >> 4) + v;
-                const int cc = wc * 32 + c * 16 + (lane & 15);
+                const int rr = wr * 64 + a * 16 + 4 * (lane >> 4) + v;
+                const int cc = wc * 64 + c * 16 + (lane & 15);
                 C[(size_t)rr * M.ldN + cc] = acc[a][c][v];
             }
-#undef SYRK_BF16_LOAD
 }
 
 void launch_syrk_bf16_b(hipStream_t s, const BatchDev& bd, int max_ncb)
 {
     const int nt = max_ncb * (max_ncb + 1) / 2;
-    hipLaunchKernelGGL(k_syrk_bf16_b, dim3(nt, bd.B), dim3(256), 0, s, bd);
+    const size_t lds = (size_t)4 * 16 * BP2 * sizeof(uint16_t);
+    allow_big_lds((const void*)k_syrk_bf16_b);
+    hipLaunchKernelGGL(k_syrk_bf16_b, dim3(nt, bd.B), dim3(256), lds, s, bd);
 }
 
 // ------------------------------------------------------------------------------------------------

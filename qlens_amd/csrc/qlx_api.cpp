@@ -62,6 +62,8 @@ struct Timer {
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // R's factorization beside F's (same default priority)
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::string err;
     size_t hbm_budget = 0;
     // image
@@ -315,6 +317,16 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     bd.psf = c->psf.as<double>();
     bd.pnx = c->pnx;
     bd.pny = c->pny;
+    bd.tstride = 1;
+    for (int cand : {11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53}) {
+        const int nt = c->pnx * c->pny;
+        int a = cand, b = nt;
+        while (b) { const int t = a % b; a = b; b = t; }
+        if (a == 1 && cand < nt) {
+            bd.tstride = cand;
+            break;
+        }
+    }
     bd.tol = S.integral_tol;
     bd.lambda = S.regparam;
     bd.models = c->d_models.as<ModelPrep>();
@@ -387,9 +399,10 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
             continue;
         }
         M.valid = 1;
-        M.ldN = (int)round_up((size_t)n_src + 1, 64);
+        M.cbs = mixed ? 7 : 6;  // Lb column blocks of 128 (mixed) or 64 (FP64) source pixels
+        M.ldN = (int)round_up((size_t)n_src + 1, (size_t)1 << M.cbs);
         M.nrows = (int)std::max<size_t>(64, round_up((size_t)n_img, 64));
-        M.ncb = M.ldN / 64;
+        M.ncb = M.ldN >> M.cbs;
         M.ncbw = (M.ncb + 31) / 32;
         M.nkt = M.nrows / 64;
         M.bwb = (2 * M.snx) / 64 + 1;
@@ -450,13 +463,13 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         M.nblk = counts[4 * b + 2];
         M.nsup_max = counts[4 * b + 3];
         M.lb_off = (long long)lb_tot;
-        lb_tot += (size_t)M.nblk * 4096 * (mixed ? 2 : 1);
+        lb_tot += (size_t)M.nblk * ((size_t)64 << M.cbs) * (mixed ? 2 : 1);
         max_nsup = std::max(max_nsup, M.nsup_max);
     }
     if (!c->lb.ensure(lb_tot * (mixed ? 2 : 8))) return fail(c, QLX_ERR_NOMEM, "Lb pool allocation failed");
     bd.lb = c->lb.p;
     HIPCHK(c, hipMemcpyAsync(c->d_meta.p, meta.data(), sizeof(ItemMeta) * B, hipMemcpyHostToDevice, st));
-    launch_build_lb_b(st, bd, max_rows, max_nsup, mixed ? 1 : 0);
+    launch_build_lb_b(st, bd, max_rows, max_nsup, mixed ? 1 : 0, mixed ? 128 : 64);
     timer_end(c, "lbuild");
     if (!mixed) launch_gather_cvec_b(st, bd, max_rows);
     timer_begin(c, "syrk");
@@ -466,8 +479,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     launch_curvature_b(st, bd, max_cells, max_rtiles);
     launch_prep_fr_b(st, bd, max_ld, mixed ? 1 : 0);
 
-    // ---- factorizations: F (dense, FP64 or FP32) and R (banded FP64) of every item together ----
-    // mixed mode: the FP32 F matrices first, then the FP64 R matrices (one update launch per type)
+    // ---- factorizations: F (dense, FP64 or FP32) and R (banded FP64) of every item ----
     std::vector<MatDesc> fm, rm;
     for (int b = 0; b < B; b++) {
         const ItemMeta& M = meta[b];
@@ -498,12 +510,18 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     }
     std::vector<MatDesc> mats(fm);
     mats.insert(mats.end(), rm.begin(), rm.end());
-    const int n32 = mixed ? (int)fm.size() : 0;
-    const int n64 = (int)mats.size() - n32;
     HIPCHK(c, hipMemcpyAsync(c->d_mats.p, mats.data(), sizeof(MatDesc) * mats.size(), hipMemcpyHostToDevice, st));
+    // F (dense, FP32 or FP64) factors on the main stream, R (banded FP64, ln det R only) on the
+    // second stream; they join before the log-determinants
+    const int nF = (int)fm.size(), nR = (int)rm.size();
+    HIPCHK(c, hipEventRecord(c->ev_fork, st));
+    HIPCHK(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    cholesky_batched(c->stream2, c->d_mats.as<MatDesc>() + nF, 0, nR, max_ld / 64);
+    HIPCHK(c, hipEventRecord(c->ev_join, c->stream2));
     timer_begin(c, "cholesky");
-    cholesky_batched(st, c->d_mats.as<MatDesc>(), n32, n64, max_ld / 64);
+    cholesky_batched(st, c->d_mats.as<MatDesc>(), mixed ? nF : 0, mixed ? 0 : nF, max_ld / 64);
     timer_end(c, "cholesky");
+    HIPCHK(c, hipStreamWaitEvent(st, c->ev_join, 0));
     launch_logdets_b(st, bd, mixed ? 1 : 0);
     timer_begin(c, "solve");
     launch_trisolve_b(st, bd, max_ld, mixed ? 1 : 0);
@@ -544,10 +562,11 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         HIPCHK(c, hipMemcpy(nzh.data(), bd.nz, nz_tot * 4, hipMemcpyDeviceToHost));
         qlx_stats& T = c->stats;
         memset(&T, 0, sizeof(T));
-        const double tile = 2.0 * 64 * 64 * 64;
         for (int b = 0; b < B; b++) {
             const ItemMeta& M = meta[b];
             if (!M.valid) continue;
+            const double cbw = (double)(1 << M.cbs);
+            const double tile = 2.0 * 64 * cbw * cbw;
             double exec_pairs = 0, nblk = 0;
             for (int kt = 0; kt < M.nkt; kt++) {
                 double bits = 0;
@@ -558,7 +577,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
             T.syrk_exec_flops += exec_pairs * tile * (mixed ? 3 : 1);
             T.syrk_dense_flops += (double)M.ncb * (M.ncb + 1) / 2 * M.nkt * tile;
             T.syrk_alg_flops += (double)M.n_img * M.n_src * (M.n_src + 1) + 2.0 * M.n_img * M.n_src;
-            T.lb_bytes += nblk * 64 * 64 * (mixed ? 4 : 8);
+            T.lb_bytes += nblk * 64 * cbw * (mixed ? 4 : 8);
             T.integrand_evals += (double)nev[b];
             T.chol_flops += (double)M.n_src * M.n_src * M.n_src / 3.0;
             T.n_items++;
@@ -664,7 +683,10 @@ int qlx_ctx_create(const qlx_opts* opts, void** out)
         delete c;
         return fail(nullptr, QLX_ERR_ARG, "invalid device ordinal");
     }
-    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
         delete c;
         return fail(nullptr, QLX_ERR_HIP, "stream creation failed");
     }
@@ -683,6 +705,10 @@ int qlx_ctx_destroy(void* ctx)
         if (kv.second.a) hipEventDestroy(kv.second.a);
         if (kv.second.b) hipEventDestroy(kv.second.b);
     }
+    hipStreamSynchronize(c->stream2);
+    hipEventDestroy(c->ev_fork);
+    hipEventDestroy(c->ev_join);
+    hipStreamDestroy(c->stream2);
     hipStreamDestroy(c->stream);
     delete c;
     return QLX_OK;

@@ -54,7 +54,7 @@ struct ItemMeta {
     int n_src, n_img, ldN, nrows;    // active counts; padded source dim (>= n_src+1); padded rows
     int ncb, ncbw, nkt;              // column blocks, mask words per chunk, 64-row chunks
     int snx, sny, bwb, nblk;         // source grid; R band in blocks; Lb support blocks
-    int nsup_max, pad0;              // most support blocks of one chunk
+    int nsup_max, cbs;               // most support blocks of one chunk; log2 of the Lb column-block width
     long long lb_off;                // Lb pool (elements: double, or 2 x nblk x 4096 bf16)
     long long f_off;                 // F (elements of F or F32)
     long long r_off;                 // banded R tiles (doubles)
@@ -72,6 +72,7 @@ struct BatchDev {
     const double* covinv;
     const double* psf;
     int pnx, pny;
+    int tstride;                // coprime with pnx*pny: the Lb build's tap order across lanes
     double tol, lambda;
     // per-item inputs
     const ModelPrep* models;
@@ -142,7 +143,7 @@ void launch_compact_b(hipStream_t s, const BatchDev& bd);
 void launch_merge_lrows_b(hipStream_t s, const BatchDev& bd, int max_img);
 void launch_sort_rows_b(hipStream_t s, const BatchDev& bd, int max_img, int max_src);
 void launch_row_support_b(hipStream_t s, const BatchDev& bd, int max_nkt);
-void launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_nsup, int bf16);
+void launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_nsup, int bf16, int cbw);
 void launch_gather_cvec_b(hipStream_t s, const BatchDev& bd, int max_rows);
 void launch_syrk_f64_b(hipStream_t s, const BatchDev& bd, int max_ncb);
 void launch_syrk_bf16_b(hipStream_t s, const BatchDev& bd, int max_ncb);
