@@ -1,24 +1,27 @@
 """Benchmark: pixellated-source log-evidence evaluations per second on one MI355X.
 
-Workload (BASELINE metric shape): 200x200 image at 0.04"/pix with 2x2 supersampling, 60x60
-Cartesian source grid, NFW (Gauss-Patterson quadrature, tol 1e-6) + SIE + shear, Moffat 21x21
-PSF, sigma = 0.01, curvature lambda = 1e3 (qlens_amd.workloads.c2).  The synthetic data are made
-on the GPU by the library's own mock generator.  A step is one qlx_eval call over one batch of
-lens-model proposals drawn from a 1% Gaussian ball around the truth; `value` is evaluations per
-second over the timed steps (inputs resident in HBM).
+Workload: the 200x200-pixel / 60x60-source numerical-profile configuration
+(qlens_amd.workloads.c2): 0.04"/pix with 2x2 supersampling, elliptical NFW (Gauss-Patterson
+quadrature, tol 1e-6) + SIE + shear, three Sersic source components, Moffat 21x21 PSF, sigma =
+0.01, curvature lambda = 1e3.  The data are synthetic, made on the GPU by the library's own mock
+generator from a fixed seed.  A step is one qlx_eval call over one batch of lens-model
+proposals drawn from a seeded 1% Gaussian ball around the truth; inputs (image, PSF, settings)
+are resident in HBM before the timed region.
 
-Paths:
-  plain   FP64 quadrature ray shooting + FP64 inversion, one proposal per call (QLens's own
-          one-proposal-per-likelihood-call pattern)
-  fast    the fastest path available in the library (see `path` in the output)
+Two paths are measured on the same proposals:
+  plain  FP64 everywhere (quadrature ray shooting, FP64 MFMA F and Cholesky), one proposal per
+         call: the reference's one-likelihood-per-call pattern
+  fast   `value`: mixed precision (BF16-split F on MFMA with FP32 accumulation, FP32 Cholesky,
+         FP64 iterative refinement and log-determinants), a batch of proposals per call
+|delta ln E| of the fast path against the plain FP64 path is reported per run (contract < 0.1).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--path plain|fast]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--plain-steps P]
                     [--no-cpu-baseline]
+    python bench.py --copy-calibration MIB   # one device copy of MIB MiB (HBM counter scale)
 """
 import argparse
 import json
 import os
-import subprocess
 import sys
 import tempfile
 import time
@@ -28,45 +31,53 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
-PEAK = {"fp64_mfma_tflops": 78.6, "fp32_tflops": 157.3, "bf16_mfma_tflops": 2500.0, "hbm_gbps": 8000.0}
+# MI355X published peaks (dense)
+PEAK_BF16_TFLOPS = 2500.0
+PEAK_FP64_TFLOPS = 78.6
 METRIC = "log-evidence evals/s (200² img, 60² src): surrogate vs quad, |ΔlogE|<0.1"
 
 
 def proposals(case, n, seed=1234):
+    """Seeded 1% Gaussian ball around the truth: SIE (b, q), NFW (ks, rs, q), shear gamma."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n):
         m = []
         for t, p in case["lenses"]:
             p = list(p)
-            idx = {1: (0, 1), 2: (0, 1, 2), 3: (0,)}[t]  # SIE b,q ; NFW ks,rs,q ; shear gamma
+            idx = {1: (0, 1), 2: (0, 1, 2), 3: (0,)}[t]
             for i in idx:
                 p[i] *= 1 + 0.01 * rng.standard_normal()
             if t in (1, 2):
-                p[1 if t == 1 else 2] = min(p[1 if t == 1 else 2], 0.999)
-            out_p = tuple(p)
-            m.append((t, out_p))
+                k = 1 if t == 1 else 2
+                p[k] = min(p[k], 0.999)
+            m.append((t, tuple(p)))
         out.append(m)
     return out
 
 
-def read_profile_traffic(kernel_tag):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary (profiles/), or None."""
-    path = os.path.join(REPO, "profiles", "%s_traffic.json" % kernel_tag)
+def read_traffic(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE) and scaled by
+    a copy-kernel calibration; stored as 'key: value' lines in profiles/<name>.txt."""
+    path = os.path.join(REPO, "profiles", name + ".txt")
     if not os.path.exists(path):
         return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            if ":" in line and not line.startswith("#"):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = v.strip()
     try:
-        with open(path) as f:
-            d = json.load(f)
-        return d.get("hbm_bytes_per_launch")
-    except Exception:
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
         return None
 
 
-def cpu_baseline(case, data, budget_s=60.0):
-    """Time the compiled QLens reference (oracle/_ref) on one evaluation of the same workload,
-    falling back to the oracle restatement; checker/baseline leg only."""
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+def cpu_baseline(case, data):
+    """The QLens reference compiled from its sources (oracle/_ref), or the oracle's C restatement
+    when the reference build is absent, timed on this host for one evaluation of the workload."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     try:
         from oracle import refrun
         if refrun.have_reference():
@@ -75,33 +86,75 @@ def cpu_baseline(case, data, budget_s=60.0):
                 sc, _ = refrun.run_invert(case, data, d, nrep=1, threads=threads)
                 wall = time.time() - t0
             sec = sc["median_seconds"]
-            return {"value": 1.0 / sec, "unit": "evals/s", "cores": int(sc.get("omp_threads", threads)),
+            return {"value": round(1.0 / sec, 5), "unit": "evals/s", "cores": int(sc.get("omp_threads", threads)),
                     "kind": "reference",
-                    "sample": "1 full evaluation of the same 200^2/60^2 workload by QLens compiled from its "
-                              "sources (-O3 -fopenmp, default dense path), %.1f s wall" % wall,
+                    "sample": "1 evaluation of the same 200^2/60^2 workload (truth model) by QLens built from its "
+                              "sources with -O3 -fopenmp (default dense path): %.2f s in the inversion, %.1f s wall "
+                              "with setup" % (sec, wall),
                     "minus2lnE": sc["minus2lnE"]}
-    except Exception as e:  # pragma: no cover - falls back to the restatement
+    except Exception as e:  # noqa: BLE001 - any harness failure falls back to the restatement
         sys.stderr.write("reference baseline unavailable: %s\n" % e)
-    from oracle import oracle as O
-    t0 = time.time()
-    r, _ = O.invert(case, data, want_arrays=False)
-    sec = time.time() - t0
-    return {"value": 1.0 / sec, "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": "1 full evaluation by the C/OpenMP restatement", "minus2lnE": r["minus2lnE"]}
+    try:
+        from oracle import oracle as O
+        t0 = time.time()
+        r, _ = O.invert(case, data, want_arrays=False)
+        sec = time.time() - t0
+        return {"value": round(1.0 / sec, 5), "unit": "evals/s", "cores": threads, "kind": "port",
+                "sample": "1 evaluation by the oracle's C/OpenMP restatement of the reference path",
+                "minus2lnE": r["minus2lnE"]}
+    except Exception as e:  # noqa: BLE001
+        sys.stderr.write("oracle baseline unavailable: %s\n" % e)
+        return None
+
+
+def timed_run(inv, props, batch, warmup, steps):
+    """Run warmup + steps calls of `batch` proposals; returns (evals/s, ms/step, results of timed
+    calls, SYRK launch times (ms))."""
+    k = 0
+    for _ in range(warmup):
+        inv.invert_image_surface_brightness_map(props[k:k + batch])
+        k += batch
+    # qlx_eval returns after its stream has drained (it reads the results back), so the clock
+    # around the calls brackets completed device work; torch is not loaded in this process (its
+    # own HIP runtime would be a second one next to the library's)
+    results, syrk_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        results += inv.invert_image_surface_brightness_map(props[k:k + batch])
+        k += batch
+        ms, _n = inv.kernel_time("syrk")
+        if ms:
+            syrk_ms.append(ms)
+    el = time.perf_counter() - t0
+    return steps * batch / el, el * 1e3 / steps, results, syrk_ms
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=1)
-    ap.add_argument("--path", default="plain", choices=["plain", "fast"])
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--plain-steps", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run one device-to-device copy of this many MiB (known HBM bytes: it reads and "
+                         "writes that amount once) to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4}))
+        return
     if args.gpus != 1:
-        print("bench.py measures one MI355X: the batched path shards by proposals across GPUs "
-              "(design in DESIGN.md), but this build runs and reports N=1 only.", file=sys.stderr)
+        print("bench.py measures one MI355X only: the batched path would shard proposals across GPUs "
+              "(each GPU its own batches, only scalars returned), but no multi-GPU figure is produced "
+              "by this build; run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     from qlens_amd import qlx
@@ -114,56 +167,77 @@ def main():
     inv.set_psf(case["psf"])
     inv.set_settings(case["nsplit"], case["srcgrid"], case["src_n"], case["regparam"], case["integral_tol"])
     data = inv.mock_image(case["lenses"], case["sources"], case["sigma"], case["seed"])
-    inv.load_workload(case, data)
 
     B = args.batch
-    props = proposals(case, (args.warmup + args.steps) * B)
-    for w in range(args.warmup):
-        inv.invert_image_surface_brightness_map(props[w * B:(w + 1) * B])
-    t0 = time.perf_counter()
-    results = []
-    syrk_ms = []
-    for k in range(args.steps):
-        batch = props[(args.warmup + k) * B:(args.warmup + k + 1) * B]
-        results += inv.invert_image_surface_brightness_map(batch)
-        ms, n = inv.kernel_time("syrk")
-        if ms:
-            syrk_ms.append(ms)
-    elapsed = time.perf_counter() - t0
-    n_eval = args.steps * B
-    value = n_eval / elapsed
-    bad = [r["status"] for r in results if r["status"] != 0]
+    n_fast = (args.warmup + args.steps) * B
+    props = proposals(case, max(n_fast, args.plain_steps + 2))
 
-    stats = qlx.get_stats(inv)
-    syrk_avg_s = float(np.mean(syrk_ms)) * 1e-3 if syrk_ms else None
+    # plain path: FP64, one proposal per call
+    inv.load_workload(case, data, precision=qlx.QLX_PREC_FP64)
+    plain_rate, plain_ms, plain_res, plain_syrk = timed_run(inv, props, 1, 2, args.plain_steps)
+    plain_stats = qlx.get_stats(inv)
+
+    # fast path: mixed precision, B proposals per call
+    inv.load_workload(case, data, precision=qlx.QLX_PREC_MIXED)
+    rate, ms_step, fast_res, syrk_ms = timed_run(inv, props, B, args.warmup, args.steps)
+    st = qlx.get_stats(inv)
+    bad = sorted(set(r["status"] for r in fast_res if r["status"] != 0))
+
+    # accuracy: the fast path's first timed batch against FP64 on the same proposals
+    first = props[args.warmup * B:(args.warmup + 1) * B]
+    inv.load_workload(case, data, precision=qlx.QLX_PREC_FP64)
+    ref = inv.invert_image_surface_brightness_map(first)
+    dlnE = max(abs(f["minus2lnE"] - r["minus2lnE"]) / 2 for f, r in zip(fast_res[:B], ref))
+
+    # roofline of the dominant fast-path kernel: the BF16 SYRK (F = Lb^T C Lb), per launch = one wave
+    syrk_s = float(np.mean(syrk_ms)) * 1e-3 if syrk_ms else None
     roof = None
-    if syrk_avg_s:
-        achieved = stats["syrk_exec_flops"] / syrk_avg_s / 1e12
-        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK["fp64_mfma_tflops"], "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK["fp64_mfma_tflops"], 4), "traffic": read_profile_traffic("syrk_f64"),
-                "kernel": "k_syrk_f64 (F = Lb^T C Lb, FP64 MFMA)",
-                "flops_executed_per_launch": stats["syrk_exec_flops"],
-                "flops_dense_equiv_per_launch": stats["syrk_dense_flops"],
-                "flops_algorithmic_per_launch": stats["syrk_alg_flops"],
-                "avg_launch_ms": round(syrk_avg_s * 1e3, 4)}
+    if syrk_s:
+        achieved = st["syrk_alg_flops"] / syrk_s / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_BF16_TFLOPS, 4),
+                "traffic": read_traffic("syrk_bf16_c2_traffic"),
+                "kernel": "k_syrk_bf16_b (F = Lb^T C Lb: BF16 hi/lo split, FP32 accumulation, 128x128 tiles)",
+                "items_per_launch": st["n_items"],
+                "flops_algorithmic_per_launch": st["syrk_alg_flops"],
+                "flops_executed_per_launch": st["syrk_exec_flops"],
+                "executed_tflops": round(st["syrk_exec_flops"] / syrk_s / 1e12, 2),
+                "avg_launch_ms": round(syrk_s * 1e3, 3)}
+    plain_syrk_s = float(np.mean(plain_syrk)) * 1e-3 if plain_syrk else None
 
     out = {
-        "metric": METRIC, "value": round(value, 4), "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (GPU mock: 3 Sersic sources, seed 10)",
-        "config": {"workload": "C2: 200x200 img 0.04\"/pix nsplit 2, 60x60 src, NFW(quad tol 1e-6)+SIE+shear, "
-                               "Moffat 21x21, lambda 1e3", "batch_per_step": B, "path": args.path,
-                   "n_img_active": stats["n_img"], "n_src_active": stats["n_src"]},
+        "metric": METRIC,
+        "value": round(rate, 2),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic (GPU mock of the workload from a fixed seed; proposals from a seeded 1% ball)",
+        "config": {"workload": "200x200 img 0.04\"/pix nsplit 2, 60x60 source, NFW(quadrature tol 1e-6)+SIE+shear, "
+                               "Moffat 21x21 PSF, lambda 1e3",
+                   "batch": B, "path": "mixed precision + batched (quadrature deflections)",
+                   "n_img_active": st["n_img"], "n_src_active": st["n_src"]},
+        "plain_fp64": {"value": round(plain_rate, 2), "unit": "evals/s", "batch": 1, "ms_per_eval": round(plain_ms, 3),
+                       "syrk_f64_tflops": round(plain_stats["syrk_alg_flops"] / plain_syrk_s / 1e12, 2) if plain_syrk_s else None},
+        "speedup_vs_plain": round(rate / plain_rate, 2),
+        "max_abs_dlnE_vs_fp64": float("%.3g" % dlnE),
+        "item_status_errors": bad,
         "roofline": roof,
-        "status_nonzero": len(bad),
-        "minus2lnE_first": results[0]["minus2lnE"] if results else None,
+        "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(case, data)
-        if "minus2lnE" in out["cpu_baseline"]:
-            truth = inv.invert_image_surface_brightness_map(case["lenses"])[0]["minus2lnE"]
-            out["cpu_baseline"]["gpu_minus2lnE_same_model"] = truth
-    print(json.dumps(out))
+        cb = cpu_baseline(case, data)
+        if cb:
+            cb.pop("minus2lnE", None)
+            out["cpu_baseline"] = cb
+            out["plain_vs_cpu"] = round(plain_rate / cb["value"], 1)
+    print(json.dumps(out), flush=True)
+    inv.close()
 
 
 if __name__ == "__main__":

@@ -598,6 +598,33 @@ __device__ __forceinline__ void tri_decode(int t, int& I, int& J)
     J = t - i * (i + 1) / 2;
 }
 
+// SYRK tile order.  Workgroups are dealt to the 8 XCDs round-robin by their linear index, and
+// each XCD has its own L2.  The lower-triangular tile set is cut into SUPER x SUPER super-tiles
+// (all (I, J) slots of one super-tile are launched, slots above the diagonal or past the matrix
+// exit at once), and the logical order is remapped so that each XCD receives a contiguous range
+// of it: the 64 workgroups resident on an XCD then work on one super-tile and share its SUPER
+// row blocks and SUPER column blocks in that XCD's L2.
+#define SUPER 8
+#define NXCD 8
+static unsigned syrk_grid(int max_ncb)
+{
+    const size_t nS = (max_ncb + SUPER - 1) / SUPER;
+    const size_t tot = nS * (nS + 1) / 2 * SUPER * SUPER;
+    return (unsigned)((tot + NXCD - 1) / NXCD * NXCD);
+}
+
+__device__ __forceinline__ bool syrk_tile(int ncb, int& I, int& J)
+{
+    const unsigned per = gridDim.x / NXCD;
+    const unsigned L = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
+    const int s = (int)(L / (SUPER * SUPER)), w = (int)(L % (SUPER * SUPER));
+    int SI, SJ;
+    tri_decode(s, SI, SJ);
+    I = SI * SUPER + w / SUPER;
+    J = SJ * SUPER + w % SUPER;
+    return I < ncb && J <= I;
+}
+
 // ------------------------------------------------------------------------------------------------
 // F = Lb^T C Lb, FP64 MFMA (v_mfma_f64_16x16x4f64; accumulator register v of lane l holds
 // D[l/16 + 4v][l%16] on gfx950).  One workgroup per lower tile; only chunks holding both blocks.
@@ -609,10 +636,8 @@ __global__ void __launch_bounds__(256) k_syrk_f64_b(BatchDev bd)
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     if (!M.valid) return;
-    const int ntiles = M.ncb * (M.ncb + 1) / 2;
-    if ((int)blockIdx.x >= ntiles) return;
     int I, J;
-    tri_decode(blockIdx.x, I, J);
+    if (!syrk_tile(M.ncb, I, J)) return;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wr = w >> 1, wc = w & 1;
     const double* pool = (const double*)bd.lb + M.lb_off;
@@ -666,8 +691,7 @@ __global__ void __launch_bounds__(256) k_syrk_f64_b(BatchDev bd)
 
 void launch_syrk_f64_b(hipStream_t s, const BatchDev& bd, int max_ncb)
 {
-    const int nt = max_ncb * (max_ncb + 1) / 2;
-    hipLaunchKernelGGL(k_syrk_f64_b, dim3(nt, bd.B), dim3(256), 0, s, bd);
+    hipLaunchKernelGGL(k_syrk_f64_b, dim3(syrk_grid(max_ncb), bd.B), dim3(256), 0, s, bd);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -694,10 +718,8 @@ __global__ void __launch_bounds__(256) k_syrk_bf16_b(BatchDev bd)
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     if (!M.valid) return;
-    const int ntiles = M.ncb * (M.ncb + 1) / 2;
-    if ((int)blockIdx.x >= ntiles) return;
     int I, J;
-    tri_decode(blockIdx.x, I, J);
+    if (!syrk_tile(M.ncb, I, J)) return;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wr = w >> 1, wc = w & 1;
     const size_t bsz = (size_t)TS * SCB;
@@ -793,10 +815,9 @@ __global__ void __launch_bounds__(256) k_syrk_bf16_b(BatchDev bd)
 
 void launch_syrk_bf16_b(hipStream_t s, const BatchDev& bd, int max_ncb)
 {
-    const int nt = max_ncb * (max_ncb + 1) / 2;
     const size_t lds = (size_t)4 * 16 * BP2 * sizeof(uint16_t);
     allow_big_lds((const void*)k_syrk_bf16_b);
-    hipLaunchKernelGGL(k_syrk_bf16_b, dim3(nt, bd.B), dim3(256), lds, s, bd);
+    hipLaunchKernelGGL(k_syrk_bf16_b, dim3(syrk_grid(max_ncb), bd.B), dim3(256), lds, s, bd);
 }
 
 // ------------------------------------------------------------------------------------------------
