@@ -181,6 +181,7 @@ def main():
     inv.load_workload(case, data, precision=qlx.QLX_PREC_MIXED)
     rate, ms_step, fast_res, syrk_ms = timed_run(inv, props, B, args.warmup, args.steps)
     st = qlx.get_stats(inv)
+    stage_ms = {k: round(inv.kernel_time(k)[0] or 0.0, 3) for k in ("raytrace", "lbuild", "syrk", "cholesky", "solve", "total")}
     bad = sorted(set(r["status"] for r in fast_res if r["status"] != 0))
 
     # accuracy: the fast path's first timed batch against FP64 on the same proposals
@@ -224,6 +225,7 @@ def main():
                    "n_img_active": st["n_img"], "n_src_active": st["n_src"]},
         "plain_fp64": {"value": round(plain_rate, 2), "unit": "evals/s", "batch": 1, "ms_per_eval": round(plain_ms, 3),
                        "syrk_f64_tflops": round(plain_stats["syrk_alg_flops"] / plain_syrk_s / 1e12, 2) if plain_syrk_s else None},
+        "fast_stage_ms_per_step": stage_ms,
         "speedup_vs_plain": round(rate / plain_rate, 2),
         "max_abs_dlnE_vs_fp64": float("%.3g" % dlnE),
         "item_status_errors": bad,

@@ -125,6 +125,35 @@ __global__ void __launch_bounds__(1024) k_compact_b(BatchDev bd)
         base += tot;
     }
     if (threadIdx.x == 0) bd.counts[4 * item] = base;
+    __syncthreads();
+    // internal order of F's source columns: the same active cells ranked tile by tile
+    // (QLX_TILE_W x QLX_TILE_H cells, tiles and cells inside a tile y-major), so that a compact
+    // source-plane patch occupies a compact index range
+    {
+        int* tix = bd.tix + item * bd.s_cell;
+        int* pis = bd.pisrc + item * bd.s_cell;
+        const int ntw = (snx + QLX_TILE_W - 1) / QLX_TILE_W, nth = (sny + QLX_TILE_H - 1) / QLX_TILE_H;
+        const int tcells = QLX_TILE_W * QLX_TILE_H;
+        const int total = ntw * nth * tcells;
+        int tbase = 0;
+        for (int t0 = 0; t0 < total; t0 += 1024) {
+            const int t = t0 + threadIdx.x;
+            int c = -1, flag = 0;
+            if (t < total) {
+                const int tile = t / tcells, wv = t - tile * tcells;
+                const int i = (tile % ntw) * QLX_TILE_W + wv % QLX_TILE_W;
+                const int j = (tile / ntw) * QLX_TILE_H + wv / QLX_TILE_W;
+                if (i < snx && j < sny) {
+                    c = i * sny + j;
+                    flag = cm[c] ? 1 : 0;
+                }
+            }
+            int tot, ex = block_excl_scan(flag, sh, tot);
+            if (c >= 0) tix[c] = flag ? tbase + ex : -1;
+            if (flag) pis[tbase + ex] = sidx[c];
+            tbase += tot;
+        }
+    }
     base = 0;
     const int npix = bd.nx * bd.ny;
     for (int t0 = 0; t0 < npix; t0 += 1024) {
@@ -159,7 +188,7 @@ __global__ void __launch_bounds__(256) k_merge_lrows_b(BatchDev bd)
     const long long p = bd.act_pix[item * bd.s_pix + r];
     const int* mcb = bd.map_cell + (size_t)item * bd.s_pix * nsub * 3;
     const double* mwb = bd.map_w + (size_t)item * bd.s_pix * nsub * 3;
-    const int* sidx = bd.sidx + item * bd.s_cell;
+    const int* tix = bd.tix + item * bd.s_cell;
     int* uc = bd.ucol + ((size_t)item * bd.s_pix + r) * U;
     double* uv = bd.uval + ((size_t)item * bd.s_pix + r) * U;
     int cnt = 0;
@@ -168,7 +197,7 @@ __global__ void __launch_bounds__(256) k_merge_lrows_b(BatchDev bd)
         const double* mw = mwb + (p * nsub + k) * 3;
         if (mc[0] < 0) continue;
         for (int e = 0; e < 3; e++) {
-            int col = sidx[mc[e]];
+            int col = tix[mc[e]];
             int q = 0;
             while (q < cnt && uc[q] != col) q++;
             if (q == cnt) {
@@ -200,12 +229,12 @@ __global__ void __launch_bounds__(256) k_row_keys_b(BatchDev bd)
     const int nsub = bd.nsub;
     const long long p = bd.act_pix[item * bd.s_pix + r];
     const int* mcb = bd.map_cell + (size_t)item * bd.s_pix * nsub * 3;
-    const int* sidx = bd.sidx + item * bd.s_cell;
+    const int* tix = bd.tix + item * bd.s_cell;
     int key = 0;
     for (int k = 0; k < nsub; k++) {
         int c0 = mcb[(p * nsub + k) * 3];
         if (c0 >= 0) {
-            key = sidx[c0];
+            key = tix[c0];
             break;
         }
     }
@@ -598,30 +627,33 @@ __device__ __forceinline__ void tri_decode(int t, int& I, int& J)
     J = t - i * (i + 1) / 2;
 }
 
-// SYRK tile order.  Workgroups are dealt to the 8 XCDs round-robin by their linear index, and
-// each XCD has its own L2.  The lower-triangular tile set is cut into SUPER x SUPER super-tiles
-// (all (I, J) slots of one super-tile are launched, slots above the diagonal or past the matrix
-// exit at once), and the logical order is remapped so that each XCD receives a contiguous range
-// of it: the 64 workgroups resident on an XCD then work on one super-tile and share its SUPER
-// row blocks and SUPER column blocks in that XCD's L2.
+// Tile order of the SYRK and Cholesky-update grids.  Workgroups are dealt to the 8 XCDs
+// round-robin by linear index, and each XCD has its own L2.  The lower-triangular tile set is cut
+// into SUPER x SUPER super-tiles (all slots of a super-tile are launched; slots above the
+// diagonal or past the matrix exit at once) and the grid is arranged so that every super-tile
+// lands whole on one XCD: its SUPER^2 workgroups run together there and share SUPER row blocks
+// and SUPER column blocks in that L2.  Super-tiles go to XCDs round-robin, starting at an XCD
+// rotated by the item (blockIdx.y), so small triangles of many items still spread over all XCDs.
 #define SUPER 8
 #define NXCD 8
 static unsigned syrk_grid(int max_ncb)
 {
-    const size_t nS = (max_ncb + SUPER - 1) / SUPER;
-    const size_t tot = nS * (nS + 1) / 2 * SUPER * SUPER;
-    return (unsigned)((tot + NXCD - 1) / NXCD * NXCD);
+    const unsigned nS = (max_ncb + SUPER - 1) / SUPER;
+    const unsigned nsup = nS * (nS + 1) / 2;
+    const unsigned rounds = (nsup + NXCD - 1) / NXCD;
+    return rounds * NXCD * SUPER * SUPER;
 }
 
 __device__ __forceinline__ bool syrk_tile(int ncb, int& I, int& J)
 {
-    const unsigned per = gridDim.x / NXCD;
-    const unsigned L = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
-    const int s = (int)(L / (SUPER * SUPER)), w = (int)(L % (SUPER * SUPER));
+    const unsigned g = blockIdx.x;
+    const unsigned x = g % NXCD, k = g / NXCD;
+    const unsigned w = k % (SUPER * SUPER), r = k / (SUPER * SUPER);
+    const unsigned vx = (x + NXCD - (blockIdx.y % NXCD)) % NXCD;
     int SI, SJ;
-    tri_decode(s, SI, SJ);
-    I = SI * SUPER + w / SUPER;
-    J = SJ * SUPER + w % SUPER;
+    tri_decode((int)(r * NXCD + vx), SI, SJ);
+    I = SI * SUPER + (int)(w / SUPER);
+    J = SJ * SUPER + (int)(w % SUPER);
     return I < ncb && J <= I;
 }
 
@@ -910,6 +942,7 @@ __global__ void __launch_bounds__(256) k_prep_f_b(BatchDev bd)
     const bool mixed = sizeof(T) == 4;
     T* F = mixed ? (T*)(bd.F32 + M.f_off) : (T*)(bd.F + M.f_off);
     double* fdiag = bd.vec + M.vec_off + 3 * (size_t)ld;
+    const int* pis = bd.pisrc + item * bd.s_cell;
     const double lam = bd.lambda;
     for (int e = threadIdx.x; e < TS * TS; e += 256) {
         const int rr = e >> 6, cc = e & 63;
@@ -918,7 +951,10 @@ __global__ void __launch_bounds__(256) k_prep_f_b(BatchDev bd)
         const size_t t = (size_t)r * ld + c;
         T v;
         if (r < n) {
-            const double rv = ((r >> 6) - (c >> 6) <= M.bwb) ? *rtile(bd, M, r, c) : 0.0;
+            // R is held in the reference order: F's internal indices map through pisrc
+            const int a = pis[r], b = pis[c];
+            const int hi = a > b ? a : b, lo = a > b ? b : a;
+            const double rv = ((hi >> 6) - (lo >> 6) <= M.bwb) ? *rtile(bd, M, hi, lo) : 0.0;
             v = (T)((double)F[t] + lam * rv);
         } else if (r == n) {
             v = (c == n && mixed) ? (T)1e30 : F[t];
@@ -1149,9 +1185,8 @@ __global__ void __launch_bounds__(256) k_chol_update_m(const MatDesc* mats, int 
     const MatDesc D = mats[blockIdx.y];
     if (kb >= D.nb) return;
     const int rest = band_last(D, kb) - kb;
-    if ((int)blockIdx.x >= rest * (rest + 1) / 2) return;
     int I, J;
-    tri_decode(blockIdx.x, I, J);
+    if (!syrk_tile(rest, I, J)) return;  // XCD-aware super-tile order: neighbours share panel blocks in L2
     I += kb + 1;
     J += kb + 1;
     int lda, ldb, ldc;
@@ -1172,7 +1207,7 @@ static void chol_step(hipStream_t s, const MatDesc* mats, int nmat, int kb, int 
     hipLaunchKernelGGL(k_chol_diag_m<T>, dim3(1, nmat), dim3(64), 0, s, mats, kb);
     if (rest > 0) {
         hipLaunchKernelGGL(k_chol_panel_m<T>, dim3(rest, nmat), dim3(256), 0, s, mats, kb);
-        hipLaunchKernelGGL(k_chol_update_m<T>, dim3(rest * (rest + 1) / 2, nmat), dim3(256), 0, s, mats, kb);
+        hipLaunchKernelGGL(k_chol_update_m<T>, dim3(syrk_grid(rest), nmat), dim3(256), 0, s, mats, kb);
     }
 }
 
@@ -1466,6 +1501,18 @@ void launch_refine_b(hipStream_t s, const BatchDev& bd, int max_ld, int iters, d
 // evidence: v = L s per active pixel, model = PSF (x) v over active taps, chi^2 over the mask,
 // Es = sum_l ||H_l s||^2 (= s^T R s), -2 ln E = ((0 + regterms) + ln det F) + chi^2
 // ------------------------------------------------------------------------------------------------
+// s in the reference order: s_ref[pisrc[p]] = s[p]
+__global__ void __launch_bounds__(256) k_unpermute_b(BatchDev bd)
+{
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!M.valid || p >= M.n_src) return;
+    const double* s = bd.vec + M.vec_off;
+    double* sref = bd.vec + M.vec_off + 6 * (size_t)M.ldN;
+    sref[bd.pisrc[item * bd.s_cell + p]] = s[p];
+}
+
 __global__ void __launch_bounds__(256) k_Ls_b(BatchDev bd)
 {
     const int item = blockIdx.y;
@@ -1519,7 +1566,7 @@ __global__ void __launch_bounds__(1024) k_evidence_b(BatchDev bd)
     const int* pm = bd.pmapped + item * bd.s_pix;
     const double* model = bd.model + (size_t)item * 2 * bd.s_pix;
     const int* sidx = bd.sidx + item * bd.s_cell;
-    const double* s = bd.vec + M.vec_off;
+    const double* s = bd.vec + M.vec_off + 6 * (size_t)M.ldN;  // reference order
     double chi = 0.0;
     for (int p = threadIdx.x; p < npix; p += 1024) {
         if (bd.mask && !bd.mask[p]) continue;
@@ -1576,9 +1623,10 @@ __global__ void __launch_bounds__(1024) k_evidence_b(BatchDev bd)
     }
 }
 
-void launch_evidence_b(hipStream_t st, const BatchDev& bd, int max_img)
+void launch_evidence_b(hipStream_t st, const BatchDev& bd, int max_img, int max_src)
 {
     launch_fill_f64(st, bd.model, 0.0, (size_t)bd.B * 2 * bd.s_pix);
+    if (max_src > 0) hipLaunchKernelGGL(k_unpermute_b, dim3((max_src + 255) / 256, bd.B), dim3(256), 0, st, bd);
     if (max_img > 0) {
         hipLaunchKernelGGL(k_Ls_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, st, bd);
         hipLaunchKernelGGL(k_model_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, st, bd);

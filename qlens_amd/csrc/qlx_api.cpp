@@ -84,7 +84,7 @@ struct Ctx {
     // per-item inputs
     DevBuf d_models, d_sgrid, d_srcn, d_meta, d_mats;
     // per-item work (stride s_pix / s_cell)
-    DevBuf beta, map_cell, map_w, pmapped, cmapped, sidx, pidx, act_pix, counts, ucol, uval, ucnt, nevals;
+    DevBuf beta, map_cell, map_w, pmapped, cmapped, sidx, tix, pisrc, pidx, act_pix, counts, ucol, uval, ucnt, nevals;
     DevBuf key, bcnt, boff, bfill, perm;
     // pools
     DevBuf lb, F, R, cvec, vec, nz, bbase, winv, model, out, status;
@@ -292,7 +292,8 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
               c->d_meta.ensure(sizeof(ItemMeta) * B) && c->d_mats.ensure(sizeof(MatDesc) * 2 * B) &&
               c->beta.ensure(B * s_pix * nsub * 16) && c->map_cell.ensure(B * s_pix * nsub * 12) &&
               c->map_w.ensure(B * s_pix * nsub * 24) && c->pmapped.ensure(B * s_pix * 4) && c->cmapped.ensure(B * s_cell * 4) &&
-              c->sidx.ensure(B * s_cell * 4) && c->pidx.ensure(B * s_pix * 4) && c->act_pix.ensure(B * s_pix * 4) &&
+              c->sidx.ensure(B * s_cell * 4) && c->tix.ensure(B * s_cell * 4) && c->pisrc.ensure(B * s_cell * 4) &&
+              c->pidx.ensure(B * s_pix * 4) && c->act_pix.ensure(B * s_pix * 4) &&
               c->counts.ensure((size_t)B * 16) && c->ucol.ensure(B * s_pix * U * 4) && c->uval.ensure(B * s_pix * U * 8) &&
               c->ucnt.ensure(B * s_pix * 4) && c->nevals.ensure((size_t)B * 8) && c->key.ensure(B * s_pix * 4) &&
               c->bcnt.ensure(B * (s_cell + 2) * 4) && c->boff.ensure(B * (s_cell + 2) * 4) && c->bfill.ensure(B * (s_cell + 2) * 4) &&
@@ -339,6 +340,8 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     bd.pmapped = c->pmapped.as<int>();
     bd.cmapped = c->cmapped.as<int>();
     bd.sidx = c->sidx.as<int>();
+    bd.tix = c->tix.as<int>();
+    bd.pisrc = c->pisrc.as<int>();
     bd.pidx = c->pidx.as<int>();
     bd.act_pix = c->act_pix.as<int>();
     bd.counts = c->counts.as<int>();
@@ -531,7 +534,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         launch_refine_b(st, bd, max_ld, iters, rtol);
     }
     timer_end(c, "solve");
-    launch_evidence_b(st, bd, max_img);
+    launch_evidence_b(st, bd, max_img, max_src);
     timer_end(c, "total");
 
     std::vector<double> o((size_t)B * QLX_OUT_STRIDE);
@@ -601,7 +604,8 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
             std::vector<double> s(M.n_src);
             std::vector<int> sidx((size_t)M.snx * M.sny);
             c->last_model.resize(s_pix);
-            HIPCHK(c, hipMemcpyAsync(s.data(), bd.vec + M.vec_off, sizeof(double) * M.n_src, hipMemcpyDeviceToHost, st));
+            HIPCHK(c, hipMemcpyAsync(s.data(), bd.vec + M.vec_off + 6 * (size_t)M.ldN, sizeof(double) * M.n_src,
+                                     hipMemcpyDeviceToHost, st));
             HIPCHK(c, hipMemcpyAsync(sidx.data(), bd.sidx + (size_t)lb * s_cell, sizeof(int) * sidx.size(), hipMemcpyDeviceToHost, st));
             HIPCHK(c, hipMemcpyAsync(c->last_model.data(), bd.model + (size_t)lb * 2 * s_pix, sizeof(double) * s_pix,
                                      hipMemcpyDeviceToHost, st));

@@ -24,7 +24,11 @@ namespace qlx {
 #define QLX_SG_SCY 1538
 // per-item output slots
 #define QLX_OUT_STRIDE 8          // -2lnE, chisq, logdetF, logdetR, Es, regterms, refine_resid, refine_steps
-#define QLX_VEC_SLOTS 6           // per item, ldN doubles each: s, diagF, diagR, Fdiag, resid, partial sums
+#define QLX_VEC_SLOTS 7           // per item, ldN doubles each: s (F order), diagF, diagR, Fdiag, resid,
+                                  // partial sums, s (reference order)
+// internal source order: active cells ranked in 8 (x) by 16 (y) cell tiles, tiles y-major
+#define QLX_TILE_W 8
+#define QLX_TILE_H 16
 
 // Per-lens meta-parameters, computed on the host with the host libm exactly as the reference's
 // update_meta_parameters does, so the device evaluates only per-ray arithmetic.
@@ -85,7 +89,9 @@ struct BatchDev {
     double* map_w;              // s_pix*nsub*3
     int* pmapped;               // s_pix
     int* cmapped;               // s_cell
-    int* sidx;                  // s_cell
+    int* sidx;                  // s_cell: reference (j-major) active index of each cell, or -1
+    int* tix;                   // s_cell: internal (tiled) active index of each cell, or -1
+    int* pisrc;                 // s_cell: reference index of internal index p
     int* pidx;                  // s_pix
     int* act_pix;               // s_pix
     int* counts;                // 4 per item: n_src, n_img, support blocks, largest chunk support
@@ -153,7 +159,7 @@ void cholesky_batched(hipStream_t s, const MatDesc* d_mats, int n32, int n64, in
 void launch_logdets_b(hipStream_t s, const BatchDev& bd, int f32);
 void launch_trisolve_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32);
 void launch_refine_b(hipStream_t s, const BatchDev& bd, int max_ld, int iters, double tol);
-void launch_evidence_b(hipStream_t s, const BatchDev& bd, int max_img);
+void launch_evidence_b(hipStream_t s, const BatchDev& bd, int max_img, int max_src);
 void launch_selftest_mfma(hipStream_t s, double* out);
 
 }  // namespace qlx
