@@ -147,27 +147,35 @@ int qlx_mock_image(void *ctx, const qlx_lens_model *model, int nsrc, const doubl
                    double sigma, unsigned long long seed, double *img_out);
 
 /* ---- learned deflection surrogate (replaces the tab/qtab option, models.cpp:4054-5148) ----
- * MLP 4 -> 128 x4 (SiLU) -> 5 over (log r, phi, q, log rs) in the NFW lens frame, predicting
- * (alpha_x, alpha_y, psi_xx, psi_xy, psi_yy) / ks.  BF16 weights, FP32 accumulation. */
+ * MLP 4 -> 128 x4 (SiLU) -> 5 in the NFW lens frame with ks factored out.  Inputs
+ * (ln r, phi, q, ln rs); the five outputs encode (ln J0, ln J1, 2x^2 K0/J0, 2|xy| K1/sqrt(J0 J1),
+ * 2y^2 K2/J1) of the deflection/Hessian integrals (profile.cpp:2532-2560), standardized, so that
+ * (alpha_x, alpha_y, psi_xx, psi_xy, psi_yy)/ks follow exactly.  BF16 weights, FP32 accumulation.
+ * Lenses with def_mode QLX_DEF_SURROGATE use it (one such lens per model); rays outside the
+ * training box (r in [1e-3, 20], q in [0.3, 1], rs in [1, 20]) fall back to quadrature. */
 typedef struct {
     int n_samples;       /* training samples generated by the quadrature kernel */
     int steps;           /* optimizer steps */
     int batch;           /* minibatch size */
-    double lr;           /* Adam learning rate */
-    double r_min, r_max, q_min, q_max, rs_min, rs_max; /* training box */
+    double lr;           /* Adam learning rate (cosine decay to lr/100 over the steps) */
+    double r_min, r_max, q_min, q_max, rs_min, rs_max; /* sampling box */
     unsigned long long seed;
     double tol;          /* quadrature tolerance for the targets */
+    int graph;           /* 1: capture one optimizer step in a HIP graph and replay it */
 } qlx_surrogate_cfg;
 int qlx_surrogate_train(void *ctx, const qlx_surrogate_cfg *cfg, double *final_loss);
 int qlx_surrogate_save(void *ctx, const char *path);
 int qlx_surrogate_load(void *ctx, const char *path);
-/* Forward pass on n lens-frame inputs x[4n] -> y[5n] (FP32 outputs). */
+/* Forward pass on n lens-frame inputs x[4n] = (ln r, phi, q, ln rs) ->
+ * y[5n] = (alpha_x, alpha_y, psi_xx, psi_xy, psi_yy) / ks (FP32). */
 int qlx_surrogate_eval(void *ctx, int n, const float *x, float *y);
-/* One forward+backward on a batch; returns loss and writes the FP32 gradient of every
- * parameter (flattened, see qlx_surrogate_param_count) -- the gradient check entry point. */
+/* Parameters: W1[128][4] b1[128] W2[128][128] b2[128] W3 b3 W4 b4 W5[5][128] b5[5] (FP32). */
 int qlx_surrogate_param_count(void *ctx);
 int qlx_surrogate_get_params(void *ctx, float *params);
 int qlx_surrogate_set_params(void *ctx, const float *params);
+/* One forward+backward of the network itself on a batch: x[4n] are the standardized features
+ * (each in [-1, 1]), y[5n] the standardized targets; writes the mean-squared-error loss over the
+ * n x 5 outputs and its FP32 gradient with respect to every parameter (the gradient check). */
 int qlx_surrogate_grad(void *ctx, int n, const float *x, const float *y, float *loss, float *grad);
 
 /* ---- timing support for benchmarks ---------------------------------------------------------

@@ -93,6 +93,9 @@ struct Ctx {
     // timers
     std::map<std::string, Timer> timers;
     qlx_stats stats{};
+    // learned deflection surrogate (surrogate_host.cpp) and its per-ray outputs
+    void* sur = nullptr;
+    DevBuf sur_alpha;
 };
 
 thread_local std::string g_err;
@@ -151,6 +154,16 @@ size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
 
 namespace qlx {
 
+hipStream_t ctx_stream(void* ctx) { return ((Ctx*)ctx)->stream; }
+int ctx_fail(void* ctx, int code, const char* msg) { return fail((Ctx*)ctx, code, msg); }
+void* ctx_surrogate(void* ctx)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c->sur) c->sur = sur_state_new();
+    return c->sur;
+}
+const SurDev* ctx_surrogate_net(void* ctx) { return sur_state_net(((Ctx*)ctx)->sur); }
+
 void prep_model(const qlx_lens_model& in, ModelPrep& out)
 {
     memset(&out, 0, sizeof(out));
@@ -183,6 +196,7 @@ void prep_model(const qlx_lens_model& in, ModelPrep& out)
             L.xc = s.p[3];
             L.yc = s.p[4];
         } else if (s.type == QLX_LENS_NFW) {
+            if (s.def_mode == QLX_DEF_SURROGATE && out.sur_lens < 0) out.sur_lens = l;
             double q = s.p[2];
             L.ks = s.p[0];
             L.rs = s.p[1];
@@ -220,9 +234,12 @@ int check_model(Ctx* c, const qlx_lens_model* m)
             return fail(c, QLX_ERR_ARG, "NFW needs rs > 0 and q in (0,1]");
         if (m->lens[l].def_mode != QLX_DEF_QUADRATURE && m->lens[l].def_mode != QLX_DEF_SURROGATE)
             return fail(c, QLX_ERR_ARG, "unknown deflection mode");
-        if (t == QLX_LENS_NFW && m->lens[l].def_mode == QLX_DEF_SURROGATE)
-            return fail(c, QLX_ERR_STATE, "surrogate deflections are not available in this build");
     }
+    int nsur = 0;
+    for (int l = 0; l < m->nlens; l++)
+        if (m->lens[l].type == QLX_LENS_NFW && m->lens[l].def_mode == QLX_DEF_SURROGATE) nsur++;
+    if (nsur > 1) return fail(c, QLX_ERR_ARG, "at most one surrogate lens per model");
+    if (nsur && !ctx_surrogate_net(c)) return fail(c, QLX_ERR_STATE, "surrogate deflection requested but no surrogate is trained or loaded");
     return QLX_OK;
 }
 
@@ -368,6 +385,17 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     launch_fill_f64(st, bd.out, 0.0, (size_t)B * QLX_OUT_STRIDE);
 
     timer_begin(c, "total");
+    bool any_sur = false;
+    for (int b = 0; b < B; b++) any_sur = any_sur || mp[b].sur_lens >= 0;
+    if (any_sur) {
+        const SurDev* net = ctx_surrogate_net(c);
+        if (!net) return fail(c, QLX_ERR_STATE, "surrogate not trained or loaded");
+        if (!c->sur_alpha.ensure((size_t)B * s_pix * nsub * 2 * sizeof(float))) return fail(c, QLX_ERR_NOMEM, "surrogate buffer allocation failed");
+        bd.sur_alpha = c->sur_alpha.as<float>();
+        timer_begin(c, "surrogate");
+        launch_sur_rays(st, *net, bd, c->sur_alpha.as<float>());
+        timer_end(c, "surrogate");
+    }
     timer_begin(c, "raytrace");
     launch_raytrace_map_b(st, bd, S.ray_precision == QLX_RAY_FP32);
     timer_end(c, "raytrace");
@@ -710,6 +738,7 @@ int qlx_ctx_destroy(void* ctx)
         if (kv.second.b) hipEventDestroy(kv.second.b);
     }
     hipStreamSynchronize(c->stream2);
+    if (c->sur) sur_state_destroy(c->sur);
     hipEventDestroy(c->ev_fork);
     hipEventDestroy(c->ev_join);
     hipStreamDestroy(c->stream2);

@@ -142,6 +142,62 @@ void launch_mock_sb(hipStream_t stream, int nx, int ny, int nsplit, const double
                     double tol, int nsrc, const double* sersic, double* sb);
 void launch_conv_image(hipStream_t stream, int nx, int ny, const double* psf, int pnx, int pny, const double* in, double* out);
 
+// ---- surrogate.hip (learned NFW deflection; host driver in surrogate_host.cpp) ----
+struct SurParams {            // device state of one network
+    float* P;                 // FP32 master parameters (QLX_SUR_NPARAM)
+    float* m;                 // Adam moments
+    float* v;
+    uint16_t* Wb;             // BF16 W2, W3, W4 ([out][in], 128x128 each)
+    uint16_t* WTb;            // their transposes
+    float* W1r;               // BF16-rounded W1 [128][4] and W5 [5][128] as floats
+    float* W5r;
+    float* norm;              // output standardization: mean[5], std[5]
+    int* step;                // Adam step counter
+};
+struct SurWork {              // per-minibatch buffers of the training kernels
+    float* zs;                // pre-activations of the 4 hidden layers, [n][4*128]
+    uint16_t* hs;             // BF16 activations of the 4 hidden layers, [n][4*128]
+    uint16_t* dzA;            // BF16 dz of a hidden layer, [n][128] (two buffers)
+    uint16_t* dzB;
+    float* dz1;               // FP32 dz of the first hidden layer, [n][128]
+    float* loss_part;         // per workgroup
+    float* g5_part;           // per workgroup: W5, b5 gradients (645)
+    float* gW_part;           // [3][nwg][128*128]
+    float* gb_part;           // [3][nwg][128]
+    float* g1_part;           // per workgroup: W1 (512), b1 (128)
+    float* grad;              // reduced gradient (QLX_SUR_NPARAM)
+    float* loss;              // reduced loss
+};
+struct SurDev {               // what the inference and training kernels read
+    const float* P;
+    const uint16_t* Wb;
+    const uint16_t* WTb;
+    const float* W1r;
+    const float* W5r;
+    const float* norm;
+};
+#define QLX_SUR_NPARAM 50821
+#define QLX_SUR_SB 64         // samples per workgroup of the surrogate kernels
+SurDev sur_dev(const SurParams& sp);
+void launch_sur_eval(hipStream_t s, const SurDev& net, int n, const float* in4, float* out5);
+void launch_sur_rays(hipStream_t s, const SurDev& net, const BatchDev& bd, float* sur_alpha);
+void launch_sur_gather(hipStream_t s, int n, int nsamp, unsigned long long seed, const int* step, const float* feat, const float* targ,
+                       float* bf, float* bt);
+void launch_sur_fwdbwd(hipStream_t s, const SurDev& net, int n, const float* bf, const float* bt, SurWork& w);
+void launch_sur_adam(hipStream_t s, SurParams& sp, const float* grad, int total_steps, float lr0, float lr_min);
+void launch_sur_refresh(hipStream_t s, SurParams& sp);
+void launch_sur_targets(hipStream_t s, int n, const double* in4, const double* j5, float* feat, float* targ);
+void launch_nfw_samples(hipStream_t stream, int n, const double* in4, double tol, double* out5);
+
+// context services used by the surrogate driver (qlx_api.cpp)
+hipStream_t ctx_stream(void* ctx);
+int ctx_fail(void* ctx, int code, const char* msg);
+void* ctx_surrogate(void* ctx);                  // the context's surrogate state (created on demand)
+const SurDev* ctx_surrogate_net(void* ctx);      // trained network, or nullptr
+void* sur_state_new();
+void sur_state_destroy(void* state);
+const SurDev* sur_state_net(void* state);
+
 // ---- batch.hip ----
 void launch_fill_i32(hipStream_t s, int* p, int v, size_t n);
 void launch_fill_f64(hipStream_t s, double* p, double v, size_t n);

@@ -308,8 +308,9 @@ __global__ void __launch_bounds__(256) k_raytrace_map_b(BatchDev bd, T tol)
     int nev = 0;
     for (int l = 0; l < m.nlens; l++) {
         T d0, d1;
-        if (l == m.sur_lens && bd.sur_alpha) {
-            d0 = (T)bd.sur_alpha[ro * 2];
+        const float sa0 = (l == m.sur_lens && bd.sur_alpha) ? bd.sur_alpha[ro * 2] : __int_as_float(0x7fc00000);
+        if (sa0 == sa0) {  // surrogate value present (NaN marks rays outside its training box)
+            d0 = (T)sa0;
             d1 = (T)bd.sur_alpha[ro * 2 + 1];
         } else {
             lens_deflection<T>(m.lens[l], (T)sx, (T)sy, tol, d0, d1, nev);
@@ -397,6 +398,37 @@ __global__ void __launch_bounds__(256) k_nfw_def_hess(LensPrep L, double tol, in
 void launch_nfw_def_hess(hipStream_t stream, const LensPrep& L, double tol, int n, const double* xy, double* out5)
 {
     hipLaunchKernelGGL(k_nfw_def_hess, dim3((n + 255) / 256), dim3(256), 0, stream, L, tol, n, xy, out5);
+}
+
+// Training targets of the deflection surrogate: per sample (x, y, q, rs) in the lens frame with
+// ks = 1, the integrals J0, J1, K0, K1, K2 of LensProfile::deflection_and_hessian_numerical
+// (profile.cpp:2532-2560; k_integral 2735-2764), by the same Patterson quadrature at `tol`.
+__global__ void __launch_bounds__(256) k_nfw_samples(int n, const double* in4, double tol, double* out5)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double x = in4[4 * t], y = in4[4 * t + 1], q = in4[4 * t + 2], rs = in4[4 * t + 3];
+    LensPrep L;
+    L.type = QLX_LENS_NFW;
+    L.ks = 1.0;
+    L.rs = rs;
+    L.rs2 = rs * rs;
+    L.q = q;
+    L.eps = 1 - q * q;
+    L.fsqinv = q;
+    L.sqrt1meps = sqrt(1 - L.eps);
+    double j0, j1;
+    nfw_j01<double>(L, x, y, tol, j0, j1);
+    out5[5 * t + 0] = j0;
+    out5[5 * t + 1] = j1;
+    out5[5 * t + 2] = patterson_k<double>(L, x, y, tol, 0.5);
+    out5[5 * t + 3] = patterson_k<double>(L, x, y, tol, 1.5);
+    out5[5 * t + 4] = patterson_k<double>(L, x, y, tol, 2.5);
+}
+
+void launch_nfw_samples(hipStream_t stream, int n, const double* in4, double tol, double* out5)
+{
+    if (n > 0) hipLaunchKernelGGL(k_nfw_samples, dim3((n + 255) / 256), dim3(256), 0, stream, n, in4, tol, out5);
 }
 
 // ---- mock surface brightness (find_surface_brightness split branch, pixelgrid.cpp:11835-11896;

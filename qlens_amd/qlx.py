@@ -53,7 +53,8 @@ class QlxResult(C.Structure):
 class QlxSurrogateCfg(C.Structure):
     _fields_ = [("n_samples", C.c_int), ("steps", C.c_int), ("batch", C.c_int), ("lr", C.c_double),
                 ("r_min", C.c_double), ("r_max", C.c_double), ("q_min", C.c_double), ("q_max", C.c_double),
-                ("rs_min", C.c_double), ("rs_max", C.c_double), ("seed", C.c_ulonglong), ("tol", C.c_double)]
+                ("rs_min", C.c_double), ("rs_max", C.c_double), ("seed", C.c_ulonglong), ("tol", C.c_double),
+                ("graph", C.c_int)]
 
 
 _LIB = None
@@ -219,6 +220,51 @@ class PixelInversion:
         self._check(self.lib.qlx_mock_image(self.ctx, C.byref(m), len(s), _dp(s), C.c_double(sigma),
                                             C.c_ulonglong(seed), _dp(out)), "qlx_mock_image")
         return out
+
+    # -- learned deflection surrogate -----------------------------------------------------------
+    def surrogate_train(self, n_samples=2000000, steps=4000, batch=8192, lr=2e-3, r_box=(1e-3, 20.0), q_box=(0.3, 1.0),
+                        rs_box=(1.0, 20.0), seed=7, tol=1e-8, graph=True):
+        """Generate quadrature targets on the GPU and train the network there; returns the final
+        minibatch loss (standardized outputs)."""
+        cfg = QlxSurrogateCfg(int(n_samples), int(steps), int(batch), float(lr), r_box[0], r_box[1], q_box[0], q_box[1],
+                              rs_box[0], rs_box[1], int(seed), float(tol), 1 if graph else 0)
+        loss = C.c_double(0)
+        self._check(self.lib.qlx_surrogate_train(self.ctx, C.byref(cfg), C.byref(loss)), "qlx_surrogate_train")
+        return loss.value
+
+    def surrogate_eval(self, x4):
+        """(ln r, phi, q, ln rs)[n, 4] in the NFW lens frame -> (alpha_x, alpha_y, psi_xx, psi_xy, psi_yy) / ks."""
+        x = np.ascontiguousarray(x4, dtype=np.float32).reshape(-1, 4)
+        y = np.zeros((len(x), 5), dtype=np.float32)
+        self._check(self.lib.qlx_surrogate_eval(self.ctx, len(x), x.ctypes.data_as(C.POINTER(C.c_float)),
+                                                y.ctypes.data_as(C.POINTER(C.c_float))), "qlx_surrogate_eval")
+        return y
+
+    def surrogate_grad(self, X, T):
+        """Loss and gradient of the network on standardized features X[n, 4] and targets T[n, 5]."""
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 4)
+        T = np.ascontiguousarray(T, dtype=np.float32).reshape(-1, 5)
+        g = np.zeros(self.lib.qlx_surrogate_param_count(self.ctx), dtype=np.float32)
+        loss = C.c_float(0)
+        self._check(self.lib.qlx_surrogate_grad(self.ctx, len(X), X.ctypes.data_as(C.POINTER(C.c_float)),
+                                                T.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss),
+                                                g.ctypes.data_as(C.POINTER(C.c_float))), "qlx_surrogate_grad")
+        return loss.value, g
+
+    def surrogate_params(self):
+        p = np.zeros(self.lib.qlx_surrogate_param_count(self.ctx), dtype=np.float32)
+        self._check(self.lib.qlx_surrogate_get_params(self.ctx, p.ctypes.data_as(C.POINTER(C.c_float))), "qlx_surrogate_get_params")
+        return p
+
+    def set_surrogate_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(self.lib.qlx_surrogate_set_params(self.ctx, p.ctypes.data_as(C.POINTER(C.c_float))), "qlx_surrogate_set_params")
+
+    def surrogate_save(self, path):
+        self._check(self.lib.qlx_surrogate_save(self.ctx, str(path).encode()), "qlx_surrogate_save")
+
+    def surrogate_load(self, path):
+        self._check(self.lib.qlx_surrogate_load(self.ctx, str(path).encode()), "qlx_surrogate_load")
 
     def kernel_time(self, name):
         ms = C.c_double(0)

@@ -111,4 +111,20 @@ def c2_small():
     return c2(n_img=60, pix=0.1333333333333333, n_src=24, src_half=1.0, psf_n=7)
 
 
-WORKLOADS = {"C1": c1, "C2": c2, "C2s": c2_small}
+def c3(n_img=400, nsplit=4):
+    """Surrogate evaluation geometry: the numerical-profile lens of c2 (NFW + SIE + shear) over
+    c1's 5"x5" field at 400x400 pixels with 4x4 supersampling (2.56 M rays), c1's source and
+    30x30 source grid, c1's Gaussian PSF sampled at the finer pixel scale."""
+    xv = corners(-2.5, 2.5, n_img)
+    pix = 5.0 / n_img
+    base = c1()
+    return dict(
+        name="C3", xvals=xv, yvals=xv.copy(), nsplit=nsplit,
+        lenses=c2()["lenses"], sources=base["sources"],
+        psf=gaussian_psf(0.1 * FWHM_TO_SIGMA, 0.02, pix, pix),
+        sigma=0.01, seed=10, srcgrid=(-0.6, 0.6, -0.6, 0.6), src_n=(30, 30),
+        regparam=1e3, integral_tol=1e-6,
+    )
+
+
+WORKLOADS = {"C1": c1, "C2": c2, "C2s": c2_small, "C3": c3}

@@ -159,3 +159,30 @@ def test_zero_source_pixels_sentinel():
     case["srcgrid"] = (50.0, 51.0, 50.0, 51.0)  # far from every ray
     res, _ = O.invert(case, np.zeros((100, 100)), want_arrays=False)
     assert res["minus2lnE"] == 2e30 and res["n_src"] == 0
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    """Pins the oracle's back-propagation: exact float64 network against central differences."""
+    from oracle import mlp
+    rng = np.random.default_rng(5)
+    P = rng.standard_normal(mlp.NPARAM) * 0.15
+    X = rng.uniform(-1, 1, size=(6, 4))
+    T = rng.standard_normal((6, 5))
+    loss, g = mlp.loss_grad(P, X, T, emulate=False)
+    h = 1e-6
+    for idx in list(rng.choice(mlp.NPARAM, 40, replace=False)) + [0, 511, 512, 50175, 50820]:
+        Pp, Pm = P.copy(), P.copy()
+        Pp[idx] += h
+        Pm[idx] -= h
+        fd = (mlp.loss_grad(Pp, X, T, emulate=False)[0] - mlp.loss_grad(Pm, X, T, emulate=False)[0]) / (2 * h)
+        assert abs(fd - g[idx]) <= 1e-7 + 1e-6 * abs(fd), (idx, fd, g[idx])
+
+
+def test_mlp_oracle_bf16_rounding():
+    from oracle import mlp
+    # bf16 spacing at 1 is 2^-7: 1 + 2^-8 and 1 + 3*2^-8 are ties (to the even neighbours 1 and
+    # 1 + 2^-6), 1 + 2^-9 rounds down, 1 + 3*2^-9 rounds up
+    x = np.array([1.0, 1.00390625, 1.01171875, 1.001953125, 1.005859375, -3.1415926])
+    r = mlp.bf16(x)
+    assert list(r[:5]) == [1.0, 1.0, 1.015625, 1.0, 1.0078125]
+    assert r[5] == -3.140625
