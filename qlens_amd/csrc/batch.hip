@@ -1201,6 +1201,103 @@ __global__ void __launch_bounds__(256) k_chol_update_m(const MatDesc* mats, int 
     store64<T>(C, ldc, acc, I == J ? 2 : 1);
 }
 
+// rank-64 update from column block kb of the tiles (I, J), kb < J < kend, J <= I < nb: the
+// columns of the current 256-wide panel that are still to be factored (dense matrices only)
+template <typename T>
+__global__ void __launch_bounds__(256) k_chol_upanel_m(const MatDesc* mats, int kb, int kend)
+{
+    __shared__ T As[TS][LDT];
+    __shared__ T Bs[TS][LDT];
+    const MatDesc D = mats[blockIdx.y];
+    if (kb >= D.nb) return;
+    const int je = min(kend, D.nb);
+    int t = blockIdx.x, I = -1, J = -1;
+    for (int j = kb + 1; j < je; j++) {
+        const int cnt = D.nb - j;
+        if (t < cnt) {
+            J = j;
+            I = j + t;
+            break;
+        }
+        t -= cnt;
+    }
+    if (J < 0) return;
+    int lda, ldb, ldc;
+    const T* Pa = tile_ptr<T>(D, I, kb, lda);
+    const T* Pb = tile_ptr<T>(D, J, kb, ldb);
+    T* C = tile_ptr<T>(D, I, J, ldc);
+    stage64<T>(Pa, lda, As);
+    stage64<T>(Pb, ldb, Bs);
+    __syncthreads();
+    typename Mfma16<T>::acc_t acc[2][2];
+    mma64<T>(As, Bs, acc);
+    store64<T>(C, ldc, acc, I == J ? 2 : 1);
+}
+
+// Trailing update after a 256-wide panel: C(I, J) -= sum over the panel's nq column blocks of
+// L(I, q) L(J, q)^T, on 128 x 128 tiles of the trailing matrix (first row block k0 + nq, XCD-aware
+// tile order).  Wave (wr, wc) owns the 64 x 64 quadrant (wr, wc): 4 x 4 MFMA tiles of 16 x 16.
+#define TT 128
+template <typename T>
+__global__ void __launch_bounds__(256) k_chol_trail_m(const MatDesc* mats, int k0, int nq)
+{
+    extern __shared__ double trail_raw[];
+    T(*As)[LDT] = (T(*)[LDT])trail_raw;
+    T(*Bs)[LDT] = (T(*)[LDT])((T*)trail_raw + TT * LDT);
+    const MatDesc D = mats[blockIdx.y];
+    const int r0b = k0 + nq;
+    if (r0b >= D.nb) return;
+    const int nt = (D.nb - r0b) / 2;
+    int I2, J2;
+    if (!syrk_tile(nt, I2, J2)) return;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    const long long ld = D.ld;
+    T* A = (T*)D.A;
+    const long long rI = (long long)(r0b + 2 * I2) * TS, rJ = (long long)(r0b + 2 * J2) * TS;
+    typename Mfma16<T>::acc_t acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[a][c] = (typename Mfma16<T>::acc_t){0, 0, 0, 0};
+    typedef typename Mfma16<T>::vec_t V;
+    constexpr int VW = sizeof(V) / sizeof(T);
+    for (int q = 0; q < nq; q++) {
+        const long long c0 = (long long)(k0 + q) * TS;
+        __syncthreads();
+        for (int e = tid * VW; e < TT * TS; e += 256 * VW) {
+            const int i = e >> 6, k = e & 63;
+            *(V*)&As[i][k] = *(const V*)&A[(rI + i) * ld + c0 + k];
+            *(V*)&Bs[i][k] = *(const V*)&A[(rJ + i) * ld + c0 + k];
+        }
+        __syncthreads();
+#pragma unroll 2
+        for (int kk = 0; kk < TS; kk += 4) {
+            const int kq = kk + (lane >> 4);
+            T av[4], bv[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) av[a] = As[wr * 64 + a * 16 + (lane & 15)][kq];
+#pragma unroll
+            for (int c = 0; c < 4; c++) bv[c] = Bs[wc * 64 + c * 16 + (lane & 15)][kq];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) acc[a][c] = Mfma16<T>::op(av[a], bv[c], acc[a][c]);
+        }
+    }
+    const bool diag = (I2 == J2);
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int rr = wr * 64 + a * 16 + Mfma16<T>::row(lane, v);
+                const int cc = wc * 64 + c * 16 + (lane & 15);
+                if (!diag || cc <= rr) A[(rI + rr) * ld + rJ + cc] -= acc[a][c][v];
+            }
+}
+
 template <typename T>
 static void chol_step(hipStream_t s, const MatDesc* mats, int nmat, int kb, int rest)
 {
@@ -1211,9 +1308,40 @@ static void chol_step(hipStream_t s, const MatDesc* mats, int nmat, int kb, int 
     }
 }
 
-// d_mats holds n32 FP32 matrices first, then n64 FP64 matrices
-void cholesky_batched(hipStream_t s, const MatDesc* d_mats, int n32, int n64, int max_nb)
+// dense matrices (padded to a multiple of 128): 256-wide panels factored with the 64-block
+// kernels and a panel-restricted update, then one rank-256 trailing update per panel
+template <typename T>
+static void chol_two_level(hipStream_t s, const MatDesc* mats, int nmat, int max_nb)
 {
+    const size_t lds = (size_t)2 * TT * LDT * sizeof(T);
+    allow_big_lds((const void*)k_chol_trail_m<T>);
+    for (int k0 = 0; k0 < max_nb; k0 += 4) {
+        const int kend = k0 + 4;
+        for (int kb = k0; kb < kend && kb < max_nb; kb++) {
+            const int rest = max_nb - kb - 1;
+            hipLaunchKernelGGL(k_chol_diag_m<T>, dim3(1, nmat), dim3(64), 0, s, mats, kb);
+            if (rest <= 0) continue;
+            hipLaunchKernelGGL(k_chol_panel_m<T>, dim3(rest, nmat), dim3(256), 0, s, mats, kb);
+            int cnt = 0;
+            for (int j = kb + 1; j < kend && j < max_nb; j++) cnt += max_nb - j;
+            if (cnt > 0) hipLaunchKernelGGL(k_chol_upanel_m<T>, dim3(cnt, nmat), dim3(256), 0, s, mats, kb, kend);
+        }
+        if (kend < max_nb) {
+            const int nt = (max_nb - kend) / 2;
+            hipLaunchKernelGGL(k_chol_trail_m<T>, dim3(syrk_grid(nt), nmat), dim3(256), lds, s, mats, k0, 4);
+        }
+    }
+}
+
+// d_mats holds n32 FP32 matrices first, then n64 FP64 matrices.  Dense matrices (all of the
+// launch) use the two-level schedule; banded ones the 64-block right-looking one.
+void cholesky_batched(hipStream_t s, const MatDesc* d_mats, int n32, int n64, int max_nb, int dense)
+{
+    if (dense) {
+        if (n32) chol_two_level<float>(s, d_mats, n32, max_nb);
+        if (n64) chol_two_level<double>(s, d_mats + n32, n64, max_nb);
+        return;
+    }
     for (int kb = 0; kb < max_nb; kb++) {
         const int rest = max_nb - kb - 1;
         if (n32) chol_step<float>(s, d_mats, n32, kb, rest);

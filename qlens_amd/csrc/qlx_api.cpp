@@ -269,7 +269,7 @@ size_t stage1_bytes(size_t s_pix, size_t s_cell, size_t nsub)
 // upper bound of the dense bytes of one item with at most ncell active source cells
 size_t dense_bytes_bound(size_t ncell, size_t snx, bool mixed)
 {
-    const size_t ldN = round_up(ncell + 1, 64), nb = ldN / 64;
+    const size_t ldN = round_up(ncell + 1, 128), nb = ldN / 64;
     const size_t bwb = (2 * snx) / 64 + 1;
     const size_t f = ldN * ldN * (mixed ? 4 : 8);
     const size_t r = nb * (bwb + 1) * 4096 * 8;
@@ -431,7 +431,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         }
         M.valid = 1;
         M.cbs = mixed ? 7 : 6;  // Lb column blocks of 128 (mixed) or 64 (FP64) source pixels
-        M.ldN = (int)round_up((size_t)n_src + 1, (size_t)1 << M.cbs);
+        M.ldN = (int)round_up((size_t)n_src + 1, 128);  // 128: whole tiles of the trailing Cholesky update
         M.nrows = (int)std::max<size_t>(64, round_up((size_t)n_img, 64));
         M.ncb = M.ldN >> M.cbs;
         M.ncbw = (M.ncb + 31) / 32;
@@ -547,10 +547,10 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     const int nF = (int)fm.size(), nR = (int)rm.size();
     HIPCHK(c, hipEventRecord(c->ev_fork, st));
     HIPCHK(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
-    cholesky_batched(c->stream2, c->d_mats.as<MatDesc>() + nF, 0, nR, max_ld / 64);
+    cholesky_batched(c->stream2, c->d_mats.as<MatDesc>() + nF, 0, nR, max_ld / 64, 0);
     HIPCHK(c, hipEventRecord(c->ev_join, c->stream2));
     timer_begin(c, "cholesky");
-    cholesky_batched(st, c->d_mats.as<MatDesc>(), mixed ? nF : 0, mixed ? 0 : nF, max_ld / 64);
+    cholesky_batched(st, c->d_mats.as<MatDesc>(), mixed ? nF : 0, mixed ? 0 : nF, max_ld / 64, 1);
     timer_end(c, "cholesky");
     HIPCHK(c, hipStreamWaitEvent(st, c->ev_join, 0));
     launch_logdets_b(st, bd, mixed ? 1 : 0);
