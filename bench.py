@@ -138,6 +138,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-deflection leg")
+    ap.add_argument("--ray-fp32", action="store_true", help="FP32 ray shooting in the fast path")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (known HBM bytes: it reads and "
                          "writes that amount once) to scale rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -178,8 +179,10 @@ def main():
     plain_rate, plain_ms, plain_res, plain_syrk = timed_run(inv, props, 1, 2, args.plain_steps)
     plain_stats = qlx.get_stats(inv)
 
-    # fast path: mixed precision, B proposals per call
-    inv.load_workload(case, data, precision=qlx.QLX_PREC_MIXED)
+    # fast path: mixed precision, B proposals per call.  FP64 ray shooting: the FP32 quadrature
+    # (--ray-fp32) needs more Patterson levels at tol 1e-6 and is slower on this GPU
+    ray = qlx.QLX_RAY_FP32 if args.ray_fp32 else qlx.QLX_RAY_FP64
+    inv.load_workload(case, data, precision=qlx.QLX_PREC_MIXED, ray_precision=ray)
     rate, ms_step, fast_res, syrk_ms = timed_run(inv, props, B, args.warmup, args.steps)
     st = qlx.get_stats(inv)
     stage_ms = {k: round(inv.kernel_time(k)[0] or 0.0, 3) for k in ("raytrace", "lbuild", "syrk", "cholesky", "solve", "total")}
@@ -198,7 +201,7 @@ def main():
         t0 = time.time()
         sur_loss = inv.surrogate_train(n_samples=1000000, steps=12000, batch=8192, lr=2e-3, seed=7)
         train_s = time.time() - t0
-        inv.load_workload(case, data, precision=qlx.QLX_PREC_MIXED)
+        inv.load_workload(case, data, precision=qlx.QLX_PREC_MIXED, ray_precision=ray)
         inv.invert_image_surface_brightness_map(first, def_mode=qlx.QLX_DEF_SURROGATE)
         t0 = time.perf_counter()
         sres = inv.invert_image_surface_brightness_map(first, def_mode=qlx.QLX_DEF_SURROGATE)
@@ -242,8 +245,8 @@ def main():
         "data": "synthetic (GPU mock of the workload from a fixed seed; proposals from a seeded 1% ball)",
         "config": {"workload": "200x200 img 0.04\"/pix nsplit 2, 60x60 source, NFW(quadrature tol 1e-6)+SIE+shear, "
                                "Moffat 21x21 PSF, lambda 1e3",
-                   "batch": B, "path": "mixed precision + batched (quadrature deflections; the surrogate leg is "
-                                       "reported beside it)",
+                   "batch": B, "path": "mixed precision + batched (FP64 quadrature deflections; the surrogate "
+                                       "leg is reported beside it)",
                    "n_img_active": st["n_img"], "n_src_active": st["n_src"]},
         "plain_fp64": {"value": round(plain_rate, 2), "unit": "evals/s", "batch": 1, "ms_per_eval": round(plain_ms, 3),
                        "syrk_f64_tflops": round(plain_stats["syrk_alg_flops"] / plain_syrk_s / 1e12, 2) if plain_syrk_s else None},
