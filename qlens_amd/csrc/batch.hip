@@ -636,8 +636,11 @@ __device__ __forceinline__ void tri_decode(int t, int& I, int& J)
 // rotated by the item (blockIdx.y), so small triangles of many items still spread over all XCDs.
 #define SUPER 8
 #define NXCD 8
+// Extents below one super-tile get a plain triangular grid (g -> tri_decode(g)); larger ones whole
+// super-tile rounds.  syrk_tile tells the two apart by the grid size.
 static unsigned syrk_grid(int max_ncb)
 {
+    if (max_ncb < SUPER) return (unsigned)std::max(1, max_ncb * (max_ncb + 1) / 2);
     const unsigned nS = (max_ncb + SUPER - 1) / SUPER;
     const unsigned nsup = nS * (nS + 1) / 2;
     const unsigned rounds = (nsup + NXCD - 1) / NXCD;
@@ -647,6 +650,10 @@ static unsigned syrk_grid(int max_ncb)
 __device__ __forceinline__ bool syrk_tile(int ncb, int& I, int& J)
 {
     const unsigned g = blockIdx.x;
+    if (gridDim.x < NXCD * SUPER * SUPER) {
+        tri_decode((int)g, I, J);
+        return I < ncb;
+    }
     const unsigned x = g % NXCD, k = g / NXCD;
     const unsigned w = k % (SUPER * SUPER), r = k / (SUPER * SUPER);
     const unsigned vx = (x + NXCD - (blockIdx.y % NXCD)) % NXCD;
@@ -1238,8 +1245,11 @@ __global__ void __launch_bounds__(256) k_chol_upanel_m(const MatDesc* mats, int 
 // L(I, q) L(J, q)^T, on 128 x 128 tiles of the trailing matrix (first row block k0 + nq, XCD-aware
 // tile order).  Wave (wr, wc) owns the 64 x 64 quadrant (wr, wc): 4 x 4 MFMA tiles of 16 x 16.
 #define TT 128
+// Two 128 x 68 staging tiles per WG (70 KB in FP32) allow two WGs per CU, so two waves per SIMD
+// is all the occupancy this kernel gets: the register budget is sized for that.
 template <typename T>
-__global__ void __launch_bounds__(256) k_chol_trail_m(const MatDesc* mats, int k0, int nq)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+k_chol_trail_m(const MatDesc* mats, int k0, int nq)
 {
     extern __shared__ double trail_raw[];
     T(*As)[LDT] = (T(*)[LDT])trail_raw;
@@ -1262,15 +1272,42 @@ __global__ void __launch_bounds__(256) k_chol_trail_m(const MatDesc* mats, int k
         for (int c = 0; c < 4; c++) acc[a][c] = (typename Mfma16<T>::acc_t){0, 0, 0, 0};
     typedef typename Mfma16<T>::vec_t V;
     constexpr int VW = sizeof(V) / sizeof(T);
+    constexpr int NPF = (sizeof(T) == 4) ? (TT * TS) / (256 * VW) : 1;  // pieces per thread and matrix
+    typedef T P __attribute__((ext_vector_type(VW)));
+    P pa[NPF], pb[NPF];
+    // FP32: the next column block is loaded into registers while the current one is multiplied
+#define TRAIL_LOAD(q)                                                    \
+    {                                                                    \
+        const long long c0 = (long long)(k0 + (q)) * TS;                 \
+        _Pragma("unroll") for (int h = 0; h < NPF; h++)                  \
+        {                                                                \
+            const int e = (tid + 256 * h) * VW;                          \
+            const int i = e >> 6, k = e & 63;                            \
+            pa[h] = *(const P*)&A[(rI + i) * ld + c0 + k];               \
+            pb[h] = *(const P*)&A[(rJ + i) * ld + c0 + k];               \
+        }                                                                \
+    }
+    if (sizeof(T) == 4) TRAIL_LOAD(0);
     for (int q = 0; q < nq; q++) {
-        const long long c0 = (long long)(k0 + q) * TS;
         __syncthreads();
-        for (int e = tid * VW; e < TT * TS; e += 256 * VW) {
-            const int i = e >> 6, k = e & 63;
-            *(V*)&As[i][k] = *(const V*)&A[(rI + i) * ld + c0 + k];
-            *(V*)&Bs[i][k] = *(const V*)&A[(rJ + i) * ld + c0 + k];
+        if (sizeof(T) == 4) {
+#pragma unroll
+            for (int h = 0; h < NPF; h++) {
+                const int e = (tid + 256 * h) * VW;
+                const int i = e >> 6, k = e & 63;
+                *(P*)&As[i][k] = pa[h];
+                *(P*)&Bs[i][k] = pb[h];
+            }
+        } else {
+            const long long c0 = (long long)(k0 + q) * TS;
+            for (int e = tid * VW; e < TT * TS; e += 256 * VW) {
+                const int i = e >> 6, k = e & 63;
+                *(V*)&As[i][k] = *(const V*)&A[(rI + i) * ld + c0 + k];
+                *(V*)&Bs[i][k] = *(const V*)&A[(rJ + i) * ld + c0 + k];
+            }
         }
         __syncthreads();
+        if (sizeof(T) == 4 && q + 1 < nq) TRAIL_LOAD(q + 1);
 #pragma unroll 2
         for (int kk = 0; kk < TS; kk += 4) {
             const int kq = kk + (lane >> 4);
@@ -1297,12 +1334,13 @@ __global__ void __launch_bounds__(256) k_chol_trail_m(const MatDesc* mats, int k
                 if (!diag || cc <= rr) A[(rI + rr) * ld + rJ + cc] -= acc[a][c][v];
             }
 }
+#undef TRAIL_LOAD
 
 template <typename T>
 static void chol_step(hipStream_t s, const MatDesc* mats, int nmat, int kb, int rest)
 {
     hipLaunchKernelGGL(k_chol_diag_m<T>, dim3(1, nmat), dim3(64), 0, s, mats, kb);
-    if (rest > 0) {
+    if (rest > 0) {  // rest: the widest trailing extent of any matrix in the launch
         hipLaunchKernelGGL(k_chol_panel_m<T>, dim3(rest, nmat), dim3(256), 0, s, mats, kb);
         hipLaunchKernelGGL(k_chol_update_m<T>, dim3(syrk_grid(rest), nmat), dim3(256), 0, s, mats, kb);
     }
@@ -1334,8 +1372,9 @@ static void chol_two_level(hipStream_t s, const MatDesc* mats, int nmat, int max
 }
 
 // d_mats holds n32 FP32 matrices first, then n64 FP64 matrices.  Dense matrices (all of the
-// launch) use the two-level schedule; banded ones the 64-block right-looking one.
-void cholesky_batched(hipStream_t s, const MatDesc* d_mats, int n32, int n64, int max_nb, int dense)
+// launch) use the two-level schedule; banded ones (lower band <= max_bwb blocks) the 64-block
+// right-looking one.
+void cholesky_batched(hipStream_t s, const MatDesc* d_mats, int n32, int n64, int max_nb, int dense, int max_bwb)
 {
     if (dense) {
         if (n32) chol_two_level<float>(s, d_mats, n32, max_nb);
@@ -1343,7 +1382,7 @@ void cholesky_batched(hipStream_t s, const MatDesc* d_mats, int n32, int n64, in
         return;
     }
     for (int kb = 0; kb < max_nb; kb++) {
-        const int rest = max_nb - kb - 1;
+        const int rest = std::min(max_nb - kb - 1, max_bwb);  // banded: at most max_bwb blocks below
         if (n32) chol_step<float>(s, d_mats, n32, kb, rest);
         if (n64) chol_step<double>(s, d_mats + n32, n64, kb, rest);
     }

@@ -547,10 +547,12 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     const int nF = (int)fm.size(), nR = (int)rm.size();
     HIPCHK(c, hipEventRecord(c->ev_fork, st));
     HIPCHK(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
-    cholesky_batched(c->stream2, c->d_mats.as<MatDesc>() + nF, 0, nR, max_ld / 64, 0);
+    int max_bwb = 1;
+    for (const MatDesc& r : rm) max_bwb = std::max(max_bwb, r.bwb);
+    cholesky_batched(c->stream2, c->d_mats.as<MatDesc>() + nF, 0, nR, max_ld / 64, 0, max_bwb);
     HIPCHK(c, hipEventRecord(c->ev_join, c->stream2));
     timer_begin(c, "cholesky");
-    cholesky_batched(st, c->d_mats.as<MatDesc>(), mixed ? nF : 0, mixed ? 0 : nF, max_ld / 64, 1);
+    cholesky_batched(st, c->d_mats.as<MatDesc>(), mixed ? nF : 0, mixed ? 0 : nF, max_ld / 64, 1, 0);
     timer_end(c, "cholesky");
     HIPCHK(c, hipStreamWaitEvent(st, c->ev_join, 0));
     launch_logdets_b(st, bd, mixed ? 1 : 0);

@@ -211,7 +211,7 @@ void launch_syrk_f64_b(hipStream_t s, const BatchDev& bd, int max_ncb);
 void launch_syrk_bf16_b(hipStream_t s, const BatchDev& bd, int max_ncb);
 void launch_curvature_b(hipStream_t s, const BatchDev& bd, int max_cells, int max_rtiles);
 void launch_prep_fr_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32);
-void cholesky_batched(hipStream_t s, const MatDesc* d_mats, int n32, int n64, int max_nb, int dense);
+void cholesky_batched(hipStream_t s, const MatDesc* d_mats, int n32, int n64, int max_nb, int dense, int max_bwb);
 void launch_logdets_b(hipStream_t s, const BatchDev& bd, int f32);
 void launch_trisolve_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32);
 void launch_refine_b(hipStream_t s, const BatchDev& bd, int max_ld, int iters, double tol);
