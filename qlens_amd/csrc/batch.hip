@@ -156,6 +156,7 @@ __global__ void __launch_bounds__(1024) k_compact_b(BatchDev bd)
     }
     base = 0;
     const int npix = bd.nx * bd.ny;
+    int* apre = bd.apre + item * (bd.s_pix + 1);
     for (int t0 = 0; t0 < npix; t0 += 1024) {
         int t = t0 + threadIdx.x, p = -1, flag = 0;
         if (t < npix) {
@@ -165,11 +166,15 @@ __global__ void __launch_bounds__(1024) k_compact_b(BatchDev bd)
         int tot, ex = block_excl_scan(flag, sh, tot);
         if (p >= 0) {
             pidx[p] = flag ? base + ex : -1;
+            apre[t] = base + ex;
             if (flag) act[base + ex] = p;
         }
         base += tot;
     }
-    if (threadIdx.x == 0) bd.counts[4 * item + 1] = base;
+    if (threadIdx.x == 0) {
+        bd.counts[4 * item + 1] = base;
+        apre[npix] = base;
+    }
 }
 
 void launch_compact_b(hipStream_t s, const BatchDev& bd)
@@ -215,6 +220,70 @@ __global__ void __launch_bounds__(256) k_merge_lrows_b(BatchDev bd)
 void launch_merge_lrows_b(hipStream_t s, const BatchDev& bd, int max_img)
 {
     if (max_img > 0) hipLaunchKernelGGL(k_merge_lrows_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, s, bd);
+}
+
+// The merged rows again as one CSR array per item, rows in active order.  Active pixels with equal
+// y index j and consecutive x index i are consecutive rows, so the rows of any x-range of one image
+// column occupy one contiguous entry range [lptr[apre[j*nx + i0]], lptr[apre[j*nx + i1 + 1]]).
+// Each entry carries its pixel's x index so a reader knows the PSF tap without a row search.
+__global__ void __launch_bounds__(1024) k_lrow_ptr_b(BatchDev bd)
+{
+    __shared__ int sh[40];
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    if (!M.valid) return;
+    const int* ucnt = bd.ucnt + item * bd.s_pix;
+    int* lptr = bd.lptr + item * (bd.s_pix + 1);
+    int base = 0;
+    for (int r0 = 0; r0 < M.n_img; r0 += 1024) {
+        const int r = r0 + threadIdx.x;
+        const int v = (r < M.n_img) ? ucnt[r] : 0;
+        int tot, ex = block_excl_scan(v, sh, tot);
+        if (r < M.n_img) lptr[r] = base + ex;
+        base += tot;
+    }
+    if (threadIdx.x == 0) lptr[M.n_img] = base;
+}
+
+__global__ void __launch_bounds__(256) k_lrow_csr_b(BatchDev bd)
+{
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!M.valid || r >= M.n_img) return;
+    const int U = 3 * bd.nsub;
+    const unsigned i = (unsigned)(bd.act_pix[item * bd.s_pix + r] / bd.ny);
+    const size_t o = (size_t)item * bd.s_pix * U + bd.lptr[item * (bd.s_pix + 1) + r];
+    const int* uc = bd.ucol + ((size_t)item * bd.s_pix + r) * U;
+    const double* uv = bd.uval + ((size_t)item * bd.s_pix + r) * U;
+    const int cnt = bd.ucnt[item * bd.s_pix + r];
+    for (int e = 0; e < cnt; e++) {
+        bd.lcj[o + e] = (unsigned)uc[e] | (i << 14);
+        bd.lv[o + e] = uv[e];
+        bd.lvf[o + e] = (float)uv[e];
+    }
+}
+
+// eptr[t] = lptr[apre[t]] for every pixel t (and t = npix): one lookup gives the CSR entry range of
+// any run of pixels
+__global__ void __launch_bounds__(256) k_eptr_b(BatchDev bd)
+{
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long npix = (long long)bd.nx * bd.ny;
+    if (!M.valid || t > npix) return;
+    const long long o = item * (bd.s_pix + 1);
+    bd.eptr[o + t] = bd.lptr[o + bd.apre[o + t]];
+}
+
+void launch_lrows_csr_b(hipStream_t s, const BatchDev& bd, int max_img)
+{
+    if (max_img <= 0) return;
+    hipLaunchKernelGGL(k_lrow_ptr_b, dim3(1, bd.B), dim3(1024), 0, s, bd);
+    hipLaunchKernelGGL(k_lrow_csr_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, s, bd);
+    const long long npix1 = (long long)bd.nx * bd.ny + 1;
+    hipLaunchKernelGGL(k_eptr_b, dim3((unsigned)((npix1 + 255) / 256), bd.B), dim3(256), 0, s, bd);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -455,11 +524,20 @@ __device__ __forceinline__ int mask_rank(const unsigned* m, int cb)
 __device__ __forceinline__ bool mask_has(const unsigned* m, int cb) { return (m[cb >> 5] >> (cb & 31)) & 1u; }
 
 // ------------------------------------------------------------------------------------------------
-// blurred rows into the chunk's support blocks (one wave per row).  The lanes of the wave take the
-// PSF taps (lane t, t+64, ...), and every (tap, merged L entry) product is added into the row's
-// support columns in LDS with ds_add_f64; each support block row is then stored as 64 contiguous
-// values.  FP64: pool of doubles.  Mixed: rows scaled by sqrt(c_r) and split x = hi + lo into two
-// BF16 pools (hi = bf16(x), lo = bf16(x - hi)); the lo pool follows the hi pool.
+// Blurred rows of one 64-row chunk into the chunk's support blocks: one workgroup per chunk, each
+// wave accumulating one row at a time in LDS.  Row r (pixel (k, l)) sums psf[pk][pl] *
+// L(pixel (k + nxh - pk, l + nyh - pl), :) over the taps.  Per window column pl the contributing
+// pixels (x in [i0, i1], y = l + nyh - pl) own one contiguous range of the CSR L entries, found by
+// two eptr lookups (made one row ahead, lane pl for column pl).  Each range is cut into quarters
+// of at most 16 entries; a wave scan numbers the quarters and each lane writes its column's
+// quarters into an LDS table {first entry, weight base, count}.  The lanes then take the quarters
+// four at a time (16 lanes each) with coalesced loads, 32 per lane in flight and no branches (idle
+// lanes add zero), and add each product into the row's support columns in LDS.
+//   FP64 path: FP64 weights, ds_add_f64 into FP64 rows, a pool of doubles.
+//   Mixed path: products rounded in FP32 to int32 multiples of 1/fixq and summed with ds_add_u32
+//   (exact and order-independent; ds_add_f32 runs at about a seventh of the rate of the integer
+//   and FP64 LDS atomics on gfx950), then scaled by sqrt(c_r) and split x = hi + lo into two BF16
+//   pools (hi = bf16(x), lo = bf16(x - hi)); the lo pool follows the hi pool.
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)
 {
@@ -469,138 +547,235 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)
 }
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(((unsigned)h) << 16); }
 
-// AccT: LDS accumulation type (double for the FP64 pool; float for the mixed path, whose values
-// are split into two BF16 terms resolving 2^-17 of each entry)
-template <typename AccT>
-__global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int nsup_cap)
+__device__ __forceinline__ void wave_sync()
 {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// CSR entry range [e0, e1) of window column pl of the row at pixel p (empty outside the image)
+__device__ __forceinline__ void window_range(const BatchDev& bd, const int* eptr, int p, int pl, int& e0, int& e1)
+{
+    e0 = 0;
+    e1 = 0;
+    if (p < 0 || pl >= bd.pny) return;
+    const int k = p / bd.ny, l = p - k * bd.ny;
+    const int nxh = bd.pnx / 2, nyh = bd.pny / 2;
+    const int jj = l + nyh - pl;
+    if (jj < 0 || jj >= bd.ny) return;
+    const int i0 = max(0, k + nxh - bd.pnx + 1), i1 = min(bd.nx - 1, k + nxh);
+    const long long t0 = (long long)jj * bd.nx;
+    e0 = eptr[t0 + i0];
+    e1 = eptr[t0 + i1 + 1];
+}
+
+#define BUILD_U 32              // entry slots per lane and pass
+#define BUILD_TQ (4 * BUILD_U)  // quarters per pass
+
+template <bool FIX>
+struct BuildTypes {             // FP64 path
+    typedef double acc_t;
+    typedef double val_t;
+    typedef double w_t;
+};
+template <>
+struct BuildTypes<true> {       // mixed path: int32 fixed point
+    typedef unsigned acc_t;
+    typedef float val_t;
+    typedef float w_t;
+};
+
+// window columns cg .. cg+63 of the row at pixel x index k: lane c holds [e0, e0 + len) of column cg + c
+template <bool FIX>
+__device__ __forceinline__ void accumulate_columns(const BatchDev& bd, const unsigned* lcj, const typename BuildTypes<FIX>::val_t* lval,
+                                                   const typename BuildTypes<FIX>::w_t* W, const short* loc, int cbs,
+                                                   typename BuildTypes<FIX>::acc_t* row, int4* qtab, int k, int cg, int e0, int len)
+{
+    typedef typename BuildTypes<FIX>::val_t V;
+    typedef typename BuildTypes<FIX>::w_t WT;
+    const int lane = threadIdx.x & 63, CB = 1 << cbs;
+    const int nq = (len + 15) >> 4;
+    const int qincl = wave_incl_scan(nq);
+    const int QS = qincl - nq;
+    const int NQ = __builtin_amdgcn_readlane(qincl, 63);
+    const int wbc = (cg + lane) * bd.pnx + k + bd.pnx / 2;
+    for (int g0 = 0; g0 < NQ; g0 += BUILD_TQ) {
+        const int gend = min(QS + nq, g0 + BUILD_TQ);
+        for (int g = max(QS, g0); g < gend; g++) {
+            const int x = g - QS;
+            qtab[g - g0] = make_int4(e0 + 16 * x, wbc, min(16, len - 16 * x), 0);
+        }
+        wave_sync();
+        const int ng = min(BUILD_TQ, NQ - g0);
+        const int sub = lane & 15;
+        unsigned cjs[BUILD_U];
+        V vs[BUILD_U];
+        int wbs[BUILD_U];
+#pragma unroll
+        for (int u = 0; u < BUILD_U; u++) {
+            const int gl = 4 * u + (lane >> 4);
+            const int4 t = qtab[min(gl, ng - 1)];
+            const bool ok = gl < ng && sub < t.z;
+            const int e = t.x + (ok ? sub : 0);
+            cjs[u] = lcj[e];
+            const V v = lval[e];
+            vs[u] = ok ? v : (V)0;
+            wbs[u] = t.y;
+        }
+#pragma unroll
+        for (int u = 0; u < BUILD_U; u++) {
+            const int col = (int)(cjs[u] & 16383u);
+            const WT wt = W[wbs[u] - (int)(cjs[u] >> 14)];
+            const int slot = (loc[col >> cbs] << cbs) + (col & (CB - 1));
+            if (FIX)
+                atomicAdd((unsigned*)row + slot, (unsigned)__float2int_rn((float)(wt * vs[u])));
+            else
+                atomicAdd((double*)row + slot, (double)(wt * vs[u]));
+        }
+        wave_sync();  // the next pass rewrites qtab
+    }
+}
+
+// FIX: mixed path (int32 fixed-point rows, BF16 hi/lo pools); else FP64.  The transposed PSF (times
+// fixq on the mixed path) is staged in LDS.  Dynamic LDS: [PSF] [nw x BUILD_TQ quarter table]
+// [nw x nsup_cap x CB rows].
+template <bool FIX>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+k_build_lb_b(BatchDev bd, int nsup_cap, int psf_elems)
+{
+    typedef typename BuildTypes<FIX>::acc_t A;
+    typedef typename BuildTypes<FIX>::w_t WT;
     extern __shared__ double lds_raw[];
-    AccT* lds_rows = (AccT*)lds_raw;
     __shared__ short loc[QLX_MAX_LD / TS];
-    const bool bf16 = sizeof(AccT) == 4;
-    const int item = blockIdx.y;
+    __shared__ int prow[TS];
+    __shared__ double rdat[4], rscl[4];  // mixed path: data value and sqrt(c)/fixq of each wave's row
+    const int item = blockIdx.y, kt = blockIdx.x;
     const ItemMeta& M = bd.meta[item];
+    if (!M.valid || kt >= M.nkt) return;
     const int nw = blockDim.x >> 6;
-    const int r0 = blockIdx.x * nw;
-    if (!M.valid || r0 >= M.nrows) return;
     const int cbs = M.cbs, CB = 1 << cbs;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    const int rs = r0 + w;
-    const int kt = r0 / TS, rr = rs % TS;
+    const int ntap = bd.pnx * bd.pny;
+    WT* lds_w = (WT*)lds_raw;
+    int4* qtab = (int4*)(lds_raw + psf_elems) + w * BUILD_TQ;
+    A* lds_rows = (A*)((int4*)(lds_raw + psf_elems) + nw * BUILD_TQ);
+    A* row = lds_rows + (size_t)w * nsup_cap * CB;
     const unsigned* m = bd.nz + M.nz_off + (size_t)kt * M.ncbw;
     for (int cb = tid; cb < M.ncb; cb += blockDim.x) loc[cb] = mask_has(m, cb) ? (short)mask_rank(m, cb) : (short)-1;
+    for (int t = tid; t < ntap; t += blockDim.x) lds_w[t] = FIX ? (WT)(bd.psfT[t] * bd.fixq) : (WT)bd.psfT[t];
+    for (int t = tid; t < TS; t += blockDim.x) {
+        const int rs = kt * TS + t;
+        prow[t] = rs < M.n_img ? bd.act_pix[item * bd.s_pix + bd.perm[item * bd.s_pix + rs]] : -1;
+    }
     const int sbase = bd.bbase[M.bb_off + kt];
     const int nsup = bd.bbase[M.bb_off + kt + 1] - sbase;
     const size_t bsz = (size_t)TS * CB;  // elements of one pool block
-    AccT* row = lds_rows + (size_t)w * nsup_cap * CB;
-    for (int c = lane; c < nsup * CB; c += 64) row[c] = (AccT)0;
+    const int* eptr = bd.eptr + item * (bd.s_pix + 1);
+    const unsigned* lcj = bd.lcj + (size_t)item * bd.s_pix * 3 * bd.nsub;
+    const typename BuildTypes<FIX>::val_t* lval;
+    if constexpr (FIX) lval = bd.lvf + (size_t)item * bd.s_pix * 3 * bd.nsub;
+    else lval = bd.lv + (size_t)item * bd.s_pix * 3 * bd.nsub;
     __syncthreads();
-    const int U = 3 * bd.nsub;
-    const bool live = rs < M.n_img;
-    const int p = live ? bd.act_pix[item * bd.s_pix + bd.perm[item * bd.s_pix + rs]] : 0;
-    if (live) {
-        const int k = p / bd.ny, l = p % bd.ny;
-        const int nxh = bd.pnx / 2, nyh = bd.pny / 2;
-        const int ntap = bd.pnx * bd.pny;
-        const int* pidx = bd.pidx + item * bd.s_pix;
-        const int* ucnt = bd.ucnt + item * bd.s_pix;
-        const int* ucol = bd.ucol + (size_t)item * bd.s_pix * U;
-        const double* uval = bd.uval + (size_t)item * bd.s_pix * U;
-        // lane l takes taps tstride*(l + 64 i) mod ntap (a bijection: tstride is coprime with ntap),
-        // so the lanes of one step sit far apart in the PSF and rarely add into the same column;
-        // eight taps per lane are looked up together to keep their loads in flight at once
-        for (int tb = 0; tb < ntap; tb += 64 * 8) {
-            int r2s[8];
-            double pvs[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int t0 = tb + 64 * u + lane;
-                r2s[u] = -1;
-                pvs[u] = 0.0;
-                if (t0 < ntap) {
-                    const int t = (int)(((long long)t0 * bd.tstride) % ntap);
-                    const int pk = t / bd.pny, pl = t - pk * bd.pny;
-                    const int ii = k + nxh - pk, jj = l + nyh - pl;
-                    if (ii >= 0 && ii < bd.nx && jj >= 0 && jj < bd.ny) {
-                        r2s[u] = pidx[ii * bd.ny + jj];
-                        pvs[u] = bd.psf[t];
-                    }
-                }
-            }
-            int cnts[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) cnts[u] = (r2s[u] >= 0) ? ucnt[r2s[u]] : 0;
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int* uc = ucol + (size_t)max(r2s[u], 0) * U;
-                const double* uv = uval + (size_t)max(r2s[u], 0) * U;
-                for (int e = 0; e < cnts[u]; e++) {
-                    const int c = uc[e];
-                    atomicAdd(row + (loc[c >> cbs] << cbs) + (c & (CB - 1)), (AccT)(pvs[u] * uv[e]));
-                }
+    const int dcol = (loc[M.n_src >> cbs] << cbs) + (M.n_src & (CB - 1));  // the data column's slot
+    const int rpw = TS / nw;  // rows per wave
+    int pe0, pe1;             // window column `lane` of the wave's next row
+    window_range(bd, eptr, prow[w], lane, pe0, pe1);
+    for (int j = 0; j < rpw; j++) {
+        const int rr = j * nw + w;
+        const int p = prow[rr];
+        const int e0 = pe0, len = pe1 - pe0;
+        if (j + 1 < rpw) window_range(bd, eptr, prow[rr + nw], lane, pe0, pe1);
+        for (int c = lane; c < nsup * CB; c += 64) row[c] = (A)0;
+        wave_sync();
+        if (p >= 0) {
+            const int k = p / bd.ny;
+            accumulate_columns<FIX>(bd, lcj, lval, lds_w, loc, cbs, row, qtab, k, 0, e0, len);
+            for (int cg = 64; cg < bd.pny; cg += 64) {
+                int f0, f1;
+                window_range(bd, eptr, p, cg + lane, f0, f1);
+                accumulate_columns<FIX>(bd, lcj, lval, lds_w, loc, cbs, row, qtab, k, cg, f0, f1 - f0);
             }
         }
-    }
-    __syncthreads();
-    if (live && lane == 0) row[(loc[M.n_src >> cbs] << cbs) + (M.n_src & (CB - 1))] = (AccT)bd.data[p];
-    __syncthreads();
-    if (live && bf16) {
-        const AccT sc = (AccT)sqrt(bd.covinv[p]);
-        for (int c = lane; c < nsup * CB; c += 64) row[c] *= sc;
-    }
-    __syncthreads();
-    if (!bf16) {
-        double* pool = (double*)bd.lb + M.lb_off;
-        for (int q = 0; q < nsup; q++)
-            for (int cc = lane; cc < CB; cc += 64) pool[(size_t)(sbase + q) * bsz + (size_t)rr * CB + cc] = (double)row[q * CB + cc];
-    } else {
-        // k4-interleaved BF16 blocks: element (k, i) of a block at ((k >> 2) * CB + i) * 4 + (k & 3),
-        // so the SYRK stages blocks with plain 16-byte copies and reads 8 consecutive k per lane
-        uint16_t* hi = (uint16_t*)bd.lb + M.lb_off;
-        uint16_t* lo = hi + (size_t)M.nblk * bsz;
-        if (nw == 4) {
-            const int kg = (r0 % TS) >> 2;
-            for (int idx = tid; idx < nsup * CB; idx += 256) {
-                const int q = idx >> cbs, i = idx & (CB - 1);
-                uint16_t h[4], g[4];
-#pragma unroll
-                for (int w2 = 0; w2 < 4; w2++) {
-                    const float x = (float)lds_rows[(size_t)w2 * nsup_cap * CB + idx];
-                    h[w2] = f32_to_bf16_rne(x);
-                    g[w2] = f32_to_bf16_rne(x - bf16_to_f32(h[w2]));
-                }
-                const size_t o = (size_t)(sbase + q) * bsz + ((size_t)kg * CB + i) * 4;
-                *(uint2*)&hi[o] = make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
-                *(uint2*)&lo[o] = make_uint2(g[0] | ((unsigned)g[1] << 16), g[2] | ((unsigned)g[3] << 16));
-            }
-        } else {
+        if constexpr (!FIX) {
+            if (p >= 0 && lane == 0) row[dcol] = bd.data[p];
+            wave_sync();
+            double* pool = (double*)bd.lb + M.lb_off;
             for (int q = 0; q < nsup; q++)
-                for (int i = lane; i < CB; i += 64) {
-                    const float x = (float)row[q * CB + i];
-                    const uint16_t h = f32_to_bf16_rne(x);
-                    const uint16_t g = f32_to_bf16_rne(x - bf16_to_f32(h));
-                    const size_t o = (size_t)(sbase + q) * bsz + ((size_t)(rr >> 2) * CB + i) * 4 + (rr & 3);
-                    hi[o] = h;
-                    lo[o] = g;
+                for (int cc = lane; cc < CB; cc += 64) pool[(size_t)(sbase + q) * bsz + (size_t)rr * CB + cc] = row[q * CB + cc];
+            wave_sync();
+        } else {
+            // int32 multiples of 1/fixq -> x = value * sqrt(c_r) (the data column: data * sqrt(c_r))
+            const double sc = p >= 0 ? sqrt(bd.covinv[p]) : 0.0;
+            const double dv = p >= 0 ? bd.data[p] * sc : 0.0;
+            const double sq = sc / bd.fixq;
+            uint16_t* hi = (uint16_t*)bd.lb + M.lb_off;
+            uint16_t* lo = hi + (size_t)M.nblk * bsz;
+            if (nw == 4) {
+                // k4-interleaved BF16 blocks: element (k, i) of a block at ((k >> 2) * CB + i) * 4 + (k & 3),
+                // so the SYRK stages blocks with plain 16-byte copies and reads 8 consecutive k per lane;
+                // the four waves' rows j*4 .. j*4+3 are one k-group
+                if (lane == 0) {
+                    rdat[w] = dv;
+                    rscl[w] = sq;
                 }
+                __syncthreads();
+                for (int idx = tid; idx < nsup * CB; idx += 256) {
+                    const int q = idx >> cbs, i = idx & (CB - 1);
+                    uint16_t h[4], g[4];
+#pragma unroll
+                    for (int w2 = 0; w2 < 4; w2++) {
+                        const double xv = idx == dcol ? rdat[w2] : (double)(int)lds_rows[(size_t)w2 * nsup_cap * CB + idx] * rscl[w2];
+                        const float x = (float)xv;
+                        h[w2] = f32_to_bf16_rne(x);
+                        g[w2] = f32_to_bf16_rne(x - bf16_to_f32(h[w2]));
+                    }
+                    const size_t o = (size_t)(sbase + q) * bsz + ((size_t)j * CB + i) * 4;
+                    *(uint2*)&hi[o] = make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
+                    *(uint2*)&lo[o] = make_uint2(g[0] | ((unsigned)g[1] << 16), g[2] | ((unsigned)g[3] << 16));
+                }
+                __syncthreads();
+            } else {
+                wave_sync();
+                for (int q = 0; q < nsup; q++)
+                    for (int i = lane; i < CB; i += 64) {
+                        const int idx = q * CB + i;
+                        const float x = (float)(idx == dcol ? dv : (double)(int)row[idx] * sq);
+                        const uint16_t h = f32_to_bf16_rne(x);
+                        const uint16_t g = f32_to_bf16_rne(x - bf16_to_f32(h));
+                        const size_t o = (size_t)(sbase + q) * bsz + ((size_t)(rr >> 2) * CB + i) * 4 + (rr & 3);
+                        hi[o] = h;
+                        lo[o] = g;
+                    }
+                wave_sync();
+            }
         }
     }
 }
 
-void launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_nsup, int bf16, int cbw)
+template <bool FIX>
+static int launch_build_typed(hipStream_t s, const BatchDev& bd, int max_nkt, int max_nsup, int cbw)
 {
-    if (max_rows <= 0 || max_nsup <= 0) return;
-    const size_t per_wave = (size_t)max_nsup * cbw * (bf16 ? sizeof(float) : sizeof(double));
+    typedef typename BuildTypes<FIX>::acc_t A;
+    typedef typename BuildTypes<FIX>::w_t WT;
+    const size_t per_wave = (size_t)max_nsup * cbw * sizeof(A) + BUILD_TQ * sizeof(int4);
+    const size_t ntap = (size_t)bd.pnx * bd.pny;
+    const size_t psf_bytes = (ntap * sizeof(WT) + 15) / 16 * 16;
+    if (psf_bytes > 64 * 1024 || psf_bytes + per_wave > 144 * 1024) return -1;  // PSF / support beyond the LDS budget
     int nw = 4;
-    while (nw > 1 && per_wave * nw > 144 * 1024) nw >>= 1;
-    const dim3 grid((max_rows + nw - 1) / nw, bd.B);
-    if (bf16) {
-        allow_big_lds((const void*)k_build_lb_b<float>);
-        hipLaunchKernelGGL(k_build_lb_b<float>, grid, dim3(64 * nw), per_wave * nw, s, bd, max_nsup);
-    } else {
-        allow_big_lds((const void*)k_build_lb_b<double>);
-        hipLaunchKernelGGL(k_build_lb_b<double>, grid, dim3(64 * nw), per_wave * nw, s, bd, max_nsup);
-    }
+    while (nw > 1 && psf_bytes + per_wave * nw > 144 * 1024) nw >>= 1;
+    const size_t lds = psf_bytes + per_wave * nw;
+    const int psf_elems = (int)(psf_bytes / 8);  // in doubles: the offset of the quarter tables
+    allow_big_lds((const void*)k_build_lb_b<FIX>);
+    hipLaunchKernelGGL((k_build_lb_b<FIX>), dim3(max_nkt, bd.B), dim3(64 * nw), lds, s, bd, max_nsup, psf_elems);
+    return 0;
+}
+
+int launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_nsup, int bf16, int cbw)
+{
+    if (max_rows <= 0 || max_nsup <= 0) return 0;
+    const int max_nkt = (max_rows + TS - 1) / TS;
+    return bf16 ? launch_build_typed<true>(s, bd, max_nkt, max_nsup, cbw) : launch_build_typed<false>(s, bd, max_nkt, max_nsup, cbw);
 }
 
 __global__ void k_gather_cvec_b(BatchDev bd)

@@ -77,7 +77,7 @@ struct Ctx {
     int pnx = 0, pny = 0;
     bool psf_set = false;
     std::vector<double> h_psf;
-    DevBuf psf;
+    DevBuf psf, psfT;
     // settings
     qlx_settings set{};
     bool settings_set = false;
@@ -85,6 +85,7 @@ struct Ctx {
     DevBuf d_models, d_sgrid, d_srcn, d_meta, d_mats;
     // per-item work (stride s_pix / s_cell)
     DevBuf beta, map_cell, map_w, pmapped, cmapped, sidx, tix, pisrc, pidx, act_pix, counts, ucol, uval, ucnt, nevals;
+    DevBuf apre, lptr, eptr, lcj, lv, lvf;
     DevBuf key, bcnt, boff, bfill, perm;
     // pools
     DevBuf lb, F, R, cvec, vec, nz, bbase, winv, model, out, status;
@@ -263,7 +264,7 @@ void fill_source_grid(const qlx_settings& s, int snx, int sny, double* g)
 size_t stage1_bytes(size_t s_pix, size_t s_cell, size_t nsub)
 {
     const size_t U = 3 * nsub;
-    return s_pix * nsub * (16 + 12 + 24) + s_pix * U * 12 + s_pix * 4 * 7 + s_cell * 4 * 5 + 64 + s_pix * 16;
+    return s_pix * nsub * (16 + 12 + 24) + s_pix * U * 28 + s_pix * 4 * 7 + (s_pix + 1) * 12 + s_cell * 4 * 5 + 64 + s_pix * 16;
 }
 
 // upper bound of the dense bytes of one item with at most ncell active source cells
@@ -313,6 +314,8 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
               c->pidx.ensure(B * s_pix * 4) && c->act_pix.ensure(B * s_pix * 4) &&
               c->counts.ensure((size_t)B * 16) && c->ucol.ensure(B * s_pix * U * 4) && c->uval.ensure(B * s_pix * U * 8) &&
               c->ucnt.ensure(B * s_pix * 4) && c->nevals.ensure((size_t)B * 8) && c->key.ensure(B * s_pix * 4) &&
+              c->apre.ensure(B * (s_pix + 1) * 4) && c->lptr.ensure(B * (s_pix + 1) * 4) && c->eptr.ensure(B * (s_pix + 1) * 4) && c->lcj.ensure(B * s_pix * U * 4) &&
+              c->lv.ensure(B * s_pix * U * 8) && c->lvf.ensure(B * s_pix * U * 4) &&
               c->bcnt.ensure(B * (s_cell + 2) * 4) && c->boff.ensure(B * (s_cell + 2) * 4) && c->bfill.ensure(B * (s_cell + 2) * 4) &&
               c->perm.ensure(B * s_pix * 4) && c->model.ensure(B * s_pix * 16) && c->out.ensure((size_t)B * QLX_OUT_STRIDE * 8) &&
               c->status.ensure((size_t)B * 16);
@@ -333,18 +336,9 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     bd.data = c->data.as<double>();
     bd.covinv = c->covinv.as<double>();
     bd.psf = c->psf.as<double>();
+    bd.psfT = c->psfT.as<double>();
     bd.pnx = c->pnx;
     bd.pny = c->pny;
-    bd.tstride = 1;
-    for (int cand : {11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53}) {
-        const int nt = c->pnx * c->pny;
-        int a = cand, b = nt;
-        while (b) { const int t = a % b; a = b; b = t; }
-        if (a == 1 && cand < nt) {
-            bd.tstride = cand;
-            break;
-        }
-    }
     bd.tol = S.integral_tol;
     bd.lambda = S.regparam;
     bd.models = c->d_models.as<ModelPrep>();
@@ -365,6 +359,20 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     bd.ucol = c->ucol.as<int>();
     bd.uval = c->uval.as<double>();
     bd.ucnt = c->ucnt.as<int>();
+    bd.apre = c->apre.as<int>();
+    bd.lptr = c->lptr.as<int>();
+    bd.eptr = c->eptr.as<int>();
+    bd.lcj = c->lcj.as<unsigned>();
+    bd.lv = c->lv.as<double>();
+    bd.lvf = c->lvf.as<float>();
+    {
+        // |blurred L entry| <= sum|psf| * max_r ||L_r||_1 <= 3 sum|psf| (per subpixel the 3-point
+        // weights are 1/nsub times barycentric coordinates with |w1|, |w2| <= 1/2, 0 <= w0 <= 2),
+        // so int32 multiples of 2^-29 sum|psf| cannot overflow
+        double apsf = 0.0;
+        for (double v : c->h_psf) apsf += fabs(v);
+        bd.fixq = std::ldexp(1.0, 29) / std::max(apsf, 1e-300);
+    }
     bd.key = c->key.as<int>();
     bd.bcnt = c->bcnt.as<int>();
     bd.boff = c->boff.as<int>();
@@ -482,6 +490,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
 
     timer_begin(c, "lbuild");
     launch_merge_lrows_b(st, bd, max_img);
+    launch_lrows_csr_b(st, bd, max_img);
     launch_sort_rows_b(st, bd, max_img, max_src);
     launch_row_support_b(st, bd, max_nkt);
     HIPCHK(c, hipMemcpyAsync(counts.data(), bd.counts, (size_t)B * 16, hipMemcpyDeviceToHost, st));
@@ -500,7 +509,8 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     if (!c->lb.ensure(lb_tot * (mixed ? 2 : 8))) return fail(c, QLX_ERR_NOMEM, "Lb pool allocation failed");
     bd.lb = c->lb.p;
     HIPCHK(c, hipMemcpyAsync(c->d_meta.p, meta.data(), sizeof(ItemMeta) * B, hipMemcpyHostToDevice, st));
-    launch_build_lb_b(st, bd, max_rows, max_nsup, mixed ? 1 : 0, mixed ? 128 : 64);
+    if (launch_build_lb_b(st, bd, max_rows, max_nsup, mixed ? 1 : 0, mixed ? 128 : 64) != 0)
+        return fail(c, QLX_ERR_RANGE, "PSF too large for the Lb build (at most 8192 taps in FP64, 16384 mixed)");
     timer_end(c, "lbuild");
     if (!mixed) launch_gather_cvec_b(st, bd, max_rows);
     timer_begin(c, "syrk");
@@ -762,6 +772,7 @@ int qlx_set_image(void* ctx, int nx, int ny, const double* xvals, const double* 
     if (!c) return QLX_ERR_ARG;
     if (nx < 1 || ny < 1 || !xvals || !yvals || !data || !covinv) return fail(c, QLX_ERR_ARG, "qlx_set_image: bad arguments");
     if ((long long)nx * ny > (1LL << 24)) return fail(c, QLX_ERR_RANGE, "image too large");
+    if (nx >= (1 << 18)) return fail(c, QLX_ERR_RANGE, "image wider than 262143 pixels in x");
     hipSetDevice(c->device);
     size_t npix = (size_t)nx * ny;
     c->nx = nx;
@@ -795,8 +806,12 @@ int qlx_set_psf(void* ctx, int pnx, int pny, const double* psf)
     c->pnx = pnx;
     c->pny = pny;
     c->h_psf.assign(psf, psf + (size_t)pnx * pny);
-    if (!c->psf.ensure((size_t)pnx * pny * 8)) return fail(c, QLX_ERR_NOMEM, "psf allocation failed");
+    std::vector<double> tr((size_t)pnx * pny);
+    for (int i = 0; i < pnx; i++)
+        for (int j = 0; j < pny; j++) tr[(size_t)j * pnx + i] = psf[(size_t)i * pny + j];
+    if (!c->psf.ensure((size_t)pnx * pny * 8) || !c->psfT.ensure((size_t)pnx * pny * 8)) return fail(c, QLX_ERR_NOMEM, "psf allocation failed");
     HIPCHK(c, hipMemcpyAsync(c->psf.p, psf, (size_t)pnx * pny * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->psfT.p, tr.data(), (size_t)pnx * pny * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->psf_set = true;
     return QLX_OK;

@@ -75,8 +75,8 @@ struct BatchDev {
     const double* data;
     const double* covinv;
     const double* psf;
+    const double* psfT;         // the PSF transposed: psfT[pl * pnx + pk] = psf[pk * pny + pl]
     int pnx, pny;
-    int tstride;                // coprime with pnx*pny: the Lb build's tap order across lanes
     double tol, lambda;
     // per-item inputs
     const ModelPrep* models;
@@ -98,6 +98,13 @@ struct BatchDev {
     int* ucol;                  // s_pix*3*nsub
     double* uval;               // s_pix*3*nsub
     int* ucnt;                  // s_pix
+    int* apre;                  // s_pix+1: active pixels before pixel t = j*nx + i (image order j-outer)
+    int* lptr;                  // s_pix+1: first CSR entry of each active pixel's merged L row
+    int* eptr;                  // s_pix+1: CSR entries of the active pixels before pixel t = j*nx + i
+    unsigned* lcj;              // s_pix*3*nsub: CSR entry column (bits 0-13) | pixel x index i << 14
+    double* lv;                 // s_pix*3*nsub: CSR entry value
+    float* lvf;                 // s_pix*3*nsub: CSR entry value in FP32 (mixed path)
+    double fixq;                // mixed path: blurred rows accumulate as int32 multiples of 1/fixq
     int* key;                   // s_pix
     int* bcnt;                  // s_cell+2
     int* boff;                  // s_cell+2
@@ -203,9 +210,10 @@ void launch_fill_i32(hipStream_t s, int* p, int v, size_t n);
 void launch_fill_f64(hipStream_t s, double* p, double v, size_t n);
 void launch_compact_b(hipStream_t s, const BatchDev& bd);
 void launch_merge_lrows_b(hipStream_t s, const BatchDev& bd, int max_img);
+void launch_lrows_csr_b(hipStream_t s, const BatchDev& bd, int max_img);
 void launch_sort_rows_b(hipStream_t s, const BatchDev& bd, int max_img, int max_src);
 void launch_row_support_b(hipStream_t s, const BatchDev& bd, int max_nkt);
-void launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_nsup, int bf16, int cbw);
+int launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_nsup, int bf16, int cbw);
 void launch_gather_cvec_b(hipStream_t s, const BatchDev& bd, int max_rows);
 void launch_syrk_f64_b(hipStream_t s, const BatchDev& bd, int max_ncb);
 void launch_syrk_bf16_b(hipStream_t s, const BatchDev& bd, int max_ncb);
