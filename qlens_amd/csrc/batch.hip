@@ -1191,6 +1191,25 @@ __device__ __forceinline__ int band_last(const MatDesc& D, int kb)
     return D.bwb > 0 ? min(D.nb - 1, kb + D.bwb) : D.nb - 1;
 }
 
+// MFMA operand/accumulator types: FP64 v_mfma_f64_16x16x4f64 (accumulator register v of lane l
+// holds D[l/16 + 4v][l%16]) and FP32 v_mfma_f32_16x16x4f32 (D[4(l/16) + v][l%16])
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+    typedef double4_t acc_t;
+    typedef double2 vec_t;
+    __device__ static acc_t op(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <>
+struct Mfma16<float> {
+    typedef float4_t acc_t;
+    typedef float4 vec_t;
+    __device__ static acc_t op(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
 // Diagonal 64x64 block, one wave: lane r keeps row r in registers.  Column step c: the pivot
 // and the column entries l_qc go through LDS (broadcast reads); every lane scales its own entry
 // and applies the rank-1 update to all of its entries right of c without branching (entries
@@ -1205,11 +1224,20 @@ __device__ void chol_diag_wave(const MatDesc& D, int kb, double* raw)
     const long long base = (long long)kb * TS;
     int ld;
     T* tile = tile_ptr<T>(D, kb, kb, ld);
-    for (int e = r; e < TS * TS; e += 64) Ls[e >> 6][e & 63] = tile[(size_t)(e >> 6) * ld + (e & 63)];
-    __syncthreads();
+    // lane r loads row r straight into registers: all of its vector loads are in flight together
+    typedef typename Mfma16<T>::vec_t V;
+    constexpr int VW = sizeof(V) / sizeof(T);
     T a[TS];
 #pragma unroll
-    for (int q = 0; q < TS; q++) a[q] = Ls[r][q];
+    for (int q = 0; q < TS; q += VW) {
+        const V v = *(const V*)&tile[(size_t)r * ld + q];
+        a[q] = v.x;
+        a[q + 1] = v.y;
+        if constexpr (VW == 4) {
+            a[q + 2] = v.z;
+            a[q + 3] = v.w;
+        }
+    }
     bool bad = false;
 #pragma unroll
     for (int c = 0; c < TS; c++) {
@@ -1230,6 +1258,7 @@ __device__ void chol_diag_wave(const MatDesc& D, int kb, double* raw)
 #pragma unroll
     for (int q = 0; q < TS; q++) Ls[r][q] = (q <= r) ? a[q] : (T)0;
     __syncthreads();
+#pragma unroll 16
     for (int e = r; e < TS * TS; e += 64)
         if ((e & 63) <= (e >> 6)) tile[(size_t)(e >> 6) * ld + (e & 63)] = Ls[e >> 6][e & 63];
     ((T*)D.diag)[base + r] = Ls[r][r];
@@ -1258,23 +1287,6 @@ __global__ void __launch_bounds__(64) k_chol_diag_m(const MatDesc* mats, int kb)
 // 64x64x64 MFMA product acc = sum_q A(i, q) B(j, q) with A and B stored "row holds the k-vector"
 // (the panel blocks as they lie in the factor), staged into LDS unchanged ([i][k], row pitch
 // LDT) and read as MFMA operands A[i = lane%16][k = kk + lane/16].
-template <typename T>
-struct Mfma16;
-template <>
-struct Mfma16<double> {
-    typedef double4_t acc_t;
-    typedef double2 vec_t;
-    __device__ static acc_t op(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-    __device__ static int row(int lane, int v) { return (lane >> 4) + 4 * v; }
-};
-template <>
-struct Mfma16<float> {
-    typedef float4_t acc_t;
-    typedef float4 vec_t;
-    __device__ static acc_t op(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-    __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
-};
-
 #define LDT 68
 template <typename T>
 __device__ __forceinline__ void stage64(const T* P, int sp, T (*S)[LDT])
@@ -1416,6 +1428,34 @@ __global__ void __launch_bounds__(256) k_chol_upanel_m(const MatDesc* mats, int 
     store64<T>(C, ldc, acc, I == J ? 2 : 1);
 }
 
+// acc += As(rows of quadrant wr) * Bs(rows of quadrant wc)^T over the 64 staged columns.  Lane group
+// g = lane/16 reads VW consecutive k of its row with one LDS vector read; MFMA s of the group takes
+// component s, so the VW MFMAs of a step sum k = kk + VW*g + s over all 4*VW values of the step.
+template <typename T>
+__device__ __forceinline__ void trail_mma(T (*As)[LDT], T (*Bs)[LDT], typename Mfma16<T>::acc_t (&acc)[4][4])
+{
+    typedef typename Mfma16<T>::vec_t V;
+    constexpr int VW = sizeof(V) / sizeof(T);
+    typedef T P __attribute__((ext_vector_type(VW)));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = w >> 1, wc = w & 1;
+#pragma unroll 1
+    for (int kk = 0; kk < TS; kk += 4 * VW) {
+        const int kq = kk + VW * (lane >> 4);
+        P av[4], bv[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) av[a] = *(const P*)&As[wr * 64 + a * 16 + (lane & 15)][kq];
+#pragma unroll
+        for (int c = 0; c < 4; c++) bv[c] = *(const P*)&Bs[wc * 64 + c * 16 + (lane & 15)][kq];
+#pragma unroll
+        for (int s2 = 0; s2 < VW; s2++)
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) acc[a][c] = Mfma16<T>::op(av[a][s2], bv[c][s2], acc[a][c]);
+    }
+}
+
 // Trailing update after a 256-wide panel: C(I, J) -= sum over the panel's nq column blocks of
 // L(I, q) L(J, q)^T, on 128 x 128 tiles of the trailing matrix (first row block k0 + nq, XCD-aware
 // tile order).  Wave (wr, wc) owns the 64 x 64 quadrant (wr, wc): 4 x 4 MFMA tiles of 16 x 16.
@@ -1423,7 +1463,7 @@ __global__ void __launch_bounds__(256) k_chol_upanel_m(const MatDesc* mats, int 
 // Two 128 x 68 staging tiles per WG (70 KB in FP32) allow two WGs per CU, so two waves per SIMD
 // is all the occupancy this kernel gets: the register budget is sized for that.
 template <typename T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_chol_trail_m(const MatDesc* mats, int k0, int nq)
 {
     extern __shared__ double trail_raw[];
@@ -1450,6 +1490,7 @@ k_chol_trail_m(const MatDesc* mats, int k0, int nq)
     constexpr int NPF = (sizeof(T) == 4) ? (TT * TS) / (256 * VW) : 1;  // pieces per thread and matrix
     typedef T P __attribute__((ext_vector_type(VW)));
     P pa[NPF], pb[NPF];
+    T cv[64];  // FP32: the C tile values of this lane (loaded during the last block)
     // FP32: the next column block is loaded into registers while the current one is multiplied
 #define TRAIL_LOAD(q)                                                    \
     {                                                                    \
@@ -1462,10 +1503,10 @@ k_chol_trail_m(const MatDesc* mats, int k0, int nq)
             pb[h] = *(const P*)&A[(rJ + i) * ld + c0 + k];               \
         }                                                                \
     }
-    if (sizeof(T) == 4) TRAIL_LOAD(0);
-    for (int q = 0; q < nq; q++) {
-        __syncthreads();
-        if (sizeof(T) == 4) {
+    if (sizeof(T) == 4) {
+        TRAIL_LOAD(0);
+        for (int q = 0; q < nq; q++) {
+            __syncthreads();
 #pragma unroll
             for (int h = 0; h < NPF; h++) {
                 const int e = (tid + 256 * h) * VW;
@@ -1473,28 +1514,33 @@ k_chol_trail_m(const MatDesc* mats, int k0, int nq)
                 *(P*)&As[i][k] = pa[h];
                 *(P*)&Bs[i][k] = pb[h];
             }
-        } else {
+            __syncthreads();
+            if (q + 1 == nq) break;  // the last block is multiplied below, with the C tile loads
+            TRAIL_LOAD(q + 1);
+            trail_mma<T>(As, Bs, acc);
+        }
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int rr = wr * 64 + a * 16 + Mfma16<T>::row(lane, v);
+                    const int cc = wc * 64 + c * 16 + (lane & 15);
+                    cv[(a * 4 + c) * 4 + v] = A[(rI + rr) * ld + rJ + cc];
+                }
+        trail_mma<T>(As, Bs, acc);
+    } else {
+        for (int q = 0; q < nq; q++) {
+            __syncthreads();
             const long long c0 = (long long)(k0 + q) * TS;
             for (int e = tid * VW; e < TT * TS; e += 256 * VW) {
                 const int i = e >> 6, k = e & 63;
                 *(V*)&As[i][k] = *(const V*)&A[(rI + i) * ld + c0 + k];
                 *(V*)&Bs[i][k] = *(const V*)&A[(rJ + i) * ld + c0 + k];
             }
-        }
-        __syncthreads();
-        if (sizeof(T) == 4 && q + 1 < nq) TRAIL_LOAD(q + 1);
-#pragma unroll 2
-        for (int kk = 0; kk < TS; kk += 4) {
-            const int kq = kk + (lane >> 4);
-            T av[4], bv[4];
-#pragma unroll
-            for (int a = 0; a < 4; a++) av[a] = As[wr * 64 + a * 16 + (lane & 15)][kq];
-#pragma unroll
-            for (int c = 0; c < 4; c++) bv[c] = Bs[wc * 64 + c * 16 + (lane & 15)][kq];
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int c = 0; c < 4; c++) acc[a][c] = Mfma16<T>::op(av[a], bv[c], acc[a][c]);
+            __syncthreads();
+            trail_mma<T>(As, Bs, acc);
         }
     }
     const bool diag = (I2 == J2);
@@ -1506,7 +1552,9 @@ k_chol_trail_m(const MatDesc* mats, int k0, int nq)
             for (int v = 0; v < 4; v++) {
                 const int rr = wr * 64 + a * 16 + Mfma16<T>::row(lane, v);
                 const int cc = wc * 64 + c * 16 + (lane & 15);
-                if (!diag || cc <= rr) A[(rI + rr) * ld + rJ + cc] -= acc[a][c][v];
+                T* d = &A[(rI + rr) * ld + rJ + cc];
+                const T old = (sizeof(T) == 4) ? cv[(a * 4 + c) * 4 + v] : *d;
+                if (!diag || cc <= rr) *d = old - acc[a][c][v];
             }
 }
 #undef TRAIL_LOAD
