@@ -607,31 +607,43 @@ __device__ __forceinline__ void accumulate_columns(const BatchDev& bd, const uns
             qtab[g - g0] = make_int4(e0 + 16 * x, wbc, min(16, len - 16 * x), 0);
         }
         wave_sync();
+        // slot u, lane l: quarter (u >> 2) + G * (l >> 2), entries 4 * (u & 3) + (l & 3).  A lane's
+        // four entries of a quarter are mostly one pixel's (distinct columns) and the 16 quarters of
+        // one instruction lie G apart, so the LDS atomics of an instruction rarely share an address;
+        // slot groups beyond the quarters of the pass are skipped
         const int ng = min(BUILD_TQ, NQ - g0);
-        const int sub = lane & 15;
+        const int G = (ng + 15) >> 4;
         unsigned cjs[BUILD_U];
         V vs[BUILD_U];
         int wbs[BUILD_U];
 #pragma unroll
         for (int u = 0; u < BUILD_U; u++) {
-            const int gl = 4 * u + (lane >> 4);
-            const int4 t = qtab[min(gl, ng - 1)];
-            const bool ok = gl < ng && sub < t.z;
-            const int e = t.x + (ok ? sub : 0);
-            cjs[u] = lcj[e];
-            const V v = lval[e];
-            vs[u] = ok ? v : (V)0;
-            wbs[u] = t.y;
+            cjs[u] = 0;
+            vs[u] = (V)0;
+            wbs[u] = 0;
+            if ((u >> 2) < G) {
+                const int gl = (u >> 2) + G * (lane >> 2);
+                const int sub = 4 * (u & 3) + (lane & 3);
+                const int4 t = qtab[min(gl, ng - 1)];
+                const bool ok = gl < ng && sub < t.z;
+                const int e = t.x + (ok ? sub : 0);
+                cjs[u] = lcj[e];
+                const V v = lval[e];
+                vs[u] = ok ? v : (V)0;
+                wbs[u] = t.y;
+            }
         }
 #pragma unroll
         for (int u = 0; u < BUILD_U; u++) {
-            const int col = (int)(cjs[u] & 16383u);
-            const WT wt = W[wbs[u] - (int)(cjs[u] >> 14)];
-            const int slot = (loc[col >> cbs] << cbs) + (col & (CB - 1));
-            if (FIX)
-                atomicAdd((unsigned*)row + slot, (unsigned)__float2int_rn((float)(wt * vs[u])));
-            else
-                atomicAdd((double*)row + slot, (double)(wt * vs[u]));
+            if ((u >> 2) < G) {
+                const int col = (int)(cjs[u] & 16383u);
+                const WT wt = W[wbs[u] - (int)(cjs[u] >> 14)];
+                const int slot = (loc[col >> cbs] << cbs) + (col & (CB - 1));
+                if (FIX)
+                    atomicAdd((unsigned*)row + slot, (unsigned)__float2int_rn((float)(wt * vs[u])));
+                else
+                    atomicAdd((double*)row + slot, (double)(wt * vs[u]));
+            }
         }
         wave_sync();  // the next pass rewrites qtab
     }
