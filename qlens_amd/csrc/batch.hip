@@ -1927,31 +1927,34 @@ __global__ void __launch_bounds__(256) k_Ls_b(BatchDev bd)
     const double* s = bd.vec + M.vec_off;
     double acc = 0.0;
     for (int e = 0; e < bd.ucnt[item * bd.s_pix + r]; e++) acc += uval[e] * s[ucol[e]];
-    bd.model[(size_t)item * 2 * bd.s_pix + bd.s_pix + r] = acc;
+    // the unblurred model as a dense image (pixel index; inactive pixels keep the zero fill)
+    bd.model[(size_t)item * 2 * bd.s_pix + bd.s_pix + bd.act_pix[item * bd.s_pix + r]] = acc;
 }
 
+// blurred model of every active pixel: the PSF taps over the dense unblurred image (zero outside
+// the active set, which is what skipping inactive neighbours gives); one lane per pixel in memory
+// order, so the lanes of one tap read consecutive pixels
 __global__ void __launch_bounds__(256) k_model_b(BatchDev bd)
 {
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!M.valid || r >= M.n_img) return;
-    const int p = bd.act_pix[item * bd.s_pix + r];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!M.valid || p >= bd.nx * bd.ny) return;
+    if (bd.pidx[item * bd.s_pix + p] < 0) return;
     const int k = p / bd.ny, l = p % bd.ny;
     const int nxh = bd.pnx / 2, nyh = bd.pny / 2;
-    const int* pidx = bd.pidx + item * bd.s_pix;
     double* model = bd.model + (size_t)item * 2 * bd.s_pix;
     const double* v = model + bd.s_pix;
     double acc = 0.0;
     for (int pk = 0; pk < bd.pnx; pk++) {
         const int ii = k + nxh - pk;
         if (ii < 0 || ii >= bd.nx) continue;
+        const double* vr = v + (size_t)ii * bd.ny;
+        const double* pr = bd.psf + (size_t)pk * bd.pny;
         for (int pl = 0; pl < bd.pny; pl++) {
             const int jj = l + nyh - pl;
             if (jj < 0 || jj >= bd.ny) continue;
-            const int r2 = pidx[ii * bd.ny + jj];
-            if (r2 < 0) continue;
-            acc += bd.psf[pk * bd.pny + pl] * v[r2];
+            acc += pr[pl] * vr[jj];
         }
     }
     model[p] = acc;
@@ -2031,7 +2034,8 @@ void launch_evidence_b(hipStream_t st, const BatchDev& bd, int max_img, int max_
     if (max_src > 0) hipLaunchKernelGGL(k_unpermute_b, dim3((max_src + 255) / 256, bd.B), dim3(256), 0, st, bd);
     if (max_img > 0) {
         hipLaunchKernelGGL(k_Ls_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, st, bd);
-        hipLaunchKernelGGL(k_model_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, st, bd);
+        const long long npix = (long long)bd.nx * bd.ny;
+        hipLaunchKernelGGL(k_model_b, dim3((unsigned)((npix + 255) / 256), bd.B), dim3(256), 0, st, bd);
     }
     hipLaunchKernelGGL(k_evidence_b, dim3(1, bd.B), dim3(1024), 0, st, bd);
 }

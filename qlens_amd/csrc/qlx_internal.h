@@ -32,6 +32,9 @@ namespace qlx {
 
 // Per-lens meta-parameters, computed on the host with the host libm exactly as the reference's
 // update_meta_parameters does, so the device evaluates only per-ray arithmetic.
+// per batch item: QLX_MAX_LENSES lenses x 511 Patterson nodes x 4 NFW node constants
+#define QLX_NFWTAB_DOUBLES (QLX_MAX_LENSES * 511 * 4)
+
 struct LensPrep {
     int type;
     int def_mode;
@@ -83,6 +86,7 @@ struct BatchDev {
     const double* sgrid;        // QLX_SG_STRIDE doubles per item
     const int* src_n;           // 2 ints per item (snx, sny)
     const float* sur_alpha;     // surrogate deflections (2 floats per ray and item), or null
+    double* nfwtab;             // NFW quadrature node constants: QLX_NFWTAB_DOUBLES per item
     // per-item work arrays (strides from s_pix, s_cell)
     double* beta;               // s_pix*nsub*2
     int* map_cell;              // s_pix*nsub*3

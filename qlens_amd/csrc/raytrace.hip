@@ -63,11 +63,11 @@ template <> struct PatTab<float> {
     __device__ static float w(int L, int i) { return c_pat_wl32[L][i]; }
 };
 
+// kappa of the NFW at x^2 = r^2 / rs^2 (models.cpp:978-999)
 template <typename T>
-__device__ __forceinline__ T nfw_kappa_rsq(const LensPrep& L, T rsq)
+__device__ __forceinline__ T nfw_kappa_x(const LensPrep& L, T xsq)
 {
     const T ks = (T)L.ks;
-    T xsq = rsq / (T)L.rs2;
     if (xsq < (T)1e-6) return -ks * ((T)2 + log(xsq / (T)4));
     if (fabs(xsq - (T)1) < (T)1e-5) return (T)2 * ks * ((T)0.3333333333333333 - (xsq - (T)1) / (T)5.0);
     T F;
@@ -79,6 +79,12 @@ __device__ __forceinline__ T nfw_kappa_rsq(const LensPrep& L, T rsq)
         F = atanh(s) / s;
     }
     return (T)2 * ks * ((T)1 - F) / (xsq - (T)1);
+}
+
+template <typename T>
+__device__ __forceinline__ T nfw_kappa_rsq(const LensPrep& L, T rsq)
+{
+    return nfw_kappa_x<T>(L, rsq / (T)L.rs2);
 }
 
 template <typename T>
@@ -148,6 +154,72 @@ __device__ __forceinline__ int nfw_j01(const LensPrep& L, T x, T y, T tol, T& j0
     return nev;
 }
 
+// Node constants of one NFW lens (every ray of a batch item shares them): with w = (1 + x_i)/2,
+// u = w^2 and qfac = 1 - eps u, node i holds A = u fsqinv / rs^2, B = A / qfac,
+// C = 2 w / sqrt(qfac) and D = C / qfac, so that the kappa argument is A x^2 + B y^2 and the J0 and
+// J1 integrands are C kappa and D kappa (profile.cpp:2703-2734 with the per-node factors hoisted).
+static_assert(QLX_NFWTAB_DOUBLES == QLX_MAX_LENSES * QLX_PAT_N * 4, "NFW node table size");
+__global__ void __launch_bounds__(256) k_nfw_tables(BatchDev bd)
+{
+    const int item = blockIdx.y / QLX_MAX_LENSES, l = blockIdx.y % QLX_MAX_LENSES;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= QLX_PAT_N) return;
+    const ModelPrep& m = bd.models[item];
+    if (l >= m.nlens || m.lens[l].type != QLX_LENS_NFW) return;
+    const LensPrep& L = m.lens[l];
+    const double w = 0.5 + 0.5 * c_pat_x[i];
+    const double u = w * w;
+    const double qfac = 1.0 - L.eps * u;
+    const double a = u * L.fsqinv / L.rs2;
+    const double c = 2.0 * w / sqrt(qfac);
+    double* t = bd.nfwtab + ((size_t)(item * QLX_MAX_LENSES + l) * QLX_PAT_N + i) * 4;
+    t[0] = a;
+    t[1] = a / qfac;
+    t[2] = c;
+    t[3] = c / qfac;
+}
+
+// nfw_j01 with the node constants of k_nfw_tables (tab: this lens's QLX_PAT_N x 4 doubles)
+template <typename T>
+__device__ __forceinline__ int nfw_j01_tab(const LensPrep& L, const double* tab, T x, T y, T tol, T& j0, T& j1)
+{
+    const T xsq = x * x, ysq = y * y;
+    T S0[QLX_PAT_NLEV], S1[QLX_PAT_NLEV];
+#pragma unroll
+    for (int k = 0; k < QLX_PAT_NLEV; k++) { S0[k] = 0; S1[k] = 0; }
+    bool d0 = false, d1 = false;
+    T r0 = 0, r1 = 0;
+    int nev = 0;
+#pragma unroll
+    for (int m = 0; m < QLX_PAT_NLEV; m++) {
+        const int order = (1 << (m + 1)) - 1;
+        const int step = (QLX_PAT_N + 1) / (order + 1);
+        for (int i = step - 1; i < QLX_PAT_N; i += 2 * step) {
+            const double* tn = tab + 4 * i;
+            const T kap = nfw_kappa_x<T>(L, (T)tn[0] * xsq + (T)tn[1] * ysq);
+            const T f0 = (T)tn[2] * kap;
+            const T f1 = (T)tn[3] * kap;
+#pragma unroll
+            for (int k = m; k < QLX_PAT_NLEV; k++) {
+                T wk = PatTab<T>::w(k, i);
+                S0[k] += wk * f0;
+                S1[k] += wk * f1;
+            }
+            nev += 2;
+        }
+        if (m > 1) {
+            if (!d0 && fabs(S0[m] - S0[m - 1]) < tol * fabs(S0[m])) { d0 = true; r0 = S0[m]; }
+            if (!d1 && fabs(S1[m] - S1[m - 1]) < tol * fabs(S1[m])) { d1 = true; r1 = S1[m]; }
+            if (d0 && d1) break;
+        }
+    }
+    if (!d0) r0 = S0[QLX_PAT_NLEV - 1];
+    if (!d1) r1 = S1[QLX_PAT_NLEV - 1];
+    j0 = (T)L.sqrt1meps * ((T)0.5 * r0);
+    j1 = (T)L.sqrt1meps * ((T)0.5 * r1);
+    return nev;
+}
+
 template <typename T>
 __device__ __forceinline__ T patterson_k(const LensPrep& L, T x, T y, T tol, T nph)
 {
@@ -177,9 +249,11 @@ __device__ __forceinline__ T patterson_k(const LensPrep& L, T x, T y, T tol, T n
     return (T)L.sqrt1meps * ((T)0.5 * r);
 }
 
-// Deflection of one lens at (x, y) (profile.cpp:1782-1805; shear models.cpp:2266-2272).
-template <typename T>
-__device__ __forceinline__ void lens_deflection(const LensPrep& L, T x, T y, T tol, T& d0, T& d1, int& nev)
+// Deflection of one lens at (x, y) (profile.cpp:1782-1805; shear models.cpp:2266-2272).  TAB: the
+// NFW integrals use the node constants tab of k_nfw_tables; otherwise they are formed per node.
+template <typename T, bool TAB = false>
+__device__ __forceinline__ void lens_deflection(const LensPrep& L, T x, T y, T tol, T& d0, T& d1, int& nev,
+                                                const double* tab = nullptr)
 {
     x -= (T)L.xc;
     y -= (T)L.yc;
@@ -215,7 +289,8 @@ __device__ __forceinline__ void lens_deflection(const LensPrep& L, T x, T y, T t
                 a1 = 0;
             } else {
                 T j0, j1;
-                nev += nfw_j01<T>(L, x, y, tol, j0, j1);
+                if constexpr (TAB) nev += nfw_j01_tab<T>(L, tab, x, y, tol, j0, j1);
+                else nev += nfw_j01<T>(L, x, y, tol, j0, j1);
                 a0 = x * j0;
                 a1 = y * j1;
             }
@@ -313,7 +388,8 @@ __global__ void __launch_bounds__(256) k_raytrace_map_b(BatchDev bd, T tol)
             d0 = (T)sa0;
             d1 = (T)bd.sur_alpha[ro * 2 + 1];
         } else {
-            lens_deflection<T>(m.lens[l], (T)sx, (T)sy, tol, d0, d1, nev);
+            lens_deflection<T, true>(m.lens[l], (T)sx, (T)sy, tol, d0, d1, nev,
+                                     bd.nfwtab + (size_t)(item * QLX_MAX_LENSES + l) * QLX_PAT_N * 4);
         }
         a0 += d0;
         a1 += d1;
@@ -354,6 +430,7 @@ void launch_raytrace_map_b(hipStream_t stream, const BatchDev& bd, int ray_fp32)
 {
     const long long total = (long long)bd.nx * bd.ny * bd.nsub;
     const unsigned blocks = (unsigned)((total + 255) / 256);
+    hipLaunchKernelGGL(k_nfw_tables, dim3((QLX_PAT_N + 255) / 256, bd.B * QLX_MAX_LENSES), dim3(256), 0, stream, bd);
     if (ray_fp32)
         hipLaunchKernelGGL(k_raytrace_map_b<float>, dim3(blocks, bd.B), dim3(256), 0, stream, bd, (float)bd.tol);
     else

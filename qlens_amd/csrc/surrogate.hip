@@ -210,47 +210,46 @@ __device__ __forceinline__ void decode(const float* o, const float* norm, double
 }
 
 // ------------------------------------------------------------------------------------------------
-// Inference: a workgroup keeps W2..W4 and the output layer (BF16, padded to 16 rows) resident in
-// LDS and runs tiles of SB samples through all layers; the output layer is one more MFMA pass.
+// Inference: 8 waves run tiles of SBI samples through all layers.  Wave w produces outputs
+// [16w, 16w+16) of every hidden layer, so it only ever needs those 16 rows of W2..W4: it keeps its
+// MFMA B operands of the three layers and of the output layer (5 rows, padded to 16) in registers
+// for the whole kernel, and LDS holds only the two activation tiles.
 // ------------------------------------------------------------------------------------------------
-#define INF_LDS_ELEMS (3 * NH * HP + 16 * HP + 2 * SB * HP)
-__device__ void infer_stage_weights(const SurDev& net, uint16_t* Wall, uint16_t* W5s)
-{
-    for (int l = 0; l < 3; l++) stage_w(net.Wb + l * NH * NH, Wall + l * NH * HP);
-    for (int e = threadIdx.x; e < 16 * NH; e += blockDim.x) {
-        const int o = e / NH, k = e - o * NH;
-        W5s[o * HP + k] = (o < 5) ? bf16_rne(net.W5r[o * NH + k]) : (uint16_t)0;
-    }
-}
+#define SBI 128  // samples per inference tile
+#define INF_LDS_ELEMS (2 * SBI * HP)
+struct InferRegs {
+    bf16x8_t w[3][4];  // W2..W4: row 16w + lane%16, k = 32 ks + 8 (lane/16) .. + 7
+    bf16x8_t w5[4];    // output layer row lane%16 (zero for rows >= 5)
+};
 
-// 64 x 128 x 128 product with 8 waves: wave w computes columns [16w, 16w+16) for all 64 rows
-__device__ __forceinline__ void mm_64x128_w8(const uint16_t* A, const uint16_t* B, float4_t acc[4])
+__device__ __forceinline__ void infer_load_regs(const SurDev& net, InferRegs& R)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int o = 16 * w + (lane & 15), o5 = lane & 15;
 #pragma unroll
-    for (int st = 0; st < 4; st++) acc[st] = (float4_t){0, 0, 0, 0};
+    for (int ks = 0; ks < 4; ks++) {
+        const int k = 32 * ks + 8 * (lane >> 4);
 #pragma unroll
-    for (int k0 = 0; k0 < NH; k0 += 32) {
-        const int kq = k0 + 8 * (lane >> 4);
-        const bf16x8_t b = *(const bf16x8_t*)&B[(16 * w + (lane & 15)) * HP + kq];
+        for (int l = 0; l < 3; l++) R.w[l][ks] = *(const bf16x8_t*)&net.Wb[l * NH * NH + o * NH + k];
+        uint16_t h[8];
 #pragma unroll
-        for (int st = 0; st < 4; st++) {
-            const bf16x8_t a = *(const bf16x8_t*)&A[(16 * st + (lane & 15)) * HP + kq];
-            acc[st] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[st], 0, 0, 0);
-        }
+        for (int e = 0; e < 8; e++) h[e] = (o5 < 5) ? bf16_rne(net.W5r[o5 * NH + k + e]) : (uint16_t)0;
+        R.w5[ks] = __builtin_bit_cast(bf16x8_t, make_uint4(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16),
+                                                           h[4] | ((unsigned)h[5] << 16), h[6] | ((unsigned)h[7] << 16)));
     }
 }
 
-// one tile of SB samples with 512 threads (8 waves); fs: standardized features; out: standardized
+// one tile of SBI samples with 512 threads (8 waves); fs: standardized features; out: standardized
 // network outputs; bias: b1..b4 (4 x 128) and b5 staged in LDS
-__device__ void infer_tile(const SurDev& net, const uint16_t* Wall, const uint16_t* W5s, const float* bias, uint16_t* Ha, uint16_t* Hb,
+__device__ void infer_tile(const SurDev& net, const InferRegs& R, const float* bias, uint16_t* Ha, uint16_t* Hb,
                            const float (*fs)[4], float (*out)[5])
 {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     {
-        const int s = tid & 63;
+        // first layer: thread (sample tid % SBI, outputs 32 (tid / SBI) .. + 31)
+        const int s = tid & (SBI - 1), oq = tid / SBI;
         const float f0 = fs[s][0], f1 = fs[s][1], f2 = fs[s][2], f3 = fs[s][3];
-        for (int o = 16 * w; o < 16 * w + 16; o++) {
+        for (int o = 32 * oq; o < 32 * oq + 32; o++) {
             const float* wr = net.W1r + o * 4;
             float z = bias[o];
             z = fmaf(wr[0], f0, z);
@@ -262,15 +261,25 @@ __device__ void infer_tile(const SurDev& net, const uint16_t* Wall, const uint16
     }
     uint16_t* Hin = Ha;
     uint16_t* Hout = Hb;
+#pragma unroll
     for (int layer = 0; layer < 3; layer++) {
         __syncthreads();
-        float4_t acc[4];
-        mm_64x128_w8(Hin, Wall + layer * NH * HP, acc);
-        const float* b = bias + (layer + 1) * NH;
-        const int o = 16 * w + (lane & 15);
-        const float bo = b[o];
+        float4_t acc[SBI / 16];
 #pragma unroll
-        for (int st = 0; st < 4; st++)
+        for (int st = 0; st < SBI / 16; st++) acc[st] = (float4_t){0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) {
+            const int kq = 32 * ks + 8 * (lane >> 4);
+#pragma unroll
+            for (int st = 0; st < SBI / 16; st++) {
+                const bf16x8_t a = *(const bf16x8_t*)&Hin[(16 * st + (lane & 15)) * HP + kq];
+                acc[st] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, R.w[layer][ks], acc[st], 0, 0, 0);
+            }
+        }
+        const int o = 16 * w + (lane & 15);
+        const float bo = bias[(layer + 1) * NH + o];
+#pragma unroll
+        for (int st = 0; st < SBI / 16; st++)
 #pragma unroll
             for (int v = 0; v < 4; v++) Hout[(16 * st + 4 * (lane >> 4) + v) * HP + o] = bf16_rne(silu_hw(acc[st][v] + bo));
         uint16_t* t = Hin;
@@ -278,15 +287,14 @@ __device__ void infer_tile(const SurDev& net, const uint16_t* Wall, const uint16
         Hout = t;
     }
     __syncthreads();
-    // output layer: waves 0..3 take samples [16w, 16w+16), 16 (5 used) outputs, K = 128
-    if (w < 4) {
+    // output layer: wave w takes samples [16w, 16w+16), 16 (5 used) outputs, K = 128
+    {
         float4_t acc = (float4_t){0, 0, 0, 0};
 #pragma unroll
-        for (int k0 = 0; k0 < NH; k0 += 32) {
-            const int kq = k0 + 8 * (lane >> 4);
+        for (int ks = 0; ks < 4; ks++) {
+            const int kq = 32 * ks + 8 * (lane >> 4);
             const bf16x8_t a = *(const bf16x8_t*)&Hin[(16 * w + (lane & 15)) * HP + kq];
-            const bf16x8_t bv = *(const bf16x8_t*)&W5s[(lane & 15) * HP + kq];
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, R.w5[ks], acc, 0, 0, 0);
         }
         const int o = lane & 15;
         if (o < 5)
@@ -310,22 +318,21 @@ __device__ void infer_stage_bias(const SurDev& net, float* bias)
 __global__ void __launch_bounds__(512) k_sur_eval(SurDev net, int n, const float* in4, float* out5)
 {
     extern __shared__ __attribute__((aligned(16))) uint16_t lds_inf[];
-    uint16_t* Wall = lds_inf;
-    uint16_t* W5s = Wall + 3 * NH * HP;
-    uint16_t* Ha = W5s + 16 * HP;
-    uint16_t* Hb = Ha + SB * HP;
-    __shared__ float fs[SB][4];
-    __shared__ float os[SB][5];
-    __shared__ double raw[SB][2];
+    uint16_t* Ha = lds_inf;
+    uint16_t* Hb = Ha + SBI * HP;
+    __shared__ float fs[SBI][4];
+    __shared__ float os[SBI][5];
+    __shared__ double raw[SBI][2];
     __shared__ float bias[4 * NH + 5];
     const int tid = threadIdx.x;
-    infer_stage_weights(net, Wall, W5s);
+    InferRegs R;
+    infer_load_regs(net, R);
     infer_stage_bias(net, bias);
-    const int ntile = (n + SB - 1) / SB;
+    const int ntile = (n + SBI - 1) / SBI;
     for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
-        const int s0 = tile * SB;
+        const int s0 = tile * SBI;
         __syncthreads();
-        if (tid < SB) {
+        if (tid < SBI) {
             const int i = s0 + tid;
             double lnr = 0, phi = 0, q = 1, lnrs = 0;
             if (i < n) {
@@ -341,8 +348,8 @@ __global__ void __launch_bounds__(512) k_sur_eval(SurDev net, int n, const float
             features(lnr, atan2(fabs(y), fabs(x)), q, lnrs, fs[tid]);
         }
         __syncthreads();
-        infer_tile(net, Wall, W5s, bias, Ha, Hb, fs, os);
-        if (tid < SB && s0 + tid < n) {
+        infer_tile(net, R, bias, Ha, Hb, fs, os);
+        if (tid < SBI && s0 + tid < n) {
             double d5[5];
             decode(os[tid], net.norm, raw[tid][0], raw[tid][1], d5);
             for (int k = 0; k < 5; k++) out5[5 * (s0 + tid) + k] = (float)d5[k];
@@ -364,7 +371,7 @@ void launch_sur_eval(hipStream_t s, const SurDev& net, int n, const float* in4, 
 {
     if (n <= 0) return;
     sur_allow_lds((const void*)k_sur_eval);
-    const int ntile = (n + SB - 1) / SB;
+    const int ntile = (n + SBI - 1) / SBI;
     hipLaunchKernelGGL(k_sur_eval, dim3(ntile < 1024 ? ntile : 1024), dim3(512), INF_LDS_ELEMS * 2, s, net, n, in4, out5);
 }
 
@@ -376,28 +383,27 @@ void launch_sur_eval(hipStream_t s, const SurDev& net, int n, const float* in4, 
 __global__ void __launch_bounds__(512) k_sur_rays(SurDev net, BatchDev bd, float* sur_alpha)
 {
     extern __shared__ __attribute__((aligned(16))) uint16_t lds_inf[];
-    uint16_t* Wall = lds_inf;
-    uint16_t* W5s = Wall + 3 * NH * HP;
-    uint16_t* Ha = W5s + 16 * HP;
-    uint16_t* Hb = Ha + SB * HP;
-    __shared__ float fs[SB][4];
-    __shared__ float os[SB][5];
-    __shared__ double raw[SB][2];
-    __shared__ int inbox[SB];
+    uint16_t* Ha = lds_inf;
+    uint16_t* Hb = Ha + SBI * HP;
+    __shared__ float fs[SBI][4];
+    __shared__ float os[SBI][5];
+    __shared__ double raw[SBI][2];
+    __shared__ int inbox[SBI];
     __shared__ float bias[4 * NH + 5];
     const int tid = threadIdx.x;
-    infer_stage_weights(net, Wall, W5s);
+    InferRegs R;
+    infer_load_regs(net, R);
     infer_stage_bias(net, bias);
     const long long nray = (long long)bd.nx * bd.ny * bd.nsub;
-    const long long tpi = (nray + SB - 1) / SB;  // tiles per item
+    const long long tpi = (nray + SBI - 1) / SBI;  // tiles per item
     for (long long tile = blockIdx.x; tile < tpi * bd.B; tile += gridDim.x) {
         const int item = (int)(tile / tpi);
         const ModelPrep& m = bd.models[item];
         if (m.sur_lens < 0) continue;  // uniform over the workgroup
         const LensPrep& L = m.lens[m.sur_lens];
-        const long long t0 = (tile - (long long)item * tpi) * SB;
+        const long long t0 = (tile - (long long)item * tpi) * SBI;
         __syncthreads();
-        if (tid < SB) {
+        if (tid < SBI) {
             const int t = (int)(t0 + tid);  // ray index within the item (the image holds < 2^24 pixels x 16 rays)
             int ok = 0;
             double x = 0, y = 0;
@@ -427,8 +433,8 @@ __global__ void __launch_bounds__(512) k_sur_rays(SurDev net, BatchDev bd, float
             inbox[tid] = ok;
         }
         __syncthreads();
-        infer_tile(net, Wall, W5s, bias, Ha, Hb, fs, os);
-        if (tid < SB && t0 + tid < nray) {
+        infer_tile(net, R, bias, Ha, Hb, fs, os);
+        if (tid < SBI && t0 + tid < nray) {
             float ax = __int_as_float(0x7fc00000), ay = ax;  // quiet NaN: quadrature fallback
             if (inbox[tid]) {
                 const float j0 = __expf(os[tid][0] * net.norm[5] + net.norm[0]);
@@ -453,7 +459,7 @@ void launch_sur_rays(hipStream_t s, const SurDev& net, const BatchDev& bd, float
 {
     sur_allow_lds((const void*)k_sur_rays);
     const long long nray = (long long)bd.nx * bd.ny * bd.nsub;
-    const long long tiles = (nray + SB - 1) / SB * bd.B;
+    const long long tiles = (nray + SBI - 1) / SBI * bd.B;
     const unsigned grid = (unsigned)(tiles < 1024 ? tiles : 1024);
     hipLaunchKernelGGL(k_sur_rays, dim3(grid), dim3(512), INF_LDS_ELEMS * 2, s, net, bd, sur_alpha);
 }
