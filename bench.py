@@ -37,25 +37,6 @@ PEAK_FP64_TFLOPS = 78.6
 METRIC = "log-evidence evals/s (200² img, 60² src): surrogate vs quad, |ΔlogE|<0.1"
 
 
-def proposals(case, n, seed=1234):
-    """Seeded 1% Gaussian ball around the truth: SIE (b, q), NFW (ks, rs, q), shear gamma."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for _ in range(n):
-        m = []
-        for t, p in case["lenses"]:
-            p = list(p)
-            idx = {1: (0, 1), 2: (0, 1, 2), 3: (0,)}[t]
-            for i in idx:
-                p[i] *= 1 + 0.01 * rng.standard_normal()
-            if t in (1, 2):
-                k = 1 if t == 1 else 2
-                p[k] = min(p[k], 0.999)
-            m.append((t, tuple(p)))
-        out.append(m)
-    return out
-
-
 def read_traffic(name):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE) and scaled by
     a copy-kernel calibration; stored as 'key: value' lines in profiles/<name>.txt."""
@@ -138,6 +119,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-deflection leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-proposal uneven-grid leg")
     ap.add_argument("--ray-fp32", action="store_true", help="FP32 ray shooting in the fast path")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (known HBM bytes: it reads and "
@@ -172,7 +154,7 @@ def main():
 
     B = args.batch
     n_fast = (args.warmup + args.steps) * B
-    props = proposals(case, max(n_fast, args.plain_steps + 2))
+    props = W.proposals(case, max(n_fast, args.plain_steps + 2))
 
     # plain path: FP64, one proposal per call
     inv.load_workload(case, data, precision=qlx.QLX_PREC_FP64)
@@ -213,6 +195,25 @@ def main():
                "quadrature_raytrace_ms": stage_ms["raytrace"], "train_seconds": round(train_s, 2),
                "train_final_loss": float("%.3g" % sur_loss),
                "note": "BF16 4x128 MLP: ~1e-3 relative deflection error; not used for `value` unless it meets |dlnE| < 0.1"}
+
+    # batched evidence over uneven source grids: 256 proposals, each with its own 20^2..60^2 grid, in
+    # one call, FP64 and mixed precision (one warm-up call each)
+    batched = None
+    if not args.no_batched:
+        _, bmodels, bgrids = W.c4()
+        bt = {}
+        for tag, prec in (("fp64", qlx.QLX_PREC_FP64), ("mixed", qlx.QLX_PREC_MIXED)):
+            inv.load_workload(case, data, precision=prec)
+            inv.invert_image_surface_brightness_map(bmodels, src_n=bgrids)
+            t0 = time.perf_counter()
+            bres = inv.invert_image_surface_brightness_map(bmodels, src_n=bgrids)
+            bt[tag] = (time.perf_counter() - t0, bres)
+        bd = max(abs(m["minus2lnE"] - f["minus2lnE"]) / 2 for m, f in zip(bt["mixed"][1], bt["fp64"][1]))
+        batched = {"evals": len(bmodels), "grid_sides": [20, 30, 40, 50, 60],
+                   "fp64_evals_per_s": round(len(bmodels) / bt["fp64"][0], 2),
+                   "mixed_evals_per_s": round(len(bmodels) / bt["mixed"][0], 2),
+                   "max_abs_dlnE_mixed_vs_fp64": float("%.3g" % bd),
+                   "status_errors": sorted(set(r["status"] for t in bt.values() for r in t[1] if r["status"] != 0))}
 
     # roofline of the dominant fast-path kernel: the BF16 SYRK (F = Lb^T C Lb), per launch = one wave
     syrk_s = float(np.mean(syrk_ms)) * 1e-3 if syrk_ms else None
@@ -255,6 +256,7 @@ def main():
         "max_abs_dlnE_vs_fp64": float("%.3g" % dlnE),
         "item_status_errors": bad,
         "surrogate_leg": sur,
+        "batched_uneven_grids": batched,
         "roofline": roof,
         "cpu_baseline": None,
     }

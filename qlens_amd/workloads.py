@@ -127,4 +127,44 @@ def c3(n_img=400, nsplit=4):
     )
 
 
-WORKLOADS = {"C1": c1, "C2": c2, "C2s": c2_small, "C3": c3}
+def c5(n_img=400, pix=0.02, n_src=100):
+    """Mixed precision at scale: c2's lens and three-Sersic source over a 400x400 image at
+    0.02"/pix (2x2 supersampling) on a 100x100 source grid (F is 10000 x 10000 dense), c2's
+    Moffat PSF sampled at the finer pixel scale."""
+    case = c2(n_img=n_img, pix=pix, n_src=n_src, src_half=1.0, psf_n=21)
+    case["name"] = "C5"
+    return case
+
+
+def proposals(case, n, seed=1234):
+    """Seeded 1% Gaussian ball around the truth: SIE (b, q), NFW (ks, rs, q), shear gamma."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        m = []
+        for t, p in case["lenses"]:
+            p = list(p)
+            idx = {SIE: (0, 1), NFW: (0, 1, 2), SHEAR: (0,)}[t]
+            for i in idx:
+                p[i] *= 1 + 0.01 * rng.standard_normal()
+            if t in (SIE, NFW):
+                k = 1 if t == SIE else 2
+                p[k] = min(p[k], 0.999)
+            m.append((t, tuple(p)))
+        out.append(m)
+    return out
+
+
+def c4(n=256, seed=4321):
+    """Batched evidence: c2's image and PSF, n proposals from the 1% ball around c2's truth, each
+    with its own square source grid drawn uniformly from {20, 30, 40, 50, 60} cells per side.
+    Returns (case, models, grids)."""
+    case = c2()
+    case["name"] = "C4"
+    rng = np.random.default_rng(seed)
+    sides = rng.choice([20, 30, 40, 50, 60], size=n)
+    grids = [(int(k), int(k)) for k in sides]
+    return case, proposals(case, n, seed + 1), grids
+
+
+WORKLOADS = {"C1": c1, "C2": c2, "C2s": c2_small, "C3": c3, "C5": c5}
