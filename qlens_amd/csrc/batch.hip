@@ -182,7 +182,9 @@ void launch_compact_b(hipStream_t s, const BatchDev& bd)
     hipLaunchKernelGGL(k_compact_b, dim3(1, bd.B), dim3(1024), 0, s, bd);
 }
 
-// merged row of L per active pixel: CSR order, duplicates summed from 0 in that order
+// merged row of L per active pixel: CSR order, duplicates summed from 0 in that order.  Up to
+// nsplit 2 (12 entries) the row is built in registers (predicated, no dynamic indexing) and stored
+// once; larger splits build it in place.
 __global__ void __launch_bounds__(256) k_merge_lrows_b(BatchDev bd)
 {
     const int item = blockIdx.y;
@@ -196,6 +198,51 @@ __global__ void __launch_bounds__(256) k_merge_lrows_b(BatchDev bd)
     const int* tix = bd.tix + item * bd.s_cell;
     int* uc = bd.ucol + ((size_t)item * bd.s_pix + r) * U;
     double* uv = bd.uval + ((size_t)item * bd.s_pix + r) * U;
+    if (nsub <= 4) {
+        constexpr int MU = 12;
+        int cols[MU];
+        double vals[MU];
+#pragma unroll
+        for (int j = 0; j < MU; j++) {
+            cols[j] = -1;
+            vals[j] = 0.0;
+        }
+        int n = 0;
+        for (int k = 0; k < nsub; k++) {
+            const int* mc = mcb + (p * nsub + k) * 3;
+            const double* mw = mwb + (p * nsub + k) * 3;
+            if (mc[0] < 0) continue;
+#pragma unroll
+            for (int e = 0; e < 3; e++) {
+                const int col = tix[mc[e]];
+                const double w = mw[e];
+                bool found = false;
+#pragma unroll
+                for (int j = 0; j < MU; j++)
+                    if (!found && j < n && cols[j] == col) {
+                        vals[j] += w;
+                        found = true;
+                    }
+                if (!found) {
+#pragma unroll
+                    for (int j = 0; j < MU; j++)
+                        if (j == n) {
+                            cols[j] = col;
+                            vals[j] = 0.0 + w;
+                        }
+                    n++;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < MU; j++)
+            if (j < n) {
+                uc[j] = cols[j];
+                uv[j] = vals[j];
+            }
+        bd.ucnt[item * bd.s_pix + r] = n;
+        return;
+    }
     int cnt = 0;
     for (int k = 0; k < nsub; k++) {
         const int* mc = mcb + (p * nsub + k) * 3;
@@ -431,25 +478,34 @@ __global__ void __launch_bounds__(256) k_row_support_b(BatchDev bd, int use_seen
     const int* perm = bd.perm + item * bd.s_pix;
     const int* act = bd.act_pix + item * bd.s_pix;
     const int rows = min(TS, M.n_img - kt * TS);
-    for (int q = tid; q < rows * ntap; q += 256) {
-        const int rr = q / ntap, t = q - rr * ntap;
-        const int p = act[perm[kt * TS + rr]];
-        const int k = p / bd.ny, l = p % bd.ny;
+    // the chunk's row pixels, then each thread keeps its taps fixed and walks the rows (no index
+    // divisions in the loop)
+    __shared__ int rk[TS], rl[TS];
+    if (tid < rows) {
+        const int p = act[perm[kt * TS + tid]];
+        rk[tid] = p / bd.ny;
+        rl[tid] = p - (p / bd.ny) * bd.ny;
+    }
+    __syncthreads();
+    for (int t = tid; t < ntap; t += 256) {
         const int pk = t / bd.pny, pl = t - pk * bd.pny;
-        const int ii = k + nxh - pk, jj = l + nyh - pl;
-        if (ii < 0 || ii >= bd.nx || jj < 0 || jj >= bd.ny) continue;
-        const int r2 = pidx[ii * bd.ny + jj];
-        if (r2 < 0) continue;
-        if (use_seen) {  // each neighbouring pixel's columns are taken once per chunk
-            const unsigned bit = 1u << (r2 & 31);
-            if (atomicOr(&seen[r2 >> 5], bit) & bit) continue;
-        }
-        const int cnt = ucnt[r2];
-        for (int e = 0; e < cnt; e++) {
-            const int cb = ucol[(size_t)r2 * U + e] >> M.cbs;
-            const unsigned bit = 1u << (cb & 31);
+        const int dk = nxh - pk, dl = nyh - pl;
+        for (int rr = 0; rr < rows; rr++) {
+            const int ii = rk[rr] + dk, jj = rl[rr] + dl;
+            if (ii < 0 || ii >= bd.nx || jj < 0 || jj >= bd.ny) continue;
+            const int r2 = pidx[ii * bd.ny + jj];
+            if (r2 < 0) continue;
+            if (use_seen) {  // each neighbouring pixel's columns are taken once per chunk
+                const unsigned bit = 1u << (r2 & 31);
+                if (atomicOr(&seen[r2 >> 5], bit) & bit) continue;
+            }
+            const int cnt = ucnt[r2];
+            for (int e = 0; e < cnt; e++) {
+                const int cb = ucol[(size_t)r2 * U + e] >> M.cbs;
+                const unsigned bit = 1u << (cb & 31);
 #pragma unroll
-            for (int w = 0; w < QLX_MAX_CBW; w++) m[w] |= (w == (cb >> 5)) ? bit : 0u;
+                for (int w = 0; w < QLX_MAX_CBW; w++) m[w] |= (w == (cb >> 5)) ? bit : 0u;
+            }
         }
     }
 #pragma unroll
