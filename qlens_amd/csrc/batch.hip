@@ -626,7 +626,7 @@ __device__ __forceinline__ void window_range(const BatchDev& bd, const int* eptr
     e1 = eptr[t0 + i1 + 1];
 }
 
-#define BUILD_U 32              // entry slots per lane and pass
+#define BUILD_U 48              // entry slots per lane and pass
 #define BUILD_TQ (4 * BUILD_U)  // quarters per pass
 
 template <bool FIX>
@@ -1194,27 +1194,30 @@ __global__ void __launch_bounds__(256) k_prep_f_b(BatchDev bd)
     double* fdiag = bd.vec + M.vec_off + 3 * (size_t)ld;
     const int* pis = bd.pisrc + item * bd.s_cell;
     const double lam = bd.lambda;
+    // 16 elements per thread, unrolled so that their gathers (pisrc, R tile, F) are in flight together
+#pragma unroll 8
     for (int e = threadIdx.x; e < TS * TS; e += 256) {
         const int rr = e >> 6, cc = e & 63;
         const int r = I * TS + rr, c = J * TS + cc;
-        if (c > r) continue;
-        const size_t t = (size_t)r * ld + c;
-        T v;
-        if (r < n) {
-            // R is held in the reference order: F's internal indices map through pisrc
-            const int a = pis[r], b = pis[c];
-            const int hi = a > b ? a : b, lo = a > b ? b : a;
-            const double rv = ((hi >> 6) - (lo >> 6) <= M.bwb) ? *rtile(bd, M, hi, lo) : 0.0;
-            v = (T)((double)F[t] + lam * rv);
-        } else if (r == n) {
-            v = (c == n && mixed) ? (T)1e30 : F[t];
-        } else {
-            v = (r == c) ? (T)1 : (T)0;
-        }
-        F[t] = v;
-        if (mixed) {
-            tt[cc][rr] = v;
-            if (c == r && r < n) fdiag[r] = (double)v;
+        if (c <= r) {
+            const size_t t = (size_t)r * ld + c;
+            T v;
+            if (r < n) {
+                // R is held in the reference order: F's internal indices map through pisrc
+                const int a = pis[r], b = pis[c];
+                const int hi = a > b ? a : b, lo = a > b ? b : a;
+                const double rv = ((hi >> 6) - (lo >> 6) <= M.bwb) ? *rtile(bd, M, hi, lo) : 0.0;
+                v = (T)((double)F[t] + lam * rv);
+            } else if (r == n) {
+                v = (c == n && mixed) ? (T)1e30 : F[t];
+            } else {
+                v = (r == c) ? (T)1 : (T)0;
+            }
+            F[t] = v;
+            if (mixed) {
+                tt[cc][rr] = v;
+                if (c == r && r < n) fdiag[r] = (double)v;
+            }
         }
     }
     if (!mixed) return;
