@@ -234,13 +234,23 @@ __global__ void __launch_bounds__(256) k_merge_lrows_b(BatchDev bd)
                 }
             }
         }
+        unsigned m[QLX_MAX_CBW];
+#pragma unroll
+        for (int w = 0; w < QLX_MAX_CBW; w++) m[w] = 0;
 #pragma unroll
         for (int j = 0; j < MU; j++)
             if (j < n) {
                 uc[j] = cols[j];
                 uv[j] = vals[j];
+                const int cb = cols[j] >> M.cbs;
+#pragma unroll
+                for (int w = 0; w < QLX_MAX_CBW; w++) m[w] |= (w == (cb >> 5)) ? (1u << (cb & 31)) : 0u;
             }
         bd.ucnt[item * bd.s_pix + r] = n;
+        unsigned* pm = bd.pmask + ((size_t)item * bd.s_pix + r) * QLX_MAX_CBW;
+#pragma unroll
+        for (int w = 0; w < QLX_MAX_CBW; w++)
+            if (w < M.ncbw) pm[w] = m[w];
         return;
     }
     int cnt = 0;
@@ -262,6 +272,12 @@ __global__ void __launch_bounds__(256) k_merge_lrows_b(BatchDev bd)
         }
     }
     bd.ucnt[item * bd.s_pix + r] = cnt;
+    unsigned* pm = bd.pmask + ((size_t)item * bd.s_pix + r) * QLX_MAX_CBW;
+    for (int w = 0; w < M.ncbw; w++) pm[w] = 0;
+    for (int e = 0; e < cnt; e++) {
+        const int cb = uc[e] >> M.cbs;
+        pm[cb >> 5] |= 1u << (cb & 31);
+    }
 }
 
 void launch_merge_lrows_b(hipStream_t s, const BatchDev& bd, int max_img)
@@ -445,67 +461,48 @@ void launch_sort_rows_b(hipStream_t s, const BatchDev& bd, int max_img, int max_
 // chunk support: union of the 64-column blocks touched by the PSF taps of a chunk's 64 rows
 // (one workgroup per chunk), then per-item slot bases of the compact block pool.
 // ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ void mask_set(unsigned (&m)[QLX_MAX_CBW], int cb)
-{
-#pragma unroll
-    for (int w = 0; w < QLX_MAX_CBW; w++)
-        if (w == (cb >> 5)) m[w] |= 1u << (cb & 31);
-}
 
-__global__ void __launch_bounds__(256) k_row_support_b(BatchDev bd, int use_seen)
+__global__ void __launch_bounds__(256) k_row_support_b(BatchDev bd)
 {
-    extern __shared__ unsigned seen[];  // one bit per active image pixel (when use_seen)
     __shared__ unsigned msk[QLX_MAX_CBW];
+    __shared__ int rk[TS], rl[TS];
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     const int kt = blockIdx.x;
     if (!M.valid || kt >= M.nkt) return;
     const int tid = threadIdx.x;
     if (tid < QLX_MAX_CBW) msk[tid] = 0;
-    if (use_seen)
-        for (int wd = tid; wd < (M.n_img + 31) / 32; wd += 256) seen[wd] = 0;
-    __syncthreads();
-    unsigned m[QLX_MAX_CBW];
-#pragma unroll
-    for (int w = 0; w < QLX_MAX_CBW; w++) m[w] = 0;
-    if (tid == 0) mask_set(m, M.n_src >> M.cbs);  // the data column
-    const int ntap = bd.pnx * bd.pny;
-    const int nxh = bd.pnx / 2, nyh = bd.pny / 2;
-    const int U = 3 * bd.nsub;
-    const int* pidx = bd.pidx + item * bd.s_pix;
-    const int* ucnt = bd.ucnt + item * bd.s_pix;
-    const int* ucol = bd.ucol + (size_t)item * bd.s_pix * U;
-    const int* perm = bd.perm + item * bd.s_pix;
-    const int* act = bd.act_pix + item * bd.s_pix;
     const int rows = min(TS, M.n_img - kt * TS);
-    // the chunk's row pixels, then each thread keeps its taps fixed and walks the rows (no index
-    // divisions in the loop)
-    __shared__ int rk[TS], rl[TS];
     if (tid < rows) {
-        const int p = act[perm[kt * TS + tid]];
+        const int p = bd.act_pix[item * bd.s_pix + bd.perm[item * bd.s_pix + kt * TS + tid]];
         rk[tid] = p / bd.ny;
         rl[tid] = p - (p / bd.ny) * bd.ny;
     }
     __syncthreads();
-    for (int t = tid; t < ntap; t += 256) {
-        const int pk = t / bd.pny, pl = t - pk * bd.pny;
-        const int dk = nxh - pk, dl = nyh - pl;
-        for (int rr = 0; rr < rows; rr++) {
-            const int ii = rk[rr] + dk, jj = rl[rr] + dl;
-            if (ii < 0 || ii >= bd.nx || jj < 0 || jj >= bd.ny) continue;
-            const int r2 = pidx[ii * bd.ny + jj];
-            if (r2 < 0) continue;
-            if (use_seen) {  // each neighbouring pixel's columns are taken once per chunk
-                const unsigned bit = 1u << (r2 & 31);
-                if (atomicOr(&seen[r2 >> 5], bit) & bit) continue;
-            }
-            const int cnt = ucnt[r2];
-            for (int e = 0; e < cnt; e++) {
-                const int cb = ucol[(size_t)r2 * U + e] >> M.cbs;
-                const unsigned bit = 1u << (cb & 31);
+    unsigned m[QLX_MAX_CBW];
 #pragma unroll
-                for (int w = 0; w < QLX_MAX_CBW; w++) m[w] |= (w == (cb >> 5)) ? bit : 0u;
-            }
+    for (int w = 0; w < QLX_MAX_CBW; w++) m[w] = 0;
+    {
+        const int cbd = M.n_src >> M.cbs;  // the data column (thread 0)
+#pragma unroll
+        for (int w = 0; w < QLX_MAX_CBW; w++) m[w] |= (tid == 0 && w == (cbd >> 5)) ? (1u << (cbd & 31)) : 0u;
+    }
+    const int nxh = bd.pnx / 2, nyh = bd.pny / 2;
+    const int* apre = bd.apre + item * (bd.s_pix + 1);
+    const unsigned* pm = bd.pmask + (size_t)item * bd.s_pix * QLX_MAX_CBW;
+    // (row, window column) pairs: the window column's pixels x in [i0, i1] at y = l + nyh - pl are
+    // the active indices [apre[y*nx + i0], apre[y*nx + i1 + 1]); their block masks are ORed
+    for (int q = tid; q < rows * bd.pny; q += 256) {
+        const int rr = q / bd.pny, pl = q - rr * bd.pny;
+        const int k = rk[rr], jj = rl[rr] + nyh - pl;
+        if (jj < 0 || jj >= bd.ny) continue;
+        const int i0 = max(0, k + nxh - bd.pnx + 1), i1 = min(bd.nx - 1, k + nxh);
+        const long long t0 = (long long)jj * bd.nx;
+        const int a0 = apre[t0 + i0], a1 = apre[t0 + i1 + 1];
+        for (int a = a0; a < a1; a++) {
+#pragma unroll
+            for (int w = 0; w < QLX_MAX_CBW; w++)
+                if (w < M.ncbw) m[w] |= pm[(size_t)a * QLX_MAX_CBW + w];
         }
     }
 #pragma unroll
@@ -563,10 +560,7 @@ __global__ void __launch_bounds__(1024) k_block_scan_b(BatchDev bd)
 void launch_row_support_b(hipStream_t s, const BatchDev& bd, int max_nkt)
 {
     if (max_nkt <= 0) return;
-    const size_t seen_bytes = ((size_t)bd.s_pix + 31) / 32 * 4;
-    const int use_seen = seen_bytes <= 96 * 1024;
-    allow_big_lds((const void*)k_row_support_b);
-    hipLaunchKernelGGL(k_row_support_b, dim3(max_nkt, bd.B), dim3(256), use_seen ? seen_bytes : 0, s, bd, use_seen);
+    hipLaunchKernelGGL(k_row_support_b, dim3(max_nkt, bd.B), dim3(256), 0, s, bd);
     hipLaunchKernelGGL(k_block_scan_b, dim3(1, bd.B), dim3(1024), 0, s, bd);
 }
 

@@ -85,7 +85,7 @@ struct Ctx {
     DevBuf d_models, d_sgrid, d_srcn, d_meta, d_mats;
     // per-item work (stride s_pix / s_cell)
     DevBuf beta, map_cell, map_w, pmapped, cmapped, sidx, tix, pisrc, pidx, act_pix, counts, ucol, uval, ucnt, nevals;
-    DevBuf apre, lptr, eptr, lcj, lv, lvf, nfwtab;
+    DevBuf apre, lptr, eptr, lcj, lv, lvf, nfwtab, pmask;
     DevBuf key, bcnt, boff, bfill, perm;
     // pools
     DevBuf lb, F, R, cvec, vec, nz, bbase, winv, model, out, status;
@@ -264,7 +264,7 @@ void fill_source_grid(const qlx_settings& s, int snx, int sny, double* g)
 size_t stage1_bytes(size_t s_pix, size_t s_cell, size_t nsub)
 {
     const size_t U = 3 * nsub;
-    return s_pix * nsub * (16 + 12 + 24) + s_pix * U * 28 + s_pix * 4 * 7 + (s_pix + 1) * 12 + s_cell * 4 * 5 + 64 + s_pix * 16 + QLX_NFWTAB_DOUBLES * 8;
+    return s_pix * nsub * (16 + 12 + 24) + s_pix * U * 28 + s_pix * 4 * 7 + (s_pix + 1) * 12 + s_cell * 4 * 5 + 64 + s_pix * 16 + QLX_NFWTAB_DOUBLES * 8 + s_pix * QLX_MAX_CBW * 4;
 }
 
 // upper bound of the dense bytes of one item with at most ncell active source cells
@@ -317,6 +317,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
               c->apre.ensure(B * (s_pix + 1) * 4) && c->lptr.ensure(B * (s_pix + 1) * 4) && c->eptr.ensure(B * (s_pix + 1) * 4) && c->lcj.ensure(B * s_pix * U * 4) &&
               c->lv.ensure(B * s_pix * U * 8) && c->lvf.ensure(B * s_pix * U * 4) &&
               c->nfwtab.ensure((size_t)B * QLX_NFWTAB_DOUBLES * 8) &&
+              c->pmask.ensure((size_t)B * s_pix * QLX_MAX_CBW * 4) &&
               c->bcnt.ensure(B * (s_cell + 2) * 4) && c->boff.ensure(B * (s_cell + 2) * 4) && c->bfill.ensure(B * (s_cell + 2) * 4) &&
               c->perm.ensure(B * s_pix * 4) && c->model.ensure(B * s_pix * 16) && c->out.ensure((size_t)B * QLX_OUT_STRIDE * 8) &&
               c->status.ensure((size_t)B * 16);
@@ -367,6 +368,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     bd.lv = c->lv.as<double>();
     bd.lvf = c->lvf.as<float>();
     bd.nfwtab = c->nfwtab.as<double>();
+    bd.pmask = c->pmask.as<unsigned>();
     {
         // |blurred L entry| <= sum|psf| * max_r ||L_r||_1 <= 3 sum|psf| (per subpixel the 3-point
         // weights are 1/nsub times barycentric coordinates with |w1|, |w2| <= 1/2, 0 <= w0 <= 2),

@@ -108,6 +108,7 @@ struct BatchDev {
     unsigned* lcj;              // s_pix*3*nsub: CSR entry column (bits 0-13) | pixel x index i << 14
     double* lv;                 // s_pix*3*nsub: CSR entry value
     float* lvf;                 // s_pix*3*nsub: CSR entry value in FP32 (mixed path)
+    unsigned* pmask;            // s_pix*QLX_MAX_CBW: column blocks touched by each active pixel's L row
     double fixq;                // mixed path: blurred rows accumulate as int32 multiples of 1/fixq
     int* key;                   // s_pix
     int* bcnt;                  // s_cell+2
