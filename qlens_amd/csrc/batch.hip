@@ -1721,76 +1721,151 @@ void launch_logdets_b(hipStream_t s, const BatchDev& bd, int f32)
 // Triangular solves with the stored block inverses (one workgroup per item).
 // y = L^-1 D is row n of the augmented factor, so the first solve is backward only: x = L^-T y.
 // ------------------------------------------------------------------------------------------------
+// Each block step: a strip product over the solved part of the vector (16 B loads, eight in flight
+// per lane), then the 64x64 block inverse applied by all 1024 threads, 16 lanes per output.
 template <typename T>
-__device__ void tri_fwd(const T* L, int ld, int nb, const T* W, T* sx, T (*part)[TS + 1], T* rhs, T (*Wl)[TS + 1])
+__device__ __forceinline__ T sum16(T v)
 {
-    const int tid = threadIdx.x;
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 4, 64);
+    v += __shfl_xor(v, 8, 64);
+    return v;
+}
+
+// forward: x_kb = W_kb (b_kb - L[kb, 0:kb] x_{0:kb}); wave w owns rows w, w+16, w+32, w+48 of the block
+template <typename T>
+__device__ void tri_fwd(const T* L, int ld, int nb, const T* W, T* sx, T (*part)[TS + 1], T* rhs)
+{
+    constexpr int VW = 16 / sizeof(T);
+    typedef T P __attribute__((ext_vector_type(VW)));
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int orow = tid >> 4, oq = 4 * (tid & 15);
     for (int kb = 0; kb < nb; kb++) {
+        const int r0 = kb * TS;
+        T wv[4];
         {
-            const int r = tid >> 4, t16 = tid & 15;
-            const T* Lr = L + (long long)(kb * TS + r) * ld;
-            T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-            int c = t16;
-            for (; c + 48 < kb * TS; c += 64) {
-                a0 = fma(Lr[c], sx[c], a0);
-                a1 = fma(Lr[c + 16], sx[c + 16], a1);
-                a2 = fma(Lr[c + 32], sx[c + 32], a2);
-                a3 = fma(Lr[c + 48], sx[c + 48], a3);
+            const T* Wr = W + (size_t)kb * TS * TS + orow * TS + oq;
+            P a = *(const P*)Wr;
+            if constexpr (VW == 4) {
+                wv[0] = a.x; wv[1] = a.y; wv[2] = a.z; wv[3] = a.w;
+            } else {
+                const P b = *(const P*)(Wr + 2);
+                wv[0] = a.x; wv[1] = a.y; wv[2] = b.x; wv[3] = b.y;
             }
-            for (; c < kb * TS; c += 16) a0 = fma(Lr[c], sx[c], a0);
-            part[t16][r] = (a0 + a1) + (a2 + a3);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (oq + j > orow) wv[j] = (T)0;
         }
-        const T* Wk = W + (size_t)kb * TS * TS;
-        for (int e = tid; e < TS * TS; e += 1024) Wl[e >> 6][e & 63] = Wk[e];
+        const T* L0 = L + (size_t)(r0 + w) * ld;
+        const size_t s16 = (size_t)16 * ld;
+        P acc[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[q] = (P)0;
+        int c = VW * lane;
+        for (; c + 64 * VW < r0; c += 128 * VW) {
+            P v[8];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                v[q] = *(const P*)(L0 + q * s16 + c);
+                v[4 + q] = *(const P*)(L0 + q * s16 + c + 64 * VW);
+            }
+            const P sa = *(const P*)(sx + c), sb = *(const P*)(sx + c + 64 * VW);
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[q] = __builtin_elementwise_fma(v[4 + q], sb, __builtin_elementwise_fma(v[q], sa, acc[q]));
+        }
+        if (c < r0) {
+            const P sa = *(const P*)(sx + c);
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[q] = __builtin_elementwise_fma(*(const P*)(L0 + q * s16 + c), sa, acc[q]);
+        }
+        T a[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            a[q] = acc[q].x + acc[q].y;
+            if constexpr (VW == 4) a[q] += acc[q].z + acc[q].w;
+        }
+        // 4 row sums over 64 lanes: halve the set twice, then a 16-lane sum
+        const bool h5 = lane & 32, h4 = lane & 16;
+        const T k0 = (h5 ? a[2] : a[0]) + __shfl_xor(h5 ? a[0] : a[2], 32, 64);
+        const T k1 = (h5 ? a[3] : a[1]) + __shfl_xor(h5 ? a[1] : a[3], 32, 64);
+        const T k = sum16<T>((h4 ? k1 : k0) + __shfl_xor(h4 ? k0 : k1, 16, 64));
+        if ((lane & 15) == 0) {
+            const int r = w + 16 * (lane >> 4);
+            rhs[r] = sx[r0 + r] - k;
+        }
         __syncthreads();
-        if (tid < TS) {
-            T s = sx[kb * TS + tid];
-            for (int q = 0; q < 16; q++) s -= part[q][tid];
-            rhs[tid] = s;
-        }
-        __syncthreads();
-        if (tid < TS) {
-            T y = 0;
-            for (int q = 0; q <= tid; q++) y = fma(Wl[tid][q], rhs[q], y);
-            sx[kb * TS + tid] = y;
-        }
+        T y = wv[0] * rhs[oq];
+#pragma unroll
+        for (int j = 1; j < 4; j++) y = fma(wv[j], rhs[oq + j], y);
+        y = sum16<T>(y);
+        if ((tid & 15) == 0) sx[r0 + orow] = y;
         __syncthreads();
     }
 }
 
+// backward: x_kb = W_kb^T (b_kb - L[kb+1:, kb]^T x_{kb+1:}); lanes cover the block's 64 columns with
+// 16 B loads, the rest of the wave and the 16 waves split the rows
 template <typename T>
-__device__ void tri_bwd(const T* L, int ld, int nb, const T* W, T* sx, T (*part)[TS + 1], T* rhs, T (*Wl)[TS + 1])
+__device__ void tri_bwd(const T* L, int ld, int nb, const T* W, T* sx, T (*part)[TS + 1], T* rhs)
 {
-    const int tid = threadIdx.x;
+    constexpr int VW = 16 / sizeof(T);
+    constexpr int LPR = TS / VW;   // lanes per row
+    constexpr int RPW = 64 / LPR;  // rows per wave per pass
+    constexpr int RPP = 16 * RPW;  // rows per pass over the workgroup
+    typedef T P __attribute__((ext_vector_type(VW)));
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int ocol = tid >> 4, oq = 4 * (tid & 15);
+    const int cv = VW * (lane % LPR), rsub = lane / LPR;
+    const int rend = nb * TS;
     for (int kb = nb - 1; kb >= 0; kb--) {
+        const int r0 = kb * TS;
+        T wv[4];
         {
-            const int r = tid & 63, t16 = tid >> 6;
-            const int col = kb * TS + r;
-            T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-            int rr = (kb + 1) * TS + t16;
-            for (; rr + 48 < nb * TS; rr += 64) {
-                a0 = fma(L[(long long)rr * ld + col], sx[rr], a0);
-                a1 = fma(L[(long long)(rr + 16) * ld + col], sx[rr + 16], a1);
-                a2 = fma(L[(long long)(rr + 32) * ld + col], sx[rr + 32], a2);
-                a3 = fma(L[(long long)(rr + 48) * ld + col], sx[rr + 48], a3);
-            }
-            for (; rr < nb * TS; rr += 16) a0 = fma(L[(long long)rr * ld + col], sx[rr], a0);
-            part[t16][r] = (a0 + a1) + (a2 + a3);
+            const T* Wc = W + (size_t)kb * TS * TS + (size_t)oq * TS + ocol;
+#pragma unroll
+            for (int j = 0; j < 4; j++) wv[j] = (oq + j >= ocol) ? Wc[j * TS] : (T)0;
         }
-        const T* Wk = W + (size_t)kb * TS * TS;
-        for (int e = tid; e < TS * TS; e += 1024) Wl[e >> 6][e & 63] = Wk[e];
+        P acc = (P)0;
+        int rr = r0 + TS + RPW * w + rsub;
+        for (; rr + 7 * RPP < rend; rr += 8 * RPP) {
+            P v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = *(const P*)(L + (size_t)(rr + u * RPP) * ld + r0 + cv);
+#pragma unroll
+            for (int u = 0; u < 8; u++) acc = __builtin_elementwise_fma(v[u], (P)sx[rr + u * RPP], acc);
+        }
+        for (; rr < rend; rr += RPP) acc = __builtin_elementwise_fma(*(const P*)(L + (size_t)rr * ld + r0 + cv), (P)sx[rr], acc);
+#pragma unroll
+        for (int off = LPR; off < 64; off <<= 1) {
+            acc.x += __shfl_xor(acc.x, off, 64);
+            acc.y += __shfl_xor(acc.y, off, 64);
+            if constexpr (VW == 4) {
+                acc.z += __shfl_xor(acc.z, off, 64);
+                acc.w += __shfl_xor(acc.w, off, 64);
+            }
+        }
+        if (rsub == 0) {
+            part[w][cv] = acc.x;
+            part[w][cv + 1] = acc.y;
+            if constexpr (VW == 4) {
+                part[w][cv + 2] = acc.z;
+                part[w][cv + 3] = acc.w;
+            }
+        }
         __syncthreads();
         if (tid < TS) {
-            T s = sx[kb * TS + tid];
+            T s = sx[r0 + tid];
+#pragma unroll
             for (int q = 0; q < 16; q++) s -= part[q][tid];
             rhs[tid] = s;
         }
         __syncthreads();
-        if (tid < TS) {
-            T v = 0;
-            for (int q = tid; q < TS; q++) v = fma(Wl[q][tid], rhs[q], v);
-            sx[kb * TS + tid] = v;
-        }
+        T y = wv[0] * rhs[oq];
+#pragma unroll
+        for (int j = 1; j < 4; j++) y = fma(wv[j], rhs[oq + j], y);
+        y = sum16<T>(y);
+        if ((tid & 15) == 0) sx[r0 + ocol] = y;
         __syncthreads();
     }
 }
@@ -1801,7 +1876,6 @@ __global__ void __launch_bounds__(1024) k_trisolve_b(BatchDev bd)
     extern __shared__ double sx_raw[];
     __shared__ T part[16][TS + 1];
     __shared__ T rhs[TS];
-    __shared__ T Wl[TS][TS + 1];
     T* sx = (T*)sx_raw;
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
@@ -1811,7 +1885,7 @@ __global__ void __launch_bounds__(1024) k_trisolve_b(BatchDev bd)
     const T* W = (const T*)(bd.winv + M.w_off);
     for (int c = threadIdx.x; c < ld; c += 1024) sx[c] = (c < n) ? L[(size_t)n * ld + c] : (T)0;
     __syncthreads();
-    tri_bwd<T>(L, ld, nb, W, sx, part, rhs, Wl);
+    tri_bwd<T>(L, ld, nb, W, sx, part, rhs);
     double* s = bd.vec + M.vec_off;
     for (int c = threadIdx.x; c < ld; c += 1024) s[c] = (c < n) ? (double)sx[c] : 0.0;
 }
@@ -1834,10 +1908,35 @@ void launch_trisolve_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32)
 // sums of r^2 and D^2; the refine kernel stops an item once ||r|| <= tol ||D||.
 // status[4*item+2] counts steps, status[4*item+3] marks a converged item.
 // ------------------------------------------------------------------------------------------------
+// sum of a[k] over the 64 lanes for 16 values at once: four halvings, then a 4-lane sum; lane l ends
+// with the total of a[(l >> 2) & 15]
+__device__ __forceinline__ double lane_sum16(const double* a)
+{
+    const int lane = threadIdx.x & 63;
+    double b[8], c[4], d[2];
+    const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8, h2 = lane & 4;
+#pragma unroll
+    for (int j = 0; j < 8; j++) b[j] = a[(h5 ? 8 : 0) + j] + __shfl_xor(a[(h5 ? 0 : 8) + j], 32, 64);
+#pragma unroll
+    for (int j = 0; j < 4; j++) c[j] = b[(h4 ? 4 : 0) + j] + __shfl_xor(b[(h4 ? 0 : 4) + j], 16, 64);
+#pragma unroll
+    for (int j = 0; j < 2; j++) d[j] = c[(h3 ? 2 : 0) + j] + __shfl_xor(c[(h3 ? 0 : 2) + j], 8, 64);
+    double e = d[h2 ? 1 : 0] + __shfl_xor(d[h2 ? 0 : 1], 4, 64);
+    e += __shfl_xor(e, 2, 64);
+    e += __shfl_xor(e, 1, 64);
+    return e;
+}
+
+// r = D - F s for the 64 rows of block I.  F(i,c) is read from the upper triangle only: the column
+// strip (rows c < 64 I of columns i), the diagonal tile (through LDS) and the row strip (columns
+// c >= 64 (I+1) of rows i); products are formed and summed in FP64.
 __global__ void __launch_bounds__(256) k_resid_b(BatchDev bd)
 {
-    __shared__ double colacc[4][TS];
-    __shared__ double rowacc[TS];
+    typedef double dvec4 __attribute__((ext_vector_type(4)));
+    __shared__ float tile[TS][TS + 1];
+    __shared__ double cacc[16][TS];
+    __shared__ double dacc[4][TS];
+    __shared__ double racc[TS];
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     if (!M.valid || bd.status[4 * item + 3]) return;
@@ -1851,28 +1950,92 @@ __global__ void __launch_bounds__(256) k_resid_b(BatchDev bd)
     const double* fdiag = v + 3 * (size_t)ld;
     double* r = v + 4 * (size_t)ld;
     double* part = v + 5 * (size_t)ld;
-    const int i = I * TS + lane;
-    // column strip: sum_{c < i} F(i,c) s_c = sum_c U[c][i] s_c, c < min(i, n)
-    double acc = 0.0;
-    const int cend = min(I * TS + TS, n);
-    for (int c = w; c < cend; c += 4)
-        if (c < i && i < n) acc += (double)U[(size_t)c * ld + i] * s[c];
-    colacc[w][lane] = acc;
-    // row strip: sum_{c > i} F(i,c) s_c = sum_c U[i][c] s_c, one wave per row
-    for (int q = w; q < TS; q += 4) {
-        const int ir = I * TS + q;
+    const int c0 = I * TS;
+    // diagonal tile
+#pragma unroll
+    for (int e = tid; e < TS * TS / 4; e += 256) {
+        const int row = e >> 4, cc = 4 * (e & 15);
+        const float4_t t = *(const float4_t*)(U + (size_t)(c0 + row) * ld + c0 + cc);
+        tile[row][cc] = t.x;
+        tile[row][cc + 1] = t.y;
+        tile[row][cc + 2] = t.z;
+        tile[row][cc + 3] = t.w;
+    }
+    // column strip: F(i,c) = U[c][i] for c < c0; 16 lanes per row, 16 rows per pass
+    {
+        const int cv = 4 * (lane & 15);
+        const int rs = 4 * w + (lane >> 4);
+        dvec4 acc = (dvec4)0.0;
+        const float* Uc = U + c0 + cv;
+        int c = rs;
+        for (; c + 7 * 16 < c0; c += 8 * 16) {
+            float4_t t[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) t[u] = *(const float4_t*)(Uc + (size_t)(c + 16 * u) * ld);
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const double sc = s[c + 16 * u];
+                acc.x = fma((double)t[u].x, sc, acc.x);
+                acc.y = fma((double)t[u].y, sc, acc.y);
+                acc.z = fma((double)t[u].z, sc, acc.z);
+                acc.w = fma((double)t[u].w, sc, acc.w);
+            }
+        }
+        for (; c < c0; c += 16) {
+            const float4_t t = *(const float4_t*)(Uc + (size_t)c * ld);
+            const double sc = s[c];
+            acc.x = fma((double)t.x, sc, acc.x);
+            acc.y = fma((double)t.y, sc, acc.y);
+            acc.z = fma((double)t.z, sc, acc.z);
+            acc.w = fma((double)t.w, sc, acc.w);
+        }
+        cacc[rs][cv] = acc.x;
+        cacc[rs][cv + 1] = acc.y;
+        cacc[rs][cv + 2] = acc.z;
+        cacc[rs][cv + 3] = acc.w;
+    }
+    // row strip: F(i,c) = U[i][c] for c0 + 64 <= c < n; wave w owns rows c0 + w + 4k, k = 0..15
+    {
+        double ra[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) ra[k] = 0.0;
+        const float* Ur = U + (size_t)(c0 + w) * ld;
+        for (int c = c0 + TS + 4 * lane; c < n; c += 256) {
+            const double s0 = s[c], s1 = (c + 1 < n) ? s[c + 1] : 0.0;
+            const double s2 = (c + 2 < n) ? s[c + 2] : 0.0, s3 = (c + 3 < n) ? s[c + 3] : 0.0;
+            float4_t t[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) t[k] = *(const float4_t*)(Ur + (size_t)(4 * k) * ld + c);
+#pragma unroll
+            for (int k = 0; k < 16; k++)
+                ra[k] = fma((double)t[k].x, s0, fma((double)t[k].y, s1, fma((double)t[k].z, s2, fma((double)t[k].w, s3, ra[k]))));
+        }
+        const double tot = lane_sum16(ra);
+        if ((lane & 3) == 0) racc[w + 4 * (lane >> 2)] = tot;
+    }
+    __syncthreads();
+    // diagonal tile: thread (i, g) sums c in [16 g, 16 g + 16), c != i, c0 + c < n
+    {
+        const int i = tid & 63, g = tid >> 6;
         double a = 0.0;
-        if (ir < n)
-            for (int c = ir + 1 + lane; c < n; c += 64) a += (double)U[(size_t)ir * ld + c] * s[c];
-        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-        if (lane == 0) rowacc[q] = a;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int c = 16 * g + q;
+            if (c != i && c0 + c < n) a = fma((double)(c > i ? tile[i][c] : tile[c][i]), s[c0 + c], a);
+        }
+        dacc[g][i] = a;
     }
     __syncthreads();
     if (tid < TS) {
+        const int i = c0 + tid;
         double rr = 0.0, dd = 0.0;
         if (i < n) {
+            double cs = 0.0;
+#pragma unroll
+            for (int q = 0; q < 16; q++) cs += cacc[q][tid];
+            const double ds = (dacc[0][tid] + dacc[1][tid]) + (dacc[2][tid] + dacc[3][tid]);
             const double D = (double)U[(size_t)i * ld + n];
-            const double Fs = fdiag[i] * s[i] + ((colacc[0][tid] + colacc[1][tid]) + (colacc[2][tid] + colacc[3][tid])) + rowacc[tid];
+            const double Fs = fdiag[i] * s[i] + cs + ds + racc[tid];
             rr = D - Fs;
             r[i] = rr;
             rr = rr * rr;
@@ -1895,7 +2058,6 @@ __global__ void __launch_bounds__(1024) k_refine_b(BatchDev bd, int final_check,
     extern __shared__ double sx_raw[];
     __shared__ float part[16][TS + 1];
     __shared__ float rhs[TS];
-    __shared__ float Wl[TS][TS + 1];
     __shared__ double red[2][64];
     __shared__ int done;
     float* sx = (float*)sx_raw;
@@ -1937,8 +2099,8 @@ __global__ void __launch_bounds__(1024) k_refine_b(BatchDev bd, int final_check,
     const float* W = (const float*)(bd.winv + M.w_off);
     for (int c = tid; c < ld; c += 1024) sx[c] = (c < n) ? (float)r[c] : 0.0f;
     __syncthreads();
-    tri_fwd<float>(L, ld, nb, W, sx, part, rhs, Wl);
-    tri_bwd<float>(L, ld, nb, W, sx, part, rhs, Wl);
+    tri_fwd<float>(L, ld, nb, W, sx, part, rhs);
+    tri_bwd<float>(L, ld, nb, W, sx, part, rhs);
     for (int c = tid; c < n; c += 1024) s[c] += (double)sx[c];
 }
 
