@@ -209,8 +209,9 @@ int qlx_get_stats(void *ctx, qlx_stats *out);
  * B[k][j] = 1 + k + 0.1j (register r of lane l = D[l/16 + 4r][l%16] on gfx950); kind 1: the same
  * operands on v_mfma_f32_16x16x4f32 (register r of lane l = D[4(l/16) + r][l%16]); kind 2:
  * v_mfma_f32_16x16x32_bf16 with A[i][k] = (i+1)(k%4+1)/4, B[k][j] = (j+1)((k/4)%8+1)/8 where lane
- * l holds row/column l%16 and k = 8(l/16) .. 8(l/16)+7 (output layout as kind 1).  Lets a test
- * confirm the operand and accumulator layouts the tile kernels assume. */
+ * l holds row/column l%16 and k = 8(l/16) .. 8(l/16)+7 (output layout as kind 1); kind 3: the
+ * kind 2 operands on v_mfma_f32_16x16x32_f16.  Lets a test confirm the operand and accumulator
+ * layouts the tile kernels assume. */
 int qlx_selftest_mfma(void *ctx, int kind, double *out256);
 
 #ifdef __cplusplus

@@ -1,5 +1,5 @@
 // Batched inversion kernels for gfx950: active sets, merged lensing rows, row ordering, block-sparse
-// blurred L, F = Lb^T C Lb (FP64 MFMA, or split-BF16 MFMA with FP32 accumulation), curvature R
+// blurred L, F = Lb^T C Lb (FP64 MFMA, or FP16 MFMA with FP32 accumulation), curvature R
 // (banded FP64 factorization for ln det R), blocked dense Cholesky of F (FP64 or FP32), solves,
 // FP64 iterative refinement and the evidence terms.
 //
@@ -586,16 +586,10 @@ __device__ __forceinline__ bool mask_has(const unsigned* m, int cb) { return (m[
 //   FP64 path: FP64 weights, ds_add_f64 into FP64 rows, a pool of doubles.
 //   Mixed path: products rounded in FP32 to int32 multiples of 1/fixq and summed with ds_add_u32
 //   (exact and order-independent; ds_add_f32 runs at about a seventh of the rate of the integer
-//   and FP64 LDS atomics on gfx950), then scaled by sqrt(c_r) and split x = hi + lo into two BF16
-//   pools (hi = bf16(x), lo = bf16(x - hi)); the lo pool follows the hi pool.
+//   and FP64 LDS atomics on gfx950), then scaled by sqrt(c_r) s16 and rounded to one FP16 pool
+//   (s16 a power of two with |x s16| <= 2^14; 11 significant bits per entry).
 // ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)
-{
-    unsigned u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(((unsigned)h) << 16); }
+__device__ __forceinline__ uint16_t f32_to_f16_rne(float f) { return __builtin_bit_cast(uint16_t, (_Float16)f); }
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -699,7 +693,7 @@ __device__ __forceinline__ void accumulate_columns(const BatchDev& bd, const uns
     }
 }
 
-// FIX: mixed path (int32 fixed-point rows, BF16 hi/lo pools); else FP64.  The transposed PSF (times
+// FIX: mixed path (int32 fixed-point rows, an FP16 pool); else FP64.  The transposed PSF (times
 // fixq on the mixed path) is staged in LDS.  Dynamic LDS: [PSF] [nw x BUILD_TQ quarter table]
 // [nw x nsup_cap x CB rows].
 template <bool FIX>
@@ -767,14 +761,13 @@ k_build_lb_b(BatchDev bd, int nsup_cap, int psf_elems)
                 for (int cc = lane; cc < CB; cc += 64) pool[(size_t)(sbase + q) * bsz + (size_t)rr * CB + cc] = row[q * CB + cc];
             wave_sync();
         } else {
-            // int32 multiples of 1/fixq -> x = value * sqrt(c_r) (the data column: data * sqrt(c_r))
-            const double sc = p >= 0 ? sqrt(bd.covinv[p]) : 0.0;
+            // int32 multiples of 1/fixq -> x = value * sqrt(c_r) * s16 (the data column: data * sqrt(c_r) * s16)
+            const double sc = p >= 0 ? sqrt(bd.covinv[p]) * bd.s16 : 0.0;
             const double dv = p >= 0 ? bd.data[p] * sc : 0.0;
             const double sq = sc / bd.fixq;
-            uint16_t* hi = (uint16_t*)bd.lb + M.lb_off;
-            uint16_t* lo = hi + (size_t)M.nblk * bsz;
+            uint16_t* pool = (uint16_t*)bd.lb + M.lb_off;
             if (nw == 4) {
-                // k4-interleaved BF16 blocks: element (k, i) of a block at ((k >> 2) * CB + i) * 4 + (k & 3),
+                // k4-interleaved FP16 blocks: element (k, i) of a block at ((k >> 2) * CB + i) * 4 + (k & 3),
                 // so the SYRK stages blocks with plain 16-byte copies and reads 8 consecutive k per lane;
                 // the four waves' rows j*4 .. j*4+3 are one k-group
                 if (lane == 0) {
@@ -784,17 +777,14 @@ k_build_lb_b(BatchDev bd, int nsup_cap, int psf_elems)
                 __syncthreads();
                 for (int idx = tid; idx < nsup * CB; idx += 256) {
                     const int q = idx >> cbs, i = idx & (CB - 1);
-                    uint16_t h[4], g[4];
+                    uint16_t h[4];
 #pragma unroll
                     for (int w2 = 0; w2 < 4; w2++) {
                         const double xv = idx == dcol ? rdat[w2] : (double)(int)lds_rows[(size_t)w2 * nsup_cap * CB + idx] * rscl[w2];
-                        const float x = (float)xv;
-                        h[w2] = f32_to_bf16_rne(x);
-                        g[w2] = f32_to_bf16_rne(x - bf16_to_f32(h[w2]));
+                        h[w2] = f32_to_f16_rne((float)xv);
                     }
                     const size_t o = (size_t)(sbase + q) * bsz + ((size_t)j * CB + i) * 4;
-                    *(uint2*)&hi[o] = make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
-                    *(uint2*)&lo[o] = make_uint2(g[0] | ((unsigned)g[1] << 16), g[2] | ((unsigned)g[3] << 16));
+                    *(uint2*)&pool[o] = make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
                 }
                 __syncthreads();
             } else {
@@ -803,11 +793,7 @@ k_build_lb_b(BatchDev bd, int nsup_cap, int psf_elems)
                     for (int i = lane; i < CB; i += 64) {
                         const int idx = q * CB + i;
                         const float x = (float)(idx == dcol ? dv : (double)(int)row[idx] * sq);
-                        const uint16_t h = f32_to_bf16_rne(x);
-                        const uint16_t g = f32_to_bf16_rne(x - bf16_to_f32(h));
-                        const size_t o = (size_t)(sbase + q) * bsz + ((size_t)(rr >> 2) * CB + i) * 4 + (rr & 3);
-                        hi[o] = h;
-                        lo[o] = g;
+                        pool[(size_t)(sbase + q) * bsz + ((size_t)(rr >> 2) * CB + i) * 4 + (rr & 3)] = f32_to_f16_rne(x);
                     }
                 wave_sync();
             }
@@ -833,11 +819,11 @@ static int launch_build_typed(hipStream_t s, const BatchDev& bd, int max_nkt, in
     return 0;
 }
 
-int launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_nsup, int bf16, int cbw)
+int launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_nsup, int mixed, int cbw)
 {
     if (max_rows <= 0 || max_nsup <= 0) return 0;
     const int max_nkt = (max_rows + TS - 1) / TS;
-    return bf16 ? launch_build_typed<true>(s, bd, max_nkt, max_nsup, cbw) : launch_build_typed<false>(s, bd, max_nkt, max_nsup, cbw);
+    return mixed ? launch_build_typed<true>(s, bd, max_nkt, max_nsup, cbw) : launch_build_typed<false>(s, bd, max_nkt, max_nsup, cbw);
 }
 
 __global__ void k_gather_cvec_b(BatchDev bd)
@@ -971,26 +957,27 @@ void launch_syrk_f64_b(hipStream_t s, const BatchDev& bd, int max_ncb)
 }
 
 // ------------------------------------------------------------------------------------------------
-// F32 = Lb^T Lb with Lb = hi + lo (BF16 pools of sqrt(c)-scaled rows in k4-interleaved 64 x 128
-// blocks), on v_mfma_f32_16x16x32_bf16: hi*hi + hi*lo + lo*hi accumulated in FP32.  One
-// workgroup per 128x128 lower tile of F; wave (wr, wc) computes the 64x64 quadrant (wr, wc).  A
-// operand lane l holds A[i = l%16][8 consecutive k], B the same with j; accumulator register v of
-// lane l holds D[4*(l/16) + v][l%16].  The next chunk's four blocks are loaded into registers
-// while the current chunk is multiplied from LDS.
+// F32 = Lb^T Lb on the mixed path: one FP16 pool of sqrt(c) s16-scaled rows in k4-interleaved
+// 64 x 128 blocks, v_mfma_f32_16x16x32_f16 accumulated in FP32, scaled back by s16^-2 (exact) at
+// the store.  One workgroup per 128x128 lower tile of F; wave (wr, wc) computes the 64x64 quadrant
+// (wr, wc).  A operand lane l holds A[i = l%16][8 consecutive k], B the same with j; accumulator
+// register v of lane l holds D[4*(l/16) + v][l%16].  The next chunk's two blocks are loaded into
+// registers while the current chunk is multiplied from LDS.
 // ------------------------------------------------------------------------------------------------
 #define SCB 128  // column-block width of the mixed path
 #define BP2 544  // LDS elements per k-group: 128 columns x 4 k + 32, so the four lane groups of a
                  // 64-bit operand read fall on alternating bank halves
-__device__ __forceinline__ bf16x8_t lds_k8(const uint16_t* arr, int g, int i)
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ f16x8_t lds_k8(const uint16_t* arr, int g, int i)
 {
     const uint2 a = *(const uint2*)&arr[g * BP2 + i * 4];
     const uint2 b = *(const uint2*)&arr[(g + 1) * BP2 + i * 4];
-    return __builtin_bit_cast(bf16x8_t, make_uint4(a.x, a.y, b.x, b.y));
+    return __builtin_bit_cast(f16x8_t, make_uint4(a.x, a.y, b.x, b.y));
 }
 
-__global__ void __launch_bounds__(256) k_syrk_bf16_b(BatchDev bd)
+__global__ void __launch_bounds__(256) k_syrk_f16_b(BatchDev bd)
 {
-    extern __shared__ __attribute__((aligned(16))) uint16_t lds2[];  // A hi, A lo, B hi, B lo: 16 x BP2 each
+    extern __shared__ __attribute__((aligned(16))) uint16_t lds2[];  // A, B: 16 x BP2 each
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     if (!M.valid) return;
@@ -999,33 +986,26 @@ __global__ void __launch_bounds__(256) k_syrk_bf16_b(BatchDev bd)
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wr = w >> 1, wc = w & 1;
     const size_t bsz = (size_t)TS * SCB;
-    const uint16_t* hi = (const uint16_t*)bd.lb + M.lb_off;
-    const uint16_t* lo = hi + (size_t)M.nblk * bsz;
+    const uint16_t* pool = (const uint16_t*)bd.lb + M.lb_off;
     const unsigned* nz = bd.nz + M.nz_off;
     const int* bb = bd.bbase + M.bb_off;
     const int nkt = M.nkt, ncbw = M.ncbw;
-    uint16_t* const a_hi = lds2;
-    uint16_t* const a_lo = lds2 + 16 * BP2;
-    uint16_t* const b_hi = lds2 + 32 * BP2;
-    uint16_t* const b_lo = lds2 + 48 * BP2;
+    uint16_t* const as = lds2;
+    uint16_t* const bs = lds2 + 16 * BP2;
     // thread piece h (of 4) of every block: 16 bytes at element (tid + 256 h) * 8
     const int e0 = tid * 8;
     const int o0 = (e0 >> 9) * BP2 + (e0 & 511);                    // pieces h = 0..3 are 2048
     const int o1 = ((e0 + 2048) >> 9) * BP2 + ((e0 + 2048) & 511);  // elements apart: 4 k-groups
     const int o2 = ((e0 + 4096) >> 9) * BP2 + ((e0 + 4096) & 511);
     const int o3 = ((e0 + 6144) >> 9) * BP2 + ((e0 + 6144) & 511);
-    uint4 pa0, pa1, pa2, pa3, pb0, pb1, pb2, pb3, pc0, pc1, pc2, pc3, pd0, pd1, pd2, pd3;
+    uint4 pa0, pa1, pa2, pa3, pb0, pb1, pb2, pb3;
 #define SYRK_LOAD(kk_)                                                                                 \
     {                                                                                                  \
         const unsigned* m_ = nz + (size_t)(kk_) * ncbw;                                                \
-        const uint4* ah_ = (const uint4*)(hi + (size_t)(bb[kk_] + mask_rank(m_, I)) * bsz) + tid;      \
-        const uint4* al_ = (const uint4*)(lo + (size_t)(bb[kk_] + mask_rank(m_, I)) * bsz) + tid;      \
-        const uint4* bh_ = (const uint4*)(hi + (size_t)(bb[kk_] + mask_rank(m_, J)) * bsz) + tid;      \
-        const uint4* bl_ = (const uint4*)(lo + (size_t)(bb[kk_] + mask_rank(m_, J)) * bsz) + tid;      \
-        pa0 = ah_[0]; pa1 = ah_[256]; pa2 = ah_[512]; pa3 = ah_[768];                                  \
-        pb0 = al_[0]; pb1 = al_[256]; pb2 = al_[512]; pb3 = al_[768];                                  \
-        pc0 = bh_[0]; pc1 = bh_[256]; pc2 = bh_[512]; pc3 = bh_[768];                                  \
-        pd0 = bl_[0]; pd1 = bl_[256]; pd2 = bl_[512]; pd3 = bl_[768];                                  \
+        const uint4* a_ = (const uint4*)(pool + (size_t)(bb[kk_] + mask_rank(m_, I)) * bsz) + tid;     \
+        const uint4* b_ = (const uint4*)(pool + (size_t)(bb[kk_] + mask_rank(m_, J)) * bsz) + tid;     \
+        pa0 = a_[0]; pa1 = a_[256]; pa2 = a_[512]; pa3 = a_[768];                                      \
+        pb0 = b_[0]; pb1 = b_[256]; pb2 = b_[512]; pb3 = b_[768];                                      \
     }
     float4_t acc[4][4];
 #pragma unroll
@@ -1040,10 +1020,8 @@ __global__ void __launch_bounds__(256) k_syrk_bf16_b(BatchDev bd)
     if (kt < nkt) SYRK_LOAD(kt);
     while (kt < nkt) {
         __syncthreads();
-        *(uint4*)&a_hi[o0] = pa0; *(uint4*)&a_hi[o1] = pa1; *(uint4*)&a_hi[o2] = pa2; *(uint4*)&a_hi[o3] = pa3;
-        *(uint4*)&a_lo[o0] = pb0; *(uint4*)&a_lo[o1] = pb1; *(uint4*)&a_lo[o2] = pb2; *(uint4*)&a_lo[o3] = pb3;
-        *(uint4*)&b_hi[o0] = pc0; *(uint4*)&b_hi[o1] = pc1; *(uint4*)&b_hi[o2] = pc2; *(uint4*)&b_hi[o3] = pc3;
-        *(uint4*)&b_lo[o0] = pd0; *(uint4*)&b_lo[o1] = pd1; *(uint4*)&b_lo[o2] = pd2; *(uint4*)&b_lo[o3] = pd3;
+        *(uint4*)&as[o0] = pa0; *(uint4*)&as[o1] = pa1; *(uint4*)&as[o2] = pa2; *(uint4*)&as[o3] = pa3;
+        *(uint4*)&bs[o0] = pb0; *(uint4*)&bs[o1] = pb1; *(uint4*)&bs[o2] = pb2; *(uint4*)&bs[o3] = pb3;
         __syncthreads();
         int nk = kt + 1;
         for (; nk < nkt; nk++) {
@@ -1054,28 +1032,21 @@ __global__ void __launch_bounds__(256) k_syrk_bf16_b(BatchDev bd)
 #pragma unroll
         for (int k0 = 0; k0 < TS; k0 += 32) {
             const int g = (k0 >> 2) + 2 * (lane >> 4);
-            bf16x8_t ah[4], al[4], bh[4], bl[4];
+            f16x8_t av[4], bv[4];
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                const int ia = wr * 64 + t * 16 + (lane & 15);
-                const int jb = wc * 64 + t * 16 + (lane & 15);
-                ah[t] = lds_k8(a_hi, g, ia);
-                al[t] = lds_k8(a_lo, g, ia);
-                bh[t] = lds_k8(b_hi, g, jb);
-                bl[t] = lds_k8(b_lo, g, jb);
+                av[t] = lds_k8(as, g, wr * 64 + t * 16 + (lane & 15));
+                bv[t] = lds_k8(bs, g, wc * 64 + t * 16 + (lane & 15));
             }
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[c], acc[a][c], 0, 0, 0);
-                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[c], acc[a][c], 0, 0, 0);
-                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[c], acc[a][c], 0, 0, 0);
-                }
+                for (int c = 0; c < 4; c++) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av[a], bv[c], acc[a][c], 0, 0, 0);
         }
         kt = nk;
     }
 #undef SYRK_LOAD
+    const float unscale = (float)(1.0 / (bd.s16 * bd.s16));
     float* C = bd.F32 + M.f_off + (size_t)I * SCB * M.ldN + (size_t)J * SCB;
 #pragma unroll
     for (int a = 0; a < 4; a++)
@@ -1085,15 +1056,15 @@ __global__ void __launch_bounds__(256) k_syrk_bf16_b(BatchDev bd)
             for (int v = 0; v < 4; v++) {
                 const int rr = wr * 64 + a * 16 + 4 * (lane >> 4) + v;
                 const int cc = wc * 64 + c * 16 + (lane & 15);
-                C[(size_t)rr * M.ldN + cc] = acc[a][c][v];
+                C[(size_t)rr * M.ldN + cc] = acc[a][c][v] * unscale;
             }
 }
 
-void launch_syrk_bf16_b(hipStream_t s, const BatchDev& bd, int max_ncb)
+void launch_syrk_f16_b(hipStream_t s, const BatchDev& bd, int max_ncb)
 {
-    const size_t lds = (size_t)4 * 16 * BP2 * sizeof(uint16_t);
-    allow_big_lds((const void*)k_syrk_bf16_b);
-    hipLaunchKernelGGL(k_syrk_bf16_b, dim3(syrk_grid(max_ncb), bd.B), dim3(256), lds, s, bd);
+    const size_t lds = (size_t)2 * 16 * BP2 * sizeof(uint16_t);
+    allow_big_lds((const void*)k_syrk_f16_b);
+    hipLaunchKernelGGL(k_syrk_f16_b, dim3(syrk_grid(max_ncb), bd.B), dim3(256), lds, s, bd);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -2289,6 +2260,18 @@ __global__ void k_selftest_mfma(double* out)
         float4_t acc = {0, 0, 0, 0};
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
         for (int v = 0; v < 4; v++) out[512 + lane * 4 + v] = acc[v];
+    }
+    // f16 16x16x32: the same operand values and lane layout in FP16
+    {
+        f16x8_t a, b;
+        for (int t = 0; t < 8; t++) {
+            const int k = 8 * (lane >> 4) + t;
+            a[t] = (_Float16)((float)((lane & 15) + 1) * (float)(k % 4 + 1) * 0.25f);
+            b[t] = (_Float16)((float)((lane & 15) + 1) * (float)((k / 4) % 8 + 1) * 0.125f);
+        }
+        float4_t acc = {0, 0, 0, 0};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc, 0, 0, 0);
+        for (int v = 0; v < 4; v++) out[768 + lane * 4 + v] = acc[v];
     }
 }
 

@@ -70,6 +70,7 @@ struct Ctx {
     int nx = 0, ny = 0;
     bool image_set = false;
     std::vector<double> h_xvals, h_yvals, h_data, h_covinv;
+    double max_sqrtc = 0.0, max_dsc = 0.0;  // max sqrt(covinv) and max |data| sqrt(covinv) over the image
     std::vector<unsigned char> h_mask;
     bool has_mask = false;
     DevBuf xvals, yvals, data, covinv, mask;
@@ -376,6 +377,12 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         double apsf = 0.0;
         for (double v : c->h_psf) apsf += fabs(v);
         bd.fixq = std::ldexp(1.0, 29) / std::max(apsf, 1e-300);
+        // mixed-path pool entries are x = (blurred L entry) sqrt(c) or data sqrt(c); the power of two
+        // s16 puts the bound on |x| at or below 2^14, inside FP16's range with a factor 4 to spare
+        const double xmax = std::max(3.0 * apsf * c->max_sqrtc, c->max_dsc);
+        int e = 0;
+        if (xmax > 0 && std::isfinite(xmax)) e = (int)std::floor(std::log2(16384.0 / xmax));
+        bd.s16 = std::ldexp(1.0, std::min(60, std::max(-60, e)));
     }
     bd.key = c->key.as<int>();
     bd.bcnt = c->bcnt.as<int>();
@@ -507,7 +514,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         M.nblk = counts[4 * b + 2];
         M.nsup_max = counts[4 * b + 3];
         M.lb_off = (long long)lb_tot;
-        lb_tot += (size_t)M.nblk * ((size_t)64 << M.cbs) * (mixed ? 2 : 1);
+        lb_tot += (size_t)M.nblk * ((size_t)64 << M.cbs);
         max_nsup = std::max(max_nsup, M.nsup_max);
     }
     if (!c->lb.ensure(lb_tot * (mixed ? 2 : 8))) return fail(c, QLX_ERR_NOMEM, "Lb pool allocation failed");
@@ -518,7 +525,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     timer_end(c, "lbuild");
     if (!mixed) launch_gather_cvec_b(st, bd, max_rows);
     timer_begin(c, "syrk");
-    if (mixed) launch_syrk_bf16_b(st, bd, max_ncb);
+    if (mixed) launch_syrk_f16_b(st, bd, max_ncb);
     else launch_syrk_f64_b(st, bd, max_ncb);
     timer_end(c, "syrk");
     launch_curvature_b(st, bd, max_cells, max_rtiles);
@@ -621,10 +628,10 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
                 exec_pairs += bits * (bits + 1) / 2;
                 nblk += bits;
             }
-            T.syrk_exec_flops += exec_pairs * tile * (mixed ? 3 : 1);
+            T.syrk_exec_flops += exec_pairs * tile;
             T.syrk_dense_flops += (double)M.ncb * (M.ncb + 1) / 2 * M.nkt * tile;
             T.syrk_alg_flops += (double)M.n_img * M.n_src * (M.n_src + 1) + 2.0 * M.n_img * M.n_src;
-            T.lb_bytes += nblk * 64 * cbw * (mixed ? 4 : 8);
+            T.lb_bytes += nblk * 64 * cbw * (mixed ? 2 : 8);
             T.integrand_evals += (double)nev[b];
             T.chol_flops += (double)M.n_src * M.n_src * M.n_src / 3.0;
             T.n_items++;
@@ -695,7 +702,7 @@ int eval_items(Ctx* c, int nbatch, const qlx_lens_model* models, const int* src_
     }
     const size_t ncell_bound = std::min<size_t>(max_cell, QLX_MAX_LD - 1);
     const size_t ldN = round_up(ncell_bound + 1, 64);
-    const size_t lb_est = round_up(std::max<size_t>(s_pix, 64), 64) * std::min<size_t>(ldN, 32 * 64) * (mixed ? 4 : 8);
+    const size_t lb_est = round_up(std::max<size_t>(s_pix, 64), 64) * std::min<size_t>(ldN, 32 * 64) * (mixed ? 2 : 8);
     const size_t per_item = stage1_bytes(s_pix, max_cell, nsub) + dense_bytes_bound(ncell_bound, max_snx, mixed) + lb_est;
     size_t budget = c->hbm_budget;
     if (budget == 0) {
@@ -785,6 +792,13 @@ int qlx_set_image(void* ctx, int nx, int ny, const double* xvals, const double* 
     c->h_yvals.assign(yvals, yvals + ny + 1);
     c->h_data.assign(data, data + npix);
     c->h_covinv.assign(covinv, covinv + npix);
+    c->max_sqrtc = 0.0;
+    c->max_dsc = 0.0;
+    for (size_t p = 0; p < npix; p++) {
+        const double sc = covinv[p] > 0 ? sqrt(covinv[p]) : 0.0;
+        c->max_sqrtc = std::max(c->max_sqrtc, sc);
+        c->max_dsc = std::max(c->max_dsc, fabs(data[p]) * sc);
+    }
     c->has_mask = (mask != nullptr);
     if (mask) c->h_mask.assign(mask, mask + npix);
     if (!c->xvals.ensure((nx + 1) * 8) || !c->yvals.ensure((ny + 1) * 8) || !c->data.ensure(npix * 8) || !c->covinv.ensure(npix * 8) ||
@@ -1068,10 +1082,10 @@ int qlx_get_stats(void* ctx, qlx_stats* out)
 int qlx_selftest_mfma(void* ctx, int kind, double* out256)
 {
     Ctx* c = (Ctx*)ctx;
-    if (!c || !out256 || kind < 0 || kind > 2) return QLX_ERR_ARG;
+    if (!c || !out256 || kind < 0 || kind > 3) return QLX_ERR_ARG;
     hipSetDevice(c->device);
     DevBuf d;
-    if (!d.ensure(768 * 8)) return QLX_ERR_NOMEM;
+    if (!d.ensure(1024 * 8)) return QLX_ERR_NOMEM;
     launch_selftest_mfma(c->stream, d.as<double>());
     HIPCHK(c, hipMemcpyAsync(out256, d.as<double>() + 256 * kind, 256 * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -110,6 +110,7 @@ struct BatchDev {
     float* lvf;                 // s_pix*3*nsub: CSR entry value in FP32 (mixed path)
     unsigned* pmask;            // s_pix*QLX_MAX_CBW: column blocks touched by each active pixel's L row
     double fixq;                // mixed path: blurred rows accumulate as int32 multiples of 1/fixq
+    double s16;                 // mixed path: power of two that scales the FP16 pool entries (|x s16| <= 2^14)
     int* key;                   // s_pix
     int* bcnt;                  // s_cell+2
     int* boff;                  // s_cell+2
@@ -218,10 +219,10 @@ void launch_merge_lrows_b(hipStream_t s, const BatchDev& bd, int max_img);
 void launch_lrows_csr_b(hipStream_t s, const BatchDev& bd, int max_img);
 void launch_sort_rows_b(hipStream_t s, const BatchDev& bd, int max_img, int max_src);
 void launch_row_support_b(hipStream_t s, const BatchDev& bd, int max_nkt);
-int launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_nsup, int bf16, int cbw);
+int launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_nsup, int mixed, int cbw);
 void launch_gather_cvec_b(hipStream_t s, const BatchDev& bd, int max_rows);
 void launch_syrk_f64_b(hipStream_t s, const BatchDev& bd, int max_ncb);
-void launch_syrk_bf16_b(hipStream_t s, const BatchDev& bd, int max_ncb);
+void launch_syrk_f16_b(hipStream_t s, const BatchDev& bd, int max_ncb);
 void launch_curvature_b(hipStream_t s, const BatchDev& bd, int max_cells, int max_rtiles);
 void launch_prep_fr_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32);
 void cholesky_batched(hipStream_t s, const MatDesc* d_mats, int n32, int n64, int max_nb, int dense, int max_bwb);

@@ -273,7 +273,7 @@ class PixelInversion:
         return (ms.value, n.value) if rc == 0 else (None, 0)
 
     def selftest_mfma(self, kind=0):
-        """kind 0: f64 16x16x4, 1: f32 16x16x4, 2: bf16 16x16x32 (see include/qlx.h)."""
+        """kind 0: f64 16x16x4, 1: f32 16x16x4, 2: bf16 16x16x32, 3: f16 16x16x32 (see include/qlx.h)."""
         out = np.zeros(256)
         self._check(self.lib.qlx_selftest_mfma(self.ctx, int(kind), _dp(out)), "qlx_selftest_mfma")
         return out

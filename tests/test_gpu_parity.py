@@ -51,6 +51,15 @@ def test_mfma_bf16_operand_and_accumulator_layout(qlx_ctx):
     assert np.array_equal(got, A @ B), "bf16 16x16x32 operand / accumulator layout does not hold"
 
 
+def test_mfma_f16_operand_and_accumulator_layout(qlx_ctx):
+    # the mixed-path SYRK (FP16 pool) relies on this layout
+    out = qlx_ctx.selftest_mfma(3)
+    A = np.array([[(i + 1) * (k % 4 + 1) / 4 for k in range(32)] for i in range(16)])
+    B = np.array([[(j + 1) * ((k // 4) % 8 + 1) / 8 for j in range(16)] for k in range(32)])
+    got = _mfma_got(out, lambda l, v: 4 * (l // 16) + v)
+    assert np.array_equal(got, A @ B), "f16 16x16x32 operand / accumulator layout does not hold"
+
+
 def test_raytrace_matches_oracle(qlx_ctx, golden_dir):
     g = np.load(os.path.join(golden_dir, "deflection.npz"))
     sets = json.load(open(os.path.join(golden_dir, "deflection_lenses.json")))
