@@ -1595,6 +1595,132 @@ k_chol_trail_m(const MatDesc* mats, int k0, int nq)
 }
 #undef TRAIL_LOAD
 
+// FP32 matrices (mixed path): the same trailing update with the panel operands split x = hi + lo
+// into BF16 while they are staged (hi = bf16(x), lo = bf16(x - hi): 2^-17 of each entry) and all
+// four products hi*hi + hi*lo + lo*hi + lo*lo on v_mfma_f32_16x16x32_bf16, accumulated in FP32.
+// The accumulators start from the C tile and the A operand is negated while staging, so the MFMAs
+// form C - A B^T directly.  Loads go through a buffer resource (32-bit offsets: one VGPR per lane,
+// row offsets in SGPRs), which keeps the kernel inside the 256 registers of two waves per SIMD.
+// LDS: A hi, A lo, B hi, B lo as 128 rows x TBP BF16 (144-byte rows: the 16 rows of an operand read
+// fall on distinct bank groups).
+#define TBP 72
+static_assert((long long)QLX_MAX_LD * QLX_MAX_LD * 4 < (1LL << 31), "trailing-update byte offsets must fit in 31 bits");
+__device__ __forceinline__ uint16_t bf16_rne(float f)
+{
+    unsigned u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_val(uint16_t h) { return __uint_as_float(((unsigned)h) << 16); }
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_chol_trail_bf16(const MatDesc* mats, int k0, int nq)
+{
+    typedef float P __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(1))) float gfloat;
+    extern __shared__ double trail_raw[];
+    uint16_t* Ah = (uint16_t*)trail_raw;
+    uint16_t* Al = Ah + TT * TBP;
+    uint16_t* Bh = Al + TT * TBP;
+    uint16_t* Bl = Bh + TT * TBP;
+    const MatDesc D = mats[blockIdx.y];
+    const int r0b = k0 + nq;
+    if (r0b >= D.nb) return;
+    const int nt = (D.nb - r0b) / 2;
+    int I2, J2;
+    if (!syrk_tile(nt, I2, J2)) return;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    const int ld = D.ld;  // ld <= QLX_MAX_LD: byte offsets within the matrix fit in 31 bits
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(D.A, 0, ld * ld * 4, 0x00020000);
+    const int rI = (r0b + 2 * I2) * TS, rJ = (r0b + 2 * J2) * TS;
+    // byte offsets: the lane's part in a VGPR, the uniform part in an SGPR.  Panel piece h of a
+    // thread: row (tid >> 4) + 16 h, columns 4 (tid & 15) .. +3 of column block k0 + q
+    const int vp = ((tid >> 4) * ld + 4 * (tid & 15)) * 4;
+    const int sA = (rI * ld + k0 * TS) * 4, sB = (rJ * ld + k0 * TS) * 4;
+    const int vc = ((wr * 64 + 4 * (lane >> 4)) * ld + wc * 64 + (lane & 15)) * 4;
+    const int sC = (rI * ld + rJ) * 4;
+    float4_t acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                acc[a][c][v] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, vc + c * 64, sC + (a * 16 + v) * ld * 4, 0));
+    P pa[8], pb[8];
+#define TRAIL_LOAD_BF(q)                                                                                           \
+    {                                                                                                              \
+        _Pragma("unroll") for (int h = 0; h < 8; h++)                                                              \
+        {                                                                                                          \
+            pa[h] = __builtin_bit_cast(P, __builtin_amdgcn_raw_buffer_load_b128(rs, vp, sA + (16 * h * ld + (q) * TS) * 4, 0)); \
+            pb[h] = __builtin_bit_cast(P, __builtin_amdgcn_raw_buffer_load_b128(rs, vp, sB + (16 * h * ld + (q) * TS) * 4, 0)); \
+        }                                                                                                          \
+    }
+    TRAIL_LOAD_BF(0);
+    for (int q = 0; q < nq; q++) {
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 8; h++) {
+            const int i = (tid >> 4) + 16 * h, k = 4 * (tid & 15);
+            uint16_t ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const float xa = -pa[h][t], xb = pb[h][t];
+                ah[t] = bf16_rne(xa);
+                al[t] = bf16_rne(xa - bf16_val(ah[t]));
+                bh[t] = bf16_rne(xb);
+                bl[t] = bf16_rne(xb - bf16_val(bh[t]));
+            }
+            *(uint2*)&Ah[i * TBP + k] = make_uint2(ah[0] | ((unsigned)ah[1] << 16), ah[2] | ((unsigned)ah[3] << 16));
+            *(uint2*)&Al[i * TBP + k] = make_uint2(al[0] | ((unsigned)al[1] << 16), al[2] | ((unsigned)al[3] << 16));
+            *(uint2*)&Bh[i * TBP + k] = make_uint2(bh[0] | ((unsigned)bh[1] << 16), bh[2] | ((unsigned)bh[3] << 16));
+            *(uint2*)&Bl[i * TBP + k] = make_uint2(bl[0] | ((unsigned)bl[1] << 16), bl[2] | ((unsigned)bl[3] << 16));
+        }
+        __syncthreads();
+        if (q + 1 < nq) TRAIL_LOAD_BF(q + 1);
+#pragma unroll
+        for (int kk = 0; kk < TS; kk += 32) {
+            const int kq = kk + 8 * (lane >> 4);
+            bf16x8_t bhv[4], blv[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int jb = (wc * 64 + t * 16 + (lane & 15)) * TBP + kq;
+                bhv[t] = *(const bf16x8_t*)&Bh[jb];
+                blv[t] = *(const bf16x8_t*)&Bl[jb];
+            }
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const int ia = (wr * 64 + a * 16 + (lane & 15)) * TBP + kq;
+                const bf16x8_t ahv = *(const bf16x8_t*)&Ah[ia];
+                const bf16x8_t alv = *(const bf16x8_t*)&Al[ia];
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahv, bhv[c], acc[a][c], 0, 0, 0);
+                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahv, blv[c], acc[a][c], 0, 0, 0);
+                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alv, bhv[c], acc[a][c], 0, 0, 0);
+                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alv, blv[c], acc[a][c], 0, 0, 0);
+                }
+            }
+        }
+    }
+#undef TRAIL_LOAD_BF
+    // plain global stores (the buffer-store form stored one register for all four rows of a
+    // 16 x 16 block under the diagonal predicate with this compiler)
+    const bool diag = (I2 == J2);
+    gfloat* Cg = (gfloat*)D.A + (size_t)rI * ld + rJ;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int rr = wr * 64 + a * 16 + 4 * (lane >> 4) + v;
+                const int cc = wc * 64 + c * 16 + (lane & 15);
+                if (!diag || cc <= rr) Cg[(size_t)rr * ld + cc] = acc[a][c][v];
+            }
+}
+
 template <typename T>
 static void chol_step(hipStream_t s, const MatDesc* mats, int nmat, int kb, int rest)
 {
@@ -1611,7 +1737,9 @@ template <typename T>
 static void chol_two_level(hipStream_t s, const MatDesc* mats, int nmat, int max_nb)
 {
     const size_t lds = (size_t)2 * TT * LDT * sizeof(T);
+    const size_t lds_bf = (size_t)4 * TT * TBP * sizeof(uint16_t);
     allow_big_lds((const void*)k_chol_trail_m<T>);
+    allow_big_lds((const void*)k_chol_trail_bf16);
     for (int k0 = 0; k0 < max_nb; k0 += 4) {
         const int kend = k0 + 4;
         for (int kb = k0; kb < kend && kb < max_nb; kb++) {
@@ -1625,7 +1753,10 @@ static void chol_two_level(hipStream_t s, const MatDesc* mats, int nmat, int max
         }
         if (kend < max_nb) {
             const int nt = (max_nb - kend) / 2;
-            hipLaunchKernelGGL(k_chol_trail_m<T>, dim3(syrk_grid(nt), nmat), dim3(256), lds, s, mats, k0, 4);
+            if (sizeof(T) == 4)
+                hipLaunchKernelGGL(k_chol_trail_bf16, dim3(syrk_grid(nt), nmat), dim3(256), lds_bf, s, mats, k0, 4);
+            else
+                hipLaunchKernelGGL(k_chol_trail_m<T>, dim3(syrk_grid(nt), nmat), dim3(256), lds, s, mats, k0, 4);
         }
     }
 }
