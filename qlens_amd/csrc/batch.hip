@@ -1378,6 +1378,42 @@ __device__ __forceinline__ void store64(T* C, int ldc, const typename Mfma16<T>:
             }
 }
 
+// the 16 C values of a lane (store64's layout), loaded before the staging barrier so their latency
+// overlaps the operand loads; store64c then writes C - acc (mode 1) or its lower triangle (mode 2)
+template <typename T>
+__device__ __forceinline__ void load64c(const T* C, int ldc, T (&cv)[16])
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = w >> 1, wc = w & 1;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int rr = wr * 32 + a * 16 + Mfma16<T>::row(lane, v);
+                const int cc = wc * 32 + c * 16 + (lane & 15);
+                cv[(a * 2 + c) * 4 + v] = C[(size_t)rr * ldc + cc];
+            }
+}
+
+template <typename T>
+__device__ __forceinline__ void store64c(T* C, int ldc, const typename Mfma16<T>::acc_t acc[2][2], const T (&cv)[16], int mode)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = w >> 1, wc = w & 1;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int rr = wr * 32 + a * 16 + Mfma16<T>::row(lane, v);
+                const int cc = wc * 32 + c * 16 + (lane & 15);
+                if (mode == 1 || cc <= rr) C[(size_t)rr * ldc + cc] = cv[(a * 2 + c) * 4 + v] - acc[a][c][v];
+            }
+}
+
 #define MM_LDS (2 * TS * LDT)
 template <typename T>
 __device__ void chol_panel_body(const MatDesc& D, int kb, int ib, double* raw)
@@ -1423,12 +1459,14 @@ __global__ void __launch_bounds__(256) k_chol_update_m(const MatDesc* mats, int 
     const T* Pa = tile_ptr<T>(D, I, kb, lda);
     const T* Pb = tile_ptr<T>(D, J, kb, ldb);
     T* C = tile_ptr<T>(D, I, J, ldc);
+    T cv[16];
+    load64c<T>(C, ldc, cv);
     stage64<T>(Pa, lda, As);
     stage64<T>(Pb, ldb, Bs);
     __syncthreads();
     typename Mfma16<T>::acc_t acc[2][2];
     mma64<T>(As, Bs, acc);
-    store64<T>(C, ldc, acc, I == J ? 2 : 1);
+    store64c<T>(C, ldc, acc, cv, I == J ? 2 : 1);
 }
 
 // rank-64 update from column block kb of the tiles (I, J), kb < J < kend, J <= I < nb: the
@@ -1456,12 +1494,14 @@ __global__ void __launch_bounds__(256) k_chol_upanel_m(const MatDesc* mats, int 
     const T* Pa = tile_ptr<T>(D, I, kb, lda);
     const T* Pb = tile_ptr<T>(D, J, kb, ldb);
     T* C = tile_ptr<T>(D, I, J, ldc);
+    T cv[16];
+    load64c<T>(C, ldc, cv);
     stage64<T>(Pa, lda, As);
     stage64<T>(Pb, ldb, Bs);
     __syncthreads();
     typename Mfma16<T>::acc_t acc[2][2];
     mma64<T>(As, Bs, acc);
-    store64<T>(C, ldc, acc, I == J ? 2 : 1);
+    store64c<T>(C, ldc, acc, cv, I == J ? 2 : 1);
 }
 
 // acc += As(rows of quadrant wr) * Bs(rows of quadrant wc)^T over the 64 staged columns.  Lane group
