@@ -89,7 +89,7 @@ struct Ctx {
     DevBuf apre, lptr, eptr, lcj, lv, lvf, nfwtab, pmask;
     DevBuf key, bcnt, boff, bfill, perm;
     // pools
-    DevBuf lb, F, R, cvec, vec, nz, bbase, winv, model, out, status;
+    DevBuf lb, F, R, Rf, cvec, vec, nz, bbase, winv, model, out, status;  // Rf: R's copy for its factorization
     // last solution
     std::vector<double> last_src, last_model;
     // timers
@@ -486,7 +486,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         timers_collect(c);
         return QLX_OK;
     }
-    ok = c->R.ensure(r_tot * 8) && c->cvec.ensure(cv_tot * 8) && c->vec.ensure(vec_tot * 8) && c->nz.ensure(nz_tot * 4) &&
+    ok = c->R.ensure(r_tot * 8) && c->Rf.ensure(r_tot * 8) && c->cvec.ensure(cv_tot * 8) && c->vec.ensure(vec_tot * 8) && c->nz.ensure(nz_tot * 4) &&
          c->bbase.ensure(bb_tot * 4) && c->winv.ensure(w_tot * 8) && c->F.ensure(f_tot * (mixed ? 4 : 8));
     if (!ok) return fail(c, QLX_ERR_NOMEM, "dense buffer allocation failed (F/R)");
     bd.F = mixed ? nullptr : c->F.as<double>();
@@ -499,6 +499,39 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     bd.winv = c->winv.as<double>();
     HIPCHK(c, hipMemcpyAsync(c->d_meta.p, meta.data(), sizeof(ItemMeta) * B, hipMemcpyHostToDevice, st));
 
+    // ---- factorization descriptors: F (dense, FP64 or FP32) and R (banded FP64) of every item ----
+    std::vector<MatDesc> fm, rm;
+    for (int b = 0; b < B; b++) {
+        const ItemMeta& M = meta[b];
+        if (!M.valid) continue;
+        double* v = bd.vec + M.vec_off;
+        MatDesc f{};
+        f.A = mixed ? (void*)(bd.F32 + M.f_off) : (void*)(bd.F + M.f_off);
+        f.ld = M.ldN;
+        f.n = M.n_src;
+        f.nb = M.ldN / 64;
+        f.f32 = mixed ? 1 : 0;
+        f.bwb = 0;
+        f.diag = (void*)(v + M.ldN);
+        f.status = bd.status + 4 * b;
+        f.W = (void*)(bd.winv + M.w_off);
+        fm.push_back(f);
+        MatDesc r{};
+        r.A = (void*)(c->Rf.as<double>() + M.r_off);
+        r.ld = 64;
+        r.n = M.n_src;
+        r.nb = M.ldN / 64;
+        r.f32 = 0;
+        r.bwb = M.bwb;
+        r.diag = (void*)(v + 2 * (size_t)M.ldN);
+        r.status = bd.status + 4 * b + 1;
+        r.W = (void*)(bd.winv + M.w_off + (size_t)M.ldN * 64);
+        rm.push_back(r);
+    }
+    std::vector<MatDesc> mats(fm);
+    mats.insert(mats.end(), rm.begin(), rm.end());
+    HIPCHK(c, hipMemcpyAsync(c->d_mats.p, mats.data(), sizeof(MatDesc) * mats.size(), hipMemcpyHostToDevice, st));
+    const int nF = (int)fm.size(), nR = (int)rm.size();
     timer_begin(c, "lbuild");
     launch_merge_lrows_b(st, bd, max_img);
     launch_lrows_csr_b(st, bd, max_img);
@@ -523,55 +556,25 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     if (launch_build_lb_b(st, bd, max_rows, max_nsup, mixed ? 1 : 0, mixed ? 128 : 64) != 0)
         return fail(c, QLX_ERR_RANGE, "PSF too large for the Lb build (at most 8192 taps in FP64, 16384 mixed)");
     timer_end(c, "lbuild");
+    // R depends only on the active cells: it is built now and factored (ln det R only) on the second
+    // stream, from a copy, while the main stream forms F (the SYRK, then k_prep_f_b, which reads R
+    // itself); the streams join before the log-determinants.  (Started before the Lb build instead,
+    // its small launches slow that latency-bound kernel by more than they save.)
+    launch_curvature_b(st, bd, max_cells, max_rtiles);
+    HIPCHK(c, hipEventRecord(c->ev_fork, st));
+    HIPCHK(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    HIPCHK(c, hipMemcpyAsync(c->Rf.p, c->R.p, r_tot * 8, hipMemcpyDeviceToDevice, c->stream2));
+    int max_bwb = 1;
+    for (const MatDesc& r : rm) max_bwb = std::max(max_bwb, r.bwb);
+    cholesky_batched(c->stream2, c->d_mats.as<MatDesc>() + nF, 0, nR, max_ld / 64, 0, max_bwb);
+    HIPCHK(c, hipEventRecord(c->ev_join, c->stream2));
     if (!mixed) launch_gather_cvec_b(st, bd, max_rows);
     timer_begin(c, "syrk");
     if (mixed) launch_syrk_f16_b(st, bd, max_ncb);
     else launch_syrk_f64_b(st, bd, max_ncb);
     timer_end(c, "syrk");
-    launch_curvature_b(st, bd, max_cells, max_rtiles);
     launch_prep_fr_b(st, bd, max_ld, mixed ? 1 : 0);
 
-    // ---- factorizations: F (dense, FP64 or FP32) and R (banded FP64) of every item ----
-    std::vector<MatDesc> fm, rm;
-    for (int b = 0; b < B; b++) {
-        const ItemMeta& M = meta[b];
-        if (!M.valid) continue;
-        double* v = bd.vec + M.vec_off;
-        MatDesc f{};
-        f.A = mixed ? (void*)(bd.F32 + M.f_off) : (void*)(bd.F + M.f_off);
-        f.ld = M.ldN;
-        f.n = M.n_src;
-        f.nb = M.ldN / 64;
-        f.f32 = mixed ? 1 : 0;
-        f.bwb = 0;
-        f.diag = (void*)(v + M.ldN);
-        f.status = bd.status + 4 * b;
-        f.W = (void*)(bd.winv + M.w_off);
-        fm.push_back(f);
-        MatDesc r{};
-        r.A = (void*)(bd.R + M.r_off);
-        r.ld = 64;
-        r.n = M.n_src;
-        r.nb = M.ldN / 64;
-        r.f32 = 0;
-        r.bwb = M.bwb;
-        r.diag = (void*)(v + 2 * (size_t)M.ldN);
-        r.status = bd.status + 4 * b + 1;
-        r.W = (void*)(bd.winv + M.w_off + (size_t)M.ldN * 64);
-        rm.push_back(r);
-    }
-    std::vector<MatDesc> mats(fm);
-    mats.insert(mats.end(), rm.begin(), rm.end());
-    HIPCHK(c, hipMemcpyAsync(c->d_mats.p, mats.data(), sizeof(MatDesc) * mats.size(), hipMemcpyHostToDevice, st));
-    // F (dense, FP32 or FP64) factors on the main stream, R (banded FP64, ln det R only) on the
-    // second stream; they join before the log-determinants
-    const int nF = (int)fm.size(), nR = (int)rm.size();
-    HIPCHK(c, hipEventRecord(c->ev_fork, st));
-    HIPCHK(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
-    int max_bwb = 1;
-    for (const MatDesc& r : rm) max_bwb = std::max(max_bwb, r.bwb);
-    cholesky_batched(c->stream2, c->d_mats.as<MatDesc>() + nF, 0, nR, max_ld / 64, 0, max_bwb);
-    HIPCHK(c, hipEventRecord(c->ev_join, c->stream2));
     timer_begin(c, "cholesky");
     cholesky_batched(st, c->d_mats.as<MatDesc>(), mixed ? nF : 0, mixed ? 0 : nF, max_ld / 64, 1, 0);
     timer_end(c, "cholesky");
