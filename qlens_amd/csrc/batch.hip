@@ -975,7 +975,14 @@ __device__ __forceinline__ f16x8_t lds_k8(const uint16_t* arr, int g, int i)
     return __builtin_bit_cast(f16x8_t, make_uint4(a.x, a.y, b.x, b.y));
 }
 
-__global__ void __launch_bounds__(256) k_syrk_f16_b(BatchDev bd)
+// R entry (r, c), c <= r, of the banded curvature tiles (reference order, |r - c| within the band)
+__device__ __forceinline__ double* rtile(const BatchDev& bd, const ItemMeta& M, int r, int c)
+{
+    const int I = r >> 6, J = c >> 6;
+    return bd.R + M.r_off + ((size_t)I * (M.bwb + 1) + (I - J)) * TS * TS + (r & 63) * TS + (c & 63);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))) k_syrk_f16_b(BatchDev bd)
 {
     extern __shared__ __attribute__((aligned(16))) uint16_t lds2[];  // A, B: 16 x BP2 each
     const int item = blockIdx.y;
@@ -1046,18 +1053,74 @@ __global__ void __launch_bounds__(256) k_syrk_f16_b(BatchDev bd)
         kt = nk;
     }
 #undef SYRK_LOAD
+    // F's preparation, fused: scale back, F += lambda R over the source block (R is held in the
+    // reference order: internal indices map through pisrc), the data row n kept with a large pivot
+    // (the refined solve never reads it), identity beyond n and the FP64 diagonal into Fdiag.  A
+    // diagonal tile's upper half gets the same symmetric formula; the upper copy of the other tiles
+    // is written by k_upper_copy_b.
     const float unscale = (float)(1.0 / (bd.s16 * bd.s16));
-    float* C = bd.F32 + M.f_off + (size_t)I * SCB * M.ldN + (size_t)J * SCB;
+    const int ld = M.ldN, n = M.n_src;
+    const double lam = bd.lambda;
+    const int* pis = bd.pisrc + item * bd.s_cell;
+    double* fdiag = bd.vec + M.vec_off + 3 * (size_t)ld;
+    float* C = bd.F32 + M.f_off + (size_t)I * SCB * ld + (size_t)J * SCB;
+    const int r0 = I * SCB + wr * 64 + 4 * (lane >> 4), c0 = J * SCB + wc * 64 + (lane & 15);
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int c = 0; c < 4; c++) {
+        const int cg = c0 + c * 16;
+        const int pb = cg < n ? pis[cg] : -1;
 #pragma unroll
-        for (int c = 0; c < 4; c++)
+        for (int a = 0; a < 4; a++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int rr = wr * 64 + a * 16 + 4 * (lane >> 4) + v;
-                const int cc = wc * 64 + c * 16 + (lane & 15);
-                C[(size_t)rr * M.ldN + cc] = acc[a][c][v] * unscale;
+                const int r = r0 + a * 16 + v;
+                float x = acc[a][c][v] * unscale;
+                if (r < n && cg < n) {
+                    const int pa = pis[r];
+                    const int hi = pa > pb ? pa : pb, lo = pa > pb ? pb : pa;
+                    if ((hi >> 6) - (lo >> 6) <= M.bwb) x = (float)((double)x + lam * *rtile(bd, M, hi, lo));
+                    if (r == cg) fdiag[r] = (double)x;
+                } else if (r > n || cg > n) {
+                    x = (r == cg) ? 1.0f : 0.0f;
+                } else if (r == cg) {
+                    x = 1e30f;  // (n, n); the rest of row n (D) is kept
+                }
+                C[(size_t)(r - I * SCB) * ld + (cg - J * SCB)] = x;
             }
+    }
+}
+
+// Mixed path: the transposed copy of F's off-diagonal lower 64 x 64 tiles into the upper triangle,
+// through LDS, with 16-byte loads and stores (the residual of the iterative refinement reads F there
+// after the lower triangle has been factored; entries beyond row n are copied too and never read).
+__global__ void __launch_bounds__(256) k_upper_copy_b(BatchDev bd)
+{
+    __shared__ float tt[TS][TS + 1];
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    if (!M.valid) return;
+    const int ld = M.ldN, nb = ld / TS;
+    if ((int)blockIdx.x >= nb * (nb - 1) / 2) return;
+    int I, J;
+    tri_decode(blockIdx.x, I, J);  // J < I: the strictly lower tiles
+    I += 1;
+    float* F = bd.F32 + M.f_off;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        const int e = (tid + 256 * h) * 4, rr = e >> 6, cc = e & 63;
+        const float4_t v = *(const float4_t*)&F[(size_t)(I * TS + rr) * ld + J * TS + cc];
+        tt[cc][rr] = v.x;
+        tt[cc + 1][rr] = v.y;
+        tt[cc + 2][rr] = v.z;
+        tt[cc + 3][rr] = v.w;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        const int e = (tid + 256 * h) * 4, rr = e >> 6, cc = e & 63;
+        *(float4_t*)&F[(size_t)(J * TS + rr) * ld + I * TS + cc] = (float4_t){tt[rr][cc], tt[rr][cc + 1], tt[rr][cc + 2], tt[rr][cc + 3]};
+    }
 }
 
 void launch_syrk_f16_b(hipStream_t s, const BatchDev& bd, int max_ncb)
@@ -1072,11 +1135,6 @@ void launch_syrk_f16_b(hipStream_t s, const BatchDev& bd, int max_ncb)
 // couples cells at most two source rows apart, |a - b| <= 2*snx, so the lower band in 64-blocks is
 // bwb = (2*snx)/64 + 1 and the Cholesky factor stays inside it.
 // ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ double* rtile(const BatchDev& bd, const ItemMeta& M, int r, int c)
-{
-    const int I = r >> 6, J = c >> 6;
-    return bd.R + M.r_off + ((size_t)I * (M.bwb + 1) + (I - J)) * TS * TS + (r & 63) * TS + (c & 63);
-}
 
 __global__ void __launch_bounds__(256) k_zero_r_b(BatchDev bd)
 {
@@ -1197,6 +1255,11 @@ __global__ void __launch_bounds__(256) k_prep_f_b(BatchDev bd)
 
 void launch_prep_fr_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32)
 {
+    if (f32) {  // the mixed path's FP16 SYRK prepares F in its epilogue; the upper copy remains
+        const int nbm = max_ld / TS;
+        hipLaunchKernelGGL(k_upper_copy_b, dim3(nbm * (nbm - 1) / 2, bd.B), dim3(256), 0, s, bd);
+        return;
+    }
     const int nb = max_ld / TS;
     const unsigned g = nb * (nb + 1) / 2;
     if (f32) hipLaunchKernelGGL(k_prep_f_b<float>, dim3(g, bd.B), dim3(256), 0, s, bd);
