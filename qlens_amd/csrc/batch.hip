@@ -659,29 +659,35 @@ __device__ __forceinline__ void accumulate_columns(const BatchDev& bd, const uns
         const int G = (ng + 15) >> 4;
         unsigned cjs[BUILD_U];
         V vs[BUILD_U];
-        int wbs[BUILD_U];
+        int wbq[BUILD_U / 4];  // weight base of the quarter that slots 4g .. 4g+3 read
 #pragma unroll
-        for (int u = 0; u < BUILD_U; u++) {
-            cjs[u] = 0;
-            vs[u] = (V)0;
-            wbs[u] = 0;
-            if ((u >> 2) < G) {
-                const int gl = (u >> 2) + G * (lane >> 2);
-                const int sub = 4 * (u & 3) + (lane & 3);
+        for (int g = 0; g < BUILD_U / 4; g++) {
+            wbq[g] = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                cjs[4 * g + j] = 0;
+                vs[4 * g + j] = (V)0;
+            }
+            if (g < G) {
+                const int gl = g + G * (lane >> 2);
                 const int4 t = qtab[min(gl, ng - 1)];
-                const bool ok = gl < ng && sub < t.z;
-                const int e = t.x + (ok ? sub : 0);
-                cjs[u] = lcj[e];
-                const V v = lval[e];
-                vs[u] = ok ? v : (V)0;
-                wbs[u] = t.y;
+                wbq[g] = t.y;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int sub = 4 * j + (lane & 3);
+                    const bool ok = gl < ng && sub < t.z;
+                    const int e = t.x + (ok ? sub : 0);
+                    cjs[4 * g + j] = lcj[e];
+                    const V v = lval[e];
+                    vs[4 * g + j] = ok ? v : (V)0;
+                }
             }
         }
 #pragma unroll
         for (int u = 0; u < BUILD_U; u++) {
             if ((u >> 2) < G) {
                 const int col = (int)(cjs[u] & 16383u);
-                const WT wt = W[wbs[u] - (int)(cjs[u] >> 14)];
+                const WT wt = W[wbq[u >> 2] - (int)(cjs[u] >> 14)];
                 const int slot = (loc[col >> cbs] << cbs) + (col & (CB - 1));
                 if (FIX)
                     atomicAdd((unsigned*)row + slot, (unsigned)__float2int_rn((float)(wt * vs[u])));
@@ -697,7 +703,7 @@ __device__ __forceinline__ void accumulate_columns(const BatchDev& bd, const uns
 // fixq on the mixed path) is staged in LDS.  Dynamic LDS: [PSF] [nw x BUILD_TQ quarter table]
 // [nw x nsup_cap x CB rows].
 template <bool FIX>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3)))
 k_build_lb_b(BatchDev bd, int nsup_cap, int psf_elems)
 {
     typedef typename BuildTypes<FIX>::acc_t A;
