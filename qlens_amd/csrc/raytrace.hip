@@ -347,8 +347,10 @@ __device__ __forceinline__ int upper_index(const double* v, int n, double x)
 
 // One thread per (item, ray): item = blockIdx.y.  A lens listed as sur_lens takes its deflection
 // from sur_alpha (image-frame FP32 values written by the surrogate kernel for the same rays).
+// Three waves per SIMD: at its natural 195 FP64 registers the kernel ran two and was latency-bound
+// (5.0 ms per 64 evaluations); held to 168 it spills 36 registers and runs in 4.5 ms.
 template <typename T>
-__global__ void __launch_bounds__(256) k_raytrace_map_b(BatchDev bd, T tol)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_raytrace_map_b(BatchDev bd, T tol)
 {
     const int item = blockIdx.y;
     const int nsub = bd.nsub;
