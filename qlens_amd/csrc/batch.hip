@@ -1714,13 +1714,7 @@ k_chol_trail_m(const MatDesc* mats, int k0, int nq)
 // fall on distinct bank groups).
 #define TBP 72
 static_assert((long long)QLX_MAX_LD * QLX_MAX_LD * 4 < (1LL << 31), "trailing-update byte offsets must fit in 31 bits");
-__device__ __forceinline__ uint16_t bf16_rne(float f)
-{
-    unsigned u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf16_val(uint16_t h) { return __uint_as_float(((unsigned)h) << 16); }
+typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_chol_trail_bf16(const MatDesc* mats, int k0, int nq)
@@ -1771,20 +1765,17 @@ k_chol_trail_bf16(const MatDesc* mats, int k0, int nq)
         __syncthreads();
 #pragma unroll
         for (int h = 0; h < 8; h++) {
+            // round-to-nearest-even conversions (v_cvt_pk_bf16_f32 on gfx950)
             const int i = (tid >> 4) + 16 * h, k = 4 * (tid & 15);
-            uint16_t ah[4], al[4], bh[4], bl[4];
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const float xa = -pa[h][t], xb = pb[h][t];
-                ah[t] = bf16_rne(xa);
-                al[t] = bf16_rne(xa - bf16_val(ah[t]));
-                bh[t] = bf16_rne(xb);
-                bl[t] = bf16_rne(xb - bf16_val(bh[t]));
-            }
-            *(uint2*)&Ah[i * TBP + k] = make_uint2(ah[0] | ((unsigned)ah[1] << 16), ah[2] | ((unsigned)ah[3] << 16));
-            *(uint2*)&Al[i * TBP + k] = make_uint2(al[0] | ((unsigned)al[1] << 16), al[2] | ((unsigned)al[3] << 16));
-            *(uint2*)&Bh[i * TBP + k] = make_uint2(bh[0] | ((unsigned)bh[1] << 16), bh[2] | ((unsigned)bh[3] << 16));
-            *(uint2*)&Bl[i * TBP + k] = make_uint2(bl[0] | ((unsigned)bl[1] << 16), bl[2] | ((unsigned)bl[3] << 16));
+            const P xa = -pa[h], xb = pb[h];
+            const bf16x4_t ah = __builtin_convertvector(xa, bf16x4_t);
+            const bf16x4_t bh = __builtin_convertvector(xb, bf16x4_t);
+            const bf16x4_t al = __builtin_convertvector(xa - __builtin_convertvector(ah, P), bf16x4_t);
+            const bf16x4_t bl = __builtin_convertvector(xb - __builtin_convertvector(bh, P), bf16x4_t);
+            *(bf16x4_t*)&Ah[i * TBP + k] = ah;
+            *(bf16x4_t*)&Al[i * TBP + k] = al;
+            *(bf16x4_t*)&Bh[i * TBP + k] = bh;
+            *(bf16x4_t*)&Bl[i * TBP + k] = bl;
         }
         __syncthreads();
         if (q + 1 < nq) TRAIL_LOAD_BF(q + 1);
