@@ -1721,27 +1721,27 @@ k_chol_trail_m(const MatDesc* mats, int k0, int nq)
 #undef TRAIL_LOAD
 
 // FP32 matrices (mixed path): the same trailing update with the panel operands split x = hi + lo
-// into BF16 while they are staged (hi = bf16(x), lo = bf16(x - hi): 2^-17 of each entry) and all
-// four products hi*hi + hi*lo + lo*hi + lo*lo on v_mfma_f32_16x16x32_bf16, accumulated in FP32.
-// The accumulators start from the C tile and the A operand is negated while staging, so the MFMAs
-// form C - A B^T directly.  Loads go through a buffer resource (32-bit offsets: one VGPR per lane,
-// row offsets in SGPRs), which keeps the kernel inside the 256 registers of two waves per SIMD.
-// LDS: A hi, A lo, B hi, B lo as 128 rows x TBP BF16 (144-byte rows: the 16 rows of an operand read
-// fall on distinct bank groups).
-#define TBP 72
+// into BF16 while they are staged (hi = bf16(x), lo = bf16(x - hi): 2^-17 of each entry, with the
+// hardware round-to-nearest conversion) and all four products hi*hi + hi*lo + lo*hi + lo*lo on
+// v_mfma_f32_16x16x32_bf16, accumulated in FP32.  The accumulators start from the C tile and the A
+// operand is negated while staging, so the MFMAs form C - A B^T directly.  Loads go through a buffer
+// resource (32-bit offsets: one VGPR per lane, row offsets in SGPRs).  Operands are staged 32 k at
+// a time (LDS: A hi, A lo, B hi, B lo as 128 rows x TBP BF16, 40 KB; 80-byte rows put the 16 rows
+// of an operand read on distinct bank groups), which with 150 registers allows three waves per SIMD.
+#define TBP 40
 static_assert((long long)QLX_MAX_LD * QLX_MAX_LD * 4 < (1LL << 31), "trailing-update byte offsets must fit in 31 bits");
 typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
 k_chol_trail_bf16(const MatDesc* mats, int k0, int nq)
 {
     typedef float P __attribute__((ext_vector_type(4)));
     typedef __attribute__((address_space(1))) float gfloat;
     extern __shared__ double trail_raw[];
-    uint16_t* Ah = (uint16_t*)trail_raw;
-    uint16_t* Al = Ah + TT * TBP;
-    uint16_t* Bh = Al + TT * TBP;
-    uint16_t* Bl = Bh + TT * TBP;
+    __bf16* Ah = (__bf16*)trail_raw;
+    __bf16* Al = Ah + TT * TBP;
+    __bf16* Bh = Al + TT * TBP;
+    __bf16* Bl = Bh + TT * TBP;
     const MatDesc D = mats[blockIdx.y];
     const int r0b = k0 + nq;
     if (r0b >= D.nb) return;
@@ -1753,9 +1753,10 @@ k_chol_trail_bf16(const MatDesc* mats, int k0, int nq)
     const int ld = D.ld;  // ld <= QLX_MAX_LD: byte offsets within the matrix fit in 31 bits
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(D.A, 0, ld * ld * 4, 0x00020000);
     const int rI = (r0b + 2 * I2) * TS, rJ = (r0b + 2 * J2) * TS;
-    // byte offsets: the lane's part in a VGPR, the uniform part in an SGPR.  Panel piece h of a
-    // thread: row (tid >> 4) + 16 h, columns 4 (tid & 15) .. +3 of column block k0 + q
-    const int vp = ((tid >> 4) * ld + 4 * (tid & 15)) * 4;
+    // byte offsets: the lane's part in a VGPR, the uniform part in an SGPR.  Staging step st covers
+    // columns 32 st .. 32 st + 31 of the panel; piece h of a thread: row (tid >> 3) + 32 h, columns
+    // 4 (tid & 7) .. +3 of the step
+    const int vp = ((tid >> 3) * ld + 4 * (tid & 7)) * 4;
     const int sA = (rI * ld + k0 * TS) * 4, sB = (rJ * ld + k0 * TS) * 4;
     const int vc = ((wr * 64 + 4 * (lane >> 4)) * ld + wc * 64 + (lane & 15)) * 4;
     const int sC = (rI * ld + rJ) * 4;
@@ -1767,22 +1768,22 @@ k_chol_trail_bf16(const MatDesc* mats, int k0, int nq)
 #pragma unroll
             for (int v = 0; v < 4; v++)
                 acc[a][c][v] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, vc + c * 64, sC + (a * 16 + v) * ld * 4, 0));
-    P pa[8], pb[8];
-#define TRAIL_LOAD_BF(q)                                                                                           \
+    P pa[4], pb[4];
+#define TRAIL_LOAD_BF(st)                                                                                          \
     {                                                                                                              \
-        _Pragma("unroll") for (int h = 0; h < 8; h++)                                                              \
+        _Pragma("unroll") for (int h = 0; h < 4; h++)                                                              \
         {                                                                                                          \
-            pa[h] = __builtin_bit_cast(P, __builtin_amdgcn_raw_buffer_load_b128(rs, vp, sA + (16 * h * ld + (q) * TS) * 4, 0)); \
-            pb[h] = __builtin_bit_cast(P, __builtin_amdgcn_raw_buffer_load_b128(rs, vp, sB + (16 * h * ld + (q) * TS) * 4, 0)); \
+            pa[h] = __builtin_bit_cast(P, __builtin_amdgcn_raw_buffer_load_b128(rs, vp, sA + (32 * h * ld + (st) * 32) * 4, 0)); \
+            pb[h] = __builtin_bit_cast(P, __builtin_amdgcn_raw_buffer_load_b128(rs, vp, sB + (32 * h * ld + (st) * 32) * 4, 0)); \
         }                                                                                                          \
     }
     TRAIL_LOAD_BF(0);
-    for (int q = 0; q < nq; q++) {
+    const int nst = 2 * nq;
+    for (int st = 0; st < nst; st++) {
         __syncthreads();
 #pragma unroll
-        for (int h = 0; h < 8; h++) {
-            // round-to-nearest-even conversions (v_cvt_pk_bf16_f32 on gfx950)
-            const int i = (tid >> 4) + 16 * h, k = 4 * (tid & 15);
+        for (int h = 0; h < 4; h++) {
+            const int i = (tid >> 3) + 32 * h, k = 4 * (tid & 7);
             const P xa = -pa[h], xb = pb[h];
             const bf16x4_t ah = __builtin_convertvector(xa, bf16x4_t);
             const bf16x4_t bh = __builtin_convertvector(xb, bf16x4_t);
@@ -1794,29 +1795,26 @@ k_chol_trail_bf16(const MatDesc* mats, int k0, int nq)
             *(bf16x4_t*)&Bl[i * TBP + k] = bl;
         }
         __syncthreads();
-        if (q + 1 < nq) TRAIL_LOAD_BF(q + 1);
+        if (st + 1 < nst) TRAIL_LOAD_BF(st + 1);
+        const int kq = 8 * (lane >> 4);
+        bf16x8_t bhv[4], blv[4];
 #pragma unroll
-        for (int kk = 0; kk < TS; kk += 32) {
-            const int kq = kk + 8 * (lane >> 4);
-            bf16x8_t bhv[4], blv[4];
+        for (int t = 0; t < 4; t++) {
+            const int jb = (wc * 64 + t * 16 + (lane & 15)) * TBP + kq;
+            bhv[t] = *(const bf16x8_t*)&Bh[jb];
+            blv[t] = *(const bf16x8_t*)&Bl[jb];
+        }
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const int jb = (wc * 64 + t * 16 + (lane & 15)) * TBP + kq;
-                bhv[t] = *(const bf16x8_t*)&Bh[jb];
-                blv[t] = *(const bf16x8_t*)&Bl[jb];
-            }
+        for (int a = 0; a < 4; a++) {
+            const int ia = (wr * 64 + a * 16 + (lane & 15)) * TBP + kq;
+            const bf16x8_t ahv = *(const bf16x8_t*)&Ah[ia];
+            const bf16x8_t alv = *(const bf16x8_t*)&Al[ia];
 #pragma unroll
-            for (int a = 0; a < 4; a++) {
-                const int ia = (wr * 64 + a * 16 + (lane & 15)) * TBP + kq;
-                const bf16x8_t ahv = *(const bf16x8_t*)&Ah[ia];
-                const bf16x8_t alv = *(const bf16x8_t*)&Al[ia];
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahv, bhv[c], acc[a][c], 0, 0, 0);
-                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahv, blv[c], acc[a][c], 0, 0, 0);
-                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alv, bhv[c], acc[a][c], 0, 0, 0);
-                    acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alv, blv[c], acc[a][c], 0, 0, 0);
-                }
+            for (int c = 0; c < 4; c++) {
+                acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahv, bhv[c], acc[a][c], 0, 0, 0);
+                acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahv, blv[c], acc[a][c], 0, 0, 0);
+                acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alv, bhv[c], acc[a][c], 0, 0, 0);
+                acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alv, blv[c], acc[a][c], 0, 0, 0);
             }
         }
     }
