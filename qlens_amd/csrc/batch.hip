@@ -699,11 +699,15 @@ __device__ __forceinline__ void accumulate_columns(const BatchDev& bd, const uns
     }
 }
 
-// FIX: mixed path (int32 fixed-point rows, an FP16 pool); else FP64.  The transposed PSF (times
+// FIX: mixed path (int32 fixed-point rows, an FP16 pool); else FP64.  Occupancy: the mixed path's
+// 165 registers give three waves per SIMD (as many as its LDS allows on the benchmark workload);
+// 32 slots per lane fit four waves but make rows of ~140 quarters take two passes (slower there).
+// Split into two kernels with their own occupancy attributes, the mixed one ran 8.5 against 7.2 ms.  The transposed PSF (times
 // fixq on the mixed path) is staged in LDS.  Dynamic LDS: [PSF] [nw x BUILD_TQ quarter table]
 // [nw x nsup_cap x CB rows].
 template <bool FIX>
-__device__ __forceinline__ void build_lb_body(const BatchDev& bd, int nsup_cap, int psf_elems)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3)))
+k_build_lb_b(BatchDev bd, int nsup_cap, int psf_elems)
 {
     typedef typename BuildTypes<FIX>::acc_t A;
     typedef typename BuildTypes<FIX>::w_t WT;
@@ -806,21 +810,6 @@ __device__ __forceinline__ void build_lb_body(const BatchDev& bd, int nsup_cap, 
     }
 }
 
-// Occupancy: the mixed path's int32 rows fit 168 registers (three waves per SIMD, as many as its
-// LDS allows on the benchmark workload); the FP64 path needs more.  (32 slots per lane fit four
-// waves, but rows of ~140 quarters then take two passes: slower on the 200^2 / 21^2-PSF workload.)
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
-k_build_lb_mixed(BatchDev bd, int nsup_cap, int psf_elems)
-{
-    build_lb_body<true>(bd, nsup_cap, psf_elems);
-}
-
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3)))
-k_build_lb_f64(BatchDev bd, int nsup_cap, int psf_elems)
-{
-    build_lb_body<false>(bd, nsup_cap, psf_elems);
-}
-
 template <bool FIX>
 static int launch_build_typed(hipStream_t s, const BatchDev& bd, int max_nkt, int max_nsup, int cbw)
 {
@@ -834,10 +823,8 @@ static int launch_build_typed(hipStream_t s, const BatchDev& bd, int max_nkt, in
     while (nw > 1 && psf_bytes + per_wave * nw > 144 * 1024) nw >>= 1;
     const size_t lds = psf_bytes + per_wave * nw;
     const int psf_elems = (int)(psf_bytes / 8);  // in doubles: the offset of the quarter tables
-    const void* kern = FIX ? (const void*)k_build_lb_mixed : (const void*)k_build_lb_f64;
-    allow_big_lds(kern);
-    if (FIX) hipLaunchKernelGGL(k_build_lb_mixed, dim3(max_nkt, bd.B), dim3(64 * nw), lds, s, bd, max_nsup, psf_elems);
-    else hipLaunchKernelGGL(k_build_lb_f64, dim3(max_nkt, bd.B), dim3(64 * nw), lds, s, bd, max_nsup, psf_elems);
+    allow_big_lds((const void*)k_build_lb_b<FIX>);
+    hipLaunchKernelGGL((k_build_lb_b<FIX>), dim3(max_nkt, bd.B), dim3(64 * nw), lds, s, bd, max_nsup, psf_elems);
     return 0;
 }
 
