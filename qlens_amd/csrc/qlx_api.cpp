@@ -416,7 +416,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         timer_end(c, "surrogate");
     }
     timer_begin(c, "raytrace");
-    launch_raytrace_map_b(st, bd, S.ray_precision == QLX_RAY_FP32);
+    launch_raytrace_map_b(st, bd, S.ray_precision == QLX_RAY_FP32, mixed ? 1 : 0);
     timer_end(c, "raytrace");
     launch_compact_b(st, bd);
     std::vector<int> counts((size_t)B * 4);

@@ -148,7 +148,7 @@ struct MatDesc {
 void upload_patterson(int device);
 
 // ---- raytrace.hip ----
-void launch_raytrace_map_b(hipStream_t s, const BatchDev& bd, int ray_fp32);
+void launch_raytrace_map_b(hipStream_t s, const BatchDev& bd, int ray_fp32, int kappa_table);
 void launch_raytrace_points(hipStream_t stream, const ModelPrep& m, double tol, int n, const double* xy, double* beta);
 void launch_nfw_def_hess(hipStream_t stream, const LensPrep& L, double tol, int n, const double* xy, double* out5);
 void launch_mock_sb(hipStream_t stream, int nx, int ny, int nsplit, const double* xvals, const double* yvals, const ModelPrep& m,

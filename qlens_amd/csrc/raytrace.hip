@@ -18,6 +18,7 @@
 //   L weights               calculate_Lmatrix_interpolate          pixelgrid.cpp:2501-2527
 #include "qlx_internal.h"
 #include "patterson_table.h"
+#include "nfw_gtab.h"
 
 namespace qlx {
 
@@ -80,6 +81,29 @@ __device__ __forceinline__ T nfw_kappa_x(const LensPrep& L, T xsq)
     }
     return (T)2 * ks * ((T)1 - F) / (xsq - (T)1);
 }
+
+// Mixed path: kappa = 2 ks g(x^2) with g(t) = (1 - F(t)) / (t - 1) from the piecewise polynomials
+// of nfw_gtab.h (relative error below 5e-13; gen_nfw_gtab.py): the piece comes from the exponent
+// and the top five mantissa bits of t, u in [0, 1) from the remaining bits, then five FMAs.  Below
+// 1e-6 (the reference's logarithmic form) and outside the table the closed form is used.
+__device__ __forceinline__ double nfw_kappa_gtab(const LensPrep& L, double t)
+{
+    if (t < 1e-6) return nfw_kappa_x<double>(L, t);
+    const long long b = __double_as_longlong(t);
+    const int e = (int)((b >> 52) & 0x7ff) - 1023;
+    if (e < QLX_GTAB_EMIN || e >= QLX_GTAB_EMAX) return nfw_kappa_x<double>(L, t);
+    const int p = (int)((b >> 47) & (QLX_GTAB_NP - 1));
+    const double u = __longlong_as_double(((b & ((1LL << 47) - 1)) << 5) | (1023LL << 52)) - 1.0;
+    const double* c = qlx_gtab + ((e - QLX_GTAB_EMIN) * QLX_GTAB_NP + p) * QLX_GTAB_NC;
+    double g = c[5];
+    g = fma(g, u, c[4]);
+    g = fma(g, u, c[3]);
+    g = fma(g, u, c[2]);
+    g = fma(g, u, c[1]);
+    g = fma(g, u, c[0]);
+    return 2.0 * L.ks * g;
+}
+static_assert(QLX_GTAB_NP == 32 && QLX_GTAB_NC == 6, "nfw_kappa_gtab assumes 32 pieces per octave, degree 5");
 
 template <typename T>
 __device__ __forceinline__ T nfw_kappa_rsq(const LensPrep& L, T rsq)
@@ -180,7 +204,7 @@ __global__ void __launch_bounds__(256) k_nfw_tables(BatchDev bd)
 }
 
 // nfw_j01 with the node constants of k_nfw_tables (tab: this lens's QLX_PAT_N x 4 doubles)
-template <typename T>
+template <typename T, bool KT = false>
 __device__ __forceinline__ int nfw_j01_tab(const LensPrep& L, const double* tab, T x, T y, T tol, T& j0, T& j1)
 {
     const T xsq = x * x, ysq = y * y;
@@ -196,7 +220,10 @@ __device__ __forceinline__ int nfw_j01_tab(const LensPrep& L, const double* tab,
         const int step = (QLX_PAT_N + 1) / (order + 1);
         for (int i = step - 1; i < QLX_PAT_N; i += 2 * step) {
             const double* tn = tab + 4 * i;
-            const T kap = nfw_kappa_x<T>(L, (T)tn[0] * xsq + (T)tn[1] * ysq);
+            const T xs = (T)tn[0] * xsq + (T)tn[1] * ysq;
+            T kap;
+            if constexpr (KT) kap = nfw_kappa_gtab(L, xs);
+            else kap = nfw_kappa_x<T>(L, xs);
             const T f0 = (T)tn[2] * kap;
             const T f1 = (T)tn[3] * kap;
 #pragma unroll
@@ -251,7 +278,7 @@ __device__ __forceinline__ T patterson_k(const LensPrep& L, T x, T y, T tol, T n
 
 // Deflection of one lens at (x, y) (profile.cpp:1782-1805; shear models.cpp:2266-2272).  TAB: the
 // NFW integrals use the node constants tab of k_nfw_tables; otherwise they are formed per node.
-template <typename T, bool TAB = false>
+template <typename T, bool TAB = false, bool KT = false>
 __device__ __forceinline__ void lens_deflection(const LensPrep& L, T x, T y, T tol, T& d0, T& d1, int& nev,
                                                 const double* tab = nullptr)
 {
@@ -289,7 +316,7 @@ __device__ __forceinline__ void lens_deflection(const LensPrep& L, T x, T y, T t
                 a1 = 0;
             } else {
                 T j0, j1;
-                if constexpr (TAB) nev += nfw_j01_tab<T>(L, tab, x, y, tol, j0, j1);
+                if constexpr (TAB) nev += nfw_j01_tab<T, KT>(L, tab, x, y, tol, j0, j1);
                 else nev += nfw_j01<T>(L, x, y, tol, j0, j1);
                 a0 = x * j0;
                 a1 = y * j1;
@@ -349,7 +376,7 @@ __device__ __forceinline__ int upper_index(const double* v, int n, double x)
 // from sur_alpha (image-frame FP32 values written by the surrogate kernel for the same rays).
 // Three waves per SIMD: at its natural 195 FP64 registers the kernel ran two and was latency-bound
 // (5.0 ms per 64 evaluations); held to 168 it spills 36 registers and runs in 4.5 ms.
-template <typename T>
+template <typename T, bool KT = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_raytrace_map_b(BatchDev bd, T tol)
 {
     const int item = blockIdx.y;
@@ -390,7 +417,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
             d0 = (T)sa0;
             d1 = (T)bd.sur_alpha[ro * 2 + 1];
         } else {
-            lens_deflection<T, true>(m.lens[l], (T)sx, (T)sy, tol, d0, d1, nev,
+            lens_deflection<T, true, KT>(m.lens[l], (T)sx, (T)sy, tol, d0, d1, nev,
                                      bd.nfwtab + (size_t)(item * QLX_MAX_LENSES + l) * QLX_PAT_N * 4);
         }
         a0 += d0;
@@ -428,13 +455,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     bd.pmapped[(size_t)item * bd.s_pix + p] = 1;
 }
 
-void launch_raytrace_map_b(hipStream_t stream, const BatchDev& bd, int ray_fp32)
+// ray_fp32: FP32 ray shooting; kappa_table: FP64 rays with the NFW convergence from nfw_gtab.h
+// (the mixed path); otherwise FP64 with the closed-form convergence (the parity path)
+void launch_raytrace_map_b(hipStream_t stream, const BatchDev& bd, int ray_fp32, int kappa_table)
 {
     const long long total = (long long)bd.nx * bd.ny * bd.nsub;
     const unsigned blocks = (unsigned)((total + 255) / 256);
     hipLaunchKernelGGL(k_nfw_tables, dim3((QLX_PAT_N + 255) / 256, bd.B * QLX_MAX_LENSES), dim3(256), 0, stream, bd);
     if (ray_fp32)
         hipLaunchKernelGGL(k_raytrace_map_b<float>, dim3(blocks, bd.B), dim3(256), 0, stream, bd, (float)bd.tol);
+    else if (kappa_table)
+        hipLaunchKernelGGL((k_raytrace_map_b<double, true>), dim3(blocks, bd.B), dim3(256), 0, stream, bd, bd.tol);
     else
         hipLaunchKernelGGL(k_raytrace_map_b<double>, dim3(blocks, bd.B), dim3(256), 0, stream, bd, bd.tol);
 }
