@@ -991,6 +991,48 @@ __device__ __forceinline__ double* rtile(const BatchDev& bd, const ItemMeta& M, 
     return bd.R + M.r_off + ((size_t)I * (M.bwb + 1) + (I - J)) * TS * TS + (r & 63) * TS + (c & 63);
 }
 
+// F's preparation, fused into the FP16 SYRK's store of tile (I, J): scale back, F += lambda R over
+// the source block (R is held in the reference order: internal indices map through pisrc), the data
+// row n kept with a large pivot (the refined solve never reads it), identity beyond n and the FP64
+// diagonal into Fdiag.  A diagonal tile's upper half gets the same symmetric formula; the upper copy
+// of the other tiles is written by k_upper_copy_b.  Wave (wr, wc) holds quadrant (wr, wc).
+__device__ __forceinline__ void syrk_f16_store(const BatchDev& bd, const ItemMeta& M, int item, int I, int J,
+                                               const float4_t (&acc)[4][4])
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = w >> 1, wc = w & 1;
+    const float unscale = (float)(1.0 / (bd.s16 * bd.s16));
+    const int ld = M.ldN, n = M.n_src;
+    const double lam = bd.lambda;
+    const int* pis = bd.pisrc + item * bd.s_cell;
+    double* fdiag = bd.vec + M.vec_off + 3 * (size_t)ld;
+    float* C = bd.F32 + M.f_off + (size_t)I * SCB * ld + (size_t)J * SCB;
+    const int r0 = I * SCB + wr * 64 + 4 * (lane >> 4), c0 = J * SCB + wc * 64 + (lane & 15);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int cg = c0 + c * 16;
+        const int pb = cg < n ? pis[cg] : -1;
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = r0 + a * 16 + v;
+                float x = acc[a][c][v] * unscale;
+                if (r < n && cg < n) {
+                    const int pa = pis[r];
+                    const int hi = pa > pb ? pa : pb, lo = pa > pb ? pb : pa;
+                    if ((hi >> 6) - (lo >> 6) <= M.bwb) x = (float)((double)x + lam * *rtile(bd, M, hi, lo));
+                    if (r == cg) fdiag[r] = (double)x;
+                } else if (r > n || cg > n) {
+                    x = (r == cg) ? 1.0f : 0.0f;
+                } else if (r == cg) {
+                    x = 1e30f;  // (n, n); the rest of row n (D) is kept
+                }
+                C[(size_t)(r - I * SCB) * ld + (cg - J * SCB)] = x;
+            }
+    }
+}
+
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))) k_syrk_f16_b(BatchDev bd)
 {
     extern __shared__ __attribute__((aligned(16))) uint16_t lds2[];  // A, B: 16 x BP2 each
@@ -1062,41 +1104,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
         kt = nk;
     }
 #undef SYRK_LOAD
-    // F's preparation, fused: scale back, F += lambda R over the source block (R is held in the
-    // reference order: internal indices map through pisrc), the data row n kept with a large pivot
-    // (the refined solve never reads it), identity beyond n and the FP64 diagonal into Fdiag.  A
-    // diagonal tile's upper half gets the same symmetric formula; the upper copy of the other tiles
-    // is written by k_upper_copy_b.
-    const float unscale = (float)(1.0 / (bd.s16 * bd.s16));
-    const int ld = M.ldN, n = M.n_src;
-    const double lam = bd.lambda;
-    const int* pis = bd.pisrc + item * bd.s_cell;
-    double* fdiag = bd.vec + M.vec_off + 3 * (size_t)ld;
-    float* C = bd.F32 + M.f_off + (size_t)I * SCB * ld + (size_t)J * SCB;
-    const int r0 = I * SCB + wr * 64 + 4 * (lane >> 4), c0 = J * SCB + wc * 64 + (lane & 15);
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const int cg = c0 + c * 16;
-        const int pb = cg < n ? pis[cg] : -1;
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int r = r0 + a * 16 + v;
-                float x = acc[a][c][v] * unscale;
-                if (r < n && cg < n) {
-                    const int pa = pis[r];
-                    const int hi = pa > pb ? pa : pb, lo = pa > pb ? pb : pa;
-                    if ((hi >> 6) - (lo >> 6) <= M.bwb) x = (float)((double)x + lam * *rtile(bd, M, hi, lo));
-                    if (r == cg) fdiag[r] = (double)x;
-                } else if (r > n || cg > n) {
-                    x = (r == cg) ? 1.0f : 0.0f;
-                } else if (r == cg) {
-                    x = 1e30f;  // (n, n); the rest of row n (D) is kept
-                }
-                C[(size_t)(r - I * SCB) * ld + (cg - J * SCB)] = x;
-            }
-    }
+    syrk_f16_store(bd, M, item, I, J, acc);
 }
 
 // Mixed path: the transposed copy of F's off-diagonal lower 64 x 64 tiles into the upper triangle,
