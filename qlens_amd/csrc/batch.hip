@@ -462,6 +462,35 @@ void launch_sort_rows_b(hipStream_t s, const BatchDev& bd, int max_img, int max_
 // (one workgroup per chunk), then per-item slot bases of the compact block pool.
 // ------------------------------------------------------------------------------------------------
 
+// PSF x window, per grid pixel t = y*nx + x: hmask[t] = OR of pmask over the active pixels x' in
+// [x + nxh - pnx + 1, x + nxh] of row y (one contiguous active-index range [apre[..i0], apre[..i1+1]))
+__global__ void __launch_bounds__(256) k_psf_hdilate_b(BatchDev bd)
+{
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    const int npix = bd.nx * bd.ny;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (!M.valid || t >= npix) return;
+    const int y = t / bd.nx, x = t - y * bd.nx;
+    const int nxh = bd.pnx / 2;
+    const int i0 = max(0, x + nxh - bd.pnx + 1), i1 = min(bd.nx - 1, x + nxh);
+    const int* apre = bd.apre + item * (bd.s_pix + 1);
+    const unsigned* pm = bd.pmask + (size_t)item * bd.s_pix * QLX_MAX_CBW;
+    unsigned m[QLX_MAX_CBW];
+#pragma unroll
+    for (int w = 0; w < QLX_MAX_CBW; w++) m[w] = 0;
+    const int a0 = apre[y * bd.nx + i0], a1 = apre[y * bd.nx + i1 + 1];
+    for (int a = a0; a < a1; a++) {
+#pragma unroll
+        for (int w = 0; w < QLX_MAX_CBW; w++)
+            if (w < M.ncbw) m[w] |= pm[(size_t)a * QLX_MAX_CBW + w];
+    }
+    unsigned* hm = bd.hmask + ((size_t)item * bd.s_pix + t) * QLX_MAX_CBW;
+#pragma unroll
+    for (int w = 0; w < QLX_MAX_CBW; w++)
+        if (w < M.ncbw) hm[w] = m[w];
+}
+
 __global__ void __launch_bounds__(256) k_row_support_b(BatchDev bd)
 {
     __shared__ unsigned msk[QLX_MAX_CBW];
@@ -487,23 +516,18 @@ __global__ void __launch_bounds__(256) k_row_support_b(BatchDev bd)
 #pragma unroll
         for (int w = 0; w < QLX_MAX_CBW; w++) m[w] |= (tid == 0 && w == (cbd >> 5)) ? (1u << (cbd & 31)) : 0u;
     }
-    const int nxh = bd.pnx / 2, nyh = bd.pny / 2;
-    const int* apre = bd.apre + item * (bd.s_pix + 1);
-    const unsigned* pm = bd.pmask + (size_t)item * bd.s_pix * QLX_MAX_CBW;
-    // (row, window column) pairs: the window column's pixels x in [i0, i1] at y = l + nyh - pl are
-    // the active indices [apre[y*nx + i0], apre[y*nx + i1 + 1]); their block masks are ORed
+    const int nyh = bd.pny / 2;
+    const unsigned* hm = bd.hmask + (size_t)item * bd.s_pix * QLX_MAX_CBW;
+    // (row, window row) pairs: the PSF window of row (k, l) at y = l + nyh - pl is the x-window
+    // union already held by hmask[y*nx + k]
     for (int q = tid; q < rows * bd.pny; q += 256) {
         const int rr = q / bd.pny, pl = q - rr * bd.pny;
-        const int k = rk[rr], jj = rl[rr] + nyh - pl;
+        const int jj = rl[rr] + nyh - pl;
         if (jj < 0 || jj >= bd.ny) continue;
-        const int i0 = max(0, k + nxh - bd.pnx + 1), i1 = min(bd.nx - 1, k + nxh);
-        const long long t0 = (long long)jj * bd.nx;
-        const int a0 = apre[t0 + i0], a1 = apre[t0 + i1 + 1];
-        for (int a = a0; a < a1; a++) {
+        const unsigned* h = hm + ((size_t)jj * bd.nx + rk[rr]) * QLX_MAX_CBW;
 #pragma unroll
-            for (int w = 0; w < QLX_MAX_CBW; w++)
-                if (w < M.ncbw) m[w] |= pm[(size_t)a * QLX_MAX_CBW + w];
-        }
+        for (int w = 0; w < QLX_MAX_CBW; w++)
+            if (w < M.ncbw) m[w] |= h[w];
     }
 #pragma unroll
     for (int w = 0; w < QLX_MAX_CBW; w++)
@@ -560,6 +584,8 @@ __global__ void __launch_bounds__(1024) k_block_scan_b(BatchDev bd)
 void launch_row_support_b(hipStream_t s, const BatchDev& bd, int max_nkt)
 {
     if (max_nkt <= 0) return;
+    const int npix = bd.nx * bd.ny;
+    hipLaunchKernelGGL(k_psf_hdilate_b, dim3((npix + 255) / 256, bd.B), dim3(256), 0, s, bd);
     hipLaunchKernelGGL(k_row_support_b, dim3(max_nkt, bd.B), dim3(256), 0, s, bd);
     hipLaunchKernelGGL(k_block_scan_b, dim3(1, bd.B), dim3(1024), 0, s, bd);
 }

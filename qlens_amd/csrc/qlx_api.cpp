@@ -86,7 +86,7 @@ struct Ctx {
     DevBuf d_models, d_sgrid, d_srcn, d_meta, d_mats;
     // per-item work (stride s_pix / s_cell)
     DevBuf beta, map_cell, map_w, pmapped, cmapped, sidx, tix, pisrc, pidx, act_pix, counts, ucol, uval, ucnt, nevals;
-    DevBuf apre, lptr, eptr, lcj, lv, lvf, nfwtab, pmask;
+    DevBuf apre, lptr, eptr, lcj, lv, lvf, nfwtab, pmask, hmask;
     DevBuf key, bcnt, boff, bfill, perm;
     // pools
     DevBuf lb, F, R, Rf, cvec, vec, nz, bbase, winv, model, out, status;  // Rf: R's copy for its factorization
@@ -318,7 +318,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
               c->apre.ensure(B * (s_pix + 1) * 4) && c->lptr.ensure(B * (s_pix + 1) * 4) && c->eptr.ensure(B * (s_pix + 1) * 4) && c->lcj.ensure(B * s_pix * U * 4) &&
               c->lv.ensure(B * s_pix * U * 8) && c->lvf.ensure(B * s_pix * U * 4) &&
               c->nfwtab.ensure((size_t)B * QLX_NFWTAB_DOUBLES * 8) &&
-              c->pmask.ensure((size_t)B * s_pix * QLX_MAX_CBW * 4) &&
+              c->pmask.ensure((size_t)B * s_pix * QLX_MAX_CBW * 4) && c->hmask.ensure((size_t)B * s_pix * QLX_MAX_CBW * 4) &&
               c->bcnt.ensure(B * (s_cell + 2) * 4) && c->boff.ensure(B * (s_cell + 2) * 4) && c->bfill.ensure(B * (s_cell + 2) * 4) &&
               c->perm.ensure(B * s_pix * 4) && c->model.ensure(B * s_pix * 16) && c->out.ensure((size_t)B * QLX_OUT_STRIDE * 8) &&
               c->status.ensure((size_t)B * 16);
@@ -370,6 +370,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     bd.lvf = c->lvf.as<float>();
     bd.nfwtab = c->nfwtab.as<double>();
     bd.pmask = c->pmask.as<unsigned>();
+    bd.hmask = c->hmask.as<unsigned>();
     {
         // |blurred L entry| <= sum|psf| * max_r ||L_r||_1 <= 3 sum|psf| (per subpixel the 3-point
         // weights are 1/nsub times barycentric coordinates with |w1|, |w2| <= 1/2, 0 <= w0 <= 2),

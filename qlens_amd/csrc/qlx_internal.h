@@ -109,6 +109,7 @@ struct BatchDev {
     double* lv;                 // s_pix*3*nsub: CSR entry value
     float* lvf;                 // s_pix*3*nsub: CSR entry value in FP32 (mixed path)
     unsigned* pmask;            // s_pix*QLX_MAX_CBW: column blocks touched by each active pixel's L row
+    unsigned* hmask;            // s_pix*QLX_MAX_CBW per grid pixel (y-major): pmask ORed over the PSF's x window
     double fixq;                // mixed path: blurred rows accumulate as int32 multiples of 1/fixq
     double s16;                 // mixed path: power of two that scales the FP16 pool entries (|x s16| <= 2^14)
     int* key;                   // s_pix
