@@ -1020,8 +1020,9 @@ __device__ __forceinline__ double* rtile(const BatchDev& bd, const ItemMeta& M, 
 // F's preparation, fused into the FP16 SYRK's store of tile (I, J): scale back, F += lambda R over
 // the source block (R is held in the reference order: internal indices map through pisrc), the data
 // row n kept with a large pivot (the refined solve never reads it), identity beyond n and the FP64
-// diagonal into Fdiag.  A diagonal tile's upper half gets the same symmetric formula; the upper copy
-// of the other tiles is written by k_upper_copy_b.  Wave (wr, wc) holds quadrant (wr, wc).
+// diagonal into Fdiag.  A diagonal tile's upper half gets the same symmetric formula.  The tiles go to
+// Ff32 (F as formed, which the refinement residual reads); the factor buffer F32 receives them from
+// k_strip_copy_b and the first trailing update.  Wave (wr, wc) holds quadrant (wr, wc).
 __device__ __forceinline__ void syrk_f16_store(const BatchDev& bd, const ItemMeta& M, int item, int I, int J,
                                                const float4_t (&acc)[4][4])
 {
@@ -1032,7 +1033,7 @@ __device__ __forceinline__ void syrk_f16_store(const BatchDev& bd, const ItemMet
     const double lam = bd.lambda;
     const int* pis = bd.pisrc + item * bd.s_cell;
     double* fdiag = bd.vec + M.vec_off + 3 * (size_t)ld;
-    float* C = bd.F32 + M.f_off + (size_t)I * SCB * ld + (size_t)J * SCB;
+    float* C = bd.Ff32 + M.f_off + (size_t)I * SCB * ld + (size_t)J * SCB;
     const int r0 = I * SCB + wr * 64 + 4 * (lane >> 4), c0 = J * SCB + wc * 64 + (lane & 15);
 #pragma unroll
     for (int c = 0; c < 4; c++) {
@@ -1133,36 +1134,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
     syrk_f16_store(bd, M, item, I, J, acc);
 }
 
-// Mixed path: the transposed copy of F's off-diagonal lower 64 x 64 tiles into the upper triangle,
-// through LDS, with 16-byte loads and stores (the residual of the iterative refinement reads F there
-// after the lower triangle has been factored; entries beyond row n are copied too and never read).
-__global__ void __launch_bounds__(256) k_upper_copy_b(BatchDev bd)
+// Mixed path: F's first four block columns (the lower 64 x 64 tiles (I, J), J < 4), which the first
+// panel factors, from Ff32 into the factor buffer F32; the first trailing update moves the rest
+// (it reads its C tiles from Ff32 and writes F32).
+__global__ void __launch_bounds__(256) k_strip_copy_b(BatchDev bd)
 {
-    __shared__ float tt[TS][TS + 1];
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     if (!M.valid) return;
     const int ld = M.ldN, nb = ld / TS;
-    if ((int)blockIdx.x >= nb * (nb - 1) / 2) return;
-    int I, J;
-    tri_decode(blockIdx.x, I, J);  // J < I: the strictly lower tiles
-    I += 1;
-    float* F = bd.F32 + M.f_off;
+    const int I = blockIdx.x >> 2, J = blockIdx.x & 3;
+    if (I >= nb || J > I) return;
+    const size_t o = M.f_off + (size_t)I * TS * ld + (size_t)J * TS;
     const int tid = threadIdx.x;
 #pragma unroll
     for (int h = 0; h < 4; h++) {
         const int e = (tid + 256 * h) * 4, rr = e >> 6, cc = e & 63;
-        const float4_t v = *(const float4_t*)&F[(size_t)(I * TS + rr) * ld + J * TS + cc];
-        tt[cc][rr] = v.x;
-        tt[cc + 1][rr] = v.y;
-        tt[cc + 2][rr] = v.z;
-        tt[cc + 3][rr] = v.w;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int h = 0; h < 4; h++) {
-        const int e = (tid + 256 * h) * 4, rr = e >> 6, cc = e & 63;
-        *(float4_t*)&F[(size_t)(J * TS + rr) * ld + I * TS + cc] = (float4_t){tt[rr][cc], tt[rr][cc + 1], tt[rr][cc + 2], tt[rr][cc + 3]};
+        *(float4_t*)&bd.F32[o + (size_t)rr * ld + cc] = *(const float4_t*)&bd.Ff32[o + (size_t)rr * ld + cc];
     }
 }
 
@@ -1238,16 +1226,11 @@ void launch_curvature_b(hipStream_t s, const BatchDev& bd, int max_cells, int ma
     hipLaunchKernelGGL(k_curvature_b, dim3((n + 255) / 256, bd.B), dim3(256), 0, s, bd);
 }
 
-// F_lower += lambda R over the n x n source block; row n (the data row D^T | d^T C d of the
-// augmented factorization) is kept, rows > n are identity.
-// Mixed (FP32 F): every regularized lower entry is also copied to the transposed upper position
-// (the residual of the iterative refinement reads F there after the lower triangle has been
-// factored), the diagonal goes to the Fdiag slot in FP64, and the data-row pivot is set to a
-// large value: the refined solve takes D from the upper copy and y from row n, never the pivot.
-template <typename T>
+// FP64 path: F_lower += lambda R over the n x n source block; row n (the data row D^T | d^T C d of
+// the augmented factorization) is kept, rows > n are identity.  (The mixed path does this in the
+// FP16 SYRK's epilogue, syrk_f16_store.)
 __global__ void __launch_bounds__(256) k_prep_f_b(BatchDev bd)
 {
-    __shared__ T tt[TS][TS + 1];
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     if (!M.valid) return;
@@ -1255,9 +1238,7 @@ __global__ void __launch_bounds__(256) k_prep_f_b(BatchDev bd)
     if ((int)blockIdx.x >= nb * (nb + 1) / 2) return;
     int I, J;
     tri_decode(blockIdx.x, I, J);
-    const bool mixed = sizeof(T) == 4;
-    T* F = mixed ? (T*)(bd.F32 + M.f_off) : (T*)(bd.F + M.f_off);
-    double* fdiag = bd.vec + M.vec_off + 3 * (size_t)ld;
+    double* F = bd.F + M.f_off;
     const int* pis = bd.pisrc + item * bd.s_cell;
     const double lam = bd.lambda;
     // 16 elements per thread, unrolled so that their gathers (pisrc, R tile, F) are in flight together
@@ -1267,46 +1248,28 @@ __global__ void __launch_bounds__(256) k_prep_f_b(BatchDev bd)
         const int r = I * TS + rr, c = J * TS + cc;
         if (c <= r) {
             const size_t t = (size_t)r * ld + c;
-            T v;
             if (r < n) {
                 // R is held in the reference order: F's internal indices map through pisrc
                 const int a = pis[r], b = pis[c];
                 const int hi = a > b ? a : b, lo = a > b ? b : a;
                 const double rv = ((hi >> 6) - (lo >> 6) <= M.bwb) ? *rtile(bd, M, hi, lo) : 0.0;
-                v = (T)((double)F[t] + lam * rv);
-            } else if (r == n) {
-                v = (c == n && mixed) ? (T)1e30 : F[t];
-            } else {
-                v = (r == c) ? (T)1 : (T)0;
-            }
-            F[t] = v;
-            if (mixed) {
-                tt[cc][rr] = v;
-                if (c == r && r < n) fdiag[r] = (double)v;
+                F[t] = F[t] + lam * rv;
+            } else if (r > n) {
+                F[t] = (r == c) ? 1.0 : 0.0;
             }
         }
-    }
-    if (!mixed) return;
-    __syncthreads();
-    // transposed copy into the upper triangle, written row by row of the upper tile (J, I)
-    for (int e = threadIdx.x; e < TS * TS; e += 256) {
-        const int rr2 = e >> 6, cc2 = e & 63;
-        const int c = J * TS + rr2, r = I * TS + cc2;
-        if (c < r && r <= n) F[(size_t)c * ld + r] = tt[rr2][cc2];
     }
 }
 
 void launch_prep_fr_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32)
 {
-    if (f32) {  // the mixed path's FP16 SYRK prepares F in its epilogue; the upper copy remains
-        const int nbm = max_ld / TS;
-        hipLaunchKernelGGL(k_upper_copy_b, dim3(nbm * (nbm - 1) / 2, bd.B), dim3(256), 0, s, bd);
+    const int nb = max_ld / TS;
+    if (f32) {  // the mixed path's FP16 SYRK prepares F in its epilogue; the first panel's columns remain
+        hipLaunchKernelGGL(k_strip_copy_b, dim3(4 * nb, bd.B), dim3(256), 0, s, bd);
         return;
     }
-    const int nb = max_ld / TS;
     const unsigned g = nb * (nb + 1) / 2;
-    if (f32) hipLaunchKernelGGL(k_prep_f_b<float>, dim3(g, bd.B), dim3(256), 0, s, bd);
-    else hipLaunchKernelGGL(k_prep_f_b<double>, dim3(g, bd.B), dim3(256), 0, s, bd);
+    hipLaunchKernelGGL(k_prep_f_b, dim3(g, bd.B), dim3(256), 0, s, bd);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1352,13 +1315,14 @@ struct Mfma16<float> {
     __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-// Diagonal 64x64 block, one wave: lane r keeps row r in registers.  Column step c: the pivot
+// FP64 diagonal 64x64 block, one wave (measured ~10 us per launch faster in FP64 than the
+// readlane / right-looking form below): lane r keeps row r in registers.  Column step c: the pivot
 // and the column entries l_qc go through LDS (broadcast reads); every lane scales its own entry
 // and applies the rank-1 update to all of its entries right of c without branching (entries
 // above the diagonal collect values that are never stored).  Then lane c forms column c of L^-1
 // by forward substitution reading L from LDS.
 template <typename T>
-__device__ void chol_diag_wave(const MatDesc& D, int kb, double* raw)
+__device__ void chol_diag_wave_lds(const MatDesc& D, int kb, double* raw)
 {
     T(*Ls)[TS + 1] = (T(*)[TS + 1])raw;
     T* col = (T*)raw + TS * (TS + 1);
@@ -1417,13 +1381,108 @@ __device__ void chol_diag_wave(const MatDesc& D, int kb, double* raw)
     for (int i = 0; i < TS; i++) W[i * TS + r] = x[i];
 }
 
+// value of lane `src` in every lane (src is a constant of the unrolled loops that call this)
+__device__ __forceinline__ float lane_bcast(float v, int src)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
+// FP32 diagonal 64x64 block, one wave: lane r keeps row r in registers.  Column step c: lane c's
+// pivot reaches every lane by readlane (no LDS round trip); d = |p| rsqrt|p| and the column
+// entries l_ic = a_ic rsqrt|p| are written to row c of Lt (Lt[c][i] = L(i, c), one LDS row per
+// column, so no step overwrites what another still reads); after one barrier every lane applies
+// the rank-1 update to its entries right of c with broadcast vector reads of Lt[c] (entries above
+// the diagonal collect values that are never stored).  L^-1 is then formed right-looking: lane r
+// holds column r of L^-1 and step q scales entry q by 1/L(q, q) and updates entries q+1..63 with
+// independent FMAs, so the only serial chain is one multiply per step.
+__device__ void chol_diag_wave_f32(const MatDesc& D, int kb, float (*Lt)[TS])
+{
+    const int r = threadIdx.x;
+    const long long base = (long long)kb * TS;
+    int ld;
+    typedef float T;
+    T* tile = tile_ptr<T>(D, kb, kb, ld);
+    // lane r loads row r straight into registers: all of its vector loads are in flight together
+    typedef typename Mfma16<T>::vec_t V;
+    constexpr int VW = sizeof(V) / sizeof(T);
+    T a[TS];
+#pragma unroll
+    for (int q = 0; q < TS; q += VW) {
+        const V v = *(const V*)&tile[(size_t)r * ld + q];
+        a[q] = v.x;
+        a[q + 1] = v.y;
+        if constexpr (VW == 4) {
+            a[q + 2] = v.z;
+            a[q + 3] = v.w;
+        }
+    }
+    bool bad = false;
+    T myd = 0, myrd = 0;
+#pragma unroll
+    for (int c = 0; c < TS; c++) {
+        const T piv = lane_bcast(a[c], c);
+        bad |= (piv < 0 && base + c < D.n);
+        const T pa = fabs(piv);
+        const T rd = rsqrtf(pa);
+        const T d = (pa > 0) ? pa * rd : (T)0;
+        if (r == c) {
+            myd = d;
+            myrd = rd;
+        }
+        const T l = (r > c) ? a[c] * rd : d;
+        a[c] = l;
+        Lt[c][r] = l;
+        __syncthreads();
+#pragma unroll
+        for (int q = c + 1; q < TS; q++) a[q] = fma(-l, Lt[c][q], a[q]);
+    }
+    if (r == 0 && bad) atomicOr(D.status, 1);
+    // row r of the lower triangle straight from registers
+#pragma unroll
+    for (int q = 0; q < TS; q += VW) {
+        T* dst = tile + (size_t)r * ld + q;
+        if (q + VW - 1 <= r) {
+            V v;
+            v.x = a[q];
+            v.y = a[q + 1];
+            if constexpr (VW == 4) {
+                v.z = a[q + 2];
+                v.w = a[q + 3];
+            }
+            *(V*)dst = v;
+        } else if (q <= r) {
+#pragma unroll
+            for (int j = 0; j < VW; j++)
+                if (q + j <= r) dst[j] = a[q + j];
+        }
+    }
+    ((T*)D.diag)[base + r] = myd;
+    T x[TS];
+#pragma unroll
+    for (int i = 0; i < TS; i++) x[i] = (i == r) ? (T)1 : (T)0;
+#pragma unroll
+    for (int q = 0; q < TS; q++) {
+        x[q] *= lane_bcast(myrd, q);
+#pragma unroll
+        for (int i = q + 1; i < TS; i++) x[i] = fma(-Lt[q][i], x[q], x[i]);
+    }
+    T* W = (T*)D.W + (size_t)kb * TS * TS;
+#pragma unroll
+    for (int i = 0; i < TS; i++) W[i * TS + r] = x[i];
+}
+
 template <typename T>
 __global__ void __launch_bounds__(64) k_chol_diag_m(const MatDesc* mats, int kb)
 {
-    __shared__ double raw[TS * (TS + 1) + TS];
     const MatDesc D = mats[blockIdx.y];
     if (kb >= D.nb) return;
-    chol_diag_wave<T>(D, kb, raw);
+    if constexpr (sizeof(T) == 8) {
+        __shared__ double raw[TS * (TS + 1) + TS];
+        chol_diag_wave_lds<T>(D, kb, raw);
+    } else {
+        __shared__ __align__(16) T Lt[TS][TS];
+        chol_diag_wave_f32(D, kb, Lt);
+    }
 }
 
 // 64x64x64 MFMA product acc = sum_q A(i, q) B(j, q) with A and B stored "row holds the k-vector"
@@ -1773,6 +1832,8 @@ k_chol_trail_bf16(const MatDesc* mats, int k0, int nq)
     const int wr = w >> 1, wc = w & 1;
     const int ld = D.ld;  // ld <= QLX_MAX_LD: byte offsets within the matrix fit in 31 bits
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(D.A, 0, ld * ld * 4, 0x00020000);
+    // C tiles: from F as formed (A0) in the first update, which thereby moves them into A
+    const __amdgpu_buffer_rsrc_t rc = (k0 == 0 && D.A0) ? __builtin_amdgcn_make_buffer_rsrc(D.A0, 0, ld * ld * 4, 0x00020000) : rs;
     const int rI = (r0b + 2 * I2) * TS, rJ = (r0b + 2 * J2) * TS;
     // byte offsets: the lane's part in a VGPR, the uniform part in an SGPR.  Staging step st covers
     // columns 32 st .. 32 st + 31 of the panel; piece h of a thread: row (tid >> 3) + 32 h, columns
@@ -1788,7 +1849,7 @@ k_chol_trail_bf16(const MatDesc* mats, int k0, int nq)
         for (int c = 0; c < 4; c++)
 #pragma unroll
             for (int v = 0; v < 4; v++)
-                acc[a][c][v] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, vc + c * 64, sC + (a * 16 + v) * ld * 4, 0));
+                acc[a][c][v] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rc, vc + c * 64, sC + (a * 16 + v) * ld * 4, 0));
     P pa[4], pb[4];
 #define TRAIL_LOAD_BF(st)                                                                                          \
     {                                                                                                              \
@@ -2140,7 +2201,7 @@ void launch_trisolve_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32)
 
 // ------------------------------------------------------------------------------------------------
 // FP64 iterative refinement of the mixed-precision solve (classical mixed-precision refinement,
-// Wilkinson / Moler): r = D - F s in FP64 arithmetic on the stored FP32 F (upper copy + Fdiag),
+// Wilkinson / Moler): r = D - F s in FP64 arithmetic on the FP32 F as formed (Ff32 + Fdiag),
 // dx = F^-1 r with the FP32 factor, s += dx.  The residual kernel writes per-row-block partial
 // sums of r^2 and D^2; the refine kernel stops an item once ||r|| <= tol ||D||.
 // status[4*item+2] counts steps, status[4*item+3] marks a converged item.
@@ -2164,9 +2225,10 @@ __device__ __forceinline__ double lane_sum16(const double* a)
     return e;
 }
 
-// r = D - F s for the 64 rows of block I.  F(i,c) is read from the upper triangle only: the column
-// strip (rows c < 64 I of columns i), the diagonal tile (through LDS) and the row strip (columns
-// c >= 64 (I+1) of rows i); products are formed and summed in FP64.
+// r = D - F s for the 64 rows of block I, on F as formed (Ff32, lower 128-tiles; a diagonal tile is
+// held whole): the row strip (columns c < 64 I of rows i), the diagonal tile (through LDS, its upper
+// half) and the column strip (rows c >= 64 (I+1) of columns i, F(i,c) = F(c,i)); D(i) = F(n, i).
+// Products are formed and summed in FP64.
 __global__ void __launch_bounds__(256) k_resid_b(BatchDev bd)
 {
     typedef double dvec4 __attribute__((ext_vector_type(4)));
@@ -2181,7 +2243,7 @@ __global__ void __launch_bounds__(256) k_resid_b(BatchDev bd)
     const int I = blockIdx.x;
     if (I >= nb || I * TS >= n) return;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const float* U = bd.F32 + M.f_off;
+    const float* U = bd.Ff32 + M.f_off;
     double* v = bd.vec + M.vec_off;
     const double* s = v;
     const double* fdiag = v + 3 * (size_t)ld;
@@ -2198,14 +2260,14 @@ __global__ void __launch_bounds__(256) k_resid_b(BatchDev bd)
         tile[row][cc + 2] = t.z;
         tile[row][cc + 3] = t.w;
     }
-    // column strip: F(i,c) = U[c][i] for c < c0; 16 lanes per row, 16 rows per pass
+    // column strip: F(i,c) = U[c][i] for c0 + 64 <= c < n; 16 lanes per row, 16 rows per pass
     {
         const int cv = 4 * (lane & 15);
         const int rs = 4 * w + (lane >> 4);
         dvec4 acc = (dvec4)0.0;
         const float* Uc = U + c0 + cv;
-        int c = rs;
-        for (; c + 7 * 16 < c0; c += 8 * 16) {
+        int c = c0 + TS + rs;
+        for (; c + 7 * 16 < n; c += 8 * 16) {
             float4_t t[8];
 #pragma unroll
             for (int u = 0; u < 8; u++) t[u] = *(const float4_t*)(Uc + (size_t)(c + 16 * u) * ld);
@@ -2218,7 +2280,7 @@ __global__ void __launch_bounds__(256) k_resid_b(BatchDev bd)
                 acc.w = fma((double)t[u].w, sc, acc.w);
             }
         }
-        for (; c < c0; c += 16) {
+        for (; c < n; c += 16) {
             const float4_t t = *(const float4_t*)(Uc + (size_t)c * ld);
             const double sc = s[c];
             acc.x = fma((double)t.x, sc, acc.x);
@@ -2231,15 +2293,15 @@ __global__ void __launch_bounds__(256) k_resid_b(BatchDev bd)
         cacc[rs][cv + 2] = acc.z;
         cacc[rs][cv + 3] = acc.w;
     }
-    // row strip: F(i,c) = U[i][c] for c0 + 64 <= c < n; wave w owns rows c0 + w + 4k, k = 0..15
+    // row strip: F(i,c) = U[i][c] for c < c0 (c0 is a multiple of 64, so c + 3 < c0 <= n); wave w
+    // owns rows c0 + w + 4k, k = 0..15
     {
         double ra[16];
 #pragma unroll
         for (int k = 0; k < 16; k++) ra[k] = 0.0;
         const float* Ur = U + (size_t)(c0 + w) * ld;
-        for (int c = c0 + TS + 4 * lane; c < n; c += 256) {
-            const double s0 = s[c], s1 = (c + 1 < n) ? s[c + 1] : 0.0;
-            const double s2 = (c + 2 < n) ? s[c + 2] : 0.0, s3 = (c + 3 < n) ? s[c + 3] : 0.0;
+        for (int c = 4 * lane; c < c0; c += 256) {
+            const double s0 = s[c], s1 = s[c + 1], s2 = s[c + 2], s3 = s[c + 3];
             float4_t t[16];
 #pragma unroll
             for (int k = 0; k < 16; k++) t[k] = *(const float4_t*)(Ur + (size_t)(4 * k) * ld + c);
@@ -2271,7 +2333,7 @@ __global__ void __launch_bounds__(256) k_resid_b(BatchDev bd)
 #pragma unroll
             for (int q = 0; q < 16; q++) cs += cacc[q][tid];
             const double ds = (dacc[0][tid] + dacc[1][tid]) + (dacc[2][tid] + dacc[3][tid]);
-            const double D = (double)U[(size_t)i * ld + n];
+            const double D = (double)U[(size_t)n * ld + i];
             const double Fs = fdiag[i] * s[i] + cs + ds + racc[tid];
             rr = D - Fs;
             r[i] = rr;

@@ -89,7 +89,7 @@ struct Ctx {
     DevBuf apre, lptr, eptr, lcj, lv, lvf, nfwtab, pmask, hmask;
     DevBuf key, bcnt, boff, bfill, perm;
     // pools
-    DevBuf lb, F, R, Rf, cvec, vec, nz, bbase, winv, model, out, status;  // Rf: R's copy for its factorization
+    DevBuf lb, F, Ff, R, Rf, cvec, vec, nz, bbase, winv, model, out, status;  // Rf: R's copy for its factorization
     // last solution
     std::vector<double> last_src, last_model;
     // timers
@@ -269,12 +269,12 @@ size_t stage1_bytes(size_t s_pix, size_t s_cell, size_t nsub)
 }
 
 // upper bound of the dense bytes of one item with at most ncell active source cells
-size_t dense_bytes_bound(size_t ncell, size_t snx, bool mixed)
+size_t dense_bytes_bound(size_t ncell, size_t snx)
 {
     const size_t ldN = round_up(ncell + 1, 128), nb = ldN / 64;
     const size_t bwb = (2 * snx) / 64 + 1;
-    const size_t f = ldN * ldN * (mixed ? 4 : 8);
-    const size_t r = nb * (bwb + 1) * 4096 * 8;
+    const size_t f = ldN * ldN * 8;                     // FP64 F, or FP32 F and its formed copy
+    const size_t r = 2 * nb * (bwb + 1) * 4096 * 8;     // R and the copy that is factored
     return f + r + ldN * 8 * (QLX_VEC_SLOTS + 128);
 }
 
@@ -488,10 +488,12 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         return QLX_OK;
     }
     ok = c->R.ensure(r_tot * 8) && c->Rf.ensure(r_tot * 8) && c->cvec.ensure(cv_tot * 8) && c->vec.ensure(vec_tot * 8) && c->nz.ensure(nz_tot * 4) &&
-         c->bbase.ensure(bb_tot * 4) && c->winv.ensure(w_tot * 8) && c->F.ensure(f_tot * (mixed ? 4 : 8));
+         c->bbase.ensure(bb_tot * 4) && c->winv.ensure(w_tot * 8) && c->F.ensure(f_tot * (mixed ? 4 : 8)) &&
+         (!mixed || c->Ff.ensure(f_tot * 4));
     if (!ok) return fail(c, QLX_ERR_NOMEM, "dense buffer allocation failed (F/R)");
     bd.F = mixed ? nullptr : c->F.as<double>();
     bd.F32 = mixed ? c->F.as<float>() : nullptr;
+    bd.Ff32 = mixed ? c->Ff.as<float>() : nullptr;
     bd.R = c->R.as<double>();
     bd.cvec = c->cvec.as<double>();
     bd.vec = c->vec.as<double>();
@@ -516,6 +518,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         f.diag = (void*)(v + M.ldN);
         f.status = bd.status + 4 * b;
         f.W = (void*)(bd.winv + M.w_off);
+        f.A0 = mixed ? (void*)(bd.Ff32 + M.f_off) : nullptr;
         fm.push_back(f);
         MatDesc r{};
         r.A = (void*)(c->Rf.as<double>() + M.r_off);
@@ -560,21 +563,23 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     // R depends only on the active cells: it is built now and factored (ln det R only) on the second
     // stream, from a copy, while the main stream forms F (the SYRK, then k_prep_f_b, which reads R
     // itself); the streams join before the log-determinants.  (Started before the Lb build instead,
-    // its small launches slow that latency-bound kernel by more than they save.)
+    // its small launches slow that latency-bound kernel by more than they save.)  R's many small
+    // launches are enqueued after the SYRK's: the host's time spent enqueueing them then overlaps
+    // the SYRK on the GPU instead of leaving the main stream idle.
     launch_curvature_b(st, bd, max_cells, max_rtiles);
     HIPCHK(c, hipEventRecord(c->ev_fork, st));
-    HIPCHK(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
-    HIPCHK(c, hipMemcpyAsync(c->Rf.p, c->R.p, r_tot * 8, hipMemcpyDeviceToDevice, c->stream2));
-    int max_bwb = 1;
-    for (const MatDesc& r : rm) max_bwb = std::max(max_bwb, r.bwb);
-    cholesky_batched(c->stream2, c->d_mats.as<MatDesc>() + nF, 0, nR, max_ld / 64, 0, max_bwb);
-    HIPCHK(c, hipEventRecord(c->ev_join, c->stream2));
     if (!mixed) launch_gather_cvec_b(st, bd, max_rows);
     timer_begin(c, "syrk");
     if (mixed) launch_syrk_f16_b(st, bd, max_ncb);
     else launch_syrk_f64_b(st, bd, max_ncb);
     timer_end(c, "syrk");
     launch_prep_fr_b(st, bd, max_ld, mixed ? 1 : 0);
+    HIPCHK(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    HIPCHK(c, hipMemcpyAsync(c->Rf.p, c->R.p, r_tot * 8, hipMemcpyDeviceToDevice, c->stream2));
+    int max_bwb = 1;
+    for (const MatDesc& r : rm) max_bwb = std::max(max_bwb, r.bwb);
+    cholesky_batched(c->stream2, c->d_mats.as<MatDesc>() + nF, 0, nR, max_ld / 64, 0, max_bwb);
+    HIPCHK(c, hipEventRecord(c->ev_join, c->stream2));
 
     timer_begin(c, "cholesky");
     cholesky_batched(st, c->d_mats.as<MatDesc>(), mixed ? nF : 0, mixed ? 0 : nF, max_ld / 64, 1, 0);
@@ -707,7 +712,7 @@ int eval_items(Ctx* c, int nbatch, const qlx_lens_model* models, const int* src_
     const size_t ncell_bound = std::min<size_t>(max_cell, QLX_MAX_LD - 1);
     const size_t ldN = round_up(ncell_bound + 1, 64);
     const size_t lb_est = round_up(std::max<size_t>(s_pix, 64), 64) * std::min<size_t>(ldN, 32 * 64) * (mixed ? 2 : 8);
-    const size_t per_item = stage1_bytes(s_pix, max_cell, nsub) + dense_bytes_bound(ncell_bound, max_snx, mixed) + lb_est;
+    const size_t per_item = stage1_bytes(s_pix, max_cell, nsub) + dense_bytes_bound(ncell_bound, max_snx) + lb_est;
     size_t budget = c->hbm_budget;
     if (budget == 0) {
         size_t fr = 0, tot = 0;

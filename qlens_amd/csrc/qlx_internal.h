@@ -123,7 +123,9 @@ struct BatchDev {
     const ItemMeta* meta;
     void* lb;                   // blurred L support blocks (double, or bf16 hi/lo when mixed)
     double* F;                  // FP64 F (parity mode)
-    float* F32;                 // FP32 F (mixed mode)
+    float* F32;                 // FP32 F (mixed mode), factored in place
+    float* Ff32;                // FP32 F as formed by the mixed SYRK (lower 128-tiles), kept for the
+                                // refinement residual
     double* R;                  // banded R tiles
     double* cvec;
     double* vec;                // QLX_VEC_SLOTS * ldN doubles per item
@@ -144,6 +146,8 @@ struct MatDesc {
     void* diag;                 // nb*64 pivots (element type of A)
     int* status;                // bit 0 set on a negative pivot of a row < n
     void* W;                    // nb block inverses (64x64 each, element type of A)
+    void* A0;                   // dense FP32: F as formed; the first trailing update reads its C
+                                // tiles here and writes A (null: in place)
 };
 
 void upload_patterson(int device);
