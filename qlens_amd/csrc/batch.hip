@@ -1501,14 +1501,10 @@ __device__ __forceinline__ void stage64(const T* P, int sp, T (*S)[LDT])
 }
 
 template <typename T>
-__device__ __forceinline__ void mma64(T (*As)[LDT], T (*Bs)[LDT], typename Mfma16<T>::acc_t acc[2][2])
+__device__ __forceinline__ void mma64_add(T (*As)[LDT], T (*Bs)[LDT], typename Mfma16<T>::acc_t acc[2][2])
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int wr = w >> 1, wc = w & 1;
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int c = 0; c < 2; c++) acc[a][c] = (typename Mfma16<T>::acc_t){0, 0, 0, 0};
 #pragma unroll 4
     for (int kk = 0; kk < TS; kk += 4) {
         const int kq = kk + (lane >> 4);
@@ -1521,6 +1517,16 @@ __device__ __forceinline__ void mma64(T (*As)[LDT], T (*Bs)[LDT], typename Mfma1
         acc[1][0] = Mfma16<T>::op(a1, b0, acc[1][0]);
         acc[1][1] = Mfma16<T>::op(a1, b1, acc[1][1]);
     }
+}
+
+template <typename T>
+__device__ __forceinline__ void mma64(T (*As)[LDT], T (*Bs)[LDT], typename Mfma16<T>::acc_t acc[2][2])
+{
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) acc[a][c] = (typename Mfma16<T>::acc_t){0, 0, 0, 0};
+    mma64_add<T>(As, Bs, acc);
 }
 
 // C (64x64 at row pitch ldc) = acc (mode 0), C -= acc (mode 1), lower triangle only (mode 2)
@@ -1667,6 +1673,70 @@ __global__ void __launch_bounds__(256) k_chol_upanel_m(const MatDesc* mats, int 
     typename Mfma16<T>::acc_t acc[2][2];
     mma64<T>(As, Bs, acc);
     store64c<T>(C, ldc, acc, cv, I == J ? 2 : 1);
+}
+
+// Left-looking step of a 256-wide panel (columns k0 .. kend-1, dense matrices): column block kb.
+// Workgroup x takes row block ib = kb + 1 + x and forms
+//   A_ib,kb - sum_{k0 <= k' < kb} L_ib,k' L_kb,k'^T  (the earlier columns of the panel, summed in
+//   the MFMA accumulators while the block waits in registers),  L_ib,kb = that times W_kb^T,
+// so each block is read and written once per panel column.  The workgroup of ib = kb + 1 then
+// brings the next diagonal block up to date for the diagonal kernel, when it lies in the panel:
+//   A_ib,ib -= sum_{k0 <= k' <= kb} L_ib,k' L_ib,k'^T  (lower half stored).
+template <typename T>
+__global__ void __launch_bounds__(256) k_chol_lpanel_m(const MatDesc* mats, int kb, int k0, int kend)
+{
+    __shared__ T As[TS][LDT];
+    __shared__ T Bs[TS][LDT];
+    typedef typename Mfma16<T>::acc_t Acc;
+    const MatDesc D = mats[blockIdx.y];
+    if (kb >= D.nb) return;
+    const int ib = kb + 1 + blockIdx.x;
+    if (ib > band_last(D, kb)) return;
+    int ld, lp;
+    T* blk = tile_ptr<T>(D, ib, kb, ld);
+    T cv[16];
+    load64c<T>(blk, ld, cv);
+    Acc acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) acc[a][c] = (Acc){0, 0, 0, 0};
+    for (int kp = k0; kp < kb; kp++) {
+        const T* Pa = tile_ptr<T>(D, ib, kp, lp);
+        const T* Pb = tile_ptr<T>(D, kb, kp, lp);
+        __syncthreads();
+        stage64<T>(Pa, lp, As);
+        stage64<T>(Pb, lp, Bs);
+        __syncthreads();
+        mma64_add<T>(As, Bs, acc);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc[a][c][v] = cv[(a * 2 + c) * 4 + v] - acc[a][c][v];
+    __syncthreads();
+    store64<T>(&As[0][0], LDT, acc, 0);
+    stage64<T>((const T*)D.W + (size_t)kb * TS * TS, TS, Bs);
+    __syncthreads();
+    mma64<T>(As, Bs, acc);  // L_ib,kb(i, j) = sum_q A'(i, q) W_kb(j, q)
+    store64<T>(blk, ld, acc, 0);
+    if (ib != kb + 1 || ib >= kend) return;
+    T* dg = tile_ptr<T>(D, ib, ib, ld);
+    load64c<T>(dg, ld, cv);
+    __syncthreads();
+    store64<T>(&As[0][0], LDT, acc, 0);
+    __syncthreads();
+    mma64<T>(As, As, acc);
+    for (int kp = k0; kp < kb; kp++) {
+        const T* Pa = tile_ptr<T>(D, ib, kp, lp);
+        __syncthreads();
+        stage64<T>(Pa, lp, As);
+        __syncthreads();
+        mma64_add<T>(As, As, acc);
+    }
+    store64c<T>(dg, ld, acc, cv, 2);
 }
 
 // acc += As(rows of quadrant wr) * Bs(rows of quadrant wc)^T over the 64 staged columns.  Lane group
@@ -1927,8 +1997,14 @@ static void chol_step(hipStream_t s, const MatDesc* mats, int nmat, int kb, int 
     }
 }
 
-// dense matrices (padded to a multiple of 128): 256-wide panels factored with the 64-block
-// kernels and a panel-restricted update, then one rank-256 trailing update per panel
+// dense matrices (padded to a multiple of 128): 256-wide panels factored column block by column
+// block, then one rank-256 trailing update per panel.  A column block is the diagonal kernel, then
+// for FP32 (the mixed path, evaluated in batches) the left-looking panel kernel, which reads and
+// writes each block once per column (0.8 ms faster per batch of 64 3600 x 3600 matrices than the
+// right-looking pair), and for FP64 the right-looking panel and panel-update kernels, whose three
+// times more workgroups fill the GPU better when one matrix is factored alone (0.25 ms faster for
+// one 3600 x 3600 matrix).  The form depends on the element type only, never on the batch, so an
+// item's arithmetic is the same alone and in any batch.
 template <typename T>
 static void chol_two_level(hipStream_t s, const MatDesc* mats, int nmat, int max_nb)
 {
@@ -1936,12 +2012,17 @@ static void chol_two_level(hipStream_t s, const MatDesc* mats, int nmat, int max
     const size_t lds_bf = (size_t)4 * TT * TBP * sizeof(uint16_t);
     allow_big_lds((const void*)k_chol_trail_m<T>);
     allow_big_lds((const void*)k_chol_trail_bf16);
+    const bool left = sizeof(T) == 4;
     for (int k0 = 0; k0 < max_nb; k0 += 4) {
         const int kend = k0 + 4;
         for (int kb = k0; kb < kend && kb < max_nb; kb++) {
             const int rest = max_nb - kb - 1;
             hipLaunchKernelGGL(k_chol_diag_m<T>, dim3(1, nmat), dim3(64), 0, s, mats, kb);
             if (rest <= 0) continue;
+            if (left) {
+                hipLaunchKernelGGL(k_chol_lpanel_m<T>, dim3(rest, nmat), dim3(256), 0, s, mats, kb, k0, kend);
+                continue;
+            }
             hipLaunchKernelGGL(k_chol_panel_m<T>, dim3(rest, nmat), dim3(256), 0, s, mats, kb);
             int cnt = 0;
             for (int j = kb + 1; j < kend && j < max_nb; j++) cnt += max_nb - j;
