@@ -1878,12 +1878,14 @@ k_chol_trail_m(const MatDesc* mats, int k0, int nq)
 // resource (32-bit offsets: one VGPR per lane, row offsets in SGPRs).  Operands are staged 32 k at
 // a time (LDS: A hi, A lo, B hi, B lo as 128 rows x TBP BF16, 40 KB; 80-byte rows put the 16 rows
 // of an operand read on distinct bank groups), which with 150 registers allows three waves per SIMD.
+// The update covers the 128 x 128 tiles of rows and columns >= k0 + nq (64-blocks) from the nq panel
+// blocks k0 .. k0+nq-1; ncolt > 0 restricts it to the first ncolt tile columns (a strip).
 #define TBP 40
 static_assert((long long)QLX_MAX_LD * QLX_MAX_LD * 4 < (1LL << 31), "trailing-update byte offsets must fit in 31 bits");
 typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
-k_chol_trail_bf16(const MatDesc* mats, int k0, int nq)
+k_chol_trail_bf16(const MatDesc* mats, int k0, int nq, int ncolt)
 {
     typedef float P __attribute__((ext_vector_type(4)));
     typedef __attribute__((address_space(1))) float gfloat;
@@ -1896,8 +1898,21 @@ k_chol_trail_bf16(const MatDesc* mats, int k0, int nq)
     const int r0b = k0 + nq;
     if (r0b >= D.nb) return;
     const int nt = (D.nb - r0b) / 2;
-    int I2, J2;
-    if (!syrk_tile(nt, I2, J2)) return;
+    int I2 = -1, J2 = -1;
+    if (ncolt == 0) {
+        if (!syrk_tile(nt, I2, J2)) return;
+    } else {
+        int t = blockIdx.x;
+        for (int j = 0; j < ncolt && j < nt; j++) {
+            if (t < nt - j) {
+                J2 = j;
+                I2 = j + t;
+                break;
+            }
+            t -= nt - j;
+        }
+        if (J2 < 0) return;
+    }
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wr = w >> 1, wc = w & 1;
     const int ld = D.ld;  // ld <= QLX_MAX_LD: byte offsets within the matrix fit in 31 bits
@@ -1997,43 +2012,61 @@ static void chol_step(hipStream_t s, const MatDesc* mats, int nmat, int kb, int 
     }
 }
 
-// dense matrices (padded to a multiple of 128): 256-wide panels factored column block by column
-// block, then one rank-256 trailing update per panel.  A column block is the diagonal kernel, then
-// for FP32 (the mixed path, evaluated in batches) the left-looking panel kernel, which reads and
-// writes each block once per column (0.8 ms faster per batch of 64 3600 x 3600 matrices than the
-// right-looking pair), and for FP64 the right-looking panel and panel-update kernels, whose three
-// times more workgroups fill the GPU better when one matrix is factored alone (0.25 ms faster for
-// one 3600 x 3600 matrix).  The form depends on the element type only, never on the batch, so an
+// one column block after another of the panel [k0, kend): the diagonal kernel, then the panel kernel
+template <typename T>
+static void chol_subpanel(hipStream_t s, const MatDesc* mats, int nmat, int max_nb, int k0, int kend, bool left)
+{
+    for (int kb = k0; kb < kend && kb < max_nb; kb++) {
+        const int rest = max_nb - kb - 1;
+        hipLaunchKernelGGL(k_chol_diag_m<T>, dim3(1, nmat), dim3(64), 0, s, mats, kb);
+        if (rest <= 0) continue;
+        if (left) {
+            hipLaunchKernelGGL(k_chol_lpanel_m<T>, dim3(rest, nmat), dim3(256), 0, s, mats, kb, k0, kend);
+            continue;
+        }
+        hipLaunchKernelGGL(k_chol_panel_m<T>, dim3(rest, nmat), dim3(256), 0, s, mats, kb);
+        int cnt = 0;
+        for (int j = kb + 1; j < kend && j < max_nb; j++) cnt += max_nb - j;
+        if (cnt > 0) hipLaunchKernelGGL(k_chol_upanel_m<T>, dim3(cnt, nmat), dim3(256), 0, s, mats, kb, kend);
+    }
+}
+
+// dense matrices (padded to a multiple of 128), factored panel by panel, each panel column block
+// by column block and then one trailing update of the rest.  FP32 (the mixed path, evaluated in
+// batches): 512-wide panels of two 256-wide left-looking halves (the left-looking panel kernel
+// reads and writes each block once per column: 0.8 ms faster per batch of 64 3600 x 3600
+// matrices than the right-looking pair), the second half brought up to date by a rank-256 strip
+// update, then one rank-512 BF16-split trailing update, which reads and writes the trailing
+// matrix half as often as rank-256 updates would.  FP64: 256-wide panels factored with the
+// right-looking panel and panel-update kernels, whose three times more workgroups fill the GPU
+// better when one matrix is factored alone (0.25 ms faster for one 3600 x 3600 matrix), and the
+// FP64 trailing update.  The schedule depends on the element type only, never on the batch, so an
 // item's arithmetic is the same alone and in any batch.
 template <typename T>
 static void chol_two_level(hipStream_t s, const MatDesc* mats, int nmat, int max_nb)
 {
-    const size_t lds = (size_t)2 * TT * LDT * sizeof(T);
-    const size_t lds_bf = (size_t)4 * TT * TBP * sizeof(uint16_t);
-    allow_big_lds((const void*)k_chol_trail_m<T>);
-    allow_big_lds((const void*)k_chol_trail_bf16);
-    const bool left = sizeof(T) == 4;
-    for (int k0 = 0; k0 < max_nb; k0 += 4) {
-        const int kend = k0 + 4;
-        for (int kb = k0; kb < kend && kb < max_nb; kb++) {
-            const int rest = max_nb - kb - 1;
-            hipLaunchKernelGGL(k_chol_diag_m<T>, dim3(1, nmat), dim3(64), 0, s, mats, kb);
-            if (rest <= 0) continue;
-            if (left) {
-                hipLaunchKernelGGL(k_chol_lpanel_m<T>, dim3(rest, nmat), dim3(256), 0, s, mats, kb, k0, kend);
-                continue;
-            }
-            hipLaunchKernelGGL(k_chol_panel_m<T>, dim3(rest, nmat), dim3(256), 0, s, mats, kb);
-            int cnt = 0;
-            for (int j = kb + 1; j < kend && j < max_nb; j++) cnt += max_nb - j;
-            if (cnt > 0) hipLaunchKernelGGL(k_chol_upanel_m<T>, dim3(cnt, nmat), dim3(256), 0, s, mats, kb, kend);
+    if (sizeof(T) == 4) {
+        const size_t lds_bf = (size_t)4 * TT * TBP * sizeof(uint16_t);
+        allow_big_lds((const void*)k_chol_trail_bf16);
+        for (int k0 = 0; k0 < max_nb; k0 += 8) {
+            chol_subpanel<T>(s, mats, nmat, max_nb, k0, k0 + 4, true);
+            if (k0 + 4 >= max_nb) break;
+            const int nts = (max_nb - k0 - 4) / 2;
+            hipLaunchKernelGGL(k_chol_trail_bf16, dim3(2 * nts - 1, nmat), dim3(256), lds_bf, s, mats, k0, 4, 2);
+            chol_subpanel<T>(s, mats, nmat, max_nb, k0 + 4, k0 + 8, true);
+            if (k0 + 8 >= max_nb) break;
+            const int nt = (max_nb - k0 - 8) / 2;
+            hipLaunchKernelGGL(k_chol_trail_bf16, dim3(syrk_grid(nt), nmat), dim3(256), lds_bf, s, mats, k0, 8, 0);
         }
-        if (kend < max_nb) {
-            const int nt = (max_nb - kend) / 2;
-            if (sizeof(T) == 4)
-                hipLaunchKernelGGL(k_chol_trail_bf16, dim3(syrk_grid(nt), nmat), dim3(256), lds_bf, s, mats, k0, 4);
-            else
-                hipLaunchKernelGGL(k_chol_trail_m<T>, dim3(syrk_grid(nt), nmat), dim3(256), lds, s, mats, k0, 4);
+        return;
+    }
+    const size_t lds = (size_t)2 * TT * LDT * sizeof(T);
+    allow_big_lds((const void*)k_chol_trail_m<T>);
+    for (int k0 = 0; k0 < max_nb; k0 += 4) {
+        chol_subpanel<T>(s, mats, nmat, max_nb, k0, k0 + 4, false);
+        if (k0 + 4 < max_nb) {
+            const int nt = (max_nb - k0 - 4) / 2;
+            hipLaunchKernelGGL(k_chol_trail_m<T>, dim3(syrk_grid(nt), nmat), dim3(256), lds, s, mats, k0, 4);
         }
     }
 }
