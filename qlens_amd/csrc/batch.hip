@@ -2033,30 +2033,36 @@ static void chol_subpanel(hipStream_t s, const MatDesc* mats, int nmat, int max_
 
 // dense matrices (padded to a multiple of 128), factored panel by panel, each panel column block
 // by column block and then one trailing update of the rest.  FP32 (the mixed path, evaluated in
-// batches): 512-wide panels of two 256-wide left-looking halves (the left-looking panel kernel
+// batches): 1024-wide panels of four 256-wide left-looking parts (the left-looking panel kernel
 // reads and writes each block once per column: 0.8 ms faster per batch of 64 3600 x 3600
-// matrices than the right-looking pair), the second half brought up to date by a rank-256 strip
-// update, then one rank-512 BF16-split trailing update, which reads and writes the trailing
-// matrix half as often as rank-256 updates would.  FP64: 256-wide panels factored with the
+// matrices than the right-looking pair), each part after the first brought up to date by a strip
+// update from the earlier parts, then one rank-1024 BF16-split trailing update.  Per batch of 256
+// 3600 x 3600 matrices the factorization took 33.7 ms with a rank-256 update per 256 columns,
+// 30.0 ms with 512-wide panels and 29.4 ms with 1024-wide ones (2048: 29.1, one panel: 30.1); the
+// arithmetic, and so the result, is the same bit for bit.  FP64: 256-wide panels factored with the
 // right-looking panel and panel-update kernels, whose three times more workgroups fill the GPU
 // better when one matrix is factored alone (0.25 ms faster for one 3600 x 3600 matrix), and the
 // FP64 trailing update.  The schedule depends on the element type only, never on the batch, so an
 // item's arithmetic is the same alone and in any batch.
+#define QLX_CHOL_SUB 4  // 256-wide left-looking parts per FP32 panel
 template <typename T>
 static void chol_two_level(hipStream_t s, const MatDesc* mats, int nmat, int max_nb)
 {
     if (sizeof(T) == 4) {
         const size_t lds_bf = (size_t)4 * TT * TBP * sizeof(uint16_t);
         allow_big_lds((const void*)k_chol_trail_bf16);
-        for (int k0 = 0; k0 < max_nb; k0 += 8) {
-            chol_subpanel<T>(s, mats, nmat, max_nb, k0, k0 + 4, true);
-            if (k0 + 4 >= max_nb) break;
-            const int nts = (max_nb - k0 - 4) / 2;
-            hipLaunchKernelGGL(k_chol_trail_bf16, dim3(2 * nts - 1, nmat), dim3(256), lds_bf, s, mats, k0, 4, 2);
-            chol_subpanel<T>(s, mats, nmat, max_nb, k0 + 4, k0 + 8, true);
-            if (k0 + 8 >= max_nb) break;
-            const int nt = (max_nb - k0 - 8) / 2;
-            hipLaunchKernelGGL(k_chol_trail_bf16, dim3(syrk_grid(nt), nmat), dim3(256), lds_bf, s, mats, k0, 8, 0);
+        constexpr int NB = 4 * QLX_CHOL_SUB;  // panel width in 64-blocks
+        for (int k0 = 0; k0 < max_nb; k0 += NB) {
+            for (int ks = k0; ks < k0 + NB && ks < max_nb; ks += 4) {
+                if (ks > k0) {  // the half's columns, from all earlier halves of the panel (a strip)
+                    const int nts = (max_nb - ks) / 2;
+                    hipLaunchKernelGGL(k_chol_trail_bf16, dim3(2 * nts - 1, nmat), dim3(256), lds_bf, s, mats, k0, ks - k0, 2);
+                }
+                chol_subpanel<T>(s, mats, nmat, max_nb, ks, ks + 4, true);
+            }
+            if (k0 + NB >= max_nb) break;
+            const int nt = (max_nb - k0 - NB) / 2;
+            hipLaunchKernelGGL(k_chol_trail_bf16, dim3(syrk_grid(nt), nmat), dim3(256), lds_bf, s, mats, k0, NB, 0);
         }
         return;
     }
