@@ -44,3 +44,21 @@ def test_no_cpu_fallback_in_product():
                 txt = open(os.path.join(root, f)).read()
                 assert "oracle" not in re.sub(r"#.*", "", txt).replace('"""', "").split("import")[0] or True
                 assert "from oracle" not in txt and "import oracle" not in txt, f
+
+
+def test_batch_packing_matches_field_by_field():
+    # make_models / the numpy result view give the same bytes and values as the per-field ctypes
+    # forms, for uniform and ragged lens lists (SHEAR has four parameters, the others six)
+    import numpy as np
+    from qlens_amd import qlx, workloads as W
+    props = W.proposals(W.c2(), 16)
+    models = props + [[props[0][0]], props[1][:2], []]
+    ref = (qlx.QlxLensModel * len(models))()
+    for k, lm in enumerate(models):
+        ref[k] = qlx.make_model(lm, qlx.QLX_DEF_SURROGATE)
+    assert bytes(qlx.make_models(models, qlx.QLX_DEF_SURROGATE)) == bytes(ref)
+    res = (qlx.QlxResult * 3)()
+    res[1].minus2lnE, res[1].status, res[2].refine_steps = -3.25, -7, 4
+    rows = np.frombuffer(res, dtype=qlx._RESULT_DT).tolist()
+    got = [dict(zip(qlx._RESULT_DT.names, r)) for r in rows]
+    assert got == [{f: getattr(res[k], f) for f, _ in qlx.QlxResult._fields_} for k in range(3)]
