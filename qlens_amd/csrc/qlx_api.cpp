@@ -91,7 +91,12 @@ struct Ctx {
     // pools
     DevBuf lb, F, Ff, R, Rf, cvec, vec, nz, bbase, winv, model, out, status;  // Rf: R's copy for its factorization
     // last solution
-    std::vector<double> last_src, last_model;
+    // the last valid item's solution, kept on the device (copied there at the end of a call, read
+    // back only by qlx_get_solution): source amplitudes, the cell map and the model image
+    DevBuf keep;
+    int keep_nsrc = 0, keep_ncell = 0;
+    size_t keep_npix = 0;
+    std::vector<unsigned> nzh;  // host copy of the support masks (work accounting)
     // timers
     std::map<std::string, Timer> timers;
     qlx_stats stats{};
@@ -542,6 +547,8 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     launch_sort_rows_b(st, bd, max_img, max_src);
     launch_row_support_b(st, bd, max_nkt);
     HIPCHK(c, hipMemcpyAsync(counts.data(), bd.counts, (size_t)B * 16, hipMemcpyDeviceToHost, st));
+    c->nzh.resize(nz_tot);
+    HIPCHK(c, hipMemcpyAsync(c->nzh.data(), bd.nz, nz_tot * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipStreamSynchronize(st));
     size_t lb_tot = 0;
     int max_nsup = 0;
@@ -621,8 +628,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
 
     // ---- work accounting over the wave (the launches timed above) ----
     {
-        std::vector<unsigned> nzh(nz_tot);
-        HIPCHK(c, hipMemcpy(nzh.data(), bd.nz, nz_tot * 4, hipMemcpyDeviceToHost));
+        const std::vector<unsigned>& nzh = c->nzh;  // read back with the support counts
         qlx_stats& T = c->stats;
         memset(&T, 0, sizeof(T));
         for (int b = 0; b < B; b++) {
@@ -661,18 +667,19 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
             }
         if (lb >= 0) {
             const ItemMeta& M = meta[lb];
-            std::vector<double> s(M.n_src);
-            std::vector<int> sidx((size_t)M.snx * M.sny);
-            c->last_model.resize(s_pix);
-            HIPCHK(c, hipMemcpyAsync(s.data(), bd.vec + M.vec_off + 6 * (size_t)M.ldN, sizeof(double) * M.n_src,
-                                     hipMemcpyDeviceToHost, st));
-            HIPCHK(c, hipMemcpyAsync(sidx.data(), bd.sidx + (size_t)lb * s_cell, sizeof(int) * sidx.size(), hipMemcpyDeviceToHost, st));
-            HIPCHK(c, hipMemcpyAsync(c->last_model.data(), bd.model + (size_t)lb * 2 * s_pix, sizeof(double) * s_pix,
-                                     hipMemcpyDeviceToHost, st));
-            HIPCHK(c, hipStreamSynchronize(st));
-            c->last_src.assign(sidx.size(), 0.0);
-            for (size_t k = 0; k < sidx.size(); k++)
-                if (sidx[k] >= 0) c->last_src[k] = s[sidx[k]];
+            const int ncell = M.snx * M.sny;
+            const size_t bytes = sizeof(double) * ((size_t)M.n_src + s_pix) + sizeof(int) * (size_t)ncell;
+            if (!c->keep.ensure(bytes)) return fail(c, QLX_ERR_NOMEM, "solution buffer allocation failed");
+            char* k = (char*)c->keep.p;
+            HIPCHK(c, hipMemcpyAsync(k, bd.vec + M.vec_off + 6 * (size_t)M.ldN, sizeof(double) * M.n_src,
+                                     hipMemcpyDeviceToDevice, st));
+            HIPCHK(c, hipMemcpyAsync(k + sizeof(double) * M.n_src, bd.model + (size_t)lb * 2 * s_pix, sizeof(double) * s_pix,
+                                     hipMemcpyDeviceToDevice, st));
+            HIPCHK(c, hipMemcpyAsync(k + sizeof(double) * ((size_t)M.n_src + s_pix), bd.sidx + (size_t)lb * s_cell,
+                                     sizeof(int) * ncell, hipMemcpyDeviceToDevice, st));
+            c->keep_nsrc = M.n_src;
+            c->keep_ncell = ncell;
+            c->keep_npix = s_pix;
         }
     }
     return QLX_OK;
@@ -920,9 +927,18 @@ int qlx_get_solution(void* ctx, double* src_cells, double* model_image)
 {
     Ctx* c = (Ctx*)ctx;
     if (!c) return QLX_ERR_ARG;
-    if (c->last_src.empty()) return fail(c, QLX_ERR_STATE, "no evaluation yet");
-    if (src_cells) memcpy(src_cells, c->last_src.data(), sizeof(double) * c->last_src.size());
-    if (model_image) memcpy(model_image, c->last_model.data(), sizeof(double) * c->last_model.size());
+    if (c->keep_ncell == 0) return fail(c, QLX_ERR_STATE, "no evaluation yet");
+    std::vector<double> s(c->keep_nsrc), model(c->keep_npix);
+    std::vector<int> sidx(c->keep_ncell);
+    const char* k = (const char*)c->keep.p;
+    HIPCHK(c, hipMemcpyAsync(s.data(), k, sizeof(double) * s.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(model.data(), k + sizeof(double) * s.size(), sizeof(double) * model.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(sidx.data(), k + sizeof(double) * (s.size() + model.size()), sizeof(int) * sidx.size(),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (src_cells)
+        for (size_t q = 0; q < sidx.size(); q++) src_cells[q] = sidx[q] >= 0 ? s[sidx[q]] : 0.0;
+    if (model_image) memcpy(model_image, model.data(), sizeof(double) * model.size());
     return QLX_OK;
 }
 
