@@ -207,15 +207,42 @@ __global__ void __launch_bounds__(256) k_merge_lrows_b(BatchDev bd)
             cols[j] = -1;
             vals[j] = 0.0;
         }
+        // the pixel's nsub x 3 ray entries; with nsub = 4 (2 x 2 split) they are 48 + 96 contiguous,
+        // 16-byte aligned bytes, read with six vector loads
+        int mcv[MU];
+        double mwv[MU];
+        if (nsub == 4) {
+            const int4* mc4 = (const int4*)(mcb + p * MU);
+            const double2* mw2 = (const double2*)(mwb + p * MU);
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int4 t = mc4[j];
+                mcv[4 * j] = t.x;
+                mcv[4 * j + 1] = t.y;
+                mcv[4 * j + 2] = t.z;
+                mcv[4 * j + 3] = t.w;
+            }
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                const double2 t = mw2[j];
+                mwv[2 * j] = t.x;
+                mwv[2 * j + 1] = t.y;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < MU; j++) {
+                mcv[j] = (j < 3 * nsub) ? mcb[p * nsub * 3 + j] : -1;
+                mwv[j] = (j < 3 * nsub) ? mwb[p * nsub * 3 + j] : 0.0;
+            }
+        }
         int n = 0;
-        for (int k = 0; k < nsub; k++) {
-            const int* mc = mcb + (p * nsub + k) * 3;
-            const double* mw = mwb + (p * nsub + k) * 3;
-            if (mc[0] < 0) continue;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (mcv[3 * k] < 0) continue;
 #pragma unroll
             for (int e = 0; e < 3; e++) {
-                const int col = tix[mc[e]];
-                const double w = mw[e];
+                const int col = tix[mcv[3 * k + e]];
+                const double w = mwv[3 * k + e];
                 bool found = false;
 #pragma unroll
                 for (int j = 0; j < MU; j++)
@@ -240,12 +267,23 @@ __global__ void __launch_bounds__(256) k_merge_lrows_b(BatchDev bd)
 #pragma unroll
         for (int j = 0; j < MU; j++)
             if (j < n) {
-                uc[j] = cols[j];
-                uv[j] = vals[j];
                 const int cb = cols[j] >> M.cbs;
 #pragma unroll
                 for (int w = 0; w < QLX_MAX_CBW; w++) m[w] |= (w == (cb >> 5)) ? (1u << (cb & 31)) : 0u;
             }
+        if (nsub == 4) {  // the whole 12-slot row with vector stores (slots >= n: col -1, value 0, never read)
+#pragma unroll
+            for (int j = 0; j < 3; j++) ((int4*)uc)[j] = make_int4(cols[4 * j], cols[4 * j + 1], cols[4 * j + 2], cols[4 * j + 3]);
+#pragma unroll
+            for (int j = 0; j < 6; j++) ((double2*)uv)[j] = make_double2(vals[2 * j], vals[2 * j + 1]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < MU; j++)
+                if (j < n) {
+                    uc[j] = cols[j];
+                    uv[j] = vals[j];
+                }
+        }
         bd.ucnt[item * bd.s_pix + r] = n;
         unsigned* pm = bd.pmask + ((size_t)item * bd.s_pix + r) * QLX_MAX_CBW;
 #pragma unroll
