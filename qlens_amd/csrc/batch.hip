@@ -2604,32 +2604,51 @@ __global__ void __launch_bounds__(256) k_Ls_b(BatchDev bd)
 }
 
 // blurred model of every active pixel: the PSF taps over the dense unblurred image (zero outside
-// the active set, which is what skipping inactive neighbours gives); one lane per pixel in memory
-// order, so the lanes of one tap read consecutive pixels
+// the active set, which is what skipping inactive neighbours gives).  A lane takes MKB pixels
+// (k0 .. k0+MKB-1, l) and lanes of a wave consecutive l, so the lanes of one load read consecutive
+// pixels; each image value it loads serves every one of its pixels that the tap reaches
+// ((pnx + MKB - 1) pny loads for MKB pixels instead of MKB pnx pny).  Image rows go in descending
+// order, so each pixel still sums its taps pk ascending, pl ascending: the result is the
+// one-pixel-per-lane gather's, bit for bit.
+#define MKB 4
 __global__ void __launch_bounds__(256) k_model_b(BatchDev bd)
 {
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!M.valid || p >= bd.nx * bd.ny) return;
-    if (bd.pidx[item * bd.s_pix + p] < 0) return;
-    const int k = p / bd.ny, l = p % bd.ny;
-    const int nxh = bd.pnx / 2, nyh = bd.pny / 2;
+    const int nx = bd.nx, ny = bd.ny;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!M.valid || t >= (nx + MKB - 1) / MKB * ny) return;
+    const int kb = t / ny, l = t - kb * ny, k0 = kb * MKB;
+    const int* pidx = bd.pidx + item * bd.s_pix;
+    bool any = false;
+#pragma unroll
+    for (int a = 0; a < MKB; a++) any |= (k0 + a < nx) && pidx[(k0 + a) * ny + l] >= 0;
+    if (!any) return;
+    const int nxh = bd.pnx / 2, nyh = bd.pny / 2, pnx = bd.pnx, pny = bd.pny;
     double* model = bd.model + (size_t)item * 2 * bd.s_pix;
     const double* v = model + bd.s_pix;
-    double acc = 0.0;
-    for (int pk = 0; pk < bd.pnx; pk++) {
-        const int ii = k + nxh - pk;
-        if (ii < 0 || ii >= bd.nx) continue;
-        const double* vr = v + (size_t)ii * bd.ny;
-        const double* pr = bd.psf + (size_t)pk * bd.pny;
-        for (int pl = 0; pl < bd.pny; pl++) {
+    double acc[MKB];
+#pragma unroll
+    for (int a = 0; a < MKB; a++) acc[a] = 0.0;
+    for (int ii = k0 + MKB - 1 + nxh; ii >= k0 + nxh - pnx + 1; ii--) {
+        if (ii < 0 || ii >= nx) continue;
+        const double* vr = v + (size_t)ii * ny;
+        for (int pl = 0; pl < pny; pl++) {
             const int jj = l + nyh - pl;
-            if (jj < 0 || jj >= bd.ny) continue;
-            acc += pr[pl] * vr[jj];
+            if (jj < 0 || jj >= ny) continue;
+            const double x = vr[jj];
+#pragma unroll
+            for (int a = 0; a < MKB; a++) {
+                const int pk = k0 + a + nxh - ii;
+                if (pk >= 0 && pk < pnx) acc[a] += bd.psf[pk * pny + pl] * x;
+            }
         }
     }
-    model[p] = acc;
+#pragma unroll
+    for (int a = 0; a < MKB; a++) {
+        const int pa = (k0 + a) * ny + l;
+        if (k0 + a < nx && pidx[pa] >= 0) model[pa] = acc[a];
+    }
 }
 
 __global__ void __launch_bounds__(1024) k_evidence_b(BatchDev bd)
@@ -2706,8 +2725,8 @@ void launch_evidence_b(hipStream_t st, const BatchDev& bd, int max_img, int max_
     if (max_src > 0) hipLaunchKernelGGL(k_unpermute_b, dim3((max_src + 255) / 256, bd.B), dim3(256), 0, st, bd);
     if (max_img > 0) {
         hipLaunchKernelGGL(k_Ls_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, st, bd);
-        const long long npix = (long long)bd.nx * bd.ny;
-        hipLaunchKernelGGL(k_model_b, dim3((unsigned)((npix + 255) / 256), bd.B), dim3(256), 0, st, bd);
+        const long long nlane = (long long)(bd.nx + MKB - 1) / MKB * bd.ny;
+        hipLaunchKernelGGL(k_model_b, dim3((unsigned)((nlane + 255) / 256), bd.B), dim3(256), 0, st, bd);
     }
     hipLaunchKernelGGL(k_evidence_b, dim3(1, bd.B), dim3(1024), 0, st, bd);
 }
