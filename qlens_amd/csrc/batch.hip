@@ -346,6 +346,25 @@ __global__ void __launch_bounds__(1024) k_lrow_ptr_b(BatchDev bd)
     if (threadIdx.x == 0) lptr[M.n_img] = base;
 }
 
+// a merged 12-slot L row (2 x 2 split) as 3 + 6 16-byte loads
+__device__ __forceinline__ void load_row12(const int* uc, const double* uv, int (&c)[12], double (&w)[12])
+{
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int4 t = ((const int4*)uc)[j];
+        c[4 * j] = t.x;
+        c[4 * j + 1] = t.y;
+        c[4 * j + 2] = t.z;
+        c[4 * j + 3] = t.w;
+    }
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        const double2 t = ((const double2*)uv)[j];
+        w[2 * j] = t.x;
+        w[2 * j + 1] = t.y;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_lrow_csr_b(BatchDev bd)
 {
     const int item = blockIdx.y;
@@ -358,6 +377,19 @@ __global__ void __launch_bounds__(256) k_lrow_csr_b(BatchDev bd)
     const int* uc = bd.ucol + ((size_t)item * bd.s_pix + r) * U;
     const double* uv = bd.uval + ((size_t)item * bd.s_pix + r) * U;
     const int cnt = bd.ucnt[item * bd.s_pix + r];
+    if (U == 12) {  // 2 x 2 split: the merge wrote all 12 slots; read them as six 16-byte vectors
+        int c[12];
+        double w[12];
+        load_row12(uc, uv, c, w);
+#pragma unroll
+        for (int e = 0; e < 12; e++)
+            if (e < cnt) {
+                bd.lcj[o + e] = (unsigned)c[e] | (i << 14);
+                bd.lv[o + e] = w[e];
+                bd.lvf[o + e] = (float)w[e];
+            }
+        return;
+    }
     for (int e = 0; e < cnt; e++) {
         bd.lcj[o + e] = (unsigned)uc[e] | (i << 14);
         bd.lv[o + e] = uv[e];
@@ -2598,7 +2630,17 @@ __global__ void __launch_bounds__(256) k_Ls_b(BatchDev bd)
     const double* uval = bd.uval + ((size_t)item * bd.s_pix + r) * U;
     const double* s = bd.vec + M.vec_off;
     double acc = 0.0;
-    for (int e = 0; e < bd.ucnt[item * bd.s_pix + r]; e++) acc += uval[e] * s[ucol[e]];
+    const int cnt = bd.ucnt[item * bd.s_pix + r];
+    if (U == 12) {  // 2 x 2 split: all 12 slots written by the merge, same summation order
+        int c[12];
+        double w[12];
+        load_row12(ucol, uval, c, w);
+#pragma unroll
+        for (int e = 0; e < 12; e++)
+            if (e < cnt) acc += w[e] * s[c[e]];
+    } else {
+        for (int e = 0; e < cnt; e++) acc += uval[e] * s[ucol[e]];
+    }
     // the unblurred model as a dense image (pixel index; inactive pixels keep the zero fill)
     bd.model[(size_t)item * 2 * bd.s_pix + bd.s_pix + bd.act_pix[item * bd.s_pix + r]] = acc;
 }
