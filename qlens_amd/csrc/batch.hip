@@ -492,7 +492,8 @@ __global__ void __launch_bounds__(256) k_row_scatter_b(BatchDev bd)
     bd.perm[item * bd.s_pix + pos] = r;
 }
 
-// ascending original order inside each bucket: each bucket's slice is ranked by a workgroup
+// ascending original order inside each bucket: each wave ranks one bucket's slice (buckets hold a
+// few rows each, so a workgroup per bucket left most of its lanes idle through two barriers)
 __global__ void __launch_bounds__(256) k_bucket_rank_b(BatchDev bd)
 {
     const int item = blockIdx.y;
@@ -501,18 +502,25 @@ __global__ void __launch_bounds__(256) k_bucket_rank_b(BatchDev bd)
     const int* off = bd.boff + item * (bd.s_cell + 2);
     int* perm = bd.perm + item * bd.s_pix;
     int* key = bd.key + item * bd.s_pix;  // reused as scratch for the ranked slice
-    for (int k = blockIdx.x; k < M.n_src; k += gridDim.x) {
-        const int lo = off[k], hi = off[k + 1], n = hi - lo;
-        if (n <= 1) continue;
-        // rank = number of smaller row ids in the slice (ids are distinct); write to key[lo+rank]
-        for (int i = threadIdx.x; i < n; i += 256) {
-            const int v = perm[lo + i];
-            int rank = 0;
-            for (int j = 0; j < n; j++) rank += (perm[lo + j] < v);
-            key[lo + rank] = v;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int k0 = blockIdx.x * 4; k0 < M.n_src; k0 += gridDim.x * 4) {
+        const int k = k0 + w;
+        int lo = 0, n = 0;
+        if (k < M.n_src) {
+            lo = off[k];
+            n = off[k + 1] - lo;
         }
+        // rank = number of smaller row ids in the slice (ids are distinct); write to key[lo+rank]
+        if (n > 1)
+            for (int i = lane; i < n; i += 64) {
+                const int v = perm[lo + i];
+                int rank = 0;
+                for (int j = 0; j < n; j++) rank += (perm[lo + j] < v);
+                key[lo + rank] = v;
+            }
         __syncthreads();
-        for (int i = threadIdx.x; i < n; i += 256) perm[lo + i] = key[lo + i];
+        if (n > 1)
+            for (int i = lane; i < n; i += 64) perm[lo + i] = key[lo + i];
         __syncthreads();
     }
 }
@@ -524,7 +532,7 @@ void launch_sort_rows_b(hipStream_t s, const BatchDev& bd, int max_img, int max_
     hipLaunchKernelGGL(k_row_keys_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, s, bd);
     hipLaunchKernelGGL(k_bucket_scan_b, dim3(1, bd.B), dim3(1024), 0, s, bd);
     hipLaunchKernelGGL(k_row_scatter_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, s, bd);
-    hipLaunchKernelGGL(k_bucket_rank_b, dim3(std::min(max_src, 256), bd.B), dim3(256), 0, s, bd);
+    hipLaunchKernelGGL(k_bucket_rank_b, dim3(std::min((max_src + 3) / 4, 256), bd.B), dim3(256), 0, s, bd);
 }
 
 // ------------------------------------------------------------------------------------------------
