@@ -1099,8 +1099,9 @@ __device__ __forceinline__ double* rtile(const BatchDev& bd, const ItemMeta& M, 
 // the source block (R is held in the reference order: internal indices map through pisrc), the data
 // row n kept with a large pivot (the refined solve never reads it), identity beyond n and the FP64
 // diagonal into Fdiag.  A diagonal tile's upper half gets the same symmetric formula.  The tiles go to
-// Ff32 (F as formed, which the refinement residual reads); the factor buffer F32 receives them from
-// k_strip_copy_b and the first trailing update.  Wave (wr, wc) holds quadrant (wr, wc).
+// Ff32 (F as formed, which the refinement residual reads); the Cholesky kernels that touch a block
+// first (the first part's diagonal and panel kernels, the first strip and trailing updates) read it
+// there and write the factor buffer F32.  Wave (wr, wc) holds quadrant (wr, wc).
 __device__ __forceinline__ void syrk_f16_store(const BatchDev& bd, const ItemMeta& M, int item, int I, int J,
                                                const float4_t (&acc)[4][4])
 {
@@ -1210,26 +1211,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
     }
 #undef SYRK_LOAD
     syrk_f16_store(bd, M, item, I, J, acc);
-}
-
-// Mixed path: F's first four block columns (the lower 64 x 64 tiles (I, J), J < 4), which the first
-// panel factors, from Ff32 into the factor buffer F32; the first trailing update moves the rest
-// (it reads its C tiles from Ff32 and writes F32).
-__global__ void __launch_bounds__(256) k_strip_copy_b(BatchDev bd)
-{
-    const int item = blockIdx.y;
-    const ItemMeta& M = bd.meta[item];
-    if (!M.valid) return;
-    const int ld = M.ldN, nb = ld / TS;
-    const int I = blockIdx.x >> 2, J = blockIdx.x & 3;
-    if (I >= nb || J > I) return;
-    const size_t o = M.f_off + (size_t)I * TS * ld + (size_t)J * TS;
-    const int tid = threadIdx.x;
-#pragma unroll
-    for (int h = 0; h < 4; h++) {
-        const int e = (tid + 256 * h) * 4, rr = e >> 6, cc = e & 63;
-        *(float4_t*)&bd.F32[o + (size_t)rr * ld + cc] = *(const float4_t*)&bd.Ff32[o + (size_t)rr * ld + cc];
-    }
 }
 
 void launch_syrk_f16_b(hipStream_t s, const BatchDev& bd, int max_ncb)
@@ -1342,10 +1323,7 @@ __global__ void __launch_bounds__(256) k_prep_f_b(BatchDev bd)
 void launch_prep_fr_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32)
 {
     const int nb = max_ld / TS;
-    if (f32) {  // the mixed path's FP16 SYRK prepares F in its epilogue; the first panel's columns remain
-        hipLaunchKernelGGL(k_strip_copy_b, dim3(4 * nb, bd.B), dim3(256), 0, s, bd);
-        return;
-    }
+    if (f32) return;  // the mixed path's FP16 SYRK prepares F in its epilogue
     const unsigned g = nb * (nb + 1) / 2;
     hipLaunchKernelGGL(k_prep_f_b, dim3(g, bd.B), dim3(256), 0, s, bd);
 }
@@ -1480,13 +1458,15 @@ __device__ void chol_diag_wave_f32(const MatDesc& D, int kb, float (*Lt)[TS])
     int ld;
     typedef float T;
     T* tile = tile_ptr<T>(D, kb, kb, ld);
+    // the first diagonal block comes from F as formed (A0), when the matrix has it
+    const T* src = (kb == 0 && D.A0) ? (const T*)D.A0 + (tile - (T*)D.A) : tile;
     // lane r loads row r straight into registers: all of its vector loads are in flight together
     typedef typename Mfma16<T>::vec_t V;
     constexpr int VW = sizeof(V) / sizeof(T);
     T a[TS];
 #pragma unroll
     for (int q = 0; q < TS; q += VW) {
-        const V v = *(const V*)&tile[(size_t)r * ld + q];
+        const V v = *(const V*)&src[(size_t)r * ld + q];
         a[q] = v.x;
         a[q + 1] = v.y;
         if constexpr (VW == 4) {
@@ -1772,8 +1752,11 @@ __global__ void __launch_bounds__(256) k_chol_lpanel_m(const MatDesc* mats, int 
     if (ib > band_last(D, kb)) return;
     int ld, lp;
     T* blk = tile_ptr<T>(D, ib, kb, ld);
+    // the part that starts at column 0 reads its blocks from F as formed (A0), when the matrix has
+    // it: each is read there once and written to A
+    const bool first = (k0 == 0 && D.A0);
     T cv[16];
-    load64c<T>(blk, ld, cv);
+    load64c<T>(first ? (const T*)D.A0 + (blk - (T*)D.A) : blk, ld, cv);
     Acc acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; a++)
@@ -1802,7 +1785,7 @@ __global__ void __launch_bounds__(256) k_chol_lpanel_m(const MatDesc* mats, int 
     store64<T>(blk, ld, acc, 0);
     if (ib != kb + 1 || ib >= kend) return;
     T* dg = tile_ptr<T>(D, ib, ib, ld);
-    load64c<T>(dg, ld, cv);
+    load64c<T>(first ? (const T*)D.A0 + (dg - (T*)D.A) : dg, ld, cv);
     __syncthreads();
     store64<T>(&As[0][0], LDT, acc, 0);
     __syncthreads();

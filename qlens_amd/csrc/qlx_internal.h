@@ -146,8 +146,9 @@ struct MatDesc {
     void* diag;                 // nb*64 pivots (element type of A)
     int* status;                // bit 0 set on a negative pivot of a row < n
     void* W;                    // nb block inverses (64x64 each, element type of A)
-    void* A0;                   // dense FP32: F as formed; the first trailing update reads its C
-                                // tiles here and writes A (null: in place)
+    void* A0;                   // dense FP32: F as formed; the kernels that touch a block first (the
+                                // first part's diagonal and panel kernels, the first strip and
+                                // trailing updates) read it here and write A (null: in place)
 };
 
 void upload_patterson(int device);
