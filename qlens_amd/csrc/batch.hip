@@ -540,6 +540,17 @@ void launch_sort_rows_b(hipStream_t s, const BatchDev& bd, int max_img, int max_
 // (one workgroup per chunk), then per-item slot bases of the compact block pool.
 // ------------------------------------------------------------------------------------------------
 
+// m[o .. o+3] |= the four words of v.  Mask rows hold QLX_MAX_CBW = 8 words; words from ncbw on may
+// hold anything (they are never written to the support masks)
+static_assert(QLX_MAX_CBW == 8, "mask rows are read as two 16-byte vectors");
+__device__ __forceinline__ void or_mask4(unsigned (&m)[QLX_MAX_CBW], int o, uint4 v)
+{
+    m[o] |= v.x;
+    m[o + 1] |= v.y;
+    m[o + 2] |= v.z;
+    m[o + 3] |= v.w;
+}
+
 // PSF x window, per grid pixel t = y*nx + x: hmask[t] = OR of pmask over the active pixels x' in
 // [x + nxh - pnx + 1, x + nxh] of row y (one contiguous active-index range [apre[..i0], apre[..i1+1]))
 __global__ void __launch_bounds__(256) k_psf_hdilate_b(BatchDev bd)
@@ -558,15 +569,15 @@ __global__ void __launch_bounds__(256) k_psf_hdilate_b(BatchDev bd)
 #pragma unroll
     for (int w = 0; w < QLX_MAX_CBW; w++) m[w] = 0;
     const int a0 = apre[y * bd.nx + i0], a1 = apre[y * bd.nx + i1 + 1];
+    const bool wide = M.ncbw > 4;  // mask rows are 32-byte aligned: one or two 16-byte loads per row
     for (int a = a0; a < a1; a++) {
-#pragma unroll
-        for (int w = 0; w < QLX_MAX_CBW; w++)
-            if (w < M.ncbw) m[w] |= pm[(size_t)a * QLX_MAX_CBW + w];
+        const uint4* p4 = (const uint4*)(pm + (size_t)a * QLX_MAX_CBW);
+        or_mask4(m, 0, p4[0]);
+        if (wide) or_mask4(m, 4, p4[1]);
     }
-    unsigned* hm = bd.hmask + ((size_t)item * bd.s_pix + t) * QLX_MAX_CBW;
-#pragma unroll
-    for (int w = 0; w < QLX_MAX_CBW; w++)
-        if (w < M.ncbw) hm[w] = m[w];
+    uint4* h4 = (uint4*)(bd.hmask + ((size_t)item * bd.s_pix + t) * QLX_MAX_CBW);
+    h4[0] = make_uint4(m[0], m[1], m[2], m[3]);
+    if (wide) h4[1] = make_uint4(m[4], m[5], m[6], m[7]);
 }
 
 __global__ void __launch_bounds__(256) k_row_support_b(BatchDev bd)
@@ -602,10 +613,9 @@ __global__ void __launch_bounds__(256) k_row_support_b(BatchDev bd)
         const int rr = q / bd.pny, pl = q - rr * bd.pny;
         const int jj = rl[rr] + nyh - pl;
         if (jj < 0 || jj >= bd.ny) continue;
-        const unsigned* h = hm + ((size_t)jj * bd.nx + rk[rr]) * QLX_MAX_CBW;
-#pragma unroll
-        for (int w = 0; w < QLX_MAX_CBW; w++)
-            if (w < M.ncbw) m[w] |= h[w];
+        const uint4* h4 = (const uint4*)(hm + ((size_t)jj * bd.nx + rk[rr]) * QLX_MAX_CBW);
+        or_mask4(m, 0, h4[0]);
+        if (M.ncbw > 4) or_mask4(m, 4, h4[1]);
     }
 #pragma unroll
     for (int w = 0; w < QLX_MAX_CBW; w++)
