@@ -1241,9 +1241,11 @@ __global__ void __launch_bounds__(256) k_zero_r_b(BatchDev bd)
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     if (!M.valid) return;
-    double* R = bd.R + M.r_off;
-    const size_t n = (size_t)(M.ldN / TS) * (M.bwb + 1) * TS * TS;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) R[t] = 0.0;
+    // 16-byte stores: the band holds whole 64 x 64 tiles and starts on a tile boundary
+    double2* R2 = (double2*)(bd.R + M.r_off);
+    const size_t n2 = (size_t)(M.ldN / TS) * (M.bwb + 1) * TS * TS / 2;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += (size_t)gridDim.x * blockDim.x)
+        R2[t] = make_double2(0.0, 0.0);
 }
 
 __global__ void __launch_bounds__(256) k_curvature_b(BatchDev bd)
