@@ -29,7 +29,9 @@ typedef enum {
     QLX_ERR_NOMEM = -4,    /* device allocation failed / admission refused */
     QLX_ERR_NOT_PD = -5,   /* F or R not positive definite (reference: warn + die, pixelgrid.cpp:17095-17099, 17384) */
     QLX_ERR_NOSOURCE = -6, /* no source pixel mapped (reference sentinel 2e30, pixelgrid.cpp:12807) */
-    QLX_ERR_RANGE = -7     /* size above a compiled-in maximum */
+    QLX_ERR_RANGE = -7,    /* size above a compiled-in maximum */
+    QLX_ERR_REFINE = -8,   /* MIXED item: the FP64 refinement stopped above refine_tol (value kept) */
+    QLX_ITEMS_FAILED = 1   /* return of qlx_eval*: the call completed, some items carry a non-ok status */
 } qlx_status;
 
 /* lens types */
@@ -39,7 +41,8 @@ enum { QLX_DEF_QUADRATURE = 0, QLX_DEF_SURROGATE = 1 };
 /* precision modes */
 enum {
     QLX_PREC_FP64 = 0,  /* parity mode: FP64 everywhere (MFMA f64 for F and Cholesky) */
-    QLX_PREC_MIXED = 1  /* BF16-in/FP32-acc F, FP32 Cholesky + FP64 refinement, FP64 log-dets */
+    QLX_PREC_MIXED = 1  /* F from an FP16 pool on MFMA with FP32 accumulation, FP32 Cholesky + FP64
+                           iterative refinement, FP64 log-dets and everything after the solve */
 };
 /* ray-shooting arithmetic */
 enum { QLX_RAY_FP64 = 0, QLX_RAY_FP32 = 1 };
@@ -87,8 +90,12 @@ typedef struct {
 
 /* Per-evaluation result (the scalars QLens fills, qlens.h:750-761, 891). */
 typedef struct {
-    double minus2lnE;   /* -2 ln E without noise constants (lens.cpp:13033-13043 off), or a
-                           sentinel: 2e30 (no/too-small source grid, lens.cpp:12453-12458) */
+    double minus2lnE;   /* -2 ln E without noise constants (lens.cpp:13033-13043 off), or the
+                           sentinel 2e30 for an item whose status is an error: no or too small a
+                           source grid (lens.cpp:12453-12458, pixelgrid.cpp:12807), invalid lens
+                           parameters, a source grid beyond QLX_MAX_LD, or F / R not positive
+                           definite (where the reference dies, pixelgrid.cpp:17384).  Items with
+                           QLX_ERR_REFINE keep their value (accurate to refine_resid). */
     double chisq0;      /* chi-square of the best-fit model over mask pixels */
     double logdetF, logdetR, Es; /* ln det F, ln det R, s^T R s */
     double refine_resid;/* MIXED: final relative residual of the refined solve (FP64: 0) */
@@ -119,8 +126,13 @@ int qlx_set_settings(void *ctx, const qlx_settings *s);
 /* ---- evaluation ----------------------------------------------------------------------------
  * Replaces one call of QLens::invert_image_surface_brightness_map (lens.cpp:12293) per model:
  * nbatch independent lens models (e.g. sampler proposals) are evaluated on the same image and
- * settings.  Synchronous.  The per-item status is in out[b].status; the return value is the
- * first non-ok item status or an API error. */
+ * settings.  Synchronous.  Returns QLX_OK when every item succeeded, QLX_ITEMS_FAILED (> 0) when
+ * the call completed but some items carry an error status in out[b].status (their minus2lnE
+ * holds the 2e30 sentinel), or a negative qlx_status when the call as a whole failed (bad
+ * arguments, an unknown lens type, no surrogate loaded, HIP or memory errors).  Lens parameters
+ * that are not finite, or outside the profile's domain (q in (0, 1], rs > 0), fail their own item
+ * with QLX_ERR_ARG.  Items are processed in waves sized to the HBM budget; an item's result does
+ * not depend on the other items or on the wave it lands in. */
 int qlx_eval(void *ctx, int nbatch, const qlx_lens_model *models, qlx_result *out);
 
 /* Batched evaluation with per-item source grids (uneven source sizes in one submission).
@@ -128,11 +140,33 @@ int qlx_eval(void *ctx, int nbatch, const qlx_lens_model *models, qlx_result *ou
 int qlx_eval_batch_grids(void *ctx, int nbatch, const qlx_lens_model *models, const int *src_n,
                          qlx_result *out);
 
-/* Read back item `item` of the last qlx_eval: source amplitudes per cell (x-major
- * [src_nx*src_ny], 0 for inactive cells; update_source_amplitudes pixelgrid.cpp:17850-17881)
- * and the model image (x-major [nx*ny]; store_image_pixel_surface_brightness 18483-18498).
- * Either pointer may be NULL.  Only the last item of the last call is kept. */
-int qlx_get_solution(void *ctx, double *src_cells, double *model_image);
+/* Read back item `item` (0 .. nbatch-1) of the last qlx_eval / qlx_eval_batch_grids call.  Every
+ * successful item of that call is kept on the device until the next evaluation starts.
+ *   src_nx, src_ny  (out, may be NULL) the item's own source grid
+ *   src_cells       amplitude per cell, x-major [src_nx*src_ny], 0 for inactive cells
+ *                   (update_source_amplitudes, pixelgrid.cpp:17850-17881); src_cap = its length
+ *                   in doubles, which must be at least src_nx*src_ny
+ *   active_index    the reference's active index of each cell (x-major, -1 = inactive;
+ *                   SourcePixel::assign_active_indices, pixelgrid.cpp:3586-3626), same capacity
+ *   model_image     the model image, x-major [nx*ny] (store_image_pixel_surface_brightness,
+ *                   pixelgrid.cpp:18483-18498); model_cap = its length, at least nx*ny
+ * Any array pointer may be NULL (its capacity is then ignored).  QLX_ERR_ARG for a capacity
+ * below the item's size or an item out of range; QLX_ERR_STATE if the item has no solution (its
+ * status was an error, or no evaluation has run since the image, PSF or settings changed). */
+int qlx_get_source(void *ctx, int item, int *src_nx, int *src_ny, double *src_cells,
+                   int *active_index, size_t src_cap, double *model_image, size_t model_cap);
+
+/* Source-plane points of every subpixel ray of item `item` of the last evaluation, as the
+ * reference keeps them in ImagePixelGrid::subpixel_center_sourcepts after an inversion
+ * (calculate_sourcepts_and_areas, pixelgrid.cpp:10434-10441): beta[((i*ny + j)*nsplit^2 + k)*2 + c]
+ * for pixel (i, j), subpixel k = ii*nsplit + jj, component c; 0 for pixels outside the mask.
+ * Available for the items of the call's last wave (every item unless HBM admission split the
+ * call); cap = length of beta in doubles, at least nx*ny*nsplit^2*2. */
+int qlx_get_rays(void *ctx, int item, double *beta, size_t cap);
+
+/* The PSF matrix in use (x-major [pnx*pny], as passed to qlx_set_psf or built by
+ * qlx_set_psf_gaussian); psf may be NULL to query the size only. */
+int qlx_get_psf(void *ctx, int *pnx, int *pny, double *psf, size_t cap);
 
 /* ---- ray shooting and data (building blocks, host buffers) --------------------------------
  * beta = theta - alpha at n points (QLens::find_sourcept, lenscalc.cpp:318-330). */
@@ -182,25 +216,28 @@ int qlx_surrogate_grad(void *ctx, int n, const float *x, const float *y, float *
 /* ---- timing support for benchmarks ---------------------------------------------------------
  * Average device time (ms) per launch of the named stage over the last qlx_eval, from HIP events
  * on the context's stream; one launch covers one wave (every item of the wave); names:
- * "raytrace", "lbuild", "syrk", "cholesky", "solve", "total". */
+ * "raytrace", "lbuild" (L rows, support masks and the blurred-L pool), "lbkernel" (the pool
+ * build kernel alone), "syrk", "cholesky", "solve", "total", "surrogate". */
 int qlx_kernel_time(void *ctx, const char *name, double *ms, int *launches);
 
 /* Work counters of the last wave of the last qlx_eval (no reference counterpart; benchmark
  * accounting).  Sums over the wave's items unless noted:
  *   syrk_dense_flops   2*64^3 per (lower tile, 64-row chunk) pair of a dense SYRK
- *   syrk_exec_flops    MFMA flops executed: the pairs whose chunk holds both blocks (x3 in
- *                      MIXED mode: hi*hi + hi*lo + lo*hi products)
+ *   syrk_exec_flops    MFMA flops executed: 2*64*CB^2 per pair of nonzero 64 x CB pool blocks
+ *                      of one 64-row chunk (CB = 64 in FP64, 128 in MIXED; one product each)
  *   syrk_alg_flops     N_img*N_src*(N_src+1) + 2*N_img*N_src (the F and D arithmetic)
  *   lb_bytes           bytes of the blurred lensing matrix blocks written
  *   integrand_evals    NFW integrand evaluations of the ray shooting
  *   chol_flops         N_src^3 / 3 (F's factorization; R's banded one is not counted)
  *   n_items            items of the wave
- *   n_img, n_src, ld_src, rows_pad: sizes of the wave's last item */
+ *   n_img, n_src, ld_src, rows_pad: sizes of the wave's last item
+ *   n_waves            waves the last call was split into (HBM admission) */
 typedef struct {
     double syrk_dense_flops, syrk_exec_flops, syrk_alg_flops;
     double lb_bytes, integrand_evals, chol_flops;
     int n_items;
     int n_img, n_src, ld_src, rows_pad;
+    int n_waves;
 } qlx_stats;
 int qlx_get_stats(void *ctx, qlx_stats *out);
 

@@ -5,22 +5,38 @@
 // language, commands.cpp), then runs one of these modes and writes raw little-endian
 // arrays plus a text file of scalars into <outdir>:
 //
-//   invert  <script> <outdir> [sigma] [nrep] [psf=<file>:<nx>:<ny>]
+//   invert  <script> <outdir> [sigma] [nrep] [psf=<file>:<nx>:<ny>] [mask=<file>:<nx>:<ny>]
 //       Runs QLens::invert_surface_brightness_map_from_data (lens.cpp:12246) on the image
 //       loaded by the script, with a uniform noise map of dispersion sigma (the state that
 //       'sbmap generate_uniform_noisemap' produces, commands.cpp:9453-9458).  Dumps -2 ln E,
 //       chisq0, ln det F, ln det R, active counts, the subpixel source points (rays), the
 //       image mapping flags, the model image and the source-cell amplitudes.  With nrep > 1
-//       the inversion is repeated and the median wall time per call is reported.  With
+//       the inversion is repeated and the median wall time per call over all but the first
+//       (warm-up) call is reported.  With
 //       psf=..., the PSF matrix is injected from a raw x-major file (QLens's psf_matrix with
-//       use_input_psf_matrix, qlens.h:877-880), as the reference does for a loaded PSF.
+//       use_input_psf_matrix, qlens.h:877-880), as the reference does for a loaded PSF.  With
+//       mask=..., primary mask 0 of the image is set from a raw x-major byte file (the state
+//       'sbmap loadmask' leaves, pixelgrid.cpp:6512-6647; FITS loading is compiled out here).
 //   deflect <script> <outdir> points=<points.bin>:<n>
 //       Evaluates QLens::deflection (lenscalc.cpp:81-113) and the per-lens
 //       LensProfile::deflection / hessian (profile.cpp:1782-1805) at n points (x,y pairs).
 //   patterson <script> <outdir>
 //       Dumps the Gauss-Patterson nodes and weights held by lens 0 (gauss.cpp:521-2099).
-//   mock <script> <outdir> <sigma>
-//       Runs the reference's mock-image path (ray trace, analytic source SB, PSF, noise).
+//   mock <script> <outdir> [psf=<file>:<nx>:<ny>]
+//       Runs QLens::plot_lensed_surface_brightness (lens.cpp:11882-12066) on the no-data image
+//       grid the script sets up ('grid xmin xmax ymin ymax', 'img_npixels'): the analytic
+//       sources of the script ray-traced per subpixel and averaged
+//       (ImagePixelGrid::find_surface_brightness, pixelgrid.cpp:11700-11940), PSF-convolved
+//       (PSF_convolution_pixel_vector, pixelgrid.cpp:14224-14496) and, with
+//       'simulate_pixel_noise on', given noise from the generator seeded by 'random_seed'
+//       (add_pixel_noise, pixelgrid.cpp:10987-10996).  Dumps the image x-major (mock.bin).
+//   rng <script> <outdir> <seed> <n>
+//       The first n normal deviates of the reference's generator (Random, rand.h:19-76) after
+//       set_random_seed(seed) (deviates.bin), and the first n raw 64-bit outputs of a second
+//       generator with the same seed (int64.bin).
+//   psf <script> <outdir> <xstep> <ystep>
+//       QLens::generate_PSF_matrix (pixelgrid.cpp:14601-14668) for the script's psf_width and
+//       psf_threshold on pixels of xstep by ystep (psf.bin x-major, sizes in scalars.txt).
 //
 // Build: make -f oracle/ref.mk (needs /root/reference; output lives only in oracle/_ref/).
 
@@ -86,9 +102,10 @@ static int mode_invert(QLens& lens, const string& out, double sigma, int nrep)
 		auto t1 = chrono::steady_clock::now();
 		times.push_back(chrono::duration<double>(t1 - t0).count());
 	}
-	vector<double> sorted = times;
+	// with more than one repetition the first is a warm-up and the median is over the others
+	vector<double> sorted(times.begin() + (nrep > 1 ? 1 : 0), times.end());
 	sort(sorted.begin(), sorted.end());
-	double median = sorted[sorted.size() / 2];
+	double median = sorted.size() % 2 ? sorted[sorted.size() / 2] : 0.5 * (sorted[sorted.size() / 2 - 1] + sorted[sorted.size() / 2]);
 
 	ImagePixelGrid* g = lens.image_pixel_grids[0];
 	int nx = g->x_N, ny = g->y_N;
@@ -138,6 +155,7 @@ static int mode_invert(QLens& lens, const string& out, double sigma, int nrep)
 		sg->srcgrid_xmin, sg->srcgrid_xmax, sg->srcgrid_ymin, sg->srcgrid_ymax);
 	fprintf(f, "psf_nx %d\npsf_ny %d\n", lens.psf_npixels_x, lens.psf_npixels_y);
 	fprintf(f, "median_seconds %.9g\n", median);
+	fprintf(f, "min_seconds %.9g\nmax_seconds %.9g\n", sorted.front(), sorted.back());
 	fprintf(f, "nrep %d\n", nrep);
 	fprintf(f, "omp_threads %d\n", omp_get_max_threads());
 	fclose(f);
@@ -182,6 +200,57 @@ static int mode_deflect(QLens& lens, const string& out, const string& ptsfile, i
 	return 0;
 }
 
+static int mode_mock(QLens& lens, const string& out)
+{
+	lens.set_random_seed(lens.random_seed);
+	if (!lens.plot_lensed_surface_brightness(out + "/mock_plot", false, false, false, false, false, false, false, false, false,
+		false, false, false, -1, false)) {
+		fprintf(stderr, "plot_lensed_surface_brightness failed\n");
+		return 2;
+	}
+	ImagePixelGrid* g = lens.image_pixel_grids[0];
+	int nx = g->x_N, ny = g->y_N;
+	vector<double> img((size_t)nx * ny);
+	for (int i = 0; i < nx; i++)
+		for (int j = 0; j < ny; j++) img[(size_t)i * ny + j] = g->surface_brightness[i][j];
+	write_bin(out + "/mock.bin", img.data(), img.size() * sizeof(double));
+	FILE* f = fopen((out + "/scalars.txt").c_str(), "w");
+	fprintf(f, "img_nx %d\nimg_ny %d\nrandom_seed %lld\nsigma %.17g\n", nx, ny, (long long)lens.random_seed,
+		lens.background_pixel_noise);
+	fprintf(f, "psf_nx %d\npsf_ny %d\n", lens.psf_npixels_x, lens.psf_npixels_y);
+	fclose(f);
+	return 0;
+}
+
+static int mode_rng(const string& out, unsigned long long seed, int n)
+{
+	Random r(seed), r64(seed);
+	r.set_random_seed(seed);
+	vector<double> dev((size_t)n);
+	vector<unsigned long long> raw((size_t)n);
+	for (int k = 0; k < n; k++) dev[k] = r.NormalDeviate();
+	for (int k = 0; k < n; k++) raw[k] = r64.int64();
+	write_bin(out + "/deviates.bin", dev.data(), dev.size() * sizeof(double));
+	write_bin(out + "/int64.bin", raw.data(), raw.size() * sizeof(unsigned long long));
+	return 0;
+}
+
+static int mode_psf(QLens& lens, const string& out, double xstep, double ystep)
+{
+	if (!lens.generate_PSF_matrix(xstep, ystep, false)) {
+		fprintf(stderr, "generate_PSF_matrix failed\n");
+		return 2;
+	}
+	vector<double> psf((size_t)lens.psf_npixels_x * lens.psf_npixels_y);
+	for (int i = 0; i < lens.psf_npixels_x; i++)
+		for (int j = 0; j < lens.psf_npixels_y; j++) psf[(size_t)i * lens.psf_npixels_y + j] = lens.psf_matrix[i][j];
+	write_bin(out + "/psf.bin", psf.data(), psf.size() * sizeof(double));
+	FILE* f = fopen((out + "/scalars.txt").c_str(), "w");
+	fprintf(f, "psf_nx %d\npsf_ny %d\n", lens.psf_npixels_x, lens.psf_npixels_y);
+	fclose(f);
+	return 0;
+}
+
 static int mode_patterson(QLens& lens, const string& out)
 {
 	if (lens.nlens < 1) { fprintf(stderr, "need one lens\n"); return 2; }
@@ -197,7 +266,7 @@ static int mode_patterson(QLens& lens, const string& out)
 int main(int argc, char* argv[])
 {
 	if (argc < 4) {
-		fprintf(stderr, "usage: ref_harness invert|deflect|patterson|mock <script> <outdir> ...\n");
+		fprintf(stderr, "usage: ref_harness invert|deflect|patterson|mock|rng|psf <script> <outdir> ...\n");
 		return 2;
 	}
 	string mode = argv[1], script = argv[2], out = argv[3];
@@ -221,9 +290,9 @@ int main(int argc, char* argv[])
 		lens.set_quit_after_reading_file(true);
 		lens.process_commands(true);
 
-		// key=value options: psf=<file>:<nx>:<ny>, points=<file>:<n>
-		string psf_file, pts_file;
-		int psf_nx = 0, psf_ny = 0, npts = 0;
+		// key=value options: psf=<file>:<nx>:<ny>, points=<file>:<n>, mask=<file>:<nx>:<ny>
+		string psf_file, pts_file, mask_file;
+		int psf_nx = 0, psf_ny = 0, npts = 0, mask_nx = 0, mask_ny = 0;
 		vector<string> pos;
 		for (int a = 4; a < argc; a++) {
 			string arg = argv[a];
@@ -235,7 +304,33 @@ int main(int argc, char* argv[])
 				char buf[4096];
 				if (sscanf(arg.c_str() + 7, "%4095[^:]:%d", buf, &npts) != 2) { fprintf(stderr, "bad points=\n"); return 2; }
 				pts_file = buf;
+			} else if (arg.compare(0, 5, "mask=") == 0) {
+				char buf[4096];
+				if (sscanf(arg.c_str() + 5, "%4095[^:]:%d:%d", buf, &mask_nx, &mask_ny) != 3) { fprintf(stderr, "bad mask=\n"); return 2; }
+				mask_file = buf;
 			} else pos.push_back(arg);
+		}
+		if (!mask_file.empty()) {
+			// primary mask 0 of the loaded image, as ImagePixelData::load_mask_fits leaves it for a
+			// primary mask (pixelgrid.cpp:6620-6647): in_mask from the file (x-major bytes, nonzero =
+			// in), the extended mask keeping every pixel, n_mask_pixels recounted
+			ImagePixelData* d = lens.image_pixel_data;
+			if (!d || d->npixels_x != mask_nx || d->npixels_y != mask_ny) { fprintf(stderr, "mask size does not match the image\n"); return 2; }
+			vector<unsigned char> mk((size_t)mask_nx * mask_ny);
+			FILE* mf = fopen(mask_file.c_str(), "rb");
+			if (!mf || fread(mk.data(), 1, mk.size(), mf) != mk.size()) { fprintf(stderr, "bad mask file\n"); return 2; }
+			fclose(mf);
+			int count = 0;
+			for (int i = 0; i < mask_nx; i++)
+				for (int j = 0; j < mask_ny; j++) {
+					d->in_mask[0][i][j] = mk[(size_t)i * mask_ny + j] != 0;
+					if (d->in_mask[0][i][j]) {
+						d->extended_mask[0][i][j] = true;
+						count++;
+					}
+				}
+			d->n_mask_pixels[0] = count;
+			d->extended_mask_n_neighbors[0] = -1;
 		}
 		if (!psf_file.empty()) {
 			vector<double> buf((size_t)psf_nx * psf_ny);
@@ -260,6 +355,14 @@ int main(int argc, char* argv[])
 			rc = mode_deflect(lens, out, pts_file, npts);
 		} else if (mode == "patterson") {
 			rc = mode_patterson(lens, out);
+		} else if (mode == "mock") {
+			rc = mode_mock(lens, out);
+		} else if (mode == "rng") {
+			if (pos.size() < 2) { fprintf(stderr, "rng needs <seed> <n>\n"); return 2; }
+			rc = mode_rng(out, strtoull(pos[0].c_str(), NULL, 10), atoi(pos[1].c_str()));
+		} else if (mode == "psf") {
+			if (pos.size() < 2) { fprintf(stderr, "psf needs <xstep> <ystep>\n"); return 2; }
+			rc = mode_psf(lens, out, atof(pos[0].c_str()), atof(pos[1].c_str()));
 		} else {
 			fprintf(stderr, "unknown mode %s\n", mode.c_str());
 			rc = 2;

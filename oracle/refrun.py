@@ -68,7 +68,9 @@ def write_inputs(case, data, workdir, lenses=None, psf_mode="input"):
         "bg_pixel_noise %s" % _fmt(case["sigma"]),
     ]
     if psf_mode == "gaussian":
-        lines += ["psf_width %s" % _fmt(case["psf_sigma"]), "psf_threshold %s" % _fmt(case["psf_threshold"])]
+        sig = case["psf_sigma"]
+        sig = (sig, sig) if np.isscalar(sig) else tuple(sig)
+        lines += ["psf_width %s %s" % (_fmt(sig[0]), _fmt(sig[1])), "psf_threshold %s" % _fmt(case["psf_threshold"])]
     lines += lens_commands(case["lenses"] if lenses is None else lenses)
     lines += ["sbmap loadimg %s" % os.path.join(workdir, "img")]
     with open(os.path.join(workdir, "script.in"), "w") as f:
@@ -91,6 +93,10 @@ def run_invert(case, data, workdir, lenses=None, nrep=1, psf_mode="input", threa
     if psf_mode == "input":
         psf = np.asarray(case["psf"])
         args.append("psf=%s:%d:%d" % (os.path.join(workdir, "psf.bin"), psf.shape[0], psf.shape[1]))
+    if case.get("mask") is not None:
+        mk = np.ascontiguousarray(case["mask"], dtype=np.uint8)
+        mk.tofile(os.path.join(workdir, "mask.bin"))
+        args.append("mask=%s:%d:%d" % (os.path.join(workdir, "mask.bin"), mk.shape[0], mk.shape[1]))
     env = dict(os.environ)
     if threads:
         env["OMP_NUM_THREADS"] = str(threads)
@@ -111,6 +117,70 @@ def run_invert(case, data, workdir, lenses=None, nrep=1, psf_mode="input", threa
     if os.path.exists(os.path.join(workdir, "psf_used.bin")):
         arr["psf"] = np.fromfile(os.path.join(workdir, "psf_used.bin")).reshape(int(sc["psf_nx"]), int(sc["psf_ny"]))
     return sc, arr
+
+
+def _run(args, workdir, threads=None):
+    env = dict(os.environ)
+    if threads:
+        env["OMP_NUM_THREADS"] = str(threads)
+    r = subprocess.run(args, cwd=workdir, capture_output=True, text=True, env=env)
+    if r.returncode != 0:
+        raise RuntimeError("reference harness failed (%d): %s %s" % (r.returncode, r.stdout[-2000:], r.stderr[-2000:]))
+    return r
+
+
+def run_mock(case, workdir, sigma=None, seed=None):
+    """The reference's own mock image of a workload (ref_harness mock): the case's lenses and
+    Sersic sources on the no-data image grid spanned by its corners, its PSF matrix injected, and
+    pixel noise sigma from the reference generator seeded with seed (sigma 0: no noise)."""
+    os.makedirs(workdir, exist_ok=True)
+    xv, yv = case["xvals"], case["yvals"]
+    nx, ny = len(xv) - 1, len(yv) - 1
+    sigma = case["sigma"] if sigma is None else sigma
+    seed = case["seed"] if seed is None else seed
+    lines = [
+        "fit source_mode sbprofile",
+        "split_imgpixels on",
+        "imgpixel_nsplit %d" % case["nsplit"],
+        "grid %s %s %s %s" % (_fmt(xv[0]), _fmt(xv[-1]), _fmt(yv[0]), _fmt(yv[-1])),
+        "img_npixels %d %d" % (nx, ny),
+        "integral_tolerance %s" % _fmt(case["integral_tol"]),
+    ]
+    lines += lens_commands(case["lenses"])
+    for (s0, reff, n, q, th, xc, yc) in case["sources"]:
+        lines.append("source sersic %s %s %s %s %s %s %s" % tuple(_fmt(v) for v in (s0, reff, n, q, th, xc, yc)))
+    if sigma > 0:
+        lines += ["bg_pixel_noise %s" % _fmt(sigma), "simulate_pixel_noise on", "random_seed %d" % seed]
+    script = os.path.join(workdir, "script.in")
+    with open(script, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    psf = np.ascontiguousarray(case["psf"], dtype=np.float64)
+    psf.tofile(os.path.join(workdir, "psf.bin"))
+    _run([HARNESS, "mock", script, workdir, "psf=%s:%d:%d" % (os.path.join(workdir, "psf.bin"), psf.shape[0], psf.shape[1])],
+         workdir)
+    return np.fromfile(os.path.join(workdir, "mock.bin")).reshape(nx, ny)
+
+
+def run_rng(seed, n, workdir):
+    """First n normal deviates and n raw 64-bit draws of the reference generator (rand.h)."""
+    os.makedirs(workdir, exist_ok=True)
+    script = os.path.join(workdir, "script.in")
+    with open(script, "w") as f:
+        f.write("\n")
+    _run([HARNESS, "rng", script, workdir, str(int(seed)), str(int(n))], workdir)
+    return (np.fromfile(os.path.join(workdir, "deviates.bin")),
+            np.fromfile(os.path.join(workdir, "int64.bin"), dtype=np.uint64))
+
+
+def run_psf(sigma_x, sigma_y, threshold, xstep, ystep, workdir):
+    """QLens::generate_PSF_matrix for psf_width sigma_x sigma_y and psf_threshold on xstep x ystep pixels."""
+    os.makedirs(workdir, exist_ok=True)
+    script = os.path.join(workdir, "script.in")
+    with open(script, "w") as f:
+        f.write("psf_width %s %s\npsf_threshold %s\n" % (_fmt(sigma_x), _fmt(sigma_y), _fmt(threshold)))
+    _run([HARNESS, "psf", script, workdir, _fmt(xstep), _fmt(ystep)], workdir)
+    sc = read_scalars(workdir)
+    return np.fromfile(os.path.join(workdir, "psf.bin")).reshape(int(sc["psf_nx"]), int(sc["psf_ny"]))
 
 
 def run_deflect(lenses, xy, workdir, tol=1e-6):

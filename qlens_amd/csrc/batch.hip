@@ -1345,8 +1345,10 @@ void launch_prep_fr_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32)
 // or banded per matrix: per block step a diagonal kernel (factor + block inverse), a panel kernel
 // (A_ib,kb <- A_ib,kb W^T on MFMA) and a trailing-update kernel (A_IJ -= P_I P_J^T on MFMA).
 // Matrices with fewer blocks skip the step; the upper triangle is never written.
-// Restates Cholesky_dcmp_packed (pixelgrid.cpp:17071-17105) blockwise: a negative pivot of a row
-// < n sets *status and the factorization continues with sqrt(|a|) as the reference does.
+// Restates Cholesky_dcmp_packed (pixelgrid.cpp:17071-17105) blockwise: a pivot of a row < n that
+// is not positive (negative, zero or NaN) sets *status and the factorization continues with
+// sqrt(|a|) as that routine does; the item is then reported as not positive definite (the
+// reference's non-MKL caller dies there, pixelgrid.cpp:17384).
 // ------------------------------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ T* tile_ptr(const MatDesc& D, int I, int J, int& ld)
@@ -1418,7 +1420,7 @@ __device__ void chol_diag_wave_lds(const MatDesc& D, int kb, double* raw)
         if (r == c) col[c] = a[c];
         __syncthreads();
         const T piv = col[c];
-        bad |= (piv < 0 && base + c < D.n);
+        bad |= (!(piv > 0) && base + c < D.n);  // negative, zero or NaN pivot
         const T d = sqrt(fabs(piv));
         const T l = (r > c) ? a[c] / d : d;
         a[c] = l;
@@ -1491,7 +1493,7 @@ __device__ void chol_diag_wave_f32(const MatDesc& D, int kb, float (*Lt)[TS])
 #pragma unroll
     for (int c = 0; c < TS; c++) {
         const T piv = lane_bcast(a[c], c);
-        bad |= (piv < 0 && base + c < D.n);
+        bad |= (!(piv > 0) && base + c < D.n);  // negative, zero or NaN pivot
         const T pa = fabs(piv);
         const T rd = rsqrtf(pa);
         const T d = (pa > 0) ? pa * rd : (T)0;
@@ -2397,7 +2399,8 @@ void launch_trisolve_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32)
 // Wilkinson / Moler): r = D - F s in FP64 arithmetic on the FP32 F as formed (Ff32 + Fdiag),
 // dx = F^-1 r with the FP32 factor, s += dx.  The residual kernel writes per-row-block partial
 // sums of r^2 and D^2; the refine kernel stops an item once ||r|| <= tol ||D||.
-// status[4*item+2] counts steps, status[4*item+3] marks a converged item.
+// status[4*item+2] counts steps; status[4*item+3] marks a finished item: 1 converged, 2 stopped after
+// the last allowed step with the residual still above the tolerance.
 // ------------------------------------------------------------------------------------------------
 // sum of a[k] over the 64 lanes for 16 values at once: four halvings, then a 4-lane sum; lane l ends
 // with the total of a[(l >> 2) & 15]
@@ -2582,7 +2585,7 @@ __global__ void __launch_bounds__(1024) k_refine_b(BatchDev bd, int final_check,
         const double rel = (b > 0) ? sqrt(a / b) : 0.0;
         bd.out[QLX_OUT_STRIDE * item + 6] = rel;
         done = (rel <= tol || final_check) ? 1 : 0;
-        if (done) bd.status[4 * item + 3] = 1;
+        if (done) bd.status[4 * item + 3] = (rel <= tol) ? 1 : 2;  // 2: stopped above the tolerance
         else bd.status[4 * item + 2] += 1;
     }
     __syncthreads();
@@ -2762,6 +2765,37 @@ __global__ void __launch_bounds__(1024) k_evidence_b(BatchDev bd)
         o[5] = regterms;
         o[7] = (double)bd.status[4 * item + 2];
     }
+}
+
+__global__ void __launch_bounds__(256) k_keep_b(BatchDev bd, char* keep)
+{
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    if (!M.valid) return;
+    const int ncell = M.snx * M.sny;
+    double* ks = (double*)(keep + M.keep_off);
+    double* km = ks + ncell;
+    int* ki = (int*)(km + bd.s_pix);
+    const double* sref = bd.vec + M.vec_off + 6 * (size_t)M.ldN;
+    const double* model = bd.model + (size_t)item * 2 * bd.s_pix;
+    const int* sidx = bd.sidx + (size_t)item * bd.s_cell;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < bd.s_pix; t += stride) {
+        km[t] = model[t];
+        if (t < ncell) ki[t] = sidx[t];
+        if (t < M.n_src) ks[t] = sref[t];
+    }
+    for (long long t = bd.s_pix + (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += stride) {
+        ki[t] = sidx[t];
+        if (t < M.n_src) ks[t] = sref[t];
+    }
+}
+
+void launch_keep_b(hipStream_t s, const BatchDev& bd, void* keep, int max_cells)
+{
+    const long long n = std::max<long long>(bd.s_pix, max_cells);
+    const unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 64);
+    hipLaunchKernelGGL(k_keep_b, dim3(nb, bd.B), dim3(256), 0, s, bd, (char*)keep);
 }
 
 void launch_evidence_b(hipStream_t st, const BatchDev& bd, int max_img, int max_src)

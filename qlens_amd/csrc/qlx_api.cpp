@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -26,6 +27,8 @@ namespace {
 const double DEG2RAD = 0.017453292519943295;  // mathexpr.h:27
 const double HALFPI_REF = 1.57079632679;       // mathexpr.h:10 (the reference's truncated pi/2)
 
+std::atomic<size_t> g_dev_bytes{0};  // device bytes held by every DevBuf of this process
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -33,6 +36,7 @@ struct DevBuf {
     void release()
     {
         if (p) hipFree(p);
+        g_dev_bytes -= bytes;
         p = nullptr;
         bytes = 0;
     }
@@ -47,6 +51,7 @@ struct DevBuf {
             return false;
         }
         bytes = n;
+        g_dev_bytes += n;
         return true;
     }
     template <typename T> T* as() const { return (T*)p; }
@@ -90,16 +95,23 @@ struct Ctx {
     DevBuf key, bcnt, boff, bfill, perm;
     // pools
     DevBuf lb, F, Ff, R, Rf, cvec, vec, nz, bbase, winv, model, out, status;  // Rf: R's copy for its factorization
-    // last solution
-    // the last valid item's solution, kept on the device (copied there at the end of a call, read
-    // back only by qlx_get_solution): source amplitudes, the cell map and the model image
+    // solutions of the last call, kept on the device for qlx_get_source: per caller item a block
+    // at kept[item].off of [cells doubles: source amplitudes in the reference order]
+    // [s_pix doubles: model image] [cells ints: reference active index of each cell]
     DevBuf keep;
-    int keep_nsrc = 0, keep_ncell = 0;
+    struct Kept {
+        int ok, n_src, snx, sny;
+        size_t off;
+    };
+    std::vector<Kept> kept;
     size_t keep_npix = 0;
+    std::vector<int> last_wave;  // caller indices of the last wave's items, in wave order (their rays
+                                 // stay in `beta` until the next evaluation)
     std::vector<unsigned> nzh;  // host copy of the support masks (work accounting)
     // timers
     std::map<std::string, Timer> timers;
     qlx_stats stats{};
+    int stats_waves = 0;  // waves of the last evaluation
     // learned deflection surrogate (surrogate_host.cpp) and its per-ray outputs
     void* sur = nullptr;
     DevBuf sur_alpha;
@@ -230,17 +242,35 @@ void prep_model(const qlx_lens_model& in, ModelPrep& out)
 
 namespace {
 
-int check_model(Ctx* c, const qlx_lens_model* m)
+// Parameter values inside each profile's domain: every parameter finite, SIE q in (0, 1], NFW
+// q in (0, 1] and rs > 0.  The reference itself has no such check (a sampler's prior limits keep
+// proposals in range, lens.cpp:10657-10748); here a bad proposal fails only its own item.
+const char* lens_params_error(const qlx_lens_model& m)
+{
+    for (int l = 0; l < m.nlens; l++) {
+        const qlx_lens& L = m.lens[l];
+        for (int i = 0; i < 6; i++)
+            if (!std::isfinite(L.p[i])) return "lens parameter is not finite";
+        if (L.type == QLX_LENS_SIE && !(L.p[1] > 0 && L.p[1] <= 1)) return "SIE q must be in (0,1]";
+        if (L.type == QLX_LENS_NFW && !(L.p[2] > 0 && L.p[2] <= 1 && L.p[1] > 0)) return "NFW needs rs > 0 and q in (0,1]";
+    }
+    return nullptr;
+}
+
+// Structure of a model (lens count and types, deflection modes, a loaded surrogate where one is
+// asked for); with `values`, also the parameter domain (lens_params_error).
+int check_model(Ctx* c, const qlx_lens_model* m, bool values = true)
 {
     if (!m || m->nlens < 0 || m->nlens > QLX_MAX_LENSES) return fail(c, QLX_ERR_ARG, "invalid lens model");
     for (int l = 0; l < m->nlens; l++) {
         int t = m->lens[l].type;
         if (t != QLX_LENS_SIE && t != QLX_LENS_NFW && t != QLX_LENS_SHEAR) return fail(c, QLX_ERR_ARG, "unknown lens type");
-        if (t == QLX_LENS_SIE && !(m->lens[l].p[1] > 0 && m->lens[l].p[1] <= 1)) return fail(c, QLX_ERR_ARG, "SIE q must be in (0,1]");
-        if (t == QLX_LENS_NFW && !(m->lens[l].p[2] > 0 && m->lens[l].p[2] <= 1 && m->lens[l].p[1] > 0))
-            return fail(c, QLX_ERR_ARG, "NFW needs rs > 0 and q in (0,1]");
         if (m->lens[l].def_mode != QLX_DEF_QUADRATURE && m->lens[l].def_mode != QLX_DEF_SURROGATE)
             return fail(c, QLX_ERR_ARG, "unknown deflection mode");
+    }
+    if (values) {
+        const char* e = lens_params_error(*m);
+        if (e) return fail(c, QLX_ERR_ARG, e);
     }
     int nsur = 0;
     for (int l = 0; l < m->nlens; l++)
@@ -288,8 +318,14 @@ struct WaveItem {
     int snx, sny;
 };
 
-// One wave of items through the whole pipeline.  Results go to out[item.idx].
-int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& items, qlx_result* out, bool keep_last)
+// run_wave's answer when the wave's exact dense and Lb-pool bytes exceed the budget: the caller
+// evaluates the same items again in smaller waves (an item's result does not depend on its wave)
+const int WAVE_TOO_BIG = 100;
+
+// One wave of items through the whole pipeline.  Results go to out[item.idx]; each valid item's
+// solution is copied to its block of the keep buffer.  With more than one item, a wave whose
+// exact dense + Lb bytes exceed `budget` returns WAVE_TOO_BIG before allocating them.
+int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& items, qlx_result* out, size_t budget)
 {
     const int B = (int)items.size();
     if (B == 0) return QLX_OK;
@@ -328,6 +364,8 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
               c->perm.ensure(B * s_pix * 4) && c->model.ensure(B * s_pix * 16) && c->out.ensure((size_t)B * QLX_OUT_STRIDE * 8) &&
               c->status.ensure((size_t)B * 16);
     if (!ok) return fail(c, QLX_ERR_NOMEM, "work buffer allocation failed");
+    c->last_wave.clear();
+    for (int b = 0; b < B; b++) c->last_wave.push_back(items[b].idx);
     HIPCHK(c, hipMemcpyAsync(c->d_models.p, mp.data(), sizeof(ModelPrep) * B, hipMemcpyHostToDevice, st));
     HIPCHK(c, hipMemcpyAsync(c->d_sgrid.p, sg.data(), sg.size() * 8, hipMemcpyHostToDevice, st));
     HIPCHK(c, hipMemcpyAsync(c->d_srcn.p, srcn.data(), srcn.size() * 4, hipMemcpyHostToDevice, st));
@@ -451,10 +489,12 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
             continue;
         }
         if (n_src + 1 > QLX_MAX_LD) {
+            r->minus2lnE = 2e30;
             r->status = QLX_ERR_RANGE;
             continue;
         }
         M.valid = 1;
+        M.keep_off = (long long)c->kept[items[b].idx].off;
         M.cbs = mixed ? 7 : 6;  // Lb column blocks of 128 (mixed) or 64 (FP64) source pixels
         M.ldN = (int)round_up((size_t)n_src + 1, 128);  // 128: whole tiles of the trailing Cholesky update
         M.nrows = (int)std::max<size_t>(64, round_up((size_t)n_img, 64));
@@ -492,6 +532,9 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         timers_collect(c);
         return QLX_OK;
     }
+    const size_t fixed_bytes = (size_t)B * stage1_bytes(s_pix, s_cell, nsub);
+    const size_t dense_bytes = f_tot * 8 + r_tot * 16 + cv_tot * 8 + vec_tot * 8 + nz_tot * 4 + bb_tot * 4 + w_tot * 8;
+    if (B > 1 && fixed_bytes + dense_bytes > budget) return WAVE_TOO_BIG;
     ok = c->R.ensure(r_tot * 8) && c->Rf.ensure(r_tot * 8) && c->cvec.ensure(cv_tot * 8) && c->vec.ensure(vec_tot * 8) && c->nz.ensure(nz_tot * 4) &&
          c->bbase.ensure(bb_tot * 4) && c->winv.ensure(w_tot * 8) && c->F.ensure(f_tot * (mixed ? 4 : 8)) &&
          (!mixed || c->Ff.ensure(f_tot * 4));
@@ -561,11 +604,14 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         lb_tot += (size_t)M.nblk * ((size_t)64 << M.cbs);
         max_nsup = std::max(max_nsup, M.nsup_max);
     }
+    if (B > 1 && fixed_bytes + dense_bytes + lb_tot * (mixed ? 2 : 8) > budget) return WAVE_TOO_BIG;
     if (!c->lb.ensure(lb_tot * (mixed ? 2 : 8))) return fail(c, QLX_ERR_NOMEM, "Lb pool allocation failed");
     bd.lb = c->lb.p;
     HIPCHK(c, hipMemcpyAsync(c->d_meta.p, meta.data(), sizeof(ItemMeta) * B, hipMemcpyHostToDevice, st));
+    timer_begin(c, "lbkernel");
     if (launch_build_lb_b(st, bd, max_rows, max_nsup, mixed ? 1 : 0, mixed ? 128 : 64) != 0)
         return fail(c, QLX_ERR_RANGE, "PSF too large for the Lb build (at most 8192 taps in FP64, 16384 mixed)");
+    timer_end(c, "lbkernel");
     timer_end(c, "lbuild");
     // R depends only on the active cells: it is built now and factored (ln det R only) on the second
     // stream, from a copy, while the main stream forms F (the SYRK, then k_prep_f_b, which reads R
@@ -603,6 +649,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     timer_end(c, "solve");
     launch_evidence_b(st, bd, max_img, max_src);
     timer_end(c, "total");
+    launch_keep_b(st, bd, c->keep.p, max_cells);
 
     std::vector<double> o((size_t)B * QLX_OUT_STRIDE);
     std::vector<int> stat((size_t)B * 4);
@@ -623,7 +670,17 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         r->Es = ob[4];
         r->refine_resid = ob[6];
         r->refine_steps = (int)ob[7];
-        r->status = (stat[4 * b] || stat[4 * b + 1]) ? QLX_ERR_NOT_PD : QLX_OK;
+        r->status = QLX_OK;
+        if (stat[4 * b + 3] == 2) r->status = QLX_ERR_REFINE;  // value kept, accurate to refine_resid
+        if (stat[4 * b] || stat[4 * b + 1] || !std::isfinite(r->minus2lnE)) {
+            r->status = QLX_ERR_NOT_PD;  // the reference dies here (pixelgrid.cpp:17384)
+            r->minus2lnE = 2e30;
+        }
+        Ctx::Kept& K = c->kept[items[b].idx];
+        K.ok = (r->status == QLX_OK || r->status == QLX_ERR_REFINE) ? 1 : 0;
+        K.n_src = meta[b].n_src;
+        K.snx = meta[b].snx;
+        K.sny = meta[b].sny;
     }
 
     // ---- work accounting over the wave (the launches timed above) ----
@@ -657,31 +714,6 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         }
     }
 
-    // ---- keep the last valid item's solution for qlx_get_solution ----
-    if (keep_last) {
-        int lb = -1;
-        for (int b = B - 1; b >= 0; b--)
-            if (meta[b].valid) {
-                lb = b;
-                break;
-            }
-        if (lb >= 0) {
-            const ItemMeta& M = meta[lb];
-            const int ncell = M.snx * M.sny;
-            const size_t bytes = sizeof(double) * ((size_t)M.n_src + s_pix) + sizeof(int) * (size_t)ncell;
-            if (!c->keep.ensure(bytes)) return fail(c, QLX_ERR_NOMEM, "solution buffer allocation failed");
-            char* k = (char*)c->keep.p;
-            HIPCHK(c, hipMemcpyAsync(k, bd.vec + M.vec_off + 6 * (size_t)M.ldN, sizeof(double) * M.n_src,
-                                     hipMemcpyDeviceToDevice, st));
-            HIPCHK(c, hipMemcpyAsync(k + sizeof(double) * M.n_src, bd.model + (size_t)lb * 2 * s_pix, sizeof(double) * s_pix,
-                                     hipMemcpyDeviceToDevice, st));
-            HIPCHK(c, hipMemcpyAsync(k + sizeof(double) * ((size_t)M.n_src + s_pix), bd.sidx + (size_t)lb * s_cell,
-                                     sizeof(int) * ncell, hipMemcpyDeviceToDevice, st));
-            c->keep_nsrc = M.n_src;
-            c->keep_ncell = ncell;
-            c->keep_npix = s_pix;
-        }
-    }
     return QLX_OK;
 }
 
@@ -692,25 +724,42 @@ int eval_items(Ctx* c, int nbatch, const qlx_lens_model* models, const int* src_
         kv.second.total_ms = 0;
         kv.second.launches = 0;
     }
+    c->kept.assign(nbatch, Ctx::Kept{0, 0, 0, 0, 0});
+    c->last_wave.clear();
+    c->stats_waves = 0;
     const bool mixed = c->set.precision == QLX_PREC_MIXED;
     const int nsub = c->set.nsplit * c->set.nsplit;
     const size_t s_pix = (size_t)c->nx * c->ny;
     std::vector<WaveItem> todo;
-    int first_bad = QLX_OK;
     for (int b = 0; b < nbatch; b++) {
-        int rc = check_model(c, &models[b]);
+        int rc = check_model(c, &models[b], false);
         if (rc) return rc;
         const int snx = src_n[2 * b], sny = src_n[2 * b + 1];
         if (snx > QLX_MAX_SRC_AXIS || sny > QLX_MAX_SRC_AXIS) return fail(c, QLX_ERR_RANGE, "source grid larger than 512 per axis");
+        memset(&out[b], 0, sizeof(out[b]));
         if (snx < 2 || sny < 2) {  // lens.cpp:12453-12458
-            memset(&out[b], 0, sizeof(out[b]));
             out[b].minus2lnE = 2e30;
             out[b].status = QLX_ERR_NOSOURCE;
             continue;
         }
+        if (lens_params_error(models[b])) {
+            out[b].minus2lnE = 2e30;
+            out[b].status = QLX_ERR_ARG;
+            continue;
+        }
         todo.push_back({b, snx, sny});
     }
-    // admission: items per wave from the budget (ray/row arrays + a dense bound + an Lb estimate)
+    // every item's solution block in the keep buffer (upper bound: n_src <= cells)
+    size_t keep_tot = 0;
+    for (auto& it : todo) {
+        const size_t cells = (size_t)it.snx * it.sny;
+        c->kept[it.idx].off = keep_tot;
+        keep_tot += round_up(cells * 12 + s_pix * 8, 256);
+    }
+    c->keep_npix = s_pix;
+    if (!c->keep.ensure(keep_tot)) return fail(c, QLX_ERR_NOMEM, "solution buffer allocation failed");
+    // admission: items per wave from the budget (ray/row arrays + a dense bound + an Lb estimate);
+    // run_wave checks the exact dense and Lb bytes and the wave is halved when they do not fit
     size_t max_cell = 4, max_snx = 2;
     for (auto& it : todo) {
         max_cell = std::max(max_cell, (size_t)it.snx * it.sny);
@@ -722,19 +771,29 @@ int eval_items(Ctx* c, int nbatch, const qlx_lens_model* models, const int* src_
     const size_t per_item = stage1_bytes(s_pix, max_cell, nsub) + dense_bytes_bound(ncell_bound, max_snx) + lb_est;
     size_t budget = c->hbm_budget;
     if (budget == 0) {
+        // 85 % of what is free plus what this process's buffers already hold (they are reused)
         size_t fr = 0, tot = 0;
-        if (hipMemGetInfo(&fr, &tot) == hipSuccess) budget = (size_t)(0.85 * (double)fr);
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess) budget = (size_t)(0.85 * (double)(fr + g_dev_bytes.load()));
         else budget = (size_t)16 << 30;
     }
-    int wave = (int)std::max<size_t>(1, std::min<size_t>(budget / std::max<size_t>(per_item, 1), 1024));
-    for (size_t w0 = 0; w0 < todo.size(); w0 += wave) {
-        std::vector<WaveItem> items(todo.begin() + w0, todo.begin() + std::min(todo.size(), w0 + wave));
-        int rc = run_wave(c, models, items, out, w0 + wave >= todo.size());
+    budget = budget > keep_tot ? budget - keep_tot : 0;
+    size_t wave = std::max<size_t>(1, std::min<size_t>(budget / std::max<size_t>(per_item, 1), 1024));
+    size_t w0 = 0;
+    while (w0 < todo.size()) {
+        const size_t n = std::min(wave, todo.size() - w0);
+        std::vector<WaveItem> items(todo.begin() + w0, todo.begin() + w0 + n);
+        int rc = run_wave(c, models, items, out, budget);
+        if (rc == WAVE_TOO_BIG) {
+            wave = std::max<size_t>(1, n / 2);
+            continue;
+        }
         if (rc) return rc;
+        c->stats_waves++;
+        w0 += n;
     }
     for (int b = 0; b < nbatch; b++)
-        if (out[b].status != QLX_OK && first_bad == QLX_OK) first_bad = out[b].status;
-    return first_bad;
+        if (out[b].status != QLX_OK) return QLX_ITEMS_FAILED;
+    return QLX_OK;
 }
 
 }  // namespace
@@ -797,6 +856,10 @@ int qlx_set_image(void* ctx, int nx, int ny, const double* xvals, const double* 
 {
     Ctx* c = (Ctx*)ctx;
     if (!c) return QLX_ERR_ARG;
+    c->kept.clear();  // solutions of earlier calls no longer match the inputs
+    c->last_wave.clear();
+    c->last_wave.clear();
+    c->last_wave.clear();
     if (nx < 1 || ny < 1 || !xvals || !yvals || !data || !covinv) return fail(c, QLX_ERR_ARG, "qlx_set_image: bad arguments");
     if ((long long)nx * ny > (1LL << 24)) return fail(c, QLX_ERR_RANGE, "image too large");
     if (nx >= (1 << 18)) return fail(c, QLX_ERR_RANGE, "image wider than 262143 pixels in x");
@@ -834,6 +897,10 @@ int qlx_set_psf(void* ctx, int pnx, int pny, const double* psf)
 {
     Ctx* c = (Ctx*)ctx;
     if (!c) return QLX_ERR_ARG;
+    c->kept.clear();  // solutions of earlier calls no longer match the inputs
+    c->last_wave.clear();
+    c->last_wave.clear();
+    c->last_wave.clear();
     if (pnx < 1 || pny < 1 || !psf || (pnx % 2) == 0 || (pny % 2) == 0) return fail(c, QLX_ERR_ARG, "PSF must be odd-sized and non-empty");
     if ((long long)pnx * pny > 4096 * 4096) return fail(c, QLX_ERR_RANGE, "PSF too large");
     hipSetDevice(c->device);
@@ -886,6 +953,10 @@ int qlx_set_settings(void* ctx, const qlx_settings* s)
 {
     Ctx* c = (Ctx*)ctx;
     if (!c) return QLX_ERR_ARG;
+    c->kept.clear();  // solutions of earlier calls no longer match the inputs
+    c->last_wave.clear();
+    c->last_wave.clear();
+    c->last_wave.clear();
     if (!s) return fail(c, QLX_ERR_ARG, "null settings");
     if (s->nsplit < 1 || s->nsplit > 4) return fail(c, QLX_ERR_RANGE, "nsplit must be 1..4");
     if (!(s->src_xmax > s->src_xmin) || !(s->src_ymax > s->src_ymin)) return fail(c, QLX_ERR_ARG, "empty source grid bounds");
@@ -923,22 +994,61 @@ int qlx_eval_batch_grids(void* ctx, int nbatch, const qlx_lens_model* models, co
     return eval_items(c, nbatch, models, src_n, out);
 }
 
-int qlx_get_solution(void* ctx, double* src_cells, double* model_image)
+int qlx_get_source(void* ctx, int item, int* src_nx, int* src_ny, double* src_cells, int* active_index, size_t src_cap,
+                   double* model_image, size_t model_cap)
 {
     Ctx* c = (Ctx*)ctx;
     if (!c) return QLX_ERR_ARG;
-    if (c->keep_ncell == 0) return fail(c, QLX_ERR_STATE, "no evaluation yet");
-    std::vector<double> s(c->keep_nsrc), model(c->keep_npix);
-    std::vector<int> sidx(c->keep_ncell);
-    const char* k = (const char*)c->keep.p;
-    HIPCHK(c, hipMemcpyAsync(s.data(), k, sizeof(double) * s.size(), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(model.data(), k + sizeof(double) * s.size(), sizeof(double) * model.size(), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(sidx.data(), k + sizeof(double) * (s.size() + model.size()), sizeof(int) * sidx.size(),
-                             hipMemcpyDeviceToHost, c->stream));
+    if (item < 0 || item >= (int)c->kept.size()) return fail(c, QLX_ERR_ARG, "qlx_get_source: item out of range of the last call");
+    const Ctx::Kept& K = c->kept[item];
+    if (!K.ok) return fail(c, QLX_ERR_STATE, "qlx_get_source: the item has no solution (its status was an error)");
+    const size_t ncell = (size_t)K.snx * K.sny, npix = c->keep_npix;
+    if (src_nx) *src_nx = K.snx;
+    if (src_ny) *src_ny = K.sny;
+    if ((src_cells || active_index) && src_cap < ncell) return fail(c, QLX_ERR_ARG, "qlx_get_source: source buffer smaller than the item's grid");
+    if (model_image && model_cap < npix) return fail(c, QLX_ERR_ARG, "qlx_get_source: model buffer smaller than the image");
+    hipSetDevice(c->device);
+    const char* k = (const char*)c->keep.p + K.off;
+    std::vector<double> s((size_t)K.n_src);
+    std::vector<int> sidx(ncell);
+    if (src_cells) HIPCHK(c, hipMemcpyAsync(s.data(), k, sizeof(double) * s.size(), hipMemcpyDeviceToHost, c->stream));
+    if (src_cells || active_index)
+        HIPCHK(c, hipMemcpyAsync(sidx.data(), k + sizeof(double) * (ncell + npix), sizeof(int) * ncell, hipMemcpyDeviceToHost, c->stream));
+    if (model_image) HIPCHK(c, hipMemcpyAsync(model_image, k + sizeof(double) * ncell, sizeof(double) * npix, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (src_cells)
-        for (size_t q = 0; q < sidx.size(); q++) src_cells[q] = sidx[q] >= 0 ? s[sidx[q]] : 0.0;
-    if (model_image) memcpy(model_image, model.data(), sizeof(double) * model.size());
+        for (size_t q = 0; q < ncell; q++) src_cells[q] = sidx[q] >= 0 ? s[sidx[q]] : 0.0;
+    if (active_index) memcpy(active_index, sidx.data(), sizeof(int) * ncell);
+    return QLX_OK;
+}
+
+int qlx_get_rays(void* ctx, int item, double* beta, size_t cap)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !beta) return QLX_ERR_ARG;
+    const size_t n = (size_t)c->nx * c->ny * c->set.nsplit * c->set.nsplit * 2;
+    if (cap < n) return fail(c, QLX_ERR_ARG, "qlx_get_rays: buffer smaller than nx*ny*nsplit^2*2");
+    int pos = -1;
+    for (size_t w = 0; w < c->last_wave.size(); w++)
+        if (c->last_wave[w] == item) pos = (int)w;
+    if (pos < 0) return fail(c, QLX_ERR_STATE, "qlx_get_rays: the item was not in the last wave of the last evaluation");
+    hipSetDevice(c->device);
+    HIPCHK(c, hipMemcpyAsync(beta, c->beta.as<double>() + (size_t)pos * n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return QLX_OK;
+}
+
+int qlx_get_psf(void* ctx, int* pnx, int* pny, double* psf, size_t cap)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return QLX_ERR_ARG;
+    if (!c->psf_set) return fail(c, QLX_ERR_STATE, "no PSF set");
+    if (pnx) *pnx = c->pnx;
+    if (pny) *pny = c->pny;
+    if (psf) {
+        if (cap < c->h_psf.size()) return fail(c, QLX_ERR_ARG, "qlx_get_psf: buffer smaller than the PSF");
+        memcpy(psf, c->h_psf.data(), c->h_psf.size() * sizeof(double));
+    }
     return QLX_OK;
 }
 
@@ -1101,6 +1211,7 @@ int qlx_get_stats(void* ctx, qlx_stats* out)
     Ctx* c = (Ctx*)ctx;
     if (!c || !out) return QLX_ERR_ARG;
     *out = c->stats;
+    out->n_waves = c->stats_waves;
     return QLX_OK;
 }
 

@@ -66,6 +66,7 @@ struct ItemMeta {
     long long f_off;                 // F (elements of F or F32)
     long long r_off;                 // banded R tiles (doubles)
     long long cvec_off, vec_off, nz_off, bb_off, w_off;
+    long long keep_off;              // byte offset of the item's solution block in the keep buffer
 };
 
 struct BatchDev {
@@ -236,6 +237,9 @@ void launch_logdets_b(hipStream_t s, const BatchDev& bd, int f32);
 void launch_trisolve_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32);
 void launch_refine_b(hipStream_t s, const BatchDev& bd, int max_ld, int iters, double tol);
 void launch_evidence_b(hipStream_t s, const BatchDev& bd, int max_img, int max_src);
+// copy each valid item's source amplitudes (reference order), model image and cell map into its
+// block of `keep` at meta.keep_off: [cells doubles] [s_pix doubles] [cells ints]
+void launch_keep_b(hipStream_t s, const BatchDev& bd, void* keep, int max_cells);
 void launch_selftest_mfma(hipStream_t s, double* out);
 
 }  // namespace qlx

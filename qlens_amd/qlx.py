@@ -20,8 +20,11 @@ QLX_PREC_FP64, QLX_PREC_MIXED = 0, 1
 QLX_RAY_FP64, QLX_RAY_FP32 = 0, 1
 QLX_MAX_LENSES = 8
 
-STATUS = {0: "ok", -1: "bad argument", -2: "bad state", -3: "HIP error", -4: "out of memory",
-          -5: "not positive definite", -6: "no source pixels", -7: "out of range"}
+STATUS = {0: "ok", 1: "some items failed", -1: "bad argument", -2: "bad state", -3: "HIP error",
+          -4: "out of memory", -5: "not positive definite", -6: "no source pixels", -7: "out of range",
+          -8: "refinement above tolerance"}
+QLX_ITEMS_FAILED = 1
+SENTINEL = 2e30  # minus2lnE of an item whose status is an error (lens.cpp:12453-12458)
 
 
 class QlxLens(C.Structure):
@@ -72,11 +75,18 @@ def load_library(path=None):
     lib.qlx_last_error.restype = C.c_char_p
     lib.qlx_last_error.argtypes = [C.c_void_p]
     for name in ["qlx_ctx_create", "qlx_ctx_destroy", "qlx_set_image", "qlx_set_psf", "qlx_set_psf_gaussian",
-                 "qlx_set_settings", "qlx_eval", "qlx_eval_batch_grids", "qlx_get_solution", "qlx_raytrace",
+                 "qlx_set_settings", "qlx_eval", "qlx_eval_batch_grids", "qlx_get_source", "qlx_raytrace",
                  "qlx_nfw_def_hess", "qlx_mock_image", "qlx_kernel_time", "qlx_selftest_mfma",
                  "qlx_surrogate_train", "qlx_surrogate_eval", "qlx_surrogate_grad", "qlx_surrogate_param_count",
-                 "qlx_surrogate_get_params", "qlx_surrogate_set_params", "qlx_surrogate_save", "qlx_surrogate_load"]:
+                 "qlx_surrogate_get_params", "qlx_surrogate_set_params", "qlx_surrogate_save", "qlx_surrogate_load",
+                 "qlx_get_stats"]:
         getattr(lib, name).restype = C.c_int
+    lib.qlx_get_rays.restype = C.c_int
+    lib.qlx_get_rays.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_size_t]
+    lib.qlx_get_psf.restype = C.c_int
+    lib.qlx_get_psf.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_size_t]
+    lib.qlx_get_source.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int), C.c_size_t, C.POINTER(C.c_double), C.c_size_t]
     if path is None:
         _LIB = lib
     return lib
@@ -143,10 +153,12 @@ class QlxError(RuntimeError):
 class PixelInversion:
     """One qlx context (one device stream) holding the image, PSF and inversion settings."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, hbm_budget=0):
+        """hbm_budget: bytes the context may use for batch admission (0 = 85% of free HBM); a
+        smaller budget splits a call into more waves (same results)."""
         self.lib = load_library()
         self.ctx = C.c_void_p()
-        opts = QlxOpts(device, 0)
+        opts = QlxOpts(device, int(hbm_budget))
         rc = self.lib.qlx_ctx_create(C.byref(opts), C.byref(self.ctx))
         if rc != 0:
             raise QlxError("qlx_ctx_create failed (%s): %s" % (STATUS.get(rc, rc), self.lib.qlx_last_error(None).decode()))
@@ -222,17 +234,50 @@ class PixelInversion:
         else:
             sn = np.ascontiguousarray(np.asarray(src_n, dtype=np.int32).reshape(-1, 2))
             rc = self.lib.qlx_eval_batch_grids(self.ctx, len(models), arr, sn.ctypes.data_as(C.POINTER(C.c_int)), res)
-        if rc < 0 and rc not in (-5, -6):
+        if rc < 0:
             self._check(rc, "qlx_eval")
+        self._last_n = len(models)
         names = _RESULT_DT.names
         return [dict(zip(names, row)) for row in np.frombuffer(res, dtype=_RESULT_DT)[:len(models)].tolist()]
 
-    def get_solution(self):
-        nx, ny = self.settings["src_n"]
-        src = np.zeros((nx, ny))
+    def get_source(self, item=-1):
+        """Item `item` of the last evaluation (negative: from the end), as update_source_amplitudes
+        leaves it (pixelgrid.cpp:17850-17881): dict with 'src' [snx, sny] (0 off the active set),
+        'active_index' [snx, sny] (-1 inactive) and 'model' [nx, ny]."""
+        if item < 0:
+            item += getattr(self, "_last_n", 0)
+        snx, sny = C.c_int(0), C.c_int(0)
+        self._check(self.lib.qlx_get_source(self.ctx, int(item), C.byref(snx), C.byref(sny), None, None, 0, None, 0),
+                    "qlx_get_source")
+        src = np.zeros((snx.value, sny.value))
+        act = np.zeros((snx.value, sny.value), dtype=np.int32)
         model = np.zeros((self.nx, self.ny))
-        self._check(self.lib.qlx_get_solution(self.ctx, _dp(src), _dp(model)), "qlx_get_solution")
-        return src, model
+        self._check(self.lib.qlx_get_source(self.ctx, int(item), None, None, _dp(src),
+                                            act.ctypes.data_as(C.POINTER(C.c_int)), src.size, _dp(model), model.size),
+                    "qlx_get_source")
+        return {"src": src, "active_index": act, "model": model}
+
+    def get_rays(self, item=-1):
+        """Subpixel source-plane points of an item of the last evaluation: [nx, ny, nsplit^2, 2]."""
+        if item < 0:
+            item += getattr(self, "_last_n", 0)
+        ns = self.settings["nsplit"]
+        out = np.zeros((self.nx, self.ny, ns * ns, 2))
+        self._check(self.lib.qlx_get_rays(self.ctx, int(item), _dp(out), out.size), "qlx_get_rays")
+        return out
+
+    def get_psf(self):
+        """The PSF matrix in use, [pnx, pny]."""
+        nx, ny = C.c_int(0), C.c_int(0)
+        self._check(self.lib.qlx_get_psf(self.ctx, C.byref(nx), C.byref(ny), None, 0), "qlx_get_psf")
+        psf = np.zeros((nx.value, ny.value))
+        self._check(self.lib.qlx_get_psf(self.ctx, None, None, _dp(psf), psf.size), "qlx_get_psf")
+        return psf
+
+    def get_solution(self, item=-1):
+        """(source amplitudes [snx, sny], model image [nx, ny]) of an item of the last evaluation."""
+        r = self.get_source(item)
+        return r["src"], r["model"]
 
     def raytrace(self, lenses, xy):
         xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
@@ -319,12 +364,11 @@ class QlxStats(C.Structure):
     _fields_ = [("syrk_dense_flops", C.c_double), ("syrk_exec_flops", C.c_double), ("syrk_alg_flops", C.c_double),
                 ("lb_bytes", C.c_double), ("integrand_evals", C.c_double), ("chol_flops", C.c_double),
                 ("n_items", C.c_int), ("n_img", C.c_int), ("n_src", C.c_int), ("ld_src", C.c_int),
-                ("rows_pad", C.c_int)]
+                ("rows_pad", C.c_int), ("n_waves", C.c_int)]
 
 
 def get_stats(inv):
     """Work counters of the last wave of the last evaluation (qlx_get_stats)."""
     s = QlxStats()
-    inv.lib.qlx_get_stats.restype = C.c_int
     inv._check(inv.lib.qlx_get_stats(inv.ctx, C.byref(s)), "qlx_get_stats")
     return {f: getattr(s, f) for f, _ in QlxStats._fields_}

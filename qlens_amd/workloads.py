@@ -17,8 +17,10 @@ Sizes and parameter values follow the five workloads of the benchmark descriptio
   C1 SIE+shear 100^2 image / 30^2 source, Gaussian PSF, FP64
   C2 NFW(quadrature)+SIE+shear 200^2 / 60^2, Moffat 21x21 PSF
   C3 surrogate deflection evaluation on a 400^2 x 4x4 image
-  C4 256 proposals around C1's truth on a 200^2 image with uneven source grids
-  C5 400^2 image / 100^2 source, mixed precision
+  C4 256 proposals around C1's truth on a 200^2 image with uneven source grids (c4; c4_nfw is
+     the same batch shape on C2's numerical-profile truth)
+  C5 400^2 image / 100^2 source, mixed precision, batches of 16
+  CNS a non-square, masked, anisotropic-PSF orientation check (c_ns)
 """
 import numpy as np
 
@@ -50,6 +52,29 @@ def gaussian_psf(sigma, threshold, xstep, ystep):
         y = -ymax
         for j in range(ny):
             psf[i, j] = np.exp(-0.5 * ((x / sigma) ** 2 + (y / sigma) ** 2))
+            y += ystep
+        x += xstep
+    return psf / psf.sum()
+
+
+def gaussian_psf_xy(sigma_x, sigma_y, threshold, xstep, ystep):
+    """gaussian_psf with separate widths along x and y (QLens 'psf_width <x> <y>')."""
+    frac = np.sqrt(-2 * np.log(threshold))
+    nxd, nyd = frac * sigma_x / xstep, frac * sigma_y / ystep
+    nxh, nyh = int(nxd), int(nyd)
+    if nxd - nxh > 0.5:
+        nxh += 1
+    if nyd - nyh > 0.5:
+        nyh += 1
+    nx, ny = 2 * nxh + 1, 2 * nyh + 1
+    xmax = ((2 * nxh + 1) * xstep - xstep) / 2
+    ymax = ((2 * nyh + 1) * ystep - ystep) / 2
+    psf = np.zeros((nx, ny))
+    x = -xmax
+    for i in range(nx):
+        y = -ymax
+        for j in range(ny):
+            psf[i, j] = np.exp(-0.5 * ((x / sigma_x) ** 2 + (y / sigma_y) ** 2))
             y += ystep
         x += xstep
     return psf / psf.sum()
@@ -136,6 +161,47 @@ def c5(n_img=400, pix=0.02, n_src=100):
     return case
 
 
+def c_ns():
+    """Orientation check: a non-square image (90 x 72 pixels of 0.05" x 0.045"), an anisotropic
+    Gaussian PSF (sigma 0.06" in x, 0.035" in y, threshold 0.015, so 7 x 5 taps), an elliptical
+    annulus mask with a rectangular hole, off-centre SIE + NFW + shear, and a non-square 28 x 22
+    source grid.  No symmetry of the other workloads (square images, circular PSFs, full masks,
+    square grids) can hide an x/y mix-up here."""
+    xv = corners(-2.3, 2.2, 90)
+    yv = corners(-1.62, 1.62, 72)
+    xc = 0.5 * (xv[:-1] + xv[1:])
+    yc = 0.5 * (yv[:-1] + yv[1:])
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    r = np.sqrt(X ** 2 + (Y / 0.8) ** 2)
+    mask = (r > 0.45) & (r < 1.95) & ~((X > 0.9) & (X < 1.3) & (Y > -0.3) & (Y < 0.2))
+    return dict(
+        name="CNS", xvals=xv, yvals=yv, nsplit=2,
+        lenses=[(SIE, (1.1, 0.7, 55.0, 0.05, -0.04, 0.0)), (NFW, (0.15, 3.0, 0.8, 20.0, 0.1, 0.02)),
+                (SHEAR, (0.04, 75.0, 0.0, 0.0))],
+        sources=[(1.0, 0.18, 1.3, 0.65, 40.0, 0.06, 0.04), (0.4, 0.1, 2.2, 0.8, -25.0, -0.2, 0.15)],
+        psf=gaussian_psf_xy(0.06, 0.035, 0.015, 0.05, 0.045),
+        psf_sigma=(0.06, 0.035), psf_threshold=0.015,
+        mask=mask.astype(np.uint8),
+        sigma=0.01, seed=10,
+        srcgrid=(-0.75, 0.55, -0.45, 0.7), src_n=(28, 22),
+        regparam=300.0, integral_tol=1e-6,
+    )
+
+
+def c1_geometry_200(src_side):
+    """The 200 x 200 batched-evidence geometry (0.05"/pix, 10" field, 2 x 2 supersampling) with
+    config 1's SIE + shear lens, Sersic source and 0.1" FWHM Gaussian PSF, on a src_side^2 grid."""
+    base = c1()
+    xv = corners(-5.0, 5.0, 200)
+    return dict(
+        name="C4s%d" % src_side, xvals=xv, yvals=xv.copy(), nsplit=2,
+        lenses=base["lenses"], sources=base["sources"],
+        psf=base["psf"], sigma=0.01, seed=10,
+        srcgrid=(-0.6, 0.6, -0.6, 0.6), src_n=(src_side, src_side),
+        regparam=1e3, integral_tol=1e-6,
+    )
+
+
 def proposals(case, n, seed=1234):
     """Seeded 1% Gaussian ball around the truth: SIE (b, q), NFW (ks, rs, q), shear gamma."""
     rng = np.random.default_rng(seed)
@@ -156,10 +222,11 @@ def proposals(case, n, seed=1234):
 
 
 def c4(n=256, seed=4321):
-    """Batched evidence: c2's image and PSF, n proposals from the 1% ball around c2's truth, each
-    with its own square source grid drawn uniformly from {20, 30, 40, 50, 60} cells per side.
-    Returns (case, models, grids)."""
-    case = c2()
+    """Batched evidence: n proposals from a 1% Gaussian ball around config 1's truth (SIE + shear)
+    on the 200 x 200 image of c1_geometry_200 (0.05"/pix, 2 x 2 supersampling), each with its own
+    square source grid drawn uniformly from {20, 30, 40, 50, 60} cells per side (F up to
+    3600 x 3600).  Returns (case, models, grids); the case's own grid is 60 x 60."""
+    case = c1_geometry_200(60)
     case["name"] = "C4"
     rng = np.random.default_rng(seed)
     sides = rng.choice([20, 30, 40, 50, 60], size=n)
@@ -167,4 +234,14 @@ def c4(n=256, seed=4321):
     return case, proposals(case, n, seed + 1), grids
 
 
-WORKLOADS = {"C1": c1, "C2": c2, "C2s": c2_small, "C3": c3, "C5": c5}
+def c4_nfw(n=256, seed=4321):
+    """c4 on the numerical-profile workload instead: c2's image, PSF and NFW + SIE + shear truth."""
+    case = c2()
+    case["name"] = "C4nfw"
+    rng = np.random.default_rng(seed)
+    sides = rng.choice([20, 30, 40, 50, 60], size=n)
+    grids = [(int(k), int(k)) for k in sides]
+    return case, proposals(case, n, seed + 1), grids
+
+
+WORKLOADS = {"C1": c1, "C2": c2, "C2s": c2_small, "C3": c3, "C5": c5, "CNS": c_ns}

@@ -1,5 +1,10 @@
-"""GPU parity: the HIP path (through the C ABI) against the oracle and the reference's golden
-vectors.  Bar: integer/index outputs identical; FP64 results within 1e-10 relative."""
+"""GPU parity: the HIP path (through the C ABI) against the compiled reference's golden vectors
+and the oracle.  Bar: integer/index outputs identical; FP64 results within 1e-10 relative.
+
+Source amplitudes and the model image are compared as whole vectors: ||x - x_ref|| / ||x_ref||
+within 1e-10.  A per-cell relative bound would be meaningless for cells whose amplitude is ~0
+(the solution is regularized, so edge cells sit at noise level and their relative error is
+unbounded); the vector norm is what the evidence terms (chi^2, s^T R s) are built from."""
 import json
 import os
 
@@ -7,6 +12,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from qlens_amd import qlx
 from qlens_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -73,6 +79,40 @@ def test_raytrace_matches_oracle(qlx_ctx, golden_dir):
         assert np.allclose(beta, ref, rtol=1e-12, atol=1e-13), name
 
 
+def test_nfw_hessian_matches_reference(qlx_ctx, golden_dir):
+    """Lens-frame deflection + Hessian from the GPU quadrature kernel, rotated to the sky frame,
+    against the reference's LensProfile::deflection / hessian (psi_xy included).  Deflections agree
+    to 1e-10.  The Hessian's K integrals evaluate kappa'(x^2) = -(ks/r^2)(x^2(2 - 3F) + 1)/(x^2 - 1)^2
+    (models.cpp:986-994), which loses up to 10 digits to cancellation at nodes with x^2 near 1 (down
+    to the series window |x^2 - 1| < 1e-5); the device atan/atanh differ from the host libm's in the
+    last place, so the bound there is the quadrature tolerance (1e-6) rather than 1e-10.  The
+    evidence never uses kappa' (only the surrogate's training targets do)."""
+    g = np.load(os.path.join(golden_dir, "deflection.npz"))
+    sets = json.load(open(os.path.join(golden_dir, "deflection_lenses.json")))
+    deg = 0.017453292519943295
+    for name, lenses in sets.items():
+        pts = g["points"]
+        for l, (t, p) in enumerate(lenses):
+            if t != W.NFW or p[2] == 1.0:
+                continue
+            ks, rs, q, th, xc, yc = p
+            sinth, costh = np.cos(deg * th), -np.sin(deg * th)
+            R = np.array([[costh, sinth], [-sinth, costh]])
+            x, y = pts[:, 0] - xc, pts[:, 1] - yc
+            xy = np.stack([x * costh + y * sinth, -x * sinth + y * costh], 1)
+            keep = np.abs(xy).max(1) > 1e-14
+            out = qlx_ctx.nfw_def_hess(p, xy[keep], tol=1e-6)
+            Hl = np.zeros((len(out), 2, 2))
+            Hl[:, 0, 0], Hl[:, 0, 1], Hl[:, 1, 0], Hl[:, 1, 1] = out[:, 2], out[:, 3], out[:, 3], out[:, 4]
+            H = np.einsum("ji,njk,kl->nil", R, Hl, R)
+            ref_h = g[name + "_hess"][l][keep]
+            eh = np.abs(H - ref_h).max(axis=(1, 2)) / np.abs(ref_h).max(axis=(1, 2))
+            print("%s: Hessian max rel err %.3g (median %.3g, %d of %d points above 1e-10)" % (
+                name, eh.max(), np.median(eh), int((eh > 1e-10).sum()), len(eh)))
+            assert eh.max() < 1e-6, name
+            assert np.allclose(out[:, :2] @ R, g[name + "_per_lens"][l][keep], rtol=1e-10, atol=1e-12), name
+
+
 def test_nfw_def_hess_matches_oracle(qlx_ctx):
     rng = np.random.default_rng(3)
     pts = rng.uniform(-6, 6, size=(300, 2))
@@ -82,27 +122,97 @@ def test_nfw_def_hess_matches_oracle(qlx_ctx):
     assert np.allclose(got, ref, rtol=1e-10, atol=1e-13)
 
 
-def test_mock_image_matches_oracle(qlx_ctx, golden_dir):
-    case = W.c1()
-    g = np.load(os.path.join(golden_dir, "inversion_C1.npz"))
+def cns_case(g):
+    """The non-square masked workload with the reference's generated anisotropic PSF."""
+    case = W.c_ns()
+    case["psf"] = g["psf_used"]
+    return case
+
+
+CASES = {"C1": W.c1, "C2s": W.c2_small, "C2": W.c2, "C4s": lambda: W.c1_geometry_200(20)}
+
+
+def _case(name, g):
+    return cns_case(g) if name == "CNS" else CASES[name]()
+
+
+def vrel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("name", ["C1", "C2s", "CNS", "C2", "C4s"])
+def test_mock_image_matches_reference(qlx_ctx, golden_dir, name):
+    """The GPU mock generator (ray-traced Sersic SB per subpixel, PSF, NR3 noise at seed 10) against
+    the reference's own mock of the same workload (the fixture's data)."""
+    g = np.load(os.path.join(golden_dir, "inversion_%s.npz" % name))
+    case = _case(name, g)
+    print("case", name, flush=True)
     qlx_ctx.load_workload(case, g["data"])
     img = qlx_ctx.mock_image(case["lenses"], case["sources"], case["sigma"], case["seed"])
     assert np.allclose(img, g["data"], rtol=1e-12, atol=1e-14)
 
 
-@pytest.mark.parametrize("name,builder", [("C1", W.c1), ("C2s", W.c2_small)])
-def test_inversion_matches_reference(qlx_ctx, golden_dir, name, builder):
+def test_gaussian_psf_matches_reference(qlx_ctx, golden_dir):
+    """qlx_set_psf_gaussian against QLens::generate_PSF_matrix on the same pixel steps, bit for bit
+    (isotropic, anisotropic, non-square pixels)."""
+    g = np.load(os.path.join(golden_dir, "psf_gaussian.npz"))
+    for k, (sx, sy, th, dx, dy) in enumerate(g["cases"]):
+        xv, yv = W.corners(0.0, 40 * dx, 40), W.corners(0.0, 30 * dy, 30)
+        qlx_ctx.load_image(xv, yv, np.zeros((40, 30)), sigma=1.0)
+        qlx_ctx.set_psf_gaussian(sx, sy, th)
+        assert np.array_equal(qlx_ctx.get_psf(), g["psf_%d" % k]), k
+
+
+@pytest.mark.parametrize("name", ["C1", "C2s", "CNS", "C2"])
+def test_inversion_matches_reference(qlx_ctx, golden_dir, name):
     g = np.load(os.path.join(golden_dir, "inversion_%s.npz" % name))
-    case = builder()
+    case = _case(name, g)
+    print("case", name, flush=True)
     qlx_ctx.load_workload(case, g["data"])
+    if name == "CNS":  # the PSF built by the library itself from the widths and threshold
+        qlx_ctx.set_psf_gaussian(case["psf_sigma"][0], case["psf_sigma"][1], case["psf_threshold"])
+        assert np.array_equal(qlx_ctx.get_psf(), g["psf_used"])
     r = qlx_ctx.invert_image_surface_brightness_map(case["lenses"])[0]
     assert r["status"] == 0
     assert r["n_img_active"] == int(g["n_img"]) and r["n_src_active"] == int(g["n_src"])
-    for k, gk in [("minus2lnE", "minus2lnE"), ("chisq0", "chisq0"), ("logdetF", "logdetF"), ("logdetR", "logdetR")]:
-        assert rel(r[k], float(g[gk])) < RTOL_EVIDENCE, (k, r[k], float(g[gk]))
-    src, model = qlx_ctx.get_solution()
-    assert np.allclose(src, g["src"], rtol=1e-8, atol=1e-9 * np.abs(g["src"]).max())
-    assert np.allclose(model, g["model"], rtol=1e-8, atol=1e-9 * np.abs(g["model"]).max())
+    for k in ("minus2lnE", "chisq0", "logdetF", "logdetR"):
+        assert rel(r[k], float(g[k])) < RTOL_EVIDENCE, (k, r[k], float(g[k]))
+    sol = qlx_ctx.get_source(0)
+    es, em = vrel(sol["src"], g["src"]), vrel(sol["model"], g["model"])
+    print("%s: -2lnE rel %.2e, |ds|/|s| %.2e, |dmodel|/|model| %.2e" % (
+        name, rel(r["minus2lnE"], float(g["minus2lnE"])), es, em))
+    assert es < RTOL_EVIDENCE and em < RTOL_EVIDENCE
+    assert np.array_equal(sol["active_index"], g["active"])
+    # subpixel source-plane points of the in-mask pixels against the reference's
+    rays = qlx_ctx.get_rays(0)
+    nx, ny = rays.shape[:2]
+    got = rays.reshape(nx * ny, -1, 2)[g["ray_pixels"]]
+    d = np.abs(got - g["rays"])
+    print("%s rays: %d compared, max |d beta| %.3g, %d above 1e-13" % (name, d.size // 2, d.max(), int((d > 1e-13).sum())))
+    assert np.allclose(got, g["rays"], rtol=1e-12, atol=1e-13)
+
+
+def test_batched_grid_sizes_match_reference(qlx_ctx, golden_dir):
+    """The 200 x 200 batched geometry on config 1's truth: one call with one item per source-grid
+    side (20..60, uneven grids in one submission) against the reference's evaluation of each, and
+    each item's own source grid read back."""
+    g = np.load(os.path.join(golden_dir, "inversion_C4s.npz"))
+    case = W.c1_geometry_200(30)
+    qlx_ctx.load_workload(case, g["data"])
+    sides = [int(v) for v in g["sides"]]
+    res = qlx_ctx.invert_image_surface_brightness_map([case["lenses"]] * len(sides), src_n=[(k, k) for k in sides])
+    for b, (k, r) in enumerate(zip(sides, res)):
+        assert r["status"] == 0, (k, r)
+        assert r["n_src_active"] == int(g["n_src_%d" % k]) and r["n_img_active"] == int(g["n_img_%d" % k]), k
+        for f in ("minus2lnE", "chisq0", "logdetF", "logdetR"):
+            assert rel(r[f], float(g["%s_%d" % (f, k)])) < RTOL_EVIDENCE, (k, f, r[f])
+        sol = qlx_ctx.get_source(b)
+        assert sol["src"].shape == (k, k)
+        es = vrel(sol["src"], g["src_%d" % k])
+        em = vrel(sol["model"].reshape(-1)[g["model_pixels"]], g["model_%d" % k])
+        print("side %d: -2lnE rel %.2e |ds|/|s| %.2e |dmodel|/|model| %.2e" % (
+            k, rel(r["minus2lnE"], float(g["minus2lnE_%d" % k])), es, em))
+        assert es < RTOL_EVIDENCE and em < RTOL_EVIDENCE, k
 
 
 def test_inversion_is_deterministic(qlx_ctx, golden_dir):
@@ -130,6 +240,76 @@ def test_batch_of_perturbed_models_matches_oracle(qlx_ctx, golden_dir):
         ro, _ = O.invert(case, g["data"], lenses=m, want_arrays=False)
         assert r["n_src_active"] == ro["n_src"] and r["n_img_active"] == ro["n_img"]
         assert rel(r["minus2lnE"], ro["minus2lnE"]) < RTOL_EVIDENCE
+
+
+def test_failed_items_carry_sentinels(qlx_ctx, golden_dir):
+    """Items with invalid lens parameters (NaN, q out of range) or an oversize grid fail alone:
+    their status is an error and minus2lnE the 2e30 sentinel, the other items are unaffected, and
+    the call returns QLX_ITEMS_FAILED rather than raising."""
+    g = np.load(os.path.join(golden_dir, "inversion_C1.npz"))
+    case = W.c1()
+    qlx_ctx.load_workload(case, g["data"])
+    (t0, p0), s = case["lenses"]
+    bad_nan = [(t0, (float("nan"),) + p0[1:]), s]
+    bad_q = [(t0, (p0[0], 1.5) + p0[2:]), s]
+    res = qlx_ctx.invert_image_surface_brightness_map([bad_nan, case["lenses"], bad_q], src_n=[(30, 30), (30, 30), (1, 5)])
+    assert res[0]["status"] == -1 and res[0]["minus2lnE"] == 2e30
+    assert res[2]["status"] == -6 and res[2]["minus2lnE"] == 2e30
+    assert res[1]["status"] == 0 and rel(res[1]["minus2lnE"], float(g["minus2lnE"])) < RTOL_EVIDENCE
+    res = qlx_ctx.invert_image_surface_brightness_map([bad_q])
+    assert res[0]["status"] == -1 and res[0]["minus2lnE"] == 2e30
+    with pytest.raises(qlx.QlxError):
+        qlx_ctx.get_source(0)  # a failed item keeps no solution
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_singular_F_is_reported(qlx_ctx, golden_dir, precision):
+    """lambda = 0, a mask of six pixels and a 60 x 60 source grid: F = Lb^T C Lb has rank <= 6 with
+    37 active cells, so 31 pivots of its Cholesky are rounding noise of either sign (the chance
+    that all are positive is ~2^-31).  The reference dies there (pixelgrid.cpp:17384); here the
+    item reports QLX_ERR_NOT_PD with the 2e30 sentinel.  The oracle flags the same case."""
+    from oracle import oracle as O
+    g = np.load(os.path.join(golden_dir, "inversion_C1.npz"))
+    case = W.c1()
+    mask = np.zeros((100, 100), dtype=np.uint8)
+    mask[62:65, 48:50] = 1
+    case["mask"] = mask
+    case["regparam"] = 0.0
+    case["src_n"] = (60, 60)
+    ro, _ = O.invert(case, g["data"], want_arrays=False)
+    assert ro["status"] != 0 and ro["n_src"] > 30
+    qlx_ctx.load_workload(case, g["data"], precision=precision)
+    r = qlx_ctx.invert_image_surface_brightness_map(case["lenses"])[0]
+    assert r["status"] == -5 and r["minus2lnE"] == 2e30, r
+    assert r["n_src_active"] == ro["n_src"] and r["n_img_active"] == ro["n_img"]
+
+
+def test_multi_wave_equals_single_wave(golden_dir):
+    """A small HBM budget splits a call into several waves (admission); every item's result and
+    its kept solution equal the single-wave call's bit for bit."""
+    from qlens_amd import qlx as Q
+    g = np.load(os.path.join(golden_dir, "inversion_C2s.npz"))
+    case = W.c2_small()
+    models = _perturbed(case, 7, 21)
+    grids = [(24, 24), (20, 28), (24, 24), (16, 16), (30, 30), (24, 24), (22, 18)]
+    out = {}
+    for budget in (0, 48 << 20):
+        inv = Q.PixelInversion(0, hbm_budget=budget)
+        try:
+            inv.load_workload(case, g["data"])
+            res = inv.invert_image_surface_brightness_map(models, src_n=grids)
+            waves = Q.get_stats(inv)["n_waves"]
+            sols = [inv.get_source(b)["src"] for b in range(len(models))]
+        finally:
+            inv.close()
+        out[budget] = (res, waves, sols)
+    (r1, w1, s1), (r2, w2, s2) = out[0], out[48 << 20]
+    print("waves: default budget %d, small budget %d" % (w1, w2))
+    assert w1 == 1 and w2 >= 2
+    for a, b in zip(r1, r2):
+        assert a == b
+    for a, b in zip(s1, s2):
+        assert np.array_equal(a, b)
 
 
 def test_sentinels(qlx_ctx, golden_dir):
@@ -194,11 +374,11 @@ def test_uneven_source_grids_in_one_batch(qlx_ctx, golden_dir):
 
 # Mixed precision (BF16-split F, FP32 Cholesky, FP64 refinement and log-dets): the accuracy
 # contract is |delta ln E| < 0.1 against the FP64 evidence; the refined solve must converge.
-@pytest.mark.parametrize("name,builder", [("C1", W.c1), ("C2s", W.c2_small)])
-def test_mixed_precision_evidence(qlx_ctx, golden_dir, name, builder):
+@pytest.mark.parametrize("name", ["C1", "C2s", "CNS", "C2"])
+def test_mixed_precision_evidence(qlx_ctx, golden_dir, name):
     from qlens_amd import qlx
     g = np.load(os.path.join(golden_dir, "inversion_%s.npz" % name))
-    case = builder()
+    case = _case(name, g)
     qlx_ctx.load_workload(case, g["data"], precision=qlx.QLX_PREC_MIXED)
     models = [case["lenses"]] + _perturbed(case, 3, 9)
     res = qlx_ctx.invert_image_surface_brightness_map(models)
@@ -209,4 +389,4 @@ def test_mixed_precision_evidence(qlx_ctx, golden_dir, name, builder):
         assert r["status"] == 0 and r["n_src_active"] == f["n_src_active"]
         assert abs(r["minus2lnE"] - f["minus2lnE"]) / 2 < 0.1, (r["minus2lnE"], f["minus2lnE"])
         assert r["logdetR"] == f["logdetR"]  # R stays FP64 in both modes
-        assert r["refine_resid"] < 1e-6, r["refine_resid"]
+        assert r["refine_resid"] <= 1e-10, r["refine_resid"]  # the configured refine_tol

@@ -92,7 +92,8 @@ def test_surrogate_falls_back_to_quadrature_outside_box(qlx_ctx, golden_dir, tra
 def test_mixed_precision_at_scale_c5(qlx_ctx, trained):
     """400x400 image at 0.02"/pix, 100x100 source grid (F is 10000 x 10000): FP64 with quadrature
     deflections (A), mixed precision with quadrature (B) and mixed precision with the learned
-    deflection (C) on two proposals.  B against A is the precision error, asserted within the
+    deflection (C) on a batch of 16 proposals (the configuration's batch size); B's FP64
+    refinement must reach the configured 1e-10 relative residual on every item.  B against A is the precision error, asserted within the
     stated |delta lnE| < 0.1 (A is the FP64 path pinned to the reference at 1e-10 on C1/C2s; the
     CPU oracle would need hours at this size); C against A is stated, not asserted: the network's
     ~1e-3 deflection error dominates it."""
@@ -102,17 +103,19 @@ def test_mixed_precision_at_scale_c5(qlx_ctx, trained):
     qlx_ctx.set_psf(case["psf"])
     qlx_ctx.set_settings(case["nsplit"], case["srcgrid"], case["src_n"], case["regparam"], case["integral_tol"])
     data = qlx_ctx.mock_image(case["lenses"], case["sources"], case["sigma"], case["seed"])
-    props = W.proposals(case, 2, seed=55)
+    props = W.proposals(case, 16, seed=55)
     out = {}
     for tag, prec, mode in (("A", qlx.QLX_PREC_FP64, qlx.QLX_DEF_QUADRATURE),
                             ("B", qlx.QLX_PREC_MIXED, qlx.QLX_DEF_QUADRATURE),
                             ("C", qlx.QLX_PREC_MIXED, qlx.QLX_DEF_SURROGATE)):
         qlx_ctx.load_workload(case, data, precision=prec)
         res = qlx_ctx.invert_image_surface_brightness_map(props, def_mode=mode)
-        src, _ = qlx_ctx.get_solution()
+        src = np.stack([qlx_ctx.get_source(b)["src"] for b in range(len(props))])
         out[tag] = (res, src, qlx_ctx.kernel_time("total")[0])
         for r in res:
             assert r["status"] == 0 and np.isfinite(r["minus2lnE"]), (tag, r)
+            if tag == "B":
+                assert r["refine_resid"] <= 1e-10, r
     A, B, Cs = out["A"], out["B"], out["C"]
     assert A[0][0]["n_src_active"] > 8000, A[0][0]["n_src_active"]
     dB = max(abs(b["minus2lnE"] - a["minus2lnE"]) / 2 for a, b in zip(A[0], B[0]))
@@ -120,7 +123,7 @@ def test_mixed_precision_at_scale_c5(qlx_ctx, trained):
     scale = np.abs(A[1]).max()
     eB = np.abs(B[1] - A[1]).max() / scale
     eC = np.abs(Cs[1] - A[1]).max() / scale
-    print("C5 n_src %d n_img %d: FP64 %.1f ms, mixed %.1f ms, mixed+surrogate %.1f ms per 2 evaluations"
+    print("C5 n_src %d n_img %d: FP64 %.1f ms, mixed %.1f ms, mixed+surrogate %.1f ms per 16 evaluations"
           % (A[0][0]["n_src_active"], A[0][0]["n_img_active"], A[2], B[2], Cs[2]))
     print("C5 mixed: |dlnE| %.3g, source max err %.3g (relative to max |s|); mixed+surrogate: |dlnE| %.3g, "
           "source max err %.3g" % (dB, eB, dC, eC))

@@ -1,11 +1,17 @@
-"""Batched evidence over uneven source grids: 256 proposals around the 200x200 / 60x60 workload's
-truth, each with its own square source grid of 20..60 cells per side, in one call.
+"""Batched evidence over uneven source grids and the FP32 ray-shooting leg, at full size.
 
-Properties checked at full size (the oracle would need hours for 256 evaluations; per-item parity
-with the reference is pinned on the C1 and C2s fixtures in test_gpu_parity.py):
+C4: 256 proposals around config 1's truth (SIE + shear) on the 200 x 200 image, each with its own
+square source grid of 20..60 cells per side, in one call.  Per-item parity with the reference is
+pinned for one item of every grid size by test_gpu_parity.py::test_batched_grid_sizes_match_reference
+(the oracle would need hours for 256 evaluations); here, at the batch's full size:
   * every item succeeds,
   * an item's batched result equals its stand-alone evaluation bit for bit,
-  * the mixed-precision mode stays within its stated |delta lnE| < 0.1 of FP64 on every item.
+  * the mixed-precision mode stays within its stated |delta lnE| < 0.1 of FP64 on every item and its
+    refinement reaches the configured 1e-10 residual.
+The same batch on the numerical-profile (NFW) truth runs as a second case.
+
+C2 with FP32 ray shooting: the configuration states FP32 ray shooting with an FP64 solve; its
+stated per-result error against the FP64 rays is asserted.
 """
 import numpy as np
 import pytest
@@ -16,8 +22,8 @@ pytestmark = pytest.mark.gpu
 
 
 def _load(inv, case, precision):
-    nx = len(case["xvals"]) - 1
-    inv.load_image(case["xvals"], case["yvals"], np.zeros((nx, nx)), sigma=case["sigma"])
+    nx, ny = len(case["xvals"]) - 1, len(case["yvals"]) - 1
+    inv.load_image(case["xvals"], case["yvals"], np.zeros((nx, ny)), sigma=case["sigma"])
     inv.set_psf(case["psf"])
     inv.set_settings(case["nsplit"], case["srcgrid"], case["src_n"], case["regparam"], case["integral_tol"])
     data = inv.mock_image(case["lenses"], case["sources"], case["sigma"], case["seed"])
@@ -25,8 +31,10 @@ def _load(inv, case, precision):
     return data
 
 
-def test_batched_uneven_source_grids_256(qlx_ctx):
-    case, models, grids = W.c4()
+@pytest.mark.parametrize("builder", [W.c4, W.c4_nfw])
+def test_batched_uneven_source_grids_256(qlx_ctx, builder):
+    case, models, grids = builder()
+    print("case", case["name"], flush=True)
     data = _load(qlx_ctx, case, qlx.QLX_PREC_FP64)
     res = qlx_ctx.invert_image_surface_brightness_map(models, src_n=grids)
     assert len(res) == 256
@@ -35,17 +43,46 @@ def test_batched_uneven_source_grids_256(qlx_ctx):
         assert np.isfinite(r["minus2lnE"]) and r["n_src_active"] > 0, (k, r)
     print("grid sides", sorted(set(g[0] for g in grids)), "-2lnE range %.6g .. %.6g" % (
         min(r["minus2lnE"] for r in res), max(r["minus2lnE"] for r in res)))
-    # one item of each grid size, evaluated alone
+    # one item of each grid size, evaluated alone; its kept source has the item's own grid
     picked = {}
     for k, g in enumerate(grids):
         picked.setdefault(g[0], k)
+    sols = {k: qlx_ctx.get_source(k)["src"] for k in picked.values()}
     for side, k in sorted(picked.items()):
+        assert sols[k].shape == (side, side)
         one = qlx_ctx.invert_image_surface_brightness_map(models[k], src_n=[grids[k]])[0]
         assert one["minus2lnE"] == res[k]["minus2lnE"], (side, k)
         assert one["logdetF"] == res[k]["logdetF"] and one["chisq0"] == res[k]["chisq0"], (side, k)
+        assert np.array_equal(qlx_ctx.get_source(0)["src"], sols[k]), (side, k)
     # the same batch in mixed precision
     qlx_ctx.load_workload(case, data, precision=qlx.QLX_PREC_MIXED)
     resm = qlx_ctx.invert_image_surface_brightness_map(models, src_n=grids)
     d = [abs(a["minus2lnE"] - b["minus2lnE"]) / 2 for a, b in zip(resm, res)]
-    print("mixed vs FP64 over 256 uneven items: max |dlnE| %.3g, median %.3g" % (max(d), float(np.median(d))))
+    print("mixed vs FP64 over 256 uneven items: max |dlnE| %.3g, median %.3g; max refine residual %.3g" % (
+        max(d), float(np.median(d)), max(r["refine_resid"] for r in resm)))
     assert max(d) < 0.1
+    for r in resm:
+        assert r["status"] == 0 and r["refine_resid"] <= 1e-10, r
+
+
+def test_fp32_ray_shooting_c2(qlx_ctx, golden_dir):
+    """The 200 x 200 / 60 x 60 NFW workload with FP32 ray shooting (quadrature in FP32) and the FP64
+    solve, against FP64 rays on the same data.  Stated errors: source-plane points within 5e-5"
+    (measured 2.3e-5": FP32 rounding of the ~1" deflections, ~1e-7 each, is amplified where the
+    FP32 level sums of the quadrature stop at a different Patterson level than FP64 does, a
+    relative 1e-6 tolerance of a ~4" quantity), |delta lnE| < 0.1 (measured 1e-3)."""
+    g = np.load(golden_dir + "/inversion_C2.npz")
+    case = W.c2()
+    out = {}
+    for tag, ray in (("fp64", qlx.QLX_RAY_FP64), ("fp32", qlx.QLX_RAY_FP32)):
+        qlx_ctx.load_workload(case, g["data"], ray_precision=ray)
+        r = qlx_ctx.invert_image_surface_brightness_map(case["lenses"])[0]
+        out[tag] = (r, qlx_ctx.get_rays(0))
+    (r64, b64), (r32, b32) = out["fp64"], out["fp32"]
+    db = np.abs(b32 - b64).max()
+    dlnE = abs(r32["minus2lnE"] - r64["minus2lnE"]) / 2
+    print("C2 FP32 rays: max |d beta| %.3g arcsec, |dlnE| %.4g, n_src %d/%d, n_img %d/%d" % (
+        db, dlnE, r32["n_src_active"], r64["n_src_active"], r32["n_img_active"], r64["n_img_active"]))
+    assert r32["status"] == 0
+    assert db < 5e-5
+    assert dlnE < 0.1

@@ -18,10 +18,17 @@ def load(golden_dir, name):
     return np.load(os.path.join(golden_dir, name))
 
 
-@pytest.mark.parametrize("name,builder", [("C1", W.c1), ("C2s", W.c2_small)])
-def test_oracle_inversion_matches_reference(golden_dir, name, builder):
+def cns_case(g):
+    """The non-square masked workload with the reference's generated PSF (stored in its fixture)."""
+    case = W.c_ns()
+    case["psf"] = g["psf_used"]
+    return case
+
+
+@pytest.mark.parametrize("name", ["C1", "C2s", "CNS"])
+def test_oracle_inversion_matches_reference(golden_dir, name):
     g = load(golden_dir, "inversion_%s.npz" % name)
-    case = builder()
+    case = {"C1": W.c1, "C2s": W.c2_small}[name]() if name != "CNS" else cns_case(g)
     res, arr = O.invert(case, g["data"])
     # same operation order as the reference: equal to the last bit
     assert res["minus2lnE"] == float(g["minus2lnE"])
@@ -38,9 +45,25 @@ def test_oracle_inversion_matches_reference(golden_dir, name, builder):
     assert np.array_equal(rays, g["rays"])
 
 
-def test_oracle_mock_reproduces_fixture_data(golden_dir):
-    g = load(golden_dir, "inversion_C1.npz")
-    assert np.array_equal(O.mock_image(W.c1()), g["data"])
+@pytest.mark.parametrize("name", ["C1", "C2s", "CNS", "C4s"])
+def test_oracle_mock_equals_reference_mock(golden_dir, name):
+    """Every fixture's data is the reference's own mock (ref_harness mock: lensed Sersic sources,
+    PSF, noise from its generator seeded with 10); the oracle's mock equals it bit for bit."""
+    g = load(golden_dir, "inversion_%s.npz" % name)
+    case = {"C1": W.c1, "C2s": W.c2_small, "C4s": lambda: W.c1_geometry_200(20)}[name]() if name != "CNS" else cns_case(g)
+    assert np.array_equal(O.mock_image(case), g["data"])
+
+
+def test_oracle_inversion_matches_reference_200px_sie(golden_dir):
+    """The 200 x 200 geometry of the batched workload with config 1's SIE + shear (20 x 20 source
+    grid, the smallest of the batched sizes; the larger ones are pinned on the GPU)."""
+    g = load(golden_dir, "inversion_C4s.npz")
+    res, arr = O.invert(W.c1_geometry_200(20), g["data"])
+    for k in ("minus2lnE", "chisq0", "logdetF", "logdetR"):
+        assert res[k] == float(g["%s_20" % k]), k
+    assert res["n_src"] == int(g["n_src_20"]) and res["n_img"] == int(g["n_img_20"])
+    assert np.array_equal(arr["src"], g["src_20"])
+    assert np.array_equal(arr["model"].reshape(-1)[g["model_pixels"]], g["model_20"])
 
 
 def test_oracle_deflection_matches_reference(golden_dir):
@@ -130,21 +153,62 @@ def test_sie_deflection_is_potential_gradient_consistent():
         assert abs(dax_dy - day_dx) < 1e-7
 
 
-def test_normal_deviates_are_standard_normal():
-    z = O.normal_deviates(10, 200000)
+@pytest.mark.parametrize("seed", [10, 987654321])
+def test_normal_deviates_equal_reference_generator(golden_dir, seed):
+    """The first deviates of the NR3 generator + polar Box-Muller equal the reference's Random
+    (rand.h:19-76, dumped by ref_harness rng) bit for bit: a wrong constant, shift or pair order
+    changes them all."""
+    g = load(golden_dir, "rng.npz")
+    ref = g["deviates_%d" % seed]
+    assert np.array_equal(O.normal_deviates(seed, len(ref)), ref)
+    # and the distribution is standard normal (a coarse sanity check beside the exact pin)
+    z = O.normal_deviates(seed, 200000)
     assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
-    # pairs come out second-member-first; both seeds deterministic
-    assert np.array_equal(O.normal_deviates(10, 8), O.normal_deviates(10, 8))
 
 
-def test_gaussian_psf_generation_matches_reference_rule():
+def test_gaussian_psf_equals_reference(golden_dir):
+    """orc_gaussian_psf against QLens::generate_PSF_matrix (ref_harness psf) for isotropic,
+    anisotropic and non-square-pixel cases, bit for bit."""
+    g = load(golden_dir, "psf_gaussian.npz")
+    for k, (sx, sy, th, dx, dy) in enumerate(g["cases"]):
+        ref = g["psf_%d" % k]
+        got = O.gaussian_psf(sx, sy, th, dx, dy)
+        assert got.shape == ref.shape and np.array_equal(got, ref), k
     # threshold rounding: sigma_fraction*sigma/step = 2.3756 -> half-width 2 -> 5x5 (pixelgrid.cpp:14616-14621)
-    p = O.gaussian_psf(0.1 / 2.35482, 0.1 / 2.35482, 0.02, 0.05, 0.05)
-    assert p.shape == (5, 5)
-    assert abs(p.sum() - 1) < 1e-15
-    assert np.array_equal(p, W.gaussian_psf(0.1 / 2.35482, 0.02, 0.05, 0.05))
-    # default threshold 0.01 gives 7x7 for the same sigma
-    assert O.gaussian_psf(0.1 / 2.35482, 0.1 / 2.35482, 0.01, 0.05, 0.05).shape == (7, 7)
+    assert g["psf_0"].shape == (5, 5) and g["psf_1"].shape == (7, 7) and g["psf_2"].shape == (7, 5)
+
+
+def test_nfw_hessian_matches_reference(golden_dir):
+    """Lens-frame deflection and Hessian (psi_xx, psi_xy, psi_yy from the J and K integrals,
+    profile.cpp:2532-2623) rotated to the sky frame against the reference's LensProfile::hessian
+    of the NFW lens in every deflection lens set (psi_xy is pinned here, not only the trace)."""
+    g = load(golden_dir, "deflection.npz")
+    with open(os.path.join(golden_dir, "deflection_lenses.json")) as f:
+        sets = json.load(f)
+    deg = 0.017453292519943295
+    for name, lenses in sets.items():
+        pts = g["points"]
+        for l, (t, p) in enumerate(lenses):
+            if t != W.NFW:
+                continue
+            ks, rs, q, th, xc, yc = p
+            # orient-north rotation into the lens frame (profile.cpp:1625-1637, 1743-1749)
+            sinth, costh = np.cos(deg * th), -np.sin(deg * th)
+            R = np.array([[costh, sinth], [-sinth, costh]])
+            x, y = pts[:, 0] - xc, pts[:, 1] - yc
+            xy = np.stack([x * costh + y * sinth, -x * sinth + y * costh], 1)  # the reference's operation order
+            keep = np.abs(xy).max(1) > 1e-14
+            out = O.nfw_def_hess(p, xy, tol=1e-6)
+            Hl = np.zeros((len(pts), 2, 2))
+            Hl[:, 0, 0], Hl[:, 0, 1], Hl[:, 1, 0], Hl[:, 1, 1] = out[:, 2], out[:, 3], out[:, 3], out[:, 4]
+            H = np.einsum("ji,njk,kl->nil", R, Hl, R)
+            A = out[:, :2] @ R
+            ref_h, ref_a = g[name + "_hess"][l], g[name + "_per_lens"][l]
+            # q < 1: the same quadrature as the reference (rounding of the frame rotation only);
+            # q = 1: the reference's closed form against the tol-1e-6 quadrature
+            tol = 1e-13 if q < 1 else 1e-6
+            assert np.allclose(H[keep], ref_h[keep], rtol=tol, atol=tol), name
+            assert np.allclose(A[keep], ref_a[keep], rtol=tol, atol=tol), name
 
 
 def test_sentinel_for_too_small_source_grid():
