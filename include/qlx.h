@@ -114,7 +114,7 @@ const char *qlx_last_error(void *ctx);
  * qlx_set_image replaces ImagePixelData (pixelgrid.h:513-641, loaded by load_data
  * pixelgrid.cpp:5797-5904): corners xvals[nx+1], yvals[ny+1]; data, covinv (1/sigma^2, the
  * noise map's covinv_map) and mask (1 = in mask; NULL = all) are x-major [nx*ny].
- * Limits: nx*ny <= 2^24 and nx < 2^18. */
+ * Limits: nx*ny <= 2^24 and nx, ny <= 65535. */
 int qlx_set_image(void *ctx, int nx, int ny, const double *xvals, const double *yvals,
                   const double *data, const double *covinv, const unsigned char *mask);
 /* PSF matrix psf[i*pny + j] = QLens psf_matrix[i][j] (use_input_psf_matrix, qlens.h:877-880). */

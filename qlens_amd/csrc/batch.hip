@@ -323,29 +323,6 @@ void launch_merge_lrows_b(hipStream_t s, const BatchDev& bd, int max_img)
     if (max_img > 0) hipLaunchKernelGGL(k_merge_lrows_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, s, bd);
 }
 
-// The merged rows again as one CSR array per item, rows in active order.  Active pixels with equal
-// y index j and consecutive x index i are consecutive rows, so the rows of any x-range of one image
-// column occupy one contiguous entry range [lptr[apre[j*nx + i0]], lptr[apre[j*nx + i1 + 1]]).
-// Each entry carries its pixel's x index so a reader knows the PSF tap without a row search.
-__global__ void __launch_bounds__(1024) k_lrow_ptr_b(BatchDev bd)
-{
-    __shared__ int sh[40];
-    const int item = blockIdx.y;
-    const ItemMeta& M = bd.meta[item];
-    if (!M.valid) return;
-    const int* ucnt = bd.ucnt + item * bd.s_pix;
-    int* lptr = bd.lptr + item * (bd.s_pix + 1);
-    int base = 0;
-    for (int r0 = 0; r0 < M.n_img; r0 += 1024) {
-        const int r = r0 + threadIdx.x;
-        const int v = (r < M.n_img) ? ucnt[r] : 0;
-        int tot, ex = block_excl_scan(v, sh, tot);
-        if (r < M.n_img) lptr[r] = base + ex;
-        base += tot;
-    }
-    if (threadIdx.x == 0) lptr[M.n_img] = base;
-}
-
 // a merged 12-slot L row (2 x 2 split) as 3 + 6 16-byte loads
 __device__ __forceinline__ void load_row12(const int* uc, const double* uv, int (&c)[12], double (&w)[12])
 {
@@ -365,174 +342,176 @@ __device__ __forceinline__ void load_row12(const int* uc, const double* uv, int 
     }
 }
 
-__global__ void __launch_bounds__(256) k_lrow_csr_b(BatchDev bd)
+// ------------------------------------------------------------------------------------------------
+// Row order of the blurred lensing matrix: active pixels in image tiles of 8 x 8 pixels (tiles and
+// pixels inside a tile y-major), so that the 64 rows of a chunk are image-compact.  Their PSF
+// windows then overlap (the Lb build applies each L entry to every row of the chunk it reaches) and
+// their blurred supports are one compact source-plane patch (fewer pool blocks and SYRK block pairs
+// than rows ordered by source position: 2 780 against 3 767 blocks and 14.6k against 25.8k pairs on
+// the 200 x 200 / 60 x 60 workload).  One wave per tile; the order is a pure function of the active
+// set, so an item's rows do not depend on its batch.
+//   perm[pos] = active row of tile-order position pos, key[r] = tile-order position of row r,
+//   tbase[t] = first position of tile t (the eptr buffer, s_pix + 1 ints per item)
+// ------------------------------------------------------------------------------------------------
+#define RT 8  // row tile side in pixels
+__global__ void __launch_bounds__(64) k_tile_count_b(BatchDev bd, int ntx, int nty)
+{
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    const int t = blockIdx.x;
+    if (!M.valid || t >= ntx * nty) return;
+    const int lane = threadIdx.x;
+    const int i = (t % ntx) * RT + (lane % RT), j = (t / ntx) * RT + lane / RT;
+    const bool act = i < bd.nx && j < bd.ny && bd.pidx[item * bd.s_pix + (size_t)i * bd.ny + j] >= 0;
+    const unsigned long long m = __ballot(act);
+    if (lane == 0) bd.eptr[item * (bd.s_pix + 1) + t] = __popcll(m);
+}
+
+__global__ void __launch_bounds__(1024) k_tile_scan_b(BatchDev bd, int ntiles)
+{
+    __shared__ int sh[40];
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    if (!M.valid) return;
+    int* tb = bd.eptr + item * (bd.s_pix + 1);
+    int base = 0;
+    for (int t0 = 0; t0 < ntiles; t0 += 1024) {
+        const int t = t0 + threadIdx.x;
+        const int v = t < ntiles ? tb[t] : 0;
+        int tot, ex = block_excl_scan(v, sh, tot);
+        if (t < ntiles) tb[t] = base + ex;
+        base += tot;
+    }
+    if (threadIdx.x == 0) tb[ntiles] = base;
+}
+
+__global__ void __launch_bounds__(64) k_tile_scatter_b(BatchDev bd, int ntx, int nty)
+{
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    const int t = blockIdx.x;
+    if (!M.valid || t >= ntx * nty) return;
+    const int lane = threadIdx.x;
+    const int i = (t % ntx) * RT + (lane % RT), j = (t / ntx) * RT + lane / RT;
+    const int r = (i < bd.nx && j < bd.ny) ? bd.pidx[item * bd.s_pix + (size_t)i * bd.ny + j] : -1;
+    const unsigned long long m = __ballot(r >= 0);
+    if (r < 0) return;
+    const int rank = __popcll(m & ((1ull << lane) - 1ull));
+    const int pos = bd.eptr[item * (bd.s_pix + 1) + t] + rank;
+    bd.perm[item * bd.s_pix + pos] = r;
+    bd.key[item * bd.s_pix + r] = pos;
+}
+
+void launch_sort_rows_b(hipStream_t s, const BatchDev& bd, int max_img, int max_src)
+{
+    (void)max_src;
+    if (max_img <= 0) return;
+    const int ntx = (bd.nx + RT - 1) / RT, nty = (bd.ny + RT - 1) / RT;
+    hipLaunchKernelGGL(k_tile_count_b, dim3(ntx * nty, bd.B), dim3(64), 0, s, bd, ntx, nty);
+    hipLaunchKernelGGL(k_tile_scan_b, dim3(1, bd.B), dim3(1024), 0, s, bd, ntx * nty);
+    hipLaunchKernelGGL(k_tile_scatter_b, dim3(ntx * nty, bd.B), dim3(64), 0, s, bd, ntx, nty);
+}
+
+// ------------------------------------------------------------------------------------------------
+// L by columns (CSC) for the Lb build: for every internal source column c, the entries (pixel, value)
+// of the merged L rows, ordered by the rows' tile-order position (a fixed order, so every sum over a
+// column's entries is reproducible).  Counts and bases reuse the bucket arrays (bcnt, boff, bfill:
+// s_cell + 2 ints per item); the unordered scatter goes to lcj (keys) and lv (values), the ordered
+// columns to cpix (pixel i << 16 | j) and cval.
+// ------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_csc_count_b(BatchDev bd)
 {
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (!M.valid || r >= M.n_img) return;
     const int U = 3 * bd.nsub;
-    const unsigned i = (unsigned)(bd.act_pix[item * bd.s_pix + r] / bd.ny);
-    const size_t o = (size_t)item * bd.s_pix * U + bd.lptr[item * (bd.s_pix + 1) + r];
     const int* uc = bd.ucol + ((size_t)item * bd.s_pix + r) * U;
-    const double* uv = bd.uval + ((size_t)item * bd.s_pix + r) * U;
     const int cnt = bd.ucnt[item * bd.s_pix + r];
-    if (U == 12) {  // 2 x 2 split: the merge wrote all 12 slots; read them as six 16-byte vectors
-        int c[12];
-        double w[12];
-        load_row12(uc, uv, c, w);
-#pragma unroll
-        for (int e = 0; e < 12; e++)
-            if (e < cnt) {
-                bd.lcj[o + e] = (unsigned)c[e] | (i << 14);
-                bd.lv[o + e] = w[e];
-                bd.lvf[o + e] = (float)w[e];
-            }
-        return;
-    }
-    for (int e = 0; e < cnt; e++) {
-        bd.lcj[o + e] = (unsigned)uc[e] | (i << 14);
-        bd.lv[o + e] = uv[e];
-        bd.lvf[o + e] = (float)uv[e];
-    }
+    int* cc = bd.bcnt + item * (bd.s_cell + 2);
+    for (int e = 0; e < cnt; e++) atomicAdd(&cc[uc[e]], 1);
 }
 
-// eptr[t] = lptr[apre[t]] for every pixel t (and t = npix): one lookup gives the CSR entry range of
-// any run of pixels
-__global__ void __launch_bounds__(256) k_eptr_b(BatchDev bd)
-{
-    const int item = blockIdx.y;
-    const ItemMeta& M = bd.meta[item];
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long npix = (long long)bd.nx * bd.ny;
-    if (!M.valid || t > npix) return;
-    const long long o = item * (bd.s_pix + 1);
-    bd.eptr[o + t] = bd.lptr[o + bd.apre[o + t]];
-}
-
-void launch_lrows_csr_b(hipStream_t s, const BatchDev& bd, int max_img)
-{
-    if (max_img <= 0) return;
-    hipLaunchKernelGGL(k_lrow_ptr_b, dim3(1, bd.B), dim3(1024), 0, s, bd);
-    hipLaunchKernelGGL(k_lrow_csr_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, s, bd);
-    const long long npix1 = (long long)bd.nx * bd.ny + 1;
-    hipLaunchKernelGGL(k_eptr_b, dim3((unsigned)((npix1 + 255) / 256), bd.B), dim3(256), 0, s, bd);
-}
-
-// ------------------------------------------------------------------------------------------------
-// row ordering by the source cell of each pixel's first ray (stable: ties keep pixel order)
-// ------------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_row_keys_b(BatchDev bd)
-{
-    const int item = blockIdx.y;
-    const ItemMeta& M = bd.meta[item];
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!M.valid || r >= M.n_img) return;
-    const int nsub = bd.nsub;
-    const long long p = bd.act_pix[item * bd.s_pix + r];
-    const int* mcb = bd.map_cell + (size_t)item * bd.s_pix * nsub * 3;
-    const int* tix = bd.tix + item * bd.s_cell;
-    int key = 0;
-    for (int k = 0; k < nsub; k++) {
-        int c0 = mcb[(p * nsub + k) * 3];
-        if (c0 >= 0) {
-            key = tix[c0];
-            break;
-        }
-    }
-    bd.key[item * bd.s_pix + r] = key;
-    atomicAdd(&bd.bcnt[item * (bd.s_cell + 2) + key], 1);
-}
-
-__global__ void __launch_bounds__(1024) k_bucket_scan_b(BatchDev bd)
+__global__ void __launch_bounds__(1024) k_csc_scan_b(BatchDev bd)
 {
     __shared__ int sh[40];
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     if (!M.valid) return;
-    const int n = M.n_src;
     const int* cnt = bd.bcnt + item * (bd.s_cell + 2);
     int* off = bd.boff + item * (bd.s_cell + 2);
     int* fill = bd.bfill + item * (bd.s_cell + 2);
     int base = 0;
-    for (int t0 = 0; t0 < n; t0 += 1024) {
-        int t = t0 + threadIdx.x;
-        int v = (t < n) ? cnt[t] : 0;
-        // exclusive scan of counts (values may exceed 1): wave scan + wave offsets
-        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-        int incl = wave_incl_scan(v);
-        if (lane == 63) sh[w] = incl;
-        __syncthreads();
-        if (w == 0) {
-            int x = (lane < 16) ? sh[lane] : 0;
-            int sx = wave_incl_scan(x);
-            if (lane < 16) sh[16 + lane] = sx - x;
-            if (lane == 15) sh[32] = sx;
-        }
-        __syncthreads();
-        if (t < n) {
-            off[t] = base + sh[16 + w] + incl - v;
+    for (int t0 = 0; t0 < M.n_src; t0 += 1024) {
+        const int t = t0 + threadIdx.x;
+        const int v = t < M.n_src ? cnt[t] : 0;
+        int tot, ex = block_excl_scan(v, sh, tot);
+        if (t < M.n_src) {
+            off[t] = base + ex;
             fill[t] = 0;
         }
-        int tot = sh[32];
-        __syncthreads();
         base += tot;
     }
-    if (threadIdx.x == 0) off[n] = base;
+    if (threadIdx.x == 0) off[M.n_src] = base;
 }
 
-__global__ void __launch_bounds__(256) k_row_scatter_b(BatchDev bd)
+__global__ void __launch_bounds__(256) k_csc_scatter_b(BatchDev bd)
 {
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (!M.valid || r >= M.n_img) return;
-    const int key = bd.key[item * bd.s_pix + r];
-    const int pos = bd.boff[item * (bd.s_cell + 2) + key] + atomicAdd(&bd.bfill[item * (bd.s_cell + 2) + key], 1);
-    bd.perm[item * bd.s_pix + pos] = r;
+    const int U = 3 * bd.nsub;
+    const size_t rb = ((size_t)item * bd.s_pix + r) * U;
+    const int cnt = bd.ucnt[item * bd.s_pix + r];
+    const int* off = bd.boff + item * (bd.s_cell + 2);
+    int* fill = bd.bfill + item * (bd.s_cell + 2);
+    const unsigned pos_r = (unsigned)bd.key[item * bd.s_pix + r];
+    const size_t eb = (size_t)item * bd.s_pix * U;
+    for (int e = 0; e < cnt; e++) {
+        const int c = bd.ucol[rb + e];
+        const int o = off[c] + atomicAdd(&fill[c], 1);
+        bd.lcj[eb + o] = pos_r;
+        bd.lv[eb + o] = bd.uval[rb + e];
+    }
 }
 
-// ascending original order inside each bucket: each wave ranks one bucket's slice (buckets hold a
-// few rows each, so a workgroup per bucket left most of its lanes idle through two barriers)
-__global__ void __launch_bounds__(256) k_bucket_rank_b(BatchDev bd)
+// one wave per column: rank by row position (positions are distinct within a column), write the
+// pixel coordinates and values in that order
+__global__ void __launch_bounds__(256) k_csc_order_b(BatchDev bd)
 {
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     if (!M.valid) return;
-    const int* off = bd.boff + item * (bd.s_cell + 2);
-    int* perm = bd.perm + item * bd.s_pix;
-    int* key = bd.key + item * bd.s_pix;  // reused as scratch for the ranked slice
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int k0 = blockIdx.x * 4; k0 < M.n_src; k0 += gridDim.x * 4) {
-        const int k = k0 + w;
-        int lo = 0, n = 0;
-        if (k < M.n_src) {
-            lo = off[k];
-            n = off[k + 1] - lo;
+    const int* off = bd.boff + item * (bd.s_cell + 2);
+    const size_t eb = (size_t)item * bd.s_pix * 3 * bd.nsub;
+    const unsigned* kk = bd.lcj + eb;
+    const double* vv = bd.lv + eb;
+    const int* perm = bd.perm + item * bd.s_pix;
+    const int* act = bd.act_pix + item * bd.s_pix;
+    for (int c = blockIdx.x * 4 + w; c < M.n_src; c += gridDim.x * 4) {
+        const int lo = off[c], n = off[c + 1] - lo;
+        for (int a = lane; a < n; a += 64) {
+            const unsigned k = kk[lo + a];
+            int rank = 0;
+            for (int b = 0; b < n; b++) rank += (kk[lo + b] < k) ? 1 : 0;
+            const int p = act[perm[k]];
+            const unsigned i = (unsigned)(p / bd.ny), j = (unsigned)(p - (p / bd.ny) * bd.ny);
+            bd.cpix[eb + lo + rank] = (i << 16) | j;
+            bd.cval[eb + lo + rank] = vv[lo + a];
         }
-        // rank = number of smaller row ids in the slice (ids are distinct); write to key[lo+rank]
-        if (n > 1)
-            for (int i = lane; i < n; i += 64) {
-                const int v = perm[lo + i];
-                int rank = 0;
-                for (int j = 0; j < n; j++) rank += (perm[lo + j] < v);
-                key[lo + rank] = v;
-            }
-        __syncthreads();
-        if (n > 1)
-            for (int i = lane; i < n; i += 64) perm[lo + i] = key[lo + i];
-        __syncthreads();
     }
 }
 
-void launch_sort_rows_b(hipStream_t s, const BatchDev& bd, int max_img, int max_src)
+void launch_lcols_b(hipStream_t s, const BatchDev& bd, int max_img, int max_src)
 {
-    if (max_img <= 0) return;
+    if (max_img <= 0 || max_src <= 0) return;
     launch_fill_i32(s, bd.bcnt, 0, (size_t)bd.B * (bd.s_cell + 2));
-    hipLaunchKernelGGL(k_row_keys_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, s, bd);
-    hipLaunchKernelGGL(k_bucket_scan_b, dim3(1, bd.B), dim3(1024), 0, s, bd);
-    hipLaunchKernelGGL(k_row_scatter_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, s, bd);
-    hipLaunchKernelGGL(k_bucket_rank_b, dim3(std::min((max_src + 3) / 4, 256), bd.B), dim3(256), 0, s, bd);
+    hipLaunchKernelGGL(k_csc_count_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, s, bd);
+    hipLaunchKernelGGL(k_csc_scan_b, dim3(1, bd.B), dim3(1024), 0, s, bd);
+    hipLaunchKernelGGL(k_csc_scatter_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, s, bd);
+    hipLaunchKernelGGL(k_csc_order_b, dim3(std::min((max_src + 3) / 4, 512), bd.B), dim3(256), 0, s, bd);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -688,265 +667,165 @@ __device__ __forceinline__ int mask_rank(const unsigned* m, int cb)
 __device__ __forceinline__ bool mask_has(const unsigned* m, int cb) { return (m[cb >> 5] >> (cb & 31)) & 1u; }
 
 // ------------------------------------------------------------------------------------------------
-// Blurred rows of one 64-row chunk into the chunk's support blocks: one workgroup per chunk, each
-// wave accumulating one row at a time in LDS.  Row r (pixel (k, l)) sums psf[pk][pl] *
-// L(pixel (k + nxh - pk, l + nyh - pl), :) over the taps.  Per window column pl the contributing
-// pixels (x in [i0, i1], y = l + nyh - pl) own one contiguous range of the CSR L entries, found by
-// two eptr lookups (made one row ahead, lane pl for column pl).  Each range is cut into quarters
-// of at most 16 entries; a wave scan numbers the quarters and each lane writes its column's
-// quarters into an LDS table {first entry, weight base, count}.  The lanes then take the quarters
-// four at a time (16 lanes each) with coalesced loads, 32 per lane in flight and no branches (idle
-// lanes add zero), and add each product into the row's support columns in LDS.
-//   FP64 path: FP64 weights, ds_add_f64 into FP64 rows, a pool of doubles.
-//   Mixed path: products rounded in FP32 to int32 multiples of 1/fixq and summed with ds_add_u32
-//   (exact and order-independent; ds_add_f32 runs at about a seventh of the rate of the integer
-//   and FP64 LDS atomics on gfx950), then scaled by sqrt(c_r) s16 and rounded to one FP16 pool
-//   (s16 a power of two with |x s16| <= 2^14; 11 significant bits per entry).
+// Blurred rows of one 64-row chunk into the chunk's support blocks (PSF_convolution_Lmatrix_dense,
+// direct branch, pixelgrid.cpp:14092-14176): Lb[r][c] = sum over pixels (i, j) of the chunk's PSF
+// reach of psf[k_r + hx - i][l_r + hy - j] * L[(i, j)][c], taps inside the PSF only, (i, j) active.
+// One workgroup per chunk; lane l of every wave owns chunk row l (the rows are image-compact, so an
+// L entry of a pixel near the chunk reaches most of its rows).  Per support column c (columns of a
+// block dealt to the four waves), the wave walks column c's entries in L's CSC form, keeps those
+// whose pixel lies in the chunk's PSF reach (a ballot over 64 entries at a time), broadcasts each
+// kept entry and every lane adds its own tap weight times the value: the output element is summed
+// in registers in a fixed order and written once, with no atomics.  A block's 64 x CB values go to
+// an LDS tile and from there to the pool with 16-byte stores.
+//   T = double: FP64 pool, 64-column blocks, row-major, unscaled (the FP64 SYRK applies C);
+//   T = float:  FP16 pool, 128-column blocks, k4-interleaved (element (k, i) at
+//               ((k >> 2) * CB + i) * 4 + (k & 3), so the SYRK stages blocks with 16-byte copies),
+//               entries x = Lb sqrt(c_r) s16 rounded once (s16 a power of two, |x| <= 2^14).
+// Column n_src carries the data: d (FP64) or d sqrt(c_r) s16 (FP16).
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ uint16_t f32_to_f16_rne(float f) { return __builtin_bit_cast(uint16_t, (_Float16)f); }
 
-__device__ __forceinline__ void wave_sync()
+__device__ __forceinline__ double readlane_f64(double v, int l)
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ float readlane_f32(float v, int l)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+template <typename T> __device__ __forceinline__ T readlane_t(T v, int l);
+template <> __device__ __forceinline__ double readlane_t<double>(double v, int l) { return readlane_f64(v, l); }
+template <> __device__ __forceinline__ float readlane_t<float>(float v, int l) { return readlane_f32(v, l); }
+
+__device__ __forceinline__ int wave_min_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
 }
 
-// CSR entry range [e0, e1) of window column pl of the row at pixel p (empty outside the image)
-__device__ __forceinline__ void window_range(const BatchDev& bd, const int* eptr, int p, int pl, int& e0, int& e1)
+template <typename T>
+__global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int psf_bytes)
 {
-    e0 = 0;
-    e1 = 0;
-    if (p < 0 || pl >= bd.pny) return;
-    const int k = p / bd.ny, l = p - k * bd.ny;
-    const int nxh = bd.pnx / 2, nyh = bd.pny / 2;
-    const int jj = l + nyh - pl;
-    if (jj < 0 || jj >= bd.ny) return;
-    const int i0 = max(0, k + nxh - bd.pnx + 1), i1 = min(bd.nx - 1, k + nxh);
-    const long long t0 = (long long)jj * bd.nx;
-    e0 = eptr[t0 + i0];
-    e1 = eptr[t0 + i1 + 1];
-}
-
-#define BUILD_U 48              // entry slots per lane and pass
-#define BUILD_TQ (4 * BUILD_U)  // quarters per pass
-
-template <bool FIX>
-struct BuildTypes {             // FP64 path
-    typedef double acc_t;
-    typedef double val_t;
-    typedef double w_t;
-};
-template <>
-struct BuildTypes<true> {       // mixed path: int32 fixed point
-    typedef unsigned acc_t;
-    typedef float val_t;
-    typedef float w_t;
-};
-
-// window columns cg .. cg+63 of the row at pixel x index k: lane c holds [e0, e0 + len) of column cg + c
-template <bool FIX>
-__device__ __forceinline__ void accumulate_columns(const BatchDev& bd, const unsigned* lcj, const typename BuildTypes<FIX>::val_t* lval,
-                                                   const typename BuildTypes<FIX>::w_t* W, const short* loc, int cbs,
-                                                   typename BuildTypes<FIX>::acc_t* row, int4* qtab, int k, int cg, int e0, int len)
-{
-    typedef typename BuildTypes<FIX>::val_t V;
-    typedef typename BuildTypes<FIX>::w_t WT;
-    const int lane = threadIdx.x & 63, CB = 1 << cbs;
-    const int nq = (len + 15) >> 4;
-    const int qincl = wave_incl_scan(nq);
-    const int QS = qincl - nq;
-    const int NQ = __builtin_amdgcn_readlane(qincl, 63);
-    const int wbc = (cg + lane) * bd.pnx + k + bd.pnx / 2;
-    for (int g0 = 0; g0 < NQ; g0 += BUILD_TQ) {
-        const int gend = min(QS + nq, g0 + BUILD_TQ);
-        for (int g = max(QS, g0); g < gend; g++) {
-            const int x = g - QS;
-            qtab[g - g0] = make_int4(e0 + 16 * x, wbc, min(16, len - 16 * x), 0);
-        }
-        wave_sync();
-        // slot u, lane l: quarter (u >> 2) + G * (l >> 2), entries 4 * (u & 3) + (l & 3).  A lane's
-        // four entries of a quarter are mostly one pixel's (distinct columns) and the 16 quarters of
-        // one instruction lie G apart, so the LDS atomics of an instruction rarely share an address;
-        // slot groups beyond the quarters of the pass are skipped
-        const int ng = min(BUILD_TQ, NQ - g0);
-        const int G = (ng + 15) >> 4;
-        unsigned cjs[BUILD_U];
-        V vs[BUILD_U];
-        int wbq[BUILD_U / 4];  // weight base of the quarter that slots 4g .. 4g+3 read
-#pragma unroll
-        for (int g = 0; g < BUILD_U / 4; g++) {
-            wbq[g] = 0;
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                cjs[4 * g + j] = 0;
-                vs[4 * g + j] = (V)0;
-            }
-            if (g < G) {
-                const int gl = g + G * (lane >> 2);
-                const int4 t = qtab[min(gl, ng - 1)];
-                wbq[g] = t.y;
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int sub = 4 * j + (lane & 3);
-                    const bool ok = gl < ng && sub < t.z;
-                    const int e = t.x + (ok ? sub : 0);
-                    cjs[4 * g + j] = lcj[e];
-                    const V v = lval[e];
-                    vs[4 * g + j] = ok ? v : (V)0;
-                }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < BUILD_U; u++) {
-            if ((u >> 2) < G) {
-                const int col = (int)(cjs[u] & 16383u);
-                const WT wt = W[wbq[u >> 2] - (int)(cjs[u] >> 14)];
-                const int slot = (loc[col >> cbs] << cbs) + (col & (CB - 1));
-                if (FIX)
-                    atomicAdd((unsigned*)row + slot, (unsigned)__float2int_rn((float)(wt * vs[u])));
-                else
-                    atomicAdd((double*)row + slot, (double)(wt * vs[u]));
-            }
-        }
-        wave_sync();  // the next pass rewrites qtab
-    }
-}
-
-// FIX: mixed path (int32 fixed-point rows, an FP16 pool); else FP64.  Occupancy: the mixed path's
-// 165 registers give three waves per SIMD (as many as its LDS allows on the benchmark workload);
-// 32 slots per lane fit four waves but make rows of ~140 quarters take two passes (slower there).
-// Split into two kernels with their own occupancy attributes, the mixed one ran 8.5 against 7.2 ms.  The transposed PSF (times
-// fixq on the mixed path) is staged in LDS.  Dynamic LDS: [PSF] [nw x BUILD_TQ quarter table]
-// [nw x nsup_cap x CB rows].
-template <bool FIX>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3)))
-k_build_lb_b(BatchDev bd, int nsup_cap, int psf_elems)
-{
-    typedef typename BuildTypes<FIX>::acc_t A;
-    typedef typename BuildTypes<FIX>::w_t WT;
     extern __shared__ double lds_raw[];
-    __shared__ short loc[QLX_MAX_LD / TS];
-    __shared__ int prow[TS];
-    __shared__ double rdat[4], rscl[4];  // mixed path: data value and sqrt(c)/fixq of each wave's row
     const int item = blockIdx.y, kt = blockIdx.x;
     const ItemMeta& M = bd.meta[item];
     if (!M.valid || kt >= M.nkt) return;
-    const int nw = blockDim.x >> 6;
+    constexpr bool F64 = sizeof(T) == 8;
     const int cbs = M.cbs, CB = 1 << cbs;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    const int ntap = bd.pnx * bd.pny;
-    WT* lds_w = (WT*)lds_raw;
-    int4* qtab = (int4*)(lds_raw + psf_elems) + w * BUILD_TQ;
-    A* lds_rows = (A*)((int4*)(lds_raw + psf_elems) + nw * BUILD_TQ);
-    A* row = lds_rows + (size_t)w * nsup_cap * CB;
-    const unsigned* m = bd.nz + M.nz_off + (size_t)kt * M.ncbw;
-    for (int cb = tid; cb < M.ncb; cb += blockDim.x) loc[cb] = mask_has(m, cb) ? (short)mask_rank(m, cb) : (short)-1;
-    for (int t = tid; t < ntap; t += blockDim.x) lds_w[t] = FIX ? (WT)(bd.psfT[t] * bd.fixq) : (WT)bd.psfT[t];
-    for (int t = tid; t < TS; t += blockDim.x) {
-        const int rs = kt * TS + t;
-        prow[t] = rs < M.n_img ? bd.act_pix[item * bd.s_pix + bd.perm[item * bd.s_pix + rs]] : -1;
-    }
-    const int sbase = bd.bbase[M.bb_off + kt];
-    const int nsup = bd.bbase[M.bb_off + kt + 1] - sbase;
-    const size_t bsz = (size_t)TS * CB;  // elements of one pool block
-    const int* eptr = bd.eptr + item * (bd.s_pix + 1);
-    const unsigned* lcj = bd.lcj + (size_t)item * bd.s_pix * 3 * bd.nsub;
-    const typename BuildTypes<FIX>::val_t* lval;
-    if constexpr (FIX) lval = bd.lvf + (size_t)item * bd.s_pix * 3 * bd.nsub;
-    else lval = bd.lv + (size_t)item * bd.s_pix * 3 * bd.nsub;
-    __syncthreads();
-    const int dcol = (loc[M.n_src >> cbs] << cbs) + (M.n_src & (CB - 1));  // the data column's slot
-    const int rpw = TS / nw;  // rows per wave
-    int pe0, pe1;             // window column `lane` of the wave's next row
-    window_range(bd, eptr, prow[w], lane, pe0, pe1);
-    for (int j = 0; j < rpw; j++) {
-        const int rr = j * nw + w;
-        const int p = prow[rr];
-        const int e0 = pe0, len = pe1 - pe0;
-        if (j + 1 < rpw) window_range(bd, eptr, prow[rr + nw], lane, pe0, pe1);
-        for (int c = lane; c < nsup * CB; c += 64) row[c] = (A)0;
-        wave_sync();
-        if (p >= 0) {
-            const int k = p / bd.ny;
-            accumulate_columns<FIX>(bd, lcj, lval, lds_w, loc, cbs, row, qtab, k, 0, e0, len);
-            for (int cg = 64; cg < bd.pny; cg += 64) {
-                int f0, f1;
-                window_range(bd, eptr, p, cg + lane, f0, f1);
-                accumulate_columns<FIX>(bd, lcj, lval, lds_w, loc, cbs, row, qtab, k, cg, f0, f1 - f0);
-            }
-        }
-        if constexpr (!FIX) {
-            if (p >= 0 && lane == 0) row[dcol] = bd.data[p];
-            wave_sync();
-            double* pool = (double*)bd.lb + M.lb_off;
-            for (int q = 0; q < nsup; q++)
-                for (int cc = lane; cc < CB; cc += 64) pool[(size_t)(sbase + q) * bsz + (size_t)rr * CB + cc] = row[q * CB + cc];
-            wave_sync();
+    const int pnx = bd.pnx, pny = bd.pny, ntap = pnx * pny;
+    T* psf_l = (T*)lds_raw;
+    char* tile = (char*)lds_raw + psf_bytes;
+    for (int t = tid; t < ntap; t += 256) psf_l[t] = (T)bd.psf[t];
+    // this lane's row
+    const int rs = kt * TS + lane;
+    const int p = rs < M.n_img ? bd.act_pix[item * bd.s_pix + bd.perm[item * bd.s_pix + rs]] : -1;
+    const int kr = p >= 0 ? p / bd.ny : 0, lr = p >= 0 ? p - (p / bd.ny) * bd.ny : 0;
+    // the chunk's PSF reach: pixels (i, j) with i in [kmin + hx - pnx + 1, kmax + hx], likewise j
+    const int hx = pnx / 2, hy = pny / 2;
+    const int kmin = wave_min_i(p >= 0 ? kr : 1 << 30), kmax = wave_max_i(p >= 0 ? kr : -(1 << 30));
+    const int lmin = wave_min_i(p >= 0 ? lr : 1 << 30), lmax = wave_max_i(p >= 0 ? lr : -(1 << 30));
+    const int i_lo = kmin + hx - pnx + 1, j_lo = lmin + hy - pny + 1;
+    const unsigned i_span = (unsigned)(kmax - kmin + pnx - 1), j_span = (unsigned)(lmax - lmin + pny - 1);
+    const int kr2 = kr + hx, lr2 = lr + hy;  // tap (kr2 - i, lr2 - j) of pixel (i, j)
+    double sc = 0.0, dv = 0.0;
+    if (p >= 0) {
+        if constexpr (F64) {
+            dv = bd.data[p];
         } else {
-            // int32 multiples of 1/fixq -> x = value * sqrt(c_r) * s16 (the data column: data * sqrt(c_r) * s16)
-            const double sc = p >= 0 ? sqrt(bd.covinv[p]) * bd.s16 : 0.0;
-            const double dv = p >= 0 ? bd.data[p] * sc : 0.0;
-            const double sq = sc / bd.fixq;
-            uint16_t* pool = (uint16_t*)bd.lb + M.lb_off;
-            if (nw == 4) {
-                // k4-interleaved FP16 blocks: element (k, i) of a block at ((k >> 2) * CB + i) * 4 + (k & 3),
-                // so the SYRK stages blocks with plain 16-byte copies and reads 8 consecutive k per lane;
-                // the four waves' rows j*4 .. j*4+3 are one k-group
-                if (lane == 0) {
-                    rdat[w] = dv;
-                    rscl[w] = sq;
-                }
-                __syncthreads();
-                for (int idx = tid; idx < nsup * CB; idx += 256) {
-                    const int q = idx >> cbs, i = idx & (CB - 1);
-                    uint16_t h[4];
-#pragma unroll
-                    for (int w2 = 0; w2 < 4; w2++) {
-                        const double xv = idx == dcol ? rdat[w2] : (double)(int)lds_rows[(size_t)w2 * nsup_cap * CB + idx] * rscl[w2];
-                        h[w2] = f32_to_f16_rne((float)xv);
-                    }
-                    const size_t o = (size_t)(sbase + q) * bsz + ((size_t)j * CB + i) * 4;
-                    *(uint2*)&pool[o] = make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
-                }
-                __syncthreads();
-            } else {
-                wave_sync();
-                for (int q = 0; q < nsup; q++)
-                    for (int i = lane; i < CB; i += 64) {
-                        const int idx = q * CB + i;
-                        const float x = (float)(idx == dcol ? dv : (double)(int)row[idx] * sq);
-                        pool[(size_t)(sbase + q) * bsz + ((size_t)(rr >> 2) * CB + i) * 4 + (rr & 3)] = f32_to_f16_rne(x);
-                    }
-                wave_sync();
-            }
+            sc = sqrt(bd.covinv[p]) * bd.s16;
+            dv = bd.data[p] * sc;
         }
     }
-}
-
-template <bool FIX>
-static int launch_build_typed(hipStream_t s, const BatchDev& bd, int max_nkt, int max_nsup, int cbw)
-{
-    typedef typename BuildTypes<FIX>::acc_t A;
-    typedef typename BuildTypes<FIX>::w_t WT;
-    const size_t per_wave = (size_t)max_nsup * cbw * sizeof(A) + BUILD_TQ * sizeof(int4);
-    const size_t ntap = (size_t)bd.pnx * bd.pny;
-    const size_t psf_bytes = (ntap * sizeof(WT) + 15) / 16 * 16;
-    if (psf_bytes > 64 * 1024 || psf_bytes + per_wave > 144 * 1024) return -1;  // PSF / support beyond the LDS budget
-    int nw = 4;
-    while (nw > 1 && psf_bytes + per_wave * nw > 144 * 1024) nw >>= 1;
-    const size_t lds = psf_bytes + per_wave * nw;
-    const int psf_elems = (int)(psf_bytes / 8);  // in doubles: the offset of the quarter tables
-    allow_big_lds((const void*)k_build_lb_b<FIX>);
-    hipLaunchKernelGGL((k_build_lb_b<FIX>), dim3(max_nkt, bd.B), dim3(64 * nw), lds, s, bd, max_nsup, psf_elems);
-    return 0;
+    const unsigned* m = bd.nz + M.nz_off + (size_t)kt * M.ncbw;
+    const int sbase = bd.bbase[M.bb_off + kt];
+    const size_t bsz = (size_t)TS * CB;  // elements of one pool block
+    const size_t eb = (size_t)item * bd.s_pix * 3 * bd.nsub;
+    const unsigned* cpix = bd.cpix + eb;
+    const double* cval = bd.cval + eb;
+    const int* cptr = bd.boff + item * (bd.s_cell + 2);
+    __syncthreads();
+    int q = 0;
+    for (int wd = 0; wd < M.ncbw; wd++) {
+        unsigned bits = m[wd];
+        while (bits) {
+            const int cb = wd * 32 + __builtin_ctz(bits);
+            bits &= bits - 1;
+            for (int ii = w; ii < CB; ii += 4) {
+                const int c = (cb << cbs) + ii;
+                T acc = (T)0;
+                if (c < M.n_src) {
+                    const int e1 = cptr[c + 1];
+                    for (int e0 = cptr[c]; e0 < e1; e0 += 64) {
+                        const int e = e0 + lane;
+                        unsigned pk = 0;
+                        T v = (T)0;
+                        bool rel = false;
+                        if (e < e1) {
+                            pk = cpix[e];
+                            rel = (unsigned)((int)(pk >> 16) - i_lo) <= i_span && (unsigned)((int)(pk & 0xffffu) - j_lo) <= j_span;
+                            if (rel) v = (T)cval[e];
+                        }
+                        unsigned long long bal = __ballot(rel);
+                        while (bal) {
+                            const int b = __builtin_ctzll(bal);
+                            bal &= bal - 1;
+                            const unsigned pkb = (unsigned)__builtin_amdgcn_readlane((int)pk, b);
+                            const T vb = readlane_t<T>(v, b);
+                            const int tp = kr2 - (int)(pkb >> 16), tq = lr2 - (int)(pkb & 0xffffu);
+                            if (p >= 0 && (unsigned)tp < (unsigned)pnx && (unsigned)tq < (unsigned)pny) acc += psf_l[tp * pny + tq] * vb;
+                        }
+                    }
+                }
+                if constexpr (F64) {
+                    ((double*)tile)[lane * TS + ii] = c == M.n_src ? dv : (double)acc;
+                } else {
+                    const double x = c == M.n_src ? dv : (double)acc * sc;
+                    ((uint16_t*)tile)[((lane >> 2) * CB + ii) * 4 + (lane & 3)] = f32_to_f16_rne((float)x);
+                }
+            }
+            __syncthreads();
+            // the tile is the pool block image: 256 threads copy it with 16-byte stores
+            {
+                const int nvec = (int)(bsz * (F64 ? 8 : 2) / 16);  // 16-byte vectors per block (tile_bytes / 16)
+                uint4* dst = (uint4*)((char*)bd.lb + (M.lb_off + (size_t)(sbase + q) * bsz) * (F64 ? 8 : 2));
+                const uint4* src = (const uint4*)tile;
+                for (int v = tid; v < nvec; v += 256) dst[v] = src[v];
+            }
+            __syncthreads();
+            q++;
+        }
+    }
 }
 
 int launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_nsup, int mixed, int cbw)
 {
     if (max_rows <= 0 || max_nsup <= 0) return 0;
     const int max_nkt = (max_rows + TS - 1) / TS;
-    return mixed ? launch_build_typed<true>(s, bd, max_nkt, max_nsup, cbw) : launch_build_typed<false>(s, bd, max_nkt, max_nsup, cbw);
+    const size_t ntap = (size_t)bd.pnx * bd.pny;
+    const size_t esz = mixed ? 4 : 8;
+    const size_t psf_bytes = (ntap * esz + 15) / 16 * 16;
+    const size_t tile_bytes = (size_t)TS * cbw * (mixed ? 2 : 8);
+    if (psf_bytes > 64 * 1024) return -1;  // PSF beyond the LDS budget
+    const size_t lds = psf_bytes + tile_bytes;
+    if (mixed) {
+        allow_big_lds((const void*)k_build_lb_b<float>);
+        hipLaunchKernelGGL(k_build_lb_b<float>, dim3(max_nkt, bd.B), dim3(256), lds, s, bd, (int)psf_bytes);
+    } else {
+        allow_big_lds((const void*)k_build_lb_b<double>);
+        hipLaunchKernelGGL(k_build_lb_b<double>, dim3(max_nkt, bd.B), dim3(256), lds, s, bd, (int)psf_bytes);
+    }
+    return 0;
 }
 
 __global__ void k_gather_cvec_b(BatchDev bd)

@@ -83,7 +83,7 @@ struct Ctx {
     int pnx = 0, pny = 0;
     bool psf_set = false;
     std::vector<double> h_psf;
-    DevBuf psf, psfT;
+    DevBuf psf;
     // settings
     qlx_settings set{};
     bool settings_set = false;
@@ -91,7 +91,7 @@ struct Ctx {
     DevBuf d_models, d_sgrid, d_srcn, d_meta, d_mats;
     // per-item work (stride s_pix / s_cell)
     DevBuf beta, map_cell, map_w, pmapped, cmapped, sidx, tix, pisrc, pidx, act_pix, counts, ucol, uval, ucnt, nevals;
-    DevBuf apre, lptr, eptr, lcj, lv, lvf, nfwtab, pmask, hmask;
+    DevBuf apre, eptr, lcj, lv, cpix, cval, nfwtab, pmask, hmask;
     DevBuf key, bcnt, boff, bfill, perm;
     // pools
     DevBuf lb, F, Ff, R, Rf, cvec, vec, nz, bbase, winv, model, out, status;  // Rf: R's copy for its factorization
@@ -356,8 +356,8 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
               c->pidx.ensure(B * s_pix * 4) && c->act_pix.ensure(B * s_pix * 4) &&
               c->counts.ensure((size_t)B * 16) && c->ucol.ensure(B * s_pix * U * 4) && c->uval.ensure(B * s_pix * U * 8) &&
               c->ucnt.ensure(B * s_pix * 4) && c->nevals.ensure((size_t)B * 8) && c->key.ensure(B * s_pix * 4) &&
-              c->apre.ensure(B * (s_pix + 1) * 4) && c->lptr.ensure(B * (s_pix + 1) * 4) && c->eptr.ensure(B * (s_pix + 1) * 4) && c->lcj.ensure(B * s_pix * U * 4) &&
-              c->lv.ensure(B * s_pix * U * 8) && c->lvf.ensure(B * s_pix * U * 4) &&
+              c->apre.ensure(B * (s_pix + 1) * 4) && c->eptr.ensure(B * (s_pix + 1) * 4) && c->lcj.ensure(B * s_pix * U * 4) &&
+              c->lv.ensure(B * s_pix * U * 8) && c->cpix.ensure(B * s_pix * U * 4) && c->cval.ensure(B * s_pix * U * 8) &&
               c->nfwtab.ensure((size_t)B * QLX_NFWTAB_DOUBLES * 8) &&
               c->pmask.ensure((size_t)B * s_pix * QLX_MAX_CBW * 4) && c->hmask.ensure((size_t)B * s_pix * QLX_MAX_CBW * 4) &&
               c->bcnt.ensure(B * (s_cell + 2) * 4) && c->boff.ensure(B * (s_cell + 2) * 4) && c->bfill.ensure(B * (s_cell + 2) * 4) &&
@@ -382,7 +382,6 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     bd.data = c->data.as<double>();
     bd.covinv = c->covinv.as<double>();
     bd.psf = c->psf.as<double>();
-    bd.psfT = c->psfT.as<double>();
     bd.pnx = c->pnx;
     bd.pny = c->pny;
     bd.tol = S.integral_tol;
@@ -406,23 +405,22 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     bd.uval = c->uval.as<double>();
     bd.ucnt = c->ucnt.as<int>();
     bd.apre = c->apre.as<int>();
-    bd.lptr = c->lptr.as<int>();
     bd.eptr = c->eptr.as<int>();
     bd.lcj = c->lcj.as<unsigned>();
     bd.lv = c->lv.as<double>();
-    bd.lvf = c->lvf.as<float>();
+    bd.cpix = c->cpix.as<unsigned>();
+    bd.cval = c->cval.as<double>();
     bd.nfwtab = c->nfwtab.as<double>();
     bd.pmask = c->pmask.as<unsigned>();
     bd.hmask = c->hmask.as<unsigned>();
     {
+        // mixed-path pool entries are x = (blurred L entry) sqrt(c) or data sqrt(c), with
         // |blurred L entry| <= sum|psf| * max_r ||L_r||_1 <= 3 sum|psf| (per subpixel the 3-point
-        // weights are 1/nsub times barycentric coordinates with |w1|, |w2| <= 1/2, 0 <= w0 <= 2),
-        // so int32 multiples of 2^-29 sum|psf| cannot overflow
+        // weights are 1/nsub times barycentric coordinates with |w1|, |w2| <= 1/2, 0 <= w0 <= 2); the
+        // power of two s16 puts the bound on |x| at or below 2^14, inside FP16's range with a factor
+        // 4 to spare
         double apsf = 0.0;
         for (double v : c->h_psf) apsf += fabs(v);
-        bd.fixq = std::ldexp(1.0, 29) / std::max(apsf, 1e-300);
-        // mixed-path pool entries are x = (blurred L entry) sqrt(c) or data sqrt(c); the power of two
-        // s16 puts the bound on |x| at or below 2^14, inside FP16's range with a factor 4 to spare
         const double xmax = std::max(3.0 * apsf * c->max_sqrtc, c->max_dsc);
         int e = 0;
         if (xmax > 0 && std::isfinite(xmax)) e = (int)std::floor(std::log2(16384.0 / xmax));
@@ -586,8 +584,8 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     const int nF = (int)fm.size(), nR = (int)rm.size();
     timer_begin(c, "lbuild");
     launch_merge_lrows_b(st, bd, max_img);
-    launch_lrows_csr_b(st, bd, max_img);
     launch_sort_rows_b(st, bd, max_img, max_src);
+    launch_lcols_b(st, bd, max_img, max_src);
     launch_row_support_b(st, bd, max_nkt);
     HIPCHK(c, hipMemcpyAsync(counts.data(), bd.counts, (size_t)B * 16, hipMemcpyDeviceToHost, st));
     c->nzh.resize(nz_tot);
@@ -858,11 +856,9 @@ int qlx_set_image(void* ctx, int nx, int ny, const double* xvals, const double* 
     if (!c) return QLX_ERR_ARG;
     c->kept.clear();  // solutions of earlier calls no longer match the inputs
     c->last_wave.clear();
-    c->last_wave.clear();
-    c->last_wave.clear();
     if (nx < 1 || ny < 1 || !xvals || !yvals || !data || !covinv) return fail(c, QLX_ERR_ARG, "qlx_set_image: bad arguments");
     if ((long long)nx * ny > (1LL << 24)) return fail(c, QLX_ERR_RANGE, "image too large");
-    if (nx >= (1 << 18)) return fail(c, QLX_ERR_RANGE, "image wider than 262143 pixels in x");
+    if (nx > 65535 || ny > 65535) return fail(c, QLX_ERR_RANGE, "image wider or taller than 65535 pixels");
     hipSetDevice(c->device);
     size_t npix = (size_t)nx * ny;
     c->nx = nx;
@@ -899,20 +895,14 @@ int qlx_set_psf(void* ctx, int pnx, int pny, const double* psf)
     if (!c) return QLX_ERR_ARG;
     c->kept.clear();  // solutions of earlier calls no longer match the inputs
     c->last_wave.clear();
-    c->last_wave.clear();
-    c->last_wave.clear();
     if (pnx < 1 || pny < 1 || !psf || (pnx % 2) == 0 || (pny % 2) == 0) return fail(c, QLX_ERR_ARG, "PSF must be odd-sized and non-empty");
     if ((long long)pnx * pny > 4096 * 4096) return fail(c, QLX_ERR_RANGE, "PSF too large");
     hipSetDevice(c->device);
     c->pnx = pnx;
     c->pny = pny;
     c->h_psf.assign(psf, psf + (size_t)pnx * pny);
-    std::vector<double> tr((size_t)pnx * pny);
-    for (int i = 0; i < pnx; i++)
-        for (int j = 0; j < pny; j++) tr[(size_t)j * pnx + i] = psf[(size_t)i * pny + j];
-    if (!c->psf.ensure((size_t)pnx * pny * 8) || !c->psfT.ensure((size_t)pnx * pny * 8)) return fail(c, QLX_ERR_NOMEM, "psf allocation failed");
+    if (!c->psf.ensure((size_t)pnx * pny * 8)) return fail(c, QLX_ERR_NOMEM, "psf allocation failed");
     HIPCHK(c, hipMemcpyAsync(c->psf.p, psf, (size_t)pnx * pny * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->psfT.p, tr.data(), (size_t)pnx * pny * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->psf_set = true;
     return QLX_OK;
@@ -954,8 +944,6 @@ int qlx_set_settings(void* ctx, const qlx_settings* s)
     Ctx* c = (Ctx*)ctx;
     if (!c) return QLX_ERR_ARG;
     c->kept.clear();  // solutions of earlier calls no longer match the inputs
-    c->last_wave.clear();
-    c->last_wave.clear();
     c->last_wave.clear();
     if (!s) return fail(c, QLX_ERR_ARG, "null settings");
     if (s->nsplit < 1 || s->nsplit > 4) return fail(c, QLX_ERR_RANGE, "nsplit must be 1..4");

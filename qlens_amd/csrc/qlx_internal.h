@@ -79,7 +79,6 @@ struct BatchDev {
     const double* data;
     const double* covinv;
     const double* psf;
-    const double* psfT;         // the PSF transposed: psfT[pl * pnx + pk] = psf[pk * pny + pl]
     int pnx, pny;
     double tol, lambda;
     // per-item inputs
@@ -104,20 +103,19 @@ struct BatchDev {
     double* uval;               // s_pix*3*nsub
     int* ucnt;                  // s_pix
     int* apre;                  // s_pix+1: active pixels before pixel t = j*nx + i (image order j-outer)
-    int* lptr;                  // s_pix+1: first CSR entry of each active pixel's merged L row
-    int* eptr;                  // s_pix+1: CSR entries of the active pixels before pixel t = j*nx + i
-    unsigned* lcj;              // s_pix*3*nsub: CSR entry column (bits 0-13) | pixel x index i << 14
-    double* lv;                 // s_pix*3*nsub: CSR entry value
-    float* lvf;                 // s_pix*3*nsub: CSR entry value in FP32 (mixed path)
+    int* eptr;                  // s_pix+1: first row-order position of each 8 x 8 pixel tile
+    unsigned* lcj;              // s_pix*3*nsub: L by columns, unordered: row position of each entry
+    double* lv;                 // s_pix*3*nsub: L by columns, unordered: value of each entry
+    unsigned* cpix;             // s_pix*3*nsub: L by columns, ordered: pixel (i << 16 | j) of each entry
+    double* cval;               // s_pix*3*nsub: L by columns, ordered: value of each entry
     unsigned* pmask;            // s_pix*QLX_MAX_CBW: column blocks touched by each active pixel's L row
     unsigned* hmask;            // s_pix*QLX_MAX_CBW per grid pixel (y-major): pmask ORed over the PSF's x window
-    double fixq;                // mixed path: blurred rows accumulate as int32 multiples of 1/fixq
     double s16;                 // mixed path: power of two that scales the FP16 pool entries (|x s16| <= 2^14)
-    int* key;                   // s_pix
-    int* bcnt;                  // s_cell+2
-    int* boff;                  // s_cell+2
-    int* bfill;                 // s_cell+2
-    int* perm;                  // s_pix
+    int* key;                   // s_pix: row-order position of each active row
+    int* bcnt;                  // s_cell+2: entries per L column
+    int* boff;                  // s_cell+2: first entry of each L column (n_src + 1 bounds)
+    int* bfill;                 // s_cell+2: scatter cursors
+    int* perm;                  // s_pix: active row at each row-order position
     unsigned long long* nevals; // 1 per item
     long long s_pix, s_cell;
     // pools and outputs
@@ -223,8 +221,8 @@ void launch_fill_i32(hipStream_t s, int* p, int v, size_t n);
 void launch_fill_f64(hipStream_t s, double* p, double v, size_t n);
 void launch_compact_b(hipStream_t s, const BatchDev& bd);
 void launch_merge_lrows_b(hipStream_t s, const BatchDev& bd, int max_img);
-void launch_lrows_csr_b(hipStream_t s, const BatchDev& bd, int max_img);
 void launch_sort_rows_b(hipStream_t s, const BatchDev& bd, int max_img, int max_src);
+void launch_lcols_b(hipStream_t s, const BatchDev& bd, int max_img, int max_src);
 void launch_row_support_b(hipStream_t s, const BatchDev& bd, int max_nkt);
 int launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_nsup, int mixed, int cbw);
 void launch_gather_cvec_b(hipStream_t s, const BatchDev& bd, int max_rows);
