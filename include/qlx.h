@@ -86,6 +86,16 @@ typedef struct {
     int ray_precision;      /* QLX_RAY_* */
     int refine_iters;       /* MIXED: max FP64 iterative-refinement steps (<= 0: default 6) */
     double refine_tol;      /* MIXED: relative residual target (<= 0: default 1e-10) */
+    /* Regularization-parameter optimisation (optimize_regparam on; optimize_regularization_parameter,
+     * pixelgrid.cpp:15865-15979): per item, Brent's method over log10(lambda) in
+     * [regopt_minlog, regopt_maxlog] minimising chi^2 over the active pixels + lambda Es - N ln lambda
+     * - ln det R + ln det F (chisq_regparam_dense, 16526-16699; brents_min_method, 16849-16932), then
+     * the evaluation at the optimum.  FP64 precision only.  Defaults as the reference's
+     * (lens.cpp:423-426): bounds -3 and 5 when regopt_maxlog <= regopt_minlog, tolerance 0.01 when
+     * regopt_tol <= 0, at most 20 iterations when regopt_max_iter <= 0. */
+    int optimize_regparam;
+    double regopt_minlog, regopt_maxlog, regopt_tol;
+    int regopt_max_iter;
 } qlx_settings;
 
 /* Per-evaluation result (the scalars QLens fills, qlens.h:750-761, 891). */
@@ -103,6 +113,9 @@ typedef struct {
     int n_src_active;   /* source_npixels */
     int status;         /* qlx_status of this item */
     int refine_steps;
+    double regparam;    /* lambda of the evaluation: the settings' value, or the optimised one */
+    int regopt_evals;   /* objective evaluations of the regparam optimisation (0 without it) */
+    int pad_;
 } qlx_result;
 
 /* ---- context ---------------------------------------------------------------------------- */
@@ -167,6 +180,31 @@ int qlx_get_rays(void *ctx, int item, double *beta, size_t cap);
 /* The PSF matrix in use (x-major [pnx*pny], as passed to qlx_set_psf or built by
  * qlx_set_psf_gaussian); psf may be NULL to query the size only. */
 int qlx_get_psf(void *ctx, int *pnx, int *pny, double *psf, size_t cap);
+
+/* ---- fit model and likelihood (sampler interface) --------------------------------------------
+ * The reference's fit parameters (vary flags on lens parameters with penalty limits, transforms and
+ * priors: ParamSettings, params.h:722-893) and QLens::fitmodel_loglike_extended_source
+ * (lens.cpp:10657-10748), which every sampler and optimiser calls through LogLikePtr
+ * (mcmchdr.cpp:22).  Here one call takes a whole population of sampler points. */
+enum { QLX_TRANSFORM_NONE = 0, QLX_TRANSFORM_LOG10 = 1, QLX_TRANSFORM_LINEAR = 2 };
+enum { QLX_PRIOR_UNIFORM = 0, QLX_PRIOR_LOG = 1, QLX_PRIOR_GAUSS = 2 };
+typedef struct {
+    int lens, param;        /* lens index in the base model, parameter index in its p[] */
+    double lo, hi;          /* penalty limits on the physical value (lo >= hi: none) */
+    int transform;          /* physical = x, 10^x, or (x - tb) / ta (inverse_transform_parameters) */
+    double ta, tb;
+    int include_jacobian;   /* LOG10: add -ln(physical) (add_jacobian_terms_to_loglike) */
+    int prior;              /* on the sampler coordinate x (add_prior_terms_to_loglike):
+                               UNIFORM nothing; LOG ln(norm) + ln x; GAUSS ln(norm) + ((x - mu)/sig)^2/2 */
+    double prior_mu, prior_sig, prior_norm;
+} qlx_fit_param;
+/* Set the model the sampler varies: the base lens model and nparam varied parameters. */
+int qlx_set_fit_model(void *ctx, const qlx_lens_model *base, int nparam, const qlx_fit_param *params);
+/* -ln L for nbatch sampler points x[b*nparam + k]: 1e30 when a physical value lies outside its
+ * penalty limits or the item fails (the reference's sentinels), else (-2 ln E)/2 plus the prior and
+ * Jacobian terms.  The points inside their limits are evaluated in one qlx_eval; out (may be NULL)
+ * receives each item's result (a point outside its limits: status QLX_ERR_ARG, minus2lnE 2e30). */
+int qlx_loglike(void *ctx, int nbatch, const double *x, double *loglike, qlx_result *out);
 
 /* ---- ray shooting and data (building blocks, host buffers) --------------------------------
  * beta = theta - alpha at n points (QLens::find_sourcept, lenscalc.cpp:318-330). */

@@ -153,6 +153,7 @@ static int mode_invert(QLens& lens, const string& out, double sigma, int nrep)
 	fprintf(f, "src_nx %d\nsrc_ny %d\n", snx, sny);
 	fprintf(f, "src_xmin %.17g\nsrc_xmax %.17g\nsrc_ymin %.17g\nsrc_ymax %.17g\n",
 		sg->srcgrid_xmin, sg->srcgrid_xmax, sg->srcgrid_ymin, sg->srcgrid_ymax);
+	fprintf(f, "regparam %.17g\n", sg->regparam);
 	fprintf(f, "psf_nx %d\npsf_ny %d\n", lens.psf_npixels_x, lens.psf_npixels_y);
 	fprintf(f, "median_seconds %.9g\n", median);
 	fprintf(f, "min_seconds %.9g\nmax_seconds %.9g\n", sorted.front(), sorted.back());

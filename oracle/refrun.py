@@ -65,6 +65,7 @@ def write_inputs(case, data, workdir, lenses=None, psf_mode="input"):
         "auto_srcgrid off",
         "auto_src_npixels off",
         "regparam %s" % _fmt(case["regparam"]),
+        "optimize_regparam %s" % ("on" if case.get("optimize_regparam") else "off"),
         "bg_pixel_noise %s" % _fmt(case["sigma"]),
     ]
     if psf_mode == "gaussian":

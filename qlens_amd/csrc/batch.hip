@@ -492,6 +492,18 @@ __global__ void __launch_bounds__(256) k_csc_order_b(BatchDev bd)
     const int* act = bd.act_pix + item * bd.s_pix;
     for (int c = blockIdx.x * 4 + w; c < M.n_src; c += gridDim.x * 4) {
         const int lo = off[c], n = off[c + 1] - lo;
+        if (n <= 64) {  // one entry per lane: ranks by broadcasting each key in turn
+            const unsigned k = lane < n ? kk[lo + lane] : 0xffffffffu;
+            int rank = 0;
+            for (int b = 0; b < n; b++) rank += ((unsigned)__builtin_amdgcn_readlane((int)k, b) < k) ? 1 : 0;
+            if (lane < n) {
+                const int p = act[perm[k]];
+                const unsigned i = (unsigned)(p / bd.ny), j = (unsigned)(p - (p / bd.ny) * bd.ny);
+                bd.cpix[eb + lo + rank] = (i << 16) | j;
+                bd.cval[eb + lo + rank] = vv[lo + lane];
+            }
+            continue;
+        }
         for (int a = lane; a < n; a += 64) {
             const unsigned k = kk[lo + a];
             int rank = 0;
@@ -761,36 +773,66 @@ __global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int psf_bytes)
         while (bits) {
             const int cb = wd * 32 + __builtin_ctz(bits);
             bits &= bits - 1;
-            for (int ii = w; ii < CB; ii += 4) {
-                const int c = (cb << cbs) + ii;
-                T acc = (T)0;
-                if (c < M.n_src) {
-                    const int e1 = cptr[c + 1];
-                    for (int e0 = cptr[c]; e0 < e1; e0 += 64) {
-                        const int e = e0 + lane;
-                        unsigned pk = 0;
-                        T v = (T)0;
-                        bool rel = false;
-                        if (e < e1) {
-                            pk = cpix[e];
-                            rel = (unsigned)((int)(pk >> 16) - i_lo) <= i_span && (unsigned)((int)(pk & 0xffffu) - j_lo) <= j_span;
-                            if (rel) v = (T)cval[e];
-                        }
-                        unsigned long long bal = __ballot(rel);
-                        while (bal) {
-                            const int b = __builtin_ctzll(bal);
-                            bal &= bal - 1;
-                            const unsigned pkb = (unsigned)__builtin_amdgcn_readlane((int)pk, b);
-                            const T vb = readlane_t<T>(v, b);
-                            const int tp = kr2 - (int)(pkb >> 16), tq = lr2 - (int)(pkb & 0xffffu);
-                            if (p >= 0 && (unsigned)tp < (unsigned)pnx && (unsigned)tq < (unsigned)pny) acc += psf_l[tp * pny + tq] * vb;
-                        }
+            // this wave's columns of the block are c_l = (cb << cbs) + w + 4 l, l < CB / 4: lane l
+            // loads their entry bounds once; column l + 1's first 64 entries (pixel, value) are
+            // loaded while column l is worked, longer columns load the rest in place
+            const int ncol = CB / 4;
+            const int cl = (cb << cbs) + w + 4 * lane;
+            int lb0 = 0, lb1 = 0;
+            if (lane < ncol && cl < M.n_src) {
+                lb0 = cptr[cl];
+                lb1 = cptr[cl + 1];
+            }
+            int ce0 = __builtin_amdgcn_readlane(lb0, 0), ce1 = __builtin_amdgcn_readlane(lb1, 0);
+            unsigned pk = 0;
+            double vd = 0.0;
+            if (ce0 + lane < ce1) {
+                pk = cpix[ce0 + lane];
+                vd = cval[ce0 + lane];
+            }
+            for (int l = 0; l < ncol; l++) {
+                const int ii = w + 4 * l;
+                const int e0c = ce0, e1c = ce1;
+                unsigned pkc = pk;
+                double vc = vd;
+                pk = 0;
+                vd = 0.0;
+                if (l + 1 < ncol) {
+                    ce0 = __builtin_amdgcn_readlane(lb0, l + 1);
+                    ce1 = __builtin_amdgcn_readlane(lb1, l + 1);
+                    if (ce0 + lane < ce1) {
+                        pk = cpix[ce0 + lane];
+                        vd = cval[ce0 + lane];
                     }
                 }
+                T acc = (T)0;
+                for (int e0 = e0c; e0 < e1c; e0 += 64) {
+                    if (e0 != e0c) {
+                        pkc = 0;
+                        vc = 0.0;
+                        if (e0 + lane < e1c) {
+                            pkc = cpix[e0 + lane];
+                            vc = cval[e0 + lane];
+                        }
+                    }
+                    const bool rel = e0 + lane < e1c && (unsigned)((int)(pkc >> 16) - i_lo) <= i_span &&
+                                     (unsigned)((int)(pkc & 0xffffu) - j_lo) <= j_span;
+                    const T v = (T)vc;
+                    unsigned long long bal = __ballot(rel);
+                    while (bal) {
+                        const int b = __builtin_ctzll(bal);
+                        bal &= bal - 1;
+                        const unsigned pkb = (unsigned)__builtin_amdgcn_readlane((int)pkc, b);
+                        const T vb = readlane_t<T>(v, b);
+                        const int tp = kr2 - (int)(pkb >> 16), tq = lr2 - (int)(pkb & 0xffffu);
+                        if (p >= 0 && (unsigned)tp < (unsigned)pnx && (unsigned)tq < (unsigned)pny) acc += psf_l[tp * pny + tq] * vb;
+                    }
+                }
+                const int c_ = (cb << cbs) + ii;
                 if constexpr (F64) {
-                    ((double*)tile)[lane * TS + ii] = c == M.n_src ? dv : (double)acc;
+                    ((double*)tile)[lane * TS + ii] = c_ == M.n_src ? dv : (double)acc;
                 } else {
-                    const double x = c == M.n_src ? dv : (double)acc * sc;
+                    const double x = c_ == M.n_src ? dv : (double)acc * sc;
                     ((uint16_t*)tile)[((lane >> 2) * CB + ii) * 4 + (lane & 3)] = f32_to_f16_rne((float)x);
                 }
             }
@@ -812,10 +854,10 @@ int launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_n
 {
     if (max_rows <= 0 || max_nsup <= 0) return 0;
     const int max_nkt = (max_rows + TS - 1) / TS;
-    const size_t ntap = (size_t)bd.pnx * bd.pny;
     const size_t esz = mixed ? 4 : 8;
-    const size_t psf_bytes = (ntap * esz + 15) / 16 * 16;
+    const size_t ntap = (size_t)bd.pnx * bd.pny;
     const size_t tile_bytes = (size_t)TS * cbw * (mixed ? 2 : 8);
+    const size_t psf_bytes = (ntap * esz + 15) / 16 * 16;
     if (psf_bytes > 64 * 1024) return -1;  // PSF beyond the LDS budget
     const size_t lds = psf_bytes + tile_bytes;
     if (mixed) {
@@ -977,6 +1019,8 @@ __device__ __forceinline__ f16x8_t lds_k8(const uint16_t* arr, int g, int i)
     return __builtin_bit_cast(f16x8_t, make_uint4(a.x, a.y, b.x, b.y));
 }
 
+__device__ __forceinline__ double item_lambda(const BatchDev& bd, int item) { return bd.lam ? bd.lam[item] : bd.lambda; }
+
 // R entry (r, c), c <= r, of the banded curvature tiles (reference order, |r - c| within the band)
 __device__ __forceinline__ double* rtile(const BatchDev& bd, const ItemMeta& M, int r, int c)
 {
@@ -998,7 +1042,7 @@ __device__ __forceinline__ void syrk_f16_store(const BatchDev& bd, const ItemMet
     const int wr = w >> 1, wc = w & 1;
     const float unscale = (float)(1.0 / (bd.s16 * bd.s16));
     const int ld = M.ldN, n = M.n_src;
-    const double lam = bd.lambda;
+    const double lam = item_lambda(bd, item);
     const int* pis = bd.pisrc + item * bd.s_cell;
     double* fdiag = bd.vec + M.vec_off + 3 * (size_t)ld;
     float* C = bd.Ff32 + M.f_off + (size_t)I * SCB * ld + (size_t)J * SCB;
@@ -1189,8 +1233,9 @@ __global__ void __launch_bounds__(256) k_prep_f_b(BatchDev bd)
     int I, J;
     tri_decode(blockIdx.x, I, J);
     double* F = bd.F + M.f_off;
+    const double* F0 = bd.Fraw ? bd.Fraw + M.f_off : F;  // out of place when F as formed is kept
     const int* pis = bd.pisrc + item * bd.s_cell;
-    const double lam = bd.lambda;
+    const double lam = item_lambda(bd, item);
     // 16 elements per thread, unrolled so that their gathers (pisrc, R tile, F) are in flight together
 #pragma unroll 8
     for (int e = threadIdx.x; e < TS * TS; e += 256) {
@@ -1203,9 +1248,11 @@ __global__ void __launch_bounds__(256) k_prep_f_b(BatchDev bd)
                 const int a = pis[r], b = pis[c];
                 const int hi = a > b ? a : b, lo = a > b ? b : a;
                 const double rv = ((hi >> 6) - (lo >> 6) <= M.bwb) ? *rtile(bd, M, hi, lo) : 0.0;
-                F[t] = F[t] + lam * rv;
+                F[t] = F0[t] + lam * rv;
             } else if (r > n) {
                 F[t] = (r == c) ? 1.0 : 0.0;
+            } else {
+                F[t] = F0[t];  // the data row
             }
         }
     }
@@ -2582,6 +2629,7 @@ __global__ void __launch_bounds__(1024) k_evidence_b(BatchDev bd)
 {
     __shared__ double sh[1024];
     __shared__ double sh2[1024];
+    __shared__ double sh3[1024];
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     if (!M.valid) return;
@@ -2590,13 +2638,14 @@ __global__ void __launch_bounds__(1024) k_evidence_b(BatchDev bd)
     const double* model = bd.model + (size_t)item * 2 * bd.s_pix;
     const int* sidx = bd.sidx + item * bd.s_cell;
     const double* s = bd.vec + M.vec_off + 6 * (size_t)M.ldN;  // reference order
-    double chi = 0.0;
+    double chi = 0.0, chia = 0.0;  // all mask pixels; active pixels only (chisq_regparam_dense's Ed)
     for (int p = threadIdx.x; p < npix; p += 1024) {
         if (bd.mask && !bd.mask[p]) continue;
         const double d = bd.data[p];
         if (pm[p]) {
             const double r = model[p] - d;
             chi += (r * r) * bd.covinv[p];
+            chia += (r * r) * bd.covinv[p];
         } else {
             chi += (d * d) * bd.covinv[p];
         }
@@ -2619,11 +2668,13 @@ __global__ void __launch_bounds__(1024) k_evidence_b(BatchDev bd)
     }
     sh[threadIdx.x] = chi;
     sh2[threadIdx.x] = es;
+    sh3[threadIdx.x] = chia;
     __syncthreads();
     for (int st = 512; st > 0; st >>= 1) {
         if ((int)threadIdx.x < st) {
             sh[threadIdx.x] += sh[threadIdx.x + st];
             sh2[threadIdx.x] += sh2[threadIdx.x + st];
+            sh3[threadIdx.x] += sh3[threadIdx.x + st];
         }
         __syncthreads();
     }
@@ -2631,9 +2682,10 @@ __global__ void __launch_bounds__(1024) k_evidence_b(BatchDev bd)
         double* o = bd.out + QLX_OUT_STRIDE * item;
         const double chisq = sh[0], Es = sh2[0];
         const double logdetF = o[2], logdetR = o[3];
+        const double lam = item_lambda(bd, item);
         double loglike = 0.0, regterms = 0.0;
-        if (bd.lambda != 0) {
-            regterms = bd.lambda * Es - M.n_src * log(bd.lambda) - logdetR;
+        if (lam != 0) {
+            regterms = lam * Es - M.n_src * log(lam) - logdetR;
             loglike += regterms;
         }
         loglike += logdetF;
@@ -2643,6 +2695,7 @@ __global__ void __launch_bounds__(1024) k_evidence_b(BatchDev bd)
         o[4] = Es;
         o[5] = regterms;
         o[7] = (double)bd.status[4 * item + 2];
+        o[8] = (sh3[0] + regterms) + logdetF;  // chisq_regparam_dense: Ed + loglike_reg, then + logdet
     }
 }
 

@@ -95,6 +95,7 @@ struct Ctx {
     DevBuf key, bcnt, boff, bfill, perm;
     // pools
     DevBuf lb, F, Ff, R, Rf, cvec, vec, nz, bbase, winv, model, out, status;  // Rf: R's copy for its factorization
+    DevBuf Fraw, d_lam;  // regparam optimisation: F as formed and the per-item lambda
     // solutions of the last call, kept on the device for qlx_get_source: per caller item a block
     // at kept[item].off of [cells doubles: source amplitudes in the reference order]
     // [s_pix doubles: model image] [cells ints: reference active index of each cell]
@@ -105,6 +106,10 @@ struct Ctx {
     };
     std::vector<Kept> kept;
     size_t keep_npix = 0;
+    // fit model of qlx_loglike: base lens model and the varied parameters
+    qlx_lens_model fit_base{};
+    std::vector<qlx_fit_param> fit_params;
+    bool fit_set = false;
     std::vector<int> last_wave;  // caller indices of the last wave's items, in wave order (their rays
                                  // stay in `beta` until the next evaluation)
     std::vector<unsigned> nzh;  // host copy of the support masks (work accounting)
@@ -316,6 +321,104 @@ size_t dense_bytes_bound(size_t ncell, size_t snx)
 struct WaveItem {
     int idx;       // index in the caller's arrays
     int snx, sny;
+};
+
+// Brent's method over log10(lambda) as the reference runs it (brents_min_method,
+// pixelgrid.cpp:16849-16932), restated as a resumable state machine so that every item of a wave
+// advances by one objective evaluation per batched factorization: point() is where the objective
+// is wanted next, feed() takes its value.  The arithmetic and the order of the tests follow the
+// reference line by line (the returned point is x, the best point so far).
+struct RegoptBrent {
+    static constexpr double CGOLD = 0.3819660, ZEPS = 1.0e-10, ROOTPREC = 1.0e-8;
+    double ax = 0, bx = 0, tol = 0.01;
+    int max_iter = 20;
+    double a = 0, b = 0, xstep = 0.0, fu = 0, fwprev = 0, fw = 0, fx = 0, u = 0, wprev = 0, w = 0, x = 0, e = 0.0;
+    int iter = 0, evals = 0, state = 0;  // 0: f(x) wanted, 1: f(u) wanted, 2: converged or out of iterations
+    void start(double ax_, double bx_, double tol_, int max_iter_)
+    {
+        ax = ax_;
+        bx = bx_;
+        tol = tol_;
+        max_iter = max_iter_;
+        a = ax;
+        b = bx;
+        x = w = wprev = bx - CGOLD * (bx - ax);
+        xstep = 0.0;
+        e = 0.0;
+        iter = evals = state = 0;
+    }
+    double point() const { return state == 1 ? u : x; }
+    void feed(double f)
+    {
+        evals++;
+        if (state == 0) {
+            fw = fwprev = fx = f;
+            advance();
+            return;
+        }
+        fu = f;
+        if (fu <= fx) {
+            if (u >= x) a = x;
+            else b = x;
+            wprev = w;
+            w = x;
+            x = u;
+            fwprev = fw;
+            fw = fx;
+            fx = fu;
+        } else {
+            if (u < x) a = u;
+            else b = u;
+            if (fu <= fw or w == x) {
+                wprev = w;
+                fwprev = fw;
+                w = u;
+                fw = fu;
+            } else if ((fu <= fwprev) or (wprev == x) or (wprev == w)) {
+                wprev = u;
+                fwprev = fu;
+            }
+        }
+        iter++;
+        advance();
+    }
+    // the top of the reference's loop: stop, or choose the next point u
+    void advance()
+    {
+        if (iter >= max_iter) {
+            state = 2;
+            return;
+        }
+        double tol1, tol2, p, q, r, etemp;
+        const double xmid = 0.5 * (a + b);
+        tol2 = 2.0 * ((tol1 = tol * fabs(x)) + ZEPS);
+        if (fabs(x - xmid) <= (tol2 - 0.5 * (b - a))) {
+            state = 2;
+            return;
+        }
+        if ((w != wprev) and (fabs(e) > tol1)) {
+            r = (x - w) * (fx - fwprev);
+            q = (x - wprev) * (fx - fw);
+            p = (x - wprev) * q - (x - w) * r;
+            q = 2.0 * (q - r);
+            if (q > 0.0) p = -p;
+            q = fabs(q);
+            etemp = e;
+            e = xstep;
+            if ((fabs(p) >= fabs(0.5 * q * etemp)) or (p <= q * (a - x)) or (p >= q * (b - x))) {
+                xstep = CGOLD * (e = (x >= xmid ? a - x : b - x));
+            } else {
+                xstep = p / q;
+                u = x + xstep;
+                if ((u - a < tol2) or (b - u < tol2)) xstep = ((xmid - x) >= 0 ? (tol1 >= 0 ? tol1 : -tol1) : (tol1 >= 0 ? -tol1 : tol1));
+            }
+        } else {
+            xstep = CGOLD * (e = (x >= xmid ? a - x : b - x));
+        }
+        if (fabs(xstep) >= ROOTPREC) u = x + xstep;
+        else u = x + (xstep >= 0 ? ROOTPREC : (-ROOTPREC));
+        state = 1;
+    }
 };
 
 // run_wave's answer when the wave's exact dense and Lb-pool bytes exceed the budget: the caller
@@ -530,8 +633,9 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         timers_collect(c);
         return QLX_OK;
     }
+    const bool opt = S.optimize_regparam != 0 && !mixed;
     const size_t fixed_bytes = (size_t)B * stage1_bytes(s_pix, s_cell, nsub);
-    const size_t dense_bytes = f_tot * 8 + r_tot * 16 + cv_tot * 8 + vec_tot * 8 + nz_tot * 4 + bb_tot * 4 + w_tot * 8;
+    const size_t dense_bytes = f_tot * (opt ? 16 : 8) + r_tot * 16 + cv_tot * 8 + vec_tot * 8 + nz_tot * 4 + bb_tot * 4 + w_tot * 8;
     if (B > 1 && fixed_bytes + dense_bytes > budget) return WAVE_TOO_BIG;
     ok = c->R.ensure(r_tot * 8) && c->Rf.ensure(r_tot * 8) && c->cvec.ensure(cv_tot * 8) && c->vec.ensure(vec_tot * 8) && c->nz.ensure(nz_tot * 4) &&
          c->bbase.ensure(bb_tot * 4) && c->winv.ensure(w_tot * 8) && c->F.ensure(f_tot * (mixed ? 4 : 8)) &&
@@ -617,6 +721,25 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     // its small launches slow that latency-bound kernel by more than they save.)  R's many small
     // launches are enqueued after the SYRK's: the host's time spent enqueueing them then overlaps
     // the SYRK on the GPU instead of leaving the main stream idle.
+    // regparam optimisation: per-item lambda on the device, F as formed kept beside the factor
+    std::vector<RegoptBrent> brent(opt ? B : 0);
+    std::vector<double> lamh(B, S.regparam);
+    if (opt) {
+        double lo = S.regopt_minlog, hi = S.regopt_maxlog;
+        if (!(hi > lo)) {  // lens.cpp:424-425
+            lo = -3;
+            hi = 5;
+        }
+        const double tol = S.regopt_tol > 0 ? S.regopt_tol : 0.01;
+        const int maxit = S.regopt_max_iter > 0 ? S.regopt_max_iter : 20;
+        for (int b = 0; b < B; b++) {
+            brent[b].start(lo, hi, tol, maxit);
+            lamh[b] = pow(10, brent[b].point());  // chisq_regparam_dense: regparam = 10^logreg
+        }
+        if (!c->Fraw.ensure(f_tot * 8) || !c->d_lam.ensure((size_t)B * 8)) return fail(c, QLX_ERR_NOMEM, "regparam buffers");
+        HIPCHK(c, hipMemcpyAsync(c->d_lam.p, lamh.data(), (size_t)B * 8, hipMemcpyHostToDevice, st));
+        bd.lam = c->d_lam.as<double>();
+    }
     launch_curvature_b(st, bd, max_cells, max_rtiles);
     HIPCHK(c, hipEventRecord(c->ev_fork, st));
     if (!mixed) launch_gather_cvec_b(st, bd, max_rows);
@@ -624,6 +747,10 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     if (mixed) launch_syrk_f16_b(st, bd, max_ncb);
     else launch_syrk_f64_b(st, bd, max_ncb);
     timer_end(c, "syrk");
+    if (opt) {
+        HIPCHK(c, hipMemcpyAsync(c->Fraw.p, c->F.p, f_tot * 8, hipMemcpyDeviceToDevice, st));
+        bd.Fraw = c->Fraw.as<double>();
+    }
     launch_prep_fr_b(st, bd, max_ld, mixed ? 1 : 0);
     HIPCHK(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
     HIPCHK(c, hipMemcpyAsync(c->Rf.p, c->R.p, r_tot * 8, hipMemcpyDeviceToDevice, c->stream2));
@@ -646,6 +773,34 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     }
     timer_end(c, "solve");
     launch_evidence_b(st, bd, max_img, max_src);
+    if (opt) {
+        // one objective evaluation per item per pass; items that converged are evaluated at their
+        // optimum again, and a last pass evaluates every item there (the state the reference leaves:
+        // regparam = 10^x, the source and ln det F of that evaluation)
+        std::vector<double> ob((size_t)B * QLX_OUT_STRIDE);
+        bool final_pass = false;
+        for (;;) {
+            HIPCHK(c, hipMemcpyAsync(ob.data(), bd.out, ob.size() * 8, hipMemcpyDeviceToHost, st));
+            HIPCHK(c, hipStreamSynchronize(st));
+            if (final_pass) break;
+            bool all_done = true;
+            for (int b = 0; b < B; b++) {
+                if (!meta[b].valid) continue;
+                if (brent[b].state != 2) brent[b].feed(ob[(size_t)b * QLX_OUT_STRIDE + 8]);
+                all_done = all_done && brent[b].state == 2;
+            }
+            for (int b = 0; b < B; b++) lamh[b] = meta[b].valid ? pow(10, brent[b].point()) : S.regparam;
+            final_pass = all_done;
+            HIPCHK(c, hipMemcpyAsync(c->d_lam.p, lamh.data(), (size_t)B * 8, hipMemcpyHostToDevice, st));
+            launch_prep_fr_b(st, bd, max_ld, 0);
+            timer_begin(c, "cholesky");
+            cholesky_batched(st, c->d_mats.as<MatDesc>(), 0, nF, max_ld / 64, 1, 0);
+            timer_end(c, "cholesky");
+            launch_logdets_b(st, bd, 0);
+            launch_trisolve_b(st, bd, max_ld, 0);
+            launch_evidence_b(st, bd, max_img, max_src);
+        }
+    }
     timer_end(c, "total");
     launch_keep_b(st, bd, c->keep.p, max_cells);
 
@@ -668,6 +823,8 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         r->Es = ob[4];
         r->refine_resid = ob[6];
         r->refine_steps = (int)ob[7];
+        r->regparam = lamh[b];
+        r->regopt_evals = opt ? brent[b].evals : 0;
         r->status = QLX_OK;
         if (stat[4 * b + 3] == 2) r->status = QLX_ERR_REFINE;  // value kept, accurate to refine_resid
         if (stat[4 * b] || stat[4 * b + 1] || !std::isfinite(r->minus2lnE)) {
@@ -952,6 +1109,8 @@ int qlx_set_settings(void* ctx, const qlx_settings* s)
     if (s->precision != QLX_PREC_FP64 && s->precision != QLX_PREC_MIXED) return fail(c, QLX_ERR_ARG, "unknown precision mode");
     if (s->ray_precision != QLX_RAY_FP64 && s->ray_precision != QLX_RAY_FP32) return fail(c, QLX_ERR_ARG, "unknown ray precision");
     if (s->integral_tol <= 0) return fail(c, QLX_ERR_ARG, "integral tolerance must be > 0");
+    if (s->optimize_regparam && s->precision != QLX_PREC_FP64) return fail(c, QLX_ERR_ARG, "regparam optimisation runs in FP64 precision only");
+    if (s->optimize_regparam && s->regopt_max_iter > 1000) return fail(c, QLX_ERR_RANGE, "at most 1000 regparam iterations");
     c->set = *s;
     c->settings_set = true;
     return QLX_OK;
@@ -1038,6 +1197,112 @@ int qlx_get_psf(void* ctx, int* pnx, int* pny, double* psf, size_t cap)
         memcpy(psf, c->h_psf.data(), c->h_psf.size() * sizeof(double));
     }
     return QLX_OK;
+}
+
+int qlx_set_fit_model(void* ctx, const qlx_lens_model* base, int nparam, const qlx_fit_param* params)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return QLX_ERR_ARG;
+    if (!base || nparam < 0 || (nparam > 0 && !params)) return fail(c, QLX_ERR_ARG, "qlx_set_fit_model: bad arguments");
+    int rc = check_model(c, base, false);
+    if (rc) return rc;
+    for (int k = 0; k < nparam; k++) {
+        const qlx_fit_param& f = params[k];
+        if (f.lens < 0 || f.lens >= base->nlens || f.param < 0 || f.param >= 6) return fail(c, QLX_ERR_ARG, "fit parameter outside the lens model");
+        if (f.transform < QLX_TRANSFORM_NONE || f.transform > QLX_TRANSFORM_LINEAR || (f.transform == QLX_TRANSFORM_LINEAR && f.ta == 0))
+            return fail(c, QLX_ERR_ARG, "unknown or singular parameter transform");
+        if (f.prior < QLX_PRIOR_UNIFORM || f.prior > QLX_PRIOR_GAUSS || (f.prior == QLX_PRIOR_GAUSS && !(f.prior_sig > 0)) ||
+            (f.prior != QLX_PRIOR_UNIFORM && !(f.prior_norm > 0)))
+            return fail(c, QLX_ERR_ARG, "unknown prior, or a Gaussian prior without a positive width and norm");
+    }
+    c->fit_base = *base;
+    c->fit_params.assign(params, params + nparam);
+    c->fit_set = true;
+    return QLX_OK;
+}
+
+int qlx_loglike(void* ctx, int nbatch, const double* x, double* loglike, qlx_result* out)
+{
+    // QLens::fitmodel_loglike_extended_source (lens.cpp:10657-10748) over a batch of sampler points
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return QLX_ERR_ARG;
+    if (!c->fit_set) return fail(c, QLX_ERR_STATE, "qlx_loglike: no fit model set");
+    const int np = (int)c->fit_params.size();
+    if (nbatch < 0 || (nbatch > 0 && (!x || !loglike))) return fail(c, QLX_ERR_ARG, "qlx_loglike: bad arguments");
+    if (!c->image_set || !c->settings_set || !c->psf_set) return fail(c, QLX_ERR_STATE, "image, PSF and settings must be set first");
+    hipSetDevice(c->device);
+    std::vector<qlx_lens_model> models;
+    std::vector<int> idx, srcn;
+    std::vector<double> phys((size_t)nbatch * np);
+    std::vector<qlx_result> res(nbatch);
+    for (int b = 0; b < nbatch; b++) {
+        const double* xb = x + (size_t)b * np;
+        double* tb = phys.data() + (size_t)b * np;
+        bool inside = true;
+        for (int k = 0; k < np; k++) {  // inverse_transform_parameters (params.h:722-743)
+            const qlx_fit_param& f = c->fit_params[k];
+            if (f.transform == QLX_TRANSFORM_NONE) tb[k] = xb[k];
+            else if (f.transform == QLX_TRANSFORM_LOG10) tb[k] = pow(10.0, xb[k]);
+            else tb[k] = (xb[k] - f.tb) / f.ta;
+            if (f.lo < f.hi && ((tb[k] < f.lo) or (tb[k] > f.hi))) inside = false;
+        }
+        memset(&res[b], 0, sizeof(res[b]));
+        if (!inside) {  // lens.cpp:10673-10677
+            loglike[b] = 1e30;
+            res[b].minus2lnE = 2e30;
+            res[b].status = QLX_ERR_ARG;
+            continue;
+        }
+        qlx_lens_model m = c->fit_base;  // update_model
+        for (int k = 0; k < np; k++) m.lens[c->fit_params[k].lens].p[c->fit_params[k].param] = tb[k];
+        models.push_back(m);
+        idx.push_back(b);
+        srcn.push_back(c->set.src_nx);
+        srcn.push_back(c->set.src_ny);
+    }
+    std::vector<qlx_result> ev(models.size());
+    if (!models.empty()) {
+        const int rc = eval_items(c, (int)models.size(), models.data(), srcn.data(), ev.data());
+        if (rc < 0) return rc;
+        // the kept solutions and rays are indexed by the caller's point: remap them
+        std::vector<Ctx::Kept> kept(nbatch, Ctx::Kept{0, 0, 0, 0, 0});
+        for (size_t k = 0; k < idx.size(); k++) kept[idx[k]] = c->kept[k];
+        c->kept.swap(kept);
+        for (int& w : c->last_wave) w = idx[w];
+    } else {
+        c->kept.assign(nbatch, Ctx::Kept{0, 0, 0, 0, 0});
+        c->last_wave.clear();
+    }
+    int any_bad = 0;
+    for (size_t k = 0; k < idx.size(); k++) {
+        const int b = idx[k];
+        res[b] = ev[k];
+        if (ev[k].status != QLX_OK && ev[k].status != QLX_ERR_REFINE) {
+            loglike[b] = 1e30;  // a failed inversion: the sentinel, as update_model's huge penalty
+            any_bad = 1;
+            continue;
+        }
+        double L = 0.0;
+        L += ev[k].minus2lnE / 2;  // lens.cpp:10719
+        const double* xb = x + (size_t)b * np;
+        const double* tb = phys.data() + (size_t)b * np;
+        double dl = 0.0;  // add_prior_terms_to_loglike on the sampler coordinates (params.h:810-850)
+        for (int q = 0; q < np; q++) {
+            const qlx_fit_param& f = c->fit_params[q];
+            if (f.prior == QLX_PRIOR_UNIFORM) continue;
+            dl += log(f.prior_norm);
+            if (f.prior == QLX_PRIOR_LOG) dl += log(xb[q]);
+            else dl += ((xb[q] - f.prior_mu) / f.prior_sig) * ((xb[q] - f.prior_mu) / f.prior_sig) / 2.0;
+        }
+        L += dl;
+        for (int q = 0; q < np; q++)  // add_jacobian_terms_to_loglike on the physical values (params.h:884-893)
+            if (c->fit_params[q].include_jacobian && c->fit_params[q].transform == QLX_TRANSFORM_LOG10) L -= log(tb[q]);
+        loglike[b] = L;
+    }
+    if (out) memcpy(out, res.data(), sizeof(qlx_result) * nbatch);
+    for (int b = 0; b < nbatch; b++)
+        if (res[b].status != QLX_OK) any_bad = 1;
+    return any_bad ? QLX_ITEMS_FAILED : QLX_OK;
 }
 
 int qlx_raytrace(void* ctx, const qlx_lens_model* model, int n, const double* xy, double* beta)

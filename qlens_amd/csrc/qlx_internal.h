@@ -23,7 +23,8 @@ namespace qlx {
 #define QLX_SG_SCX 1026
 #define QLX_SG_SCY 1538
 // per-item output slots
-#define QLX_OUT_STRIDE 8          // -2lnE, chisq, logdetF, logdetR, Es, regterms, refine_resid, refine_steps
+#define QLX_OUT_STRIDE 10         // -2lnE, chisq, logdetF, logdetR, Es, regterms, refine_resid, refine_steps,
+                                  // regparam objective (chi^2 over the active pixels + regterms + ln det F), spare
 #define QLX_VEC_SLOTS 7           // per item, ldN doubles each: s (F order), diagF, diagR, Fdiag, resid,
                                   // partial sums, s (reference order)
 // internal source order: active cells ranked in 8 (x) by 16 (y) cell tiles, tiles y-major
@@ -81,6 +82,8 @@ struct BatchDev {
     const double* psf;
     int pnx, pny;
     double tol, lambda;
+    const double* lam;          // per-item lambda (regparam optimisation), or null: lambda for every item
+    const double* Fraw;         // FP64 F as the SYRK formed it (regparam optimisation), or null
     // per-item inputs
     const ModelPrep* models;
     const double* sgrid;        // QLX_SG_STRIDE doubles per item

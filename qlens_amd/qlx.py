@@ -43,14 +43,16 @@ class QlxSettings(C.Structure):
     _fields_ = [("nsplit", C.c_int), ("src_xmin", C.c_double), ("src_xmax", C.c_double),
                 ("src_ymin", C.c_double), ("src_ymax", C.c_double), ("src_nx", C.c_int), ("src_ny", C.c_int),
                 ("regparam", C.c_double), ("integral_tol", C.c_double), ("precision", C.c_int),
-                ("ray_precision", C.c_int), ("refine_iters", C.c_int), ("refine_tol", C.c_double)]
+                ("ray_precision", C.c_int), ("refine_iters", C.c_int), ("refine_tol", C.c_double),
+                ("optimize_regparam", C.c_int), ("regopt_minlog", C.c_double), ("regopt_maxlog", C.c_double),
+                ("regopt_tol", C.c_double), ("regopt_max_iter", C.c_int)]
 
 
 class QlxResult(C.Structure):
     _fields_ = [("minus2lnE", C.c_double), ("chisq0", C.c_double), ("logdetF", C.c_double),
                 ("logdetR", C.c_double), ("Es", C.c_double), ("refine_resid", C.c_double),
                 ("n_img_active", C.c_int), ("n_src_active", C.c_int), ("status", C.c_int),
-                ("refine_steps", C.c_int)]
+                ("refine_steps", C.c_int), ("regparam", C.c_double), ("regopt_evals", C.c_int), ("pad_", C.c_int)]
 
 
 class QlxSurrogateCfg(C.Structure):
@@ -59,6 +61,15 @@ class QlxSurrogateCfg(C.Structure):
                 ("rs_min", C.c_double), ("rs_max", C.c_double), ("seed", C.c_ulonglong), ("tol", C.c_double),
                 ("graph", C.c_int)]
 
+
+class QlxFitParam(C.Structure):
+    _fields_ = [("lens", C.c_int), ("param", C.c_int), ("lo", C.c_double), ("hi", C.c_double),
+                ("transform", C.c_int), ("ta", C.c_double), ("tb", C.c_double), ("include_jacobian", C.c_int),
+                ("prior", C.c_int), ("prior_mu", C.c_double), ("prior_sig", C.c_double), ("prior_norm", C.c_double)]
+
+
+QLX_TRANSFORM_NONE, QLX_TRANSFORM_LOG10, QLX_TRANSFORM_LINEAR = 0, 1, 2
+QLX_PRIOR_UNIFORM, QLX_PRIOR_LOG, QLX_PRIOR_GAUSS = 0, 1, 2
 
 _LIB = None
 
@@ -79,7 +90,7 @@ def load_library(path=None):
                  "qlx_nfw_def_hess", "qlx_mock_image", "qlx_kernel_time", "qlx_selftest_mfma",
                  "qlx_surrogate_train", "qlx_surrogate_eval", "qlx_surrogate_grad", "qlx_surrogate_param_count",
                  "qlx_surrogate_get_params", "qlx_surrogate_set_params", "qlx_surrogate_save", "qlx_surrogate_load",
-                 "qlx_get_stats"]:
+                 "qlx_get_stats", "qlx_set_fit_model", "qlx_loglike"]:
         getattr(lib, name).restype = C.c_int
     lib.qlx_get_rays.restype = C.c_int
     lib.qlx_get_rays.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_size_t]
@@ -208,18 +219,23 @@ class PixelInversion:
                                                   C.c_double(threshold)), "qlx_set_psf_gaussian")
 
     def set_settings(self, nsplit, srcgrid, src_n, regparam, integral_tol=1e-3, precision=QLX_PREC_FP64,
-                     ray_precision=QLX_RAY_FP64, refine_iters=0, refine_tol=0.0):
+                     ray_precision=QLX_RAY_FP64, refine_iters=0, refine_tol=0.0, optimize_regparam=False,
+                     regopt_bounds=(-3.0, 5.0), regopt_tol=0.01, regopt_max_iter=20):
+        """QLens settings of the inversion; optimize_regparam runs Brent over log10(lambda) in
+        regopt_bounds per item (the reference's 'optimize_regparam on', FP64 only)."""
         s = QlxSettings(nsplit, srcgrid[0], srcgrid[1], srcgrid[2], srcgrid[3], src_n[0], src_n[1], regparam,
-                        integral_tol, precision, ray_precision, refine_iters, refine_tol)
+                        integral_tol, precision, ray_precision, refine_iters, refine_tol,
+                        1 if optimize_regparam else 0, regopt_bounds[0], regopt_bounds[1], regopt_tol,
+                        int(regopt_max_iter))
         self._check(self.lib.qlx_set_settings(self.ctx, C.byref(s)), "qlx_set_settings")
         self.settings = dict(nsplit=nsplit, srcgrid=srcgrid, src_n=src_n, regparam=regparam,
                              integral_tol=integral_tol, precision=precision)
 
-    def load_workload(self, case, data, precision=QLX_PREC_FP64, ray_precision=QLX_RAY_FP64):
+    def load_workload(self, case, data, precision=QLX_PREC_FP64, ray_precision=QLX_RAY_FP64, optimize_regparam=False):
         self.load_image(case["xvals"], case["yvals"], data, sigma=case["sigma"], mask=case.get("mask"))
         self.set_psf(case["psf"])
         self.set_settings(case["nsplit"], case["srcgrid"], case["src_n"], case["regparam"], case["integral_tol"],
-                          precision=precision, ray_precision=ray_precision)
+                          precision=precision, ray_precision=ray_precision, optimize_regparam=optimize_regparam)
 
     # -- evaluation --------------------------------------------------------------------------
     def invert_image_surface_brightness_map(self, lens_models, def_mode=QLX_DEF_QUADRATURE, src_n=None):
@@ -256,6 +272,44 @@ class PixelInversion:
                                             act.ctypes.data_as(C.POINTER(C.c_int)), src.size, _dp(model), model.size),
                     "qlx_get_source")
         return {"src": src, "active_index": act, "model": model}
+
+    # -- fit model and likelihood (QLens::fitmodel_loglike_extended_source) -----------------------
+    def set_fit_model(self, base_lenses, params):
+        """base_lenses: [(type, params)]; params: one dict per varied parameter with keys lens, param
+        and optionally lo, hi (penalty limits on the physical value), transform ('none', 'log10',
+        ('linear', a, b)), jacobian (bool), prior ('uniform', ('log', norm), ('gauss', mu, sig, norm))."""
+        arr = (QlxFitParam * max(1, len(params)))()
+        for k, d in enumerate(params):
+            f = arr[k]
+            f.lens, f.param = int(d["lens"]), int(d["param"])
+            f.lo, f.hi = float(d.get("lo", 0.0)), float(d.get("hi", 0.0))
+            t = d.get("transform", "none")
+            if t == "log10":
+                f.transform = QLX_TRANSFORM_LOG10
+            elif isinstance(t, tuple) and t[0] == "linear":
+                f.transform, f.ta, f.tb = QLX_TRANSFORM_LINEAR, float(t[1]), float(t[2])
+            f.include_jacobian = 1 if d.get("jacobian") else 0
+            pr = d.get("prior", "uniform")
+            if isinstance(pr, tuple) and pr[0] == "log":
+                f.prior, f.prior_norm = QLX_PRIOR_LOG, float(pr[1])
+            elif isinstance(pr, tuple) and pr[0] == "gauss":
+                f.prior, f.prior_mu, f.prior_sig, f.prior_norm = QLX_PRIOR_GAUSS, float(pr[1]), float(pr[2]), float(pr[3])
+        m = make_model(base_lenses)
+        self._check(self.lib.qlx_set_fit_model(self.ctx, C.byref(m), len(params), arr), "qlx_set_fit_model")
+        self._nfit = len(params)
+
+    def loglike(self, x):
+        """-ln L for sampler points x[n, nparam] in one batched call; returns (loglike[n], results)."""
+        x = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1, self._nfit))
+        n = len(x)
+        ll = np.zeros(n)
+        res = (QlxResult * max(1, n))()
+        rc = self.lib.qlx_loglike(self.ctx, n, _dp(x), _dp(ll), res)
+        if rc < 0:
+            self._check(rc, "qlx_loglike")
+        self._last_n = n
+        names = _RESULT_DT.names
+        return ll, [dict(zip(names, row)) for row in np.frombuffer(res, dtype=_RESULT_DT)[:n].tolist()]
 
     def get_rays(self, item=-1):
         """Subpixel source-plane points of an item of the last evaluation: [nx, ny, nsplit^2, 2]."""

@@ -163,6 +163,23 @@ def cns_case():
     return case
 
 
+def regopt_fixture(work):
+    """'optimize_regparam on' (Brent over log10 lambda in [-3, 5], tol 0.01, at most 20 steps) on
+    the C1 and CNS workloads, each on its own reference mock."""
+    out = {}
+    for name, case in (("C1", W.c1()), ("CNS", cns_case())):
+        data = refrun.run_mock(case, _sub(work, "regopt_mock_" + name))
+        case = dict(case, optimize_regparam=True)
+        sc, arr = refrun.run_invert(case, data, _sub(work, "regopt_" + name))
+        for k in ("minus2lnE", "chisq0", "logdetF", "logdetR", "regparam"):
+            out["%s_%s" % (k, name)] = sc[k]
+        out["n_src_%s" % name] = int(sc["n_src_active"])
+        out["src_%s" % name] = arr["src"]
+        out["data_%s" % name] = data
+        print("regopt", name, "regparam", repr(sc["regparam"]), "-2lnE", repr(sc["minus2lnE"]), flush=True)
+    np.savez_compressed(os.path.join(HERE, "regopt.npz"), **out)
+
+
 FIXTURES = {
     "C1": lambda w: inversion_fixture(w, "C1", W.c1()),
     "C2s": lambda w: inversion_fixture(w, "C2s", W.c2_small()),
@@ -173,6 +190,7 @@ FIXTURES = {
     "patterson": patterson_fixture,
     "rng": rng_fixture,
     "psf": psf_fixture,
+    "regopt": regopt_fixture,
 }
 
 if __name__ == "__main__":

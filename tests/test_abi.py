@@ -33,7 +33,7 @@ def test_structs_match_header_layout():
     from qlens_amd import qlx
     assert ctypes.sizeof(qlx.QlxLens) == 8 + 6 * 8
     assert ctypes.sizeof(qlx.QlxLensModel) == 8 + qlx.QLX_MAX_LENSES * ctypes.sizeof(qlx.QlxLens)
-    assert ctypes.sizeof(qlx.QlxResult) == 6 * 8 + 4 * 4
+    assert ctypes.sizeof(qlx.QlxResult) == 7 * 8 + 6 * 4
 
 
 def test_no_cpu_fallback_in_product():
