@@ -209,6 +209,12 @@ int qlx_loglike(void *ctx, int nbatch, const double *x, double *loglike, qlx_res
 /* ---- ray shooting and data (building blocks, host buffers) --------------------------------
  * beta = theta - alpha at n points (QLens::find_sourcept, lenscalc.cpp:318-330). */
 int qlx_raytrace(void *ctx, const qlx_lens_model *model, int n, const double *xy, double *beta);
+/* qlx_raytrace without any device allocation: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid it).  The points and results are staged in one file-scope device array per GPU (8 MiB,
+ * held until the process ends), so n is at most 262144 (QLX_ERR_RANGE beyond).  Calls on one GPU
+ * share that array: calls that overlap in time on the same GPU (two contexts or threads) race. */
+int qlx_raytrace_noalloc(void *ctx, const qlx_lens_model *model, int n, const double *xy, double *beta);
 /* Lens-frame NFW deflection + Hessian (alpha_x, alpha_y, psi_xx, psi_xy, psi_yy) by quadrature
  * (LensProfile::deflection_and_hessian_numerical, profile.cpp:2532-2560): surrogate targets. */
 int qlx_nfw_def_hess(void *ctx, const double *nfw_params, double tol, int n, const double *xy,

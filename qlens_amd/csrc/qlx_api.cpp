@@ -106,6 +106,7 @@ struct Ctx {
     };
     std::vector<Kept> kept;
     size_t keep_npix = 0;
+    double* rt_scratch = nullptr;  // this device's staging array of qlx_raytrace_noalloc
     // fit model of qlx_loglike: base lens model and the varied parameters
     qlx_lens_model fit_base{};
     std::vector<qlx_fit_param> fit_params;
@@ -976,6 +977,10 @@ int qlx_ctx_create(const qlx_opts* opts, void** out)
         return fail(nullptr, QLX_ERR_HIP, "stream creation failed");
     }
     upload_patterson(c->device);
+    if (raytrace_scratch(c->device, &c->rt_scratch) != 0) {
+        qlx_ctx_destroy(c);
+        return fail(nullptr, QLX_ERR_HIP, "hipGetSymbolAddress of the ray staging array failed");
+    }
     *out = c;
     return QLX_OK;
 }
@@ -1322,6 +1327,29 @@ int qlx_raytrace(void* ctx, const qlx_lens_model* model, int n, const double* xy
     HIPCHK(c, hipMemcpyAsync(dxy.p, xy, (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
     launch_raytrace_points(c->stream, mp, tol, n, dxy.as<double>(), db.as<double>());
     HIPCHK(c, hipMemcpyAsync(beta, db.p, (size_t)n * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return QLX_OK;
+}
+
+int qlx_raytrace_noalloc(void* ctx, const qlx_lens_model* model, int n, const double* xy, double* beta)
+{
+    // qlx_raytrace with the device staging in the file-scope array of raytrace.hip: no allocation
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return QLX_ERR_ARG;
+    if (n < 0 || (n > 0 && (!xy || !beta))) return fail(c, QLX_ERR_ARG, "bad arguments");
+    if (n > QLX_RT_SCRATCH_POINTS) return fail(c, QLX_ERR_RANGE, "qlx_raytrace_noalloc: more than 262144 points");
+    int rc = check_model(c, model);
+    if (rc) return rc;
+    if (n == 0) return QLX_OK;
+    hipSetDevice(c->device);
+    double tol = c->settings_set ? c->set.integral_tol : 1e-3;
+    ModelPrep mp;
+    prep_model(*model, mp);
+    double* in = c->rt_scratch;
+    double* res = c->rt_scratch + 2 * (size_t)QLX_RT_SCRATCH_POINTS;
+    HIPCHK(c, hipMemcpyAsync(in, xy, (size_t)n * 16, hipMemcpyHostToDevice, c->stream));
+    launch_raytrace_points(c->stream, mp, tol, n, in, res);
+    HIPCHK(c, hipMemcpyAsync(beta, res, (size_t)n * 16, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return QLX_OK;
 }

@@ -158,6 +158,10 @@ void upload_patterson(int device);
 // ---- raytrace.hip ----
 void launch_raytrace_map_b(hipStream_t s, const BatchDev& bd, int ray_fp32, int kappa_table);
 void launch_raytrace_points(hipStream_t stream, const ModelPrep& m, double tol, int n, const double* xy, double* beta);
+// the file-scope staging array of the allocation-free ray entry point (raytrace.hip): its address on
+// `device` (which must be current), taken once per device; 0 on success
+#define QLX_RT_SCRATCH_POINTS (1 << 18)
+int raytrace_scratch(int device, double** out);
 void launch_nfw_def_hess(hipStream_t stream, const LensPrep& L, double tol, int n, const double* xy, double* out5);
 void launch_mock_sb(hipStream_t stream, int nx, int ny, int nsplit, const double* xvals, const double* yvals, const ModelPrep& m,
                     double tol, int nsrc, const double* sersic, double* sb);

@@ -16,6 +16,9 @@
 //   quadrature              LensIntegral::PattersonIntegrate       profile.cpp:3830-3881
 //   mapping                 assign_source_mapping_flags_interpolate pixelgrid.cpp:2291-2432
 //   L weights               calculate_Lmatrix_interpolate          pixelgrid.cpp:2501-2527
+#include <map>
+#include <mutex>
+
 #include "qlx_internal.h"
 #include "patterson_table.h"
 #include "nfw_gtab.h"
@@ -485,6 +488,30 @@ __global__ void __launch_bounds__(256) k_raytrace_points(ModelPrep m, double tol
 void launch_raytrace_points(hipStream_t stream, const ModelPrep& m, double tol, int n, const double* xy, double* beta)
 {
     hipLaunchKernelGGL(k_raytrace_points, dim3((n + 255) / 256), dim3(256), 0, stream, m, tol, n, xy, beta);
+}
+
+// Staging for the allocation-free ray entry point (qlx_raytrace_noalloc): the points in
+// [0, 2 * QLX_RT_SCRATCH_POINTS) and the results in [2 * QLX_RT_SCRATCH_POINTS, 4 * ...) doubles.
+// Each GPU holds its own copy for the life of the process; host code must use the address
+// hipGetSymbolAddress returns for the current device, never the array's name.
+static __device__ __align__(256) double scratch[4 * QLX_RT_SCRATCH_POINTS];
+
+// the device address of `scratch` on `device` (taken once per device, with that device current)
+int raytrace_scratch(int device, double** out)
+{
+    static std::mutex mu;
+    static std::map<int, double*> ptrs;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = ptrs.find(device);
+    if (it == ptrs.end()) {
+        int cur = -1;
+        if (hipGetDevice(&cur) != hipSuccess || cur != device) return -1;
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -1;
+        it = ptrs.emplace(device, (double*)p).first;
+    }
+    *out = it->second;
+    return 0;
 }
 
 // Lens-frame NFW deflection and Hessian (profile.cpp:2532-2560): (ax, ay, psi_xx, psi_xy, psi_yy)
