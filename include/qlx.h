@@ -73,9 +73,9 @@ typedef struct {
 } qlx_opts;
 
 /* Inversion settings (the QLens fields the block reads, qlens.h:750-893):
- * nsplit = default_imgpixel_nsplit; source grid = srcgrid_x/ymin/max and srcgrid_npixels_x/y
- * (auto_srcgrid and auto_src_npixels off); regparam = the pixsrc regparam; integral_tol =
- * integral_tolerance; curvature regularization, 3-point interpolation, dense inversion. */
+ * nsplit = default_imgpixel_nsplit; source grid = srcgrid_x/ymin/max and srcgrid_npixels_x/y;
+ * regparam = the pixsrc regparam; integral_tol = integral_tolerance; curvature regularization,
+ * 3-point interpolation, dense inversion. */
 typedef struct {
     int nsplit;
     double src_xmin, src_xmax, src_ymin, src_ymax;
@@ -96,6 +96,20 @@ typedef struct {
     int optimize_regparam;
     double regopt_minlog, regopt_maxlog, regopt_tol;
     int regopt_max_iter;
+    /* Automatic source grid, per item from its own rays (lens.cpp:12442-12458):
+     * auto_srcgrid (auto_srcgrid on; ImagePixelGrid::find_optimal_sourcegrid,
+     *   pixelgrid.cpp:10998-11080): the bounding box of the mask pixels' mean subpixel source
+     *   points, widened by 10 % per axis, replaces src_x/ymin/max.  With noise_threshold > 0,
+     *   pixels whose data is at most noise_threshold x their noise (1/sqrt(covinv)) are left out.
+     * auto_src_npixels (auto_src_npixels on; find_optimal_sourcegrid_npixels, 11469-11486):
+     *   with count = mask pixels whose centre ray lies strictly inside the box,
+     *   nx = 2 (int)sqrt(pixel_fraction count dx/dy), ny = 2 (int)(nx/2 dy/dx), replacing
+     *   src_nx/src_ny (and the per-item grids of qlx_eval_batch_grids).  A grid below 2 cells per
+     *   axis fails its item with QLX_ERR_NOSOURCE (lens.cpp:12453-12458); above 512 per axis, or
+     *   beyond QLX_MAX_LD active cells, with QLX_ERR_RANGE.
+     * pixel_fraction <= 0 means the reference's default 0.3 (pixelgrid.cpp:559). */
+    int auto_srcgrid, auto_src_npixels;
+    double pixel_fraction, noise_threshold;
 } qlx_settings;
 
 /* Per-evaluation result (the scalars QLens fills, qlens.h:750-761, 891). */
@@ -115,8 +129,17 @@ typedef struct {
     int refine_steps;
     double regparam;    /* lambda of the evaluation: the settings' value, or the optimised one */
     int regopt_evals;   /* objective evaluations of the regparam optimisation (0 without it) */
+    int src_nx, src_ny; /* the item's source grid: cells per axis ... */
     int pad_;
+    double src_xmin, src_xmax, src_ymin, src_ymax; /* ... and its bounds (the settings' or the
+                           automatic ones; the reference's sourcegrid_x/ymin/max after the call) */
 } qlx_result;
+
+/* Sizes of the structs above as this library was compiled, in the order qlx_lens, qlx_lens_model,
+ * qlx_opts, qlx_settings, qlx_result, qlx_fit_param, qlx_surrogate_cfg, qlx_stats, for bindings to
+ * check their mirrors against (no counterpart in the reference, whose callers link C++).  Writes
+ * min(n, 8) sizes and returns 8. */
+int qlx_struct_sizes(size_t *sizes, int n);
 
 /* ---- context ---------------------------------------------------------------------------- */
 int qlx_ctx_create(const qlx_opts *opts, void **ctx);
@@ -149,7 +172,8 @@ int qlx_set_settings(void *ctx, const qlx_settings *s);
 int qlx_eval(void *ctx, int nbatch, const qlx_lens_model *models, qlx_result *out);
 
 /* Batched evaluation with per-item source grids (uneven source sizes in one submission).
- * src_n[2*b], src_n[2*b+1] = source cells of item b (bounds from the settings). */
+ * src_n[2*b], src_n[2*b+1] = source cells of item b (bounds from the settings; ignored, and src_n
+ * may be NULL, when the settings ask for auto_src_npixels). */
 int qlx_eval_batch_grids(void *ctx, int nbatch, const qlx_lens_model *models, const int *src_n,
                          qlx_result *out);
 

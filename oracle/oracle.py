@@ -4,6 +4,7 @@ Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
 the checker.  The product path (qlens_amd) never imports this module.
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -172,3 +173,58 @@ def invert(case, data, lenses=None, want_arrays=True):
         lib().orc_invert(C.byref(P), C.byref(R), None, None, None, None, None)
     res = {f: getattr(R, f) for f, _ in OrcResult._fields_}
     return res, arrays
+
+
+def auto_source_grid(case, rays, data, auto_box=True, auto_npixels=True, pixel_fraction=0.3, noise_threshold=0.0):
+    """Automatic source grid of one evaluation, from its subpixel rays (nx, ny, nsub, 2).
+
+    Box (auto_box): ImagePixelGrid::find_optimal_sourcegrid (pixelgrid.cpp:10998-11080): over mask
+    pixels whose data exceeds noise_threshold x noise (all of them when noise_threshold <= 0;
+    window_size_for_sbavg is 0), the min / max of the mean subpixel source point (summed in
+    subpixel order, divided by nsub), the limits at their defaults of +-1e30 (lens.cpp:381-384);
+    then each side moved out by 5 % of the width.  Otherwise the case's srcgrid.
+    Size (auto_npixels): find_optimal_sourcegrid_npixels (11469-11486): count = mask pixels whose
+    centre ray (the pixel corner plus half the pixel size, 9659-9666, traced at the centre) lies
+    strictly inside the box; nx0 = (int)sqrt(f count dx/dy), ny0 = (int)(nx0 dy/dx), both doubled
+    (lens.cpp:12449-12450).  Otherwise the case's src_n.
+    Returns (box [xmin, xmax, ymin, ymax], (snx, sny), count)."""
+    xv = np.asarray(case["xvals"], dtype=np.float64)
+    yv = np.asarray(case["yvals"], dtype=np.float64)
+    nx, ny = len(xv) - 1, len(yv) - 1
+    mask = case.get("mask")
+    inmask = np.ones((nx, ny), dtype=bool) if mask is None else np.asarray(mask).astype(bool)
+    if auto_box:
+        use = inmask.copy()
+        if noise_threshold > 0:
+            sig = case["sigma"]
+            use &= ~(np.asarray(data) <= noise_threshold * sig)
+        nsub = rays.shape[2]
+        xs = rays[:, :, 0, 0].copy()
+        ys = rays[:, :, 0, 1].copy()
+        for k in range(1, nsub):
+            xs = xs + rays[:, :, k, 0]
+            ys = ys + rays[:, :, k, 1]
+        xs, ys = xs / nsub, ys / nsub
+        xmin, xmax = float(xs[use].min()), float(xs[use].max())
+        ymin, ymax = float(ys[use].min()), float(ys[use].max())
+        xw, yw = 0.1 * (xmax - xmin), 0.1 * (ymax - ymin)
+        box = [xmin - xw / 2, xmax + xw / 2, ymin - yw / 2, ymax + yw / 2]
+    else:
+        box = [float(v) for v in case["srcgrid"]]
+    count = None
+    if auto_npixels:
+        pxl, pyl = (xv[-1] - xv[0]) / nx, (yv[-1] - yv[0]) / ny
+        cx = xv[:-1] + 0.5 * pxl
+        cy = yv[:-1] + 0.5 * pyl
+        pts = np.stack(np.meshgrid(cx, cy, indexing="ij"), -1).reshape(-1, 2)
+        alpha, _ = deflection(case["lenses"], pts, tol=case["integral_tol"])
+        beta = (pts - alpha).reshape(nx, ny, 2)
+        inside = (beta[..., 0] > box[0]) & (beta[..., 0] < box[1]) & (beta[..., 1] > box[2]) & (beta[..., 1] < box[3])
+        count = int((inside & inmask).sum())
+        dx, dy = box[1] - box[0], box[3] - box[2]
+        nx0 = int(math.sqrt(pixel_fraction * count * dx / dy))
+        ny0 = int(nx0 * dy / dx)
+        size = (2 * nx0, 2 * ny0)
+    else:
+        size = tuple(case["src_n"])
+    return box, size, count

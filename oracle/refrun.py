@@ -62,8 +62,8 @@ def write_inputs(case, data, workdir, lenses=None, psf_mode="input"):
         "integral_tolerance %s" % _fmt(case["integral_tol"]),
         "srcgrid %s %s %s %s" % tuple(_fmt(v) for v in case["srcgrid"]),
         "src_npixels %d %d" % tuple(case["src_n"]),
-        "auto_srcgrid off",
-        "auto_src_npixels off",
+        "auto_srcgrid %s" % ("on" if case.get("auto_srcgrid") else "off"),
+        "auto_src_npixels %s" % ("on" if case.get("auto_src_npixels") else "off"),
         "regparam %s" % _fmt(case["regparam"]),
         "optimize_regparam %s" % ("on" if case.get("optimize_regparam") else "off"),
         "bg_pixel_noise %s" % _fmt(case["sigma"]),
@@ -107,7 +107,7 @@ def run_invert(case, data, workdir, lenses=None, nrep=1, psf_mode="input", threa
     sc = read_scalars(workdir)
     nx, ny = len(case["xvals"]) - 1, len(case["yvals"]) - 1
     ns = case["nsplit"]
-    snx, sny = case["src_n"]
+    snx, sny = int(sc["src_nx"]), int(sc["src_ny"])  # the automatic size, when asked for
     arr = dict(
         rays=np.fromfile(os.path.join(workdir, "rays.bin")).reshape(nx, ny, ns * ns, 2),
         mapped=np.fromfile(os.path.join(workdir, "mapped.bin"), dtype=np.uint8).reshape(nx, ny),

@@ -96,6 +96,7 @@ struct Ctx {
     // pools
     DevBuf lb, F, Ff, R, Rf, cvec, vec, nz, bbase, winv, model, out, status;  // Rf: R's copy for its factorization
     DevBuf Fraw, d_lam;  // regparam optimisation: F as formed and the per-item lambda
+    DevBuf agrid;        // automatic source grids: per item box and centre-ray count
     // solutions of the last call, kept on the device for qlx_get_source: per caller item a block
     // at kept[item].off of [cells doubles: source amplitudes in the reference order]
     // [s_pix doubles: model image] [cells ints: reference active index of each cell]
@@ -288,14 +289,14 @@ int check_model(Ctx* c, const qlx_lens_model* m, bool values = true)
 
 // Source grid corners accumulated x += xstep and 4-corner-mean centres
 // (SourcePixelGrid::create_pixel_grid pixelgrid.cpp:441-455, SourcePixel ctor 761-783).
-void fill_source_grid(const qlx_settings& s, int snx, int sny, double* g)
+void fill_source_grid(const double* box, int snx, int sny, double* g)
 {
     double* sx = g;
     double* sy = g + QLX_SG_SYV;
     double* cx = g + QLX_SG_SCX;
     double* cy = g + QLX_SG_SCY;
-    const double xstep = (s.src_xmax - s.src_xmin) / snx, ystep = (s.src_ymax - s.src_ymin) / sny;
-    double x = s.src_xmin, y = s.src_ymin;
+    const double xstep = (box[1] - box[0]) / snx, ystep = (box[3] - box[2]) / sny;
+    double x = box[0], y = box[2];
     for (int i = 0; i <= snx; i++, x += xstep) sx[i] = x;
     for (int j = 0; j <= sny; j++, y += ystep) sy[j] = y;
     for (int i = 0; i < snx; i++) cx[i] = (sx[i] + sx[i] + sx[i + 1] + sx[i + 1]) / 4.0;
@@ -321,8 +322,29 @@ size_t dense_bytes_bound(size_t ncell, size_t snx)
 
 struct WaveItem {
     int idx;       // index in the caller's arrays
-    int snx, sny;
+    int snx, sny;  // source cells per axis (0 until an automatic size is found)
+    size_t cells;  // upper bound of snx * sny (the work arrays are sized to it)
 };
+
+// The automatic cells per axis of find_optimal_sourcegrid_npixels (pixelgrid.cpp:11481-11484,
+// doubled as lens.cpp:12449-12450 does): nx0 = (int)sqrt(f count dx/dy), ny0 = (int)(nx0 dy/dx).
+// Returns false when the box or the count gives no finite size.
+bool auto_npixels(double f, double count, const double* box, int& snx, int& sny)
+{
+    const double dx = box[1] - box[0], dy = box[3] - box[2];
+    if (!(dx > 0) || !(dy > 0) || !std::isfinite(dx) || !std::isfinite(dy)) return false;
+    const double v = sqrt(f * count * dx / dy);
+    if (!(v < 1e6)) return false;
+    const int nx0 = (int)v;
+    const double w = nx0 * dy / dx;
+    if (!(w < 1e6)) return false;
+    const int ny0 = (int)w;
+    snx = 2 * nx0;
+    sny = 2 * ny0;
+    return true;
+}
+
+double pixel_fraction(const qlx_settings& S) { return S.pixel_fraction > 0 ? S.pixel_fraction : 0.3; }
 
 // Brent's method over log10(lambda) as the reference runs it (brents_min_method,
 // pixelgrid.cpp:16849-16932), restated as a resumable state machine so that every item of a wave
@@ -429,8 +451,9 @@ const int WAVE_TOO_BIG = 100;
 // One wave of items through the whole pipeline.  Results go to out[item.idx]; each valid item's
 // solution is copied to its block of the keep buffer.  With more than one item, a wave whose
 // exact dense + Lb bytes exceed `budget` returns WAVE_TOO_BIG before allocating them.
-int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& items, qlx_result* out, size_t budget)
+int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& items_in, qlx_result* out, size_t budget)
 {
+    std::vector<WaveItem> items(items_in);  // automatic grids fill in their sizes
     const int B = (int)items.size();
     if (B == 0) return QLX_OK;
     const qlx_settings& S = c->set;
@@ -438,7 +461,8 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     const int nsub = S.nsplit * S.nsplit;
     const size_t s_pix = (size_t)c->nx * c->ny;
     size_t s_cell = 4;
-    for (auto& it : items) s_cell = std::max(s_cell, (size_t)it.snx * it.sny);
+    for (auto& it : items) s_cell = std::max(s_cell, it.cells);
+    const bool auto_box = S.auto_srcgrid != 0, auto_n = S.auto_src_npixels != 0;
     const size_t U = 3 * (size_t)nsub;
     hipStream_t st = c->stream;
 
@@ -446,9 +470,16 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     std::vector<ModelPrep> mp(B);
     std::vector<double> sg((size_t)B * QLX_SG_STRIDE, 0.0);
     std::vector<int> srcn(2 * B);
+    std::vector<double> boxes((size_t)4 * B);
+    std::vector<int> grid_status(B, QLX_OK);  // an automatic grid that failed its item
     for (int b = 0; b < B; b++) {
         prep_model(models[items[b].idx], mp[b]);
-        fill_source_grid(S, items[b].snx, items[b].sny, sg.data() + (size_t)b * QLX_SG_STRIDE);
+        double* bx = boxes.data() + 4 * b;
+        bx[0] = S.src_xmin;
+        bx[1] = S.src_xmax;
+        bx[2] = S.src_ymin;
+        bx[3] = S.src_ymax;
+        if (!auto_box && !auto_n) fill_source_grid(bx, items[b].snx, items[b].sny, sg.data() + (size_t)b * QLX_SG_STRIDE);
         srcn[2 * b] = items[b].snx;
         srcn[2 * b + 1] = items[b].sny;
     }
@@ -466,7 +497,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
               c->pmask.ensure((size_t)B * s_pix * QLX_MAX_CBW * 4) && c->hmask.ensure((size_t)B * s_pix * QLX_MAX_CBW * 4) &&
               c->bcnt.ensure(B * (s_cell + 2) * 4) && c->boff.ensure(B * (s_cell + 2) * 4) && c->bfill.ensure(B * (s_cell + 2) * 4) &&
               c->perm.ensure(B * s_pix * 4) && c->model.ensure(B * s_pix * 16) && c->out.ensure((size_t)B * QLX_OUT_STRIDE * 8) &&
-              c->status.ensure((size_t)B * 16);
+              c->status.ensure((size_t)B * 16) && c->agrid.ensure((size_t)B * 64);
     if (!ok) return fail(c, QLX_ERR_NOMEM, "work buffer allocation failed");
     c->last_wave.clear();
     for (int b = 0; b < B; b++) c->last_wave.push_back(items[b].idx);
@@ -562,8 +593,36 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         timer_end(c, "surrogate");
     }
     timer_begin(c, "raytrace");
-    launch_raytrace_map_b(st, bd, S.ray_precision == QLX_RAY_FP32, mixed ? 1 : 0);
+    launch_raytrace_map_b(st, bd, S.ray_precision == QLX_RAY_FP32, mixed ? 1 : 0, (auto_box || auto_n) ? 0 : 1);
     timer_end(c, "raytrace");
+    if (auto_box || auto_n) {
+        // each item's grid from its own rays, then the mapping of the rays onto it
+        launch_srcgrid_auto_b(st, bd, auto_box ? 1 : 0, auto_n ? 1 : 0, S.noise_threshold, boxes.data(), bd.model,
+                              c->agrid.as<double>());
+        std::vector<double> ag((size_t)B * 8);
+        HIPCHK(c, hipMemcpyAsync(ag.data(), c->agrid.p, ag.size() * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        for (int b = 0; b < B; b++) {
+            double* bx = boxes.data() + 4 * b;
+            if (auto_box)
+                for (int k = 0; k < 4; k++) bx[k] = ag[8 * b + k];
+            int snx = items[b].snx, sny = items[b].sny;
+            bool good = std::isfinite(bx[0]) && std::isfinite(bx[1]) && std::isfinite(bx[2]) && std::isfinite(bx[3]) &&
+                        bx[1] > bx[0] && bx[3] > bx[2];
+            if (good && auto_n) good = auto_npixels(pixel_fraction(S), ag[8 * b + 4], bx, snx, sny);
+            if (!good || snx < 2 || sny < 2) grid_status[b] = QLX_ERR_NOSOURCE;  // lens.cpp:12453-12458
+            else if (snx > QLX_MAX_SRC_AXIS || sny > QLX_MAX_SRC_AXIS || (size_t)snx * sny > items[b].cells)
+                grid_status[b] = QLX_ERR_RANGE;
+            items[b].snx = grid_status[b] == QLX_OK ? snx : 0;  // no cells: nothing maps, n_src = 0
+            items[b].sny = grid_status[b] == QLX_OK ? sny : 0;
+            srcn[2 * b] = items[b].snx;
+            srcn[2 * b + 1] = items[b].sny;
+            if (grid_status[b] == QLX_OK) fill_source_grid(bx, snx, sny, sg.data() + (size_t)b * QLX_SG_STRIDE);
+        }
+        HIPCHK(c, hipMemcpyAsync(c->d_sgrid.p, sg.data(), sg.size() * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(c, hipMemcpyAsync(c->d_srcn.p, srcn.data(), srcn.size() * 4, hipMemcpyHostToDevice, st));
+        launch_map_rays_b(st, bd);
+    }
     launch_compact_b(st, bd);
     std::vector<int> counts((size_t)B * 4);
     HIPCHK(c, hipMemcpyAsync(counts.data(), bd.counts, (size_t)B * 16, hipMemcpyDeviceToHost, st));
@@ -585,6 +644,17 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         M.n_img = n_img;
         M.snx = items[b].snx;
         M.sny = items[b].sny;
+        r->src_nx = items[b].snx;
+        r->src_ny = items[b].sny;
+        r->src_xmin = boxes[4 * b];
+        r->src_xmax = boxes[4 * b + 1];
+        r->src_ymin = boxes[4 * b + 2];
+        r->src_ymax = boxes[4 * b + 3];
+        if (grid_status[b] != QLX_OK) {
+            r->minus2lnE = 2e30;
+            r->status = grid_status[b];
+            continue;
+        }
         if (n_src == 0) {  // pixelgrid.cpp:12807 -> lens.cpp:12528-12530
             r->minus2lnE = 2e30;
             r->status = QLX_ERR_NOSOURCE;
@@ -886,14 +956,33 @@ int eval_items(Ctx* c, int nbatch, const qlx_lens_model* models, const int* src_
     const bool mixed = c->set.precision == QLX_PREC_MIXED;
     const int nsub = c->set.nsplit * c->set.nsplit;
     const size_t s_pix = (size_t)c->nx * c->ny;
+    const qlx_settings& S = c->set;
+    const bool auto_n = S.auto_src_npixels != 0;
+    // automatic sizes: 2 nx0 x 2 ny0 cells with nx0 ny0 <= pixel_fraction x (mask pixels), as
+    // nx0^2 <= f count dx/dy and ny0 <= nx0 dy/dx (+4 for rounding)
+    size_t auto_cells = 0;
+    if (auto_n) {
+        size_t nmask = s_pix;
+        if (c->has_mask) {
+            nmask = 0;
+            for (unsigned char m : c->h_mask) nmask += m ? 1 : 0;
+        }
+        auto_cells = std::min<size_t>((size_t)QLX_MAX_SRC_AXIS * QLX_MAX_SRC_AXIS, (size_t)(4.0 * pixel_fraction(S) * nmask) + 4);
+    }
     std::vector<WaveItem> todo;
     for (int b = 0; b < nbatch; b++) {
         int rc = check_model(c, &models[b], false);
         if (rc) return rc;
-        const int snx = src_n[2 * b], sny = src_n[2 * b + 1];
+        const int snx = auto_n ? 0 : src_n[2 * b], sny = auto_n ? 0 : src_n[2 * b + 1];
         if (snx > QLX_MAX_SRC_AXIS || sny > QLX_MAX_SRC_AXIS) return fail(c, QLX_ERR_RANGE, "source grid larger than 512 per axis");
         memset(&out[b], 0, sizeof(out[b]));
-        if (snx < 2 || sny < 2) {  // lens.cpp:12453-12458
+        out[b].src_nx = snx;
+        out[b].src_ny = sny;
+        out[b].src_xmin = S.src_xmin;
+        out[b].src_xmax = S.src_xmax;
+        out[b].src_ymin = S.src_ymin;
+        out[b].src_ymax = S.src_ymax;
+        if (!auto_n && (snx < 2 || sny < 2)) {  // lens.cpp:12453-12458
             out[b].minus2lnE = 2e30;
             out[b].status = QLX_ERR_NOSOURCE;
             continue;
@@ -903,14 +992,13 @@ int eval_items(Ctx* c, int nbatch, const qlx_lens_model* models, const int* src_
             out[b].status = QLX_ERR_ARG;
             continue;
         }
-        todo.push_back({b, snx, sny});
+        todo.push_back({b, snx, sny, auto_n ? auto_cells : (size_t)snx * sny});
     }
     // every item's solution block in the keep buffer (upper bound: n_src <= cells)
     size_t keep_tot = 0;
     for (auto& it : todo) {
-        const size_t cells = (size_t)it.snx * it.sny;
         c->kept[it.idx].off = keep_tot;
-        keep_tot += round_up(cells * 12 + s_pix * 8, 256);
+        keep_tot += round_up(it.cells * 12 + s_pix * 8, 256);
     }
     c->keep_npix = s_pix;
     if (!c->keep.ensure(keep_tot)) return fail(c, QLX_ERR_NOMEM, "solution buffer allocation failed");
@@ -918,8 +1006,8 @@ int eval_items(Ctx* c, int nbatch, const qlx_lens_model* models, const int* src_
     // run_wave checks the exact dense and Lb bytes and the wave is halved when they do not fit
     size_t max_cell = 4, max_snx = 2;
     for (auto& it : todo) {
-        max_cell = std::max(max_cell, (size_t)it.snx * it.sny);
-        max_snx = std::max(max_snx, (size_t)it.snx);
+        max_cell = std::max(max_cell, it.cells);
+        max_snx = std::max(max_snx, auto_n ? std::min<size_t>(QLX_MAX_SRC_AXIS, it.cells) : (size_t)it.snx);
     }
     const size_t ncell_bound = std::min<size_t>(max_cell, QLX_MAX_LD - 1);
     const size_t ldN = round_up(ncell_bound + 1, 64);
@@ -955,6 +1043,14 @@ int eval_items(Ctx* c, int nbatch, const qlx_lens_model* models, const int* src_
 }  // namespace
 
 extern "C" {
+
+int qlx_struct_sizes(size_t* sizes, int n)
+{
+    const size_t s[8] = {sizeof(qlx_lens),     sizeof(qlx_lens_model), sizeof(qlx_opts),          sizeof(qlx_settings),
+                         sizeof(qlx_result),   sizeof(qlx_fit_param),  sizeof(qlx_surrogate_cfg), sizeof(qlx_stats)};
+    for (int k = 0; k < n && k < 8 && sizes; k++) sizes[k] = s[k];
+    return 8;
+}
 
 int qlx_ctx_create(const qlx_opts* opts, void** out)
 {
@@ -1109,8 +1205,11 @@ int qlx_set_settings(void* ctx, const qlx_settings* s)
     c->last_wave.clear();
     if (!s) return fail(c, QLX_ERR_ARG, "null settings");
     if (s->nsplit < 1 || s->nsplit > 4) return fail(c, QLX_ERR_RANGE, "nsplit must be 1..4");
-    if (!(s->src_xmax > s->src_xmin) || !(s->src_ymax > s->src_ymin)) return fail(c, QLX_ERR_ARG, "empty source grid bounds");
-    if (s->src_nx > QLX_MAX_SRC_AXIS || s->src_ny > QLX_MAX_SRC_AXIS) return fail(c, QLX_ERR_RANGE, "source grid larger than 512 per axis");
+    if (!s->auto_srcgrid && (!(s->src_xmax > s->src_xmin) || !(s->src_ymax > s->src_ymin)))
+        return fail(c, QLX_ERR_ARG, "empty source grid bounds");
+    if (!(s->noise_threshold >= 0) || !(s->pixel_fraction < 1e6)) return fail(c, QLX_ERR_ARG, "bad noise threshold or pixel fraction");
+    if (!s->auto_src_npixels && (s->src_nx > QLX_MAX_SRC_AXIS || s->src_ny > QLX_MAX_SRC_AXIS))
+        return fail(c, QLX_ERR_RANGE, "source grid larger than 512 per axis");
     if (s->precision != QLX_PREC_FP64 && s->precision != QLX_PREC_MIXED) return fail(c, QLX_ERR_ARG, "unknown precision mode");
     if (s->ray_precision != QLX_RAY_FP64 && s->ray_precision != QLX_RAY_FP32) return fail(c, QLX_ERR_ARG, "unknown ray precision");
     if (s->integral_tol <= 0) return fail(c, QLX_ERR_ARG, "integral tolerance must be > 0");
@@ -1140,7 +1239,7 @@ int qlx_eval_batch_grids(void* ctx, int nbatch, const qlx_lens_model* models, co
 {
     Ctx* c = (Ctx*)ctx;
     if (!c) return QLX_ERR_ARG;
-    if (nbatch < 0 || (nbatch > 0 && (!models || !out || !src_n))) return fail(c, QLX_ERR_ARG, "bad arguments");
+    if (nbatch < 0 || (nbatch > 0 && (!models || !out || (!src_n && !c->set.auto_src_npixels)))) return fail(c, QLX_ERR_ARG, "bad arguments");
     if (!c->image_set || !c->settings_set || !c->psf_set) return fail(c, QLX_ERR_STATE, "image, PSF and settings must be set first");
     hipSetDevice(c->device);
     return eval_items(c, nbatch, models, src_n, out);

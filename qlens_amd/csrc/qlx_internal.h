@@ -156,7 +156,13 @@ struct MatDesc {
 void upload_patterson(int device);
 
 // ---- raytrace.hip ----
-void launch_raytrace_map_b(hipStream_t s, const BatchDev& bd, int ray_fp32, int kappa_table);
+void launch_raytrace_map_b(hipStream_t s, const BatchDev& bd, int ray_fp32, int kappa_table, int map);
+void launch_map_rays_b(hipStream_t s, const BatchDev& bd);
+// automatic source grid: crays (2 s_pix doubles per item) receive the pixel-centre rays when
+// count_centres; res (8 doubles per item) receives xmin, xmax, ymin, ymax, centre count; box (host
+// memory: xmin, xmax, ymin, ymax) is the given grid box, used when auto_box is 0
+void launch_srcgrid_auto_b(hipStream_t s, const BatchDev& bd, int auto_box, int count_centres, double nt, const double* box,
+                           double* crays, double* res);
 void launch_raytrace_points(hipStream_t stream, const ModelPrep& m, double tol, int n, const double* xy, double* beta);
 // the file-scope staging array of the allocation-free ray entry point (raytrace.hip): its address on
 // `device` (which must be current), taken once per device; 0 on success

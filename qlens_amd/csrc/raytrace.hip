@@ -379,8 +379,50 @@ __device__ __forceinline__ int upper_index(const double* v, int n, double x)
 // from sur_alpha (image-frame FP32 values written by the surrogate kernel for the same rays).
 // Three waves per SIMD: at its natural 195 FP64 registers the kernel ran two and was latency-bound
 // (5.0 ms per 64 evaluations); held to 168 it spills 36 registers and runs in 4.5 ms.
+// The 3-point mapping of ray (item, pixel p, slot ro) with source-plane point (bx, by) onto the
+// item's Cartesian grid (bisection_search_interpolate + assign_source_mapping_flags_interpolate,
+// pixelgrid.cpp:2291-2432; weights calculate_Lmatrix_interpolate 2501-2527).
+__device__ __forceinline__ void map_ray(const BatchDev& bd, int item, size_t ro, long long p, double bx, double by)
+{
+    const double* sg = bd.sgrid + (size_t)item * QLX_SG_STRIDE;
+    const double* sxv = sg;
+    const double* syv = sg + QLX_SG_SYV;
+    const double* scx = sg + QLX_SG_SCX;
+    const double* scy = sg + QLX_SG_SCY;
+    const int snx = bd.src_n[2 * item], sny = bd.src_n[2 * item + 1];
+    const int nsub = bd.nsub;
+    int* mc = bd.map_cell + ro * 3;
+    double* mw = bd.map_w + ro * 3;
+    if (!(bx >= sxv[0] && bx < sxv[snx] && by >= syv[0] && by < syv[sny])) {
+        mc[0] = mc[1] = mc[2] = -1;
+        return;
+    }
+    const int ci = upper_index(sxv, snx, bx);
+    const int cj = upper_index(syv, sny, by);
+    const int xi = (((bx > scx[ci]) && (ci < snx - 1)) || (ci == 0)) ? ci + 1 : ci - 1;
+    const int yj = (((by > scy[cj]) && (cj < sny - 1)) || (cj == 0)) ? cj + 1 : cj - 1;
+    const int c0 = ci * sny + cj, c1 = xi * sny + cj, c2 = ci * sny + yj;
+    mc[0] = c0;
+    mc[1] = c1;
+    mc[2] = c2;
+    const double weight = 1.0 / nsub;
+    const double p0x = scx[ci], p0y = scy[cj];
+    const double p1x = scx[xi], p1y = scy[cj];
+    const double p2x = scx[ci], p2y = scy[yj];
+    const double d = (p0x - p1x) * (p1y - p2y) - (p1x - p2x) * (p0y - p1y);
+    mw[0] = weight * (bx * (p1y - p2y) + by * (p2x - p1x) + p1x * p2y - p1y * p2x) / d;
+    mw[1] = weight * (bx * (p2y - p0y) + by * (p0x - p2x) + p0y * p2x - p0x * p2y) / d;
+    mw[2] = weight * (bx * (p0y - p1y) + by * (p1x - p0x) + p0x * p1y - p0y * p1x) / d;
+    int* cm = bd.cmapped + (size_t)item * bd.s_cell;
+    cm[c0] = 1;
+    cm[c1] = 1;
+    cm[c2] = 1;
+    bd.pmapped[(size_t)item * bd.s_pix + p] = 1;
+}
+
+// map = 0: rays only (the source grid is not known yet; k_map_rays_b maps them afterwards)
 template <typename T, bool KT = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_raytrace_map_b(BatchDev bd, T tol)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_raytrace_map_b(BatchDev bd, T tol, int map)
 {
     const int item = blockIdx.y;
     const int nsub = bd.nsub;
@@ -388,12 +430,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nray) return;
     const ModelPrep& m = bd.models[item];
-    const double* sg = bd.sgrid + (size_t)item * QLX_SG_STRIDE;
-    const double* sxv = sg;
-    const double* syv = sg + QLX_SG_SYV;
-    const double* scx = sg + QLX_SG_SCX;
-    const double* scy = sg + QLX_SG_SCY;
-    const int snx = bd.src_n[2 * item], sny = bd.src_n[2 * item + 1];
     const size_t ro = (size_t)item * bd.s_pix * nsub + t;  // ray slot of this item
     const int k = (int)(t % nsub);
     const long long p = t / nsub;  // pixel id i*ny + j
@@ -430,47 +466,165 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     bd.beta[ro * 2] = bx;
     bd.beta[ro * 2 + 1] = by;
     if (nev) atomicAdd(bd.nevals + item, (unsigned long long)nev);
+    if (map) map_ray(bd, item, ro, p, bx, by);
+}
 
-    if (!(bx >= sxv[0] && bx < sxv[snx] && by >= syv[0] && by < syv[sny])) {
-        mc[0] = mc[1] = mc[2] = -1;
-        return;
+// the mapping of rays traced by a map = 0 pass, once each item's grid is known
+__global__ void __launch_bounds__(256) k_map_rays_b(BatchDev bd)
+{
+    const int item = blockIdx.y;
+    const long long nray = (long long)bd.nx * bd.ny * bd.nsub;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nray) return;
+    const long long p = t / bd.nsub;
+    if (bd.mask && !bd.mask[p]) return;  // set unmapped by the ray pass
+    const size_t ro = (size_t)item * bd.s_pix * bd.nsub + t;
+    map_ray(bd, item, ro, p, bd.beta[ro * 2], bd.beta[ro * 2 + 1]);
+}
+
+// Pixel-centre rays of the mask pixels (find_sourcept of center_pts, which
+// calculate_sourcepts_and_areas(true) traces, pixelgrid.cpp:10323-10330, 10457-10464), for the
+// automatic source-grid size; the centre is the corner plus half the pixel size
+// (pixelgrid.cpp:9659-9666).  Written to out[2 (item s_pix + p)].
+__global__ void __launch_bounds__(256) k_centre_rays_b(BatchDev bd, double* out)
+{
+    const int item = blockIdx.y;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long long)bd.nx * bd.ny) return;
+    if (bd.mask && !bd.mask[p]) return;
+    const ModelPrep& m = bd.models[item];
+    const int i = (int)(p / bd.ny), j = (int)(p % bd.ny);
+    const double pxl = (bd.xvals[bd.nx] - bd.xvals[0]) / bd.nx, pyl = (bd.yvals[bd.ny] - bd.yvals[0]) / bd.ny;
+    const double x = bd.xvals[i] + 0.5 * pxl, y = bd.yvals[j] + 0.5 * pyl;
+    double a0 = 0, a1 = 0;
+    int nev = 0;
+    for (int l = 0; l < m.nlens; l++) {
+        double d0, d1;
+        lens_deflection<double, true, false>(m.lens[l], x, y, bd.tol, d0, d1, nev,
+                                             bd.nfwtab + (size_t)(item * QLX_MAX_LENSES + l) * QLX_PAT_N * 4);
+        a0 += d0;
+        a1 += d1;
     }
-    const int ci = upper_index(sxv, snx, bx);
-    const int cj = upper_index(syv, sny, by);
-    const int xi = (((bx > scx[ci]) && (ci < snx - 1)) || (ci == 0)) ? ci + 1 : ci - 1;
-    const int yj = (((by > scy[cj]) && (cj < sny - 1)) || (cj == 0)) ? cj + 1 : cj - 1;
-    const int c0 = ci * sny + cj, c1 = xi * sny + cj, c2 = ci * sny + yj;
-    mc[0] = c0;
-    mc[1] = c1;
-    mc[2] = c2;
-    const double weight = 1.0 / nsub;
-    const double p0x = scx[ci], p0y = scy[cj];
-    const double p1x = scx[xi], p1y = scy[cj];
-    const double p2x = scx[ci], p2y = scy[yj];
-    const double d = (p0x - p1x) * (p1y - p2y) - (p1x - p2x) * (p0y - p1y);
-    mw[0] = weight * (bx * (p1y - p2y) + by * (p2x - p1x) + p1x * p2y - p1y * p2x) / d;
-    mw[1] = weight * (bx * (p2y - p0y) + by * (p0x - p2x) + p0y * p2x - p0x * p2y) / d;
-    mw[2] = weight * (bx * (p0y - p1y) + by * (p1x - p0x) + p0x * p1y - p0y * p1x) / d;
-    int* cm = bd.cmapped + (size_t)item * bd.s_cell;
-    cm[c0] = 1;
-    cm[c1] = 1;
-    cm[c2] = 1;
-    bd.pmapped[(size_t)item * bd.s_pix + p] = 1;
+    out[2 * ((size_t)item * bd.s_pix + p)] = x - a0;
+    out[2 * ((size_t)item * bd.s_pix + p) + 1] = y - a1;
+}
+
+// Automatic source grid of each item (find_optimal_sourcegrid, pixelgrid.cpp:10998-11080, with
+// noise_threshold nt; find_optimal_sourcegrid_npixels, 11469-11486): the bounding box of the mask
+// pixels' mean subpixel source points (pixels with data <= nt * noise skipped when nt > 0) widened
+// by 10 %, or the given box; then the count of mask pixels whose centre ray lies strictly inside.
+// res[8 item ..]: xmin, xmax, ymin, ymax, count.  One workgroup of 1024 per item.
+__global__ void __launch_bounds__(1024) k_srcgrid_auto_b(BatchDev bd, int auto_box, double nt, double gx0, double gx1, double gy0,
+                                                         double gy1, const double* crays, double* res)
+{
+    __shared__ double sm[4][1024];
+    const int item = blockIdx.y;
+    const int npix = bd.nx * bd.ny, nsub = bd.nsub;
+    double xmin = 1e30, xmax = -1e30, ymin = 1e30, ymax = -1e30;
+    if (auto_box) {
+        for (int p = threadIdx.x; p < npix; p += 1024) {
+            if (bd.mask && !bd.mask[p]) continue;
+            if (nt > 0 && bd.data[p] <= nt * (1.0 / sqrt(bd.covinv[p]))) continue;
+            const double* b = bd.beta + ((size_t)item * bd.s_pix * nsub + (size_t)p * nsub) * 2;
+            double xs = 0, ys = 0;
+            for (int k = 0; k < nsub; k++) {
+                xs += b[2 * k];
+                ys += b[2 * k + 1];
+            }
+            xs /= nsub;
+            ys /= nsub;
+            xmin = fmin(xmin, xs);
+            xmax = fmax(xmax, xs);
+            ymin = fmin(ymin, ys);
+            ymax = fmax(ymax, ys);
+        }
+    }
+    sm[0][threadIdx.x] = xmin;
+    sm[1][threadIdx.x] = xmax;
+    sm[2][threadIdx.x] = ymin;
+    sm[3][threadIdx.x] = ymax;
+    __syncthreads();
+    for (int st = 512; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            sm[0][threadIdx.x] = fmin(sm[0][threadIdx.x], sm[0][threadIdx.x + st]);
+            sm[1][threadIdx.x] = fmax(sm[1][threadIdx.x], sm[1][threadIdx.x + st]);
+            sm[2][threadIdx.x] = fmin(sm[2][threadIdx.x], sm[2][threadIdx.x + st]);
+            sm[3][threadIdx.x] = fmax(sm[3][threadIdx.x], sm[3][threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    double bx0, bx1, by0, by1;
+    if (auto_box) {
+        bx0 = sm[0][0];
+        bx1 = sm[1][0];
+        by0 = sm[2][0];
+        by1 = sm[3][0];
+        const double xwidth_adj = 0.1 * (bx1 - bx0), ywidth_adj = 0.1 * (by1 - by0);
+        bx0 -= xwidth_adj / 2;
+        bx1 += xwidth_adj / 2;
+        by0 -= ywidth_adj / 2;
+        by1 += ywidth_adj / 2;
+    } else {
+        bx0 = gx0;
+        bx1 = gx1;
+        by0 = gy0;
+        by1 = gy1;
+    }
+    __syncthreads();
+    int cnt = 0;
+    if (crays)
+        for (int p = threadIdx.x; p < npix; p += 1024) {
+            if (bd.mask && !bd.mask[p]) continue;
+            const double cx = crays[2 * ((size_t)item * bd.s_pix + p)], cy = crays[2 * ((size_t)item * bd.s_pix + p) + 1];
+            if ((cx > bx0) && (cx < bx1) && (cy > by0) && (cy < by1)) cnt++;
+        }
+    sm[0][threadIdx.x] = (double)cnt;
+    __syncthreads();
+    for (int st = 512; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) sm[0][threadIdx.x] += sm[0][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double* r = res + 8 * item;
+        r[0] = bx0;
+        r[1] = bx1;
+        r[2] = by0;
+        r[3] = by1;
+        r[4] = sm[0][0];
+    }
+}
+
+void launch_srcgrid_auto_b(hipStream_t s, const BatchDev& bd, int auto_box, int count_centres, double nt, const double* box,
+                           double* crays, double* res)
+{
+    // box: the given bounds (host values), used when auto_box is 0
+    if (count_centres) {
+        const long long npix = (long long)bd.nx * bd.ny;
+        hipLaunchKernelGGL(k_centre_rays_b, dim3((unsigned)((npix + 255) / 256), bd.B), dim3(256), 0, s, bd, crays);
+    }
+    hipLaunchKernelGGL(k_srcgrid_auto_b, dim3(1, bd.B), dim3(1024), 0, s, bd, auto_box, nt, box[0], box[1], box[2], box[3],
+                       count_centres ? crays : nullptr, res);
+}
+
+void launch_map_rays_b(hipStream_t s, const BatchDev& bd)
+{
+    const long long total = (long long)bd.nx * bd.ny * bd.nsub;
+    hipLaunchKernelGGL(k_map_rays_b, dim3((unsigned)((total + 255) / 256), bd.B), dim3(256), 0, s, bd);
 }
 
 // ray_fp32: FP32 ray shooting; kappa_table: FP64 rays with the NFW convergence from nfw_gtab.h
 // (the mixed path); otherwise FP64 with the closed-form convergence (the parity path)
-void launch_raytrace_map_b(hipStream_t stream, const BatchDev& bd, int ray_fp32, int kappa_table)
+void launch_raytrace_map_b(hipStream_t stream, const BatchDev& bd, int ray_fp32, int kappa_table, int map)
 {
     const long long total = (long long)bd.nx * bd.ny * bd.nsub;
     const unsigned blocks = (unsigned)((total + 255) / 256);
     hipLaunchKernelGGL(k_nfw_tables, dim3((QLX_PAT_N + 255) / 256, bd.B * QLX_MAX_LENSES), dim3(256), 0, stream, bd);
     if (ray_fp32)
-        hipLaunchKernelGGL(k_raytrace_map_b<float>, dim3(blocks, bd.B), dim3(256), 0, stream, bd, (float)bd.tol);
+        hipLaunchKernelGGL(k_raytrace_map_b<float>, dim3(blocks, bd.B), dim3(256), 0, stream, bd, (float)bd.tol, map);
     else if (kappa_table)
-        hipLaunchKernelGGL((k_raytrace_map_b<double, true>), dim3(blocks, bd.B), dim3(256), 0, stream, bd, bd.tol);
+        hipLaunchKernelGGL((k_raytrace_map_b<double, true>), dim3(blocks, bd.B), dim3(256), 0, stream, bd, bd.tol, map);
     else
-        hipLaunchKernelGGL(k_raytrace_map_b<double>, dim3(blocks, bd.B), dim3(256), 0, stream, bd, bd.tol);
+        hipLaunchKernelGGL(k_raytrace_map_b<double>, dim3(blocks, bd.B), dim3(256), 0, stream, bd, bd.tol, map);
 }
 
 __global__ void __launch_bounds__(256) k_raytrace_points(ModelPrep m, double tol, int n, const double* xy, double* beta)

@@ -45,14 +45,17 @@ class QlxSettings(C.Structure):
                 ("regparam", C.c_double), ("integral_tol", C.c_double), ("precision", C.c_int),
                 ("ray_precision", C.c_int), ("refine_iters", C.c_int), ("refine_tol", C.c_double),
                 ("optimize_regparam", C.c_int), ("regopt_minlog", C.c_double), ("regopt_maxlog", C.c_double),
-                ("regopt_tol", C.c_double), ("regopt_max_iter", C.c_int)]
+                ("regopt_tol", C.c_double), ("regopt_max_iter", C.c_int), ("auto_srcgrid", C.c_int),
+                ("auto_src_npixels", C.c_int), ("pixel_fraction", C.c_double), ("noise_threshold", C.c_double)]
 
 
 class QlxResult(C.Structure):
     _fields_ = [("minus2lnE", C.c_double), ("chisq0", C.c_double), ("logdetF", C.c_double),
                 ("logdetR", C.c_double), ("Es", C.c_double), ("refine_resid", C.c_double),
                 ("n_img_active", C.c_int), ("n_src_active", C.c_int), ("status", C.c_int),
-                ("refine_steps", C.c_int), ("regparam", C.c_double), ("regopt_evals", C.c_int), ("pad_", C.c_int)]
+                ("refine_steps", C.c_int), ("regparam", C.c_double), ("regopt_evals", C.c_int), ("src_nx", C.c_int),
+                ("src_ny", C.c_int), ("pad_", C.c_int), ("src_xmin", C.c_double), ("src_xmax", C.c_double),
+                ("src_ymin", C.c_double), ("src_ymax", C.c_double)]
 
 
 class QlxSurrogateCfg(C.Structure):
@@ -220,13 +223,17 @@ class PixelInversion:
 
     def set_settings(self, nsplit, srcgrid, src_n, regparam, integral_tol=1e-3, precision=QLX_PREC_FP64,
                      ray_precision=QLX_RAY_FP64, refine_iters=0, refine_tol=0.0, optimize_regparam=False,
-                     regopt_bounds=(-3.0, 5.0), regopt_tol=0.01, regopt_max_iter=20):
+                     regopt_bounds=(-3.0, 5.0), regopt_tol=0.01, regopt_max_iter=20, auto_srcgrid=False,
+                     auto_src_npixels=False, pixel_fraction=0.3, noise_threshold=0.0):
         """QLens settings of the inversion; optimize_regparam runs Brent over log10(lambda) in
-        regopt_bounds per item (the reference's 'optimize_regparam on', FP64 only)."""
+        regopt_bounds per item (the reference's 'optimize_regparam on', FP64 only); auto_srcgrid and
+        auto_src_npixels find each item's source-grid box and size from its own rays (the
+        reference's 'auto_srcgrid on' / 'auto_src_npixels on'; srcgrid / src_n are then unused)."""
         s = QlxSettings(nsplit, srcgrid[0], srcgrid[1], srcgrid[2], srcgrid[3], src_n[0], src_n[1], regparam,
                         integral_tol, precision, ray_precision, refine_iters, refine_tol,
                         1 if optimize_regparam else 0, regopt_bounds[0], regopt_bounds[1], regopt_tol,
-                        int(regopt_max_iter))
+                        int(regopt_max_iter), 1 if auto_srcgrid else 0, 1 if auto_src_npixels else 0,
+                        pixel_fraction, noise_threshold)
         self._check(self.lib.qlx_set_settings(self.ctx, C.byref(s)), "qlx_set_settings")
         self.settings = dict(nsplit=nsplit, srcgrid=srcgrid, src_n=src_n, regparam=regparam,
                              integral_tol=integral_tol, precision=precision)
@@ -235,7 +242,9 @@ class PixelInversion:
         self.load_image(case["xvals"], case["yvals"], data, sigma=case["sigma"], mask=case.get("mask"))
         self.set_psf(case["psf"])
         self.set_settings(case["nsplit"], case["srcgrid"], case["src_n"], case["regparam"], case["integral_tol"],
-                          precision=precision, ray_precision=ray_precision, optimize_regparam=optimize_regparam)
+                          precision=precision, ray_precision=ray_precision,
+                          optimize_regparam=optimize_regparam or bool(case.get("optimize_regparam")),
+                          auto_srcgrid=bool(case.get("auto_srcgrid")), auto_src_npixels=bool(case.get("auto_src_npixels")))
 
     # -- evaluation --------------------------------------------------------------------------
     def invert_image_surface_brightness_map(self, lens_models, def_mode=QLX_DEF_QUADRATURE, src_n=None):

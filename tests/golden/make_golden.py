@@ -180,6 +180,36 @@ def regopt_fixture(work):
     np.savez_compressed(os.path.join(HERE, "regopt.npz"), **out)
 
 
+AUTOGRID_CASES = [  # (name, workload, auto_srcgrid, auto_src_npixels)
+    ("C1_both", "C1", True, True),
+    ("CNS_box", "CNS", True, False),
+    ("CNS_npix", "CNS", False, True),
+    ("CNS_both", "CNS", True, True),
+]
+
+
+def autogrid_fixture(work):
+    """'auto_srcgrid on' / 'auto_src_npixels on' (find_optimal_sourcegrid and
+    find_optimal_sourcegrid_npixels, lens.cpp:12442-12458) on the C1 and CNS workloads, each on the
+    data of its inversion fixture (the reference's mock of the same workload)."""
+    out = {}
+    cases = {"C1": W.c1(), "CNS": cns_case()}
+    for tag, wl, ab, an in AUTOGRID_CASES:
+        data = np.load(os.path.join(HERE, "inversion_%s.npz" % wl))["data"]
+        case = dict(cases[wl], auto_srcgrid=ab, auto_src_npixels=an)
+        sc, arr = refrun.run_invert(case, data, _sub(work, "auto_" + tag))
+        for k in ("minus2lnE", "chisq0", "logdetF", "logdetR"):
+            out["%s_%s" % (k, tag)] = sc[k]
+        out["n_img_%s" % tag] = int(sc["n_img_active"])
+        out["n_src_%s" % tag] = int(sc["n_src_active"])
+        out["size_%s" % tag] = np.array([int(sc["src_nx"]), int(sc["src_ny"])])
+        out["box_%s" % tag] = np.array([sc["src_xmin"], sc["src_xmax"], sc["src_ymin"], sc["src_ymax"]])
+        out["src_%s" % tag] = arr["src"]
+        out["active_%s" % tag] = arr["active"]
+        print("autogrid", tag, "grid %dx%d" % (sc["src_nx"], sc["src_ny"]), "-2lnE", repr(sc["minus2lnE"]), flush=True)
+    np.savez_compressed(os.path.join(HERE, "autogrid.npz"), **out)
+
+
 FIXTURES = {
     "C1": lambda w: inversion_fixture(w, "C1", W.c1()),
     "C2s": lambda w: inversion_fixture(w, "C2s", W.c2_small()),
@@ -191,6 +221,7 @@ FIXTURES = {
     "rng": rng_fixture,
     "psf": psf_fixture,
     "regopt": regopt_fixture,
+    "autogrid": autogrid_fixture,
 }
 
 if __name__ == "__main__":

@@ -33,7 +33,14 @@ def test_structs_match_header_layout():
     from qlens_amd import qlx
     assert ctypes.sizeof(qlx.QlxLens) == 8 + 6 * 8
     assert ctypes.sizeof(qlx.QlxLensModel) == 8 + qlx.QLX_MAX_LENSES * ctypes.sizeof(qlx.QlxLens)
-    assert ctypes.sizeof(qlx.QlxResult) == 7 * 8 + 6 * 4
+    assert ctypes.sizeof(qlx.QlxResult) == 11 * 8 + 8 * 4
+    # and the library's own sizeof of every struct (a host-only call, no device needed)
+    lib = ctypes.CDLL(LIB)
+    sizes = (ctypes.c_size_t * 8)()
+    assert lib.qlx_struct_sizes(sizes, 8) == 8
+    mirrors = [qlx.QlxLens, qlx.QlxLensModel, qlx.QlxOpts, qlx.QlxSettings, qlx.QlxResult, qlx.QlxFitParam,
+               qlx.QlxSurrogateCfg, qlx.QlxStats]
+    assert list(sizes) == [ctypes.sizeof(m) for m in mirrors]
 
 
 def test_no_cpu_fallback_in_product():

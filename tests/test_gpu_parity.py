@@ -390,3 +390,51 @@ def test_mixed_precision_evidence(qlx_ctx, golden_dir, name):
         assert abs(r["minus2lnE"] - f["minus2lnE"]) / 2 < 0.1, (r["minus2lnE"], f["minus2lnE"])
         assert r["logdetR"] == f["logdetR"]  # R stays FP64 in both modes
         assert r["refine_resid"] <= 1e-10, r["refine_resid"]  # the configured refine_tol
+
+
+AUTOGRID = [("C1_both", "C1", True, True), ("CNS_box", "CNS", True, False), ("CNS_npix", "CNS", False, True),
+            ("CNS_both", "CNS", True, True)]
+
+
+@pytest.mark.parametrize("tag,wl,ab,an", AUTOGRID)
+def test_auto_source_grid_matches_reference(qlx_ctx, golden_dir, tag, wl, ab, an):
+    """auto_srcgrid / auto_src_npixels: each item's box and size found on the device from its own
+    rays, against the reference's (box within 1e-12 relative: the rays agree to ~1e-13; size
+    exact), and the evaluation on that grid against the reference's."""
+    g = np.load(os.path.join(golden_dir, "autogrid.npz"))
+    gi = np.load(os.path.join(golden_dir, "inversion_%s.npz" % wl))
+    case = dict(_case(wl, gi), auto_srcgrid=ab, auto_src_npixels=an)
+    print("case", tag, flush=True)
+    qlx_ctx.load_workload(case, gi["data"])
+    r = qlx_ctx.invert_image_surface_brightness_map(case["lenses"])[0]
+    box = np.array([r["src_xmin"], r["src_xmax"], r["src_ymin"], r["src_ymax"]])
+    print("%s: grid %dx%d box %s, -2lnE rel %.2e" % (tag, r["src_nx"], r["src_ny"], box,
+                                                      rel(r["minus2lnE"], float(g["minus2lnE_" + tag]))))
+    assert [r["src_nx"], r["src_ny"]] == list(g["size_" + tag])
+    assert np.allclose(box, g["box_" + tag], rtol=1e-12, atol=0)
+    assert r["status"] == 0
+    assert r["n_img_active"] == int(g["n_img_" + tag]) and r["n_src_active"] == int(g["n_src_" + tag])
+    for k in ("minus2lnE", "chisq0", "logdetF", "logdetR"):
+        assert rel(r[k], float(g["%s_%s" % (k, tag)])) < RTOL_EVIDENCE, (k, r[k])
+    sol = qlx_ctx.get_source(0)
+    assert sol["src"].shape == tuple(g["size_" + tag])
+    assert vrel(sol["src"], g["src_" + tag]) < RTOL_EVIDENCE
+    assert np.array_equal(sol["active_index"], g["active_" + tag])
+
+
+def test_auto_source_grid_per_item_in_a_batch(qlx_ctx, golden_dir):
+    """A batch of perturbed models with the automatic grid: every item has its own box and size,
+    each equal, with the item's whole result, to the item evaluated alone."""
+    gi = np.load(os.path.join(golden_dir, "inversion_CNS.npz"))
+    case = dict(_case("CNS", gi), auto_srcgrid=True, auto_src_npixels=True)
+    qlx_ctx.load_workload(case, gi["data"])
+    models = [case["lenses"]] + _perturbed(case, 5, 21)
+    res = qlx_ctx.invert_image_surface_brightness_map(models)
+    sizes = set()
+    for k, m in enumerate(models):
+        one = qlx_ctx.invert_image_surface_brightness_map(m)[0]
+        assert res[k]["status"] == 0
+        for f in ("minus2lnE", "src_nx", "src_ny", "src_xmin", "src_xmax", "src_ymin", "src_ymax", "n_src_active"):
+            assert res[k][f] == one[f], (k, f)
+        sizes.add((res[k]["src_nx"], res[k]["src_ny"], res[k]["src_xmin"]))
+    assert len(sizes) > 1  # the perturbed models do get their own grids

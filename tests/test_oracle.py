@@ -250,3 +250,50 @@ def test_mlp_oracle_bf16_rounding():
     r = mlp.bf16(x)
     assert list(r[:5]) == [1.0, 1.0, 1.015625, 1.0, 1.0078125]
     assert r[5] == -3.140625
+
+
+AUTOGRID = [("C1_both", "C1", True, True), ("CNS_box", "CNS", True, False), ("CNS_npix", "CNS", False, True),
+            ("CNS_both", "CNS", True, True)]
+
+
+@pytest.mark.parametrize("tag,wl,ab,an", AUTOGRID)
+def test_oracle_auto_source_grid_matches_reference(golden_dir, tag, wl, ab, an):
+    """The automatic source-grid box and size from the oracle's rays equal the reference's
+    (auto_srcgrid / auto_src_npixels on), bit for bit.  A box from the centre rays instead of the
+    subpixel means, a missing 10 % widening, a count without the mask or over the subpixel rays,
+    or a size without the doubling would each change the box or the size."""
+    g = load(golden_dir, "autogrid.npz")
+    gi = load(golden_dir, "inversion_%s.npz" % wl)
+    case = W.c1() if wl == "C1" else cns_case(gi)
+    _, arr = O.invert(case, gi["data"])
+    box, size, count = O.auto_source_grid(case, arr["rays"], gi["data"], auto_box=ab, auto_npixels=an)
+    assert list(size) == list(g["size_" + tag])
+    assert box == list(g["box_" + tag])
+    if tag == "CNS_box":  # small enough for the oracle's dense solve: the whole evaluation on it
+        c2 = dict(case, srcgrid=box, src_n=size)
+        res, a2 = O.invert(c2, gi["data"])
+        assert res["minus2lnE"] == float(g["minus2lnE_" + tag])
+        assert res["n_src"] == int(g["n_src_" + tag])
+        assert np.array_equal(a2["src"], g["src_" + tag])
+
+
+def test_oracle_auto_source_grid_closed_form():
+    """On a hand-made ray field the box is the min / max of the per-pixel means widened by 5 % per
+    side, and the size follows nx0 = floor(sqrt(f count dx/dy)), ny0 = floor(nx0 dy/dx), doubled."""
+    case = W.c1()
+    nx, ny = len(case["xvals"]) - 1, len(case["yvals"]) - 1
+    rng = np.random.default_rng(1)
+    rays = rng.uniform(-1, 1, (nx, ny, 4, 2))
+    rays[3, 5, :, 0] = [2.0, 2.0, 2.0, 2.5]    # mean 2.125: the x maximum
+    rays[7, 1, :, 1] = [-3.0, -3.0, -3.0, -3.0]  # the y minimum
+    box, size, count = O.auto_source_grid(case, rays, None, auto_box=True, auto_npixels=False)
+    xm = rays[..., 0].mean(-1)
+    assert box[1] == pytest.approx(2.125 + 0.05 * (2.125 - xm.min()), rel=1e-15)
+    assert box[2] == pytest.approx(-3.0 - 0.05 * (rays[..., 1].mean(-1).max() + 3.0), rel=1e-15)
+    assert size == tuple(case["src_n"]) and count is None
+    # size: the whole image inside a huge box counts every pixel
+    c2 = dict(case, srcgrid=(-1e3, 1e3, -5e2, 5e2))
+    _, size, count = O.auto_source_grid(c2, rays, None, auto_box=False, auto_npixels=True)
+    assert count == nx * ny
+    nx0 = int(np.floor(np.sqrt(0.3 * nx * ny * 2.0)))
+    assert size == (2 * nx0, 2 * int(np.floor(nx0 * 0.5)))
