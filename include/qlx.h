@@ -35,7 +35,7 @@ typedef enum {
 } qlx_status;
 
 /* lens types */
-enum { QLX_LENS_SIE = 1, QLX_LENS_NFW = 2, QLX_LENS_SHEAR = 3 };
+enum { QLX_LENS_SIE = 1, QLX_LENS_NFW = 2, QLX_LENS_SHEAR = 3, QLX_LENS_SPLE = 4, QLX_LENS_SHEAR_COMP = 5 };
 /* how a numerical profile's deflection is obtained */
 enum { QLX_DEF_QUADRATURE = 0, QLX_DEF_SURROGATE = 1 };
 /* precision modes */
@@ -50,9 +50,17 @@ enum { QLX_RAY_FP64 = 0, QLX_RAY_FP32 = 1 };
 #define QLX_MAX_LENSES 8
 
 /* One lens in QLens parameters (replaces LensProfile + its subclass, profile.h / models.cpp):
- *   SIE   p = {b, q, theta_deg, xc, yc, s}     SPLE_Lens with alpha = 1 (models.cpp:101-230)
+ *   SIE   p = {b, q, theta_deg, xc, yc, s}     SPLE_Lens with alpha = 1, cored when s > 0
+ *                                              (models.cpp:101-230)
  *   NFW   p = {ks, rs, q, theta_deg, xc, yc}   NFW pmode 0 (models.cpp:953-1010)
  *   SHEAR p = {gamma, theta_deg, xc, yc}       Shear (models.cpp:2140-2272)
+ *   SPLE  p = {b, alpha, q, theta_deg, xc, yc} SPLE_Lens ('lens alpha' / 'lens sple') without a
+ *                                              core: Tessore et al. series for q < 1
+ *                                              (models.cpp:244-351), the spherical kappa average
+ *                                              for q = 1 (models.cpp:169-172); alpha in (0, 2);
+ *                                              alpha = 1 is the SIE formula, as the reference
+ *                                              dispatches it (models.cpp:141-147)
+ *   SHEAR_COMP p = {shear1, shear2, xc, yc}    Shear with 'shear_components on' (models.cpp:2215-2217)
  * def_mode applies to NFW only: Gauss-Patterson quadrature (profile.cpp:2359-2416) or the
  * trained surrogate (falls back to quadrature outside its training box). */
 typedef struct {
@@ -73,7 +81,8 @@ typedef struct {
 } qlx_opts;
 
 /* Inversion settings (the QLens fields the block reads, qlens.h:750-893):
- * nsplit = default_imgpixel_nsplit; source grid = srcgrid_x/ymin/max and srcgrid_npixels_x/y;
+ * nsplit = default_imgpixel_nsplit (0 = split_imgpixels off: one ray at each pixel centre,
+ * pixelgrid.cpp:9659-9666, 13277-13285); source grid = srcgrid_x/ymin/max and srcgrid_npixels_x/y;
  * regparam = the pixsrc regparam; integral_tol = integral_tolerance; curvature regularization,
  * 3-point interpolation, dense inversion. */
 typedef struct {

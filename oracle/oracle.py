@@ -12,7 +12,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
-SIE, NFW, SHEAR = 1, 2, 3
+SIE, NFW, SHEAR, SPLE, SHEAR_COMP = 1, 2, 3, 4, 5
 
 
 class OrcLens(C.Structure):
@@ -163,7 +163,7 @@ def invert(case, data, lenses=None, want_arrays=True):
     arrays = {}
     if want_arrays:
         arrays = dict(
-            rays=np.zeros((nx, ny, ns * ns, 2)), src=np.zeros((snx, sny)),
+            rays=np.zeros((nx, ny, max(ns * ns, 1), 2)), src=np.zeros((snx, sny)),
             active=np.zeros((snx, sny), dtype=np.int32), model=np.zeros((nx, ny)),
             mapped=np.zeros((nx, ny), dtype=np.uint8))
         lib().orc_invert(C.byref(P), C.byref(R), _dp(arrays["rays"]), _dp(arrays["src"]),

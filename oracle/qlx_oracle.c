@@ -17,6 +17,7 @@
 
 #define DEG2RAD 0.017453292519943295 /* mathexpr.h:27 */
 #define QLX_HALFPI_REF 1.57079632679 /* mathexpr.h:10: the reference's truncated pi/2 */
+#define ORC_M_PI 3.14159265358979323846 /* M_PI of <math.h>, used by models.cpp:251-256 */
 
 /* ------------------------------------------------------------------------------------------
  * Lens models
@@ -32,6 +33,8 @@ typedef struct {
     double ks, rs, eps, fsqinv, sqrt1meps;
     /* shear */
     double shear1, shear2;
+    /* SPLE without a core */
+    double b, alpha;
 } lprep;
 
 static void prep_angle(lprep *L, double theta_deg)
@@ -47,7 +50,38 @@ static void prep_lens(const orc_lens *in, lprep *L)
 {
     memset(L, 0, sizeof(*L));
     L->type = in->type;
-    if (in->type == ORC_SIE) {
+    if (in->type == ORC_SPLE && in->p[1] == 1.0) {
+        /* alpha = 1: the isothermal formulas (set_model_specific_integration_pointers, models.cpp:141-147) */
+        double b = in->p[0], q = in->p[2];
+        double fmaj = 1.0 / sqrt(q);
+        L->type = ORC_SIE;
+        L->q = q;
+        L->bprime = b * fmaj;
+        L->sprime = 0.0 * fmaj;
+        L->qsq = q * q;
+        L->ssq = L->sprime * L->sprime;
+        prep_angle(L, in->p[3]);
+        L->xc = in->p[4];
+        L->yc = in->p[5];
+    } else if (in->type == ORC_SPLE) {
+        /* SPLE_Lens::update_meta_parameters (models.cpp:101-110), emode 1, s = 0 */
+        double q = in->p[2];
+        L->b = in->p[0];
+        L->alpha = in->p[1];
+        L->q = q;
+        L->qsq = q * q;
+        L->bprime = L->b * (1.0 / sqrt(q));
+        prep_angle(L, in->p[3]);
+        L->xc = in->p[4];
+        L->yc = in->p[5];
+    } else if (in->type == ORC_SHEAR_COMP) {
+        /* use_shear_component_params (models.cpp:2215-2217) */
+        L->type = ORC_SHEAR;
+        L->shear1 = in->p[0];
+        L->shear2 = in->p[1];
+        L->xc = in->p[2];
+        L->yc = in->p[3];
+    } else if (in->type == ORC_SIE) {
         /* SPLE_Lens::update_meta_parameters (models.cpp:101-110), emode 1 (profile.cpp:1690-1693) */
         double b = in->p[0], q = in->p[1], s = in->p[5];
         double fmaj = 1.0 / sqrt(q);
@@ -207,6 +241,45 @@ static void lens_deflection(const lprep *L, double x, double y, double tol, doub
              * for q = 1, f_major_axis = 1 so b' = b, s' = s */
             double rsq = x * x + y * y;
             double kapavg = L->bprime * (sqrt(L->sprime * L->sprime + rsq) - L->sprime) / rsq;
+            a0 = kapavg * x;
+            a1 = kapavg * y;
+        }
+    } else if (L->type == ORC_SPLE) {
+        if (L->q != 1.0) {
+            /* SPLE_Lens::deflection_elliptical_nocore (models.cpp:244-262) with
+             * deflection_angular_factor (models.cpp:334-351): complex products as (ac - bd, ad + bc),
+             * the loop run while norm(omega) > 1e-16 norm(fac) */
+            double R = sqrt(x * x + y * y / L->qsq);
+            double phi = atan(fabs(y / (L->q * x)));
+            if (x < 0) {
+                if (y < 0) phi = phi - ORC_M_PI;
+                else phi = ORC_M_PI - phi;
+            } else if (y < 0) {
+                phi = -phi;
+            }
+            double beta = 2.0 / (2 - L->alpha);
+            double ff = (1 - L->q) / (1 + L->q);
+            double fr = 1.0 * cos(phi), fi = 1.0 * sin(phi);
+            double wr = fr, wi = fi;
+            int i = 1;
+            do {
+                double r = ff * (beta * i - 1) / (beta * i + 1);
+                double pr = -(r * cos(2 * phi)), pim = -(r * sin(2 * phi));
+                double ac = pr * wr, bd = pim * wi, ad = pr * wi, bc = pim * wr;
+                wr = ac - bd;
+                wi = ad + bc;
+                fr += wr;
+                fi += wi;
+                i++;
+            } while (wr * wr + wi * wi > 1e-16 * (fr * fr + fi * fi));
+            double amp = 2 * L->bprime * L->q / (1 + L->q) * pow(L->bprime / R, L->alpha - 1);
+            a0 = amp * fr;
+            a1 = amp * fi;
+        } else {
+            /* spherical default with kapavg_spherical_rsq (profile.cpp:2281-2290, models.cpp:169-172), s = 0 */
+            double rsq = x * x + y * y;
+            double s = 0.0;
+            double kapavg = (pow(L->b, L->alpha) * (pow(rsq + s * s, 1 - L->alpha / 2) - pow(s, 2 - L->alpha))) / rsq;
             a0 = kapavg * x;
             a1 = kapavg * y;
         }
@@ -393,10 +466,27 @@ static void cholesky_solve_packed(const double *a, const double *b, double *x, i
  * Inversion
  * ------------------------------------------------------------------------------------------ */
 
+/* Origin of ray (ii, jj) of pixel (i, j): the subpixel centre of an ns x ns split
+ * (ImagePixelGrid::set_nsplits, pixelgrid.cpp:10898-10938), or with split_imgpixels off (ns = 0)
+ * the pixel centre, corner plus half the pixel size (pixelgrid.cpp:9643-9666) */
+static void ray_origin(const double *xv, const double *yv, int nx, int ny, int ns, int i, int j, int ii, int jj, double *sx, double *sy)
+{
+    if (ns == 0) {
+        double pxl = (xv[nx] - xv[0]) / nx, pyl = (yv[ny] - yv[0]) / ny;
+        *sx = xv[i] + 0.5 * pxl;
+        *sy = yv[j] + 0.5 * pyl;
+        return;
+    }
+    double u0 = ((double)(1 + 2 * ii)) / (2 * ns);
+    double w0 = ((double)(1 + 2 * jj)) / (2 * ns);
+    *sx = (1 - u0) * xv[i] + u0 * xv[i + 1];
+    *sy = (1 - w0) * yv[j] + w0 * yv[j + 1];
+}
+
 int orc_invert(const orc_problem *P, orc_result *res, double *rays_out, double *src_out, int *active_out, double *model_out, unsigned char *mapped_out)
 {
     memset(res, 0, sizeof(*res));
-    const int nx = P->nx, ny = P->ny, ns = P->nsplit, nsub = ns * ns;
+    const int nx = P->nx, ny = P->ny, ns = P->nsplit, nsub = ns ? ns * ns : 1, nsl = ns ? ns : 1;
     const int snx = P->src_nx, sny = P->src_ny;
     if (snx < 2 || sny < 2) { /* lens.cpp:12453-12458 sentinel */
         res->minus2lnE = 2e30;
@@ -414,12 +504,10 @@ int orc_invert(const orc_problem *P, orc_result *res, double *rays_out, double *
     for (int i = 0; i < nx; i++) {
         for (int j = 0; j < ny; j++) {
             int k = 0;
-            for (int ii = 0; ii < ns; ii++) {
-                for (int jj = 0; jj < ns; jj++) {
-                    double u0 = ((double)(1 + 2 * ii)) / (2 * ns);
-                    double w0 = ((double)(1 + 2 * jj)) / (2 * ns);
-                    double sx = (1 - u0) * P->xvals[i] + u0 * P->xvals[i + 1];
-                    double sy = (1 - w0) * P->yvals[j] + w0 * P->yvals[j + 1];
+            for (int ii = 0; ii < nsl; ii++) {
+                for (int jj = 0; jj < nsl; jj++) {
+                    double sx, sy;
+                    ray_origin(P->xvals, P->yvals, nx, ny, ns, i, j, ii, jj, &sx, &sy);
                     double a0, a1;
                     long long nev = 0;
                     /* QLens::find_sourcept (lenscalc.cpp:325-330): beta = theta - alpha */
@@ -837,19 +925,18 @@ int orc_mock_image(int nx, int ny, const double *xvals, const double *yvals, int
 {
     lprep *Ls = (lprep *)malloc(sizeof(lprep) * (nlens > 0 ? nlens : 1));
     for (int l = 0; l < nlens; l++) prep_lens(&lenses[l], &Ls[l]);
-    const int nsub = nsplit * nsplit;
+    const int nsub = nsplit ? nsplit * nsplit : 1, nsl = nsplit ? nsplit : 1;
     double *sb = (double *)malloc(sizeof(double) * (size_t)nx * ny);
-    /* ImagePixelGrid::find_surface_brightness split branch (pixelgrid.cpp:11835-11896) */
+    /* ImagePixelGrid::find_surface_brightness split branch (pixelgrid.cpp:11835-11896); with
+     * nsplit = 0 the unsplit branch over the pixel-centre rays (11896-11925) */
 #pragma omp parallel for schedule(dynamic)
     for (int i = 0; i < nx; i++)
         for (int j = 0; j < ny; j++) {
             double sbtot = 0;
-            for (int ii = 0; ii < nsplit; ii++)
-                for (int jj = 0; jj < nsplit; jj++) {
-                    double u0 = ((double)(1 + 2 * ii)) / (2 * nsplit);
-                    double w0 = ((double)(1 + 2 * jj)) / (2 * nsplit);
-                    double sx = (1 - u0) * xvals[i] + u0 * xvals[i + 1];
-                    double sy = (1 - w0) * yvals[j] + w0 * yvals[j + 1];
+            for (int ii = 0; ii < nsl; ii++)
+                for (int jj = 0; jj < nsl; jj++) {
+                    double sx, sy;
+                    ray_origin(xvals, yvals, nx, ny, nsplit, i, j, ii, jj, &sx, &sy);
                     double a0, a1;
                     total_deflection(Ls, nlens, sx, sy, tol, &a0, &a1, NULL);
                     double bx = sx - a0, by = sy - a1;

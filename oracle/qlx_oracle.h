@@ -12,7 +12,8 @@
 extern "C" {
 #endif
 
-enum { ORC_SIE = 1, ORC_NFW = 2, ORC_SHEAR = 3 };
+enum { ORC_SIE = 1, ORC_NFW = 2, ORC_SHEAR = 3, ORC_SPLE = 4, ORC_SHEAR_COMP = 5 };
+/* SPLE p = {b, alpha, q, theta, xc, yc} (no core); SHEAR_COMP p = {shear1, shear2, xc, yc} */
 
 /* One lens in QLens conventions (emode 1, lens angle measured from +y).
  *   SIE   p = {b, q, theta_deg, xc, yc, s}        SPLE_Lens with alpha = 1 (models.cpp:101-230)
@@ -30,7 +31,8 @@ typedef struct {
     const double *data;          /* observed surface brightness */
     const double *covinv;        /* 1/sigma^2 per pixel (the reference's covinv_map) */
     const unsigned char *mask;   /* NULL = all pixels in mask */
-    int nsplit;                  /* subpixel split per axis (default_imgpixel_nsplit) */
+    int nsplit;                  /* subpixel split per axis (default_imgpixel_nsplit); 0 = split_imgpixels
+                                    off: one ray at each pixel centre */
     /* Cartesian source grid */
     double src_xmin, src_xmax, src_ymin, src_ymax;
     int src_nx, src_ny;

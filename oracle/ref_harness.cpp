@@ -109,7 +109,8 @@ static int mode_invert(QLens& lens, const string& out, double sigma, int nrep)
 
 	ImagePixelGrid* g = lens.image_pixel_grids[0];
 	int nx = g->x_N, ny = g->y_N;
-	int nsub = INTSQR(lens.default_imgpixel_nsplit);
+	// split_imgpixels off: one ray per pixel, the pixel-centre ray (center_sourcepts)
+	int nsub = lens.split_imgpixels ? INTSQR(lens.default_imgpixel_nsplit) : 1;
 
 	// rays: beta for every subpixel of every pixel, [i][j][k][2], i-major
 	vector<double> rays((size_t)nx * ny * nsub * 2);
@@ -121,8 +122,9 @@ static int mode_invert(QLens& lens, const string& out, double sigma, int nrep)
 			mapped[p] = g->maps_to_source_pixel[i][j] ? 1 : 0;
 			model[p] = g->surface_brightness[i][j];
 			for (int k = 0; k < nsub; k++) {
-				rays[(p * nsub + k) * 2 + 0] = g->subpixel_center_sourcepts[i][j][k][0];
-				rays[(p * nsub + k) * 2 + 1] = g->subpixel_center_sourcepts[i][j][k][1];
+				const lensvector& b = lens.split_imgpixels ? g->subpixel_center_sourcepts[i][j][k] : g->center_sourcepts[i][j];
+				rays[(p * nsub + k) * 2 + 0] = b[0];
+				rays[(p * nsub + k) * 2 + 1] = b[1];
 			}
 		}
 	write_bin(out + "/rays.bin", rays.data(), rays.size() * sizeof(double));

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.abspath(os.path.join(HERE, ".."))
 HARNESS = os.path.join(REPO, "oracle", "_ref", "ref_harness")
 
-SIE, NFW, SHEAR = 1, 2, 3
+SIE, NFW, SHEAR, SPLE, SHEAR_COMP = 1, 2, 3, 4, 5
 
 
 def have_reference():
@@ -27,6 +27,11 @@ def _fmt(v):
 
 def lens_commands(lenses):
     out = []
+    comps = any(t == SHEAR_COMP for t, _ in lenses)
+    if comps:
+        if any(t == SHEAR for t, _ in lenses):
+            raise ValueError("the reference takes shear as components or as (gamma, theta), not both")
+        out.append("shear_components on")
     for t, p in lenses:
         if t == SIE:  # lens sple <b> <alpha> <s> <q> [theta] [xc] [yc]  (commands.cpp:465)
             b, q, th, xc, yc, s = p
@@ -37,6 +42,12 @@ def lens_commands(lenses):
         elif t == SHEAR:  # lens shear <shear> [theta] [xc] [yc]  (commands.cpp:492)
             g, th, xc, yc = p
             out.append("lens shear %s %s %s %s" % (_fmt(g), _fmt(th), _fmt(xc), _fmt(yc)))
+        elif t == SHEAR_COMP:  # with shear_components on: lens shear <shear1> <shear2> [xc] [yc]
+            g1, g2, xc, yc = p
+            out.append("lens shear %s %s %s %s" % (_fmt(g1), _fmt(g2), _fmt(xc), _fmt(yc)))
+        elif t == SPLE:  # lens sple <b> <alpha> <s> <q> [theta] [xc] [yc], s = 0
+            b, al, q, th, xc, yc = p
+            out.append("lens sple %s %s 0 %s %s %s %s" % (_fmt(b), _fmt(al), _fmt(q), _fmt(th), _fmt(xc), _fmt(yc)))
     return out
 
 
@@ -54,8 +65,8 @@ def write_inputs(case, data, workdir, lenses=None, psf_mode="input"):
     psf.tofile(os.path.join(workdir, "psf.bin"))
     lines = [
         "fit source_mode cartesian",
-        "split_imgpixels on",
-        "imgpixel_nsplit %d" % case["nsplit"],
+        "split_imgpixels %s" % ("on" if case["nsplit"] else "off"),
+        "imgpixel_nsplit %d" % max(case["nsplit"], 1),
         "raytrace_method interpolate_3pt",
         "inversion_method dense",
         "fit regularization curvature",
@@ -106,7 +117,7 @@ def run_invert(case, data, workdir, lenses=None, nrep=1, psf_mode="input", threa
         raise RuntimeError("reference harness failed (%d): %s %s" % (r.returncode, r.stdout[-2000:], r.stderr[-2000:]))
     sc = read_scalars(workdir)
     nx, ny = len(case["xvals"]) - 1, len(case["yvals"]) - 1
-    ns = case["nsplit"]
+    ns = max(case["nsplit"], 1)
     snx, sny = int(sc["src_nx"]), int(sc["src_ny"])  # the automatic size, when asked for
     arr = dict(
         rays=np.fromfile(os.path.join(workdir, "rays.bin")).reshape(nx, ny, ns * ns, 2),
@@ -141,8 +152,8 @@ def run_mock(case, workdir, sigma=None, seed=None):
     seed = case["seed"] if seed is None else seed
     lines = [
         "fit source_mode sbprofile",
-        "split_imgpixels on",
-        "imgpixel_nsplit %d" % case["nsplit"],
+        "split_imgpixels %s" % ("on" if case["nsplit"] else "off"),
+        "imgpixel_nsplit %d" % max(case["nsplit"], 1),
         "grid %s %s %s %s" % (_fmt(xv[0]), _fmt(xv[-1]), _fmt(yv[0]), _fmt(yv[-1])),
         "img_npixels %d %d" % (nx, ny),
         "integral_tolerance %s" % _fmt(case["integral_tol"]),

@@ -200,7 +200,7 @@ void prep_model(const qlx_lens_model& in, ModelPrep& out)
         LensPrep& L = out.lens[l];
         L.type = s.type;
         L.def_mode = s.def_mode;
-        if (s.type == QLX_LENS_SIE || s.type == QLX_LENS_NFW) {
+        if (s.type == QLX_LENS_SIE || s.type == QLX_LENS_NFW || s.type == QLX_LENS_SPLE) {
             // LensProfile::set_angle + orient_major_axis_north (profile.cpp:1625-1637)
             double th = DEG2RAD * (s.type == QLX_LENS_SIE ? s.p[2] : s.p[3]);
             double cth = cos(th), sth = sin(th);
@@ -208,9 +208,12 @@ void prep_model(const qlx_lens_model& in, ModelPrep& out)
             L.costh = -sth;
             L.rotate = (L.sinth != 0);
         }
-        if (s.type == QLX_LENS_SIE) {
-            // SPLE_Lens::update_meta_parameters, emode 1 (models.cpp:101-110, profile.cpp:1690-1693)
-            double b = s.p[0], q = s.p[1], core = s.p[5];
+        if (s.type == QLX_LENS_SIE || (s.type == QLX_LENS_SPLE && s.p[1] == 1.0)) {
+            // SPLE_Lens::update_meta_parameters, emode 1 (models.cpp:101-110, profile.cpp:1690-1693);
+            // alpha = 1 takes the isothermal formulas (models.cpp:141-147)
+            const bool sple = s.type == QLX_LENS_SPLE;
+            L.type = QLX_LENS_SIE;
+            double b = s.p[0], q = sple ? s.p[2] : s.p[1], core = sple ? 0.0 : s.p[5];
             double fmaj = 1.0 / sqrt(q);
             L.q = q;
             L.bprime = b * fmaj;
@@ -219,8 +222,23 @@ void prep_model(const qlx_lens_model& in, ModelPrep& out)
             L.ssq = L.sprime * L.sprime;
             L.sie_u = sqrt(1 - L.qsq);
             L.sie_fac = L.bprime * q / L.sie_u;
-            L.xc = s.p[3];
-            L.yc = s.p[4];
+            L.xc = sple ? s.p[4] : s.p[3];
+            L.yc = sple ? s.p[5] : s.p[4];
+        } else if (s.type == QLX_LENS_SPLE) {
+            // SPLE_Lens::update_meta_parameters (models.cpp:101-110) and the constants of
+            // deflection_elliptical_nocore / deflection_angular_factor (models.cpp:244-262, 334-351)
+            double b = s.p[0], alpha = s.p[1], q = s.p[2];
+            L.q = q;
+            L.qsq = q * q;
+            L.bprime = b * (1.0 / sqrt(q));
+            L.alpha = alpha;
+            L.sple_ff = (1 - q) / (1 + q);
+            L.sple_beta = 2.0 / (2 - alpha);
+            L.sple_fac = 2 * L.bprime * q / (1 + q);
+            L.sple_balpha = pow(b, alpha);  // kapavg_spherical_rsq (models.cpp:169-172), s = 0
+            L.sple_e = 1 - alpha / 2;
+            L.xc = s.p[4];
+            L.yc = s.p[5];
         } else if (s.type == QLX_LENS_NFW) {
             if (s.def_mode == QLX_DEF_SURROGATE && out.sur_lens < 0) out.sur_lens = l;
             double q = s.p[2];
@@ -239,6 +257,13 @@ void prep_model(const qlx_lens_model& in, ModelPrep& out)
             double theta_eff = th + HALFPI_REF;
             L.shear1 = s.p[0] * cos(2 * theta_eff);
             L.shear2 = s.p[0] * sin(2 * theta_eff);
+            L.xc = s.p[2];
+            L.yc = s.p[3];
+        } else if (s.type == QLX_LENS_SHEAR_COMP) {
+            // use_shear_component_params: the components are the parameters (models.cpp:2215-2217)
+            L.type = QLX_LENS_SHEAR;
+            L.shear1 = s.p[0];
+            L.shear2 = s.p[1];
             L.xc = s.p[2];
             L.yc = s.p[3];
         }
@@ -260,6 +285,8 @@ const char* lens_params_error(const qlx_lens_model& m)
             if (!std::isfinite(L.p[i])) return "lens parameter is not finite";
         if (L.type == QLX_LENS_SIE && !(L.p[1] > 0 && L.p[1] <= 1)) return "SIE q must be in (0,1]";
         if (L.type == QLX_LENS_NFW && !(L.p[2] > 0 && L.p[2] <= 1 && L.p[1] > 0)) return "NFW needs rs > 0 and q in (0,1]";
+        if (L.type == QLX_LENS_SPLE && !(L.p[2] > 0 && L.p[2] <= 1 && L.p[1] > 0 && L.p[1] < 2 && L.p[0] > 0))
+            return "SPLE needs b > 0, alpha in (0,2) and q in (0,1]";
     }
     return nullptr;
 }
@@ -271,7 +298,8 @@ int check_model(Ctx* c, const qlx_lens_model* m, bool values = true)
     if (!m || m->nlens < 0 || m->nlens > QLX_MAX_LENSES) return fail(c, QLX_ERR_ARG, "invalid lens model");
     for (int l = 0; l < m->nlens; l++) {
         int t = m->lens[l].type;
-        if (t != QLX_LENS_SIE && t != QLX_LENS_NFW && t != QLX_LENS_SHEAR) return fail(c, QLX_ERR_ARG, "unknown lens type");
+        if (t != QLX_LENS_SIE && t != QLX_LENS_NFW && t != QLX_LENS_SHEAR && t != QLX_LENS_SPLE && t != QLX_LENS_SHEAR_COMP)
+            return fail(c, QLX_ERR_ARG, "unknown lens type");
         if (m->lens[l].def_mode != QLX_DEF_QUADRATURE && m->lens[l].def_mode != QLX_DEF_SURROGATE)
             return fail(c, QLX_ERR_ARG, "unknown deflection mode");
     }
@@ -458,7 +486,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     if (B == 0) return QLX_OK;
     const qlx_settings& S = c->set;
     const bool mixed = S.precision == QLX_PREC_MIXED;
-    const int nsub = S.nsplit * S.nsplit;
+    const int nsub = rays_per_pixel(S.nsplit);
     const size_t s_pix = (size_t)c->nx * c->ny;
     size_t s_cell = 4;
     for (auto& it : items) s_cell = std::max(s_cell, it.cells);
@@ -954,7 +982,7 @@ int eval_items(Ctx* c, int nbatch, const qlx_lens_model* models, const int* src_
     c->last_wave.clear();
     c->stats_waves = 0;
     const bool mixed = c->set.precision == QLX_PREC_MIXED;
-    const int nsub = c->set.nsplit * c->set.nsplit;
+    const int nsub = rays_per_pixel(c->set.nsplit);
     const size_t s_pix = (size_t)c->nx * c->ny;
     const qlx_settings& S = c->set;
     const bool auto_n = S.auto_src_npixels != 0;
@@ -1204,7 +1232,7 @@ int qlx_set_settings(void* ctx, const qlx_settings* s)
     c->kept.clear();  // solutions of earlier calls no longer match the inputs
     c->last_wave.clear();
     if (!s) return fail(c, QLX_ERR_ARG, "null settings");
-    if (s->nsplit < 1 || s->nsplit > 4) return fail(c, QLX_ERR_RANGE, "nsplit must be 1..4");
+    if (s->nsplit < 0 || s->nsplit > 4) return fail(c, QLX_ERR_RANGE, "nsplit must be 0..4");
     if (!s->auto_srcgrid && (!(s->src_xmax > s->src_xmin) || !(s->src_ymax > s->src_ymin)))
         return fail(c, QLX_ERR_ARG, "empty source grid bounds");
     if (!(s->noise_threshold >= 0) || !(s->pixel_fraction < 1e6)) return fail(c, QLX_ERR_ARG, "bad noise threshold or pixel fraction");
@@ -1277,8 +1305,8 @@ int qlx_get_rays(void* ctx, int item, double* beta, size_t cap)
 {
     Ctx* c = (Ctx*)ctx;
     if (!c || !beta) return QLX_ERR_ARG;
-    const size_t n = (size_t)c->nx * c->ny * c->set.nsplit * c->set.nsplit * 2;
-    if (cap < n) return fail(c, QLX_ERR_ARG, "qlx_get_rays: buffer smaller than nx*ny*nsplit^2*2");
+    const size_t n = (size_t)c->nx * c->ny * rays_per_pixel(c->set.nsplit) * 2;
+    if (cap < n) return fail(c, QLX_ERR_ARG, "qlx_get_rays: buffer smaller than nx*ny*(rays per pixel)*2");
     int pos = -1;
     for (size_t w = 0; w < c->last_wave.size(); w++)
         if (c->last_wave[w] == item) pos = (int)w;

@@ -46,6 +46,8 @@ struct LensPrep {
     double sie_u, sie_fac;                              // SIE: sqrt(1-q^2), bprime*q/u
     double ks, rs, rs2, eps, fsqinv, sqrt1meps;         // NFW
     double shear1, shear2;                              // shear
+    double alpha, sple_ff, sple_beta, sple_fac;         // SPLE: slope, (1-q)/(1+q), 2/(2-alpha), 2 b' q/(1+q)
+    double sple_balpha, sple_e;                         // SPLE, q = 1: b^alpha, 1 - alpha/2
 };
 
 struct ModelPrep {
@@ -55,6 +57,9 @@ struct ModelPrep {
 };
 
 void prep_model(const qlx_lens_model& in, ModelPrep& out);
+
+// rays per pixel: nsplit^2 subpixels, or one pixel-centre ray when nsplit is 0 (split_imgpixels off)
+__host__ __device__ inline int rays_per_pixel(int nsplit) { return nsplit ? nsplit * nsplit : 1; }
 
 // sizes and pool offsets of one item
 struct ItemMeta {

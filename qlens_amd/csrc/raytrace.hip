@@ -312,6 +312,41 @@ __device__ __forceinline__ void lens_deflection(const LensPrep& L, T x, T y, T t
             a0 = kapavg * x;
             a1 = kapavg * y;
         }
+    } else if (L.type == QLX_LENS_SPLE) {
+        if (L.q != 1.0) {
+            // SPLE_Lens::deflection_elliptical_nocore (models.cpp:244-262): elliptical radius and
+            // angle, then the series of deflection_angular_factor (models.cpp:334-351), omega_i =
+            // -f (beta i - 1)/(beta i + 1) e^{2 i phi} omega_{i-1}, summed while |omega|^2 exceeds
+            // 1e-16 |sum|^2 (1e-13 in FP32, whose rounding sits above 1e-16); at most 4096 terms
+            const T R = sqrt(x * x + y * y / (T)L.qsq);
+            T phi = atan(fabs(y / ((T)L.q * x)));
+            if (x < 0) phi = (y < 0) ? phi - (T)M_PI : (T)M_PI - phi;
+            else if (y < 0) phi = -phi;
+            const T c2 = cos(2 * phi), s2 = sin(2 * phi);
+            T wr = cos(phi), wi = sin(phi);
+            T fr = wr, fi = wi;
+            const T stol = sizeof(T) == 4 ? (T)1e-13 : (T)1e-16;
+            const T ff = (T)L.sple_ff, beta = (T)L.sple_beta;
+            for (int i = 1; i <= 4096; i++) {
+                const T r = ff * (beta * i - 1) / (beta * i + 1);
+                const T pr = -(r * c2), pi = -(r * s2);
+                const T nr = pr * wr - pi * wi;
+                wi = pr * wi + pi * wr;
+                wr = nr;
+                fr += wr;
+                fi += wi;
+                if (!(wr * wr + wi * wi > stol * (fr * fr + fi * fi))) break;
+            }
+            const T amp = (T)L.sple_fac * pow((T)L.bprime / R, (T)L.alpha - 1);
+            a0 = amp * fr;
+            a1 = amp * fi;
+        } else {
+            // spherical default with kapavg_spherical_rsq, s = 0 (profile.cpp:2281-2290, models.cpp:169-172)
+            const T rsq = x * x + y * y;
+            const T kapavg = (T)L.sple_balpha * pow(rsq, (T)L.sple_e) / rsq;
+            a0 = kapavg * x;
+            a1 = kapavg * y;
+        }
     } else {  // NFW
         if (L.q != 1.0) {
             if ((fabs(x) < (T)1e-14) && (fabs(y) < (T)1e-14)) {
@@ -420,6 +455,24 @@ __device__ __forceinline__ void map_ray(const BatchDev& bd, int item, size_t ro,
     bd.pmapped[(size_t)item * bd.s_pix + p] = 1;
 }
 
+// Origin of ray k of pixel (i, j): the centre of subpixel (k / nsplit, k % nsplit) of an
+// nsplit x nsplit split (ImagePixelGrid::set_nsplits, pixelgrid.cpp:10898-10938), or with nsplit = 0
+// (split_imgpixels off) the pixel centre, corner plus half the pixel size (pixelgrid.cpp:9643-9666)
+__device__ __forceinline__ void ray_origin(const double* xv, const double* yv, int nx, int ny, int nsplit, int i, int j, int k,
+                                           double& sx, double& sy)
+{
+    if (nsplit == 0) {
+        sx = xv[i] + 0.5 * ((xv[nx] - xv[0]) / nx);
+        sy = yv[j] + 0.5 * ((yv[ny] - yv[0]) / ny);
+        return;
+    }
+    const int ii = k / nsplit, jj = k % nsplit;
+    const double u0 = ((double)(1 + 2 * ii)) / (2 * nsplit);
+    const double w0 = ((double)(1 + 2 * jj)) / (2 * nsplit);
+    sx = (1 - u0) * xv[i] + u0 * xv[i + 1];
+    sy = (1 - w0) * yv[j] + w0 * yv[j + 1];
+}
+
 // map = 0: rays only (the source grid is not known yet; k_map_rays_b maps them afterwards)
 template <typename T, bool KT = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_raytrace_map_b(BatchDev bd, T tol, int map)
@@ -442,11 +495,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
         bd.beta[ro * 2 + 1] = 0;
         return;
     }
-    const int ii = k / bd.nsplit, jj = k % bd.nsplit;
-    const double u0 = ((double)(1 + 2 * ii)) / (2 * bd.nsplit);
-    const double w0 = ((double)(1 + 2 * jj)) / (2 * bd.nsplit);
-    const double sx = (1 - u0) * bd.xvals[i] + u0 * bd.xvals[i + 1];
-    const double sy = (1 - w0) * bd.yvals[j] + w0 * bd.yvals[j + 1];
+    double sx, sy;
+    ray_origin(bd.xvals, bd.yvals, bd.nx, bd.ny, bd.nsplit, i, j, k, sx, sy);
     T a0 = 0, a1 = 0;
     int nev = 0;
     for (int l = 0; l < m.nlens; l++) {
@@ -731,12 +781,10 @@ __global__ void __launch_bounds__(256) k_mock_sb(int nx, int ny, int nsplit, con
     if (p >= (long long)nx * ny) return;
     const int i = (int)(p / ny), j = (int)(p % ny);
     double sbtot = 0;
-    for (int ii = 0; ii < nsplit; ii++)
-        for (int jj = 0; jj < nsplit; jj++) {
-            const double u0 = ((double)(1 + 2 * ii)) / (2 * nsplit);
-            const double w0 = ((double)(1 + 2 * jj)) / (2 * nsplit);
-            const double sx = (1 - u0) * xvals[i] + u0 * xvals[i + 1];
-            const double sy = (1 - w0) * yvals[j] + w0 * yvals[j + 1];
+    const int nr = rays_per_pixel(nsplit);  // nsplit 0: the unsplit branch, pixelgrid.cpp:11896-11925
+    for (int k = 0; k < nr; k++) {
+            double sx, sy;
+            ray_origin(xvals, yvals, nx, ny, nsplit, i, j, k, sx, sy);
             double a0, a1;
             int nev = 0;
             total_deflection<double>(m, sx, sy, tol, a0, a1, nev);
@@ -756,7 +804,7 @@ __global__ void __launch_bounds__(256) k_mock_sb(int nx, int ny, int nsplit, con
                 if (v != 0) sbtot += v;
             }
         }
-    sb[p] = 0 + sbtot / (nsplit * nsplit);
+    sb[p] = 0 + sbtot / nr;
 }
 
 void launch_mock_sb(hipStream_t stream, int nx, int ny, int nsplit, const double* xvals, const double* yvals, const ModelPrep& m,

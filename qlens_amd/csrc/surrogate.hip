@@ -412,11 +412,19 @@ __global__ void __launch_bounds__(512) k_sur_rays(SurDev net, BatchDev bd, float
                 const int p = t / bd.nsub;
                 const int i = p / bd.ny, j = p - i * bd.ny;
                 if (!bd.mask || bd.mask[p]) {
-                    const int ii = k / bd.nsplit, jj = k % bd.nsplit;
-                    const double u0 = ((double)(1 + 2 * ii)) / (2 * bd.nsplit);
-                    const double w0 = ((double)(1 + 2 * jj)) / (2 * bd.nsplit);
-                    x = (1 - u0) * bd.xvals[i] + u0 * bd.xvals[i + 1] - L.xc;
-                    y = (1 - w0) * bd.yvals[j] + w0 * bd.yvals[j + 1] - L.yc;
+                    double sx, sy;
+                    if (bd.nsplit == 0) {  // the pixel-centre ray of the ray shooting kernel
+                        sx = bd.xvals[i] + 0.5 * ((bd.xvals[bd.nx] - bd.xvals[0]) / bd.nx);
+                        sy = bd.yvals[j] + 0.5 * ((bd.yvals[bd.ny] - bd.yvals[0]) / bd.ny);
+                    } else {
+                        const int ii = k / bd.nsplit, jj = k % bd.nsplit;
+                        const double u0 = ((double)(1 + 2 * ii)) / (2 * bd.nsplit);
+                        const double w0 = ((double)(1 + 2 * jj)) / (2 * bd.nsplit);
+                        sx = (1 - u0) * bd.xvals[i] + u0 * bd.xvals[i + 1];
+                        sy = (1 - w0) * bd.yvals[j] + w0 * bd.yvals[j + 1];
+                    }
+                    x = sx - L.xc;
+                    y = sy - L.yc;
                     if (L.rotate) {
                         const double xp = x * L.costh + y * L.sinth;
                         y = -x * L.sinth + y * L.costh;

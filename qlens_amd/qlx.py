@@ -14,7 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libqlx.so")
 
-QLX_LENS_SIE, QLX_LENS_NFW, QLX_LENS_SHEAR = 1, 2, 3
+QLX_LENS_SIE, QLX_LENS_NFW, QLX_LENS_SHEAR, QLX_LENS_SPLE, QLX_LENS_SHEAR_COMP = 1, 2, 3, 4, 5
 QLX_DEF_QUADRATURE, QLX_DEF_SURROGATE = 0, 1
 QLX_PREC_FP64, QLX_PREC_MIXED = 0, 1
 QLX_RAY_FP64, QLX_RAY_FP32 = 0, 1
@@ -321,11 +321,12 @@ class PixelInversion:
         return ll, [dict(zip(names, row)) for row in np.frombuffer(res, dtype=_RESULT_DT)[:n].tolist()]
 
     def get_rays(self, item=-1):
-        """Subpixel source-plane points of an item of the last evaluation: [nx, ny, nsplit^2, 2]."""
+        """Subpixel source-plane points of an item of the last evaluation: [nx, ny, nsplit^2, 2]
+        (one pixel-centre ray per pixel when nsplit is 0)."""
         if item < 0:
             item += getattr(self, "_last_n", 0)
         ns = self.settings["nsplit"]
-        out = np.zeros((self.nx, self.ny, ns * ns, 2))
+        out = np.zeros((self.nx, self.ny, max(ns * ns, 1), 2))
         self._check(self.lib.qlx_get_rays(self.ctx, int(item), _dp(out), out.size), "qlx_get_rays")
         return out
 

@@ -24,7 +24,7 @@ Sizes and parameter values follow the five workloads of the benchmark descriptio
 """
 import numpy as np
 
-SIE, NFW, SHEAR = 1, 2, 3
+SIE, NFW, SHEAR, SPLE, SHEAR_COMP = 1, 2, 3, 4, 5
 FWHM_TO_SIGMA = 1.0 / 2.35482
 
 
@@ -188,6 +188,39 @@ def c_ns():
     )
 
 
+def c_sple():
+    """A power-law slope away from isothermal: SPLE alpha = 1.25 (the Tessore series) with the
+    shear given as components, split_imgpixels off (one pixel-centre ray per pixel), a 64 x 56
+    image of 0.05" pixels and a non-square 26 x 24 source grid."""
+    xv = corners(-1.6, 1.6, 64)
+    yv = corners(-1.4, 1.4, 56)
+    return dict(
+        name="CSPL", xvals=xv, yvals=yv, nsplit=0,
+        lenses=[(SPLE, (1.05, 1.25, 0.75, 30.0, 0.05, -0.03)), (SHEAR_COMP, (0.04, -0.02, 0.0, 0.0))],
+        sources=[(1.0, 0.15, 1.2, 0.7, 10.0, 0.04, 0.02)],
+        psf=gaussian_psf(0.05, 0.01, 0.05, 0.05),
+        sigma=0.01, seed=10,
+        srcgrid=(-0.5, 0.5, -0.5, 0.5), src_n=(26, 24),
+        regparam=100.0, integral_tol=1e-6,
+    )
+
+
+def c_sple_steep():
+    """A steep SPLE (alpha = 1.6, q = 0.55: a long series) beside a spherical shallow one
+    (alpha = 0.7), 3 x 3 subpixels, 48 x 48 image of 0.06" pixels."""
+    xv = corners(-1.44, 1.44, 48)
+    return dict(
+        name="CSPS", xvals=xv, yvals=xv.copy(), nsplit=3,
+        lenses=[(SPLE, (0.9, 1.6, 0.55, -30.0, 0.02, 0.04)), (SPLE, (0.25, 0.7, 1.0, 0.0, 0.6, -0.5)),
+                (SHEAR, (0.03, 50.0, 0.0, 0.0))],
+        sources=[(1.0, 0.12, 1.0, 0.8, -20.0, -0.03, 0.05)],
+        psf=gaussian_psf(0.06, 0.02, 0.06, 0.06),
+        sigma=0.01, seed=10,
+        srcgrid=(-0.5, 0.5, -0.5, 0.5), src_n=(22, 22),
+        regparam=50.0, integral_tol=1e-6,
+    )
+
+
 def c1_geometry_200(src_side):
     """The 200 x 200 batched-evidence geometry (0.05"/pix, 10" field, 2 x 2 supersampling) with
     config 1's SIE + shear lens, Sersic source and 0.1" FWHM Gaussian PSF, on a src_side^2 grid."""
@@ -203,18 +236,19 @@ def c1_geometry_200(src_side):
 
 
 def proposals(case, n, seed=1234):
-    """Seeded 1% Gaussian ball around the truth: SIE (b, q), NFW (ks, rs, q), shear gamma."""
+    """Seeded 1% Gaussian ball around the truth: SIE (b, q), NFW (ks, rs, q), shear gamma, SPLE
+    (b, alpha, q), shear components."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n):
         m = []
         for t, p in case["lenses"]:
             p = list(p)
-            idx = {SIE: (0, 1), NFW: (0, 1, 2), SHEAR: (0,)}[t]
+            idx = {SIE: (0, 1), NFW: (0, 1, 2), SHEAR: (0,), SPLE: (0, 1, 2), SHEAR_COMP: (0, 1)}[t]
             for i in idx:
                 p[i] *= 1 + 0.01 * rng.standard_normal()
-            if t in (SIE, NFW):
-                k = 1 if t == SIE else 2
+            if t in (SIE, NFW, SPLE):
+                k = {SIE: 1, NFW: 2, SPLE: 2}[t]
                 p[k] = min(p[k], 0.999)
             m.append((t, tuple(p)))
         out.append(m)

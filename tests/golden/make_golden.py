@@ -91,6 +91,13 @@ def deflection_fixture(work):
         "offcentre": [(W.NFW, (0.35, 1.5, 0.5, 70.0, 0.2, -0.1)), (W.SIE, (0.7, 0.6, -40.0, -0.3, 0.25, 0.05)),
                       (W.SHEAR, (0.08, 135.0, 0.1, 0.0))],
         "spherical": [(W.NFW, (0.3, 2.0, 1.0, 0.0, 0.0, 0.0)), (W.SIE, (0.9, 1.0, 0.0, 0.1, 0.0, 0.0))],
+        # the shipped demo's lens (delaunay_fit_demo.in): alpha = 1.17, shear as components
+        "sple_demo": [(W.SPLE, (1.3634, 1.17163, 0.963867, 81.9, 0.0102892, 0.00358392)),
+                      (W.SHEAR_COMP, (0.0647257, -0.0575047, 0.0, 0.0))],
+        "sple_steep": [(W.SPLE, (1.1, 1.6, 0.55, -30.0, 0.1, -0.2))],
+        "sple_shallow": [(W.SPLE, (0.8, 0.45, 0.3, 65.0, -0.25, 0.15)), (W.SPLE, (0.5, 0.7, 1.0, 0.0, 0.3, 0.1))],
+        "sple_alpha1": [(W.SPLE, (1.0, 1.0, 0.8, 10.0, 0.05, 0.0))],
+        "sie_cored": [(W.SIE, (1.2, 0.7, 20.0, 0.05, -0.1, 0.15)), (W.SIE, (0.6, 1.0, 0.0, -0.4, 0.3, 0.2))],
     }
     out = {"points": pts}
     for name, lenses in lens_sets.items():
@@ -216,6 +223,8 @@ FIXTURES = {
     "CNS": lambda w: inversion_fixture(w, "CNS", cns_case(), ray_stride=3),
     "C2": lambda w: inversion_fixture(w, "C2", W.c2(), ray_stride=9),
     "C4s": c4s_fixture,
+    "CSPL": lambda w: inversion_fixture(w, "CSPL", W.c_sple(), ray_stride=2),
+    "CSPS": lambda w: inversion_fixture(w, "CSPS", W.c_sple_steep(), ray_stride=3),
     "deflection": deflection_fixture,
     "patterson": patterson_fixture,
     "rng": rng_fixture,

@@ -129,7 +129,8 @@ def cns_case(g):
     return case
 
 
-CASES = {"C1": W.c1, "C2s": W.c2_small, "C2": W.c2, "C4s": lambda: W.c1_geometry_200(20)}
+CASES = {"C1": W.c1, "C2s": W.c2_small, "C2": W.c2, "C4s": lambda: W.c1_geometry_200(20), "CSPL": W.c_sple,
+         "CSPS": W.c_sple_steep}
 
 
 def _case(name, g):
@@ -140,7 +141,7 @@ def vrel(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
 
 
-@pytest.mark.parametrize("name", ["C1", "C2s", "CNS", "C2", "C4s"])
+@pytest.mark.parametrize("name", ["C1", "C2s", "CNS", "C2", "C4s", "CSPL", "CSPS"])
 def test_mock_image_matches_reference(qlx_ctx, golden_dir, name):
     """The GPU mock generator (ray-traced Sersic SB per subpixel, PSF, NR3 noise at seed 10) against
     the reference's own mock of the same workload (the fixture's data)."""
@@ -163,7 +164,7 @@ def test_gaussian_psf_matches_reference(qlx_ctx, golden_dir):
         assert np.array_equal(qlx_ctx.get_psf(), g["psf_%d" % k]), k
 
 
-@pytest.mark.parametrize("name", ["C1", "C2s", "CNS", "C2"])
+@pytest.mark.parametrize("name", ["C1", "C2s", "CNS", "C2", "CSPL", "CSPS"])
 def test_inversion_matches_reference(qlx_ctx, golden_dir, name):
     g = np.load(os.path.join(golden_dir, "inversion_%s.npz" % name))
     case = _case(name, g)
@@ -332,10 +333,11 @@ def _perturbed(case, n, seed):
         m = []
         for t, p in case["lenses"]:
             p = list(p)
-            for i in {1: (0, 1), 2: (0, 1, 2), 3: (0,)}[t]:
+            # SIE (b, q), NFW (ks, rs, q), shear gamma, SPLE (b, alpha, q), shear components
+            for i in {1: (0, 1), 2: (0, 1, 2), 3: (0,), 4: (0, 1, 2), 5: (0, 1)}[t]:
                 p[i] *= 1 + 0.01 * rng.standard_normal()
-            if t in (1, 2):
-                k = 1 if t == 1 else 2
+            if t in (1, 2, 4):
+                k = {1: 1, 2: 2, 4: 2}[t]
                 p[k] = min(p[k], 0.999)
             m.append((t, tuple(p)))
         models.append(m)
@@ -374,7 +376,7 @@ def test_uneven_source_grids_in_one_batch(qlx_ctx, golden_dir):
 
 # Mixed precision (BF16-split F, FP32 Cholesky, FP64 refinement and log-dets): the accuracy
 # contract is |delta ln E| < 0.1 against the FP64 evidence; the refined solve must converge.
-@pytest.mark.parametrize("name", ["C1", "C2s", "CNS", "C2"])
+@pytest.mark.parametrize("name", ["C1", "C2s", "CNS", "C2", "CSPL"])
 def test_mixed_precision_evidence(qlx_ctx, golden_dir, name):
     from qlens_amd import qlx
     g = np.load(os.path.join(golden_dir, "inversion_%s.npz" % name))

@@ -25,10 +25,13 @@ def cns_case(g):
     return case
 
 
-@pytest.mark.parametrize("name", ["C1", "C2s", "CNS"])
+ORACLE_CASES = {"C1": W.c1, "C2s": W.c2_small, "CSPL": W.c_sple, "CSPS": W.c_sple_steep}
+
+
+@pytest.mark.parametrize("name", ["C1", "C2s", "CNS", "CSPL", "CSPS"])
 def test_oracle_inversion_matches_reference(golden_dir, name):
     g = load(golden_dir, "inversion_%s.npz" % name)
-    case = {"C1": W.c1, "C2s": W.c2_small}[name]() if name != "CNS" else cns_case(g)
+    case = ORACLE_CASES[name]() if name != "CNS" else cns_case(g)
     res, arr = O.invert(case, g["data"])
     # same operation order as the reference: equal to the last bit
     assert res["minus2lnE"] == float(g["minus2lnE"])
@@ -45,12 +48,12 @@ def test_oracle_inversion_matches_reference(golden_dir, name):
     assert np.array_equal(rays, g["rays"])
 
 
-@pytest.mark.parametrize("name", ["C1", "C2s", "CNS", "C4s"])
+@pytest.mark.parametrize("name", ["C1", "C2s", "CNS", "C4s", "CSPL", "CSPS"])
 def test_oracle_mock_equals_reference_mock(golden_dir, name):
     """Every fixture's data is the reference's own mock (ref_harness mock: lensed Sersic sources,
     PSF, noise from its generator seeded with 10); the oracle's mock equals it bit for bit."""
     g = load(golden_dir, "inversion_%s.npz" % name)
-    case = {"C1": W.c1, "C2s": W.c2_small, "C4s": lambda: W.c1_geometry_200(20)}[name]() if name != "CNS" else cns_case(g)
+    case = dict(ORACLE_CASES, C4s=lambda: W.c1_geometry_200(20))[name]() if name != "CNS" else cns_case(g)
     assert np.array_equal(O.mock_image(case), g["data"])
 
 
