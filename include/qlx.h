@@ -31,6 +31,7 @@ typedef enum {
     QLX_ERR_NOSOURCE = -6, /* no source pixel mapped (reference sentinel 2e30, pixelgrid.cpp:12807) */
     QLX_ERR_RANGE = -7,    /* size above a compiled-in maximum */
     QLX_ERR_REFINE = -8,   /* MIXED item: the FP64 refinement stopped above refine_tol (value kept) */
+    QLX_ERR_IO = -9,       /* a file could not be read or is not in the expected format */
     QLX_ITEMS_FAILED = 1   /* return of qlx_eval*: the call completed, some items carry a non-ok status */
 } qlx_status;
 
@@ -213,6 +214,38 @@ int qlx_get_rays(void *ctx, int item, double *beta, size_t cap);
 /* The PSF matrix in use (x-major [pnx*pny], as passed to qlx_set_psf or built by
  * qlx_set_psf_gaussian); psf may be NULL to query the size only. */
 int qlx_get_psf(void *ctx, int *pnx, int *pny, double *psf, size_t cap);
+
+/* ---- on-disk inputs (host only, no context or device needed) -------------------------------
+ * Two-call pattern: with NULL arrays the sizes are returned.  Errors: QLX_ERR_IO (unreadable or
+ * malformed file), QLX_ERR_ARG, QLX_ERR_RANGE; the message is qlx_io_last_error(). */
+const char *qlx_io_last_error(void);
+/* Text image, replaces ImagePixelData::load_data (pixelgrid.cpp:5797-5904): root.x and root.y
+ * hold the nx+1 / ny+1 pixel corners, root.dat the nx*ny surface brightness values in x-major
+ * order (i outer, j inner), all whitespace-separated.  The mask is every pixel. */
+int qlx_read_image_text(const char *root, int *nx, int *ny, double *xvals, double *yvals,
+                        double *sb, size_t sb_cap);
+/* FITS primary image as cfitsio's fits_read_pix into doubles reads it for load_data_fits,
+ * load_mask_fits and load_psf_fits (pixelgrid.cpp:6016-6280, 6512-6651, 6737-6876): BITPIX 8,
+ * 16, 32, 64, -32 or -64, NAXIS 1 or 2 (further axes: the first plane), BSCALE / BZERO applied;
+ * pix[i*naxis2 + j] = pixel (i+1, j+1), axis 1 along x.  pxsize / pxnoise (may be NULL) receive
+ * the reference's PXSIZE / PXNOISE header keywords, NaN when absent (pixelgrid.cpp:6094-6110). */
+int qlx_read_fits(const char *path, int *naxis1, int *naxis2, double *pix, size_t cap,
+                  double *pxsize, double *pxnoise);
+/* Pixel corners of an nx x ny image with square pixels of pixel_size centred on the origin, as
+ * load_data_fits builds them with a pixel size (pixelgrid.cpp:6198-6213): x from -nx s/2 in
+ * accumulated steps of s.  xvals[nx+1], yvals[ny+1]. */
+int qlx_pixel_grid(int nx, int ny, double pixel_size, double *xvals, double *yvals);
+/* The PSF matrix QLens::load_psf_fits keeps from an input image img[nx*ny] (x-major)
+ * (pixelgrid.cpp:6790-6840): the box of pixels above threshold x peak (psf_threshold), a window
+ * of 2 half + 1 pixels per axis centred on (nx/2, ny/2), normalised to unit sum.  QLX_ERR_RANGE
+ * when that window would reach past the input. */
+int qlx_psf_from_image(const double *img, int nx, int ny, double threshold, int *pnx, int *pny,
+                       double *psf, size_t cap);
+/* Applies a mask image m[mnx*mny] (x-major; nonzero = in the mask) to mask[nx*ny] as
+ * load_mask_fits does for the primary mask (pixelgrid.cpp:6532-6620): a smaller mask is centred
+ * with offsets ((nx-mnx)/2, (ny-mny)/2) and leaves the pixels it does not cover as they were;
+ * a larger one is refused (QLX_ERR_ARG). */
+int qlx_mask_from_image(const double *m, int mnx, int mny, int nx, int ny, unsigned char *mask);
 
 /* ---- fit model and likelihood (sampler interface) --------------------------------------------
  * The reference's fit parameters (vary flags on lens parameters with penalty limits, transforms and

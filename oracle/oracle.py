@@ -228,3 +228,63 @@ def auto_source_grid(case, rays, data, auto_box=True, auto_npixels=True, pixel_f
     else:
         size = tuple(case["src_n"])
     return box, size, count
+
+
+# ---- on-disk formats (restated from the FITS standard and the reference's loaders) ----------
+def read_fits_np(path):
+    """Primary-HDU FITS image by the standard (NOST 100-2.0): 80-character cards in 2880-byte
+    blocks up to END, then big-endian pixels with NAXIS1 fastest, value = BZERO + BSCALE raw (what
+    cfitsio's fits_read_pix returns as doubles).  Returns (pix [naxis1, naxis2], header dict)."""
+    raw = open(path, "rb").read()
+    hdr, pos = {}, 0
+    while True:
+        card = raw[pos:pos + 80].decode("ascii")
+        pos += 80
+        key = card[:8].strip()
+        if key == "END":
+            break
+        if card[8] == "=":
+            hdr[key] = card[10:].split("/")[0].strip()
+    data_off = -(-pos // 2880) * 2880
+    bitpix, naxis = int(hdr["BITPIX"]), int(hdr["NAXIS"])
+    n1 = int(hdr["NAXIS1"])
+    n2 = int(hdr["NAXIS2"]) if naxis >= 2 else 1
+    dt = {8: ">u1", 16: ">i2", 32: ">i4", 64: ">i8", -32: ">f4", -64: ">f8"}[bitpix]
+    a = np.frombuffer(raw, dtype=dt, count=n1 * n2, offset=data_off).astype(np.float64)
+    bscale, bzero = float(hdr.get("BSCALE", 1.0)), float(hdr.get("BZERO", 0.0))
+    if bscale != 1.0 or bzero != 0.0:
+        a = a * bscale + bzero
+    return a.reshape(n2, n1).T.copy(), hdr
+
+
+def psf_crop(img, threshold):
+    """QLens::load_psf_fits after the read (pixelgrid.cpp:6790-6840), restated: the index box of
+    pixels above threshold x peak, widened to contain (nx/2, ny/2); half widths (box + 1)/2 (integer
+    division); the window of 2 half + 1 pixels centred on (nx/2, ny/2); divided by its sum."""
+    nx, ny = img.shape
+    peak = img.max()
+    ii, jj = np.nonzero(img > threshold * peak)
+    imid, jmid = nx // 2, ny // 2
+    imin, imax = min(imid, ii.min()), max(imid, ii.max())
+    jmin, jmax = min(jmid, jj.min()), max(jmid, jj.max())
+    hx, hy = (imax - imin + 1) // 2, (jmax - jmin + 1) // 2
+    w = img[imid - hx:imid + hx + 1, jmid - hy:jmid + hy + 1].copy()
+    norm = 0.0
+    for i in range(w.shape[0]):
+        for j in range(w.shape[1]):
+            norm += w[i, j]
+    return w / norm
+
+
+def centred_grid(nx, ny, pixel_size):
+    """load_data_fits pixel corners with a pixel size (pixelgrid.cpp:6198-6213): from -n s/2 in
+    accumulated steps of s."""
+    out = []
+    for n in (nx, ny):
+        v = -(0.5 * n * pixel_size)
+        c = []
+        for _ in range(n + 1):
+            c.append(v)
+            v += pixel_size
+        out.append(np.array(c))
+    return out[0], out[1]

@@ -77,6 +77,8 @@ def write_inputs(case, data, workdir, lenses=None, psf_mode="input"):
         "auto_src_npixels %s" % ("on" if case.get("auto_src_npixels") else "off"),
         "regparam %s" % _fmt(case["regparam"]),
         "optimize_regparam %s" % ("on" if case.get("optimize_regparam") else "off"),
+        "regparam_minlog %s" % _fmt(case.get("regopt_bounds", (-3.0, 5.0))[0]),
+        "regparam_maxlog %s" % _fmt(case.get("regopt_bounds", (-3.0, 5.0))[1]),
         "bg_pixel_noise %s" % _fmt(case["sigma"]),
     ]
     if psf_mode == "gaussian":

@@ -101,6 +101,16 @@ def load_library(path=None):
     lib.qlx_get_psf.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_size_t]
     lib.qlx_get_source.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double),
                                    C.POINTER(C.c_int), C.c_size_t, C.POINTER(C.c_double), C.c_size_t]
+    lib.qlx_io_last_error.restype = C.c_char_p
+    lib.qlx_io_last_error.argtypes = []
+    pint, pdbl = C.POINTER(C.c_int), C.POINTER(C.c_double)
+    for name, args in (("qlx_read_image_text", [C.c_char_p, pint, pint, pdbl, pdbl, pdbl, C.c_size_t]),
+                       ("qlx_read_fits", [C.c_char_p, pint, pint, pdbl, C.c_size_t, pdbl, pdbl]),
+                       ("qlx_pixel_grid", [C.c_int, C.c_int, C.c_double, pdbl, pdbl]),
+                       ("qlx_psf_from_image", [pdbl, C.c_int, C.c_int, C.c_double, pint, pint, pdbl, C.c_size_t]),
+                       ("qlx_mask_from_image", [pdbl, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_ubyte)])):
+        getattr(lib, name).restype = C.c_int
+        getattr(lib, name).argtypes = args
     if path is None:
         _LIB = lib
     return lib
@@ -164,6 +174,67 @@ class QlxError(RuntimeError):
     pass
 
 
+# -- on-disk inputs (host only: no context or device) ------------------------------------------
+def _io(rc, what):
+    if rc != 0:
+        raise QlxError("%s failed (%s): %s" % (what, STATUS.get(rc, rc), load_library().qlx_io_last_error().decode()))
+
+
+def read_image_text(root):
+    """ImagePixelData::load_data (pixelgrid.cpp:5797-5904): root.x / root.y corners and root.dat
+    surface brightness -> (xvals, yvals, sb [nx, ny])."""
+    lib = load_library()
+    nx, ny = C.c_int(0), C.c_int(0)
+    _io(lib.qlx_read_image_text(root.encode(), C.byref(nx), C.byref(ny), None, None, None, 0), "qlx_read_image_text")
+    xv, yv, sb = np.zeros(nx.value + 1), np.zeros(ny.value + 1), np.zeros((nx.value, ny.value))
+    _io(lib.qlx_read_image_text(root.encode(), C.byref(nx), C.byref(ny), _dp(xv), _dp(yv), _dp(sb), sb.size),
+        "qlx_read_image_text")
+    return xv, yv, sb
+
+
+def read_fits(path):
+    """Primary FITS image as the reference reads it through cfitsio -> (pix [naxis1, naxis2],
+    PXSIZE or None, PXNOISE or None)."""
+    lib = load_library()
+    n1, n2 = C.c_int(0), C.c_int(0)
+    ps, pn = C.c_double(0), C.c_double(0)
+    _io(lib.qlx_read_fits(path.encode(), C.byref(n1), C.byref(n2), None, 0, C.byref(ps), C.byref(pn)), "qlx_read_fits")
+    pix = np.zeros((n1.value, n2.value))
+    _io(lib.qlx_read_fits(path.encode(), C.byref(n1), C.byref(n2), _dp(pix), pix.size, None, None), "qlx_read_fits")
+    return pix, (ps.value if np.isfinite(ps.value) else None), (pn.value if np.isfinite(pn.value) else None)
+
+
+def pixel_grid(nx, ny, pixel_size):
+    """Corners of an nx x ny image of square pixels centred on the origin (load_data_fits with a
+    pixel size, pixelgrid.cpp:6198-6213)."""
+    xv, yv = np.zeros(nx + 1), np.zeros(ny + 1)
+    _io(load_library().qlx_pixel_grid(int(nx), int(ny), C.c_double(pixel_size), _dp(xv), _dp(yv)), "qlx_pixel_grid")
+    return xv, yv
+
+
+def psf_from_image(img, threshold=0.01):
+    """The PSF matrix load_psf_fits keeps from an input image (pixelgrid.cpp:6790-6840)."""
+    lib = load_library()
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    pnx, pny = C.c_int(0), C.c_int(0)
+    _io(lib.qlx_psf_from_image(_dp(img), img.shape[0], img.shape[1], C.c_double(threshold), C.byref(pnx), C.byref(pny),
+                               None, 0), "qlx_psf_from_image")
+    psf = np.zeros((pnx.value, pny.value))
+    _io(lib.qlx_psf_from_image(_dp(img), img.shape[0], img.shape[1], C.c_double(threshold), C.byref(pnx), C.byref(pny),
+                               _dp(psf), psf.size), "qlx_psf_from_image")
+    return psf
+
+
+def mask_from_image(m, nx, ny, mask=None):
+    """Apply a mask image (nonzero = in) to an nx x ny mask as load_mask_fits does
+    (pixelgrid.cpp:6532-6620); mask defaults to every pixel in."""
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    out = np.ones((nx, ny), dtype=np.uint8) if mask is None else np.array(mask, dtype=np.uint8, copy=True).reshape(nx, ny)
+    _io(load_library().qlx_mask_from_image(_dp(m), m.shape[0], m.shape[1], int(nx), int(ny),
+                                           out.ctypes.data_as(C.POINTER(C.c_ubyte))), "qlx_mask_from_image")
+    return out
+
+
 class PixelInversion:
     """One qlx context (one device stream) holding the image, PSF and inversion settings."""
 
@@ -213,6 +284,47 @@ class PixelInversion:
         self.nx, self.ny = nx, ny
         self._keep = (xv, yv, d, ci, mk)
 
+    def load_image_text(self, root, sigma):
+        """'sbmap loadimg <root>' on a text image with uniform pixel noise sigma."""
+        xv, yv, sb = read_image_text(root)
+        self.load_image(xv, yv, sb, sigma=sigma)
+
+    def load_image_fits(self, path, pixel_size=None, bounds=None, sigma=None):
+        """'sbmap loadimg <file.fits>' (load_data_fits, pixelgrid.cpp:6016-6280): the pixel size
+        from the file's PXSIZE keyword, else `pixel_size` (data_pixel_size), gives a grid centred
+        on the origin; otherwise `bounds` (xmin, xmax, ymin, ymax) are split in accumulated steps.
+        The noise is `sigma`, else the file's PXNOISE.  The mask is every pixel."""
+        pix, pxs, pxn = read_fits(path)
+        nx, ny = pix.shape
+        ps = pxs if pxs is not None else pixel_size
+        if ps is not None:
+            xv, yv = pixel_grid(nx, ny, ps)
+        elif bounds is not None:
+            xv, yv = np.zeros(nx + 1), np.zeros(ny + 1)
+            for arr, lo, hi, n in ((xv, bounds[0], bounds[1], nx), (yv, bounds[2], bounds[3], ny)):
+                step, v = (hi - lo) / n, lo
+                for i in range(n + 1):
+                    arr[i] = v
+                    v += step
+        else:
+            raise QlxError("load_image_fits: no PXSIZE in %s; give pixel_size or bounds" % path)
+        sig = sigma if sigma is not None else pxn
+        if sig is None:
+            raise QlxError("load_image_fits: no PXNOISE in %s; give sigma" % path)
+        self.load_image(xv, yv, pix, sigma=sig)
+
+    def load_mask_fits(self, path):
+        """'sbmap loadmask <file.fits>' into the primary mask (load_mask_fits)."""
+        m, _, _ = read_fits(path)
+        xv, yv, d, ci, mk = self._keep
+        mask = mask_from_image(m, self.nx, self.ny, mk)
+        self.load_image(xv, yv, d, covinv=ci, mask=mask)
+
+    def load_psf_fits(self, path, threshold=0.01):
+        """'sbmap loadpsf <file.fits>' with psf_threshold (load_psf_fits)."""
+        m, _, _ = read_fits(path)
+        self.set_psf(psf_from_image(m, threshold))
+
     def set_psf(self, psf):
         psf = np.ascontiguousarray(psf, dtype=np.float64)
         self._check(self.lib.qlx_set_psf(self.ctx, psf.shape[0], psf.shape[1], _dp(psf)), "qlx_set_psf")
@@ -244,6 +356,7 @@ class PixelInversion:
         self.set_settings(case["nsplit"], case["srcgrid"], case["src_n"], case["regparam"], case["integral_tol"],
                           precision=precision, ray_precision=ray_precision,
                           optimize_regparam=optimize_regparam or bool(case.get("optimize_regparam")),
+                          regopt_bounds=case.get("regopt_bounds", (-3.0, 5.0)),
                           auto_srcgrid=bool(case.get("auto_srcgrid")), auto_src_npixels=bool(case.get("auto_src_npixels")))
 
     # -- evaluation --------------------------------------------------------------------------

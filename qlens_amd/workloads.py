@@ -221,6 +221,30 @@ def c_sple_steep():
     )
 
 
+DEMO_FILES = ("sim_hst_image.fits", "mask.fits", "hst_psf.fits")
+DEMO_TRUE = [(SPLE, (1.3634, 1.17163, 0.963867, 81.9, 0.0102892, 0.00358392)),
+             (SHEAR_COMP, (0.0647257, -0.0575047, 0.0, 0.0))]
+DEMO_START = [(SPLE, (1.35, 1.1, 0.85, 90.0, 0.0, 0.0)), (SHEAR_COMP, (0.05, 0.0, 0.0, 0.0))]
+
+
+def demo():
+    """The reference's shipped pixel-fit demo (delaunay_fit_demo.in / pixel_fit_demo.in) on a
+    Cartesian source grid: the 81 x 81 HST-like image sim_hst_image.fits at data_pixel_size
+    0.049", noise 0.00941285, the PSF hst_psf.fits cut at psf_threshold 1e-2, the mask mask.fits,
+    split_imgpixels off, curvature regularization with optimize_regparam on over log10 lambda in
+    [-5, 5], the source grid placed and sized automatically (auto_srcgrid, auto_src_npixels,
+    pixel_fraction 0.3), and an SPLE (alpha 1.17) + shear-components lens.  The image, mask and
+    PSF come from the files (`files`); DEMO_TRUE is the model that made the image, DEMO_START the
+    demo's starting guess."""
+    return dict(
+        name="DEMO", files=DEMO_FILES, pixel_size=0.049, nsplit=0,
+        lenses=DEMO_TRUE, sigma=0.00941285, psf_threshold=0.01,
+        srcgrid=(-1.0, 1.0, -1.0, 1.0), src_n=(40, 40), auto_srcgrid=True, auto_src_npixels=True,
+        optimize_regparam=True, regopt_bounds=(-5.0, 5.0),
+        regparam=20.0, integral_tol=1e-6,
+    )
+
+
 def c1_geometry_200(src_side):
     """The 200 x 200 batched-evidence geometry (0.05"/pix, 10" field, 2 x 2 supersampling) with
     config 1's SIE + shear lens, Sersic source and 0.1" FWHM Gaussian PSF, on a src_side^2 grid."""

@@ -22,6 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.abspath(os.path.join(HERE, "..", ".."))
 sys.path.insert(0, REPO)
 
+from oracle import oracle as O  # noqa: E402
 from oracle import refrun  # noqa: E402
 from qlens_amd import workloads as W  # noqa: E402
 
@@ -217,6 +218,34 @@ def autogrid_fixture(work):
     np.savez_compressed(os.path.join(HERE, "autogrid.npz"), **out)
 
 
+def demo_fixture(work):
+    """The shipped demo (workloads.demo) evaluated by the reference at the true lens and at the
+    demo's starting guess.  The FITS inputs are read with the oracle's reader (the reference's
+    cfitsio is absent here) and handed to the reference as its text image, mask and PSF."""
+    case = W.demo()
+    fits = os.path.join(HERE, "fits")
+    img, _ = O.read_fits_np(os.path.join(fits, case["files"][0]))
+    m, _ = O.read_fits_np(os.path.join(fits, case["files"][1]))
+    p, _ = O.read_fits_np(os.path.join(fits, case["files"][2]))
+    xv, yv = O.centred_grid(img.shape[0], img.shape[1], case["pixel_size"])
+    case.update(xvals=xv, yvals=yv, psf=O.psf_crop(p, case["psf_threshold"]), mask=(m != 0).astype(np.uint8))
+    out = {}
+    for tag, lenses in (("true", W.DEMO_TRUE), ("start", W.DEMO_START)):
+        sc, arr = refrun.run_invert(case, img, _sub(work, "demo_" + tag), lenses=lenses)
+        for k in ("minus2lnE", "chisq0", "logdetF", "logdetR", "regparam"):
+            out["%s_%s" % (k, tag)] = sc[k]
+        out["n_img_%s" % tag] = int(sc["n_img_active"])
+        out["n_src_%s" % tag] = int(sc["n_src_active"])
+        out["size_%s" % tag] = np.array([int(sc["src_nx"]), int(sc["src_ny"])])
+        out["box_%s" % tag] = np.array([sc["src_xmin"], sc["src_xmax"], sc["src_ymin"], sc["src_ymax"]])
+        out["src_%s" % tag] = arr["src"]
+        out["active_%s" % tag] = arr["active"]
+        out["model_%s" % tag] = arr["model"]
+        print("demo", tag, "grid %dx%d" % (sc["src_nx"], sc["src_ny"]), "regparam", repr(sc["regparam"]),
+              "-2lnE", repr(sc["minus2lnE"]), "chisq0", sc["chisq0"], "n_img", sc["n_img_active"], flush=True)
+    np.savez_compressed(os.path.join(HERE, "demo.npz"), **out)
+
+
 FIXTURES = {
     "C1": lambda w: inversion_fixture(w, "C1", W.c1()),
     "C2s": lambda w: inversion_fixture(w, "C2s", W.c2_small()),
@@ -231,6 +260,7 @@ FIXTURES = {
     "psf": psf_fixture,
     "regopt": regopt_fixture,
     "autogrid": autogrid_fixture,
+    "demo": demo_fixture,
 }
 
 if __name__ == "__main__":

@@ -116,3 +116,42 @@ def test_batched_sampler_equals_one_proposal_loop(qlx_ctx, golden_dir):
     print("one-proposal loop: %d calls; batched: %d calls" % (c1, c32))
     assert np.array_equal(u1, u32) and np.array_equal(l1, l32) and h1 == h32
     assert c32 < c1
+
+
+@pytest.mark.parametrize("tag", ["true", "start"])
+def test_shipped_demo_end_to_end(qlx_ctx, golden_dir, tag):
+    """The reference's shipped pixel-fit demo (workloads.demo) read from its FITS files by the
+    library (image at data_pixel_size 0.049", mask, PSF cut at 1e-2), split_imgpixels off, the
+    automatic source grid, optimize_regparam over [-5, 5] and the SPLE (alpha 1.17) + shear
+    components lens, against the reference (fed the same files through its text-image route).
+    At the true lens chi^2 is close to the number of fitted pixels (the reference's own mock noise
+    level): the FITS axes are read the right way round."""
+    g = np.load(os.path.join(golden_dir, "demo.npz"))
+    case = W.demo()
+    fits = os.path.join(golden_dir, "fits")
+    print("demo", tag, flush=True)
+    qlx_ctx.load_image_fits(os.path.join(fits, case["files"][0]), pixel_size=case["pixel_size"], sigma=case["sigma"])
+    qlx_ctx.load_mask_fits(os.path.join(fits, case["files"][1]))
+    qlx_ctx.load_psf_fits(os.path.join(fits, case["files"][2]), case["psf_threshold"])
+    qlx_ctx.set_settings(case["nsplit"], case["srcgrid"], case["src_n"], case["regparam"], case["integral_tol"],
+                         optimize_regparam=True, regopt_bounds=case["regopt_bounds"], auto_srcgrid=True,
+                         auto_src_npixels=True)
+    lenses = W.DEMO_TRUE if tag == "true" else W.DEMO_START
+    r = qlx_ctx.invert_image_surface_brightness_map(lenses)[0]
+    print("demo %s: grid %dx%d, regparam %.10g (reference %.10g), -2lnE %.10g (reference %.10g), chisq %.6g over %d pixels"
+          % (tag, r["src_nx"], r["src_ny"], r["regparam"], float(g["regparam_" + tag]), r["minus2lnE"],
+             float(g["minus2lnE_" + tag]), r["chisq0"], r["n_img_active"]))
+    assert r["status"] == 0
+    assert [r["src_nx"], r["src_ny"]] == list(g["size_" + tag])
+    box = [r["src_xmin"], r["src_xmax"], r["src_ymin"], r["src_ymax"]]
+    assert np.allclose(box, g["box_" + tag], rtol=1e-12, atol=0)
+    assert r["n_img_active"] == int(g["n_img_" + tag]) and r["n_src_active"] == int(g["n_src_" + tag])
+    assert rel(r["regparam"], float(g["regparam_" + tag])) < 1e-8
+    assert rel(r["minus2lnE"], float(g["minus2lnE_" + tag])) < 1e-10
+    assert rel(r["logdetF"], float(g["logdetF_" + tag])) < 1e-9
+    assert rel(r["logdetR"], float(g["logdetR_" + tag])) < 1e-10
+    sol = qlx_ctx.get_source(0)
+    assert np.linalg.norm(sol["src"] - g["src_" + tag]) / np.linalg.norm(g["src_" + tag]) < 1e-8
+    assert np.array_equal(sol["active_index"], g["active_" + tag])
+    if tag == "true":
+        assert 0.8 < r["chisq0"] / r["n_img_active"] < 1.5

@@ -300,3 +300,28 @@ def test_oracle_auto_source_grid_closed_form():
     assert count == nx * ny
     nx0 = int(np.floor(np.sqrt(0.3 * nx * ny * 2.0)))
     assert size == (2 * nx0, 2 * int(np.floor(nx0 * 0.5)))
+
+
+@pytest.mark.parametrize("tag", ["true", "start"])
+def test_oracle_demo_at_the_reference_grid_and_lambda(golden_dir, tag):
+    """The shipped demo (FITS inputs, SPLE alpha 1.17 + shear components, split_imgpixels off) in
+    the oracle, on the source grid and at the lambda the reference's automatic grid and regparam
+    optimisation chose: equal to the reference's evaluation, and the automatic grid recomputed
+    from the oracle's rays equals the reference's."""
+    g = load(golden_dir, "demo.npz")
+    case = W.demo()
+    fits = os.path.join(golden_dir, "fits")
+    img, _ = O.read_fits_np(os.path.join(fits, case["files"][0]))
+    m, _ = O.read_fits_np(os.path.join(fits, case["files"][1]))
+    p, _ = O.read_fits_np(os.path.join(fits, case["files"][2]))
+    xv, yv = O.centred_grid(81, 81, case["pixel_size"])
+    lenses = W.DEMO_TRUE if tag == "true" else W.DEMO_START
+    case.update(xvals=xv, yvals=yv, psf=O.psf_crop(p, case["psf_threshold"]), mask=(m != 0).astype(np.uint8),
+                lenses=lenses, srcgrid=tuple(g["box_" + tag]), src_n=tuple(int(v) for v in g["size_" + tag]),
+                regparam=float(g["regparam_" + tag]))
+    res, arr = O.invert(case, img)
+    assert res["n_src"] == int(g["n_src_" + tag]) and res["n_img"] == int(g["n_img_" + tag])
+    assert abs(res["minus2lnE"] - float(g["minus2lnE_" + tag])) <= 1e-12 * abs(float(g["minus2lnE_" + tag]))
+    assert np.array_equal(arr["active"], g["active_" + tag])
+    box, size, _ = O.auto_source_grid(case, arr["rays"], img)
+    assert list(size) == list(g["size_" + tag]) and box == list(g["box_" + tag])
