@@ -38,6 +38,9 @@ sys.path.insert(0, REPO)
 # MI355X published peaks (dense)
 PEAK_F16_TFLOPS = 2500.0
 PEAK_FP64_TFLOPS = 78.6
+# flops of one integrand evaluation in the reference's arithmetic (j_integrand_prime +
+# NFW::kappa_rsq, profile.cpp:2778-2783, models.cpp:978-984; divisions and transcendentals count 1)
+C_INT = 23
 PEAK_HBM_GBS = 8000.0
 METRIC = "log-evidence evals/s (200² img, 60² src): surrogate vs quad, |ΔlogE|<0.1"  # the benchmark's own label
 
@@ -282,8 +285,17 @@ def main():
     # (bytes counted from the support masks) and reads only the CSR L rows (under 3 % of that).
     # SYRK: block-sparse MFMA work, 2 * 64 * 128^2 flops per pair of nonzero 64 x 128 pool blocks of
     # one row chunk (the dense-equivalent count would exceed the peak and is not a rate).
-    lb_s, syrk_s = mean_s(kt["lbkernel"]), mean_s(kt["syrk"])
-    roof, roof_syrk = None, None
+    lb_s, syrk_s, ray_s = mean_s(kt["lbkernel"]), mean_s(kt["syrk"]), mean_s(kt["raytrace"])
+    roof, roof_syrk, roof_ray = None, None, None
+    # ray shooting: VALU-bound; algorithmic flops = counted integrand evaluations x C_INT, the
+    # reference's arithmetic per evaluation of j_integrand_prime + NFW::kappa_rsq
+    if ray_s and st["integrand_evals"] > 0:
+        achieved = st["integrand_evals"] * C_INT / ray_s / 1e12
+        roof_ray = {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                    "kernel": "k_raytrace_map_b (ray shooting with the NFW Gauss-Patterson quadrature, FP64)",
+                    "items_per_launch": st["n_items"], "integrand_evals_per_launch": st["integrand_evals"],
+                    "c_int_flops_per_eval": C_INT, "avg_launch_ms": round(ray_s * 1e3, 3)}
     if lb_s:
         ach = st["lb_bytes"] / lb_s / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
@@ -328,6 +340,7 @@ def main():
         "batched_uneven_grids": batched,
         "roofline": roof,
         "roofline_syrk": roof_syrk,
+        "roofline_raytrace": roof_ray,
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
