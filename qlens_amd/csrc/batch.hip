@@ -26,6 +26,7 @@ namespace qlx {
 typedef double double4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 #define TS 64
 #define LDSP 66
 
@@ -725,6 +726,80 @@ __device__ __forceinline__ int wave_max_i(int v)
     return v;
 }
 
+// One support block cb of a chunk by columns (the kernel's per-block body, also the mixed path's
+// fallback): wave w takes the columns (cb << cbs) + w + 4 l; lane = chunk row.  Writes the block's
+// image into `tile` (no barrier inside).
+template <typename T>
+__device__ __forceinline__ void lb_block_scalar(const BatchDev& bd, const ItemMeta& M, const T* psf_l, char* tile, int cb, int w,
+                                                int lane, int p, int kr2, int lr2, int i_lo, unsigned i_span, int j_lo,
+                                                unsigned j_span, double sc, double dv, const unsigned* cpix, const double* cval,
+                                                const int* cptr)
+{
+    constexpr bool F64 = sizeof(T) == 8;
+    const int cbs = M.cbs, CB = 1 << cbs;
+    const int pnx = bd.pnx, pny = bd.pny;
+    const int ncol = CB / 4;
+    const int cl = (cb << cbs) + w + 4 * lane;
+    int lb0 = 0, lb1 = 0;
+    if (lane < ncol && cl < M.n_src) {
+        lb0 = cptr[cl];
+        lb1 = cptr[cl + 1];
+    }
+    int ce0 = __builtin_amdgcn_readlane(lb0, 0), ce1 = __builtin_amdgcn_readlane(lb1, 0);
+    unsigned pk = 0;
+    double vd = 0.0;
+    if (ce0 + lane < ce1) {
+        pk = cpix[ce0 + lane];
+        vd = cval[ce0 + lane];
+    }
+    for (int l = 0; l < ncol; l++) {
+        const int ii = w + 4 * l;
+        const int e0c = ce0, e1c = ce1;
+        unsigned pkc = pk;
+        double vc = vd;
+        pk = 0;
+        vd = 0.0;
+        if (l + 1 < ncol) {
+            ce0 = __builtin_amdgcn_readlane(lb0, l + 1);
+            ce1 = __builtin_amdgcn_readlane(lb1, l + 1);
+            if (ce0 + lane < ce1) {
+                pk = cpix[ce0 + lane];
+                vd = cval[ce0 + lane];
+            }
+        }
+        T acc = (T)0;
+        for (int e0 = e0c; e0 < e1c; e0 += 64) {
+            if (e0 != e0c) {
+                pkc = 0;
+                vc = 0.0;
+                if (e0 + lane < e1c) {
+                    pkc = cpix[e0 + lane];
+                    vc = cval[e0 + lane];
+                }
+            }
+            const bool rel = e0 + lane < e1c && (unsigned)((int)(pkc >> 16) - i_lo) <= i_span &&
+                             (unsigned)((int)(pkc & 0xffffu) - j_lo) <= j_span;
+            const T v = (T)vc;
+            unsigned long long bal = __ballot(rel);
+            while (bal) {
+                const int b = __builtin_ctzll(bal);
+                bal &= bal - 1;
+                const unsigned pkb = (unsigned)__builtin_amdgcn_readlane((int)pkc, b);
+                const T vb = readlane_t<T>(v, b);
+                const int tp = kr2 - (int)(pkb >> 16), tq = lr2 - (int)(pkb & 0xffffu);
+                if (p >= 0 && (unsigned)tp < (unsigned)pnx && (unsigned)tq < (unsigned)pny) acc += psf_l[tp * pny + tq] * vb;
+            }
+        }
+        const int c_ = (cb << cbs) + ii;
+        if constexpr (F64) {
+            ((double*)tile)[lane * TS + ii] = c_ == M.n_src ? dv : (double)acc;
+        } else {
+            const double x = c_ == M.n_src ? dv : (double)acc * sc;
+            ((uint16_t*)tile)[((lane >> 2) * CB + ii) * 4 + (lane & 3)] = f32_to_f16_rne((float)x);
+        }
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int psf_bytes)
 {
@@ -773,69 +848,7 @@ __global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int psf_bytes)
         while (bits) {
             const int cb = wd * 32 + __builtin_ctz(bits);
             bits &= bits - 1;
-            // this wave's columns of the block are c_l = (cb << cbs) + w + 4 l, l < CB / 4: lane l
-            // loads their entry bounds once; column l + 1's first 64 entries (pixel, value) are
-            // loaded while column l is worked, longer columns load the rest in place
-            const int ncol = CB / 4;
-            const int cl = (cb << cbs) + w + 4 * lane;
-            int lb0 = 0, lb1 = 0;
-            if (lane < ncol && cl < M.n_src) {
-                lb0 = cptr[cl];
-                lb1 = cptr[cl + 1];
-            }
-            int ce0 = __builtin_amdgcn_readlane(lb0, 0), ce1 = __builtin_amdgcn_readlane(lb1, 0);
-            unsigned pk = 0;
-            double vd = 0.0;
-            if (ce0 + lane < ce1) {
-                pk = cpix[ce0 + lane];
-                vd = cval[ce0 + lane];
-            }
-            for (int l = 0; l < ncol; l++) {
-                const int ii = w + 4 * l;
-                const int e0c = ce0, e1c = ce1;
-                unsigned pkc = pk;
-                double vc = vd;
-                pk = 0;
-                vd = 0.0;
-                if (l + 1 < ncol) {
-                    ce0 = __builtin_amdgcn_readlane(lb0, l + 1);
-                    ce1 = __builtin_amdgcn_readlane(lb1, l + 1);
-                    if (ce0 + lane < ce1) {
-                        pk = cpix[ce0 + lane];
-                        vd = cval[ce0 + lane];
-                    }
-                }
-                T acc = (T)0;
-                for (int e0 = e0c; e0 < e1c; e0 += 64) {
-                    if (e0 != e0c) {
-                        pkc = 0;
-                        vc = 0.0;
-                        if (e0 + lane < e1c) {
-                            pkc = cpix[e0 + lane];
-                            vc = cval[e0 + lane];
-                        }
-                    }
-                    const bool rel = e0 + lane < e1c && (unsigned)((int)(pkc >> 16) - i_lo) <= i_span &&
-                                     (unsigned)((int)(pkc & 0xffffu) - j_lo) <= j_span;
-                    const T v = (T)vc;
-                    unsigned long long bal = __ballot(rel);
-                    while (bal) {
-                        const int b = __builtin_ctzll(bal);
-                        bal &= bal - 1;
-                        const unsigned pkb = (unsigned)__builtin_amdgcn_readlane((int)pkc, b);
-                        const T vb = readlane_t<T>(v, b);
-                        const int tp = kr2 - (int)(pkb >> 16), tq = lr2 - (int)(pkb & 0xffffu);
-                        if (p >= 0 && (unsigned)tp < (unsigned)pnx && (unsigned)tq < (unsigned)pny) acc += psf_l[tp * pny + tq] * vb;
-                    }
-                }
-                const int c_ = (cb << cbs) + ii;
-                if constexpr (F64) {
-                    ((double*)tile)[lane * TS + ii] = c_ == M.n_src ? dv : (double)acc;
-                } else {
-                    const double x = c_ == M.n_src ? dv : (double)acc * sc;
-                    ((uint16_t*)tile)[((lane >> 2) * CB + ii) * 4 + (lane & 3)] = f32_to_f16_rne((float)x);
-                }
-            }
+            lb_block_scalar<T>(bd, M, psf_l, tile, cb, w, lane, p, kr2, lr2, i_lo, i_span, j_lo, j_span, sc, dv, cpix, cval, cptr);
             __syncthreads();
             // the tile is the pool block image: 256 threads copy it with 16-byte stores
             {
@@ -1011,7 +1024,6 @@ void launch_syrk_f64_b(hipStream_t s, const BatchDev& bd, int max_ncb)
 #define SCB 128  // column-block width of the mixed path
 #define BP2 544  // LDS elements per k-group: 128 columns x 4 k + 32, so the four lane groups of a
                  // 64-bit operand read fall on alternating bank halves
-typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 __device__ __forceinline__ f16x8_t lds_k8(const uint16_t* arr, int g, int i)
 {
     const uint2 a = *(const uint2*)&arr[g * BP2 + i * 4];
