@@ -120,6 +120,11 @@ typedef struct {
      * pixel_fraction <= 0 means the reference's default 0.3 (pixelgrid.cpp:559). */
     int auto_srcgrid, auto_src_npixels;
     double pixel_fraction, noise_threshold;
+    /* MIXED: panel width of the FP32 Cholesky in columns (512, 1024 or 2048).  0 = autotune: the
+     * first wave of each size class (F's padded dimension) times the three widths on its own
+     * matrices and keeps the fastest for the context's later waves; the factor, hence every
+     * result, is bitwise the same for every width. */
+    int chol_panel;
 } qlx_settings;
 
 /* Per-evaluation result (the scalars QLens fills, qlens.h:750-761, 891). */
@@ -341,13 +346,16 @@ int qlx_kernel_time(void *ctx, const char *name, double *ms, int *launches);
  *   chol_flops         N_src^3 / 3 (F's factorization; R's banded one is not counted)
  *   n_items            items of the wave
  *   n_img, n_src, ld_src, rows_pad: sizes of the wave's last item
- *   n_waves            waves the last call was split into (HBM admission) */
+ *   n_waves            waves the last call was split into (HBM admission)
+ *   chol_panel         columns per panel of the MIXED factorization of the last wave (the
+ *                      settings' chol_panel, or the autotuner's choice for the wave's size class) */
 typedef struct {
     double syrk_dense_flops, syrk_exec_flops, syrk_alg_flops;
     double lb_bytes, integrand_evals, chol_flops;
     int n_items;
     int n_img, n_src, ld_src, rows_pad;
     int n_waves;
+    int chol_panel;
 } qlx_stats;
 int qlx_get_stats(void *ctx, qlx_stats *out);
 

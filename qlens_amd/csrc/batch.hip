@@ -2059,12 +2059,12 @@ static void chol_subpanel(hipStream_t s, const MatDesc* mats, int nmat, int max_
 // item's arithmetic is the same alone and in any batch.
 #define QLX_CHOL_SUB 4  // 256-wide left-looking parts per FP32 panel
 template <typename T>
-static void chol_two_level(hipStream_t s, const MatDesc* mats, int nmat, int max_nb)
+static void chol_two_level(hipStream_t s, const MatDesc* mats, int nmat, int max_nb, int panel_blocks)
 {
     if (sizeof(T) == 4) {
         const size_t lds_bf = (size_t)4 * TT * TBP * sizeof(uint16_t);
         allow_big_lds((const void*)k_chol_trail_bf16);
-        constexpr int NB = 4 * QLX_CHOL_SUB;  // panel width in 64-blocks
+        const int NB = panel_blocks > 0 ? panel_blocks : 4 * QLX_CHOL_SUB;  // panel width in 64-blocks
         for (int k0 = 0; k0 < max_nb; k0 += NB) {
             for (int ks = k0; ks < k0 + NB && ks < max_nb; ks += 4) {
                 if (ks > k0) {  // the half's columns, from all earlier halves of the panel (a strip)
@@ -2093,11 +2093,11 @@ static void chol_two_level(hipStream_t s, const MatDesc* mats, int nmat, int max
 // d_mats holds n32 FP32 matrices first, then n64 FP64 matrices.  Dense matrices (all of the
 // launch) use the two-level schedule; banded ones (lower band <= max_bwb blocks) the 64-block
 // right-looking one.
-void cholesky_batched(hipStream_t s, const MatDesc* d_mats, int n32, int n64, int max_nb, int dense, int max_bwb)
+void cholesky_batched(hipStream_t s, const MatDesc* d_mats, int n32, int n64, int max_nb, int dense, int max_bwb, int panel_blocks)
 {
     if (dense) {
-        if (n32) chol_two_level<float>(s, d_mats, n32, max_nb);
-        if (n64) chol_two_level<double>(s, d_mats + n32, n64, max_nb);
+        if (n32) chol_two_level<float>(s, d_mats, n32, max_nb, panel_blocks);
+        if (n64) chol_two_level<double>(s, d_mats + n32, n64, max_nb, 0);
         return;
     }
     for (int kb = 0; kb < max_nb; kb++) {

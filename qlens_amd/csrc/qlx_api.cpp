@@ -115,6 +115,8 @@ struct Ctx {
     std::vector<int> last_wave;  // caller indices of the last wave's items, in wave order (their rays
                                  // stay in `beta` until the next evaluation)
     std::vector<unsigned> nzh;  // host copy of the support masks (work accounting)
+    std::map<int, int> chol_tuned;  // autotuned FP32 Cholesky panel (64-blocks) per padded block count
+    int last_panel = 0;
     // timers
     std::map<std::string, Timer> timers;
     qlx_stats stats{};
@@ -858,8 +860,43 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     cholesky_batched(c->stream2, c->d_mats.as<MatDesc>() + nF, 0, nR, max_ld / 64, 0, max_bwb);
     HIPCHK(c, hipEventRecord(c->ev_join, c->stream2));
 
+    int panel = 0;  // FP32 panel in 64-blocks
+    if (mixed) {
+        const int nbk = max_ld / 64;
+        if (S.chol_panel > 0) {
+            panel = S.chol_panel / 64;
+        } else {
+            auto it = c->chol_tuned.find(nbk);
+            if (it != c->chol_tuned.end()) {
+                panel = it->second;
+            } else {
+                // autotune: each candidate width factors this wave's matrices (a repeat reads F as
+                // formed again, so the runs are interchangeable); the fastest is kept
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                HIPCHK(c, hipEventCreate(&e0));
+                HIPCHK(c, hipEventCreate(&e1));
+                float best = 0.f;
+                for (int cand : {8, 16, 32}) {
+                    HIPCHK(c, hipEventRecord(e0, st));
+                    cholesky_batched(st, c->d_mats.as<MatDesc>(), nF, 0, nbk, 1, 0, cand);
+                    HIPCHK(c, hipEventRecord(e1, st));
+                    HIPCHK(c, hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+                    if (panel == 0 || ms < best) {
+                        best = ms;
+                        panel = cand;
+                    }
+                }
+                hipEventDestroy(e0);
+                hipEventDestroy(e1);
+                c->chol_tuned[nbk] = panel;
+            }
+        }
+        c->last_panel = panel * 64;
+    }
     timer_begin(c, "cholesky");
-    cholesky_batched(st, c->d_mats.as<MatDesc>(), mixed ? nF : 0, mixed ? 0 : nF, max_ld / 64, 1, 0);
+    cholesky_batched(st, c->d_mats.as<MatDesc>(), mixed ? nF : 0, mixed ? 0 : nF, max_ld / 64, 1, 0, panel);
     timer_end(c, "cholesky");
     HIPCHK(c, hipStreamWaitEvent(st, c->ev_join, 0));
     launch_logdets_b(st, bd, mixed ? 1 : 0);
@@ -1243,6 +1280,8 @@ int qlx_set_settings(void* ctx, const qlx_settings* s)
     if (s->integral_tol <= 0) return fail(c, QLX_ERR_ARG, "integral tolerance must be > 0");
     if (s->optimize_regparam && s->precision != QLX_PREC_FP64) return fail(c, QLX_ERR_ARG, "regparam optimisation runs in FP64 precision only");
     if (s->optimize_regparam && s->regopt_max_iter > 1000) return fail(c, QLX_ERR_RANGE, "at most 1000 regparam iterations");
+    if (s->chol_panel != 0 && s->chol_panel != 512 && s->chol_panel != 1024 && s->chol_panel != 2048)
+        return fail(c, QLX_ERR_ARG, "chol_panel must be 0 (autotune), 512, 1024 or 2048");
     c->set = *s;
     c->settings_set = true;
     return QLX_OK;
@@ -1620,6 +1659,7 @@ int qlx_get_stats(void* ctx, qlx_stats* out)
     if (!c || !out) return QLX_ERR_ARG;
     *out = c->stats;
     out->n_waves = c->stats_waves;
+    out->chol_panel = c->last_panel;
     return QLX_OK;
 }
 

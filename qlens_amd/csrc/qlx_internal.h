@@ -248,7 +248,10 @@ void launch_syrk_f64_b(hipStream_t s, const BatchDev& bd, int max_ncb);
 void launch_syrk_f16_b(hipStream_t s, const BatchDev& bd, int max_ncb);
 void launch_curvature_b(hipStream_t s, const BatchDev& bd, int max_cells, int max_rtiles);
 void launch_prep_fr_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32);
-void cholesky_batched(hipStream_t s, const MatDesc* d_mats, int n32, int n64, int max_nb, int dense, int max_bwb);
+// panel_blocks: width of the FP32 dense schedule's panels in 64-blocks (a multiple of 4; 0 = 16);
+// the factor is bitwise the same for every width
+void cholesky_batched(hipStream_t s, const MatDesc* d_mats, int n32, int n64, int max_nb, int dense, int max_bwb,
+                      int panel_blocks = 0);
 void launch_logdets_b(hipStream_t s, const BatchDev& bd, int f32);
 void launch_trisolve_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32);
 void launch_refine_b(hipStream_t s, const BatchDev& bd, int max_ld, int iters, double tol);

@@ -448,11 +448,14 @@ __device__ __forceinline__ void map_ray(const BatchDev& bd, int item, size_t ro,
     mw[0] = weight * (bx * (p1y - p2y) + by * (p2x - p1x) + p1x * p2y - p1y * p2x) / d;
     mw[1] = weight * (bx * (p2y - p0y) + by * (p0x - p2x) + p0y * p2x - p0x * p2y) / d;
     mw[2] = weight * (bx * (p0y - p1y) + by * (p1x - p0x) + p0x * p1y - p0y * p1x) / d;
+    // the flags are set by many rays: a store only where the flag is still clear keeps the
+    // repeated partial-line writes out of the memory traffic
     int* cm = bd.cmapped + (size_t)item * bd.s_cell;
-    cm[c0] = 1;
-    cm[c1] = 1;
-    cm[c2] = 1;
-    bd.pmapped[(size_t)item * bd.s_pix + p] = 1;
+    if (!cm[c0]) cm[c0] = 1;
+    if (!cm[c1]) cm[c1] = 1;
+    if (!cm[c2]) cm[c2] = 1;
+    int* pm = bd.pmapped + (size_t)item * bd.s_pix + p;
+    if (!*pm) *pm = 1;
 }
 
 // Origin of ray k of pixel (i, j): the centre of subpixel (k / nsplit, k % nsplit) of an

@@ -440,3 +440,25 @@ def test_auto_source_grid_per_item_in_a_batch(qlx_ctx, golden_dir):
             assert res[k][f] == one[f], (k, f)
         sizes.add((res[k]["src_nx"], res[k]["src_ny"], res[k]["src_xmin"]))
     assert len(sizes) > 1  # the perturbed models do get their own grids
+
+
+def test_cholesky_panel_autotune_is_bitwise_neutral(qlx_ctx, golden_dir):
+    """The mixed path's FP32 Cholesky panel width (512, 1024, 2048 columns, or chosen by the
+    autotuner on the first wave of a size class) changes the schedule only: every result is the
+    same bit for bit, and the autotuner reports one of the three widths."""
+    g = np.load(os.path.join(golden_dir, "inversion_C2.npz"))
+    case = W.c2()
+    qlx_ctx.load_workload(case, g["data"], precision=qlx.QLX_PREC_MIXED)
+    models = [case["lenses"]] + _perturbed(case, 3, 17)
+    out = {}
+    for panel in (512, 1024, 2048, 0):
+        qlx_ctx.set_settings(case["nsplit"], case["srcgrid"], case["src_n"], case["regparam"], case["integral_tol"],
+                             precision=qlx.QLX_PREC_MIXED, chol_panel=panel)
+        res = qlx_ctx.invert_image_surface_brightness_map(models)
+        st = qlx.get_stats(qlx_ctx)
+        out[panel] = [(r["minus2lnE"], r["logdetF"], r["refine_resid"]) for r in res]
+        print("chol_panel %d: used %d columns, -2lnE[0] %.12g" % (panel, st["chol_panel"], res[0]["minus2lnE"]))
+        assert st["chol_panel"] == (panel or st["chol_panel"]) and st["chol_panel"] in (512, 1024, 2048)
+        assert all(r["status"] == 0 for r in res)
+    for panel in (1024, 2048, 0):
+        assert out[panel] == out[512], panel

@@ -148,14 +148,15 @@ def test_required_deflection_accuracy(qlx_ctx):
     props = W.proposals(case, 8, seed=5)
     base = qlx_ctx.invert_image_surface_brightness_map(props)
     worst = {}
-    for eps in (1e-3, 1e-4, 1e-5, 1e-6):
+    epss = (1e-3, 1e-4, 1e-5, 1e-6)
+    for eps in epss:
         models = [[(t, ((p[0] * (1 + eps),) + tuple(p[1:])) if t == W.NFW else p) for t, p in m] for m in props]
         res = qlx_ctx.invert_image_surface_brightness_map(models)
         worst[eps] = max(abs(a["minus2lnE"] - b["minus2lnE"]) / 2 for a, b in zip(res, base))
         print("systematic deflection error %g: max |dlnE| %.4g" % (eps, worst[eps]))
     # roughly linear in eps over three decades, and the 0.1 contract sits near eps = 1e-6
-    for e in (1e-3, 1e-4, 1e-5):
-        assert 3 < worst[e] / worst[e / 10] < 30, (e, worst)
+    for e, e10 in zip(epss[:-1], epss[1:]):
+        assert 3 < worst[e] / worst[e10] < 30, (e, worst)
     assert worst[1e-5] > 0.1 and worst[1e-3] > 10
 
 
