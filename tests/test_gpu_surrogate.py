@@ -78,6 +78,42 @@ def test_surrogate_deflection_error(qlx_ctx, trained):
     assert np.median(he) < 1e-2
 
 
+def test_surrogate_per_pixel_deflection_error_c3(qlx_ctx, trained):
+    """The network judged by quadrature on every ray of the 400x400 / 4x4 image (2.56 M rays, the
+    NFW lens of the numerical-profile workload): the source-plane points of an evaluation with the
+    learned NFW deflection against the quadrature evaluation's (the SIE and shear parts are the
+    same closed forms in both, so the difference is the NFW deflection error), relative to the
+    NFW deflection itself (FP64 quadrature at the same points); per pixel the worst of its 16 rays.
+    The network's stated contract, median < 3e-3 and 99th percentile < 3e-2, over the image."""
+    case = W.c3()
+    nx = len(case["xvals"]) - 1
+    qlx_ctx.load_image(case["xvals"], case["yvals"], np.zeros((nx, nx)), sigma=case["sigma"])
+    qlx_ctx.set_psf(case["psf"])
+    qlx_ctx.set_settings(case["nsplit"], case["srcgrid"], case["src_n"], case["regparam"], case["integral_tol"])
+    data = qlx_ctx.mock_image(case["lenses"], case["sources"], case["sigma"], case["seed"])
+    qlx_ctx.load_workload(case, data)
+    rays = {}
+    for mode in (qlx.QLX_DEF_QUADRATURE, qlx.QLX_DEF_SURROGATE):
+        qlx_ctx.invert_image_surface_brightness_map(case["lenses"], def_mode=mode)
+        rays[mode] = qlx_ctx.get_rays(0)
+    ns = case["nsplit"]
+    xv, yv = case["xvals"], case["yvals"]
+    u = (1 + 2 * np.arange(ns)) / (2 * ns)
+    sx = (1 - u)[None, :] * xv[:-1, None] + u[None, :] * xv[1:, None]  # [i, ii]
+    sy = (1 - u)[None, :] * yv[:-1, None] + u[None, :] * yv[1:, None]  # [j, jj]
+    pts = np.zeros((nx, nx, ns, ns, 2))
+    pts[..., 0] = sx[:, None, :, None]
+    pts[..., 1] = sy[None, :, None, :]
+    pts = pts.reshape(-1, 2)
+    nfw = [lz for lz in case["lenses"] if lz[0] == W.NFW]
+    a_nfw = pts - qlx_ctx.raytrace(nfw, pts)
+    d = np.hypot(*(rays[qlx.QLX_DEF_SURROGATE] - rays[qlx.QLX_DEF_QUADRATURE]).reshape(-1, 2).T)
+    rel = (d / np.hypot(*a_nfw.T)).reshape(nx * nx, ns * ns).max(1)
+    print("C3 per-pixel NFW deflection error (worst of 16 rays): median %.3g, p99 %.3g, max %.3g; "
+          "absolute median %.3g arcsec" % (np.median(rel), np.percentile(rel, 99), rel.max(), np.median(d)))
+    assert np.median(rel) < 3e-3 and np.percentile(rel, 99) < 3e-2
+
+
 def test_surrogate_falls_back_to_quadrature_outside_box(qlx_ctx, golden_dir, trained):
     g = np.load(golden_dir + "/inversion_C2s.npz")
     case = W.c2_small()
