@@ -125,6 +125,10 @@ typedef struct {
      * matrices and keeps the fastest for the context's later waves; the factor, hence every
      * result, is bitwise the same for every width. */
     int chol_panel;
+    /* The factorizations' launch sequences (F's Cholesky, R's banded one) are captured in a HIP
+     * graph per size class on first use and replayed (one launch instead of hundreds); no_graphs = 1
+     * launches them one by one instead.  The results are bitwise the same either way. */
+    int no_graphs;
 } qlx_settings;
 
 /* Per-evaluation result (the scalars QLens fills, qlens.h:750-761, 891). */
@@ -348,7 +352,8 @@ int qlx_kernel_time(void *ctx, const char *name, double *ms, int *launches);
  *   n_img, n_src, ld_src, rows_pad: sizes of the wave's last item
  *   n_waves            waves the last call was split into (HBM admission)
  *   chol_panel         columns per panel of the MIXED factorization of the last wave (the
- *                      settings' chol_panel, or the autotuner's choice for the wave's size class) */
+ *                      settings' chol_panel, or the autotuner's choice for the wave's size class)
+ *   graph_captures, graph_replays  factorization graphs captured / replayed by the context so far */
 typedef struct {
     double syrk_dense_flops, syrk_exec_flops, syrk_alg_flops;
     double lb_bytes, integrand_evals, chol_flops;
@@ -356,6 +361,7 @@ typedef struct {
     int n_img, n_src, ld_src, rows_pad;
     int n_waves;
     int chol_panel;
+    int graph_captures, graph_replays;
 } qlx_stats;
 int qlx_get_stats(void *ctx, qlx_stats *out);
 

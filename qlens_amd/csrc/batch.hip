@@ -1934,9 +1934,12 @@ k_chol_trail_bf16(const MatDesc* mats, int k0, int nq, int ncolt)
     const __amdgpu_buffer_rsrc_t rc = (k0 == 0 && D.A0) ? __builtin_amdgcn_make_buffer_rsrc(D.A0, 0, ld * ld * 4, 0x00020000) : rs;
     const int rI = (r0b + 2 * I2) * TS, rJ = (r0b + 2 * J2) * TS;
     // byte offsets: the lane's part in a VGPR, the uniform part in an SGPR.  Staging step st covers
-    // columns 32 st .. 32 st + 31 of the panel; piece h of a thread: row (tid >> 3) + 32 h, columns
-    // 4 (tid & 7) .. +3 of the step
-    const int vp = ((tid >> 3) * ld + 4 * (tid & 7)) * 4;
+    // columns 32 st .. 32 st + 31 of the panel; piece h of a thread: row srow + 32 h, columns
+    // 4 (tid & 7) .. +3 of the step.  The 16 lanes of each group of tid stage rows r and r + 4,
+    // whose 64-byte LDS pieces (row pitch 80 bytes) fall on disjoint banks
+    const int g16 = tid >> 4;
+    const int srow = (g16 & 3) + 8 * (g16 >> 2) + 4 * ((tid >> 3) & 1);
+    const int vp = (srow * ld + 4 * (tid & 7)) * 4;
     const int sA = (rI * ld + k0 * TS) * 4, sB = (rJ * ld + k0 * TS) * 4;
     const int vc = ((wr * 64 + 4 * (lane >> 4)) * ld + wc * 64 + (lane & 15)) * 4;
     const int sC = (rI * ld + rJ) * 4;
@@ -1963,7 +1966,7 @@ k_chol_trail_bf16(const MatDesc* mats, int k0, int nq, int ncolt)
         __syncthreads();
 #pragma unroll
         for (int h = 0; h < 4; h++) {
-            const int i = (tid >> 3) + 32 * h, k = 4 * (tid & 7);
+            const int i = srow + 32 * h, k = 4 * (tid & 7);
             const P xa = -pa[h], xb = pb[h];
             const bf16x4_t ah = __builtin_convertvector(xa, bf16x4_t);
             const bf16x4_t bh = __builtin_convertvector(xb, bf16x4_t);

@@ -117,6 +117,9 @@ struct Ctx {
     std::vector<unsigned> nzh;  // host copy of the support masks (work accounting)
     std::map<int, int> chol_tuned;  // autotuned FP32 Cholesky panel (64-blocks) per padded block count
     int last_panel = 0;
+    // captured factorization sequences: (stream, MatDesc array, counts, sizes, panel) -> executable
+    std::map<std::vector<long long>, hipGraphExec_t> graphs;
+    int graph_captures = 0, graph_replays = 0;
     // timers
     std::map<std::string, Timer> timers;
     qlx_stats stats{};
@@ -473,6 +476,42 @@ struct RegoptBrent {
         state = 1;
     }
 };
+
+// A factorization's launch sequence (cholesky_batched) through a HIP graph: captured on the stream
+// on first use of its arguments (the kernels read the matrix descriptors from device memory, so a
+// replay serves new contents of the same descriptor array), then replayed as one launch.  With
+// no_graphs set, or if capture fails, the launches go to the stream one by one.
+int factor_on(Ctx* c, hipStream_t s, const MatDesc* mats, int n32, int n64, int max_nb, int dense, int max_bwb, int panel)
+{
+    if (c->set.no_graphs) {
+        cholesky_batched(s, mats, n32, n64, max_nb, dense, max_bwb, panel);
+        return QLX_OK;
+    }
+    const std::vector<long long> key = {(long long)(uintptr_t)s, (long long)(uintptr_t)mats, n32, n64, max_nb, dense, max_bwb, panel};
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+        if (c->graphs.size() >= 32) {  // bound the cache: drop everything (sizes changed a lot)
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream2));
+            for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
+            c->graphs.clear();
+        }
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ex = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+        cholesky_batched(s, mats, n32, n64, max_nb, dense, max_bwb, panel);
+        HIPCHK(c, hipStreamEndCapture(s, &g));
+        const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(c, QLX_ERR_HIP, "hipGraphInstantiate of a factorization failed");
+        it = c->graphs.emplace(key, ex).first;
+        c->graph_captures++;
+    } else {
+        c->graph_replays++;
+    }
+    HIPCHK(c, hipGraphLaunch(it->second, s));
+    return QLX_OK;
+}
 
 // run_wave's answer when the wave's exact dense and Lb-pool bytes exceed the budget: the caller
 // evaluates the same items again in smaller waves (an item's result does not depend on its wave)
@@ -857,7 +896,10 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     HIPCHK(c, hipMemcpyAsync(c->Rf.p, c->R.p, r_tot * 8, hipMemcpyDeviceToDevice, c->stream2));
     int max_bwb = 1;
     for (const MatDesc& r : rm) max_bwb = std::max(max_bwb, r.bwb);
-    cholesky_batched(c->stream2, c->d_mats.as<MatDesc>() + nF, 0, nR, max_ld / 64, 0, max_bwb);
+    {
+        const int rc = factor_on(c, c->stream2, c->d_mats.as<MatDesc>() + nF, 0, nR, max_ld / 64, 0, max_bwb, 0);
+        if (rc) return rc;
+    }
     HIPCHK(c, hipEventRecord(c->ev_join, c->stream2));
 
     int panel = 0;  // FP32 panel in 64-blocks
@@ -896,7 +938,10 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         c->last_panel = panel * 64;
     }
     timer_begin(c, "cholesky");
-    cholesky_batched(st, c->d_mats.as<MatDesc>(), mixed ? nF : 0, mixed ? 0 : nF, max_ld / 64, 1, 0, panel);
+    {
+        const int rc = factor_on(c, st, c->d_mats.as<MatDesc>(), mixed ? nF : 0, mixed ? 0 : nF, max_ld / 64, 1, 0, panel);
+        if (rc) return rc;
+    }
     timer_end(c, "cholesky");
     HIPCHK(c, hipStreamWaitEvent(st, c->ev_join, 0));
     launch_logdets_b(st, bd, mixed ? 1 : 0);
@@ -930,7 +975,10 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
             HIPCHK(c, hipMemcpyAsync(c->d_lam.p, lamh.data(), (size_t)B * 8, hipMemcpyHostToDevice, st));
             launch_prep_fr_b(st, bd, max_ld, 0);
             timer_begin(c, "cholesky");
-            cholesky_batched(st, c->d_mats.as<MatDesc>(), 0, nF, max_ld / 64, 1, 0);
+            {
+                const int rc = factor_on(c, st, c->d_mats.as<MatDesc>(), 0, nF, max_ld / 64, 1, 0, 0);
+                if (rc) return rc;
+            }
             timer_end(c, "cholesky");
             launch_logdets_b(st, bd, 0);
             launch_trisolve_b(st, bd, max_ld, 0);
@@ -1157,6 +1205,7 @@ int qlx_ctx_destroy(void* ctx)
         if (kv.second.b) hipEventDestroy(kv.second.b);
     }
     hipStreamSynchronize(c->stream2);
+    for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
     if (c->sur) sur_state_destroy(c->sur);
     hipEventDestroy(c->ev_fork);
     hipEventDestroy(c->ev_join);
@@ -1660,6 +1709,8 @@ int qlx_get_stats(void* ctx, qlx_stats* out)
     *out = c->stats;
     out->n_waves = c->stats_waves;
     out->chol_panel = c->last_panel;
+    out->graph_captures = c->graph_captures;
+    out->graph_replays = c->graph_replays;
     return QLX_OK;
 }
 

@@ -47,7 +47,7 @@ class QlxSettings(C.Structure):
                 ("optimize_regparam", C.c_int), ("regopt_minlog", C.c_double), ("regopt_maxlog", C.c_double),
                 ("regopt_tol", C.c_double), ("regopt_max_iter", C.c_int), ("auto_srcgrid", C.c_int),
                 ("auto_src_npixels", C.c_int), ("pixel_fraction", C.c_double), ("noise_threshold", C.c_double),
-                ("chol_panel", C.c_int)]
+                ("chol_panel", C.c_int), ("no_graphs", C.c_int)]
 
 
 class QlxResult(C.Structure):
@@ -337,7 +337,7 @@ class PixelInversion:
     def set_settings(self, nsplit, srcgrid, src_n, regparam, integral_tol=1e-3, precision=QLX_PREC_FP64,
                      ray_precision=QLX_RAY_FP64, refine_iters=0, refine_tol=0.0, optimize_regparam=False,
                      regopt_bounds=(-3.0, 5.0), regopt_tol=0.01, regopt_max_iter=20, auto_srcgrid=False,
-                     auto_src_npixels=False, pixel_fraction=0.3, noise_threshold=0.0, chol_panel=0):
+                     auto_src_npixels=False, pixel_fraction=0.3, noise_threshold=0.0, chol_panel=0, graphs=True):
         """QLens settings of the inversion; optimize_regparam runs Brent over log10(lambda) in
         regopt_bounds per item (the reference's 'optimize_regparam on', FP64 only); auto_srcgrid and
         auto_src_npixels find each item's source-grid box and size from its own rays (the
@@ -346,7 +346,7 @@ class PixelInversion:
                         integral_tol, precision, ray_precision, refine_iters, refine_tol,
                         1 if optimize_regparam else 0, regopt_bounds[0], regopt_bounds[1], regopt_tol,
                         int(regopt_max_iter), 1 if auto_srcgrid else 0, 1 if auto_src_npixels else 0,
-                        pixel_fraction, noise_threshold, int(chol_panel))
+                        pixel_fraction, noise_threshold, int(chol_panel), 0 if graphs else 1)
         self._check(self.lib.qlx_set_settings(self.ctx, C.byref(s)), "qlx_set_settings")
         self.settings = dict(nsplit=nsplit, srcgrid=srcgrid, src_n=src_n, regparam=regparam,
                              integral_tol=integral_tol, precision=precision)
@@ -542,7 +542,8 @@ class QlxStats(C.Structure):
     _fields_ = [("syrk_dense_flops", C.c_double), ("syrk_exec_flops", C.c_double), ("syrk_alg_flops", C.c_double),
                 ("lb_bytes", C.c_double), ("integrand_evals", C.c_double), ("chol_flops", C.c_double),
                 ("n_items", C.c_int), ("n_img", C.c_int), ("n_src", C.c_int), ("ld_src", C.c_int),
-                ("rows_pad", C.c_int), ("n_waves", C.c_int), ("chol_panel", C.c_int)]
+                ("rows_pad", C.c_int), ("n_waves", C.c_int), ("chol_panel", C.c_int),
+                ("graph_captures", C.c_int), ("graph_replays", C.c_int)]
 
 
 def get_stats(inv):

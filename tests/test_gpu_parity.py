@@ -462,3 +462,32 @@ def test_cholesky_panel_autotune_is_bitwise_neutral(qlx_ctx, golden_dir):
         assert all(r["status"] == 0 for r in res)
     for panel in (1024, 2048, 0):
         assert out[panel] == out[512], panel
+
+
+def test_factorization_graphs_replay_bitwise(qlx_ctx, golden_dir):
+    """The factorizations run as captured HIP graphs, replayed on later calls with eager work and
+    synchronizations in between: every replay gives the results of launching the kernels one by
+    one, bit for bit, in FP64 (one proposal per call, and a batch) and in mixed precision."""
+    g = np.load(os.path.join(golden_dir, "inversion_C2s.npz"))
+    case = W.c2_small()
+    models = [case["lenses"]] + _perturbed(case, 3, 29)
+    for prec in (qlx.QLX_PREC_FP64, qlx.QLX_PREC_MIXED):
+        ref = {}
+        for graphs in (False, True):
+            qlx_ctx.load_workload(case, g["data"], precision=prec)
+            qlx_ctx.set_settings(case["nsplit"], case["srcgrid"], case["src_n"], case["regparam"], case["integral_tol"],
+                                 precision=prec, graphs=graphs)
+            before = qlx.get_stats(qlx_ctx)
+            runs = []
+            for rep in range(4):  # the first call captures, the next ones replay
+                one = qlx_ctx.invert_image_surface_brightness_map(models[rep % len(models)])[0]
+                batch = qlx_ctx.invert_image_surface_brightness_map(models)
+                qlx_ctx.get_source(0)  # eager work and a synchronization between replays
+                runs.append((one, batch))
+            after = qlx.get_stats(qlx_ctx)
+            if graphs:
+                assert after["graph_replays"] - before["graph_replays"] >= 8, (before, after)
+            ref[graphs] = runs
+        for (o0, b0), (o1, b1) in zip(ref[False], ref[True]):
+            assert o0 == o1 and b0 == b1, prec
+    print("graphs: %d captured, %d replayed" % (after["graph_captures"], after["graph_replays"]))
