@@ -2018,6 +2018,250 @@ k_chol_trail_bf16(const MatDesc* mats, int k0, int nq, int ncolt)
             }
 }
 
+// The left-looking panel step for FP32 matrices (k_chol_lpanel_m's schedule and operands) with the
+// products on v_mfma_f32_16x16x32_bf16: every operand is split x = hi + lo into BF16 as it is
+// staged (as in k_chol_trail_bf16) and the four products hi*hi + hi*lo + lo*hi + lo*lo are summed
+// in FP32 (1.9 ms faster than FP32 MFMA per batch of 256 3712 x 3712 matrices; |delta lnE| against
+// FP64 unchanged).  The workgroup of row block kb + 1, having brought diagonal block kb + 1 up to
+// date, also factors it (chol_diag_f32_4w: the same bits as the diagonal kernel), so within a part
+// only the first column block needs a diagonal launch; the grid is (matrices, row blocks) so that
+// these workgroups are dispatched first (dispatched last, their serial factorization prolonged
+// every launch).  LDS: A hi, A lo, B hi, B lo, 64 rows x LPB BF16 each (144-byte rows: the 16 rows
+// of an operand read start on distinct 16-byte bank groups).
+#define LPB 72
+__device__ __forceinline__ void stage64_split(const float* P, int sp, __bf16* H, __bf16* L)
+{
+    typedef float P4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        const int e = (threadIdx.x + 256 * h) * 4;
+        const int i = e >> 6, k = e & 63;
+        const P4 x = *(const P4*)&P[(size_t)i * sp + k];
+        const bf16x4_t xh = __builtin_convertvector(x, bf16x4_t);
+        *(bf16x4_t*)&H[i * LPB + k] = xh;
+        *(bf16x4_t*)&L[i * LPB + k] = __builtin_convertvector(x - __builtin_convertvector(xh, P4), bf16x4_t);
+    }
+}
+
+// an accumulator tile (mma64's layout: wave (wr, wc) holds quadrant rows wr*32.., columns wc*32..)
+// split into H / L as the rows of an operand
+__device__ __forceinline__ void acc64_split(const float4_t (&acc)[2][2], __bf16* H, __bf16* L)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = w >> 1, wc = w & 1;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int rr = wr * 32 + a * 16 + 4 * (lane >> 4) + v;
+                const int cc = wc * 32 + c * 16 + (lane & 15);
+                const float x = acc[a][c][v];
+                const __bf16 xh = (__bf16)x;
+                H[rr * LPB + cc] = xh;
+                L[rr * LPB + cc] = (__bf16)(x - (float)xh);
+            }
+}
+
+// acc(i, j) += sum_q A(i, q) B(j, q) over the 64 staged q, A = Ah + Al, B = Bh + Bl
+__device__ __forceinline__ void mma64_bf16(const __bf16* Ah, const __bf16* Al, const __bf16* Bh, const __bf16* Bl,
+                                           float4_t (&acc)[2][2])
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = w >> 1, wc = w & 1;
+#pragma unroll
+    for (int ks = 0; ks < TS; ks += 32) {
+        const int kq = ks + 8 * (lane >> 4);
+        bf16x8_t ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int ia = (wr * 32 + t * 16 + (lane & 15)) * LPB + kq;
+            const int ib = (wc * 32 + t * 16 + (lane & 15)) * LPB + kq;
+            ah[t] = *(const bf16x8_t*)&Ah[ia];
+            al[t] = *(const bf16x8_t*)&Al[ia];
+            bh[t] = *(const bf16x8_t*)&Bh[ib];
+            bl[t] = *(const bf16x8_t*)&Bl[ib];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[c], acc[a][c], 0, 0, 0);
+                acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[c], acc[a][c], 0, 0, 0);
+                acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[c], acc[a][c], 0, 0, 0);
+                acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bl[c], acc[a][c], 0, 0, 0);
+            }
+    }
+}
+
+// The FP32 diagonal block factored by the four waves of a workgroup, with chol_diag_wave_f32's
+// arithmetic (the same operations on every element in the same order, so the same bits): lane r
+// of wave w holds a[j] = A(r, 16 w + j).  Column step c: the wave owning column c forms the pivot
+// and the column into Lt[c] (and 1/sqrt of the pivot into Rd[c]); after the barrier every wave with
+// columns right of c applies the rank-1 update to them.  L^-1 by the right-looking substitution,
+// lane r of wave w holding rows 16 w .. 16 w + 15 of column r: step q scales row q (its owner) and
+// publishes it in X[q], the waves below update their rows.  Writes L's lower triangle into the
+// block (row pitch ld), the pivots and W.  LDS: Lt and X 64 x 64 each, Rd 64.
+__device__ __forceinline__ void chol_diag_f32_4w(const MatDesc& D, int kb, float* tile, int ld, float (&a)[16],
+                                                 float (*Lt)[TS], float (*X)[TS], float* Rd)
+{
+    typedef float T;
+    const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long long base = (long long)kb * TS;
+    bool bad = false;
+#pragma unroll
+    for (int c = 0; c < TS; c++) {
+        if (w == (c >> 4)) {
+            const T piv = lane_bcast(a[c & 15], c);
+            bad |= (!(piv > 0) && base + c < D.n);  // negative, zero or NaN pivot
+            const T pa = fabs(piv);
+            const T rd = rsqrtf(pa);
+            const T d = (pa > 0) ? pa * rd : (T)0;
+            const T l = (r > c) ? a[c & 15] * rd : d;
+            a[c & 15] = l;
+            Lt[c][r] = l;
+            if (r == c) Rd[c] = rd;
+        }
+        __syncthreads();
+        if (16 * w + 15 > c) {
+            const T l = Lt[c][r];
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                if (16 * w + j > c) a[j] = fma(-l, Lt[c][16 * w + j], a[j]);
+        }
+    }
+    if (r == 0 && bad) atomicOr(D.status, 1);  // bad is uniform in a wave (the pivots are broadcast)
+    // row r of the lower triangle
+#pragma unroll
+    for (int j = 0; j < 16; j += 4) {
+        const int q = 16 * w + j;
+        T* dst = tile + (size_t)r * ld + q;
+        if (q + 3 <= r) {
+            *(float4*)dst = make_float4(a[j], a[j + 1], a[j + 2], a[j + 3]);
+        } else if (q <= r) {
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (q + u <= r) dst[u] = a[j + u];
+        }
+    }
+    if (w == 0) ((T*)D.diag)[base + r] = Lt[r][r];
+    T x[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) x[j] = (16 * w + j == r) ? (T)1 : (T)0;
+#pragma unroll
+    for (int q = 0; q < TS; q++) {
+        if (w == (q >> 4)) {
+            x[q & 15] *= Rd[q];
+            X[q][r] = x[q & 15];
+        }
+        __syncthreads();
+        if (16 * w + 15 > q) {
+            const T xq = X[q][r];
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                if (16 * w + j > q) x[j] = fma(-Lt[q][16 * w + j], xq, x[j]);
+        }
+    }
+    T* W = (T*)D.W + (size_t)kb * TS * TS;
+#pragma unroll
+    for (int j = 0; j < 16; j++) W[(16 * w + j) * TS + r] = x[j];
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_chol_lpanel_bf16(const MatDesc* mats, int kb, int k0, int kend)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 sm[4 * TS * LPB];
+    __bf16* const Ah = sm;
+    __bf16* const Al = sm + TS * LPB;
+    __bf16* const Bh = sm + 2 * TS * LPB;
+    __bf16* const Bl = sm + 3 * TS * LPB;
+    typedef float T;
+    // grid (matrices, row blocks): the workgroups of the first row block, which also factor the next
+    // diagonal block, are dispatched first
+    const MatDesc D = mats[blockIdx.x];
+    if (kb >= D.nb) return;
+    const int ib = kb + 1 + blockIdx.y;
+    if (ib > band_last(D, kb)) return;
+    int ld, lp;
+    T* blk = tile_ptr<T>(D, ib, kb, ld);
+    const bool first = (k0 == 0 && D.A0);
+    T cv[16];
+    load64c<T>(first ? (const T*)D.A0 + (blk - (T*)D.A) : blk, ld, cv);
+    float4_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) acc[a][c] = (float4_t){0, 0, 0, 0};
+    for (int kp = k0; kp < kb; kp++) {
+        const T* Pa = tile_ptr<T>(D, ib, kp, lp);
+        const T* Pb = tile_ptr<T>(D, kb, kp, lp);
+        __syncthreads();
+        stage64_split(Pa, lp, Ah, Al);
+        stage64_split(Pb, lp, Bh, Bl);
+        __syncthreads();
+        mma64_bf16(Ah, Al, Bh, Bl, acc);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) acc[a][c][v] = cv[(a * 2 + c) * 4 + v] - acc[a][c][v];
+    __syncthreads();
+    acc64_split(acc, Ah, Al);
+    stage64_split((const T*)D.W + (size_t)kb * TS * TS, TS, Bh, Bl);
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) acc[a][c] = (float4_t){0, 0, 0, 0};
+    mma64_bf16(Ah, Al, Bh, Bl, acc);  // L_ib,kb(i, j) = sum_q A'(i, q) W_kb(j, q)
+    store64<T>(blk, ld, acc, 0);
+    if (ib != kb + 1 || ib >= kend) return;
+    T* dg = tile_ptr<T>(D, ib, ib, ld);
+    load64c<T>(first ? (const T*)D.A0 + (dg - (T*)D.A) : dg, ld, cv);
+    __syncthreads();
+    acc64_split(acc, Ah, Al);
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) acc[a][c] = (float4_t){0, 0, 0, 0};
+    mma64_bf16(Ah, Al, Ah, Al, acc);
+    for (int kp = k0; kp < kb; kp++) {
+        const T* Pa = tile_ptr<T>(D, ib, kp, lp);
+        __syncthreads();
+        stage64_split(Pa, lp, Ah, Al);
+        __syncthreads();
+        mma64_bf16(Ah, Al, Ah, Al, acc);
+    }
+    // the block, now up to date, is factored here, so the next column step needs no diagonal
+    // launch: through LDS (S, 64 x 65, over the operand tiles) into rows per lane
+    float* S = (float*)sm;
+    float(*Lt)[TS] = (float(*)[TS])((float*)sm + TS * 65);
+    float* Rd = (float*)sm + TS * 65 + TS * TS;
+    {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        const int wr = w >> 1, wc = w & 1;
+        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int rr = wr * 32 + a * 16 + 4 * (lane >> 4) + v;
+                    const int cc = wc * 32 + c * 16 + (lane & 15);
+                    S[rr * 65 + cc] = cv[(a * 2 + c) * 4 + v] - acc[a][c][v];
+                }
+        __syncthreads();
+        float av[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) av[j] = S[lane * 65 + 16 * w + j];
+        chol_diag_f32_4w(D, ib, dg, ld, av, Lt, (float(*)[TS])S, Rd);  // X over S: read before the first barrier
+    }
+}
+
 template <typename T>
 static void chol_step(hipStream_t s, const MatDesc* mats, int nmat, int kb, int rest)
 {
@@ -2034,10 +2278,15 @@ static void chol_subpanel(hipStream_t s, const MatDesc* mats, int nmat, int max_
 {
     for (int kb = k0; kb < kend && kb < max_nb; kb++) {
         const int rest = max_nb - kb - 1;
-        hipLaunchKernelGGL(k_chol_diag_m<T>, dim3(1, nmat), dim3(64), 0, s, mats, kb);
+        const bool f32 = sizeof(T) == 4;
+        // FP32 left-looking parts: the panel kernel of column block kb - 1 factored block kb
+        if (!(left && f32 && kb > k0)) hipLaunchKernelGGL(k_chol_diag_m<T>, dim3(1, nmat), dim3(64), 0, s, mats, kb);
         if (rest <= 0) continue;
         if (left) {
-            hipLaunchKernelGGL(k_chol_lpanel_m<T>, dim3(rest, nmat), dim3(256), 0, s, mats, kb, k0, kend);
+            if (f32)
+                hipLaunchKernelGGL(k_chol_lpanel_bf16, dim3(nmat, rest), dim3(256), 0, s, mats, kb, k0, kend);
+            else
+                hipLaunchKernelGGL(k_chol_lpanel_m<T>, dim3(rest, nmat), dim3(256), 0, s, mats, kb, k0, kend);
             continue;
         }
         hipLaunchKernelGGL(k_chol_panel_m<T>, dim3(rest, nmat), dim3(256), 0, s, mats, kb);
@@ -2054,8 +2303,8 @@ static void chol_subpanel(hipStream_t s, const MatDesc* mats, int nmat, int max_
 // matrices than the right-looking pair), each part after the first brought up to date by a strip
 // update from the earlier parts, then one rank-1024 BF16-split trailing update.  Per batch of 256
 // 3600 x 3600 matrices the factorization took 33.7 ms with a rank-256 update per 256 columns,
-// 30.0 ms with 512-wide panels and 29.4 ms with 1024-wide ones (2048: 29.1, one panel: 30.1); the
-// arithmetic, and so the result, is the same bit for bit.  FP64: 256-wide panels factored with the
+// 30.0 ms with 512-wide panels and 29.4 ms with 1024-wide ones (2048: 29.1, one panel: 30.1; FP32
+// MFMA in the panel kernel then); the arithmetic, and so the result, is the same bit for bit.  FP64: 256-wide panels factored with the
 // right-looking panel and panel-update kernels, whose three times more workgroups fill the GPU
 // better when one matrix is factored alone (0.25 ms faster for one 3600 x 3600 matrix), and the
 // FP64 trailing update.  The schedule depends on the element type only, never on the batch, so an
