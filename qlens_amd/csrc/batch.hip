@@ -726,20 +726,50 @@ __device__ __forceinline__ int wave_max_i(int v)
     return v;
 }
 
-// One support block cb of a chunk by columns (the kernel's per-block body, also the mixed path's
-// fallback): wave w takes the columns (cb << cbs) + w + 4 l; lane = chunk row.  Writes the block's
-// image into `tile` (no barrier inside).
+// Tap table in LDS: tap (tp, tq) of the PSF at (tp + toff) * tpc + tq.  Padded form (toff =
+// LB_SPAN + 1, tpc >= pny + LB_SPAN + 1, pnx + 2 LB_SPAN + 2 rows, zero outside the PSF): for a chunk
+// whose rows span at most LB_SPAN + 1 pixels in each direction, every kept L entry (i, j) of the
+// chunk's reach and every row (k, l) of it give tp = k + hx - i in [-LB_SPAN, pnx - 1 + LB_SPAN] and
+// tq = l + hy - j in [-LB_SPAN, pny - 1 + LB_SPAN], so the flat index K(k, l) - E(i, j), with
+// K = (k + hx + toff) tpc + l + hy and E = i tpc + j, lands on the tap when it lies in the PSF and
+// on a zero otherwise (a tq past either edge falls on the zero columns of that row or the one
+// before): no bounds test, and the zero products add exact zeros, so the sums are the bounds-tested
+// ones bit for bit.  Other chunks (and PSFs too large for the padded table: compact form, toff = 0,
+// tpc = pny) test the bounds per entry.
+#define LB_SPAN 15
+
 template <typename T>
-__device__ __forceinline__ void lb_block_scalar(const BatchDev& bd, const ItemMeta& M, const T* psf_l, char* tile, int cb, int w,
-                                                int lane, int p, int kr2, int lr2, int i_lo, unsigned i_span, int j_lo,
-                                                unsigned j_span, double sc, double dv, const unsigned* cpix, const double* cval,
-                                                const int* cptr)
+struct LbEnt {
+    int e;
+    T v;
+};
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Half h (CB/2 columns) of support block cb of a chunk by columns: wave w takes the half's columns
+// w + 4 l; lane = chunk row.  Per column the wave walks the column's entries in L's CSC form, keeps
+// those whose pixel lies in the chunk's PSF reach (a ballot over 64 entries at a time) and every lane
+// adds its own tap weight times each kept value, in the entries' order.  Fast chunks pass the kept
+// entries (E, value) through the wave's LDS slots (ent) and read the padded table at K - E; the
+// others broadcast each entry by readlane and test the tap's bounds.  Writes the half block's image
+// into `tile` (no barrier inside).
+template <typename T>
+__device__ __forceinline__ void lb_half(const BatchDev& bd, const ItemMeta& M, const T* tab, int toff, int tpc, LbEnt<T>* ent,
+                                        char* tile, int cb, int h, int w, int lane, int p, int kr2, int lr2, int K, bool fast,
+                                        int i_lo, unsigned i_span, int j_lo, unsigned j_span, double sc, double dv,
+                                        const unsigned* cpix, const double* cval, const int* cptr)
 {
     constexpr bool F64 = sizeof(T) == 8;
-    const int cbs = M.cbs, CB = 1 << cbs;
+    const int cbs = M.cbs, CB = 1 << cbs, HB = CB >> 1;
     const int pnx = bd.pnx, pny = bd.pny;
-    const int ncol = CB / 4;
-    const int cl = (cb << cbs) + w + 4 * lane;
+    const int ncol = HB / 4;
+    const int c0h = (cb << cbs) + h * HB;
+    const int cl = c0h + w + 4 * lane;
     int lb0 = 0, lb1 = 0;
     if (lane < ncol && cl < M.n_src) {
         lb0 = cptr[cl];
@@ -781,39 +811,67 @@ __device__ __forceinline__ void lb_block_scalar(const BatchDev& bd, const ItemMe
                              (unsigned)((int)(pkc & 0xffffu) - j_lo) <= j_span;
             const T v = (T)vc;
             unsigned long long bal = __ballot(rel);
-            while (bal) {
-                const int b = __builtin_ctzll(bal);
-                bal &= bal - 1;
-                const unsigned pkb = (unsigned)__builtin_amdgcn_readlane((int)pkc, b);
-                const T vb = readlane_t<T>(v, b);
-                const int tp = kr2 - (int)(pkb >> 16), tq = lr2 - (int)(pkb & 0xffffu);
-                if (p >= 0 && (unsigned)tp < (unsigned)pnx && (unsigned)tq < (unsigned)pny) acc += psf_l[tp * pny + tq] * vb;
+            if (fast) {
+                if (rel) {
+                    const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+                    ent[pos].e = ((int)(pkc >> 16) * tpc + (int)(pkc & 0xffffu)) * (int)sizeof(T);
+                    ent[pos].v = v;
+                }
+                wave_lds_sync();
+                const int n = __popcll(bal);
+                const char* tb = (const char*)tab + K * (int)sizeof(T);  // byte offsets: tap at tb - E
+                int k = 0;
+                for (; k + 4 <= n; k += 4) {  // four entries' reads in flight together, summed in order
+                    const LbEnt<T> x0 = ent[k], x1 = ent[k + 1], x2 = ent[k + 2], x3 = ent[k + 3];
+                    const T t0 = *(const T*)(tb - x0.e), t1 = *(const T*)(tb - x1.e);
+                    const T t2 = *(const T*)(tb - x2.e), t3 = *(const T*)(tb - x3.e);
+                    acc += t0 * x0.v;
+                    acc += t1 * x1.v;
+                    acc += t2 * x2.v;
+                    acc += t3 * x3.v;
+                }
+                for (; k < n; k++) acc += *(const T*)(tb - ent[k].e) * ent[k].v;
+                wave_lds_sync();
+            } else {
+                while (bal) {
+                    const int b = __builtin_ctzll(bal);
+                    bal &= bal - 1;
+                    const unsigned pkb = (unsigned)__builtin_amdgcn_readlane((int)pkc, b);
+                    const T vb = readlane_t<T>(v, b);
+                    const int tp = kr2 - (int)(pkb >> 16), tq = lr2 - (int)(pkb & 0xffffu);
+                    if (p >= 0 && (unsigned)tp < (unsigned)pnx && (unsigned)tq < (unsigned)pny) acc += tab[(tp + toff) * tpc + tq] * vb;
+                }
             }
         }
-        const int c_ = (cb << cbs) + ii;
+        if (p < 0) acc = (T)0;  // rows past the image: the fast path read row 0's taps for them
+        const int c_ = c0h + ii;
         if constexpr (F64) {
-            ((double*)tile)[lane * TS + ii] = c_ == M.n_src ? dv : (double)acc;
+            ((double*)tile)[lane * HB + ii] = c_ == M.n_src ? dv : (double)acc;
         } else {
             const double x = c_ == M.n_src ? dv : (double)acc * sc;
-            ((uint16_t*)tile)[((lane >> 2) * CB + ii) * 4 + (lane & 3)] = f32_to_f16_rne((float)x);
+            ((uint16_t*)tile)[((lane >> 2) * HB + ii) * 4 + (lane & 3)] = f32_to_f16_rne((float)x);
         }
     }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int psf_bytes)
+__global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int tab_bytes, int toff, int tpc, int trows)
 {
     extern __shared__ double lds_raw[];
     const int item = blockIdx.y, kt = blockIdx.x;
     const ItemMeta& M = bd.meta[item];
     if (!M.valid || kt >= M.nkt) return;
     constexpr bool F64 = sizeof(T) == 8;
-    const int cbs = M.cbs, CB = 1 << cbs;
+    const int cbs = M.cbs, CB = 1 << cbs, HB = CB >> 1;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    const int pnx = bd.pnx, pny = bd.pny, ntap = pnx * pny;
-    T* psf_l = (T*)lds_raw;
-    char* tile = (char*)lds_raw + psf_bytes;
-    for (int t = tid; t < ntap; t += 256) psf_l[t] = (T)bd.psf[t];
+    const int pnx = bd.pnx, pny = bd.pny;
+    T* tab = (T*)lds_raw;
+    LbEnt<T>* ents = (LbEnt<T>*)((char*)lds_raw + tab_bytes);
+    char* tile = (char*)(ents + 4 * 64);
+    for (int t = tid; t < trows * tpc; t += 256) {
+        const int tr = t / tpc, tq = t - tr * tpc, tp = tr - toff;
+        tab[t] = (tp >= 0 && tp < pnx && tq < pny) ? (T)bd.psf[tp * pny + tq] : (T)0;
+    }
     // this lane's row
     const int rs = kt * TS + lane;
     const int p = rs < M.n_img ? bd.act_pix[item * bd.s_pix + bd.perm[item * bd.s_pix + rs]] : -1;
@@ -825,6 +883,9 @@ __global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int psf_bytes)
     const int i_lo = kmin + hx - pnx + 1, j_lo = lmin + hy - pny + 1;
     const unsigned i_span = (unsigned)(kmax - kmin + pnx - 1), j_span = (unsigned)(lmax - lmin + pny - 1);
     const int kr2 = kr + hx, lr2 = lr + hy;  // tap (kr2 - i, lr2 - j) of pixel (i, j)
+    const bool fast = toff > 0 && kmax - kmin <= LB_SPAN && lmax - lmin <= LB_SPAN;
+    int K = (kr2 + toff) * tpc + lr2;
+    K = p >= 0 ? K : __builtin_amdgcn_readlane(K, 0);  // lane 0 holds a row of every chunk
     double sc = 0.0, dv = 0.0;
     if (p >= 0) {
         if constexpr (F64) {
@@ -841,23 +902,29 @@ __global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int psf_bytes)
     const unsigned* cpix = bd.cpix + eb;
     const double* cval = bd.cval + eb;
     const int* cptr = bd.boff + item * (bd.s_cell + 2);
+    LbEnt<T>* ent = ents + 64 * w;
     __syncthreads();
+    // a half block's image: runs of HB * 8 bytes (FP64: 64 rows; FP16: 16 groups of 4 rows, the
+    // k4-interleaved layout), each landing at byte 8 (run * CB + h * HB) of the pool block
+    const int nrun = F64 ? TS : TS / 4, upr = HB / 2;
     int q = 0;
     for (int wd = 0; wd < M.ncbw; wd++) {
         unsigned bits = m[wd];
         while (bits) {
             const int cb = wd * 32 + __builtin_ctz(bits);
             bits &= bits - 1;
-            lb_block_scalar<T>(bd, M, psf_l, tile, cb, w, lane, p, kr2, lr2, i_lo, i_span, j_lo, j_span, sc, dv, cpix, cval, cptr);
-            __syncthreads();
-            // the tile is the pool block image: 256 threads copy it with 16-byte stores
-            {
-                const int nvec = (int)(bsz * (F64 ? 8 : 2) / 16);  // 16-byte vectors per block (tile_bytes / 16)
-                uint4* dst = (uint4*)((char*)bd.lb + (M.lb_off + (size_t)(sbase + q) * bsz) * (F64 ? 8 : 2));
+            uint4* dst = (uint4*)((char*)bd.lb + (M.lb_off + (size_t)(sbase + q) * bsz) * (F64 ? 8 : 2));
+            for (int h = 0; h < 2; h++) {
+                lb_half<T>(bd, M, tab, toff, tpc, ent, tile, cb, h, w, lane, p, kr2, lr2, K, fast, i_lo, i_span, j_lo,
+                           j_span, sc, dv, cpix, cval, cptr);
+                __syncthreads();
                 const uint4* src = (const uint4*)tile;
-                for (int v = tid; v < nvec; v += 256) dst[v] = src[v];
+                for (int u = tid; u < nrun * upr; u += 256) {
+                    const int run = u >> (cbs - 2), o = u & (upr - 1);  // upr = 2^(cbs - 2)
+                    dst[(run * CB + h * HB) / 2 + o] = src[u];
+                }
+                __syncthreads();
             }
-            __syncthreads();
             q++;
         }
     }
@@ -868,17 +935,25 @@ int launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_n
     if (max_rows <= 0 || max_nsup <= 0) return 0;
     const int max_nkt = (max_rows + TS - 1) / TS;
     const size_t esz = mixed ? 4 : 8;
-    const size_t ntap = (size_t)bd.pnx * bd.pny;
-    const size_t tile_bytes = (size_t)TS * cbw * (mixed ? 2 : 8);
-    const size_t psf_bytes = (ntap * esz + 15) / 16 * 16;
-    if (psf_bytes > 64 * 1024) return -1;  // PSF beyond the LDS budget
-    const size_t lds = psf_bytes + tile_bytes;
+    // padded tap table when it fits in 32 KB, else the compact PSF (bounds tested everywhere)
+    int tpc = bd.pny + LB_SPAN + 1;
+    tpc = (tpc + 7) / 16 * 16 + 8;  // odd multiple of 8: the rows of an 8 x 8 pixel tile's taps on distinct banks
+    int toff = LB_SPAN + 1, trows = bd.pnx + 2 * LB_SPAN + 2;
+    if ((size_t)trows * tpc * esz > 32 * 1024) {
+        tpc = bd.pny;
+        toff = 0;
+        trows = bd.pnx;
+    }
+    const size_t tab_bytes = ((size_t)trows * tpc * esz + 15) / 16 * 16;
+    if (tab_bytes > 64 * 1024) return -1;  // PSF beyond the LDS budget
+    const size_t tile_bytes = (size_t)TS * (cbw / 2) * (mixed ? 2 : 8);
+    const size_t lds = tab_bytes + 4 * 64 * (mixed ? sizeof(LbEnt<float>) : sizeof(LbEnt<double>)) + tile_bytes;
     if (mixed) {
         allow_big_lds((const void*)k_build_lb_b<float>);
-        hipLaunchKernelGGL(k_build_lb_b<float>, dim3(max_nkt, bd.B), dim3(256), lds, s, bd, (int)psf_bytes);
+        hipLaunchKernelGGL(k_build_lb_b<float>, dim3(max_nkt, bd.B), dim3(256), lds, s, bd, (int)tab_bytes, toff, tpc, trows);
     } else {
         allow_big_lds((const void*)k_build_lb_b<double>);
-        hipLaunchKernelGGL(k_build_lb_b<double>, dim3(max_nkt, bd.B), dim3(256), lds, s, bd, (int)psf_bytes);
+        hipLaunchKernelGGL(k_build_lb_b<double>, dim3(max_nkt, bd.B), dim3(256), lds, s, bd, (int)tab_bytes, toff, tpc, trows);
     }
     return 0;
 }
