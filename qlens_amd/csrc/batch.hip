@@ -760,9 +760,9 @@ __device__ __forceinline__ void wave_lds_sync()
 // into `tile` (no barrier inside).
 template <typename T>
 __device__ __forceinline__ void lb_half(const BatchDev& bd, const ItemMeta& M, const T* tab, int toff, int tpc, LbEnt<T>* ent,
-                                        char* tile, int cb, int h, int w, int lane, int p, int kr2, int lr2, int K, bool fast,
-                                        int i_lo, unsigned i_span, int j_lo, unsigned j_span, double sc, double dv,
-                                        const unsigned* cpix, const double* cval, const int* cptr)
+                                        const unsigned* colbits, char* tile, int cb, int h, int w, int lane, int p, int kr2,
+                                        int lr2, int K, bool fast, int i_lo, unsigned i_span, int j_lo, unsigned j_span,
+                                        double sc, double dv, const unsigned* cpix, const double* cval, const int* cptr)
 {
     constexpr bool F64 = sizeof(T) == 8;
     const int cbs = M.cbs, CB = 1 << cbs, HB = CB >> 1;
@@ -775,23 +775,44 @@ __device__ __forceinline__ void lb_half(const BatchDev& bd, const ItemMeta& M, c
         lb0 = cptr[cl];
         lb1 = cptr[cl + 1];
     }
-    int ce0 = __builtin_amdgcn_readlane(lb0, 0), ce1 = __builtin_amdgcn_readlane(lb1, 0);
+    // the wave's columns with an entry in the chunk's reach (and the data column n_src): the others
+    // sum nothing, and their outputs are written directly
+    const bool live = lane < ncol && (cl == M.n_src || (cl < M.n_src && ((colbits[cl >> 5] >> (cl & 31)) & 1u)));
+    unsigned long long lmask = __ballot(live);
+    int nxt = lmask ? __builtin_ctzll(lmask) : ncol;
+    int ce0 = 0, ce1 = 0;
     unsigned pk = 0;
     double vd = 0.0;
-    if (ce0 + lane < ce1) {
-        pk = cpix[ce0 + lane];
-        vd = cval[ce0 + lane];
+    if (nxt < ncol) {
+        ce0 = __builtin_amdgcn_readlane(lb0, nxt);
+        ce1 = __builtin_amdgcn_readlane(lb1, nxt);
+        if (ce0 + lane < ce1) {
+            pk = cpix[ce0 + lane];
+            vd = cval[ce0 + lane];
+        }
     }
     for (int l = 0; l < ncol; l++) {
         const int ii = w + 4 * l;
+        if (l != nxt) {  // no entry in reach: acc = 0
+            const int c_ = c0h + ii;
+            if constexpr (F64) {
+                ((double*)tile)[lane * HB + ii] = c_ == M.n_src ? dv : 0.0;
+            } else {
+                const double x = c_ == M.n_src ? dv : 0.0 * sc;
+                ((uint16_t*)tile)[((lane >> 2) * HB + ii) * 4 + (lane & 3)] = f32_to_f16_rne((float)x);
+            }
+            continue;
+        }
+        lmask &= lmask - 1;
+        nxt = lmask ? __builtin_ctzll(lmask) : ncol;
         const int e0c = ce0, e1c = ce1;
         unsigned pkc = pk;
         double vc = vd;
         pk = 0;
         vd = 0.0;
-        if (l + 1 < ncol) {
-            ce0 = __builtin_amdgcn_readlane(lb0, l + 1);
-            ce1 = __builtin_amdgcn_readlane(lb1, l + 1);
+        if (nxt < ncol) {
+            ce0 = __builtin_amdgcn_readlane(lb0, nxt);
+            ce1 = __builtin_amdgcn_readlane(lb1, nxt);
             if (ce0 + lane < ce1) {
                 pk = cpix[ce0 + lane];
                 vd = cval[ce0 + lane];
@@ -855,7 +876,7 @@ __device__ __forceinline__ void lb_half(const BatchDev& bd, const ItemMeta& M, c
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int tab_bytes, int toff, int tpc, int trows)
+__global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int tab_bytes, int toff, int tpc, int trows, int nbw)
 {
     extern __shared__ double lds_raw[];
     const int item = blockIdx.y, kt = blockIdx.x;
@@ -867,7 +888,9 @@ __global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int tab_bytes, 
     const int pnx = bd.pnx, pny = bd.pny;
     T* tab = (T*)lds_raw;
     LbEnt<T>* ents = (LbEnt<T>*)((char*)lds_raw + tab_bytes);
-    char* tile = (char*)(ents + 4 * 64);
+    unsigned* colbits = (unsigned*)(ents + 4 * 64);  // nbw words: columns with an entry in the reach
+    char* tile = (char*)(colbits + nbw);
+    for (int t = tid; t < nbw; t += 256) colbits[t] = 0u;
     for (int t = tid; t < trows * tpc; t += 256) {
         const int tr = t / tpc, tq = t - tr * tpc, tp = tr - toff;
         tab[t] = (tp >= 0 && tp < pnx && tq < pny) ? (T)bd.psf[tp * pny + tq] : (T)0;
@@ -903,6 +926,27 @@ __global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int tab_bytes, 
     const double* cval = bd.cval + eb;
     const int* cptr = bd.boff + item * (bd.s_cell + 2);
     LbEnt<T>* ent = ents + 64 * w;
+    __syncthreads();  // colbits cleared
+    // the columns of the L rows of the active pixels in the chunk's reach (the rectangle the CSC
+    // filter below keeps): exactly the columns with at least one kept entry
+    {
+        const int U = 3 * bd.nsub, wj = (int)j_span + 1;
+        const int nrect = ((int)i_span + 1) * wj;
+        const int* pidx = bd.pidx + item * bd.s_pix;
+        const int* ucnt = bd.ucnt + item * bd.s_pix;
+        const int* ucol = bd.ucol + (size_t)item * bd.s_pix * U;
+        for (int t = tid; t < nrect; t += 256) {
+            const int i = i_lo + t / wj, j = j_lo + t % wj;
+            if (i < 0 || i >= bd.nx || j < 0 || j >= bd.ny) continue;
+            const int r = pidx[(size_t)i * bd.ny + j];
+            if (r < 0) continue;
+            const int nu = ucnt[r];
+            for (int e = 0; e < nu; e++) {
+                const int c = ucol[(size_t)r * U + e];
+                atomicOr(&colbits[c >> 5], 1u << (c & 31));
+            }
+        }
+    }
     __syncthreads();
     // a half block's image: runs of HB * 8 bytes (FP64: 64 rows; FP16: 16 groups of 4 rows, the
     // k4-interleaved layout), each landing at byte 8 (run * CB + h * HB) of the pool block
@@ -915,8 +959,8 @@ __global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int tab_bytes, 
             bits &= bits - 1;
             uint4* dst = (uint4*)((char*)bd.lb + (M.lb_off + (size_t)(sbase + q) * bsz) * (F64 ? 8 : 2));
             for (int h = 0; h < 2; h++) {
-                lb_half<T>(bd, M, tab, toff, tpc, ent, tile, cb, h, w, lane, p, kr2, lr2, K, fast, i_lo, i_span, j_lo,
-                           j_span, sc, dv, cpix, cval, cptr);
+                lb_half<T>(bd, M, tab, toff, tpc, ent, colbits, tile, cb, h, w, lane, p, kr2, lr2, K, fast, i_lo, i_span,
+                           j_lo, j_span, sc, dv, cpix, cval, cptr);
                 __syncthreads();
                 const uint4* src = (const uint4*)tile;
                 for (int u = tid; u < nrun * upr; u += 256) {
@@ -947,13 +991,14 @@ int launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_n
     const size_t tab_bytes = ((size_t)trows * tpc * esz + 15) / 16 * 16;
     if (tab_bytes > 64 * 1024) return -1;  // PSF beyond the LDS budget
     const size_t tile_bytes = (size_t)TS * (cbw / 2) * (mixed ? 2 : 8);
-    const size_t lds = tab_bytes + 4 * 64 * (mixed ? sizeof(LbEnt<float>) : sizeof(LbEnt<double>)) + tile_bytes;
+    const int nbw = (int)((bd.s_cell + 32) / 32 + 3) / 4 * 4;  // column bitmap words (16-byte multiple)
+    const size_t lds = tab_bytes + 4 * 64 * (mixed ? sizeof(LbEnt<float>) : sizeof(LbEnt<double>)) + (size_t)nbw * 4 + tile_bytes;
     if (mixed) {
         allow_big_lds((const void*)k_build_lb_b<float>);
-        hipLaunchKernelGGL(k_build_lb_b<float>, dim3(max_nkt, bd.B), dim3(256), lds, s, bd, (int)tab_bytes, toff, tpc, trows);
+        hipLaunchKernelGGL(k_build_lb_b<float>, dim3(max_nkt, bd.B), dim3(256), lds, s, bd, (int)tab_bytes, toff, tpc, trows, nbw);
     } else {
         allow_big_lds((const void*)k_build_lb_b<double>);
-        hipLaunchKernelGGL(k_build_lb_b<double>, dim3(max_nkt, bd.B), dim3(256), lds, s, bd, (int)tab_bytes, toff, tpc, trows);
+        hipLaunchKernelGGL(k_build_lb_b<double>, dim3(max_nkt, bd.B), dim3(256), lds, s, bd, (int)tab_bytes, toff, tpc, trows, nbw);
     }
     return 0;
 }
