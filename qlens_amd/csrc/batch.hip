@@ -416,21 +416,53 @@ void launch_sort_rows_b(hipStream_t s, const BatchDev& bd, int max_img, int max_
 // ------------------------------------------------------------------------------------------------
 // L by columns (CSC) for the Lb build: for every internal source column c, the entries (pixel, value)
 // of the merged L rows, ordered by the rows' tile-order position (a fixed order, so every sum over a
-// column's entries is reproducible).  Counts and bases reuse the bucket arrays (bcnt, boff, bfill:
-// s_cell + 2 ints per item); the unordered scatter goes to lcj (keys) and lv (values), the ordered
-// columns to cpix (pixel i << 16 | j) and cval.
+// column's entries is reproducible): cpix (pixel i << 16 | j) and cval, with the column sizes and
+// first entries in bcnt and boff (s_cell + 2 ints per item).
 // ------------------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_csc_count_b(BatchDev bd)
+// The columns are built in row-position order directly (a stable counting sort of the entries by
+// column): the positions are cut into segments of QLX_CSC_SEG; k_csc_hist_b counts each segment's
+// entries per column, k_csc_base_b turns the counts into each segment's first slot within the
+// column (segments in order) and the column's size, k_csc_scan_b places the columns, and
+// k_csc_place_b lets one wave per segment walk its rows in position order, each entry taking the
+// next slot of its column.
+__global__ void __launch_bounds__(256) k_csc_hist_b(BatchDev bd)
+{
+    extern __shared__ int hs[];  // n_src counters
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    const int seg = blockIdx.x, k0 = seg * QLX_CSC_SEG;
+    if (!M.valid || k0 >= M.n_img) return;
+    const int n = M.n_src, U = 3 * bd.nsub, tid = threadIdx.x;
+    for (int c = tid; c < n; c += 256) hs[c] = 0;
+    __syncthreads();
+    const int k1 = min(k0 + QLX_CSC_SEG, M.n_img);
+    const int* perm = bd.perm + item * bd.s_pix;
+    const int* ucnt = bd.ucnt + item * bd.s_pix;
+    const int* ucol = bd.ucol + (size_t)item * bd.s_pix * U;
+    for (int k = k0 + tid; k < k1; k += 256) {
+        const int r = perm[k], nu = ucnt[r];
+        for (int e = 0; e < nu; e++) atomicAdd(&hs[ucol[(size_t)r * U + e]], 1);
+    }
+    __syncthreads();
+    int* H = bd.csh + ((size_t)item * csc_segments(bd.s_pix) + seg) * bd.s_cell;
+    for (int c = tid; c < n; c += 256) H[c] = hs[c];
+}
+
+__global__ void __launch_bounds__(256) k_csc_base_b(BatchDev bd)
 {
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!M.valid || r >= M.n_img) return;
-    const int U = 3 * bd.nsub;
-    const int* uc = bd.ucol + ((size_t)item * bd.s_pix + r) * U;
-    const int cnt = bd.ucnt[item * bd.s_pix + r];
-    int* cc = bd.bcnt + item * (bd.s_cell + 2);
-    for (int e = 0; e < cnt; e++) atomicAdd(&cc[uc[e]], 1);
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!M.valid || c >= M.n_src) return;
+    const int nsg = (M.n_img + QLX_CSC_SEG - 1) / QLX_CSC_SEG;
+    int* H = bd.csh + (size_t)item * csc_segments(bd.s_pix) * bd.s_cell + c;
+    int run = 0;
+    for (int sg = 0; sg < nsg; sg++) {
+        const int v = H[(size_t)sg * bd.s_cell];
+        H[(size_t)sg * bd.s_cell] = run;
+        run += v;
+    }
+    bd.bcnt[item * (bd.s_cell + 2) + c] = run;
 }
 
 __global__ void __launch_bounds__(1024) k_csc_scan_b(BatchDev bd)
@@ -456,75 +488,79 @@ __global__ void __launch_bounds__(1024) k_csc_scan_b(BatchDev bd)
     if (threadIdx.x == 0) off[M.n_src] = base;
 }
 
-__global__ void __launch_bounds__(256) k_csc_scatter_b(BatchDev bd)
+// one wave per segment: the column cursors start at the segment's first slots; 64 rows at a time
+// are staged in LDS (lane l: row k + l's entries and pixel), then the rows are walked in order, lane
+// e placing entry e of the row (a merged row's columns are distinct)
+__global__ void __launch_bounds__(64) k_csc_place_b(BatchDev bd)
 {
-    const int item = blockIdx.y;
-    const ItemMeta& M = bd.meta[item];
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!M.valid || r >= M.n_img) return;
+    // dynamic LDS: the staged rows' values (64 x U doubles), columns (64 x U ints), then the n_src
+    // column cursors
+    extern __shared__ double place_raw[];
+    __shared__ int scnt[64];
+    __shared__ unsigned spk[64];
     const int U = 3 * bd.nsub;
-    const size_t rb = ((size_t)item * bd.s_pix + r) * U;
-    const int cnt = bd.ucnt[item * bd.s_pix + r];
-    const int* off = bd.boff + item * (bd.s_cell + 2);
-    int* fill = bd.bfill + item * (bd.s_cell + 2);
-    const unsigned pos_r = (unsigned)bd.key[item * bd.s_pix + r];
-    const size_t eb = (size_t)item * bd.s_pix * U;
-    for (int e = 0; e < cnt; e++) {
-        const int c = bd.ucol[rb + e];
-        const int o = off[c] + atomicAdd(&fill[c], 1);
-        bd.lcj[eb + o] = pos_r;
-        bd.lv[eb + o] = bd.uval[rb + e];
-    }
-}
-
-// one wave per column: rank by row position (positions are distinct within a column), write the
-// pixel coordinates and values in that order
-__global__ void __launch_bounds__(256) k_csc_order_b(BatchDev bd)
-{
+    double* sval = place_raw;
+    int* scol = (int*)(place_raw + 64 * U);
+    int* cur = scol + 64 * U;
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
-    if (!M.valid) return;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int seg = blockIdx.x, k0 = seg * QLX_CSC_SEG;
+    if (!M.valid || k0 >= M.n_img) return;
+    const int n = M.n_src, lane = threadIdx.x;
+    const int* H = bd.csh + ((size_t)item * csc_segments(bd.s_pix) + seg) * bd.s_cell;
     const int* off = bd.boff + item * (bd.s_cell + 2);
-    const size_t eb = (size_t)item * bd.s_pix * 3 * bd.nsub;
-    const unsigned* kk = bd.lcj + eb;
-    const double* vv = bd.lv + eb;
+    for (int c = lane; c < n; c += 64) cur[c] = off[c] + H[c];
+    const int k1 = min(k0 + QLX_CSC_SEG, M.n_img);
     const int* perm = bd.perm + item * bd.s_pix;
     const int* act = bd.act_pix + item * bd.s_pix;
-    for (int c = blockIdx.x * 4 + w; c < M.n_src; c += gridDim.x * 4) {
-        const int lo = off[c], n = off[c + 1] - lo;
-        if (n <= 64) {  // one entry per lane: ranks by broadcasting each key in turn
-            const unsigned k = lane < n ? kk[lo + lane] : 0xffffffffu;
-            int rank = 0;
-            for (int b = 0; b < n; b++) rank += ((unsigned)__builtin_amdgcn_readlane((int)k, b) < k) ? 1 : 0;
-            if (lane < n) {
-                const int p = act[perm[k]];
-                const unsigned i = (unsigned)(p / bd.ny), j = (unsigned)(p - (p / bd.ny) * bd.ny);
-                bd.cpix[eb + lo + rank] = (i << 16) | j;
-                bd.cval[eb + lo + rank] = vv[lo + lane];
+    const int* ucnt = bd.ucnt + item * bd.s_pix;
+    const int* ucol = bd.ucol + (size_t)item * bd.s_pix * U;
+    const double* uval = bd.uval + (size_t)item * bd.s_pix * U;
+    const size_t eb = (size_t)item * bd.s_pix * U;
+    unsigned* cpix = bd.cpix + eb;
+    double* cval = bd.cval + eb;
+    for (int kb = k0; kb < k1; kb += 64) {
+        const int k = kb + lane;
+        int nu = 0;
+        unsigned pk = 0;
+        if (k < k1) {
+            const int r = perm[k], p = act[r];
+            nu = ucnt[r];
+            pk = ((unsigned)(p / bd.ny) << 16) | (unsigned)(p - (p / bd.ny) * bd.ny);
+            for (int e = 0; e < nu; e++) {
+                scol[lane * U + e] = ucol[(size_t)r * U + e];
+                sval[lane * U + e] = uval[(size_t)r * U + e];
             }
-            continue;
         }
-        for (int a = lane; a < n; a += 64) {
-            const unsigned k = kk[lo + a];
-            int rank = 0;
-            for (int b = 0; b < n; b++) rank += (kk[lo + b] < k) ? 1 : 0;
-            const int p = act[perm[k]];
-            const unsigned i = (unsigned)(p / bd.ny), j = (unsigned)(p - (p / bd.ny) * bd.ny);
-            bd.cpix[eb + lo + rank] = (i << 16) | j;
-            bd.cval[eb + lo + rank] = vv[lo + a];
+        scnt[lane] = nu;
+        spk[lane] = pk;
+        __syncthreads();
+        const int nrow = min(64, k1 - kb);
+        for (int l = 0; l < nrow; l++) {
+            for (int e = lane; e < scnt[l]; e += 64) {  // U <= 48 entries per row
+                const int c = scol[l * U + e];
+                const int slot = cur[c];
+                cur[c] = slot + 1;
+                cpix[slot] = spk[l];
+                cval[slot] = sval[l * U + e];
+            }
         }
+        __syncthreads();
     }
 }
 
 void launch_lcols_b(hipStream_t s, const BatchDev& bd, int max_img, int max_src)
 {
     if (max_img <= 0 || max_src <= 0) return;
-    launch_fill_i32(s, bd.bcnt, 0, (size_t)bd.B * (bd.s_cell + 2));
-    hipLaunchKernelGGL(k_csc_count_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, s, bd);
+    const int nsg = (max_img + QLX_CSC_SEG - 1) / QLX_CSC_SEG;
+    const size_t lds = (size_t)bd.s_cell * 4;
+    allow_big_lds((const void*)k_csc_hist_b);
+    allow_big_lds((const void*)k_csc_place_b);
+    hipLaunchKernelGGL(k_csc_hist_b, dim3(nsg, bd.B), dim3(256), lds, s, bd);
+    hipLaunchKernelGGL(k_csc_base_b, dim3((max_src + 255) / 256, bd.B), dim3(256), 0, s, bd);
     hipLaunchKernelGGL(k_csc_scan_b, dim3(1, bd.B), dim3(1024), 0, s, bd);
-    hipLaunchKernelGGL(k_csc_scatter_b, dim3((max_img + 255) / 256, bd.B), dim3(256), 0, s, bd);
-    hipLaunchKernelGGL(k_csc_order_b, dim3(std::min((max_src + 3) / 4, 512), bd.B), dim3(256), 0, s, bd);
+    const size_t U = 3 * (size_t)bd.nsub;
+    hipLaunchKernelGGL(k_csc_place_b, dim3(nsg, bd.B), dim3(64), 64 * U * 12 + lds, s, bd);
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -91,7 +91,7 @@ struct Ctx {
     DevBuf d_models, d_sgrid, d_srcn, d_meta, d_mats;
     // per-item work (stride s_pix / s_cell)
     DevBuf beta, map_cell, map_w, pmapped, cmapped, sidx, tix, pisrc, pidx, act_pix, counts, ucol, uval, ucnt, nevals;
-    DevBuf apre, eptr, lcj, lv, cpix, cval, nfwtab, pmask, hmask;
+    DevBuf apre, eptr, csh, cpix, cval, nfwtab, pmask, hmask;
     DevBuf key, bcnt, boff, bfill, perm;
     // pools
     DevBuf lb, F, Ff, R, Rf, cvec, vec, nz, bbase, winv, model, out, status;  // Rf: R's copy for its factorization
@@ -340,7 +340,7 @@ void fill_source_grid(const double* box, int snx, int sny, double* g)
 size_t stage1_bytes(size_t s_pix, size_t s_cell, size_t nsub)
 {
     const size_t U = 3 * nsub;
-    return s_pix * nsub * (16 + 12 + 24) + s_pix * U * 28 + s_pix * 4 * 7 + (s_pix + 1) * 12 + s_cell * 4 * 5 + 64 + s_pix * 16 + QLX_NFWTAB_DOUBLES * 8 + s_pix * QLX_MAX_CBW * 4;
+    return s_pix * nsub * (16 + 12 + 24) + s_pix * U * 24 + (size_t)csc_segments(s_pix) * s_cell * 4 + s_pix * 4 * 7 + (s_pix + 1) * 12 + s_cell * 4 * 5 + 64 + s_pix * 16 + QLX_NFWTAB_DOUBLES * 8 + s_pix * QLX_MAX_CBW * 4;
 }
 
 // upper bound of the dense bytes of one item with at most ncell active source cells
@@ -560,8 +560,8 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
               c->pidx.ensure(B * s_pix * 4) && c->act_pix.ensure(B * s_pix * 4) &&
               c->counts.ensure((size_t)B * 16) && c->ucol.ensure(B * s_pix * U * 4) && c->uval.ensure(B * s_pix * U * 8) &&
               c->ucnt.ensure(B * s_pix * 4) && c->nevals.ensure((size_t)B * 8) && c->key.ensure(B * s_pix * 4) &&
-              c->apre.ensure(B * (s_pix + 1) * 4) && c->eptr.ensure(B * (s_pix + 1) * 4) && c->lcj.ensure(B * s_pix * U * 4) &&
-              c->lv.ensure(B * s_pix * U * 8) && c->cpix.ensure(B * s_pix * U * 4) && c->cval.ensure(B * s_pix * U * 8) &&
+              c->apre.ensure(B * (s_pix + 1) * 4) && c->eptr.ensure(B * (s_pix + 1) * 4) &&
+              c->csh.ensure(B * (size_t)csc_segments(s_pix) * s_cell * 4) && c->cpix.ensure(B * s_pix * U * 4) && c->cval.ensure(B * s_pix * U * 8) &&
               c->nfwtab.ensure((size_t)B * QLX_NFWTAB_DOUBLES * 8) &&
               c->pmask.ensure((size_t)B * s_pix * QLX_MAX_CBW * 4) && c->hmask.ensure((size_t)B * s_pix * QLX_MAX_CBW * 4) &&
               c->bcnt.ensure(B * (s_cell + 2) * 4) && c->boff.ensure(B * (s_cell + 2) * 4) && c->bfill.ensure(B * (s_cell + 2) * 4) &&
@@ -610,8 +610,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     bd.ucnt = c->ucnt.as<int>();
     bd.apre = c->apre.as<int>();
     bd.eptr = c->eptr.as<int>();
-    bd.lcj = c->lcj.as<unsigned>();
-    bd.lv = c->lv.as<double>();
+    bd.csh = c->csh.as<int>();
     bd.cpix = c->cpix.as<unsigned>();
     bd.cval = c->cval.as<double>();
     bd.nfwtab = c->nfwtab.as<double>();

@@ -17,6 +17,8 @@ namespace qlx {
 #define QLX_MAX_SRC_AXIS 512
 #define QLX_MAX_LD 16384          // padded source dimension limit of the dense F (n_src <= 16383)
 #define QLX_MAX_CBW 8             // 32-bit support-mask words per 64-row chunk (QLX_MAX_LD / 64 / 32)
+#define QLX_CSC_SEG 512           // row positions per segment of L's column-ordered (CSC) build
+__host__ __device__ inline long long csc_segments(long long s_pix) { return (s_pix + QLX_CSC_SEG - 1) / QLX_CSC_SEG; }
 // per-item source grid block: sxv[513], syv[513], scx[512], scy[512]
 #define QLX_SG_STRIDE 2052
 #define QLX_SG_SYV 513
@@ -112,8 +114,8 @@ struct BatchDev {
     int* ucnt;                  // s_pix
     int* apre;                  // s_pix+1: active pixels before pixel t = j*nx + i (image order j-outer)
     int* eptr;                  // s_pix+1: first row-order position of each 8 x 8 pixel tile
-    unsigned* lcj;              // s_pix*3*nsub: L by columns, unordered: row position of each entry
-    double* lv;                 // s_pix*3*nsub: L by columns, unordered: value of each entry
+    int* csh;                   // csc_segments(s_pix)*s_cell: per segment of QLX_CSC_SEG row positions,
+                                // the entries per L column, then the segment's first slot in it
     unsigned* cpix;             // s_pix*3*nsub: L by columns, ordered: pixel (i << 16 | j) of each entry
     double* cval;               // s_pix*3*nsub: L by columns, ordered: value of each entry
     unsigned* pmask;            // s_pix*QLX_MAX_CBW: column blocks touched by each active pixel's L row
