@@ -2767,30 +2767,31 @@ __device__ __forceinline__ double lane_sum16(const double* a)
     return e;
 }
 
-// r = D - F s for the 64 rows of block I, on F as formed (Ff32, lower 128-tiles; a diagonal tile is
-// held whole): the row strip (columns c < 64 I of rows i), the diagonal tile (through LDS, its upper
-// half) and the column strip (rows c >= 64 (I+1) of columns i, F(i,c) = F(c,i)); D(i) = F(n, i).
-// Products are formed and summed in FP64.
-__global__ void __launch_bounds__(256) k_resid_b(BatchDev bd)
+// r = D - F s on F as formed (Ff32, lower 128-tiles; a diagonal tile is held whole), reading each
+// lower element once: k_resid_rows_b takes row block I (64 rows): its row strip (columns c < 64 I
+// of rows i) gives the rows' sums, and the same loads give the strip's transposed products
+// z_I(c) = sum_i F(i, c) s(i), stored per row block (zres); with the diagonal tile (through LDS, its
+// upper half) the rows' partial sums go to the slot Yp.  k_resid_cols_b adds, for the rows of block
+// J, the column strip sum_{I > J} z_I in order of I, then r = D - (Fdiag s + that + Yp); D(i) =
+// F(n, i).  Products are formed and summed in FP64; rows >= n take no part.
+__global__ void __launch_bounds__(256) k_resid_rows_b(BatchDev bd)
 {
-    typedef double dvec4 __attribute__((ext_vector_type(4)));
     __shared__ float tile[TS][TS + 1];
-    __shared__ double cacc[16][TS];
     __shared__ double dacc[4][TS];
     __shared__ double racc[TS];
+    __shared__ double zs[4][256];
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     if (!M.valid || bd.status[4 * item + 3]) return;
     const int ld = M.ldN, n = M.n_src, nb = ld / TS;
-    const int I = blockIdx.x;
+    const int I = (int)gridDim.x - 1 - (int)blockIdx.x;  // the longest row strips are dispatched first
     if (I >= nb || I * TS >= n) return;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const float* U = bd.Ff32 + M.f_off;
     double* v = bd.vec + M.vec_off;
     const double* s = v;
-    const double* fdiag = v + 3 * (size_t)ld;
-    double* r = v + 4 * (size_t)ld;
-    double* part = v + 5 * (size_t)ld;
+    double* yp = v + 7 * (size_t)ld;
+    double* Z = bd.zres + M.f_off / 128 + (size_t)TS * I * (I - 1) / 2;  // z_I: c0 values
     const int c0 = I * TS;
     // diagonal tile
 #pragma unroll
@@ -2802,54 +2803,42 @@ __global__ void __launch_bounds__(256) k_resid_b(BatchDev bd)
         tile[row][cc + 2] = t.z;
         tile[row][cc + 3] = t.w;
     }
-    // column strip: F(i,c) = U[c][i] for c0 + 64 <= c < n; 16 lanes per row, 16 rows per pass
+    // row strip: F(i,c) = U[i][c] for c < c0; wave w owns rows c0 + w + 4k, k = 0..15, lane the four
+    // columns 256 it + 4 lane .. + 3 of pass it (every thread runs every pass: the passes' barriers)
     {
-        const int cv = 4 * (lane & 15);
-        const int rs = 4 * w + (lane >> 4);
-        dvec4 acc = (dvec4)0.0;
-        const float* Uc = U + c0 + cv;
-        int c = c0 + TS + rs;
-        for (; c + 7 * 16 < n; c += 8 * 16) {
-            float4_t t[8];
+        double ra[16], si[16];
 #pragma unroll
-            for (int u = 0; u < 8; u++) t[u] = *(const float4_t*)(Uc + (size_t)(c + 16 * u) * ld);
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const double sc = s[c + 16 * u];
-                acc.x = fma((double)t[u].x, sc, acc.x);
-                acc.y = fma((double)t[u].y, sc, acc.y);
-                acc.z = fma((double)t[u].z, sc, acc.z);
-                acc.w = fma((double)t[u].w, sc, acc.w);
-            }
+        for (int k = 0; k < 16; k++) {
+            ra[k] = 0.0;
+            const int row = c0 + w + 4 * k;
+            si[k] = row < n ? s[row] : 0.0;
         }
-        for (; c < n; c += 16) {
-            const float4_t t = *(const float4_t*)(Uc + (size_t)c * ld);
-            const double sc = s[c];
-            acc.x = fma((double)t.x, sc, acc.x);
-            acc.y = fma((double)t.y, sc, acc.y);
-            acc.z = fma((double)t.z, sc, acc.z);
-            acc.w = fma((double)t.w, sc, acc.w);
-        }
-        cacc[rs][cv] = acc.x;
-        cacc[rs][cv + 1] = acc.y;
-        cacc[rs][cv + 2] = acc.z;
-        cacc[rs][cv + 3] = acc.w;
-    }
-    // row strip: F(i,c) = U[i][c] for c < c0 (c0 is a multiple of 64, so c + 3 < c0 <= n); wave w
-    // owns rows c0 + w + 4k, k = 0..15
-    {
-        double ra[16];
-#pragma unroll
-        for (int k = 0; k < 16; k++) ra[k] = 0.0;
         const float* Ur = U + (size_t)(c0 + w) * ld;
-        for (int c = 4 * lane; c < c0; c += 256) {
-            const double s0 = s[c], s1 = s[c + 1], s2 = s[c + 2], s3 = s[c + 3];
-            float4_t t[16];
+        const int npass = (c0 + 255) / 256;
+        for (int it = 0; it < npass; it++) {
+            const int c = 256 * it + 4 * lane;
+            double z0 = 0.0, z1 = 0.0, z2 = 0.0, z3 = 0.0;
+            if (c < c0) {  // c0 is a multiple of 64, so c + 3 < c0
+                const double s0 = s[c], s1 = s[c + 1], s2 = s[c + 2], s3 = s[c + 3];
+                float4_t t[16];
 #pragma unroll
-            for (int k = 0; k < 16; k++) t[k] = *(const float4_t*)(Ur + (size_t)(4 * k) * ld + c);
+                for (int k = 0; k < 16; k++) t[k] = *(const float4_t*)(Ur + (size_t)(4 * k) * ld + c);
 #pragma unroll
-            for (int k = 0; k < 16; k++)
-                ra[k] = fma((double)t[k].x, s0, fma((double)t[k].y, s1, fma((double)t[k].z, s2, fma((double)t[k].w, s3, ra[k]))));
+                for (int k = 0; k < 16; k++) {
+                    ra[k] = fma((double)t[k].x, s0, fma((double)t[k].y, s1, fma((double)t[k].z, s2, fma((double)t[k].w, s3, ra[k]))));
+                    z0 = fma((double)t[k].x, si[k], z0);
+                    z1 = fma((double)t[k].y, si[k], z1);
+                    z2 = fma((double)t[k].z, si[k], z2);
+                    z3 = fma((double)t[k].w, si[k], z3);
+                }
+            }
+            zs[w][4 * lane] = z0;
+            zs[w][4 * lane + 1] = z1;
+            zs[w][4 * lane + 2] = z2;
+            zs[w][4 * lane + 3] = z3;
+            __syncthreads();
+            if (256 * it + tid < c0) Z[256 * it + tid] = ((zs[0][tid] + zs[1][tid]) + zs[2][tid]) + zs[3][tid];
+            __syncthreads();
         }
         const double tot = lane_sum16(ra);
         if ((lane & 3) == 0) racc[w + 4 * (lane >> 2)] = tot;
@@ -2867,29 +2856,47 @@ __global__ void __launch_bounds__(256) k_resid_b(BatchDev bd)
         dacc[g][i] = a;
     }
     __syncthreads();
-    if (tid < TS) {
-        const int i = c0 + tid;
-        double rr = 0.0, dd = 0.0;
-        if (i < n) {
-            double cs = 0.0;
-#pragma unroll
-            for (int q = 0; q < 16; q++) cs += cacc[q][tid];
-            const double ds = (dacc[0][tid] + dacc[1][tid]) + (dacc[2][tid] + dacc[3][tid]);
-            const double D = (double)U[(size_t)n * ld + i];
-            const double Fs = fdiag[i] * s[i] + cs + ds + racc[tid];
-            rr = D - Fs;
-            r[i] = rr;
-            rr = rr * rr;
-            dd = D * D;
-        }
-        for (int off = 32; off > 0; off >>= 1) {
-            rr += __shfl_xor(rr, off, 64);
-            dd += __shfl_xor(dd, off, 64);
-        }
-        if (tid == 0) {
-            part[I] = rr;
-            part[nb + I] = dd;
-        }
+    if (tid < TS && c0 + tid < n)
+        yp[c0 + tid] = ((dacc[0][tid] + dacc[1][tid]) + (dacc[2][tid] + dacc[3][tid])) + racc[tid];
+}
+
+__global__ void __launch_bounds__(64) k_resid_cols_b(BatchDev bd)
+{
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    if (!M.valid || bd.status[4 * item + 3]) return;
+    const int ld = M.ldN, n = M.n_src, nb = ld / TS;
+    const int J = blockIdx.x;
+    if (J >= nb || J * TS >= n) return;
+    const int nbr = (n + TS - 1) / TS;
+    const int tid = threadIdx.x;
+    const float* U = bd.Ff32 + M.f_off;
+    double* v = bd.vec + M.vec_off;
+    const double* s = v;
+    const double* fdiag = v + 3 * (size_t)ld;
+    double* r = v + 4 * (size_t)ld;
+    double* part = v + 5 * (size_t)ld;
+    const double* yp = v + 7 * (size_t)ld;
+    const double* Zb = bd.zres + M.f_off / 128;
+    const int i = J * TS + tid;
+    double rr = 0.0, dd = 0.0;
+    if (i < n) {
+        double cs = 0.0;
+        for (int I = J + 1; I < nbr; I++) cs += Zb[(size_t)TS * I * (I - 1) / 2 + i];
+        const double D = (double)U[(size_t)n * ld + i];
+        const double Fs = fdiag[i] * s[i] + cs + yp[i];
+        rr = D - Fs;
+        r[i] = rr;
+        rr = rr * rr;
+        dd = D * D;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        rr += __shfl_xor(rr, off, 64);
+        dd += __shfl_xor(dd, off, 64);
+    }
+    if (tid == 0) {
+        part[J] = rr;
+        part[nb + J] = dd;
     }
 }
 
@@ -2950,7 +2957,8 @@ void launch_refine_b(hipStream_t s, const BatchDev& bd, int max_ld, int iters, d
     const int nb = max_ld / TS;
     allow_big_lds((const void*)k_refine_b);
     for (int it = 0; it <= iters; it++) {
-        hipLaunchKernelGGL(k_resid_b, dim3(nb, bd.B), dim3(256), 0, s, bd);
+        hipLaunchKernelGGL(k_resid_rows_b, dim3(nb, bd.B), dim3(256), 0, s, bd);
+        hipLaunchKernelGGL(k_resid_cols_b, dim3(nb, bd.B), dim3(64), 0, s, bd);
         hipLaunchKernelGGL(k_refine_b, dim3(1, bd.B), dim3(1024), (size_t)max_ld * 4, s, bd, it == iters ? 1 : 0, tol);
     }
 }

@@ -94,7 +94,7 @@ struct Ctx {
     DevBuf apre, eptr, csh, cpix, cval, nfwtab, pmask, hmask;
     DevBuf key, bcnt, boff, bfill, perm;
     // pools
-    DevBuf lb, F, Ff, R, Rf, cvec, vec, nz, bbase, winv, model, out, status;  // Rf: R's copy for its factorization
+    DevBuf lb, F, Ff, Fz, R, Rf, cvec, vec, nz, bbase, winv, model, out, status;  // Fz: residual partials  // Rf: R's copy for its factorization
     DevBuf Fraw, d_lam;  // regparam optimisation: F as formed and the per-item lambda
     DevBuf agrid;        // automatic source grids: per item box and centre-ray count
     // solutions of the last call, kept on the device for qlx_get_source: per caller item a block
@@ -350,7 +350,7 @@ size_t dense_bytes_bound(size_t ncell, size_t snx)
     const size_t bwb = (2 * snx) / 64 + 1;
     const size_t f = ldN * ldN * 8;                     // FP64 F, or FP32 F and its formed copy
     const size_t r = 2 * nb * (bwb + 1) * 4096 * 8;     // R and the copy that is factored
-    return f + r + ldN * 8 * (QLX_VEC_SLOTS + 128);
+    return f + r + ldN * ldN / 16 + ldN * 8 * (QLX_VEC_SLOTS + 128);  // + the residual's tile partials
 }
 
 struct WaveItem {
@@ -774,15 +774,17 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     }
     const bool opt = S.optimize_regparam != 0 && !mixed;
     const size_t fixed_bytes = (size_t)B * stage1_bytes(s_pix, s_cell, nsub);
-    const size_t dense_bytes = f_tot * (opt ? 16 : 8) + r_tot * 16 + cv_tot * 8 + vec_tot * 8 + nz_tot * 4 + bb_tot * 4 + w_tot * 8;
+    const size_t dense_bytes = f_tot * (opt ? 16 : 8) + r_tot * 16 + cv_tot * 8 + vec_tot * 8 + nz_tot * 4 + bb_tot * 4 + w_tot * 8 +
+                               (mixed ? f_tot / 16 : 0);
     if (B > 1 && fixed_bytes + dense_bytes > budget) return WAVE_TOO_BIG;
     ok = c->R.ensure(r_tot * 8) && c->Rf.ensure(r_tot * 8) && c->cvec.ensure(cv_tot * 8) && c->vec.ensure(vec_tot * 8) && c->nz.ensure(nz_tot * 4) &&
          c->bbase.ensure(bb_tot * 4) && c->winv.ensure(w_tot * 8) && c->F.ensure(f_tot * (mixed ? 4 : 8)) &&
-         (!mixed || c->Ff.ensure(f_tot * 4));
+         (!mixed || (c->Ff.ensure(f_tot * 4) && c->Fz.ensure(f_tot / 16)));
     if (!ok) return fail(c, QLX_ERR_NOMEM, "dense buffer allocation failed (F/R)");
     bd.F = mixed ? nullptr : c->F.as<double>();
     bd.F32 = mixed ? c->F.as<float>() : nullptr;
     bd.Ff32 = mixed ? c->Ff.as<float>() : nullptr;
+    bd.zres = mixed ? c->Fz.as<double>() : nullptr;
     bd.R = c->R.as<double>();
     bd.cvec = c->cvec.as<double>();
     bd.vec = c->vec.as<double>();

@@ -27,8 +27,8 @@ __host__ __device__ inline long long csc_segments(long long s_pix) { return (s_p
 // per-item output slots
 #define QLX_OUT_STRIDE 10         // -2lnE, chisq, logdetF, logdetR, Es, regterms, refine_resid, refine_steps,
                                   // regparam objective (chi^2 over the active pixels + regterms + ln det F), spare
-#define QLX_VEC_SLOTS 7           // per item, ldN doubles each: s (F order), diagF, diagR, Fdiag, resid,
-                                  // partial sums, s (reference order)
+#define QLX_VEC_SLOTS 8           // per item, ldN doubles each: s (F order), diagF, diagR, Fdiag, resid,
+                                  // partial sums, s (reference order), the residual's row-strip sums
 // internal source order: active cells ranked in 8 (x) by 16 (y) cell tiles, tiles y-major
 #define QLX_TILE_W 8
 #define QLX_TILE_H 16
@@ -138,6 +138,8 @@ struct BatchDev {
     double* R;                  // banded R tiles
     double* cvec;
     double* vec;                // QLX_VEC_SLOTS * ldN doubles per item
+    double* zres;               // mixed: the refinement residual's transposed tile products, ldN^2 / 128
+                                // doubles per item (at f_off / 128)
     unsigned* nz;               // nkt * ncbw support words per item
     int* bbase;                 // nkt + 1 slot bases per item
     double* winv;               // per item 2 * ldN*64 doubles (F then R block inverses)
