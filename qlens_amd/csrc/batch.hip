@@ -1479,176 +1479,10 @@ struct Mfma16<float> {
     __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-// FP64 diagonal 64x64 block, one wave (measured ~10 us per launch faster in FP64 than the
-// readlane / right-looking form below): lane r keeps row r in registers.  Column step c: the pivot
-// and the column entries l_qc go through LDS (broadcast reads); every lane scales its own entry
-// and applies the rank-1 update to all of its entries right of c without branching (entries
-// above the diagonal collect values that are never stored).  Then lane c forms column c of L^-1
-// by forward substitution reading L from LDS.
-template <typename T>
-__device__ void chol_diag_wave_lds(const MatDesc& D, int kb, double* raw)
-{
-    T(*Ls)[TS + 1] = (T(*)[TS + 1])raw;
-    T* col = (T*)raw + TS * (TS + 1);
-    const int r = threadIdx.x;
-    const long long base = (long long)kb * TS;
-    int ld;
-    T* tile = tile_ptr<T>(D, kb, kb, ld);
-    // lane r loads row r straight into registers: all of its vector loads are in flight together
-    typedef typename Mfma16<T>::vec_t V;
-    constexpr int VW = sizeof(V) / sizeof(T);
-    T a[TS];
-#pragma unroll
-    for (int q = 0; q < TS; q += VW) {
-        const V v = *(const V*)&tile[(size_t)r * ld + q];
-        a[q] = v.x;
-        a[q + 1] = v.y;
-        if constexpr (VW == 4) {
-            a[q + 2] = v.z;
-            a[q + 3] = v.w;
-        }
-    }
-    bool bad = false;
-#pragma unroll
-    for (int c = 0; c < TS; c++) {
-        if (r == c) col[c] = a[c];
-        __syncthreads();
-        const T piv = col[c];
-        bad |= (!(piv > 0) && base + c < D.n);  // negative, zero or NaN pivot
-        const T d = sqrt(fabs(piv));
-        const T l = (r > c) ? a[c] / d : d;
-        a[c] = l;
-        col[r] = l;
-        __syncthreads();
-#pragma unroll
-        for (int q = c + 1; q < TS; q++) a[q] = fma(-l, col[q], a[q]);
-    }
-    if (r == 0 && bad) atomicOr(D.status, 1);
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < TS; q++) Ls[r][q] = (q <= r) ? a[q] : (T)0;
-    __syncthreads();
-#pragma unroll 16
-    for (int e = r; e < TS * TS; e += 64)
-        if ((e & 63) <= (e >> 6)) tile[(size_t)(e >> 6) * ld + (e & 63)] = Ls[e >> 6][e & 63];
-    ((T*)D.diag)[base + r] = Ls[r][r];
-    T x[TS];
-#pragma unroll
-    for (int i = 0; i < TS; i++) {
-        T sacc = (i == r) ? (T)1 : (T)0;
-#pragma unroll
-        for (int q = 0; q < i; q++) sacc = fma(-Ls[i][q], x[q], sacc);
-        x[i] = (r <= i) ? sacc / Ls[i][i] : (T)0;
-    }
-    T* W = (T*)D.W + (size_t)kb * TS * TS;
-#pragma unroll
-    for (int i = 0; i < TS; i++) W[i * TS + r] = x[i];
-}
-
 // value of lane `src` in every lane (src is a constant of the unrolled loops that call this)
 __device__ __forceinline__ float lane_bcast(float v, int src)
 {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
-}
-
-// FP32 diagonal 64x64 block, one wave: lane r keeps row r in registers.  Column step c: lane c's
-// pivot reaches every lane by readlane (no LDS round trip); d = |p| rsqrt|p| and the column
-// entries l_ic = a_ic rsqrt|p| are written to row c of Lt (Lt[c][i] = L(i, c), one LDS row per
-// column, so no step overwrites what another still reads); after one barrier every lane applies
-// the rank-1 update to its entries right of c with broadcast vector reads of Lt[c] (entries above
-// the diagonal collect values that are never stored).  L^-1 is then formed right-looking: lane r
-// holds column r of L^-1 and step q scales entry q by 1/L(q, q) and updates entries q+1..63 with
-// independent FMAs, so the only serial chain is one multiply per step.
-__device__ void chol_diag_wave_f32(const MatDesc& D, int kb, float (*Lt)[TS])
-{
-    const int r = threadIdx.x;
-    const long long base = (long long)kb * TS;
-    int ld;
-    typedef float T;
-    T* tile = tile_ptr<T>(D, kb, kb, ld);
-    // the first diagonal block comes from F as formed (A0), when the matrix has it
-    const T* src = (kb == 0 && D.A0) ? (const T*)D.A0 + (tile - (T*)D.A) : tile;
-    // lane r loads row r straight into registers: all of its vector loads are in flight together
-    typedef typename Mfma16<T>::vec_t V;
-    constexpr int VW = sizeof(V) / sizeof(T);
-    T a[TS];
-#pragma unroll
-    for (int q = 0; q < TS; q += VW) {
-        const V v = *(const V*)&src[(size_t)r * ld + q];
-        a[q] = v.x;
-        a[q + 1] = v.y;
-        if constexpr (VW == 4) {
-            a[q + 2] = v.z;
-            a[q + 3] = v.w;
-        }
-    }
-    bool bad = false;
-    T myd = 0, myrd = 0;
-#pragma unroll
-    for (int c = 0; c < TS; c++) {
-        const T piv = lane_bcast(a[c], c);
-        bad |= (!(piv > 0) && base + c < D.n);  // negative, zero or NaN pivot
-        const T pa = fabs(piv);
-        const T rd = rsqrtf(pa);
-        const T d = (pa > 0) ? pa * rd : (T)0;
-        if (r == c) {
-            myd = d;
-            myrd = rd;
-        }
-        const T l = (r > c) ? a[c] * rd : d;
-        a[c] = l;
-        Lt[c][r] = l;
-        __syncthreads();
-#pragma unroll
-        for (int q = c + 1; q < TS; q++) a[q] = fma(-l, Lt[c][q], a[q]);
-    }
-    if (r == 0 && bad) atomicOr(D.status, 1);
-    // row r of the lower triangle straight from registers
-#pragma unroll
-    for (int q = 0; q < TS; q += VW) {
-        T* dst = tile + (size_t)r * ld + q;
-        if (q + VW - 1 <= r) {
-            V v;
-            v.x = a[q];
-            v.y = a[q + 1];
-            if constexpr (VW == 4) {
-                v.z = a[q + 2];
-                v.w = a[q + 3];
-            }
-            *(V*)dst = v;
-        } else if (q <= r) {
-#pragma unroll
-            for (int j = 0; j < VW; j++)
-                if (q + j <= r) dst[j] = a[q + j];
-        }
-    }
-    ((T*)D.diag)[base + r] = myd;
-    T x[TS];
-#pragma unroll
-    for (int i = 0; i < TS; i++) x[i] = (i == r) ? (T)1 : (T)0;
-#pragma unroll
-    for (int q = 0; q < TS; q++) {
-        x[q] *= lane_bcast(myrd, q);
-#pragma unroll
-        for (int i = q + 1; i < TS; i++) x[i] = fma(-Lt[q][i], x[q], x[i]);
-    }
-    T* W = (T*)D.W + (size_t)kb * TS * TS;
-#pragma unroll
-    for (int i = 0; i < TS; i++) W[i * TS + r] = x[i];
-}
-
-template <typename T>
-__global__ void __launch_bounds__(64) k_chol_diag_m(const MatDesc* mats, int kb)
-{
-    const MatDesc D = mats[blockIdx.y];
-    if (kb >= D.nb) return;
-    if constexpr (sizeof(T) == 8) {
-        __shared__ double raw[TS * (TS + 1) + TS];
-        chol_diag_wave_lds<T>(D, kb, raw);
-    } else {
-        __shared__ __align__(16) T Lt[TS][TS];
-        chol_diag_wave_f32(D, kb, Lt);
-    }
 }
 
 // 64x64x64 MFMA product acc = sum_q A(i, q) B(j, q) with A and B stored "row holds the k-vector"
@@ -2251,14 +2085,16 @@ __device__ __forceinline__ void mma64_bf16(const __bf16* Ah, const __bf16* Al, c
     }
 }
 
-// The FP32 diagonal block factored by the four waves of a workgroup, with chol_diag_wave_f32's
-// arithmetic (the same operations on every element in the same order, so the same bits): lane r
-// of wave w holds a[j] = A(r, 16 w + j).  Column step c: the wave owning column c forms the pivot
-// and the column into Lt[c] (and 1/sqrt of the pivot into Rd[c]); after the barrier every wave with
-// columns right of c applies the rank-1 update to them.  L^-1 by the right-looking substitution,
-// lane r of wave w holding rows 16 w .. 16 w + 15 of column r: step q scales row q (its owner) and
-// publishes it in X[q], the waves below update their rows.  Writes L's lower triangle into the
-// block (row pitch ld), the pivots and W.  LDS: Lt and X 64 x 64 each, Rd 64.
+// The FP32 diagonal block factored by the four waves of a workgroup (one wave per 16-column panel;
+// the earlier one-wave form held 256 registers and its 64 column steps ran serially): lane r of wave
+// w holds a[j] = A(r, 16 w + j).  d = |p| rsqrt|p| and l = a rsqrt|p| for the pivot p.  Column c: its pivot is lane c's entry (readlane), the
+// column goes to Lt[c] (Lt[c][i] = L(i, c)) and 1/sqrt of the pivot to Rd[c], and every entry right
+// of c takes the rank-1 update.  L^-1 by the right-looking substitution, lane r of wave w holding
+// rows 16 w .. 16 w + 15 of column r (row q scaled by Rd[q], published in X[q], then the rows below
+// updated).  Both blocked by 16-column (row) panels, one per wave, one barrier per panel: wave p
+// does its own panel's steps, the waves after it then apply the panel's 16 updates in order.  Writes
+// L's lower triangle into the block (row pitch ld), the pivots and W.  LDS: Lt and X 64 x 64 each,
+// Rd 64.
 __device__ __forceinline__ void chol_diag_f32_4w(const MatDesc& D, int kb, float* tile, int ld, float (&a)[16],
                                                  float (*Lt)[TS], float (*X)[TS], float* Rd)
 {
@@ -2266,25 +2102,38 @@ __device__ __forceinline__ void chol_diag_f32_4w(const MatDesc& D, int kb, float
     const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
     const long long base = (long long)kb * TS;
     bool bad = false;
+    // factor, by 16-column panels: wave p factors its own panel (its columns' pivots, the columns into
+    // Lt, the rank-1 updates of its later columns), then the waves right of it apply the panel's 16
+    // updates in column order: each element sees the updates of columns 0, 1, ... in turn
 #pragma unroll
-    for (int c = 0; c < TS; c++) {
-        if (w == (c >> 4)) {
-            const T piv = lane_bcast(a[c & 15], c);
-            bad |= (!(piv > 0) && base + c < D.n);  // negative, zero or NaN pivot
-            const T pa = fabs(piv);
-            const T rd = rsqrtf(pa);
-            const T d = (pa > 0) ? pa * rd : (T)0;
-            const T l = (r > c) ? a[c & 15] * rd : d;
-            a[c & 15] = l;
-            Lt[c][r] = l;
-            if (r == c) Rd[c] = rd;
+    for (int p = 0; p < 4; p++) {
+        if (w == p) {
+#pragma unroll
+            for (int cc = 0; cc < 16; cc++) {
+                const int c = 16 * p + cc;
+                const T piv = lane_bcast(a[cc], c);
+                bad |= (!(piv > 0) && base + c < D.n);  // negative, zero or NaN pivot
+                const T pa = fabs(piv);
+                const T rd = rsqrtf(pa);
+                const T d = (pa > 0) ? pa * rd : (T)0;
+                const T l = (r > c) ? a[cc] * rd : d;
+                a[cc] = l;
+                Lt[c][r] = l;
+                if (r == c) Rd[c] = rd;
+                wave_lds_sync();
+#pragma unroll
+                for (int j = cc + 1; j < 16; j++) a[j] = fma(-l, Lt[c][16 * p + j], a[j]);
+            }
         }
         __syncthreads();
-        if (16 * w + 15 > c) {
-            const T l = Lt[c][r];
+        if (w > p) {
 #pragma unroll
-            for (int j = 0; j < 16; j++)
-                if (16 * w + j > c) a[j] = fma(-l, Lt[c][16 * w + j], a[j]);
+            for (int cc = 0; cc < 16; cc++) {
+                const int c = 16 * p + cc;
+                const T l = Lt[c][r];
+#pragma unroll
+                for (int j = 0; j < 16; j++) a[j] = fma(-l, Lt[c][16 * w + j], a[j]);
+            }
         }
     }
     if (r == 0 && bad) atomicOr(D.status, 1);  // bad is uniform in a wave (the pivots are broadcast)
@@ -2302,26 +2151,156 @@ __device__ __forceinline__ void chol_diag_f32_4w(const MatDesc& D, int kb, float
         }
     }
     if (w == 0) ((T*)D.diag)[base + r] = Lt[r][r];
+    // L^-1 by 16-row panels in the same way: wave p finalizes its rows (scale by 1/L(q, q), then the
+    // updates of its later rows), publishes them in X, and the waves below apply them in row order
     T x[16];
 #pragma unroll
     for (int j = 0; j < 16; j++) x[j] = (16 * w + j == r) ? (T)1 : (T)0;
 #pragma unroll
-    for (int q = 0; q < TS; q++) {
-        if (w == (q >> 4)) {
-            x[q & 15] *= Rd[q];
-            X[q][r] = x[q & 15];
+    for (int p = 0; p < 4; p++) {
+        if (w == p) {
+#pragma unroll
+            for (int qq = 0; qq < 16; qq++) {
+                const int q = 16 * p + qq;
+                x[qq] *= Rd[q];
+                X[q][r] = x[qq];
+#pragma unroll
+                for (int j = qq + 1; j < 16; j++) x[j] = fma(-Lt[q][16 * p + j], x[qq], x[j]);
+            }
         }
         __syncthreads();
-        if (16 * w + 15 > q) {
-            const T xq = X[q][r];
+        if (w > p) {
 #pragma unroll
-            for (int j = 0; j < 16; j++)
-                if (16 * w + j > q) x[j] = fma(-Lt[q][16 * w + j], xq, x[j]);
+            for (int qq = 0; qq < 16; qq++) {
+                const int q = 16 * p + qq;
+                const T xq = X[q][r];
+#pragma unroll
+                for (int j = 0; j < 16; j++) x[j] = fma(-Lt[q][16 * w + j], xq, x[j]);
+            }
         }
     }
     T* W = (T*)D.W + (size_t)kb * TS * TS;
 #pragma unroll
     for (int j = 0; j < 16; j++) W[(16 * w + j) * TS + r] = x[j];
+}
+
+// The FP64 diagonal block by the same four-wave panel scheme: d = sqrt|p|, l = a / d; row q of L^-1
+// divided by L(q, q), its updates applied in row order (the left-looking substitution's fused
+// multiply-adds in the same order, so the same bits as that form).  LDS: Lt
+// 64 x 64, X 2 x 16 x 64 (panel p's rows in X[p & 1]: the barrier after panel p + 1 orders their
+// readers before panel p + 2 rewrites them).
+__device__ __forceinline__ void chol_diag_f64_4w(const MatDesc& D, int kb, double* tile, int ld, double (&a)[16],
+                                                 double (*Lt)[TS], double (*X)[16][TS])
+{
+    typedef double T;
+    const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long long base = (long long)kb * TS;
+    bool bad = false;
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        if (w == p) {
+#pragma unroll
+            for (int cc = 0; cc < 16; cc++) {
+                const int c = 16 * p + cc;
+                const T piv = readlane_f64(a[cc], c);
+                bad |= (!(piv > 0) && base + c < D.n);  // negative, zero or NaN pivot
+                const T d = sqrt(fabs(piv));
+                const T l = (r > c) ? a[cc] / d : d;
+                a[cc] = l;
+                Lt[c][r] = l;
+                wave_lds_sync();
+#pragma unroll
+                for (int j = cc + 1; j < 16; j++) a[j] = fma(-l, Lt[c][16 * p + j], a[j]);
+            }
+        }
+        __syncthreads();
+        if (w > p) {
+#pragma unroll
+            for (int cc = 0; cc < 16; cc++) {
+                const int c = 16 * p + cc;
+                const T l = Lt[c][r];
+#pragma unroll
+                for (int j = 0; j < 16; j++) a[j] = fma(-l, Lt[c][16 * w + j], a[j]);
+            }
+        }
+    }
+    if (r == 0 && bad) atomicOr(D.status, 1);
+#pragma unroll
+    for (int j = 0; j < 16; j += 2) {
+        const int q = 16 * w + j;
+        T* dst = tile + (size_t)r * ld + q;
+        if (q + 1 <= r) {
+            *(double2*)dst = make_double2(a[j], a[j + 1]);
+        } else if (q <= r) {
+            dst[0] = a[j];
+        }
+    }
+    if (w == 0) ((T*)D.diag)[base + r] = Lt[r][r];
+    T x[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) x[j] = (16 * w + j == r) ? (T)1 : (T)0;
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        if (w == p) {
+#pragma unroll
+            for (int qq = 0; qq < 16; qq++) {
+                const int q = 16 * p + qq;
+                x[qq] = (r <= q) ? x[qq] / Lt[q][q] : (T)0;
+                X[p & 1][qq][r] = x[qq];
+#pragma unroll
+                for (int j = qq + 1; j < 16; j++) x[j] = fma(-Lt[q][16 * p + j], x[qq], x[j]);
+            }
+        }
+        __syncthreads();
+        if (w > p) {
+#pragma unroll
+            for (int qq = 0; qq < 16; qq++) {
+                const int q = 16 * p + qq;
+                const T xq = X[p & 1][qq][r];
+#pragma unroll
+                for (int j = 0; j < 16; j++) x[j] = fma(-Lt[q][16 * w + j], xq, x[j]);
+            }
+        }
+    }
+    T* W = (T*)D.W + (size_t)kb * TS * TS;
+#pragma unroll
+    for (int j = 0; j < 16; j++) W[(16 * w + j) * TS + r] = x[j];
+}
+
+// one diagonal block per matrix (blockIdx.y), four waves; FP32 blocks of the first column read F as
+// formed (A0), when the matrix has it
+template <typename T>
+__global__ void __launch_bounds__(256) k_chol_diag4_m(const MatDesc* mats, int kb)
+{
+    const MatDesc D = mats[blockIdx.y];
+    if (kb >= D.nb) return;
+    int ld;
+    T* tile = tile_ptr<T>(D, kb, kb, ld);
+    const T* src = (sizeof(T) == 4 && kb == 0 && D.A0) ? (const T*)D.A0 + (tile - (T*)D.A) : tile;
+    const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
+    typedef typename Mfma16<T>::vec_t V;
+    constexpr int VW = sizeof(V) / sizeof(T);
+    T a[16];
+#pragma unroll
+    for (int j = 0; j < 16; j += VW) {
+        const V v = *(const V*)&src[(size_t)r * ld + 16 * w + j];
+        a[j] = v.x;
+        a[j + 1] = v.y;
+        if constexpr (VW == 4) {
+            a[j + 2] = v.z;
+            a[j + 3] = v.w;
+        }
+    }
+    if constexpr (sizeof(T) == 8) {
+        __shared__ double Lt[TS][TS];
+        __shared__ double X[2][16][TS];
+        chol_diag_f64_4w(D, kb, tile, ld, a, Lt, X);
+    } else {
+        __shared__ __align__(16) float Lt[TS][TS];
+        __shared__ float X[TS][TS];
+        __shared__ float Rd[TS];
+        chol_diag_f32_4w(D, kb, tile, ld, a, Lt, X, Rd);
+    }
 }
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_chol_lpanel_bf16(const MatDesc* mats, int kb, int k0, int kend)
@@ -2421,7 +2400,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 template <typename T>
 static void chol_step(hipStream_t s, const MatDesc* mats, int nmat, int kb, int rest)
 {
-    hipLaunchKernelGGL(k_chol_diag_m<T>, dim3(1, nmat), dim3(64), 0, s, mats, kb);
+    hipLaunchKernelGGL(k_chol_diag4_m<T>, dim3(1, nmat), dim3(256), 0, s, mats, kb);
     if (rest > 0) {  // rest: the widest trailing extent of any matrix in the launch
         hipLaunchKernelGGL(k_chol_panel_m<T>, dim3(rest, nmat), dim3(256), 0, s, mats, kb);
         hipLaunchKernelGGL(k_chol_update_m<T>, dim3(syrk_grid(rest), nmat), dim3(256), 0, s, mats, kb);
@@ -2436,7 +2415,7 @@ static void chol_subpanel(hipStream_t s, const MatDesc* mats, int nmat, int max_
         const int rest = max_nb - kb - 1;
         const bool f32 = sizeof(T) == 4;
         // FP32 left-looking parts: the panel kernel of column block kb - 1 factored block kb
-        if (!(left && f32 && kb > k0)) hipLaunchKernelGGL(k_chol_diag_m<T>, dim3(1, nmat), dim3(64), 0, s, mats, kb);
+        if (!(left && f32 && kb > k0)) hipLaunchKernelGGL(k_chol_diag4_m<T>, dim3(1, nmat), dim3(256), 0, s, mats, kb);
         if (rest <= 0) continue;
         if (left) {
             if (f32)
