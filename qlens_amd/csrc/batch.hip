@@ -1381,6 +1381,80 @@ __global__ void __launch_bounds__(256) k_curvature_b(BatchDev bd)
     }
 }
 
+// R is a function of the grid's shape (snx, sny) and the item's active-index map (sidx): its entries
+// are sums of small integers, exact in any order.  Items with equal shapes and maps share one
+// factorization: k_rhash_b keys each item (an order-free sum of mixed (cell, index) words),
+// k_rdup_b compares an item with the earlier items of equal key, entry by entry, and takes the first
+// equal one as its representative (whose own representative is itself).
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x)
+{
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ULL;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebULL;
+    x ^= x >> 31;
+    return x;
+}
+
+__global__ void __launch_bounds__(256) k_rhash_b(BatchDev bd)
+{
+    __shared__ unsigned long long sh[256];
+    const int item = blockIdx.y;
+    const ItemMeta& M = bd.meta[item];
+    const int tid = threadIdx.x;
+    unsigned long long h = 0;
+    if (M.valid) {
+        const int* sidx = bd.sidx + item * bd.s_cell;
+        const int n = M.snx * M.sny;
+        for (int c = tid; c < n; c += 256) h += mix64(((unsigned long long)c << 32) ^ (unsigned)(sidx[c] + 1));
+    }
+    sh[tid] = h;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (tid < st) sh[tid] += sh[tid + st];
+        __syncthreads();
+    }
+    if (tid == 0) bd.rkey[item] = M.valid ? sh[0] ^ mix64(((unsigned long long)M.snx << 32) | (unsigned)M.sny) : 0ULL;
+}
+
+__global__ void __launch_bounds__(256) k_rdup_b(BatchDev bd, MatDesc* rmats, const int* ridx)
+{
+    __shared__ int eq;
+    const int b = blockIdx.y;
+    const ItemMeta& M = bd.meta[b];
+    int rep = b;
+    if (M.valid) {
+        const int n = M.snx * M.sny;
+        const int* sb = bd.sidx + b * bd.s_cell;
+        for (int a = 0; a < b; a++) {
+            const ItemMeta& A = bd.meta[a];
+            if (!A.valid || bd.rkey[a] != bd.rkey[b] || A.snx != M.snx || A.sny != M.sny || A.n_src != M.n_src) continue;
+            const int* sa = bd.sidx + a * bd.s_cell;
+            if (threadIdx.x == 0) eq = 1;
+            __syncthreads();
+            for (int c = threadIdx.x; c < n; c += 256)
+                if (sa[c] != sb[c]) eq = 0;
+            __syncthreads();
+            const int same = eq;
+            __syncthreads();
+            if (same) {
+                rep = a;
+                break;
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        bd.rrep[b] = rep;
+        if (rep != b && ridx[b] >= 0) rmats[ridx[b]].nb = 0;
+    }
+}
+
+void launch_rdup_b(hipStream_t s, const BatchDev& bd, MatDesc* rmats, const int* ridx)
+{
+    hipLaunchKernelGGL(k_rhash_b, dim3(1, bd.B), dim3(256), 0, s, bd);
+    hipLaunchKernelGGL(k_rdup_b, dim3(1, bd.B), dim3(256), 0, s, bd, rmats, ridx);
+}
+
 void launch_curvature_b(hipStream_t s, const BatchDev& bd, int max_cells, int max_rtiles)
 {
     hipLaunchKernelGGL(k_zero_r_b, dim3(grid_for((size_t)max_rtiles * TS * TS, 256, 2048), bd.B), dim3(256), 0, s, bd);
@@ -2520,12 +2594,15 @@ __global__ void __launch_bounds__(256) k_logdets_b(BatchDev bd)
     if (!M.valid) return;
     const double* v = bd.vec + M.vec_off;
     const T* diagF = (const T*)(v + M.ldN);
-    const double* diagR = v + 2 * (size_t)M.ldN;
+    // R's pivots: those of the item whose factorization serves this one (the same R)
+    const int rr = bd.rrep[item];
+    const double* diagR = bd.vec + bd.meta[rr].vec_off + 2 * (size_t)M.ldN;
     const double lf = block_sum_log<T>(diagF, M.n_src, sh);
     const double lr = block_sum_log<double>(diagR, M.n_src, sh);
     if (threadIdx.x == 0) {
         bd.out[QLX_OUT_STRIDE * item + 2] = lf;
         bd.out[QLX_OUT_STRIDE * item + 3] = lr;
+        if (rr != item) bd.status[4 * item + 1] = bd.status[4 * rr + 1];
     }
 }
 

@@ -83,6 +83,7 @@ struct Ctx {
     int pnx = 0, pny = 0;
     bool psf_set = false;
     std::vector<double> h_psf;
+    std::vector<int> h_ridx;  // item -> R descriptor of the current wave
     DevBuf psf;
     // settings
     qlx_settings set{};
@@ -91,7 +92,7 @@ struct Ctx {
     DevBuf d_models, d_sgrid, d_srcn, d_meta, d_mats;
     // per-item work (stride s_pix / s_cell)
     DevBuf beta, map_cell, map_w, pmapped, cmapped, sidx, tix, pisrc, pidx, act_pix, counts, ucol, uval, ucnt, nevals;
-    DevBuf apre, eptr, csh, cpix, cval, nfwtab, pmask, hmask;
+    DevBuf apre, eptr, csh, cpix, cval, nfwtab, pmask, hmask, rdup, d_ridx;
     DevBuf key, bcnt, boff, bfill, perm;
     // pools
     DevBuf lb, F, Ff, Fz, R, Rf, cvec, vec, nz, bbase, winv, model, out, status;  // Fz: residual partials  // Rf: R's copy for its factorization
@@ -566,7 +567,8 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
               c->pmask.ensure((size_t)B * s_pix * QLX_MAX_CBW * 4) && c->hmask.ensure((size_t)B * s_pix * QLX_MAX_CBW * 4) &&
               c->bcnt.ensure(B * (s_cell + 2) * 4) && c->boff.ensure(B * (s_cell + 2) * 4) && c->bfill.ensure(B * (s_cell + 2) * 4) &&
               c->perm.ensure(B * s_pix * 4) && c->model.ensure(B * s_pix * 16) && c->out.ensure((size_t)B * QLX_OUT_STRIDE * 8) &&
-              c->status.ensure((size_t)B * 16) && c->agrid.ensure((size_t)B * 64);
+              c->status.ensure((size_t)B * 16) && c->agrid.ensure((size_t)B * 64) &&
+              c->rdup.ensure((size_t)B * 12) && c->d_ridx.ensure((size_t)B * 4);
     if (!ok) return fail(c, QLX_ERR_NOMEM, "work buffer allocation failed");
     c->last_wave.clear();
     for (int b = 0; b < B; b++) c->last_wave.push_back(items[b].idx);
@@ -602,6 +604,8 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     bd.sidx = c->sidx.as<int>();
     bd.tix = c->tix.as<int>();
     bd.pisrc = c->pisrc.as<int>();
+    bd.rkey = c->rdup.as<unsigned long long>();
+    bd.rrep = (int*)(c->rdup.as<char>() + (size_t)B * 8);
     bd.pidx = c->pidx.as<int>();
     bd.act_pix = c->act_pix.as<int>();
     bd.counts = c->counts.as<int>();
@@ -826,6 +830,14 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     std::vector<MatDesc> mats(fm);
     mats.insert(mats.end(), rm.begin(), rm.end());
     HIPCHK(c, hipMemcpyAsync(c->d_mats.p, mats.data(), sizeof(MatDesc) * mats.size(), hipMemcpyHostToDevice, st));
+    {
+        // each item's R descriptor (rm holds the valid items in order), for launch_rdup_b
+        c->h_ridx.assign(B, -1);
+        int k = 0;
+        for (int b = 0; b < B; b++)
+            if (meta[b].valid) c->h_ridx[b] = k++;
+        HIPCHK(c, hipMemcpyAsync(c->d_ridx.p, c->h_ridx.data(), (size_t)B * 4, hipMemcpyHostToDevice, st));
+    }
     const int nF = (int)fm.size(), nR = (int)rm.size();
     timer_begin(c, "lbuild");
     launch_merge_lrows_b(st, bd, max_img);
@@ -882,6 +894,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         bd.lam = c->d_lam.as<double>();
     }
     launch_curvature_b(st, bd, max_cells, max_rtiles);
+    launch_rdup_b(st, bd, c->d_mats.as<MatDesc>() + nF, c->d_ridx.as<int>());
     HIPCHK(c, hipEventRecord(c->ev_fork, st));
     if (!mixed) launch_gather_cvec_b(st, bd, max_rows);
     timer_begin(c, "syrk");

@@ -106,6 +106,8 @@ struct BatchDev {
     int* sidx;                  // s_cell: reference (j-major) active index of each cell, or -1
     int* tix;                   // s_cell: internal (tiled) active index of each cell, or -1
     int* pisrc;                 // s_cell: reference index of internal index p
+    unsigned long long* rkey;   // 1 per item: hash of the grid shape and active-index map (R's identity)
+    int* rrep;                  // 1 per item: the first item of the wave with the same R (itself if none)
     int* pidx;                  // s_pix
     int* act_pix;               // s_pix
     int* counts;                // 4 per item: n_src, n_img, support blocks, largest chunk support
@@ -251,6 +253,9 @@ void launch_gather_cvec_b(hipStream_t s, const BatchDev& bd, int max_rows);
 void launch_syrk_f64_b(hipStream_t s, const BatchDev& bd, int max_ncb);
 void launch_syrk_f16_b(hipStream_t s, const BatchDev& bd, int max_ncb);
 void launch_curvature_b(hipStream_t s, const BatchDev& bd, int max_cells, int max_rtiles);
+// items whose R equals an earlier item's (same grid shape and active-index map) get rrep = that item
+// and their R descriptor (rmats[ridx[item]]) a zero block count: one factorization serves them all
+void launch_rdup_b(hipStream_t s, const BatchDev& bd, MatDesc* rmats, const int* ridx);
 void launch_prep_fr_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32);
 // panel_blocks: width of the FP32 dense schedule's panels in 64-blocks (a multiple of 4; 0 = 16);
 // the factor is bitwise the same for every width

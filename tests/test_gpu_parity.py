@@ -356,6 +356,33 @@ def test_batch_equals_single_bitwise(qlx_ctx, golden_dir):
         assert rb["minus2lnE"] == rs["minus2lnE"] and rb["logdetR"] == rs["logdetR"]
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_shared_regularization_factorizations(qlx_ctx, golden_dir, prec):
+    """Items whose regularization matrix R is the same (same grid and active cells) share one
+    factorization of it; items with other active cells get their own.  A batch mixing both kinds
+    (repeated models, models with another Einstein radius and so other mapped cells) must give
+    every item its solo result bit for bit."""
+    from qlens_amd import qlx
+    g = np.load(os.path.join(golden_dir, "inversion_C1.npz"))
+    case = dict(W.c1())
+    case["srcgrid"] = (-1.6, 1.6, -1.6, 1.6)  # wider than the mapped region: the active cells vary
+    qlx_ctx.load_workload(case, g["data"], precision=qlx.QLX_PREC_FP64 if prec == "fp64" else qlx.QLX_PREC_MIXED)
+    base = case["lenses"]
+
+    def scaled(f):
+        (t, p), rest = base[0], base[1:]
+        return [(t, (p[0] * f,) + tuple(p[1:]))] + list(rest)
+
+    models = [base, scaled(1.15), base, scaled(0.85), scaled(1.15), base, scaled(1.3)]
+    batch = qlx_ctx.invert_image_surface_brightness_map(models)
+    nsrc = [r["n_src_active"] for r in batch]
+    assert len(set(nsrc)) >= 3, nsrc  # several different active sets in the batch
+    for m, rb in zip(models, batch):
+        rs = qlx_ctx.invert_image_surface_brightness_map(m)[0]
+        assert rb["logdetR"] == rs["logdetR"] and rb["minus2lnE"] == rs["minus2lnE"], (rb["n_src_active"], rs["n_src_active"])
+        assert rb["status"] == rs["status"] == 0
+
+
 def test_uneven_source_grids_in_one_batch(qlx_ctx, golden_dir):
     """One submission with per-item source grids (and a too-small grid) against the oracle."""
     g = np.load(os.path.join(golden_dir, "inversion_C1.npz"))
