@@ -1195,6 +1195,11 @@ __device__ __forceinline__ double* rtile(const BatchDev& bd, const ItemMeta& M, 
     const int I = r >> 6, J = c >> 6;
     return bd.R + M.r_off + ((size_t)I * (M.bwb + 1) + (I - J)) * TS * TS + (r & 63) * TS + (c & 63);
 }
+// the same entry of an item's R as read: R is built once per distinct R, by the item rrep names
+__device__ __forceinline__ double rget(const BatchDev& bd, int item, int r, int c)
+{
+    return *rtile(bd, bd.meta[bd.rrep[item]], r, c);
+}
 
 // F's preparation, fused into the FP16 SYRK's store of tile (I, J): scale back, F += lambda R over
 // the source block (R is held in the reference order: internal indices map through pisrc), the data
@@ -1228,7 +1233,7 @@ __device__ __forceinline__ void syrk_f16_store(const BatchDev& bd, const ItemMet
                 if (r < n && cg < n) {
                     const int pa = pis[r];
                     const int hi = pa > pb ? pa : pb, lo = pa > pb ? pb : pa;
-                    if ((hi >> 6) - (lo >> 6) <= M.bwb) x = (float)((double)x + lam * *rtile(bd, M, hi, lo));
+                    if ((hi >> 6) - (lo >> 6) <= M.bwb) x = (float)((double)x + lam * rget(bd, item, hi, lo));
                     if (r == cg) fdiag[r] = (double)x;
                 } else if (r > n || cg > n) {
                     x = (r == cg) ? 1.0f : 0.0f;
@@ -1331,7 +1336,7 @@ __global__ void __launch_bounds__(256) k_zero_r_b(BatchDev bd)
 {
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
-    if (!M.valid) return;
+    if (!M.valid || bd.rrep[item] != item) return;  // an earlier item holds this R
     // 16-byte stores: the band holds whole 64 x 64 tiles and starts on a tile boundary
     double2* R2 = (double2*)(bd.R + M.r_off);
     const size_t n2 = (size_t)(M.ldN / TS) * (M.bwb + 1) * TS * TS / 2;
@@ -1344,7 +1349,7 @@ __global__ void __launch_bounds__(256) k_curvature_b(BatchDev bd)
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (!M.valid) return;
+    if (!M.valid || bd.rrep[item] != item) return;  // an earlier item holds this R
     if (c >= M.snx * M.sny) {
         // identity on the padding rows (the data row of F's augmented matrix has no R entries)
         const int r = c - M.snx * M.sny + M.n_src;
@@ -1489,7 +1494,7 @@ __global__ void __launch_bounds__(256) k_prep_f_b(BatchDev bd)
                 // R is held in the reference order: F's internal indices map through pisrc
                 const int a = pis[r], b = pis[c];
                 const int hi = a > b ? a : b, lo = a > b ? b : a;
-                const double rv = ((hi >> 6) - (lo >> 6) <= M.bwb) ? *rtile(bd, M, hi, lo) : 0.0;
+                const double rv = ((hi >> 6) - (lo >> 6) <= M.bwb) ? rget(bd, item, hi, lo) : 0.0;
                 F[t] = F0[t] + lam * rv;
             } else if (r > n) {
                 F[t] = (r == c) ? 1.0 : 0.0;

@@ -893,8 +893,8 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         HIPCHK(c, hipMemcpyAsync(c->d_lam.p, lamh.data(), (size_t)B * 8, hipMemcpyHostToDevice, st));
         bd.lam = c->d_lam.as<double>();
     }
-    launch_curvature_b(st, bd, max_cells, max_rtiles);
     launch_rdup_b(st, bd, c->d_mats.as<MatDesc>() + nF, c->d_ridx.as<int>());
+    launch_curvature_b(st, bd, max_cells, max_rtiles);  // R of the items that hold one
     HIPCHK(c, hipEventRecord(c->ev_fork, st));
     if (!mixed) launch_gather_cvec_b(st, bd, max_rows);
     timer_begin(c, "syrk");

@@ -254,7 +254,8 @@ void launch_syrk_f64_b(hipStream_t s, const BatchDev& bd, int max_ncb);
 void launch_syrk_f16_b(hipStream_t s, const BatchDev& bd, int max_ncb);
 void launch_curvature_b(hipStream_t s, const BatchDev& bd, int max_cells, int max_rtiles);
 // items whose R equals an earlier item's (same grid shape and active-index map) get rrep = that item
-// and their R descriptor (rmats[ridx[item]]) a zero block count: one factorization serves them all
+// and their R descriptor (rmats[ridx[item]]) a zero block count: one build and one factorization of
+// R serve them all (launched before launch_curvature_b, which builds the representatives' R only)
 void launch_rdup_b(hipStream_t s, const BatchDev& bd, MatDesc* rmats, const int* ridx);
 void launch_prep_fr_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32);
 // panel_blocks: width of the FP32 dense schedule's panels in 64-blocks (a multiple of 4; 0 = 16);
