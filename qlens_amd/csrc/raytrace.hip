@@ -206,19 +206,23 @@ __global__ void __launch_bounds__(256) k_nfw_tables(BatchDev bd)
     t[3] = c / qfac;
 }
 
-// nfw_j01 with the node constants of k_nfw_tables (tab: this lens's QLX_PAT_N x 4 doubles)
-template <typename T, bool KT = false>
+// nfw_j01 with the node constants of k_nfw_tables (tab: this lens's QLX_PAT_N x 4 doubles).  The
+// sums of the levels are formed as the nodes come in (a node of level m adds its weighted values to
+// every level's sum from m on); with NL < QLX_PAT_NLEV only the first NL levels are formed and a ray
+// not converged by then reports -1, to be integrated again with all levels (the sums of the first
+// NL levels take the same operations in the same order either way).
+template <typename T, bool KT = false, int NL = QLX_PAT_NLEV>
 __device__ __forceinline__ int nfw_j01_tab(const LensPrep& L, const double* tab, T x, T y, T tol, T& j0, T& j1)
 {
     const T xsq = x * x, ysq = y * y;
-    T S0[QLX_PAT_NLEV], S1[QLX_PAT_NLEV];
+    T S0[NL], S1[NL];
 #pragma unroll
-    for (int k = 0; k < QLX_PAT_NLEV; k++) { S0[k] = 0; S1[k] = 0; }
+    for (int k = 0; k < NL; k++) { S0[k] = 0; S1[k] = 0; }
     bool d0 = false, d1 = false;
     T r0 = 0, r1 = 0;
     int nev = 0;
 #pragma unroll
-    for (int m = 0; m < QLX_PAT_NLEV; m++) {
+    for (int m = 0; m < NL; m++) {
         const int order = (1 << (m + 1)) - 1;
         const int step = (QLX_PAT_N + 1) / (order + 1);
         for (int i = step - 1; i < QLX_PAT_N; i += 2 * step) {
@@ -230,7 +234,7 @@ __device__ __forceinline__ int nfw_j01_tab(const LensPrep& L, const double* tab,
             const T f0 = (T)tn[2] * kap;
             const T f1 = (T)tn[3] * kap;
 #pragma unroll
-            for (int k = m; k < QLX_PAT_NLEV; k++) {
+            for (int k = m; k < NL; k++) {
                 T wk = PatTab<T>::w(k, i);
                 S0[k] += wk * f0;
                 S1[k] += wk * f1;
@@ -243,8 +247,11 @@ __device__ __forceinline__ int nfw_j01_tab(const LensPrep& L, const double* tab,
             if (d0 && d1) break;
         }
     }
-    if (!d0) r0 = S0[QLX_PAT_NLEV - 1];
-    if (!d1) r1 = S1[QLX_PAT_NLEV - 1];
+    if constexpr (NL < QLX_PAT_NLEV) {
+        if (!(d0 && d1)) return -1;
+    }
+    if (!d0) r0 = S0[NL - 1];
+    if (!d1) r1 = S1[NL - 1];
     j0 = (T)L.sqrt1meps * ((T)0.5 * r0);
     j1 = (T)L.sqrt1meps * ((T)0.5 * r1);
     return nev;
@@ -354,7 +361,12 @@ __device__ __forceinline__ void lens_deflection(const LensPrep& L, T x, T y, T t
                 a1 = 0;
             } else {
                 T j0, j1;
-                if constexpr (TAB) nev += nfw_j01_tab<T, KT>(L, tab, x, y, tol, j0, j1);
+                if constexpr (TAB) {
+                    // seven levels (127 nodes) first: the rest only for the rays that need them
+                    int ne = nfw_j01_tab<T, KT, 7>(L, tab, x, y, tol, j0, j1);
+                    if (ne < 0) ne = nfw_j01_tab<T, KT>(L, tab, x, y, tol, j0, j1);
+                    nev += ne;
+                }
                 else nev += nfw_j01<T>(L, x, y, tol, j0, j1);
                 a0 = x * j0;
                 a1 = y * j1;
