@@ -27,6 +27,14 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+
+// 16-byte store marked non-temporal (streamed past the caches): for buffers written once and read
+// much later, so that they do not push out lines that are still being re-read
+__device__ __forceinline__ void store_nt(uint4* p, const uint4& v)
+{
+    __builtin_nontemporal_store((u32x4_t){v.x, v.y, v.z, v.w}, (u32x4_t*)p);
+}
 #define TS 64
 #define LDSP 66
 
@@ -1001,7 +1009,7 @@ __global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int tab_bytes, 
                 const uint4* src = (const uint4*)tile;
                 for (int u = tid; u < nrun * upr; u += 256) {
                     const int run = u >> (cbs - 2), o = u & (upr - 1);  // upr = 2^(cbs - 2)
-                    dst[(run * CB + h * HB) / 2 + o] = src[u];
+                    store_nt(&dst[(run * CB + h * HB) / 2 + o], src[u]);
                 }
                 __syncthreads();
             }
@@ -1240,7 +1248,7 @@ __device__ __forceinline__ void syrk_f16_store(const BatchDev& bd, const ItemMet
                 } else if (r == cg) {
                     x = 1e30f;  // (n, n); the rest of row n (D) is kept
                 }
-                C[(size_t)(r - I * SCB) * ld + (cg - J * SCB)] = x;
+                __builtin_nontemporal_store(x, &C[(size_t)(r - I * SCB) * ld + (cg - J * SCB)]);
             }
     }
 }
@@ -1262,6 +1270,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
     const int nkt = M.nkt, ncbw = M.ncbw;
     uint16_t* const as = lds2;
     uint16_t* const bs = lds2 + 16 * BP2;
+    const bool dg = I == J;  // a diagonal tile reads its one block once, as both operands
+    const uint16_t* const bsrc = dg ? as : bs;
     // thread piece h (of 4) of every block: 16 bytes at element (tid + 256 h) * 8
     const int e0 = tid * 8;
     const int o0 = (e0 >> 9) * BP2 + (e0 & 511);                    // pieces h = 0..3 are 2048
@@ -1273,9 +1283,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
     {                                                                                                  \
         const unsigned* m_ = nz + (size_t)(kk_) * ncbw;                                                \
         const uint4* a_ = (const uint4*)(pool + (size_t)(bb[kk_] + mask_rank(m_, I)) * bsz) + tid;     \
-        const uint4* b_ = (const uint4*)(pool + (size_t)(bb[kk_] + mask_rank(m_, J)) * bsz) + tid;     \
         pa0 = a_[0]; pa1 = a_[256]; pa2 = a_[512]; pa3 = a_[768];                                      \
-        pb0 = b_[0]; pb1 = b_[256]; pb2 = b_[512]; pb3 = b_[768];                                      \
+        if (!dg) {                                                                                     \
+            const uint4* b_ = (const uint4*)(pool + (size_t)(bb[kk_] + mask_rank(m_, J)) * bsz) + tid; \
+            pb0 = b_[0]; pb1 = b_[256]; pb2 = b_[512]; pb3 = b_[768];                                  \
+        }                                                                                              \
     }
     float4_t acc[4][4];
 #pragma unroll
@@ -1291,7 +1303,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
     while (kt < nkt) {
         __syncthreads();
         *(uint4*)&as[o0] = pa0; *(uint4*)&as[o1] = pa1; *(uint4*)&as[o2] = pa2; *(uint4*)&as[o3] = pa3;
-        *(uint4*)&bs[o0] = pb0; *(uint4*)&bs[o1] = pb1; *(uint4*)&bs[o2] = pb2; *(uint4*)&bs[o3] = pb3;
+        if (!dg) { *(uint4*)&bs[o0] = pb0; *(uint4*)&bs[o1] = pb1; *(uint4*)&bs[o2] = pb2; *(uint4*)&bs[o3] = pb3; }
         __syncthreads();
         int nk = kt + 1;
         for (; nk < nkt; nk++) {
@@ -1306,7 +1318,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 av[t] = lds_k8(as, g, wr * 64 + t * 16 + (lane & 15));
-                bv[t] = lds_k8(bs, g, wc * 64 + t * 16 + (lane & 15));
+                bv[t] = lds_k8(bsrc, g, wc * 64 + t * 16 + (lane & 15));
             }
 #pragma unroll
             for (int a = 0; a < 4; a++)
