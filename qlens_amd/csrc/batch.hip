@@ -1434,7 +1434,7 @@ __global__ void __launch_bounds__(256) k_rhash_b(BatchDev bd)
     if (tid == 0) bd.rkey[item] = M.valid ? sh[0] ^ mix64(((unsigned long long)M.snx << 32) | (unsigned)M.sny) : 0ULL;
 }
 
-__global__ void __launch_bounds__(256) k_rdup_b(BatchDev bd, MatDesc* rmats, const int* ridx)
+__global__ void __launch_bounds__(256) k_rdup_b(BatchDev bd)
 {
     __shared__ int eq;
     const int b = blockIdx.y;
@@ -1460,16 +1460,13 @@ __global__ void __launch_bounds__(256) k_rdup_b(BatchDev bd, MatDesc* rmats, con
             }
         }
     }
-    if (threadIdx.x == 0) {
-        bd.rrep[b] = rep;
-        if (rep != b && ridx[b] >= 0) rmats[ridx[b]].nb = 0;
-    }
+    if (threadIdx.x == 0) bd.rrep[b] = rep;
 }
 
-void launch_rdup_b(hipStream_t s, const BatchDev& bd, MatDesc* rmats, const int* ridx)
+void launch_rdup_b(hipStream_t s, const BatchDev& bd)
 {
     hipLaunchKernelGGL(k_rhash_b, dim3(1, bd.B), dim3(256), 0, s, bd);
-    hipLaunchKernelGGL(k_rdup_b, dim3(1, bd.B), dim3(256), 0, s, bd, rmats, ridx);
+    hipLaunchKernelGGL(k_rdup_b, dim3(1, bd.B), dim3(256), 0, s, bd);
 }
 
 void launch_curvature_b(hipStream_t s, const BatchDev& bd, int max_cells, int max_rtiles)

@@ -83,7 +83,6 @@ struct Ctx {
     int pnx = 0, pny = 0;
     bool psf_set = false;
     std::vector<double> h_psf;
-    std::vector<int> h_ridx;  // item -> R descriptor of the current wave
     DevBuf psf;
     // settings
     qlx_settings set{};
@@ -92,7 +91,7 @@ struct Ctx {
     DevBuf d_models, d_sgrid, d_srcn, d_meta, d_mats;
     // per-item work (stride s_pix / s_cell)
     DevBuf beta, map_cell, map_w, pmapped, cmapped, sidx, tix, pisrc, pidx, act_pix, counts, ucol, uval, ucnt, nevals;
-    DevBuf apre, eptr, csh, cpix, cval, nfwtab, pmask, hmask, rdup, d_ridx;
+    DevBuf apre, eptr, csh, cpix, cval, nfwtab, pmask, hmask, rdup;
     DevBuf key, bcnt, boff, bfill, perm;
     // pools
     DevBuf lb, F, Ff, Fz, R, Rf, cvec, vec, nz, bbase, winv, model, out, status;  // Fz: residual partials  // Rf: R's copy for its factorization
@@ -568,7 +567,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
               c->bcnt.ensure(B * (s_cell + 2) * 4) && c->boff.ensure(B * (s_cell + 2) * 4) && c->bfill.ensure(B * (s_cell + 2) * 4) &&
               c->perm.ensure(B * s_pix * 4) && c->model.ensure(B * s_pix * 16) && c->out.ensure((size_t)B * QLX_OUT_STRIDE * 8) &&
               c->status.ensure((size_t)B * 16) && c->agrid.ensure((size_t)B * 64) &&
-              c->rdup.ensure((size_t)B * 12) && c->d_ridx.ensure((size_t)B * 4);
+              c->rdup.ensure((size_t)B * 12);
     if (!ok) return fail(c, QLX_ERR_NOMEM, "work buffer allocation failed");
     c->last_wave.clear();
     for (int b = 0; b < B; b++) c->last_wave.push_back(items[b].idx);
@@ -797,7 +796,21 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     bd.winv = c->winv.as<double>();
     HIPCHK(c, hipMemcpyAsync(c->d_meta.p, meta.data(), sizeof(ItemMeta) * B, hipMemcpyHostToDevice, st));
 
-    // ---- factorization descriptors: F (dense, FP64 or FP32) and R (banded FP64) of every item ----
+    timer_begin(c, "lbuild");
+    launch_merge_lrows_b(st, bd, max_img);
+    launch_sort_rows_b(st, bd, max_img, max_src);
+    launch_lcols_b(st, bd, max_img, max_src);
+    launch_row_support_b(st, bd, max_nkt);
+    launch_rdup_b(st, bd);
+    HIPCHK(c, hipMemcpyAsync(counts.data(), bd.counts, (size_t)B * 16, hipMemcpyDeviceToHost, st));
+    c->nzh.resize(nz_tot);
+    HIPCHK(c, hipMemcpyAsync(c->nzh.data(), bd.nz, nz_tot * 4, hipMemcpyDeviceToHost, st));
+    std::vector<int> rrep(B);
+    HIPCHK(c, hipMemcpyAsync(rrep.data(), bd.rrep, (size_t)B * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    // ---- factorization descriptors: F (dense, FP64 or FP32) of every item and R (banded FP64) of
+    // every distinct R: an item whose R equals an earlier item's (rrep, k_rdup_b) reads that one's
+    // log-determinant and status, so R's chain launches cover the distinct matrices only ----
     std::vector<MatDesc> fm, rm;
     for (int b = 0; b < B; b++) {
         const ItemMeta& M = meta[b];
@@ -815,6 +828,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         f.W = (void*)(bd.winv + M.w_off);
         f.A0 = mixed ? (void*)(bd.Ff32 + M.f_off) : nullptr;
         fm.push_back(f);
+        if (rrep[b] != b) continue;
         MatDesc r{};
         r.A = (void*)(c->Rf.as<double>() + M.r_off);
         r.ld = 64;
@@ -830,24 +844,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     std::vector<MatDesc> mats(fm);
     mats.insert(mats.end(), rm.begin(), rm.end());
     HIPCHK(c, hipMemcpyAsync(c->d_mats.p, mats.data(), sizeof(MatDesc) * mats.size(), hipMemcpyHostToDevice, st));
-    {
-        // each item's R descriptor (rm holds the valid items in order), for launch_rdup_b
-        c->h_ridx.assign(B, -1);
-        int k = 0;
-        for (int b = 0; b < B; b++)
-            if (meta[b].valid) c->h_ridx[b] = k++;
-        HIPCHK(c, hipMemcpyAsync(c->d_ridx.p, c->h_ridx.data(), (size_t)B * 4, hipMemcpyHostToDevice, st));
-    }
     const int nF = (int)fm.size(), nR = (int)rm.size();
-    timer_begin(c, "lbuild");
-    launch_merge_lrows_b(st, bd, max_img);
-    launch_sort_rows_b(st, bd, max_img, max_src);
-    launch_lcols_b(st, bd, max_img, max_src);
-    launch_row_support_b(st, bd, max_nkt);
-    HIPCHK(c, hipMemcpyAsync(counts.data(), bd.counts, (size_t)B * 16, hipMemcpyDeviceToHost, st));
-    c->nzh.resize(nz_tot);
-    HIPCHK(c, hipMemcpyAsync(c->nzh.data(), bd.nz, nz_tot * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipStreamSynchronize(st));
     size_t lb_tot = 0;
     int max_nsup = 0;
     for (int b = 0; b < B; b++) {
@@ -893,7 +890,6 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
         HIPCHK(c, hipMemcpyAsync(c->d_lam.p, lamh.data(), (size_t)B * 8, hipMemcpyHostToDevice, st));
         bd.lam = c->d_lam.as<double>();
     }
-    launch_rdup_b(st, bd, c->d_mats.as<MatDesc>() + nF, c->d_ridx.as<int>());
     launch_curvature_b(st, bd, max_cells, max_rtiles);  // R of the items that hold one
     HIPCHK(c, hipEventRecord(c->ev_fork, st));
     if (!mixed) launch_gather_cvec_b(st, bd, max_rows);

@@ -253,10 +253,11 @@ void launch_gather_cvec_b(hipStream_t s, const BatchDev& bd, int max_rows);
 void launch_syrk_f64_b(hipStream_t s, const BatchDev& bd, int max_ncb);
 void launch_syrk_f16_b(hipStream_t s, const BatchDev& bd, int max_ncb);
 void launch_curvature_b(hipStream_t s, const BatchDev& bd, int max_cells, int max_rtiles);
-// items whose R equals an earlier item's (same grid shape and active-index map) get rrep = that item
-// and their R descriptor (rmats[ridx[item]]) a zero block count: one build and one factorization of
-// R serve them all (launched before launch_curvature_b, which builds the representatives' R only)
-void launch_rdup_b(hipStream_t s, const BatchDev& bd, MatDesc* rmats, const int* ridx);
+// items whose R equals an earlier item's (same grid shape and active-index map) get rrep = that item,
+// others rrep = themselves: one build and one factorization of R serve them all (the host reads rrep
+// back and gives R's factorization the representatives' descriptors only; launch_curvature_b
+// builds the representatives' R only)
+void launch_rdup_b(hipStream_t s, const BatchDev& bd);
 void launch_prep_fr_b(hipStream_t s, const BatchDev& bd, int max_ld, int f32);
 // panel_blocks: width of the FP32 dense schedule's panels in 64-blocks (a multiple of 4; 0 = 16);
 // the factor is bitwise the same for every width
