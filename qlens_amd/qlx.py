@@ -7,6 +7,7 @@ All numerical work runs in the HIP library; there is no CPU fallback: if libqlx.
 no GPU is visible, construction raises.
 """
 import ctypes as C
+import itertools
 import os
 
 import numpy as np
@@ -133,11 +134,40 @@ _RESULT_DT = np.dtype({"names": [f for f, _ in QlxResult._fields_],
 assert _LENS_DT.itemsize == C.sizeof(QlxLens)
 
 
+def _fill_same_layout(a, models, def_mode):
+    """make_models' common case: every model has the first one's lens types and parameter counts
+    (a chain's proposals), so each lens position is filled from one array.  Returns False, having
+    written nothing, when the models differ in layout."""
+    n, nl = len(models), len(models[0])
+    if nl > QLX_MAX_LENSES or set(map(len, models)) != {nl}:
+        return False
+    cols = []
+    for k in range(nl):
+        col = [lm[k] for lm in models]
+        t0 = col[0][0]
+        if [c[0] for c in col].count(t0) != n:
+            return False
+        prm = [c[1] for c in col]
+        m = len(prm[0])
+        if m > a["lens"]["p"].shape[-1] or set(map(len, prm)) != {m}:
+            return False
+        cols.append((t0, np.fromiter(itertools.chain.from_iterable(prm), dtype=np.float64, count=n * m).reshape(n, m)))
+    a["nlens"][:n] = nl
+    L = a["lens"]
+    for k, (t, p) in enumerate(cols):
+        L["type"][:n, k] = t
+        L["def_mode"][:n, k] = def_mode if t == QLX_LENS_NFW else QLX_DEF_QUADRATURE
+        L["p"][:n, k, :p.shape[1]] = p
+    return True
+
+
 def make_models(models, def_mode=QLX_DEF_QUADRATURE):
     """[[(type, params)], ...] in QLens conventions -> ctypes array of QlxLensModel"""
     n = len(models)
     arr = (QlxLensModel * max(1, n))()
     a = np.frombuffer(arr, dtype=_MODEL_DT)
+    if n and _fill_same_layout(a, models, def_mode):
+        return arr
     a["nlens"][:n] = [len(lm) for lm in models]
     flat = [(i, k, t, prm) for i, lm in enumerate(models) for k, (t, prm) in enumerate(lm)]
     if not flat:

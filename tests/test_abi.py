@@ -64,6 +64,15 @@ def test_batch_packing_matches_field_by_field():
     for k, lm in enumerate(models):
         ref[k] = qlx.make_model(lm, qlx.QLX_DEF_SURROGATE)
     assert bytes(qlx.make_models(models, qlx.QLX_DEF_SURROGATE)) == bytes(ref)
+    # models of one layout (the per-lens-position fill) and one that differs from the first only
+    # in a lens type or a parameter count (the general fill)
+    for dm in (qlx.QLX_DEF_QUADRATURE, qlx.QLX_DEF_SURROGATE):
+        for ms in (props, props[:5] + [[props[5][1], props[5][0], props[5][2]]],
+                   props[:5] + [[props[5][0], props[5][1], (props[5][2][0], props[5][2][1][:2])]]):
+            ref = (qlx.QlxLensModel * len(ms))()
+            for k, lm in enumerate(ms):
+                ref[k] = qlx.make_model(lm, dm)
+            assert bytes(qlx.make_models(ms, dm)) == bytes(ref)
     res = (qlx.QlxResult * 3)()
     res[1].minus2lnE, res[1].status, res[2].refine_steps = -3.25, -7, 4
     rows = np.frombuffer(res, dtype=qlx._RESULT_DT).tolist()
