@@ -40,19 +40,24 @@ struct DevBuf {
         p = nullptr;
         bytes = 0;
     }
+    // At least n bytes.  A buffer is given an eighth more than asked when that fits: the sizes of
+    // later calls vary a little with the proposals, and each growth frees and allocates again (a
+    // large hipFree / hipMalloc pair stalls the device).  Without room for the margin, exactly n.
     bool ensure(size_t n)
     {
         if (n <= bytes && p) return true;
         release();
         if (n == 0) n = 8;
-        n = (n + 255) & ~(size_t)255;
-        if (hipMalloc(&p, n) != hipSuccess) {
+        for (size_t m : {n + n / 8, n}) {
+            m = (m + 255) & ~(size_t)255;
+            if (hipMalloc(&p, m) == hipSuccess) {
+                bytes = m;
+                g_dev_bytes += m;
+                return true;
+            }
             p = nullptr;
-            return false;
         }
-        bytes = n;
-        g_dev_bytes += n;
-        return true;
+        return false;
     }
     template <typename T> T* as() const { return (T*)p; }
 };
