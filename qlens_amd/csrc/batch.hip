@@ -2849,7 +2849,7 @@ __global__ void __launch_bounds__(256) k_resid_rows_b(BatchDev bd)
     __shared__ float tile[TS][TS + 1];
     __shared__ double dacc[4][TS];
     __shared__ double racc[TS];
-    __shared__ double zs[4][256];
+    __shared__ double zs[2][4][256];  // column partials of passes it (even / odd)
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     if (!M.valid || bd.status[4 * item + 3]) return;
@@ -2874,7 +2874,9 @@ __global__ void __launch_bounds__(256) k_resid_rows_b(BatchDev bd)
         tile[row][cc + 3] = t.w;
     }
     // row strip: F(i,c) = U[i][c] for c < c0; wave w owns rows c0 + w + 4k, k = 0..15, lane the four
-    // columns 256 it + 4 lane .. + 3 of pass it (every thread runs every pass: the passes' barriers)
+    // columns 256 it + 4 lane .. + 3 of pass it (every thread runs every pass: the passes' barrier).
+    // A pass's loads are issued before the previous pass's barrier and column sums; the column
+    // partials alternate between two LDS buffers, so one barrier per pass orders them
     {
         double ra[16], si[16];
 #pragma unroll
@@ -2885,14 +2887,16 @@ __global__ void __launch_bounds__(256) k_resid_rows_b(BatchDev bd)
         }
         const float* Ur = U + (size_t)(c0 + w) * ld;
         const int npass = (c0 + 255) / 256;
+        float4_t t[16];
+        if (4 * lane < c0) {  // c0 is a multiple of 64, so c + 3 < c0 for every c < c0 below
+#pragma unroll
+            for (int k = 0; k < 16; k++) t[k] = *(const float4_t*)(Ur + (size_t)(4 * k) * ld + 4 * lane);
+        }
         for (int it = 0; it < npass; it++) {
             const int c = 256 * it + 4 * lane;
             double z0 = 0.0, z1 = 0.0, z2 = 0.0, z3 = 0.0;
-            if (c < c0) {  // c0 is a multiple of 64, so c + 3 < c0
+            if (c < c0) {
                 const double s0 = s[c], s1 = s[c + 1], s2 = s[c + 2], s3 = s[c + 3];
-                float4_t t[16];
-#pragma unroll
-                for (int k = 0; k < 16; k++) t[k] = *(const float4_t*)(Ur + (size_t)(4 * k) * ld + c);
 #pragma unroll
                 for (int k = 0; k < 16; k++) {
                     ra[k] = fma((double)t[k].x, s0, fma((double)t[k].y, s1, fma((double)t[k].z, s2, fma((double)t[k].w, s3, ra[k]))));
@@ -2902,13 +2906,17 @@ __global__ void __launch_bounds__(256) k_resid_rows_b(BatchDev bd)
                     z3 = fma((double)t[k].w, si[k], z3);
                 }
             }
-            zs[w][4 * lane] = z0;
-            zs[w][4 * lane + 1] = z1;
-            zs[w][4 * lane + 2] = z2;
-            zs[w][4 * lane + 3] = z3;
+            double* zb = &zs[it & 1][0][0];
+            zb[w * 256 + 4 * lane] = z0;
+            zb[w * 256 + 4 * lane + 1] = z1;
+            zb[w * 256 + 4 * lane + 2] = z2;
+            zb[w * 256 + 4 * lane + 3] = z3;
+            if (c + 256 < c0) {
+#pragma unroll
+                for (int k = 0; k < 16; k++) t[k] = *(const float4_t*)(Ur + (size_t)(4 * k) * ld + c + 256);
+            }
             __syncthreads();
-            if (256 * it + tid < c0) Z[256 * it + tid] = ((zs[0][tid] + zs[1][tid]) + zs[2][tid]) + zs[3][tid];
-            __syncthreads();
+            if (256 * it + tid < c0) Z[256 * it + tid] = ((zb[tid] + zb[256 + tid]) + zb[512 + tid]) + zb[768 + tid];
         }
         const double tot = lane_sum16(ra);
         if ((lane & 3) == 0) racc[w + 4 * (lane >> 2)] = tot;
