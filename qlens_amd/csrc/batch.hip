@@ -2107,18 +2107,34 @@ k_chol_trail_bf16(const MatDesc* mats, int k0, int nq, int ncolt)
 // every launch).  LDS: A hi, A lo, B hi, B lo, 64 rows x LPB BF16 each (144-byte rows: the 16 rows
 // of an operand read start on distinct 16-byte bank groups).
 #define LPB 72
-__device__ __forceinline__ void stage64_split(const float* P, int sp, __bf16* H, __bf16* L)
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+// a 64 x 64 FP32 block (row pitch sp) into registers: thread piece h holds row e >> 6, columns
+// e & 63 .. +3 of e = (tid + 256 h) * 4
+__device__ __forceinline__ void load64_regs(const float* P, int sp, f32x4_t (&r)[4])
 {
-    typedef float P4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        const int e = (threadIdx.x + 256 * h) * 4;
+        r[h] = *(const f32x4_t*)&P[(size_t)(e >> 6) * sp + (e & 63)];
+    }
+}
+// those registers split x = hi + lo into BF16 rows of an operand
+__device__ __forceinline__ void split64_store(const f32x4_t (&r)[4], __bf16* H, __bf16* L)
+{
 #pragma unroll
     for (int h = 0; h < 4; h++) {
         const int e = (threadIdx.x + 256 * h) * 4;
         const int i = e >> 6, k = e & 63;
-        const P4 x = *(const P4*)&P[(size_t)i * sp + k];
-        const bf16x4_t xh = __builtin_convertvector(x, bf16x4_t);
+        const bf16x4_t xh = __builtin_convertvector(r[h], bf16x4_t);
         *(bf16x4_t*)&H[i * LPB + k] = xh;
-        *(bf16x4_t*)&L[i * LPB + k] = __builtin_convertvector(x - __builtin_convertvector(xh, P4), bf16x4_t);
+        *(bf16x4_t*)&L[i * LPB + k] = __builtin_convertvector(r[h] - __builtin_convertvector(xh, f32x4_t), bf16x4_t);
     }
+}
+__device__ __forceinline__ void stage64_split(const float* P, int sp, __bf16* H, __bf16* L)
+{
+    f32x4_t r[4];
+    load64_regs(P, sp, r);
+    split64_store(r, H, L);
 }
 
 // an accumulator tile (mma64's layout: wave (wr, wc) holds quadrant rows wr*32.., columns wc*32..)
@@ -2391,7 +2407,7 @@ __global__ void __launch_bounds__(256) k_chol_diag4_m(const MatDesc* mats, int k
     }
 }
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_chol_lpanel_bf16(const MatDesc* mats, int kb, int k0, int kend)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_chol_lpanel_bf16(const MatDesc* mats, int kb, int k0, int kend)
 {
     __shared__ __attribute__((aligned(16))) __bf16 sm[4 * TS * LPB];
     __bf16* const Ah = sm;
@@ -2408,20 +2424,34 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     int ld, lp;
     T* blk = tile_ptr<T>(D, ib, kb, ld);
     const bool first = (k0 == 0 && D.A0);
+    // operand blocks are loaded into registers one step ahead of their use (the first with the C
+    // block, W_kb during the last step), so their latency overlaps the barriers and products
     T cv[16];
     load64c<T>(first ? (const T*)D.A0 + (blk - (T*)D.A) : blk, ld, cv);
+    f32x4_t ra[4], rb[4];
+    const T* const Wkb = (const T*)D.W + (size_t)kb * TS * TS;
+    if (k0 < kb) {
+        load64_regs(tile_ptr<T>(D, ib, k0, lp), lp, ra);
+        load64_regs(tile_ptr<T>(D, kb, k0, lp), lp, rb);
+    } else {
+        load64_regs(Wkb, TS, ra);
+    }
     float4_t acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int c = 0; c < 2; c++) acc[a][c] = (float4_t){0, 0, 0, 0};
     for (int kp = k0; kp < kb; kp++) {
-        const T* Pa = tile_ptr<T>(D, ib, kp, lp);
-        const T* Pb = tile_ptr<T>(D, kb, kp, lp);
         __syncthreads();
-        stage64_split(Pa, lp, Ah, Al);
-        stage64_split(Pb, lp, Bh, Bl);
+        split64_store(ra, Ah, Al);
+        split64_store(rb, Bh, Bl);
         __syncthreads();
+        if (kp + 1 < kb) {
+            load64_regs(tile_ptr<T>(D, ib, kp + 1, lp), lp, ra);
+            load64_regs(tile_ptr<T>(D, kb, kp + 1, lp), lp, rb);
+        } else {
+            load64_regs(Wkb, TS, ra);
+        }
         mma64_bf16(Ah, Al, Bh, Bl, acc);
     }
 #pragma unroll
@@ -2432,7 +2462,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             for (int v = 0; v < 4; v++) acc[a][c][v] = cv[(a * 2 + c) * 4 + v] - acc[a][c][v];
     __syncthreads();
     acc64_split(acc, Ah, Al);
-    stage64_split((const T*)D.W + (size_t)kb * TS * TS, TS, Bh, Bl);
+    split64_store(ra, Bh, Bl);  // W_kb
     __syncthreads();
 #pragma unroll
     for (int a = 0; a < 2; a++)
@@ -2443,6 +2473,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     if (ib != kb + 1 || ib >= kend) return;
     T* dg = tile_ptr<T>(D, ib, ib, ld);
     load64c<T>(first ? (const T*)D.A0 + (dg - (T*)D.A) : dg, ld, cv);
+    if (k0 < kb) load64_regs(tile_ptr<T>(D, ib, k0, lp), lp, ra);
     __syncthreads();
     acc64_split(acc, Ah, Al);
     __syncthreads();
@@ -2452,10 +2483,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         for (int c = 0; c < 2; c++) acc[a][c] = (float4_t){0, 0, 0, 0};
     mma64_bf16(Ah, Al, Ah, Al, acc);
     for (int kp = k0; kp < kb; kp++) {
-        const T* Pa = tile_ptr<T>(D, ib, kp, lp);
         __syncthreads();
-        stage64_split(Pa, lp, Ah, Al);
+        split64_store(ra, Ah, Al);
         __syncthreads();
+        if (kp + 1 < kb) load64_regs(tile_ptr<T>(D, ib, kp + 1, lp), lp, ra);
         mma64_bf16(Ah, Al, Ah, Al, acc);
     }
     // the block, now up to date, is factored here, so the next column step needs no diagonal
