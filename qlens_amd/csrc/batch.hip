@@ -886,6 +886,18 @@ __device__ __forceinline__ void lb_half(const BatchDev& bd, const ItemMeta& M, c
                 const int n = __popcll(bal);
                 const char* tb = (const char*)tab + K * (int)sizeof(T);  // byte offsets: tap at tb - E
                 int k = 0;
+                if constexpr (sizeof(T) == 4) {
+                    // two entries per 16-byte LDS read (the same value in every lane)
+                    for (; k + 4 <= n; k += 4) {
+                        const uint4 q0 = *(const uint4*)&ent[k], q1 = *(const uint4*)&ent[k + 2];
+                        const T t0 = *(const T*)(tb - (int)q0.x), t1 = *(const T*)(tb - (int)q0.z);
+                        const T t2 = *(const T*)(tb - (int)q1.x), t3 = *(const T*)(tb - (int)q1.z);
+                        acc += t0 * __uint_as_float(q0.y);
+                        acc += t1 * __uint_as_float(q0.w);
+                        acc += t2 * __uint_as_float(q1.y);
+                        acc += t3 * __uint_as_float(q1.w);
+                    }
+                }
                 for (; k + 4 <= n; k += 4) {  // four entries' reads in flight together, summed in order
                     const LbEnt<T> x0 = ent[k], x1 = ent[k + 1], x2 = ent[k + 2], x3 = ent[k + 3];
                     const T t0 = *(const T*)(tb - x0.e), t1 = *(const T*)(tb - x1.e);
