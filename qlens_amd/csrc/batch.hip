@@ -1997,8 +1997,11 @@ k_chol_trail_m(const MatDesc* mats, int k0, int nq)
 static_assert((long long)QLX_MAX_LD * QLX_MAX_LD * 4 < (1LL << 31), "trailing-update byte offsets must fit in 31 bits");
 typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 
+__device__ __forceinline__ void chol_diag_f32_4w(const MatDesc& D, int kb, float* tile, int ld, float (&a)[16],
+                                                 float (*Lt)[TS], float (*X)[TS], float* Rd);
+
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
-k_chol_trail_bf16(const MatDesc* mats, int k0, int nq, int ncolt)
+k_chol_trail_bf16(const MatDesc* mats, int k0, int nq, int ncolt, int fdiag)
 {
     typedef float P __attribute__((ext_vector_type(4)));
     typedef __attribute__((address_space(1))) float gfloat;
@@ -2116,6 +2119,28 @@ k_chol_trail_bf16(const MatDesc* mats, int k0, int nq, int ncolt)
                 const int cc = wc * 64 + c * 16 + (lane & 15);
                 if (!diag || cc <= rr) Cg[(size_t)rr * ld + cc] = acc[a][c][v];
             }
+    // fdiag: the workgroup of tile (0, 0) also factors diagonal block r0b, now up to date (wave 0's
+    // quadrant), so the part that starts there needs no diagonal launch; through LDS (S, 64 x 65,
+    // over the operand tiles) into rows per lane, as the panel kernel does
+    if (fdiag && I2 == 0 && J2 == 0) {
+        float* S = (float*)trail_raw;
+        float(*Lt)[TS] = (float(*)[TS])(S + TS * 65);
+        float* Rd = S + TS * 65 + TS * TS;
+        __syncthreads();
+        if (w == 0) {
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int c = 0; c < 4; c++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) S[(a * 16 + 4 * (lane >> 4) + v) * 65 + c * 16 + (lane & 15)] = acc[a][c][v];
+        }
+        __syncthreads();
+        float av[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) av[j] = S[lane * 65 + 16 * w + j];
+        chol_diag_f32_4w(D, r0b, (float*)D.A + (size_t)rI * ld + rJ, ld, av, Lt, (float(*)[TS])S, Rd);  // X over S: read before its first barrier
+    }
 }
 
 // The left-looking panel step for FP32 matrices (k_chol_lpanel_m's schedule and operands) with the
@@ -2550,13 +2575,15 @@ static void chol_step(hipStream_t s, const MatDesc* mats, int nmat, int kb, int 
 
 // one column block after another of the panel [k0, kend): the diagonal kernel, then the panel kernel
 template <typename T>
-static void chol_subpanel(hipStream_t s, const MatDesc* mats, int nmat, int max_nb, int k0, int kend, bool left)
+static void chol_subpanel(hipStream_t s, const MatDesc* mats, int nmat, int max_nb, int k0, int kend, bool left,
+                          bool first_done = false)
 {
     for (int kb = k0; kb < kend && kb < max_nb; kb++) {
         const int rest = max_nb - kb - 1;
         const bool f32 = sizeof(T) == 4;
-        // FP32 left-looking parts: the panel kernel of column block kb - 1 factored block kb
-        if (!(left && f32 && kb > k0)) hipLaunchKernelGGL(k_chol_diag4_m<T>, dim3(1, nmat), dim3(256), 0, s, mats, kb);
+        // FP32 left-looking parts: the panel kernel of column block kb - 1 factored block kb, and
+        // the update before the part factored block k0 (first_done)
+        if (!(left && f32 && (kb > k0 || first_done))) hipLaunchKernelGGL(k_chol_diag4_m<T>, dim3(1, nmat), dim3(256), 0, s, mats, kb);
         if (rest <= 0) continue;
         if (left) {
             if (f32)
@@ -2597,13 +2624,14 @@ static void chol_two_level(hipStream_t s, const MatDesc* mats, int nmat, int max
             for (int ks = k0; ks < k0 + NB && ks < max_nb; ks += 4) {
                 if (ks > k0) {  // the half's columns, from all earlier halves of the panel (a strip)
                     const int nts = (max_nb - ks) / 2;
-                    hipLaunchKernelGGL(k_chol_trail_bf16, dim3(2 * nts - 1, nmat), dim3(256), lds_bf, s, mats, k0, ks - k0, 2);
+                    hipLaunchKernelGGL(k_chol_trail_bf16, dim3(2 * nts - 1, nmat), dim3(256), lds_bf, s, mats, k0, ks - k0, 2, 1);
                 }
-                chol_subpanel<T>(s, mats, nmat, max_nb, ks, ks + 4, true);
+                // the part's first diagonal block was factored by the strip or trailing update before it
+                chol_subpanel<T>(s, mats, nmat, max_nb, ks, ks + 4, true, ks > 0);
             }
             if (k0 + NB >= max_nb) break;
             const int nt = (max_nb - k0 - NB) / 2;
-            hipLaunchKernelGGL(k_chol_trail_bf16, dim3(syrk_grid(nt), nmat), dim3(256), lds_bf, s, mats, k0, NB, 0);
+            hipLaunchKernelGGL(k_chol_trail_bf16, dim3(syrk_grid(nt), nmat), dim3(256), lds_bf, s, mats, k0, NB, 0, 1);
         }
         return;
     }
