@@ -257,6 +257,15 @@ __device__ __forceinline__ int nfw_j01_tab(const LensPrep& L, const double* tab,
     return nev;
 }
 
+// the nine-level quadrature as a called function: a ray that needs more than seven levels is rare,
+// and inlining this second copy of the unrolled level loop doubles the kernel's code (19.7 k
+// instructions, beyond what the instruction cache holds)
+template <typename T, bool KT>
+__device__ __attribute__((noinline)) int nfw_j01_tab_all(const LensPrep& L, const double* tab, T x, T y, T tol, T& j0, T& j1)
+{
+    return nfw_j01_tab<T, KT>(L, tab, x, y, tol, j0, j1);
+}
+
 template <typename T>
 __device__ __forceinline__ T patterson_k(const LensPrep& L, T x, T y, T tol, T nph)
 {
@@ -364,7 +373,12 @@ __device__ __forceinline__ void lens_deflection(const LensPrep& L, T x, T y, T t
                 if constexpr (TAB) {
                     // seven levels (127 nodes) first: the rest only for the rays that need them
                     int ne = nfw_j01_tab<T, KT, 7>(L, tab, x, y, tol, j0, j1);
-                    if (ne < 0) ne = nfw_j01_tab<T, KT>(L, tab, x, y, tol, j0, j1);
+                    if (ne < 0) {
+                        // called in the tabulated-kappa kernel (half its code; 9.6 -> 9.0 ms), inlined
+                        // in the closed-form one (called there, that kernel spilled more: 16.6 -> 19.1 ms)
+                        if constexpr (KT) ne = nfw_j01_tab_all<T, KT>(L, tab, x, y, tol, j0, j1);
+                        else ne = nfw_j01_tab<T, KT>(L, tab, x, y, tol, j0, j1);
+                    }
                     nev += ne;
                 }
                 else nev += nfw_j01<T>(L, x, y, tol, j0, j1);
