@@ -1290,13 +1290,30 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
     const int o1 = ((e0 + 2048) >> 9) * BP2 + ((e0 + 2048) & 511);  // elements apart: 4 k-groups
     const int o2 = ((e0 + 4096) >> 9) * BP2 + ((e0 + 4096) & 511);
     const int o3 = ((e0 + 6144) >> 9) * BP2 + ((e0 + 6144) & 511);
-    // the next chunk after k holding both blocks
-    auto next = [&](int k) {
-        for (k = k + 1; k < nkt; k++) {
+    // the chunks holding both blocks as a bit mask in LDS (64 chunks a word, one ballot each), so
+    // that finding the next one is a word test rather than a scan of the chunks' supports
+    unsigned long long* const cmask = (unsigned long long*)(lds2 + 32 * BP2);
+    const int nwd = (nkt + 63) >> 6;
+    for (int q = w; q < nwd; q += 4) {
+        const int k = 64 * q + lane;
+        bool b = false;
+        if (k < nkt) {
             const unsigned* m = nz + (size_t)k * ncbw;
-            if (mask_has(m, I) && mask_has(m, J)) break;
+            b = mask_has(m, I) && mask_has(m, J);
         }
-        return k;
+        const unsigned long long bal = __ballot(b);
+        if (lane == 0) cmask[q] = bal;
+    }
+    __syncthreads();
+    // the next chunk after k holding both blocks (nkt if none)
+    auto next = [&](int k) {
+        ++k;
+        for (int q = k >> 6; q < nwd; q++) {
+            unsigned long long v = cmask[q];
+            if (q == (k >> 6)) v &= ~0ULL << (k & 63);
+            if (v) return 64 * q + __builtin_ctzll(v);
+        }
+        return nkt;
     };
 #define SYRK_LOAD(kk_, A0, A1, A2, A3, B0, B1, B2, B3)                                                  \
     {                                                                                                    \
@@ -1353,9 +1370,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
     syrk_f16_store(bd, M, item, I, J, acc);
 }
 
-void launch_syrk_f16_b(hipStream_t s, const BatchDev& bd, int max_ncb)
+void launch_syrk_f16_b(hipStream_t s, const BatchDev& bd, int max_ncb, int max_nkt)
 {
-    const size_t lds = (size_t)2 * 16 * BP2 * sizeof(uint16_t);
+    const size_t lds = (size_t)2 * 16 * BP2 * sizeof(uint16_t) + (size_t)((max_nkt + 63) / 64) * 8;
     allow_big_lds((const void*)k_syrk_f16_b);
     hipLaunchKernelGGL(k_syrk_f16_b, dim3(syrk_grid(max_ncb), bd.B), dim3(256), lds, s, bd);
 }

@@ -899,7 +899,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     HIPCHK(c, hipEventRecord(c->ev_fork, st));
     if (!mixed) launch_gather_cvec_b(st, bd, max_rows);
     timer_begin(c, "syrk");
-    if (mixed) launch_syrk_f16_b(st, bd, max_ncb);
+    if (mixed) launch_syrk_f16_b(st, bd, max_ncb, max_nkt);
     else launch_syrk_f64_b(st, bd, max_ncb);
     timer_end(c, "syrk");
     if (opt) {
