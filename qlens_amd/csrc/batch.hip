@@ -1128,6 +1128,7 @@ __global__ void __launch_bounds__(256) k_syrk_f64_b(BatchDev bd)
 {
     __shared__ double As[TS][LDSP];
     __shared__ double Bs[TS][LDSP];
+    extern __shared__ unsigned long long cmask[];  // chunks holding both blocks, 64 a word
     const int item = blockIdx.y;
     const ItemMeta& M = bd.meta[item];
     if (!M.valid) return;
@@ -1139,14 +1140,27 @@ __global__ void __launch_bounds__(256) k_syrk_f64_b(BatchDev bd)
     const double* cvec = bd.cvec + M.cvec_off;
     const unsigned* nz = bd.nz + M.nz_off;
     const int* bb = bd.bbase + M.bb_off;
+    const int nwd = (M.nkt + 63) >> 6;
+    for (int q = w; q < nwd; q += 4) {
+        const int k = 64 * q + lane;
+        bool b = false;
+        if (k < M.nkt) {
+            const unsigned* m = nz + (size_t)k * M.ncbw;
+            b = mask_has(m, I) && mask_has(m, J);
+        }
+        const unsigned long long bal = __ballot(b);
+        if (lane == 0) cmask[q] = bal;
+    }
+    __syncthreads();
     double4_t acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int c = 0; c < 2; c++) acc[a][c] = (double4_t){0, 0, 0, 0};
-    for (int kt = 0; kt < M.nkt; kt++) {
+    for (int q = 0; q < nwd; q++)
+    for (unsigned long long bits = cmask[q]; bits; bits &= bits - 1) {
+        const int kt = 64 * q + __builtin_ctzll(bits);
         const unsigned* m = nz + (size_t)kt * M.ncbw;
-        if (!(mask_has(m, I) && mask_has(m, J))) continue;
         const double* Ab = pool + (size_t)(bb[kt] + mask_rank(m, I)) * TS * TS;
         const double* Bb = pool + (size_t)(bb[kt] + mask_rank(m, J)) * TS * TS;
         const double* cv = cvec + (size_t)kt * TS;
@@ -1184,9 +1198,9 @@ __global__ void __launch_bounds__(256) k_syrk_f64_b(BatchDev bd)
             }
 }
 
-void launch_syrk_f64_b(hipStream_t s, const BatchDev& bd, int max_ncb)
+void launch_syrk_f64_b(hipStream_t s, const BatchDev& bd, int max_ncb, int max_nkt)
 {
-    hipLaunchKernelGGL(k_syrk_f64_b, dim3(syrk_grid(max_ncb), bd.B), dim3(256), 0, s, bd);
+    hipLaunchKernelGGL(k_syrk_f64_b, dim3(syrk_grid(max_ncb), bd.B), dim3(256), (size_t)((max_nkt + 63) / 64) * 8, s, bd);
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -900,7 +900,7 @@ int run_wave(Ctx* c, const qlx_lens_model* models, const std::vector<WaveItem>& 
     if (!mixed) launch_gather_cvec_b(st, bd, max_rows);
     timer_begin(c, "syrk");
     if (mixed) launch_syrk_f16_b(st, bd, max_ncb, max_nkt);
-    else launch_syrk_f64_b(st, bd, max_ncb);
+    else launch_syrk_f64_b(st, bd, max_ncb, max_nkt);
     timer_end(c, "syrk");
     if (opt) {
         HIPCHK(c, hipMemcpyAsync(c->Fraw.p, c->F.p, f_tot * 8, hipMemcpyDeviceToDevice, st));

@@ -250,7 +250,7 @@ void launch_lcols_b(hipStream_t s, const BatchDev& bd, int max_img, int max_src)
 void launch_row_support_b(hipStream_t s, const BatchDev& bd, int max_nkt);
 int launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_nsup, int mixed, int cbw);
 void launch_gather_cvec_b(hipStream_t s, const BatchDev& bd, int max_rows);
-void launch_syrk_f64_b(hipStream_t s, const BatchDev& bd, int max_ncb);
+void launch_syrk_f64_b(hipStream_t s, const BatchDev& bd, int max_ncb, int max_nkt);
 void launch_syrk_f16_b(hipStream_t s, const BatchDev& bd, int max_ncb, int max_nkt);
 void launch_curvature_b(hipStream_t s, const BatchDev& bd, int max_cells, int max_rtiles);
 // items whose R equals an earlier item's (same grid shape and active-index map) get rrep = that item,
