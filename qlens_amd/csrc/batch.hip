@@ -1157,21 +1157,45 @@ __global__ void __launch_bounds__(256) k_syrk_f64_b(BatchDev bd)
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int c = 0; c < 2; c++) acc[a][c] = (double4_t){0, 0, 0, 0};
-    for (int q = 0; q < nwd; q++)
-    for (unsigned long long bits = cmask[q]; bits; bits &= bits - 1) {
-        const int kt = 64 * q + __builtin_ctzll(bits);
+    // the next chunk after k holding both blocks (M.nkt if none)
+    auto next = [&](int k) {
+        ++k;
+        for (int q = k >> 6; q < nwd; q++) {
+            unsigned long long v = cmask[q];
+            if (q == (k >> 6)) v &= ~0ULL << (k & 63);
+            if (v) return 64 * q + __builtin_ctzll(v);
+        }
+        return M.nkt;
+    };
+    // the next chunk's blocks and weights are loaded into registers while this one is multiplied:
+    // thread element i is e = tid + 256 i (row kk = e >> 6 = tid / 64 + 4 i, column tid % 64)
+    double ra[16], rb[16], rc[16];
+    auto load = [&](int kt) {
         const unsigned* m = nz + (size_t)kt * M.ncbw;
         const double* Ab = pool + (size_t)(bb[kt] + mask_rank(m, I)) * TS * TS;
         const double* Bb = pool + (size_t)(bb[kt] + mask_rank(m, J)) * TS * TS;
         const double* cv = cvec + (size_t)kt * TS;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            ra[i] = Ab[tid + 256 * i];
+            rb[i] = Bb[tid + 256 * i];
+            rc[i] = cv[(tid >> 6) + 4 * i];
+        }
+    };
+    int kt = next(-1);
+    if (kt < M.nkt) load(kt);
+    while (kt < M.nkt) {
         __syncthreads();
-#pragma unroll 4
-        for (int e = tid; e < TS * TS; e += 256) {
-            const int kk = e >> 6, ii = e & 63;
-            As[kk][ii] = Ab[e];
-            Bs[kk][ii] = Bb[e] * cv[kk];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int kk = (tid >> 6) + 4 * i, ii = tid & 63;
+            As[kk][ii] = ra[i];
+            Bs[kk][ii] = rb[i] * rc[i];
         }
         __syncthreads();
+        const int nk = next(kt);
+        if (nk < M.nkt) load(nk);
+        kt = nk;
 #pragma unroll 4
         for (int kk = 0; kk < TS; kk += 4) {
             const int kq = kk + (lane >> 4);
