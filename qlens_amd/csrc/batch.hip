@@ -1942,7 +1942,7 @@ __device__ __forceinline__ void trail_mma(T (*As)[LDT], T (*Bs)[LDT], typename M
 // Two 128 x 68 staging tiles per WG (70 KB in FP32) allow two WGs per CU, so two waves per SIMD
 // is all the occupancy this kernel gets: the register budget is sized for that.
 template <typename T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 1 : 2, sizeof(T) == 8 ? 1 : 2)))
 k_chol_trail_m(const MatDesc* mats, int k0, int nq)
 {
     extern __shared__ double trail_raw[];
@@ -1966,11 +1966,12 @@ k_chol_trail_m(const MatDesc* mats, int k0, int nq)
         for (int c = 0; c < 4; c++) acc[a][c] = (typename Mfma16<T>::acc_t){0, 0, 0, 0};
     typedef typename Mfma16<T>::vec_t V;
     constexpr int VW = sizeof(V) / sizeof(T);
-    constexpr int NPF = (sizeof(T) == 4) ? (TT * TS) / (256 * VW) : 1;  // pieces per thread and matrix
+    constexpr int NPF = (TT * TS) / (256 * VW);  // pieces per thread and matrix
     typedef T P __attribute__((ext_vector_type(VW)));
     P pa[NPF], pb[NPF];
-    T cv[64];  // FP32: the C tile values of this lane (loaded during the last block)
-    // FP32: the next column block is loaded into registers while the current one is multiplied
+    T cv[64];  // the C tile values of this lane (loaded during the last block)
+    // the next column block is loaded into registers while the current one is multiplied (FP64: one
+    // workgroup per CU for its 139 KB of LDS, so the registers are there)
 #define TRAIL_LOAD(q)                                                    \
     {                                                                    \
         const long long c0 = (long long)(k0 + (q)) * TS;                 \
@@ -1982,7 +1983,7 @@ k_chol_trail_m(const MatDesc* mats, int k0, int nq)
             pb[h] = *(const P*)&A[(rJ + i) * ld + c0 + k];               \
         }                                                                \
     }
-    if (sizeof(T) == 4) {
+    {
         TRAIL_LOAD(0);
         for (int q = 0; q < nq; q++) {
             __syncthreads();
@@ -2009,18 +2010,6 @@ k_chol_trail_m(const MatDesc* mats, int k0, int nq)
                     cv[(a * 4 + c) * 4 + v] = A[(rI + rr) * ld + rJ + cc];
                 }
         trail_mma<T>(As, Bs, acc);
-    } else {
-        for (int q = 0; q < nq; q++) {
-            __syncthreads();
-            const long long c0 = (long long)(k0 + q) * TS;
-            for (int e = tid * VW; e < TT * TS; e += 256 * VW) {
-                const int i = e >> 6, k = e & 63;
-                *(V*)&As[i][k] = *(const V*)&A[(rI + i) * ld + c0 + k];
-                *(V*)&Bs[i][k] = *(const V*)&A[(rJ + i) * ld + c0 + k];
-            }
-            __syncthreads();
-            trail_mma<T>(As, Bs, acc);
-        }
     }
     const bool diag = (I2 == J2);
 #pragma unroll
@@ -2032,7 +2021,7 @@ k_chol_trail_m(const MatDesc* mats, int k0, int nq)
                 const int rr = wr * 64 + a * 16 + Mfma16<T>::row(lane, v);
                 const int cc = wc * 64 + c * 16 + (lane & 15);
                 T* d = &A[(rI + rr) * ld + rJ + cc];
-                const T old = (sizeof(T) == 4) ? cv[(a * 4 + c) * 4 + v] : *d;
+                const T old = cv[(a * 4 + c) * 4 + v];
                 if (!diag || cc <= rr) *d = old - acc[a][c][v];
             }
 }
