@@ -840,10 +840,10 @@ __device__ __forceinline__ void lb_half(const BatchDev& bd, const ItemMeta& M, c
         if (l != nxt) {  // no entry in reach: acc = 0
             const int c_ = c0h + ii;
             if constexpr (F64) {
-                ((double*)tile)[lane * HB + ii] = c_ == M.n_src ? dv : 0.0;
+                ((double*)tile)[lane * (HB + 2) + ii] = c_ == M.n_src ? dv : 0.0;
             } else {
                 const double x = c_ == M.n_src ? dv : 0.0 * sc;
-                ((uint16_t*)tile)[((lane >> 2) * HB + ii) * 4 + (lane & 3)] = f32_to_f16_rne((float)x);
+                ((uint16_t*)tile)[((lane >> 2) * (HB + 2) + ii) * 4 + (lane & 3)] = f32_to_f16_rne((float)x);
             }
             continue;
         }
@@ -923,10 +923,10 @@ __device__ __forceinline__ void lb_half(const BatchDev& bd, const ItemMeta& M, c
         if (p < 0) acc = (T)0;  // rows past the image: the fast path read row 0's taps for them
         const int c_ = c0h + ii;
         if constexpr (F64) {
-            ((double*)tile)[lane * HB + ii] = c_ == M.n_src ? dv : (double)acc;
+            ((double*)tile)[lane * (HB + 2) + ii] = c_ == M.n_src ? dv : (double)acc;
         } else {
             const double x = c_ == M.n_src ? dv : (double)acc * sc;
-            ((uint16_t*)tile)[((lane >> 2) * HB + ii) * 4 + (lane & 3)] = f32_to_f16_rne((float)x);
+            ((uint16_t*)tile)[((lane >> 2) * (HB + 2) + ii) * 4 + (lane & 3)] = f32_to_f16_rne((float)x);
         }
     }
 }
@@ -1005,7 +1005,9 @@ __global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int tab_bytes, 
     }
     __syncthreads();
     // a half block's image: runs of HB * 8 bytes (FP64: 64 rows; FP16: 16 groups of 4 rows, the
-    // k4-interleaved layout), each landing at byte 8 (run * CB + h * HB) of the pool block
+    // k4-interleaved layout), each landing at byte 8 (run * CB + h * HB) of the pool block.  In LDS
+    // the runs are HB * 8 + 16 bytes apart: at HB * 8 bytes (a multiple of 256) every lane's store of
+    // a column fell on the same two banks
     const int nrun = F64 ? TS : TS / 4, upr = HB / 2;
     int q = 0;
     for (int wd = 0; wd < M.ncbw; wd++) {
@@ -1021,7 +1023,7 @@ __global__ void __launch_bounds__(256) k_build_lb_b(BatchDev bd, int tab_bytes, 
                 const uint4* src = (const uint4*)tile;
                 for (int u = tid; u < nrun * upr; u += 256) {
                     const int run = u >> (cbs - 2), o = u & (upr - 1);  // upr = 2^(cbs - 2)
-                    store_nt(&dst[(run * CB + h * HB) / 2 + o], src[u]);
+                    store_nt(&dst[(run * CB + h * HB) / 2 + o], src[run * (upr + 1) + o]);
                 }
                 __syncthreads();
             }
@@ -1046,7 +1048,7 @@ int launch_build_lb_b(hipStream_t s, const BatchDev& bd, int max_rows, int max_n
     }
     const size_t tab_bytes = ((size_t)trows * tpc * esz + 15) / 16 * 16;
     if (tab_bytes > 64 * 1024) return -1;  // PSF beyond the LDS budget
-    const size_t tile_bytes = (size_t)TS * (cbw / 2) * (mixed ? 2 : 8);
+    const size_t tile_bytes = (size_t)TS * (cbw / 2) * (mixed ? 2 : 8) + (mixed ? TS / 4 : TS) * 16;  // + 16 B a run
     const int nbw = (int)((bd.s_cell + 32) / 32 + 3) / 4 * 4;  // column bitmap words (16-byte multiple)
     const size_t lds = tab_bytes + 4 * 64 * (mixed ? sizeof(LbEnt<float>) : sizeof(LbEnt<double>)) + (size_t)nbw * 4 + tile_bytes;
     if (mixed) {
